@@ -1,0 +1,206 @@
+/*
+ * opentm.h -- C ABI of the MI355X (gfx950) thermal inverse-homogenization
+ * engine. Plain pointers and sizes only; every call returns an otm_status.
+ *
+ * The reference (quanyuchen2000/OPENTM) has no C ABI: its boundary is the
+ * C++ classes homo::Homogenization_H, homo::MG_H, homo::OCOptimizer and the
+ * (commented-out) pybind module openTM. Each entry point below names the
+ * reference interface it replaces. C++ classes with the reference's names
+ * sit on top of this ABI in include/opentm/*.hpp.
+ *
+ * Conventions
+ *  - Scalar fields are lexicographic with x fastest; a row pitch (in
+ *    floats, >= reso[0]) may be given where the reference takes pitchT.
+ *  - `where` says whether a pointer is host (OTM_HOST) or device
+ *    (OTM_DEVICE) memory. Device pointers must be allocated on the
+ *    context's device.
+ *  - The homogenized tensor uses the reference's load-case order: load case
+ *    0 is a unit temperature gradient along +y of the grid, 1 along -x,
+ *    2 along +z (the element node remap of templateMatrix.cpp:74-82).
+ *  - A context owns one MG hierarchy and one HIP stream; calls on
+ *    different contexts are independent and may run from different host
+ *    threads. Calls on one context must not overlap.
+ */
+#ifndef OPENTM_H
+#define OPENTM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OTM_API __attribute__((visibility("default")))
+
+typedef struct otm_ctx otm_ctx;
+typedef struct otm_oc otm_oc;
+
+typedef enum otm_status {
+  OTM_OK = 0,
+  OTM_E_ARG = -1,          /* invalid argument (sizes, pointers, flags) */
+  OTM_E_HIP = -2,          /* HIP runtime error */
+  OTM_E_NOCONV = -3,       /* solver did not reach the tolerance */
+  OTM_E_UNSUPPORTED = -4,  /* resolution or option not supported */
+  OTM_E_NUMERIC = -5,      /* NaN/Inf in the solve (reference: "MG numerical explode", MG.cpp:116) */
+  OTM_E_NODEV = -6,        /* no HIP device */
+  OTM_E_STATE = -7,        /* call order violated (e.g. solve before update) */
+  OTM_E_OOM = -8           /* device allocation failed */
+} otm_status;
+
+enum { OTM_HOST = 0, OTM_DEVICE = 1 };
+
+/* Solver configuration. Mirrors cfg::HomoConfig (cmdline.h:12-33) for the
+ * fields the solve uses, plus the multigrid knobs the reference hard-codes. */
+typedef struct otm_config {
+  int reso[3];              /* HomoConfig.reso: cells per axis (even, >= 8) */
+  double heat_ratio[2];     /* HomoConfig.heatRatio: kappa at rho=1 and rho=0 */
+  double fem_rel_thres;     /* HomoConfig.femRelThres: relative residual target */
+  int max_cg_iter;          /* PCG iteration cap per solve */
+  int pre_sweeps;           /* Gauss-Seidel sweeps before restriction */
+  int post_sweeps;          /* Gauss-Seidel sweeps after prolongation */
+  int min_coarse;           /* stop coarsening once a level edge would fall below this */
+  int device;               /* HIP device ordinal */
+  int verbose;              /* print per-solve statistics to stderr */
+} otm_config;
+
+/* cfg::HomoConfig::init (cmdline.cpp:4-35) defaults where they apply;
+ * fem_rel_thres defaults to 1e-6 (the reference's 1e-2 stationary-MG
+ * threshold is not accurate enough for kappa^H parity). */
+OTM_API void otm_config_init(otm_config* cfg);
+
+/* Message of the last failing call on this host thread. */
+OTM_API const char* otm_last_error(void);
+OTM_API int otm_version(void);
+
+/* Homogenization_H(cfg::HomoConfig) / Homogenization_H::build
+ * (homogenization.h:15-32, homogenization.cpp:31-63): allocates the grid
+ * hierarchy and element constants. */
+OTM_API int otm_create(const otm_config* cfg, otm_ctx** out);
+OTM_API int otm_destroy(otm_ctx* ctx);
+
+/* Number of multigrid levels and the edge lengths of level `l`
+ * (MG_H::grids, MG.h:14-32). */
+OTM_API int otm_levels(const otm_ctx* ctx, int* nlevels);
+OTM_API int otm_level_reso(const otm_ctx* ctx, int l, int reso[3]);
+
+/* Homogenization_H::update(float* rho, int pitchT) (homogenization.cpp:76-81):
+ * takes the SIMP-mapped conductivity per element (pitch >= reso[0] floats
+ * per row) and rebuilds the coarse Galerkin operators. kappa == NULL only
+ * rebuilds the operators. */
+OTM_API int otm_update(otm_ctx* ctx, const float* kappa, int pitch, int where);
+
+/* Homogenization_H::heatMatrix(double C[3][3]) (homogenization.cu:180-252):
+ * solves the three load cases (warm-started from the previous solution)
+ * and returns the symmetric 3x3 homogenized conductivity, row-major. */
+OTM_API int otm_heat_matrix(otm_ctx* ctx, double C[9]);
+
+/* PCG iterations and achieved relative residual of the last solve. */
+OTM_API int otm_solve_stats(const otm_ctx* ctx, int iters[3], double relres[3]);
+
+/* Homogenization_H::Sensitivity(float dC[3][3], float* sens, int pitchT,
+ * bool lexiOrder=true) (homogenization.cu:415-479): per-element
+ * d(sum_ij dC_ij kappa^H_ij)/d(kappa_e), written lexicographically with
+ * row pitch `pitch`. Requires a prior otm_heat_matrix. */
+OTM_API int otm_sensitivity(otm_ctx* ctx, const double dC[9], float* sens, int pitch, int where);
+
+/* The six element energies S_e(i,j) = d kappa^H_ij / d kappa_e in the order
+ * (00,01,02,11,12,22), each an array of reso[0]*reso[1]*reso[2] floats. */
+OTM_API int otm_element_energy(otm_ctx* ctx, float* S6, int where);
+
+/* The three fluctuation fields chi_k (3 * nodes floats, load case major),
+ * mean-zero. Setting them warm-starts the next solve (Grid_H::useUchar,
+ * grid.cpp:318-321). */
+OTM_API int otm_get_chi(otm_ctx* ctx, float* chi, int where);
+OTM_API int otm_set_chi(otm_ctx* ctx, const float* chi, int where);
+
+/* MG_H::reset_displacement (MG.h:14-32): zero the warm start. */
+OTM_API int otm_reset_displacement(otm_ctx* ctx);
+
+/* Level operators, for tests and custom drivers: q = A_l u (three vectors
+ * of level l, 3*nodes_l floats) and z = M^{-1} r (one V-cycle of the
+ * preconditioner on the fine level; MG_H::v_cycle, MG.cpp:46-71). */
+OTM_API int otm_apply_level(otm_ctx* ctx, int level, const float* u, float* q, int where);
+OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where);
+/* Level-l 27-point stencil, 27*nodes_l floats (offset major), l >= 1. */
+OTM_API int otm_get_stencil(otm_ctx* ctx, int level, float* st, int where);
+
+/* Radial convolution filter (conv_tsexp_t, TensorExpression.h:1095-1185;
+ * kernels TensorExpression.cuh:303-384). kind 0 = Spline4, 1 = Linear;
+ * periodic 0 renormalises at the cell boundary. adjoint != 0 applies the
+ * transpose. Arrays are contiguous reso[0]*reso[1]*reso[2] floats. */
+OTM_API int otm_conv_filter(const int reso[3], int kind, double radius, int periodic, int adjoint,
+                            const float* in, float* out, int where);
+
+/* Design-density pipeline of runCustom (Framework.cu:163):
+ * kappa = k1 + (k0-k1) * filter(rho)^3 with the Spline4 r=1.5 filter. */
+typedef struct otm_density_params {
+  int filter_kind;       /* 0 Spline4, 1 Linear, -1 none */
+  double filter_radius;  /* 1.5 in runCustom */
+  int filter_periodic;   /* 0 in runCustom (TensorExpression.h:1083) */
+  double penal;          /* SIMP exponent; only 3 is supported */
+} otm_density_params;
+OTM_API void otm_density_params_init(otm_density_params* p);
+
+/* Design density held by the context (var_tsexp_t rho, Framework.cu:155). */
+OTM_API int otm_set_design(otm_ctx* ctx, const float* rho, int where);
+OTM_API int otm_get_design(otm_ctx* ctx, float* rho, int where);
+
+/* objective.eval() path for the density expression and heat_tensor_t
+ * (homoExpression.h:89-105): filter + SIMP + update + heatMatrix. */
+OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]);
+
+/* objective.backward() (homoExpression.h:106-120): sensitivity with dC,
+ * chained through SIMP and the filter adjoint into d obj / d rho, kept in
+ * the context; copied to drho if it is non-NULL. */
+OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int where);
+
+/* homo::OCOptimizer (OCOptimizer.h:7-20). */
+OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp, int device, otm_oc** out);
+OTM_API int otm_oc_destroy(otm_oc* oc);
+/* Bisection control (the reference hard-codes 20 steps and 1e-4). */
+OTM_API int otm_oc_set_bisection(otm_oc* oc, int max_steps, double vol_tol);
+/* OCOptimizer::filterSens (OCOptimizer.cu:134-152): periodic, density
+ * weighted, linear kernel of the given radius; in place on sens. */
+OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pitch, const int reso[3],
+                               double radius, int where);
+/* OCOptimizer::update (OCOptimizer.cu:38-67): Lagrange bisection, in place
+ * on rho (ne contiguous floats). Reports bisection steps and final volume. */
+OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volratio, int where, int* steps,
+                          double* volume);
+
+/* One OC design iteration on the context's design density, using its
+ * current d obj / d rho (after otm_backward): filterSens + update. */
+OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double volratio, int* steps,
+                        double* volume);
+
+/* Live kernel timing with HIP events recorded on the context's stream
+ * around every launch of a kernel class. Classes: */
+enum {
+  OTM_PROF_FINE_GS = 0,      /* fine-level Gauss-Seidel colour sweep */
+  OTM_PROF_FINE_APPLY = 1,   /* fine-level operator apply + p.q (PCG) */
+  OTM_PROF_FINE_RESID = 2,   /* fine-level residual (V-cycle and PCG start) */
+  OTM_PROF_COARSE_GS = 3,    /* stored-stencil Gauss-Seidel colour sweep */
+  OTM_PROF_COARSE_RESID = 4, /* stored-stencil residual */
+  OTM_PROF_TRANSFER = 5,     /* restriction and prolongation */
+  OTM_PROF_COARSEST = 6,     /* coarsest dense solve */
+  OTM_PROF_VECTOR = 7,       /* PCG vector updates and dots */
+  OTM_PROF_ENERGY = 8,       /* kappa^H, sensitivity, element energies */
+  OTM_PROF_SETUP = 9,        /* Galerkin assembly and coarsest factorisation */
+  OTM_PROF_NCLASSES = 10
+};
+/* enable != 0 starts timing (and clears the totals); 0 stops it. */
+OTM_API int otm_profile_enable(otm_ctx* ctx, int enable);
+/* Launch count, summed kernel time (ms) and algorithmic bytes moved by the
+ * launches of one class since timing was enabled. */
+OTM_API int otm_profile_read(otm_ctx* ctx, int kclass, long long* launches, double* ms, double* bytes);
+
+/* Device memory helpers for callers without their own HIP allocator. */
+OTM_API int otm_device_alloc(int device, size_t bytes, void** ptr);
+OTM_API int otm_device_free(void* ptr);
+OTM_API int otm_memcpy(void* dst, const void* src, size_t bytes, int dst_where, int src_where);
+OTM_API int otm_synchronize(otm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

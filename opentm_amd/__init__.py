@@ -1,0 +1,394 @@
+"""opentm_amd -- MI355X (gfx950) thermal inverse-homogenization engine.
+
+Python mirror of the reference's operator interface, over the C ABI of
+include/opentm.h (libopentm.so, hand-written HIP kernels):
+
+* ``HomoConfig``        -- cfg::HomoConfig (cmdline.h:12-33)
+* ``Homogenization_H``  -- homo::Homogenization_H (homogenization.h:15-32)
+* ``OCOptimizer``       -- homo::OCOptimizer (OCOptimizer.h:7-20)
+* ``conv_filter``       -- the radial convolution of conv_tsexp_t
+                           (TensorExpression.h:1095-1185)
+
+Host arrays are numpy float32 (x fastest, i.e. shape (nz, ny, nx) in C
+order); ``DeviceBuffer`` holds device-resident data for loops that never
+leave HBM.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import OTM_DEVICE, OTM_HOST, OtmError
+
+__all__ = [
+    "HomoConfig", "Homogenization_H", "OCOptimizer", "DeviceBuffer", "DensityParams",
+    "conv_filter", "OtmError", "load_library",
+]
+
+
+def load_library():
+    return _capi.load()
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceBuffer:
+    """A device allocation made through the engine (otm_device_alloc)."""
+
+    def __init__(self, nfloats, device=0):
+        lib = _capi.load()
+        self.n = int(nfloats)
+        self.device = device
+        p = ctypes.c_void_p()
+        _capi.check(lib.otm_device_alloc(device, 4 * self.n, ctypes.byref(p)), "otm_device_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a, device=0):
+        a = _f32(a)
+        b = cls(a.size, device)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = _f32(a, self.n)
+        _capi.check(_capi.load().otm_memcpy(self.ptr, _ptr(a), 4 * self.n, OTM_DEVICE, OTM_HOST), "otm_memcpy")
+
+    def numpy(self):
+        out = np.empty(self.n, np.float32)
+        _capi.check(_capi.load().otm_memcpy(_ptr(out), self.ptr, 4 * self.n, OTM_HOST, OTM_DEVICE), "otm_memcpy")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _capi.load().otm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _arg(a, n):
+    """(pointer, where, keepalive) for a numpy array or a DeviceBuffer."""
+    if isinstance(a, DeviceBuffer):
+        if a.n < n:
+            raise ValueError("device buffer too small")
+        return a.ptr, OTM_DEVICE, a
+    arr = _f32(a)
+    if arr.size < n:
+        raise ValueError(f"expected at least {n} values, got {arr.size}")
+    return _ptr(arr), OTM_HOST, arr
+
+
+class HomoConfig:
+    """cfg::HomoConfig (cmdline.h:12-33) with the defaults of
+    HomoConfig::init (cmdline.cpp:4-35), plus the multigrid knobs."""
+
+    def __init__(self, reso=128, heatRatio=(1.0, 0.0)):
+        r = (reso, reso, reso) if np.isscalar(reso) else tuple(reso)
+        self.reso = [int(v) for v in r]
+        self.heatRatio = [float(heatRatio[0]), float(heatRatio[1])]
+        self.volRatio = 0.3
+        self.finthres = 5e-4
+        self.filterRadius = 2.0
+        self.designStep = 0.05
+        self.dampRatio = 0.5
+        self.femRelThres = 1e-6
+        self.target_tensor = [0.3, 0.2, 0.1, 0.0, 0.0, 0.0]
+        self.max_iter = 500
+        self.max_cg_iter = 200
+        self.pre_sweeps = 2
+        self.post_sweeps = 2
+        self.min_coarse = 4
+        self.device = 0
+        self.verbose = 0
+
+    def _c(self):
+        c = _capi.OtmConfig()
+        _capi.load().otm_config_init(ctypes.byref(c))
+        for a in range(3):
+            c.reso[a] = self.reso[a]
+        c.heat_ratio[0], c.heat_ratio[1] = self.heatRatio
+        c.fem_rel_thres = self.femRelThres
+        c.max_cg_iter = self.max_cg_iter
+        c.pre_sweeps = self.pre_sweeps
+        c.post_sweeps = self.post_sweeps
+        c.min_coarse = self.min_coarse
+        c.device = self.device
+        c.verbose = self.verbose
+        return c
+
+
+class DensityParams:
+    """Density expression of runCustom (Framework.cu:163): Spline4 r=1.5
+    non-periodic filter, then kappa = k1 + (k0-k1) rho^3."""
+
+    def __init__(self, filter_kind=0, filter_radius=1.5, filter_periodic=False):
+        self.filter_kind = filter_kind
+        self.filter_radius = filter_radius
+        self.filter_periodic = filter_periodic
+
+    def _c(self):
+        p = _capi.OtmDensityParams()
+        _capi.load().otm_density_params_init(ctypes.byref(p))
+        p.filter_kind = self.filter_kind
+        p.filter_radius = self.filter_radius
+        p.filter_periodic = 1 if self.filter_periodic else 0
+        return p
+
+
+class Homogenization_H:
+    """homo::Homogenization_H: build / update / heatMatrix / Sensitivity."""
+
+    def __init__(self, config):
+        self._ctx = None
+        self.build(config)
+
+    def build(self, config):
+        lib = _capi.load()
+        self.close()
+        self.config = config
+        ctx = ctypes.c_void_p()
+        cfg = config._c()
+        _capi.check(lib.otm_create(ctypes.byref(cfg), ctypes.byref(ctx)), "otm_create")
+        self._ctx = ctx
+        self.reso = tuple(config.reso)
+        self.ne = self.reso[0] * self.reso[1] * self.reso[2]
+
+    def close(self):
+        if self._ctx:
+            _capi.load().otm_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- hierarchy -----------------------------------------------------
+    def levels(self):
+        lib = _capi.load()
+        n = ctypes.c_int()
+        _capi.check(lib.otm_levels(self._ctx, ctypes.byref(n)), "otm_levels")
+        out = []
+        for l in range(n.value):
+            r = (ctypes.c_int * 3)()
+            _capi.check(lib.otm_level_reso(self._ctx, l, r), "otm_level_reso")
+            out.append(tuple(r))
+        return out
+
+    # -- Homogenization_H interface --------------------------------------
+    def update(self, rho=None, pitchT=-1):
+        """Homogenization_H::update: rho is the SIMP-mapped conductivity."""
+        lib = _capi.load()
+        if rho is None:
+            _capi.check(lib.otm_update(self._ctx, None, self.reso[0], OTM_HOST), "otm_update")
+            return
+        pitch = self.reso[0] if pitchT is None or pitchT < 0 else int(pitchT)
+        p, where, keep = _arg(rho, pitch * self.reso[1] * self.reso[2])
+        _capi.check(lib.otm_update(self._ctx, p, pitch, where), "otm_update")
+
+    def heatMatrix(self):
+        C = (ctypes.c_double * 9)()
+        _capi.check(_capi.load().otm_heat_matrix(self._ctx, C), "otm_heat_matrix")
+        return np.array(C[:], dtype=np.float64).reshape(3, 3)
+
+    def solve_stats(self):
+        it = (ctypes.c_int * 3)()
+        rr = (ctypes.c_double * 3)()
+        _capi.check(_capi.load().otm_solve_stats(self._ctx, it, rr), "otm_solve_stats")
+        return list(it), list(rr)
+
+    def Sensitivity(self, dC, sens=None, pitchT=-1, lexiOrder=True):
+        """Homogenization_H::Sensitivity with lexiOrder=true. Returns the
+        sensitivity array (host) or writes into a DeviceBuffer."""
+        if not lexiOrder:
+            raise ValueError("only lexicographic output is provided (the reference callers use lexiOrder=true)")
+        dCa = (ctypes.c_double * 9)(*np.asarray(dC, dtype=np.float64).reshape(9).tolist())
+        pitch = self.reso[0] if pitchT is None or pitchT < 0 else int(pitchT)
+        n = pitch * self.reso[1] * self.reso[2]
+        if sens is None:
+            sens = np.zeros(n, np.float32)
+        p, where, keep = _arg(sens, n)
+        _capi.check(_capi.load().otm_sensitivity(self._ctx, dCa, p, pitch, where), "otm_sensitivity")
+        return keep if where == OTM_HOST else sens
+
+    def element_energy(self):
+        out = np.empty(6 * self.ne, np.float32)
+        _capi.check(_capi.load().otm_element_energy(self._ctx, _ptr(out), OTM_HOST), "otm_element_energy")
+        return out.reshape(6, self.ne)
+
+    def get_chi(self):
+        out = np.empty(3 * self.ne, np.float32)
+        _capi.check(_capi.load().otm_get_chi(self._ctx, _ptr(out), OTM_HOST), "otm_get_chi")
+        return out.reshape(3, self.ne)
+
+    def set_chi(self, chi):
+        a = _f32(chi, 3 * self.ne)
+        _capi.check(_capi.load().otm_set_chi(self._ctx, _ptr(a), OTM_HOST), "otm_set_chi")
+
+    def reset_displacement(self):
+        _capi.check(_capi.load().otm_reset_displacement(self._ctx), "otm_reset_displacement")
+
+    def apply_level(self, level, u):
+        n = 1
+        for v in self.levels()[level]:
+            n *= v
+        a = _f32(u, 3 * n)
+        q = np.empty(3 * n, np.float32)
+        _capi.check(_capi.load().otm_apply_level(self._ctx, level, _ptr(a), _ptr(q), OTM_HOST), "otm_apply_level")
+        return q
+
+    def vcycle(self, r):
+        a = _f32(r, 3 * self.ne)
+        z = np.empty(3 * self.ne, np.float32)
+        _capi.check(_capi.load().otm_vcycle(self._ctx, _ptr(a), _ptr(z), OTM_HOST), "otm_vcycle")
+        return z
+
+    def stencil(self, level):
+        n = 1
+        for v in self.levels()[level]:
+            n *= v
+        st = np.empty(27 * n, np.float32)
+        _capi.check(_capi.load().otm_get_stencil(self._ctx, level, _ptr(st), OTM_HOST), "otm_get_stencil")
+        return st.reshape(27, n)
+
+    # -- density pipeline (AutoDiff parts of runCustom) ------------------
+    def set_design(self, rho):
+        p, where, keep = _arg(rho, self.ne)
+        _capi.check(_capi.load().otm_set_design(self._ctx, p, where), "otm_set_design")
+
+    def get_design(self):
+        out = np.empty(self.ne, np.float32)
+        _capi.check(_capi.load().otm_get_design(self._ctx, _ptr(out), OTM_HOST), "otm_get_design")
+        return out
+
+    def eval(self, params=None):
+        params = params or DensityParams()
+        p = params._c()
+        C = (ctypes.c_double * 9)()
+        _capi.check(_capi.load().otm_eval(self._ctx, ctypes.byref(p), C), "otm_eval")
+        return np.array(C[:], dtype=np.float64).reshape(3, 3)
+
+    def backward(self, dC, want=True):
+        dCa = (ctypes.c_double * 9)(*np.asarray(dC, dtype=np.float64).reshape(9).tolist())
+        if not want:
+            _capi.check(_capi.load().otm_backward(self._ctx, dCa, None, OTM_HOST), "otm_backward")
+            return None
+        out = np.empty(self.ne, np.float32)
+        _capi.check(_capi.load().otm_backward(self._ctx, dCa, _ptr(out), OTM_HOST), "otm_backward")
+        return out
+
+    def synchronize(self):
+        _capi.check(_capi.load().otm_synchronize(self._ctx), "otm_synchronize")
+
+    # -- live kernel timing --------------------------------------------
+    def profile_enable(self, on=True):
+        _capi.check(_capi.load().otm_profile_enable(self._ctx, 1 if on else 0), "otm_profile_enable")
+
+    def profile_read(self):
+        """{class: (launches, total_ms, algorithmic_bytes)} since enabling."""
+        lib = _capi.load()
+        out = {}
+        for k, name in enumerate(_capi.PROF_CLASSES):
+            n = ctypes.c_longlong()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            _capi.check(lib.otm_profile_read(self._ctx, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
+                        "otm_profile_read")
+            out[name] = (n.value, ms.value, by.value)
+        return out
+
+
+class OCOptimizer:
+    """homo::OCOptimizer(int ne, float min_density, float stepLimit,
+    float dampExponent) with filterSens and update."""
+
+    def __init__(self, ne, min_density=0.001, stepLimit=0.02, dampExponent=0.5, device=0,
+                 bisect_steps=20, bisect_tol=1e-4):
+        lib = _capi.load()
+        self.ne = int(ne)
+        h = ctypes.c_void_p()
+        _capi.check(lib.otm_oc_create(self.ne, min_density, stepLimit, dampExponent, device, ctypes.byref(h)),
+                    "otm_oc_create")
+        self._h = h
+        _capi.check(lib.otm_oc_set_bisection(h, int(bisect_steps), float(bisect_tol)), "otm_oc_set_bisection")
+        self.last_steps = 0
+        self.last_volume = 0.0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.load().otm_oc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def filterSens(self, sens, rho, pitchT, reso, radius=2.0):
+        """In place on ``sens`` (numpy float32 array or DeviceBuffer)."""
+        reso3 = (ctypes.c_int * 3)(*[int(v) for v in reso])
+        n = int(pitchT) * reso3[1] * reso3[2]
+        if isinstance(sens, np.ndarray):
+            if sens.dtype != np.float32 or not sens.flags.c_contiguous:
+                raise ValueError("sens must be a contiguous float32 array (updated in place)")
+        ps, ws, ks = _arg(sens, n)
+        pr, wr, kr = _arg(rho, n)
+        if ws != wr:
+            raise ValueError("sens and rho must both be host or both device")
+        _capi.check(_capi.load().otm_oc_filter_sens(self._h, ps, pr, int(pitchT), reso3, float(radius), ws),
+                    "otm_oc_filter_sens")
+        return sens
+
+    def update(self, sens, rho, volratio):
+        """In place on ``rho``; returns it."""
+        if isinstance(rho, np.ndarray):
+            if rho.dtype != np.float32 or not rho.flags.c_contiguous:
+                raise ValueError("rho must be a contiguous float32 array (updated in place)")
+        ps, ws, ks = _arg(sens, self.ne)
+        pr, wr, kr = _arg(rho, self.ne)
+        if ws != wr:
+            raise ValueError("sens and rho must both be host or both device")
+        st = ctypes.c_int()
+        vol = ctypes.c_double()
+        _capi.check(_capi.load().otm_oc_update(self._h, ps, pr, float(volratio), ws, ctypes.byref(st),
+                                               ctypes.byref(vol)), "otm_oc_update")
+        self.last_steps, self.last_volume = st.value, vol.value
+        return rho
+
+    def step(self, hom, filter_radius, volratio):
+        """filterSens + update on the design held by a Homogenization_H."""
+        st = ctypes.c_int()
+        vol = ctypes.c_double()
+        _capi.check(_capi.load().otm_oc_step(hom._ctx, self._h, float(filter_radius), float(volratio),
+                                             ctypes.byref(st), ctypes.byref(vol)), "otm_oc_step")
+        self.last_steps, self.last_volume = st.value, vol.value
+
+
+def conv_filter(x, kind=0, radius=1.5, periodic=False, adjoint=False):
+    """Radial renormalised convolution of a (nz, ny, nx) float32 field."""
+    a = _f32(x)
+    if a.ndim != 3:
+        raise ValueError("x must be 3-D (nz, ny, nx)")
+    nz, ny, nx = a.shape
+    reso = (ctypes.c_int * 3)(nx, ny, nz)
+    out = np.empty_like(a)
+    _capi.check(_capi.load().otm_conv_filter(reso, int(kind), float(radius), 1 if periodic else 0,
+                                             1 if adjoint else 0, _ptr(a), _ptr(out), OTM_HOST),
+                "otm_conv_filter")
+    return out

@@ -1,0 +1,127 @@
+"""ctypes declarations of the C ABI in include/opentm.h.
+
+Loading fails loudly when libopentm.so is missing: there is no CPU
+fallback for any engine call.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopentm.so")
+
+OTM_OK = 0
+OTM_E_ARG = -1
+OTM_E_HIP = -2
+OTM_E_NOCONV = -3
+OTM_E_UNSUPPORTED = -4
+OTM_E_NUMERIC = -5
+OTM_E_NODEV = -6
+OTM_E_STATE = -7
+OTM_E_OOM = -8
+OTM_HOST = 0
+OTM_DEVICE = 1
+PROF_CLASSES = ["fine_gs", "fine_apply", "fine_resid", "coarse_gs", "coarse_resid", "transfer",
+                "coarsest", "vector", "energy", "setup"]
+
+# Every function the header declares, with (restype, argtypes).
+_c = ctypes
+_p = _c.c_void_p
+_i = _c.c_int
+_d = _c.c_double
+_ip = _c.POINTER(_c.c_int)
+_dp = _c.POINTER(_c.c_double)
+
+
+class OtmConfig(ctypes.Structure):
+    _fields_ = [
+        ("reso", _c.c_int * 3),
+        ("heat_ratio", _c.c_double * 2),
+        ("fem_rel_thres", _c.c_double),
+        ("max_cg_iter", _c.c_int),
+        ("pre_sweeps", _c.c_int),
+        ("post_sweeps", _c.c_int),
+        ("min_coarse", _c.c_int),
+        ("device", _c.c_int),
+        ("verbose", _c.c_int),
+    ]
+
+
+class OtmDensityParams(ctypes.Structure):
+    _fields_ = [
+        ("filter_kind", _c.c_int),
+        ("filter_radius", _c.c_double),
+        ("filter_periodic", _c.c_int),
+        ("penal", _c.c_double),
+    ]
+
+
+SIGNATURES = {
+    "otm_config_init": (None, [_c.POINTER(OtmConfig)]),
+    "otm_last_error": (_c.c_char_p, []),
+    "otm_version": (_i, []),
+    "otm_create": (_i, [_c.POINTER(OtmConfig), _c.POINTER(_p)]),
+    "otm_destroy": (_i, [_p]),
+    "otm_levels": (_i, [_p, _ip]),
+    "otm_level_reso": (_i, [_p, _i, _ip]),
+    "otm_update": (_i, [_p, _p, _i, _i]),
+    "otm_heat_matrix": (_i, [_p, _dp]),
+    "otm_solve_stats": (_i, [_p, _ip, _dp]),
+    "otm_sensitivity": (_i, [_p, _dp, _p, _i, _i]),
+    "otm_element_energy": (_i, [_p, _p, _i]),
+    "otm_get_chi": (_i, [_p, _p, _i]),
+    "otm_set_chi": (_i, [_p, _p, _i]),
+    "otm_reset_displacement": (_i, [_p]),
+    "otm_apply_level": (_i, [_p, _i, _p, _p, _i]),
+    "otm_vcycle": (_i, [_p, _p, _p, _i]),
+    "otm_get_stencil": (_i, [_p, _i, _p, _i]),
+    "otm_conv_filter": (_i, [_ip, _i, _d, _i, _i, _p, _p, _i]),
+    "otm_density_params_init": (None, [_c.POINTER(OtmDensityParams)]),
+    "otm_set_design": (_i, [_p, _p, _i]),
+    "otm_get_design": (_i, [_p, _p, _i]),
+    "otm_eval": (_i, [_p, _c.POINTER(OtmDensityParams), _dp]),
+    "otm_backward": (_i, [_p, _dp, _p, _i]),
+    "otm_oc_create": (_i, [_i, _d, _d, _d, _i, _c.POINTER(_p)]),
+    "otm_oc_destroy": (_i, [_p]),
+    "otm_oc_set_bisection": (_i, [_p, _i, _d]),
+    "otm_oc_filter_sens": (_i, [_p, _p, _p, _i, _ip, _d, _i]),
+    "otm_oc_update": (_i, [_p, _p, _p, _d, _i, _ip, _dp]),
+    "otm_oc_step": (_i, [_p, _p, _d, _d, _ip, _dp]),
+    "otm_profile_enable": (_i, [_p, _i]),
+    "otm_profile_read": (_i, [_p, _i, _c.POINTER(_c.c_longlong), _dp, _dp]),
+    "otm_device_alloc": (_i, [_i, _c.c_size_t, _c.POINTER(_p)]),
+    "otm_device_free": (_i, [_p]),
+    "otm_memcpy": (_i, [_p, _p, _c.c_size_t, _i, _i]),
+    "otm_synchronize": (_i, [_p]),
+}
+
+_lib = None
+
+
+class OtmError(RuntimeError):
+    def __init__(self, code, func, msg):
+        super().__init__(f"{func} failed with status {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libopentm.so (built by opentm_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP engine first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, func):
+    if status != OTM_OK:
+        msg = load().otm_last_error()
+        raise OtmError(status, func, msg.decode() if msg else "")

@@ -1,0 +1,67 @@
+"""Build recipe for the gfx950 engine (libopentm.so) and the FP64 oracle.
+
+The engine is compiled with hipcc for gfx950 only, into the package
+directory so the shared object travels with the source tree. The oracle
+(test infrastructure) is compiled with gcc through oracle/Makefile.
+"""
+import concurrent.futures
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libopentm.so")
+SOURCES = ["otm_fine.hip", "otm_coarse.hip", "otm_vec.hip", "otm_opt.hip", "otm_engine.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIPFLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]) + " ...")
+    return r
+
+
+def build_engine(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    jobs = []
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        for src in SOURCES:
+            obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+            objs.append(obj)
+            cmd = [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+            jobs.append(ex.submit(_run, cmd))
+        for j in jobs:
+            j.result()
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def build_oracle(verbose=False):
+    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
+    if verbose:
+        print("built oracle/liboracle.so")
+
+
+def build_all(verbose=False):
+    build_oracle(verbose)
+    build_engine(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)

@@ -1,0 +1,384 @@
+// otm_coarse.hip -- coarse multigrid levels: stored 27-point stencils,
+// transfers, Galerkin assembly and the coarsest dense FP64 solve.
+//
+// Stencil layout: st[o*nn + v] couples node v to v + off(o) with
+// off(o) = (o%3-1, o/3%3-1, o/9-1); o = 13 is the diagonal (the reference
+// keeps 27 SoA arrays the same way, grid.cpp:189-220, diagonal at index 13,
+// grid.cu:707-809).
+//
+// Galerkin coarse operators A_{l+1} = P^T A_l P with trilinear P
+// (grid.cu:811-940, grid_ex.cu:16-227) are formed element by element: a
+// coarse element's 8x8 matrix is the sum over its 8 children c of
+// Q_c^T E_c Q_c, where Q_c interpolates the coarse element's corners to the
+// child's corners. This equals the stencil-level triple product exactly.
+#include "otm_kernels.hpp"
+
+namespace otm {
+
+static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__device__ inline long long lin(const Geom& g, int x, int y, int z) {
+  return (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+}
+
+// GS update of one of 8 colours (parity of x, y, z), colour = cx + 2cy + 4cz.
+__global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __restrict__ st,
+                                                        const float* __restrict__ f, float* u, int cx, int cy,
+                                                        int cz) {
+  const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
+  const long long cnt = (long long)hx * hy * hz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int x = 2 * (int)(t % hx) + cx;
+  const long long r = t / hx;
+  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+  const long long v = lin(g, x, y, z);
+  const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+  const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+  const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+  float a[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) a[o] = st[o * g.nn + v];
+  const float inv = 1.f / a[13];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * g.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+      if (o == 13) continue;
+      s += a[o] * uk[lin(g, xs[o % 3], ys[(o / 3) % 3], zs[o / 9])];
+    }
+    u[k * g.nn + v] = (f[k * g.nn + v] - s) * inv;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, const float* __restrict__ st,
+                                                        const float* __restrict__ f, const float* __restrict__ u,
+                                                        float* __restrict__ res) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+  const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+  const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+  float a[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) a[o] = st[o * g.nn + t];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * g.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) s += a[o] * uk[lin(g, xs[o % 3], ys[(o / 3) % 3], zs[o / 9])];
+    res[k * g.nn + t] = f[k * g.nn + t] - s;
+  }
+}
+
+// f_c(I) = sum_{d in {-1,0,1}^3} w(d) res(2I + d), w = prod(d_i == 0 ? 1 : 1/2)
+// (restrict_residual_kernel_H, grid.cu:811-878, weights (2-|d|)/2 per axis).
+__global__ void __launch_bounds__(kBlock) k_restrict(Geom fg, Geom cg, const float* __restrict__ res,
+                                                     float* __restrict__ fc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cg.nn) return;
+  const int X = (int)(t % cg.nx);
+  const long long r = t / cg.nx;
+  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  const int x = 2 * X, y = 2 * Y, z = 2 * Z;
+  const int xs[3] = {x == 0 ? fg.nx - 1 : x - 1, x, x + 1};
+  const int ys[3] = {y == 0 ? fg.ny - 1 : y - 1, y, y + 1};
+  const int zs[3] = {z == 0 ? fg.nz - 1 : z - 1, z, z + 1};
+  const float w1[3] = {0.5f, 1.f, 0.5f};
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* rk = res + k * fg.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        float row = 0.f;
+        const long long rb = (long long)fg.nx * ((long long)ys[b] + (long long)fg.ny * zs[c]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) row += w1[a] * rk[rb + xs[a]];
+        s += w1[b] * w1[c] * row;
+      }
+    fc[k * cg.nn + t] = s;
+  }
+}
+
+// u_f(i) += sum_I P(i, I) u_c(I), trilinear (prolongate_correction_kernel_H,
+// grid.cu:881-940).
+__global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const float* __restrict__ uc,
+                                                        float* __restrict__ uf) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.nn) return;
+  const int x = (int)(t % fg.nx);
+  const long long r = t / fg.nx;
+  const int y = (int)(r % fg.ny), z = (int)(r / fg.ny);
+  int X[2], Y[2], Z[2];
+  float wx[2], wy[2], wz[2];
+  X[0] = x >> 1; X[1] = (x & 1) ? ((x >> 1) + 1 == cg.nx ? 0 : (x >> 1) + 1) : X[0];
+  wx[0] = (x & 1) ? 0.5f : 1.f; wx[1] = (x & 1) ? 0.5f : 0.f;
+  Y[0] = y >> 1; Y[1] = (y & 1) ? ((y >> 1) + 1 == cg.ny ? 0 : (y >> 1) + 1) : Y[0];
+  wy[0] = (y & 1) ? 0.5f : 1.f; wy[1] = (y & 1) ? 0.5f : 0.f;
+  Z[0] = z >> 1; Z[1] = (z & 1) ? ((z >> 1) + 1 == cg.nz ? 0 : (z >> 1) + 1) : Z[0];
+  wz[0] = (z & 1) ? 0.5f : 1.f; wz[1] = (z & 1) ? 0.5f : 0.f;
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* ck = uc + k * cg.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          const float w = wx[a] * wy[b] * wz[c];
+          if (w != 0.f) s += w * ck[lin(cg, X[a], Y[b], Z[c])];
+        }
+    uf[k * fg.nn + t] += s;
+  }
+}
+
+// Level-1 element matrices from kappa: E(E) = sum_c kappa_c G[c].
+__global__ void __launch_bounds__(kBlock) k_galerkin_first(Geom fg, Geom cg, GalerkinConsts G,
+                                                           const float* __restrict__ kappa, float* __restrict__ E) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cg.nn) return;
+  const int X = (int)(t % cg.nx);
+  const long long r = t / cg.nx;
+  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  float kc[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    kc[c] = kappa[lin(fg, 2 * X + (c & 1), 2 * Y + ((c >> 1) & 1), 2 * Z + (c >> 2))];
+#pragma unroll
+  for (int q = 0; q < 36; q++) {
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) s += kc[c] * G.G[c][q];
+    E[q * cg.nn + t] = s;
+  }
+}
+
+// 1-D interpolation weight of a fine point p (0..2) from coarse corner M (0/1).
+__device__ constexpr float w1d(int p, int M) {
+  return (p - 2 * M == 0) ? 1.f : ((p - 2 * M == 1 || p - 2 * M == -1) ? 0.5f : 0.f);
+}
+
+// Deeper levels: Eout(E) = sum_c Q_c^T Ein(child c) Q_c.
+__global__ void __launch_bounds__(kBlock) k_galerkin_next(Geom fg, Geom cg, const float* __restrict__ Ein,
+                                                          float* __restrict__ Eout) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cg.nn) return;
+  const int X = (int)(t % cg.nx);
+  const long long r = t / cg.nx;
+  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  float acc[36];
+#pragma unroll
+  for (int q = 0; q < 36; q++) acc[q] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+    const long long child = lin(fg, 2 * X + cx, 2 * Y + cy, 2 * Z + cz);
+    float e[36];
+#pragma unroll
+    for (int q = 0; q < 36; q++) e[q] = Ein[q * fg.nn + child];
+    // T = E Q  (8 x 8)
+    float T[8][8];
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        float s = 0.f;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          const float qnb = w1d(cx + (n & 1), b & 1) * w1d(cy + ((n >> 1) & 1), (b >> 1) & 1) *
+                            w1d(cz + (n >> 2), b >> 2);
+          if (qnb != 0.f) s += e[tri8(m, n)] * qnb;
+        }
+        T[m][b] = s;
+      }
+    // acc(a,b) += sum_m Q(m,a) T(m,b)
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int b = a; b < 8; b++) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+          const float qma = w1d(cx + (m & 1), a & 1) * w1d(cy + ((m >> 1) & 1), (a >> 1) & 1) *
+                            w1d(cz + (m >> 2), a >> 2);
+          if (qma != 0.f) s += qma * T[m][b];
+        }
+        acc[tri8(a, b)] += s;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < 36; q++) Eout[q * cg.nn + t] = acc[q];
+}
+
+// Node stencil from element matrices; the diagonal is reset to minus the sum
+// of the off-diagonal couplings so that constants stay in the null space.
+__global__ void __launch_bounds__(kBlock) k_stencil_from_elements(Geom g, const float* __restrict__ E,
+                                                                  float* __restrict__ st) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float a[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) a[o] = 0.f;
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const int ax = e & 1, ay = (e >> 1) & 1, az = e >> 2;
+    const int ex = wrapm(x - 1 + ax, g.nx), ey = wrapm(y - 1 + ay, g.ny), ez = wrapm(z - 1 + az, g.nz);
+    const long long eid = lin(g, ex, ey, ez);
+    const int lv = (1 - ax) + 2 * (1 - ay) + 4 * (1 - az);
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const int o = (ax + (m & 1)) + 3 * (ay + ((m >> 1) & 1)) + 9 * (az + (m >> 2));
+      a[o] += E[tri8(lv, m) * g.nn + eid];
+    }
+  }
+  double off = 0.0;
+#pragma unroll
+  for (int o = 0; o < 27; o++)
+    if (o != 13) off += (double)a[o];
+  a[13] = (float)(-off);
+#pragma unroll
+  for (int o = 0; o < 27; o++) st[o * g.nn + t] = a[o];
+}
+
+// Coarsest level: dense FP64 matrix from the stencil, symmetrised, zero row
+// sums, plus (tr/n^2) * 1 1^T to remove the constant null space, Cholesky
+// factor L, X = L^{-1}, Binv = X^T X. One 1024-thread workgroup.
+constexpr int kFactorThreads = 1024;
+__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, const float* __restrict__ st, double* B,
+                                                                    double* Xm, double* Binv) {
+  const int n = (int)g.nn;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  for (long long i = tid; i < (long long)n * n; i += bs) B[i] = 0.0;
+  __syncthreads();
+  for (int v = tid; v < n; v += bs) {
+    const int x = v % g.nx, y = (v / g.nx) % g.ny, z = v / (g.nx * g.ny);
+    for (int o = 0; o < 27; o++) {
+      const int w = (int)lin(g, wrapm(x + o % 3 - 1, g.nx), wrapm(y + (o / 3) % 3 - 1, g.ny), wrapm(z + o / 9 - 1, g.nz));
+      B[(long long)v * n + w] += (double)st[o * g.nn + v];
+    }
+  }
+  __syncthreads();
+  for (long long i = tid; i < (long long)n * n; i += bs) {
+    const int a = (int)(i / n), b = (int)(i % n);
+    if (a < b) {
+      const double m = 0.5 * (B[(long long)a * n + b] + B[(long long)b * n + a]);
+      B[(long long)a * n + b] = m;
+      B[(long long)b * n + a] = m;
+    }
+  }
+  __syncthreads();
+  double tr = 0.0;
+  for (int a = tid; a < n; a += bs) {
+    double s = 0.0;
+    for (int b = 0; b < n; b++)
+      if (b != a) s += B[(long long)a * n + b];
+    B[(long long)a * n + a] = -s;
+    tr += -s;
+  }
+  tr = wave_sum(tr);
+  if ((tid & 63) == 0) red[tid >> 6] = tr;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int w = 0; w < bs / 64; w++) s += red[w];
+    alpha_sh = s / ((double)n * (double)n);
+  }
+  __syncthreads();
+  const double alpha = alpha_sh;
+  for (long long i = tid; i < (long long)n * n; i += bs) B[i] += alpha;
+  __syncthreads();
+  // right-looking Cholesky, lower triangle in place
+  for (int k = 0; k < n; k++) {
+    if (tid == 0) B[(long long)k * n + k] = sqrt(B[(long long)k * n + k]);
+    __syncthreads();
+    const double dk = B[(long long)k * n + k];
+    for (int i = k + 1 + tid; i < n; i += bs) B[(long long)i * n + k] /= dk;
+    __syncthreads();
+    const int m = n - k - 1;
+    for (long long q = tid; q < (long long)m * m; q += bs) {
+      const int i = k + 1 + (int)(q / m), j = k + 1 + (int)(q % m);
+      if (j <= i) B[(long long)i * n + j] -= B[(long long)i * n + k] * B[(long long)j * n + k];
+    }
+    __syncthreads();
+  }
+  // X = L^{-1}, one column per thread
+  for (int j = tid; j < n; j += bs) {
+    for (int i = 0; i < j; i++) Xm[(long long)i * n + j] = 0.0;
+    for (int i = j; i < n; i++) {
+      double s = (i == j) ? 1.0 : 0.0;
+      for (int m2 = j; m2 < i; m2++) s -= B[(long long)i * n + m2] * Xm[(long long)m2 * n + j];
+      Xm[(long long)i * n + j] = s / B[(long long)i * n + i];
+    }
+  }
+  __syncthreads();
+  for (long long q = tid; q < (long long)n * n; q += bs) {
+    const int i = (int)(q / n), j = (int)(q % n);
+    const int m0 = i > j ? i : j;
+    double s = 0.0;
+    for (int m2 = m0; m2 < n; m2++) s += Xm[(long long)m2 * n + i] * Xm[(long long)m2 * n + j];
+    Binv[q] = s;
+  }
+}
+
+// u = Binv f per right-hand side, FP64 accumulation.
+__global__ void __launch_bounds__(kBlock) k_coarsest_solve(Geom g, const double* __restrict__ Binv,
+                                                           const float* __restrict__ f, float* __restrict__ u) {
+  const int n = (int)g.nn;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= n) return;
+  const float* fk = f + (long long)k * n;
+  double s = 0.0;
+  for (int j = 0; j < n; j++) s += Binv[(long long)i * n + j] * (double)fk[j];
+  u[(long long)k * n + i] = (float)s;
+}
+
+// ---- launchers ----
+void launch_st_gs_color(hipStream_t s, const Geom& g, const float* st, const float* f, float* u, int color) {
+  const long long cnt = (long long)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
+  hipLaunchKernelGGL(k_st_gs_color, dim3(nblk(cnt)), dim3(kBlock), 0, s, g, st, f, u, color & 1, (color >> 1) & 1,
+                     color >> 2);
+}
+void launch_st_residual(hipStream_t s, const Geom& g, const float* st, const float* f, const float* u, float* res) {
+  hipLaunchKernelGGL(k_st_residual, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, st, f, u, res);
+}
+void launch_restrict(hipStream_t s, const Geom& fine, const Geom& coarse, const float* res, float* fc) {
+  hipLaunchKernelGGL(k_restrict, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, res, fc);
+}
+void launch_prolong_add(hipStream_t s, const Geom& fine, const Geom& coarse, const float* uc, float* uf) {
+  hipLaunchKernelGGL(k_prolong_add, dim3(nblk(fine.nn)), dim3(kBlock), 0, s, fine, coarse, uc, uf);
+}
+void launch_galerkin_first(hipStream_t s, const Geom& fine, const Geom& coarse, const GalerkinConsts& G,
+                           const float* kappa, float* E) {
+  hipLaunchKernelGGL(k_galerkin_first, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, G, kappa, E);
+}
+void launch_galerkin_next(hipStream_t s, const Geom& fine, const Geom& coarse, const float* Ein, float* Eout) {
+  hipLaunchKernelGGL(k_galerkin_next, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, Ein, Eout);
+}
+void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, float* st) {
+  hipLaunchKernelGGL(k_stencil_from_elements, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, E, st);
+}
+void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv) {
+  hipLaunchKernelGGL(k_coarsest_factor, dim3(1), dim3(kFactorThreads), 0, s, g, st, B, X, Binv);
+}
+void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u) {
+  hipLaunchKernelGGL(k_coarsest_solve, dim3(nblk(g.nn), OTM_NRHS), dim3(kBlock), 0, s, g, Binv, f, u);
+}
+
+}  // namespace otm

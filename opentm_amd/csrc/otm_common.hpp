@@ -1,0 +1,188 @@
+// otm_common.hpp -- shared device/host definitions for the gfx950 engine.
+//
+// Layout of every level: a periodic nx*ny*nz lattice, lexicographic with x
+// fastest (index = x + nx*(y + ny*z)). Nodes and elements share the index:
+// element (x,y,z) spans nodes (x..x+1, y..y+1, z..z+1) modulo the size. The
+// three load cases are stored structure-of-arrays: right-hand side k of a
+// level vector lives at base + k*nn.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OTM_NRHS 3
+
+namespace otm {
+
+// Level geometry passed by value to every kernel (no global device state).
+struct Geom {
+  int nx, ny, nz;
+  long long nn;  // nx*ny*nz
+};
+
+__host__ __device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Element constants of the fine level (closed forms of the reference's
+// quadrature, templateMatrix.cpp:21-87): Ke = c * P with P = 4 on the
+// diagonal, 0 between nodes that differ along one axis, -1 otherwise,
+// c = h/12. disp / feMu are the reference's 8x3 tables (local node, load).
+struct FineConsts {
+  float c;            // h/12
+  float disp[8][3];   // element-local macro temperature per load case
+  float femu[8][3];   // element load vectors
+};
+
+// Coarse-level Galerkin constants: G[c][t] = (Q_c^T Ke Q_c) packed upper
+// triangle (t over a<=b) for child position c of a 2x coarsened element.
+struct GalerkinConsts {
+  float G[8][36];
+};
+
+// Packed upper-triangle index of (a,b), a<=b, of an 8x8 symmetric matrix.
+__host__ __device__ constexpr int tri8(int a, int b) {
+  return a <= b ? (a * 8 - (a * (a - 1)) / 2 + (b - a)) : (b * 8 - (b * (b - 1)) / 2 + (a - b));
+}
+
+// Scalars of the three lock-step conjugate-gradient recurrences (FP64).
+struct PcgScalars {
+  double rz[OTM_NRHS];     // r.z of the current iteration
+  double pq[OTM_NRHS];     // p.Kp
+  double rr[OTM_NRHS];     // r.r
+  double bb[OTM_NRHS];     // b.b (reference norm)
+  double alpha[OTM_NRHS];
+  double beta[OTM_NRHS];
+  double mean[OTM_NRHS];   // scratch for mean removal
+  int active[OTM_NRHS];    // 1 while the right-hand side is iterating
+  int fault;               // set when a reduction produced NaN/Inf
+};
+
+// Reduction finalisers run by the last block of a reduction kernel.
+enum FinalMode : int {
+  FIN_STORE = 0,     // store sums into out[0..nv)
+  FIN_BB_RR = 1,     // sums = {b.b x3, r.r x3}: initial norms
+  FIN_RZ_FIRST = 2,  // sums = r.z x3: rz = sum, beta = 0
+  FIN_RZ = 3,        // sums = r.z x3: beta = sum/rz, rz = sum
+  FIN_PQ = 4,        // sums = p.q x3: alpha = rz/pq
+  FIN_RR = 5,        // sums = r.r x3: update activity against rtol
+  FIN_MEAN = 6,      // sums = sum(x) x3: mean = sum / nn
+};
+
+struct Reduce {
+  double* partials;      // [nblocks * nv]
+  unsigned int* counter; // zero between launches; reset by the last block
+  double* out;           // FIN_STORE target
+  PcgScalars* pcg;       // for PCG modes
+  double rtol2;          // squared relative tolerance for FIN_RR
+  double inv_nn;         // 1/nn for FIN_MEAN
+  int mode;
+};
+
+constexpr int kBlock = 256;       // threads per block for streaming kernels
+constexpr int kWaves = kBlock / 64;
+
+// Wave64 sum through cross-lane shuffles.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of NV values; result valid in thread 0. `sh` holds
+// NV*kWaves doubles.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) sh[i * kWaves + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      double s = 0;
+      for (int w = 0; w < kWaves; w++) s += sh[i * kWaves + w];
+      v[i] = s;
+    }
+  }
+}
+
+// Deterministic two-level reduction: every block stores its partial sums;
+// the last block to finish (atomic ticket after a device-scope fence) sums
+// the partials in block order and applies the finaliser. Fixed summation
+// order makes results independent of scheduling.
+template <int NV>
+__device__ inline void reduce_finish(double (&v)[NV], const Reduce& R) {
+  __shared__ double sh[NV * kWaves];
+  __shared__ bool last;
+  block_sum<NV>(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) R.partials[(size_t)blockIdx.x * NV + i] = v[i];
+    __threadfence();
+    unsigned t = atomicAdd(R.counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int nb = gridDim.x;
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) acc[i] = 0.0;
+  // each thread sums a contiguous strip of blocks, then a fixed-order block sum
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) acc[i] += R.partials[(size_t)b * NV + i];
+  }
+  __syncthreads();
+  block_sum<NV>(acc, sh);
+  if (threadIdx.x != 0) return;
+  *R.counter = 0u;
+  PcgScalars* P = R.pcg;
+  switch (R.mode) {
+    case FIN_STORE:
+#pragma unroll
+      for (int i = 0; i < NV; i++) R.out[i] = acc[i];
+      break;
+    case FIN_BB_RR:
+      for (int k = 0; k < OTM_NRHS; k++) {
+        P->bb[k] = acc[k];
+        P->rr[k] = acc[OTM_NRHS + k];
+        double bb = acc[k] > 0 ? acc[k] : 1.0;
+        P->active[k] = (acc[OTM_NRHS + k] > R.rtol2 * bb) ? 1 : 0;
+        P->alpha[k] = 0.0;
+        P->beta[k] = 0.0;
+      }
+      break;
+    case FIN_RZ_FIRST:
+      for (int k = 0; k < OTM_NRHS; k++) { P->rz[k] = acc[k]; P->beta[k] = 0.0; }
+      break;
+    case FIN_RZ:
+      for (int k = 0; k < OTM_NRHS; k++) {
+        P->beta[k] = (P->active[k] && P->rz[k] != 0.0) ? acc[k] / P->rz[k] : 0.0;
+        P->rz[k] = acc[k];
+      }
+      break;
+    case FIN_PQ:
+      for (int k = 0; k < OTM_NRHS; k++) {
+        P->pq[k] = acc[k];
+        P->alpha[k] = (P->active[k] && acc[k] > 0.0) ? P->rz[k] / acc[k] : 0.0;
+      }
+      break;
+    case FIN_RR:
+      for (int k = 0; k < OTM_NRHS; k++) {
+        P->rr[k] = acc[k];
+        double bb = P->bb[k] > 0 ? P->bb[k] : 1.0;
+        if (!(acc[k] - acc[k] == 0.0)) P->fault = 1;  // NaN or Inf
+        if (!(acc[k] > R.rtol2 * bb)) P->active[k] = 0;
+      }
+      break;
+    case FIN_MEAN:
+      for (int k = 0; k < OTM_NRHS; k++) P->mean[k] = acc[k] * R.inv_nn;
+      break;
+  }
+}
+
+}  // namespace otm

@@ -1,0 +1,1195 @@
+// otm_engine.hip -- host engine behind include/opentm.h.
+//
+// One context = one periodic cell with its multigrid hierarchy, three
+// lock-step PCG recurrences (one per load case), a HIP stream and the
+// design-density buffers of the optimisation loop. There is no global device
+// state: every kernel receives its level geometry and constants by value,
+// so any number of contexts can be resident at once (the reference keeps
+// per-grid state in __constant__ symbols, grid.cu:19-56, and allows one
+// hierarchy per process).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/opentm.h"
+#include "otm_kernels.hpp"
+
+using namespace otm;
+
+// ---- error plumbing --------------------------------------------------------
+static thread_local std::string g_last_error;
+
+namespace {
+struct OtmError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw OtmError{code, msg}; }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(OTM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return OTM_OK;
+  } catch (const OtmError& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return OTM_E_ARG;
+  }
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    fail(OTM_E_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+  }
+  return static_cast<T*>(p);
+}
+
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+inline hipMemcpyKind kind_of(int dst_where, int src_where) {
+  if (dst_where == OTM_DEVICE && src_where == OTM_DEVICE) return hipMemcpyDeviceToDevice;
+  if (dst_where == OTM_DEVICE) return hipMemcpyHostToDevice;
+  if (src_where == OTM_DEVICE) return hipMemcpyDeviceToHost;
+  return hipMemcpyHostToHost;
+}
+
+void check_where(int w) {
+  if (w != OTM_HOST && w != OTM_DEVICE) fail(OTM_E_ARG, "where must be OTM_HOST or OTM_DEVICE");
+}
+
+// Element constants (closed forms of templateMatrix.cpp:21-87 after the node
+// remap {1,3,2,0,5,7,6,4}): load case k is a unit gradient along grid axis
+// kAxis[k] with sign kSign[k].
+constexpr int kAxis[3] = {1, 0, 2};
+constexpr double kSign[3] = {1.0, -1.0, 1.0};
+
+inline double ke_pattern(int a, int b) {
+  if (a == b) return 4.0;
+  return __builtin_popcount(a ^ b) == 1 ? 0.0 : -1.0;
+}
+
+FineConsts make_fine_consts(double h) {
+  FineConsts fc;
+  fc.c = (float)(h / 12.0);
+  for (int l = 0; l < 8; l++)
+    for (int k = 0; k < 3; k++) {
+      const int bit = (l >> kAxis[k]) & 1;
+      fc.femu[l][k] = (float)(kSign[k] * (bit ? 1.0 : -1.0) * h * h / 4.0);
+      fc.disp[l][k] = (float)(kSign[k] * bit * h);
+    }
+  return fc;
+}
+
+inline double w1d_h(int p, int M) {
+  const int d = p - 2 * M;
+  return d == 0 ? 1.0 : ((d == 1 || d == -1) ? 0.5 : 0.0);
+}
+
+GalerkinConsts make_galerkin_consts(double h) {
+  GalerkinConsts G;
+  const double c = h / 12.0;
+  for (int ch = 0; ch < 8; ch++) {
+    const int cx = ch & 1, cy = (ch >> 1) & 1, cz = ch >> 2;
+    double Q[8][8];
+    for (int m = 0; m < 8; m++)
+      for (int M = 0; M < 8; M++)
+        Q[m][M] = w1d_h(cx + (m & 1), M & 1) * w1d_h(cy + ((m >> 1) & 1), (M >> 1) & 1) *
+                  w1d_h(cz + (m >> 2), M >> 2);
+    for (int a = 0; a < 8; a++)
+      for (int b = a; b < 8; b++) {
+        double s = 0;
+        for (int m = 0; m < 8; m++)
+          for (int n = 0; n < 8; n++) s += Q[m][a] * c * ke_pattern(m, n) * Q[n][b];
+        G.G[ch][tri8(a, b)] = (float)s;
+      }
+  }
+  return G;
+}
+
+// Radial taps (TensorExpression.h:1053-1093): Spline4 1-6q+8q^1.5-3q^2,
+// Linear 1-sqrt(q), q = |d|^2/r^2 < 1, window half-width int(r+0.5), each
+// weight divided by the window total.
+Taps make_taps(int kind, double r, int periodic) {
+  if (!(r > 0.0)) fail(OTM_E_ARG, "filter radius must be positive");
+  const int h = (int)(r + 0.5);
+  if ((2 * h + 1) * (2 * h + 1) * (2 * h + 1) > kMaxTaps) fail(OTM_E_UNSUPPORTED, "filter radius too large (max 3.49)");
+  if (kind != 0 && kind != 1) fail(OTM_E_ARG, "filter kind must be 0 (Spline4) or 1 (Linear)");
+  Taps T;
+  memset(&T, 0, sizeof(T));
+  T.periodic = periodic ? 1 : 0;
+  double tot = 0.0;
+  std::vector<double> ws;
+  std::vector<int> offs;
+  for (int dz = -h; dz <= h; dz++)
+    for (int dy = -h; dy <= h; dy++)
+      for (int dx = -h; dx <= h; dx++) {
+        const double q = (double)(dx * dx + dy * dy + dz * dz) / (r * r);
+        double w = 0.0;
+        if (q < 1.0) w = kind == 0 ? 1.0 - 6.0 * q + 8.0 * q * sqrt(q) - 3.0 * q * q : 1.0 - sqrt(q);
+        tot += w;
+        if (w != 0.0) {
+          ws.push_back(w);
+          offs.push_back(dx);
+          offs.push_back(dy);
+          offs.push_back(dz);
+        }
+      }
+  T.n = (int)ws.size();
+  for (int i = 0; i < T.n; i++) {
+    T.w[i] = (float)(ws[i] / tot);
+    T.dx[i] = (signed char)offs[3 * i];
+    T.dy[i] = (signed char)offs[3 * i + 1];
+    T.dz[i] = (signed char)offs[3 * i + 2];
+  }
+  return T;
+}
+
+unsigned red_blocks_for(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+struct Level {
+  Geom g;
+  float* u = nullptr;    // correction (level 0: the PCG z vector)
+  float* f = nullptr;    // right-hand side (level 0: the PCG residual)
+  float* res = nullptr;  // residual before restriction
+  float* st = nullptr;   // 27-point stencil (levels >= 1)
+  float* E = nullptr;    // element matrices, 36 per element (levels >= 1)
+  bool own_uf = true;
+};
+
+}  // namespace
+
+struct otm_ctx {
+  otm_config cfg;
+  int device = 0;
+  hipStream_t s = nullptr;
+  Geom g0;
+  FineConsts fc;
+  GalerkinConsts G;
+  std::vector<Level> lv;
+  // fine-level PCG state, 3 right-hand sides each
+  float* kappa = nullptr;  // per element, with the 1e-9 floor
+  float* x = nullptr;      // chi (warm start)
+  float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  // coarsest dense solve
+  double *cB = nullptr, *cX = nullptr, *cBinv = nullptr;
+  // reductions
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  double* red_out = nullptr;
+  PcgScalars* pcg = nullptr;
+  PcgScalars* pcg_host = nullptr;  // pinned
+  double* out_host = nullptr;      // pinned, 8 doubles
+  // design pipeline
+  float *rho = nullptr, *rho_f = nullptr, *W = nullptr, *kraw = nullptr, *dkappa = nullptr;
+  float *sens = nullptr, *tmp = nullptr, *drho = nullptr;
+  Taps taps;
+  int taps_kind = -2;
+  double taps_r = 0;
+  int taps_periodic = -1;
+  // live kernel timing (otm_profile_enable)
+  bool prof_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending {
+    int cls;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  long long prof_launches[OTM_PROF_NCLASSES] = {};
+  double prof_ms[OTM_PROF_NCLASSES] = {};
+  double prof_bytes[OTM_PROF_NCLASSES] = {};
+  // status
+  bool has_kappa = false, has_solution = false, has_drho = false;
+  int iters[3] = {0, 0, 0};
+  double relres[3] = {0, 0, 0};
+  double C[9];
+
+  Reduce reducer(int mode, double rtol2 = 0.0, double inv_nn = 0.0) const {
+    Reduce R;
+    R.partials = partials;
+    R.counter = counter;
+    R.out = red_out;
+    R.pcg = pcg;
+    R.rtol2 = rtol2;
+    R.inv_nn = inv_nn;
+    R.mode = mode;
+    return R;
+  }
+  void sync() {
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (!pending.empty()) collect();
+  }
+  hipEvent_t take_event() {
+    if (ev_pool.empty()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      return e;
+    }
+    hipEvent_t e = ev_pool.back();
+    ev_pool.pop_back();
+    return e;
+  }
+  void collect() {
+    for (auto& p : pending) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      prof_ms[p.cls] += ms;
+      prof_launches[p.cls] += 1;
+      prof_bytes[p.cls] += p.bytes;
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void read_pcg() {
+    hip_check(hipMemcpyAsync(pcg_host, pcg, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), "read pcg scalars");
+    sync();
+  }
+};
+
+// Records a start/stop event pair around the launches in its scope when
+// timing is on; `bytes` is the algorithmic traffic of those launches.
+struct ProfScope {
+  otm_ctx* c;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(otm_ctx* c_, int cls_, double bytes_) : c(c_), cls(cls_), bytes(bytes_) {
+    if (c->prof_on) {
+      a = c->take_event();
+      hip_check(hipEventRecord(a, c->s), "hipEventRecord");
+    }
+  }
+  ~ProfScope() {
+    if (a) {
+      hipEvent_t b = c->take_event();
+      if (hipEventRecord(b, c->s) == hipSuccess) c->pending.push_back({cls, bytes, a, b});
+    }
+  }
+};
+
+struct otm_oc {
+  int ne = 0;
+  int device = 0;
+  double min_rho = 0.001, step = 0.02, damp = 0.5;
+  int max_steps = 20;
+  double vol_tol = 1e-4;
+  hipStream_t s = nullptr;
+  float *newrho = nullptr, *newsens = nullptr, *a = nullptr, *b = nullptr;
+  unsigned* maxbits = nullptr;
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  double* red_out = nullptr;
+  double* host = nullptr;  // pinned
+  unsigned* host_bits = nullptr;
+};
+
+// ---- context lifetime --------------------------------------------------------
+static void ctx_free(otm_ctx* c) {
+  if (!c) return;
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (auto& L : c->lv) {
+    if (L.own_uf) {
+      dfree(L.u);
+      dfree(L.f);
+    }
+    dfree(L.res);
+    dfree(L.st);
+    dfree(L.E);
+  }
+  dfree(c->kappa); dfree(c->x); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q);
+  dfree(c->cB); dfree(c->cX); dfree(c->cBinv);
+  dfree(c->partials); dfree(c->counter); dfree(c->red_out); dfree(c->pcg);
+  if (c->pcg_host) (void)hipHostFree(c->pcg_host);
+  if (c->out_host) (void)hipHostFree(c->out_host);
+  dfree(c->rho); dfree(c->rho_f); dfree(c->W); dfree(c->kraw); dfree(c->dkappa);
+  dfree(c->sens); dfree(c->tmp); dfree(c->drho);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+static void build_hierarchy(otm_ctx* c) {
+  const otm_config& cfg = c->cfg;
+  for (int a = 0; a < 3; a++) {
+    if (cfg.reso[a] < 8 || (cfg.reso[a] & 1)) fail(OTM_E_UNSUPPORTED, "each resolution must be even and >= 8");
+    if (cfg.reso[a] > 1024) fail(OTM_E_UNSUPPORTED, "resolution exceeds 1024 (grid.cpp:23-25 limit)");
+  }
+  const int minc = cfg.min_coarse > 2 ? cfg.min_coarse : 4;
+  int d[3] = {cfg.reso[0], cfg.reso[1], cfg.reso[2]};
+  std::vector<Geom> geoms;
+  while (true) {
+    Geom g{d[0], d[1], d[2], (long long)d[0] * d[1] * d[2]};
+    geoms.push_back(g);
+    bool can = true;
+    for (int a = 0; a < 3; a++)
+      if ((d[a] & 1) || d[a] / 2 < minc) can = false;
+    if (!can) break;
+    for (int a = 0; a < 3; a++) d[a] /= 2;
+  }
+  if (geoms.size() < 2) fail(OTM_E_UNSUPPORTED, "resolution too small for a two-level hierarchy");
+  const long long ncoarse = geoms.back().nn;
+  if (ncoarse > 4096) fail(OTM_E_UNSUPPORTED, "coarsest level has " + std::to_string(ncoarse) +
+                                                  " nodes (> 4096); use resolutions with more factors of two");
+  c->g0 = geoms[0];
+  const long long n0 = c->g0.nn;
+  c->kappa = dalloc<float>(n0);
+  c->x = dalloc<float>(3 * n0);
+  c->r = dalloc<float>(3 * n0);
+  c->z = dalloc<float>(3 * n0);
+  c->p = dalloc<float>(3 * n0);
+  c->q = dalloc<float>(3 * n0);
+  for (size_t l = 0; l < geoms.size(); l++) {
+    Level L;
+    L.g = geoms[l];
+    const long long n = L.g.nn;
+    if (l == 0) {
+      L.u = c->z;
+      L.f = c->r;
+      L.own_uf = false;
+    } else {
+      L.u = dalloc<float>(3 * n);
+      L.f = dalloc<float>(3 * n);
+      L.st = dalloc<float>(27 * n);
+      L.E = dalloc<float>(36 * n);
+    }
+    if (l + 1 < geoms.size()) L.res = dalloc<float>(3 * n);
+    c->lv.push_back(L);
+  }
+  c->cB = dalloc<double>(ncoarse * ncoarse);
+  c->cX = dalloc<double>(ncoarse * ncoarse);
+  c->cBinv = dalloc<double>(ncoarse * ncoarse);
+  c->partials = dalloc<double>(2048 * 8);
+  c->counter = dalloc<unsigned>(4);
+  hip_check(hipMemset(c->counter, 0, 4 * sizeof(unsigned)), "hipMemset counter");
+  c->red_out = dalloc<double>(16);
+  c->pcg = dalloc<PcgScalars>(1);
+  hip_check(hipMemset(c->pcg, 0, sizeof(PcgScalars)), "hipMemset pcg");
+  hip_check(hipHostMalloc((void**)&c->pcg_host, sizeof(PcgScalars), hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipHostMalloc((void**)&c->out_host, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+  launch_fill(c->s, c->x, 3 * n0, 0.f);
+  launch_fill(c->s, c->p, 3 * n0, 0.f);
+  hip_check(hipGetLastError(), "initial fill");
+  const double h = 1.0 / cfg.reso[0];  // element side (homogenization.cpp:47 passes h/2 as half-length)
+  c->fc = make_fine_consts(h);
+  c->G = make_galerkin_consts(h);
+}
+
+// Galerkin coarse operators + coarsest factorisation (MG_H::updateStencils,
+// MG.cpp:144-150).
+static void build_operators(otm_ctx* c) {
+  ProfScope ps(c, OTM_PROF_SETUP, 0.0);
+  for (size_t l = 1; l < c->lv.size(); l++) {
+    Level& L = c->lv[l];
+    if (l == 1)
+      launch_galerkin_first(c->s, c->g0, L.g, c->G, c->kappa, L.E);
+    else
+      launch_galerkin_next(c->s, c->lv[l - 1].g, L.g, c->lv[l - 1].E, L.E);
+    launch_stencil_from_elements(c->s, L.g, L.E, L.st);
+  }
+  const Level& Lc = c->lv.back();
+  launch_coarsest_factor(c->s, Lc.g, Lc.st, c->cB, c->cX, c->cBinv);
+  hip_check(hipGetLastError(), "operator build");
+}
+
+// ---- multigrid V-cycle (MG_H::v_cycle, MG.cpp:46-71) ----------------------------
+static void smooth(otm_ctx* c, size_t l, int sweeps, bool forward) {
+  Level& L = c->lv[l];
+  const int ncol = l == 0 ? 4 : 8;
+  for (int s = 0; s < sweeps; s++)
+    for (int i = 0; i < ncol; i++) {
+      const int col = forward ? i : ncol - 1 - i;
+      const double nn = (double)L.g.nn;
+      if (l == 0) {
+        // kappa 4 + other colours' rows 3*4*3/4 + f and u of the colour 2*3*4/4 bytes per node
+        ProfScope ps(c, OTM_PROF_FINE_GS, 19.0 * nn);
+        launch_fine_gs_color(c->s, L.g, c->fc, c->kappa, L.f, L.u, col);
+      } else {
+        // stencil rows of the colour 27*4/8 + other colours 3*4*7/8 + f and u 2*3*4/8
+        ProfScope ps(c, OTM_PROF_COARSE_GS, 27.0 * nn);
+        launch_st_gs_color(c->s, L.g, L.st, L.f, L.u, col);
+      }
+    }
+}
+
+static void vcycle(otm_ctx* c, size_t l) {
+  Level& L = c->lv[l];
+  const double nn = (double)L.g.nn;
+  if (l + 1 == c->lv.size()) {
+    ProfScope ps(c, OTM_PROF_COARSEST, 8.0 * nn * nn + 24.0 * nn);
+    launch_coarsest_solve(c->s, L.g, c->cBinv, L.f, L.u);
+    return;
+  }
+  {
+    ProfScope ps(c, OTM_PROF_VECTOR, 12.0 * nn);
+    launch_fill(c->s, L.u, 3 * L.g.nn, 0.f);
+  }
+  smooth(c, l, c->cfg.pre_sweeps, true);
+  if (l == 0) {
+    ProfScope ps(c, OTM_PROF_FINE_RESID, 40.0 * nn);
+    launch_fine_residual(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.res);
+  } else {
+    ProfScope ps(c, OTM_PROF_COARSE_RESID, 144.0 * nn);
+    launch_st_residual(c->s, L.g, L.st, L.f, L.u, L.res);
+  }
+  Level& C = c->lv[l + 1];
+  {
+    ProfScope ps(c, OTM_PROF_TRANSFER, 12.0 * nn + 12.0 * (double)C.g.nn);
+    launch_restrict(c->s, L.g, C.g, L.res, C.f);
+  }
+  vcycle(c, l + 1);
+  {
+    ProfScope ps(c, OTM_PROF_TRANSFER, 24.0 * nn + 12.0 * (double)C.g.nn);
+    launch_prolong_add(c->s, L.g, C.g, C.u, L.u);
+  }
+  smooth(c, l, c->cfg.post_sweeps, false);
+}
+
+// ---- PCG over the three load cases -------------------------------------------
+static void pcg_solve(otm_ctx* c) {
+  const long long n = c->g0.nn;
+  const unsigned nb = red_blocks_for(n);
+  const double rtol = c->cfg.fem_rel_thres;
+  const double rtol2 = rtol * rtol;
+  int it = 0;
+  int its[3] = {0, 0, 0};
+  for (int restart = 0; restart < 3; restart++) {
+    {
+      ProfScope ps(c, OTM_PROF_FINE_RESID, 28.0 * (double)n);
+      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2), nb);
+    }
+    c->read_pcg();
+    if (c->pcg_host->fault) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+    if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
+    vcycle(c, 0);
+    {
+      ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 12.0 * (double)n + 36.0 * (double)n);
+      launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ_FIRST), nb);
+      launch_fill(c->s, c->p, 3 * n, 0.f);
+      launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
+    }
+    bool done = false;
+    while (it < c->cfg.max_cg_iter) {
+      {
+        ProfScope ps(c, OTM_PROF_FINE_APPLY, 28.0 * (double)n);
+        launch_fine_apply_dot(c->s, c->g0, c->fc, c->kappa, c->p, c->q, c->reducer(FIN_PQ), nb);
+      }
+      {
+        ProfScope ps(c, OTM_PROF_VECTOR, 72.0 * (double)n);
+        launch_pcg_xr(c->s, n, c->x, c->r, c->p, c->q, c->pcg, c->reducer(FIN_RR, rtol2), nb);
+      }
+      it++;
+      c->read_pcg();
+      for (int k = 0; k < 3; k++)
+        if (c->pcg_host->active[k]) its[k]++;
+      if (c->pcg_host->fault) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+      if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) {
+        done = true;
+        break;
+      }
+      vcycle(c, 0);
+      {
+        ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 36.0 * (double)n);
+        launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ), nb);
+        launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
+      }
+    }
+    if (!done) break;
+  }
+  // true residual of the final iterate
+  launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2), nb);
+  // remove the constant component (translateForce, grid.cpp:654-664)
+  launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+  launch_sub_mean3(c->s, n, c->x, c->pcg);
+  c->read_pcg();
+  for (int k = 0; k < 3; k++) {
+    const double bb = c->pcg_host->bb[k] > 0 ? c->pcg_host->bb[k] : 1.0;
+    c->relres[k] = sqrt(c->pcg_host->rr[k] / bb);
+    c->iters[k] = its[k] + (its[k] < it ? 1 : 0);
+  }
+  if (c->pcg_host->fault) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+  if (c->cfg.verbose)
+    fprintf(stderr, "[otm] PCG its=%d relres=(%.3e, %.3e, %.3e)\n", it, c->relres[0], c->relres[1], c->relres[2]);
+}
+
+static void kappa_h(otm_ctx* c, double C[9]) {
+  const unsigned nb = red_blocks_for(c->g0.nn);
+  {
+    ProfScope ps(c, OTM_PROF_ENERGY, 16.0 * (double)c->g0.nn);
+    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->x, c->reducer(FIN_STORE), nb);
+  }
+  hip_check(hipMemcpyAsync(c->out_host, c->red_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s), "read C");
+  c->sync();
+  const double* s = c->out_host;
+  // order (00,01,02,11,12,22); |Y| = 1 (homogenization.cu:207)
+  C[0] = s[0]; C[1] = s[1]; C[2] = s[2];
+  C[3] = s[1]; C[4] = s[3]; C[5] = s[4];
+  C[6] = s[2]; C[7] = s[4]; C[8] = s[5];
+}
+
+static void ensure_design_buffers(otm_ctx* c) {
+  if (c->rho) return;
+  const long long n = c->g0.nn;
+  c->rho = dalloc<float>(n);
+  c->rho_f = dalloc<float>(n);
+  c->W = dalloc<float>(n);
+  c->kraw = dalloc<float>(n);
+  c->dkappa = dalloc<float>(n);
+  c->sens = dalloc<float>(n);
+  c->tmp = dalloc<float>(n);
+  c->drho = dalloc<float>(n);
+}
+
+static void sens_weights(const double dC[9], float w[6]) {
+  const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+  for (int t = 0; t < 6; t++) {
+    const int i = ii[t], j = jj[t];
+    w[t] = (float)(dC[i * 3 + j] + (i != j ? dC[j * 3 + i] : 0.0));
+  }
+}
+
+// ---- C ABI -----------------------------------------------------------------------
+extern "C" {
+
+OTM_API void otm_config_init(otm_config* cfg) {
+  if (!cfg) return;
+  memset(cfg, 0, sizeof(*cfg));
+  cfg->reso[0] = cfg->reso[1] = cfg->reso[2] = 128;  // cmdline.cpp:6-8
+  cfg->heat_ratio[0] = 1.0;                           // cmdline.cpp:12
+  cfg->heat_ratio[1] = 0.0;                           // cmdline.cpp:13
+  cfg->fem_rel_thres = 1e-6;
+  cfg->max_cg_iter = 200;  // the reference caps its V-cycle loop at 200 (MG.cpp:91)
+  cfg->pre_sweeps = 2;
+  cfg->post_sweeps = 2;
+  cfg->min_coarse = 4;
+  cfg->device = 0;
+  cfg->verbose = 0;
+}
+
+OTM_API const char* otm_last_error(void) { return g_last_error.c_str(); }
+OTM_API int otm_version(void) { return 100; }
+
+OTM_API int otm_create(const otm_config* cfg, otm_ctx** out) {
+  return guarded([&] {
+    if (!cfg || !out) fail(OTM_E_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+      (void)hipGetLastError();
+      fail(OTM_E_NODEV, "no HIP device available");
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) fail(OTM_E_ARG, "device ordinal out of range");
+    if (cfg->pre_sweeps < 0 || cfg->post_sweeps < 0 || cfg->pre_sweeps + cfg->post_sweeps == 0)
+      fail(OTM_E_ARG, "need at least one smoothing sweep");
+    if (cfg->pre_sweeps != cfg->post_sweeps)
+      fail(OTM_E_ARG, "pre and post sweeps must match (symmetric preconditioner for CG)");
+    if (!(cfg->fem_rel_thres > 0)) fail(OTM_E_ARG, "fem_rel_thres must be positive");
+    otm_ctx* c = new otm_ctx();
+    c->cfg = *cfg;
+    c->device = cfg->device;
+    try {
+      hip_check(hipSetDevice(c->device), "hipSetDevice");
+      hip_check(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking), "hipStreamCreate");
+      build_hierarchy(c);
+      c->sync();
+    } catch (...) {
+      ctx_free(c);
+      throw;
+    }
+    *out = c;
+  });
+}
+
+OTM_API int otm_destroy(otm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    ctx_free(ctx);
+  });
+}
+
+OTM_API int otm_levels(const otm_ctx* ctx, int* nlevels) {
+  return guarded([&] {
+    if (!ctx || !nlevels) fail(OTM_E_ARG, "null argument");
+    *nlevels = (int)ctx->lv.size();
+  });
+}
+
+OTM_API int otm_level_reso(const otm_ctx* ctx, int l, int reso[3]) {
+  return guarded([&] {
+    if (!ctx || !reso) fail(OTM_E_ARG, "null argument");
+    if (l < 0 || l >= (int)ctx->lv.size()) fail(OTM_E_ARG, "level out of range");
+    reso[0] = ctx->lv[l].g.nx;
+    reso[1] = ctx->lv[l].g.ny;
+    reso[2] = ctx->lv[l].g.nz;
+  });
+}
+
+static void do_update(otm_ctx* c, const float* kappa, int pitch, int where) {
+  if (kappa) {
+    check_where(where);
+    if (pitch < c->g0.nx) fail(OTM_E_ARG, "pitch smaller than reso[0]");
+    const long long rows = (long long)c->g0.ny * c->g0.nz;
+    const float* src = kappa;
+    float* staging = nullptr;
+    if (where == OTM_HOST) {
+      staging = c->q;  // 3n floats of scratch; the PCG rewrites q before use
+      if ((long long)pitch * rows > 3 * c->g0.nn) {
+        // pitched host rows wider than the scratch: pack on the host first
+        std::vector<float> packed((size_t)c->g0.nn);
+        for (long long r = 0; r < rows; r++)
+          memcpy(&packed[(size_t)(r * c->g0.nx)], kappa + r * pitch, sizeof(float) * c->g0.nx);
+        hip_check(hipMemcpyAsync(staging, packed.data(), sizeof(float) * c->g0.nn, hipMemcpyHostToDevice, c->s),
+                  "upload kappa");
+        c->sync();
+        pitch = c->g0.nx;
+      } else {
+        hip_check(hipMemcpyAsync(staging, kappa, sizeof(float) * pitch * rows, hipMemcpyHostToDevice, c->s),
+                  "upload kappa");
+      }
+      src = staging;
+    }
+    // kappa_eff = rhoPenalMin + kappa (grid.h:429, grid.cu:405)
+    launch_copy_pitched(c->s, c->g0, src, pitch, c->kappa, 1e-9f);
+    c->has_kappa = true;
+  }
+  if (!c->has_kappa) fail(OTM_E_STATE, "otm_update without a conductivity field");
+  build_operators(c);
+}
+
+OTM_API int otm_update(otm_ctx* ctx, const float* kappa, int pitch, int where) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    do_update(ctx, kappa, pitch, where);
+    ctx->sync();
+  });
+}
+
+static void do_heat_matrix(otm_ctx* c, double C[9]) {
+  if (!c->has_kappa) fail(OTM_E_STATE, "otm_heat_matrix before otm_update");
+  pcg_solve(c);
+  kappa_h(c, C);
+  memcpy(c->C, C, sizeof(c->C));
+  c->has_solution = true;
+}
+
+OTM_API int otm_heat_matrix(otm_ctx* ctx, double C[9]) {
+  return guarded([&] {
+    if (!ctx || !C) fail(OTM_E_ARG, "null argument");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    do_heat_matrix(ctx, C);
+  });
+}
+
+OTM_API int otm_solve_stats(const otm_ctx* ctx, int iters[3], double relres[3]) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    for (int k = 0; k < 3; k++) {
+      if (iters) iters[k] = ctx->iters[k];
+      if (relres) relres[k] = ctx->relres[k];
+    }
+  });
+}
+
+OTM_API int otm_sensitivity(otm_ctx* ctx, const double dC[9], float* sens, int pitch, int where) {
+  return guarded([&] {
+    if (!ctx || !dC || !sens) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->has_solution) fail(OTM_E_STATE, "otm_sensitivity before otm_heat_matrix");
+    if (pitch < ctx->g0.nx) fail(OTM_E_ARG, "pitch smaller than reso[0]");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    float w[6];
+    sens_weights(dC, w);
+    const long long rows = (long long)ctx->g0.ny * ctx->g0.nz;
+    if (where == OTM_DEVICE) {
+      // the reference zero-fills the pitched destination first (homogenization.cu:427)
+      if (pitch != ctx->g0.nx)
+        hip_check(hipMemsetAsync(sens, 0, sizeof(float) * pitch * rows, ctx->s), "zero sens");
+      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->x, w, sens, pitch);
+      ctx->sync();
+    } else {
+      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->x, w, ctx->q, ctx->g0.nx);
+      std::vector<float> h((size_t)ctx->g0.nn);
+      hip_check(hipMemcpyAsync(h.data(), ctx->q, sizeof(float) * ctx->g0.nn, hipMemcpyDeviceToHost, ctx->s),
+                "download sens");
+      ctx->sync();
+      for (long long r = 0; r < rows; r++) {
+        memcpy(sens + r * pitch, &h[(size_t)(r * ctx->g0.nx)], sizeof(float) * ctx->g0.nx);
+        for (int x = ctx->g0.nx; x < pitch; x++) sens[r * pitch + x] = 0.f;
+      }
+    }
+  });
+}
+
+OTM_API int otm_element_energy(otm_ctx* ctx, float* S6, int where) {
+  return guarded([&] {
+    if (!ctx || !S6) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->has_solution) fail(OTM_E_STATE, "otm_element_energy before otm_heat_matrix");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    if (where == OTM_DEVICE) {
+      launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->x, S6);
+      ctx->sync();
+    } else {
+      float* d = dalloc<float>(6 * n);
+      launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->x, d);
+      hipError_t e = hipMemcpyAsync(S6, d, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, ctx->s);
+      hipError_t e2 = hipStreamSynchronize(ctx->s);
+      dfree(d);
+      hip_check(e, "download S6");
+      hip_check(e2, "sync");
+    }
+  });
+}
+
+OTM_API int otm_get_chi(otm_ctx* ctx, float* chi, int where) {
+  return guarded([&] {
+    if (!ctx || !chi) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    hip_check(hipMemcpyAsync(chi, ctx->x, sizeof(float) * 3 * ctx->g0.nn, kind_of(where, OTM_DEVICE), ctx->s),
+              "copy chi");
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_set_chi(otm_ctx* ctx, const float* chi, int where) {
+  return guarded([&] {
+    if (!ctx || !chi) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    hip_check(hipMemcpyAsync(ctx->x, chi, sizeof(float) * 3 * ctx->g0.nn, kind_of(OTM_DEVICE, where), ctx->s),
+              "copy chi");
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_reset_displacement(otm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    launch_fill(ctx->s, ctx->x, 3 * ctx->g0.nn, 0.f);
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_apply_level(otm_ctx* ctx, int level, const float* u, float* q, int where) {
+  return guarded([&] {
+    if (!ctx || !u || !q) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (level < 0 || level >= (int)ctx->lv.size()) fail(OTM_E_ARG, "level out of range");
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_apply_level before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const Level& L = ctx->lv[level];
+    const long long n = L.g.nn;
+    float* du = dalloc<float>(3 * n);
+    float* dq = dalloc<float>(3 * n);
+    float* dz = dalloc<float>(3 * n);
+    try {
+      hip_check(hipMemcpyAsync(du, u, sizeof(float) * 3 * n, kind_of(OTM_DEVICE, where), ctx->s), "copy u");
+      launch_fill(ctx->s, dz, 3 * n, 0.f);
+      // q = 0 - (-A u): residual with f = 0 gives -A u
+      if (level == 0)
+        launch_fine_residual(ctx->s, L.g, ctx->fc, ctx->kappa, dz, du, dq);
+      else
+        launch_st_residual(ctx->s, L.g, L.st, dz, du, dq);
+      std::vector<float> h((size_t)(3 * n));
+      hip_check(hipMemcpyAsync(h.data(), dq, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->s), "copy q");
+      ctx->sync();
+      for (auto& v : h) v = -v;
+      hip_check(hipMemcpy(q, h.data(), sizeof(float) * 3 * n, kind_of(where, OTM_HOST)), "copy q out");
+    } catch (...) {
+      dfree(du); dfree(dq); dfree(dz);
+      throw;
+    }
+    dfree(du); dfree(dq); dfree(dz);
+  });
+}
+
+OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where) {
+  return guarded([&] {
+    if (!ctx || !r || !z) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_vcycle before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    hip_check(hipMemcpyAsync(ctx->r, r, sizeof(float) * 3 * n, kind_of(OTM_DEVICE, where), ctx->s), "copy r");
+    vcycle(ctx, 0);
+    hip_check(hipMemcpyAsync(z, ctx->z, sizeof(float) * 3 * n, kind_of(where, OTM_DEVICE), ctx->s), "copy z");
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_get_stencil(otm_ctx* ctx, int level, float* st, int where) {
+  return guarded([&] {
+    if (!ctx || !st) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (level < 1 || level >= (int)ctx->lv.size()) fail(OTM_E_ARG, "stencils exist on levels 1..L-1");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const Level& L = ctx->lv[level];
+    hip_check(hipMemcpyAsync(st, L.st, sizeof(float) * 27 * L.g.nn, kind_of(where, OTM_DEVICE), ctx->s),
+              "copy stencil");
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_conv_filter(const int reso[3], int kind, double radius, int periodic, int adjoint, const float* in,
+                            float* out, int where) {
+  return guarded([&] {
+    if (!reso || !in || !out) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    for (int a = 0; a < 3; a++)
+      if (reso[a] <= 0) fail(OTM_E_ARG, "reso must be positive");
+    Taps T = make_taps(kind, radius, periodic);
+    Geom g{reso[0], reso[1], reso[2], (long long)reso[0] * reso[1] * reso[2]};
+    float* din = dalloc<float>(g.nn);
+    float* dout = dalloc<float>(g.nn);
+    float* dW = dalloc<float>(g.nn);
+    float* dtmp = dalloc<float>(g.nn);
+    hipStream_t s = nullptr;
+    hipError_t e = hipMemcpy(din, in, sizeof(float) * g.nn, kind_of(OTM_DEVICE, where));
+    if (e == hipSuccess) {
+      if (adjoint) {
+        launch_filter_fwd(s, g, T, din, dtmp, dW);  // weight sums W
+        launch_filter_adj(s, g, T, din, dW, dout);
+      } else {
+        launch_filter_fwd(s, g, T, din, dout, dW);
+      }
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(float) * g.nn, kind_of(where, OTM_DEVICE));
+    dfree(din); dfree(dout); dfree(dW); dfree(dtmp);
+    hip_check(e, "conv filter");
+  });
+}
+
+OTM_API void otm_density_params_init(otm_density_params* p) {
+  if (!p) return;
+  p->filter_kind = 0;       // Spline4 (Framework.cu:163)
+  p->filter_radius = 1.5;
+  p->filter_periodic = 0;   // TensorExpression.h:1083
+  p->penal = 3.0;
+}
+
+OTM_API int otm_set_design(otm_ctx* ctx, const float* rho, int where) {
+  return guarded([&] {
+    if (!ctx || !rho) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ensure_design_buffers(ctx);
+    hip_check(hipMemcpyAsync(ctx->rho, rho, sizeof(float) * ctx->g0.nn, kind_of(OTM_DEVICE, where), ctx->s),
+              "copy rho");
+    ctx->sync();
+    ctx->has_drho = false;
+  });
+}
+
+OTM_API int otm_get_design(otm_ctx* ctx, float* rho, int where) {
+  return guarded([&] {
+    if (!ctx || !rho) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->rho) fail(OTM_E_STATE, "no design density set");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    hip_check(hipMemcpyAsync(rho, ctx->rho, sizeof(float) * ctx->g0.nn, kind_of(where, OTM_DEVICE), ctx->s),
+              "copy rho");
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
+  return guarded([&] {
+    if (!ctx || !p || !C) fail(OTM_E_ARG, "null argument");
+    if (!ctx->rho) fail(OTM_E_STATE, "otm_eval before otm_set_design");
+    if (p->penal != 3.0) fail(OTM_E_UNSUPPORTED, "only the cubic SIMP penalty is supported");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    const float* src = ctx->rho;
+    if (p->filter_kind >= 0) {
+      if (ctx->taps_kind != p->filter_kind || ctx->taps_r != p->filter_radius ||
+          ctx->taps_periodic != p->filter_periodic) {
+        ctx->taps = make_taps(p->filter_kind, p->filter_radius, p->filter_periodic);
+        ctx->taps_kind = p->filter_kind;
+        ctx->taps_r = p->filter_radius;
+        ctx->taps_periodic = p->filter_periodic;
+      }
+      launch_filter_fwd(ctx->s, ctx->g0, ctx->taps, ctx->rho, ctx->rho_f, ctx->W);
+      src = ctx->rho_f;
+    }
+    launch_simp_fwd(ctx->s, n, src, (float)ctx->cfg.heat_ratio[0], (float)ctx->cfg.heat_ratio[1], 0.f, ctx->kraw,
+                    ctx->dkappa);
+    do_update(ctx, ctx->kraw, ctx->g0.nx, OTM_DEVICE);
+    do_heat_matrix(ctx, C);
+    ctx->has_drho = false;
+    ctx->taps_kind = p->filter_kind >= 0 ? ctx->taps_kind : -1;
+  });
+}
+
+OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int where) {
+  return guarded([&] {
+    if (!ctx || !dC) fail(OTM_E_ARG, "null argument");
+    if (!ctx->rho || !ctx->has_solution) fail(OTM_E_STATE, "otm_backward before otm_eval");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    float w[6];
+    sens_weights(dC, w);
+    launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->x, w, ctx->sens, ctx->g0.nx);
+    launch_mul(ctx->s, n, ctx->sens, ctx->dkappa, ctx->tmp);
+    if (ctx->taps_kind >= 0)
+      launch_filter_adj(ctx->s, ctx->g0, ctx->taps, ctx->tmp, ctx->W, ctx->drho);
+    else
+      hip_check(hipMemcpyAsync(ctx->drho, ctx->tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->s), "copy");
+    if (drho) {
+      check_where(where);
+      hip_check(hipMemcpyAsync(drho, ctx->drho, sizeof(float) * n, kind_of(where, OTM_DEVICE), ctx->s), "copy drho");
+    }
+    ctx->sync();
+    ctx->has_drho = true;
+  });
+}
+
+// ---- OC optimizer ------------------------------------------------------------------
+static void oc_free(otm_oc* o) {
+  if (!o) return;
+  if (o->s) (void)hipStreamSynchronize(o->s);
+  dfree(o->newrho); dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits);
+  dfree(o->partials); dfree(o->counter); dfree(o->red_out);
+  if (o->host) (void)hipHostFree(o->host);
+  if (o->host_bits) (void)hipHostFree(o->host_bits);
+  if (o->s) (void)hipStreamDestroy(o->s);
+  delete o;
+}
+
+OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp, int device, otm_oc** out) {
+  return guarded([&] {
+    if (!out || ne <= 0) fail(OTM_E_ARG, "invalid argument");
+    *out = nullptr;
+    otm_oc* o = new otm_oc();
+    o->ne = ne;
+    o->device = device;
+    o->min_rho = min_rho;
+    o->step = step_limit;
+    o->damp = damp;
+    try {
+      hip_check(hipSetDevice(device), "hipSetDevice");
+      hip_check(hipStreamCreateWithFlags(&o->s, hipStreamNonBlocking), "hipStreamCreate");
+      o->newrho = dalloc<float>(ne);
+      o->newsens = dalloc<float>(ne);
+      o->a = dalloc<float>(ne);
+      o->b = dalloc<float>(ne);
+      o->maxbits = dalloc<unsigned>(1);
+      o->partials = dalloc<double>(2048);
+      o->counter = dalloc<unsigned>(1);
+      hip_check(hipMemset(o->counter, 0, sizeof(unsigned)), "memset");
+      o->red_out = dalloc<double>(4);
+      hip_check(hipHostMalloc((void**)&o->host, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&o->host_bits, sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc");
+    } catch (...) {
+      oc_free(o);
+      throw;
+    }
+    *out = o;
+  });
+}
+
+OTM_API int otm_oc_destroy(otm_oc* oc) {
+  return guarded([&] { oc_free(oc); });
+}
+
+OTM_API int otm_oc_set_bisection(otm_oc* oc, int max_steps, double vol_tol) {
+  return guarded([&] {
+    if (!oc || max_steps <= 0 || !(vol_tol > 0)) fail(OTM_E_ARG, "invalid argument");
+    oc->max_steps = max_steps;
+    oc->vol_tol = vol_tol;
+  });
+}
+
+static void oc_filter_dev(otm_oc* o, float* sens, const float* rho, const Geom& g, double radius) {
+  Taps T = make_taps(1, radius, 1);  // OCOptimizer.cu:143: Linear, periodic
+  launch_oc_filter_sens(o->s, g, T, sens, rho, o->newsens);
+  hip_check(hipMemcpyAsync(sens, o->newsens, sizeof(float) * g.nn, hipMemcpyDeviceToDevice, o->s), "copy sens");
+}
+
+OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pitch, const int reso[3],
+                               double radius, int where) {
+  return guarded([&] {
+    if (!oc || !sens || !rho || !reso) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    Geom g{reso[0], reso[1], reso[2], (long long)reso[0] * reso[1] * reso[2]};
+    if (g.nn != oc->ne) fail(OTM_E_ARG, "reso does not match ne");
+    if (pitch < reso[0]) fail(OTM_E_ARG, "pitch smaller than reso[0]");
+    hip_check(hipSetDevice(oc->device), "hipSetDevice");
+    const long long rows = (long long)reso[1] * reso[2];
+    // gather to contiguous device copies
+    hip_check(hipMemcpy2DAsync(oc->a, sizeof(float) * reso[0], sens, sizeof(float) * pitch, sizeof(float) * reso[0],
+                               rows, kind_of(OTM_DEVICE, where), oc->s),
+              "gather sens");
+    hip_check(hipMemcpy2DAsync(oc->b, sizeof(float) * reso[0], rho, sizeof(float) * pitch, sizeof(float) * reso[0],
+                               rows, kind_of(OTM_DEVICE, where), oc->s),
+              "gather rho");
+    oc_filter_dev(oc, oc->a, oc->b, g, radius);
+    hip_check(hipMemcpy2DAsync(sens, sizeof(float) * pitch, oc->a, sizeof(float) * reso[0], sizeof(float) * reso[0],
+                               rows, kind_of(where, OTM_DEVICE), oc->s),
+              "scatter sens");
+    hip_check(hipStreamSynchronize(oc->s), "sync");
+  });
+}
+
+static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volratio, int* steps, double* volume) {
+  const long long n = o->ne;
+  hip_check(hipMemsetAsync(o->maxbits, 0, sizeof(unsigned), o->s), "memset");
+  launch_maxabs(o->s, n, sens, o->maxbits);
+  hip_check(hipMemcpyAsync(o->host_bits, o->maxbits, sizeof(unsigned), hipMemcpyDeviceToHost, o->s), "read max");
+  hip_check(hipStreamSynchronize(o->s), "sync");
+  float mx;
+  memcpy(&mx, o->host_bits, sizeof(float));
+  double lo = 0.0, hi = fabs((double)mx);
+  const unsigned nb = red_blocks_for(n);
+  Reduce R;
+  R.partials = o->partials;
+  R.counter = o->counter;
+  R.out = o->red_out;
+  R.pcg = nullptr;
+  R.rtol2 = 0;
+  R.inv_nn = 0;
+  R.mode = FIN_STORE;
+  int it = 0;
+  double vol = 0.0;
+  for (it = 0; it < o->max_steps;) {
+    // the reference evaluates the midpoint in FP32 (OCOptimizer.cu:44-48)
+    const float gm = (float)((hi + lo) / 2.0);
+    launch_oc_apply_sum(o->s, n, sens, rho, o->newrho, gm, (float)o->min_rho, (float)o->step, (float)o->damp, R, nb);
+    hip_check(hipMemcpyAsync(o->host, o->red_out, sizeof(double), hipMemcpyDeviceToHost, o->s), "read vol");
+    hip_check(hipStreamSynchronize(o->s), "sync");
+    vol = o->host[0] / (double)n;
+    it++;
+    if (vol < volratio - o->vol_tol)
+      hi = gm;
+    else if (vol > volratio + o->vol_tol)
+      lo = gm;
+    else
+      break;
+  }
+  hip_check(hipMemcpyAsync(rho, o->newrho, sizeof(float) * n, hipMemcpyDeviceToDevice, o->s), "copy rho");
+  hip_check(hipStreamSynchronize(o->s), "sync");
+  if (steps) *steps = it;
+  if (volume) *volume = vol;
+}
+
+OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volratio, int where, int* steps,
+                          double* volume) {
+  return guarded([&] {
+    if (!oc || !sens || !rho) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    hip_check(hipSetDevice(oc->device), "hipSetDevice");
+    const long long n = oc->ne;
+    if (where == OTM_DEVICE) {
+      oc_update_dev(oc, sens, rho, volratio, steps, volume);
+    } else {
+      hip_check(hipMemcpyAsync(oc->a, sens, sizeof(float) * n, hipMemcpyHostToDevice, oc->s), "upload sens");
+      hip_check(hipMemcpyAsync(oc->b, rho, sizeof(float) * n, hipMemcpyHostToDevice, oc->s), "upload rho");
+      oc_update_dev(oc, oc->a, oc->b, volratio, steps, volume);
+      hip_check(hipMemcpyAsync(rho, oc->b, sizeof(float) * n, hipMemcpyDeviceToHost, oc->s), "download rho");
+      hip_check(hipStreamSynchronize(oc->s), "sync");
+    }
+  });
+}
+
+OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double volratio, int* steps,
+                        double* volume) {
+  return guarded([&] {
+    if (!ctx || !oc) fail(OTM_E_ARG, "null argument");
+    if (!ctx->has_drho) fail(OTM_E_STATE, "otm_oc_step before otm_backward");
+    if (oc->ne != ctx->g0.nn) fail(OTM_E_ARG, "optimizer size does not match the cell");
+    if (oc->device != ctx->device) fail(OTM_E_ARG, "optimizer and context on different devices");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->sync();
+    if (filter_radius > 0) oc_filter_dev(oc, ctx->drho, ctx->rho, ctx->g0, filter_radius);
+    oc_update_dev(oc, ctx->drho, ctx->rho, volratio, steps, volume);
+    ctx->has_drho = false;
+  });
+}
+
+OTM_API int otm_profile_enable(otm_ctx* ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    ctx->sync();
+    ctx->prof_on = enable != 0;
+    if (ctx->prof_on) {
+      for (int k = 0; k < OTM_PROF_NCLASSES; k++) {
+        ctx->prof_launches[k] = 0;
+        ctx->prof_ms[k] = 0.0;
+        ctx->prof_bytes[k] = 0.0;
+      }
+    }
+  });
+}
+
+OTM_API int otm_profile_read(otm_ctx* ctx, int kclass, long long* launches, double* ms, double* bytes) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    if (kclass < 0 || kclass >= OTM_PROF_NCLASSES) fail(OTM_E_ARG, "kernel class out of range");
+    ctx->sync();
+    if (launches) *launches = ctx->prof_launches[kclass];
+    if (ms) *ms = ctx->prof_ms[kclass];
+    if (bytes) *bytes = ctx->prof_bytes[kclass];
+  });
+}
+
+OTM_API int otm_device_alloc(int device, size_t bytes, void** ptr) {
+  return guarded([&] {
+    if (!ptr) fail(OTM_E_ARG, "null argument");
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    *ptr = dalloc<char>(bytes);
+  });
+}
+
+OTM_API int otm_device_free(void* ptr) {
+  return guarded([&] { dfree(ptr); });
+}
+
+OTM_API int otm_memcpy(void* dst, const void* src, size_t bytes, int dst_where, int src_where) {
+  return guarded([&] {
+    check_where(dst_where);
+    check_where(src_where);
+    hip_check(hipMemcpy(dst, src, bytes, kind_of(dst_where, src_where)), "hipMemcpy");
+  });
+}
+
+OTM_API int otm_synchronize(otm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    ctx->sync();
+  });
+}
+
+}  // extern "C"

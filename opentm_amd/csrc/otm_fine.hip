@@ -1,0 +1,310 @@
+// otm_fine.hip -- matrix-free fine-level kernels (gfx950, wave64).
+//
+// The fine operator is the SIMP-weighted trilinear conductivity stencil
+// assembled on the fly from per-element kappa (reference: grid.cu:318-471
+// smoother, grid.cu:1027-1164 residual, grid.cu:1282-1342 load vectors).
+// With the element matrix Ke = c*[4 diag, 0 one-axis neighbours, -1 else],
+// node v couples only to its 12 edge and 8 corner neighbours:
+//   (K u)_v = c * sum_{e ni v} kappa_e * (4 u_v - T_e(u)),
+// where T_e sums u over the three face-diagonal and the body-diagonal nodes
+// of v inside element e. Because one-axis neighbours do not couple, the
+// reference's 8-colour Gauss-Seidel (colour = parity of x,y,z, in order
+// 0..7) is exactly a 4-colour sweep over whole x-rows with colour
+// (y%2) + 2*(z%2): colours 2k and 2k+1 of the 8-colour order never couple.
+#include "otm_kernels.hpp"
+
+namespace otm {
+
+// kappa of the 8 elements around node (x,y,z) and their sum.
+struct NodeElems {
+  float k[8];  // index ax + 2 ay + 4 az: element at (x-1+ax, y-1+ay, z-1+az)
+  float ksum;
+};
+
+struct NodeIdx {
+  int x, y, z, xm, xp, ym, yp, zm, zp;
+  long long v;
+};
+
+__device__ inline NodeIdx node_idx(const Geom& g, int x, int y, int z) {
+  NodeIdx n;
+  n.x = x; n.y = y; n.z = z;
+  n.xm = x == 0 ? g.nx - 1 : x - 1;
+  n.xp = x == g.nx - 1 ? 0 : x + 1;
+  n.ym = y == 0 ? g.ny - 1 : y - 1;
+  n.yp = y == g.ny - 1 ? 0 : y + 1;
+  n.zm = z == 0 ? g.nz - 1 : z - 1;
+  n.zp = z == g.nz - 1 ? 0 : z + 1;
+  n.v = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+  return n;
+}
+
+__device__ inline long long rowb(const Geom& g, int y, int z) {
+  return (long long)g.nx * ((long long)y + (long long)g.ny * z);
+}
+
+__device__ inline NodeElems load_elems(const Geom& g, const NodeIdx& n, const float* __restrict__ kappa) {
+  NodeElems E;
+  const long long r00 = rowb(g, n.ym, n.zm), r10 = rowb(g, n.y, n.zm);
+  const long long r01 = rowb(g, n.ym, n.z), r11 = rowb(g, n.y, n.z);
+  E.k[0] = kappa[r00 + n.xm]; E.k[1] = kappa[r00 + n.x];
+  E.k[2] = kappa[r10 + n.xm]; E.k[3] = kappa[r10 + n.x];
+  E.k[4] = kappa[r01 + n.xm]; E.k[5] = kappa[r01 + n.x];
+  E.k[6] = kappa[r11 + n.xm]; E.k[7] = kappa[r11 + n.x];
+  E.ksum = ((E.k[0] + E.k[1]) + (E.k[2] + E.k[3])) + ((E.k[4] + E.k[5]) + (E.k[6] + E.k[7]));
+  return E;
+}
+
+// sum_e kappa_e T_e(u) for one right-hand side.
+__device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems& E,
+                                const float* __restrict__ u) {
+  // rows touched: (y+-1, z), (y, z+-1), (y+-1, z+-1)
+  const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
+  const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
+  const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
+  const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
+  // U(dx,dy,dz) naming: first letter x (m/0/p), then y, then z
+  const float u_mm0 = u[rM0 + n.xm], u_pm0 = u[rM0 + n.xp];
+  const float u_mp0 = u[rP0 + n.xm], u_pp0 = u[rP0 + n.xp];
+  const float u_m0m = u[r0m + n.xm], u_p0m = u[r0m + n.xp];
+  const float u_m0p = u[r0p + n.xm], u_p0p = u[r0p + n.xp];
+  const float u_mmm = u[rMm + n.xm], u_0mm = u[rMm + n.x], u_pmm = u[rMm + n.xp];
+  const float u_mmp = u[rMp + n.xm], u_0mp = u[rMp + n.x], u_pmp = u[rMp + n.xp];
+  const float u_mpm = u[rPm + n.xm], u_0pm = u[rPm + n.x], u_ppm = u[rPm + n.xp];
+  const float u_mpp = u[rPp + n.xm], u_0pp = u[rPp + n.x], u_ppp = u[rPp + n.xp];
+  // element (ax,ay,az) lies on side s = 2a-1; far nodes (sx,sy,0) (sx,0,sz) (0,sy,sz) (sx,sy,sz)
+  float s = 0.f;
+  s += E.k[0] * (u_mm0 + u_m0m + u_0mm + u_mmm);
+  s += E.k[1] * (u_pm0 + u_p0m + u_0mm + u_pmm);
+  s += E.k[2] * (u_mp0 + u_m0m + u_0pm + u_mpm);
+  s += E.k[3] * (u_pp0 + u_p0m + u_0pm + u_ppm);
+  s += E.k[4] * (u_mm0 + u_m0p + u_0mp + u_mmp);
+  s += E.k[5] * (u_pm0 + u_p0p + u_0mp + u_pmp);
+  s += E.k[6] * (u_mp0 + u_m0p + u_0pp + u_mpp);
+  s += E.k[7] * (u_pp0 + u_p0p + u_0pp + u_ppp);
+  return s;
+}
+
+// Load vector of case k at the node (grid.cu:1282-1342): sum_e kappa_e feMu[7-e][k].
+__device__ inline float load_vec(const NodeElems& E, const FineConsts& fc, int k) {
+  float s = 0.f;
+#pragma unroll
+  for (int e = 0; e < 8; e++) s += E.k[e] * fc.femu[7 - e][k];
+  return s;
+}
+
+// One colour of the x-row Gauss-Seidel sweep: u_v = (f_v + c*far)/(4c*ksum).
+__global__ void __launch_bounds__(kBlock) k_fine_gs_color(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                          const float* __restrict__ f, float* u, int cy, int cz) {
+  const int hy = g.ny >> 1, hz = g.nz >> 1;
+  const long long cnt = (long long)g.nx * hy * hz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+  const NodeIdx n = node_idx(g, x, y, z);
+  const NodeElems E = load_elems(g, n, kappa);
+  const float inv = 1.f / (4.f * fc.c * E.ksum);
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * g.nn;
+    const float fs = far_sum(g, n, E, uk);
+    u[k * g.nn + n.v] = (f[k * g.nn + n.v] + fc.c * fs) * inv;
+  }
+}
+
+// res = f - K u on every node.
+__global__ void __launch_bounds__(kBlock) k_fine_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                          const float* __restrict__ f, const float* __restrict__ u,
+                                                          float* __restrict__ res) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  const NodeIdx n = node_idx(g, x, y, z);
+  const NodeElems E = load_elems(g, n, kappa);
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * g.nn;
+    const float ku = fc.c * (4.f * E.ksum * uk[n.v] - far_sum(g, n, E, uk));
+    res[k * g.nn + n.v] = f[k * g.nn + n.v] - ku;
+  }
+}
+
+// q = K p and partial sums of p.q (PCG step).
+__global__ void __launch_bounds__(kBlock) k_fine_apply_dot(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                           const float* __restrict__ p, float* __restrict__ q,
+                                                           Reduce R) {
+  double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    const NodeIdx n = node_idx(g, x, y, z);
+    const NodeElems E = load_elems(g, n, kappa);
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const float* pk = p + k * g.nn;
+      const float pv = pk[n.v];
+      const float kp = fc.c * (4.f * E.ksum * pv - far_sum(g, n, E, pk));
+      q[k * g.nn + n.v] = kp;
+      acc[k] += (double)pv * (double)kp;
+    }
+  }
+  reduce_finish<OTM_NRHS>(acc, R);
+}
+
+// r = b - K x with b the load vectors built on the fly; partial sums of
+// b.b and r.r (PCG start).
+__global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                               const float* __restrict__ x_, float* __restrict__ rvec,
+                                                               Reduce R) {
+  double acc[2 * OTM_NRHS] = {0, 0, 0, 0, 0, 0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    const NodeIdx n = node_idx(g, x, y, z);
+    const NodeElems E = load_elems(g, n, kappa);
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const float* xk = x_ + k * g.nn;
+      const float b = load_vec(E, fc, k);
+      const float kx = fc.c * (4.f * E.ksum * xk[n.v] - far_sum(g, n, E, xk));
+      const float rv = b - kx;
+      rvec[k * g.nn + n.v] = rv;
+      acc[k] += (double)b * (double)b;
+      acc[OTM_NRHS + k] += (double)rv * (double)rv;
+    }
+  }
+  reduce_finish<2 * OTM_NRHS>(acc, R);
+}
+
+// Element energies (homogenization.cu:13-153 and :255-412). For element e
+// with nodal fields d_k = disp[:,k] - chi_k(nodes): S_ij = d_i^T Ke d_j,
+// using (Ke d)_a = c (5 d_a + d_{a^1} + d_{a^2} + d_{a^4} - sum d).
+__device__ inline void element_S(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
+                                 int x, int y, int z, float S[6]) {
+  const int xp = x == g.nx - 1 ? 0 : x + 1;
+  const int yp = y == g.ny - 1 ? 0 : y + 1;
+  const int zp = z == g.nz - 1 ? 0 : z + 1;
+  const long long r0 = rowb(g, y, z), r1 = rowb(g, yp, z), r2 = rowb(g, y, zp), r3 = rowb(g, yp, zp);
+  const long long nid[8] = {r0 + x, r0 + xp, r1 + x, r1 + xp, r2 + x, r2 + xp, r3 + x, r3 + xp};
+  float d[3][8], kd[3][8];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float tot = 0.f;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      d[k][m] = fc.disp[m][k] - chi[k * g.nn + nid[m]];
+      tot += d[k][m];
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+      kd[k][a] = fc.c * (5.f * d[k][a] + d[k][a ^ 1] + d[k][a ^ 2] + d[k][a ^ 4] - tot);
+  }
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = i; j < 3; j++) {
+      float s = 0.f;
+#pragma unroll
+      for (int a = 0; a < 8; a++) s += d[i][a] * kd[j][a];
+      S[c++] = s;
+    }
+}
+
+// kappa^H_ij = sum_e (kappa_e) S_e(i,j) with FP64 accumulation; kappa
+// already holds the 1e-9 floor.
+__global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                         const float* __restrict__ chi, Reduce R) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    float S[6];
+    element_S(g, fc, chi, x, y, z, S);
+    const double ke = (double)kappa[t];
+#pragma unroll
+    for (int c = 0; c < 6; c++) acc[c] += ke * (double)S[c];
+  }
+  reduce_finish<6>(acc, R);
+}
+
+// sens[x + pitch*(y + ny*z)] = sum_c w_c S_c (contracted sensitivity with
+// respect to kappa; the penalty exponent is 1, homogenization.cu:344-346).
+__global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, const float* __restrict__ chi,
+                                                             float w0, float w1, float w2, float w3, float w4, float w5,
+                                                             float* __restrict__ sens, int pitch) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float S[6];
+  element_S(g, fc, chi, x, y, z, S);
+  const float s = S[0] * w0 + S[1] * w1 + S[2] * w2 + S[3] * w3 + S[4] * w4 + S[5] * w5;
+  sens[(long long)x + (long long)pitch * ((long long)y + (long long)g.ny * z)] = s;
+}
+
+// S6[c*nn + e]: the six element energies, for callers that keep them.
+__global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, const float* __restrict__ chi,
+                                                         float* __restrict__ S6) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float S[6];
+  element_S(g, fc, chi, x, y, z, S);
+#pragma unroll
+  for (int c = 0; c < 6; c++) S6[c * g.nn + t] = S[c];
+}
+
+// ---- launchers ---------------------------------------------------------
+static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void launch_fine_gs_color(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
+                          float* u, int color) {
+  const long long cnt = (long long)g.nx * (g.ny / 2) * (g.nz / 2);
+  hipLaunchKernelGGL(k_fine_gs_color, dim3(nblk(cnt)), dim3(kBlock), 0, s, g, fc, kappa, f, u, color & 1, color >> 1);
+}
+
+void launch_fine_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
+                          const float* u, float* res) {
+  hipLaunchKernelGGL(k_fine_residual, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, kappa, f, u, res);
+}
+
+void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* p,
+                           float* q, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_fine_apply_dot, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, p, q, R);
+}
+
+void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
+                               const float* x, float* r, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_fine_init_residual, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
+}
+
+void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
+                         const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_fine_kappa_h, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, chi, R);
+}
+
+void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
+                             float* sens, int pitch) {
+  hipLaunchKernelGGL(k_fine_sensitivity, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, chi, w[0], w[1], w[2], w[3],
+                     w[4], w[5], sens, pitch);
+}
+
+void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6) {
+  hipLaunchKernelGGL(k_fine_energy6, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, chi, S6);
+}
+
+}  // namespace otm

@@ -1,0 +1,65 @@
+// otm_kernels.hpp -- host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include "otm_common.hpp"
+
+namespace otm {
+
+// ---- fine level (matrix-free, otm_fine.hip) ----
+void launch_fine_gs_color(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
+                          float* u, int color);
+void launch_fine_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
+                          const float* u, float* res);
+void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* p,
+                           float* q, const Reduce& R, unsigned nblocks);
+void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
+                               const float* x, float* r, const Reduce& R, unsigned nblocks);
+void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
+                         const Reduce& R, unsigned nblocks);
+void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
+                             float* sens, int pitch);
+void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6);
+
+// ---- coarse levels (stored 27-point stencils, otm_coarse.hip) ----
+void launch_st_gs_color(hipStream_t s, const Geom& g, const float* st, const float* f, float* u, int color);
+void launch_st_residual(hipStream_t s, const Geom& g, const float* st, const float* f, const float* u, float* res);
+void launch_restrict(hipStream_t s, const Geom& fine, const Geom& coarse, const float* res, float* fc);
+void launch_prolong_add(hipStream_t s, const Geom& fine, const Geom& coarse, const float* uc, float* uf);
+void launch_galerkin_first(hipStream_t s, const Geom& fine, const Geom& coarse, const GalerkinConsts& G,
+                           const float* kappa, float* E);
+void launch_galerkin_next(hipStream_t s, const Geom& fine, const Geom& coarse, const float* Ein, float* Eout);
+void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, float* st);
+void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv);
+void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u);
+
+// ---- vectors and reductions (otm_vec.hip) ----
+void launch_fill(hipStream_t s, float* a, long long n, float v);
+void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
+void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks);
+void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P);
+void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,
+                   const PcgScalars* P, const Reduce& R, unsigned nblocks);
+void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P);
+void launch_copy_pitched(hipStream_t s, const Geom& g, const float* src, int pitch, float* dst, float add);
+void launch_copy_to_pitched(hipStream_t s, const Geom& g, const float* src, float* dst, int pitch);
+
+// ---- density filter, SIMP and OC (otm_opt.hip) ----
+constexpr int kMaxTaps = 343;  // (2*3+1)^3: radius up to 3.5 voxels
+struct Taps {
+  int n;                 // number of non-zero taps
+  int periodic;          // wrap around the cell instead of renormalising
+  signed char dx[kMaxTaps], dy[kMaxTaps], dz[kMaxTaps];
+  float w[kMaxTaps];
+};
+void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W);
+void launch_filter_adj(hipStream_t s, const Geom& g, const Taps& T, const float* in, const float* W, float* out);
+void launch_simp_fwd(hipStream_t s, long long n, const float* rho, float k0, float k1, float floor_, float* kappa,
+                     float* dkappa);
+void launch_mul(hipStream_t s, long long n, const float* a, const float* b, float* out);
+void launch_oc_filter_sens(hipStream_t s, const Geom& g, const Taps& T, const float* sens, const float* rho,
+                           float* out);
+void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out_bits);
+void launch_oc_apply_sum(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new, float g,
+                         float min_rho, float step, float damp, const Reduce& R, unsigned nblocks);
+
+}  // namespace otm

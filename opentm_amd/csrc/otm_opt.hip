@@ -1,0 +1,156 @@
+// otm_opt.hip -- density filter, SIMP map and OC update kernels.
+//
+// Filter: renormalised radial convolution (TensorExpression.cuh:303-384):
+// out_i = sum_j w_ij in_j / W_i over taps inside the cell (all taps when
+// periodic), adjoint out_j = sum_i w_ij in_i / W_i. OC: density-weighted
+// periodic sensitivity filter (OCOptimizer.cu:76-104) and the bisection
+// update rule (OCOptimizer.cu:12-34).
+#include "otm_kernels.hpp"
+
+namespace otm {
+
+static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__global__ void __launch_bounds__(kBlock) k_filter_fwd(Geom g, Taps T, const float* __restrict__ in,
+                                                       float* __restrict__ out, float* __restrict__ W) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float s = 0.f, ws = 0.f;
+  for (int i = 0; i < T.n; i++) {
+    int X = x + T.dx[i], Y = y + T.dy[i], Z = z + T.dz[i];
+    const bool inside = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz;
+    if (!inside) {
+      if (!T.periodic) continue;
+      X = wrapm(X, g.nx); Y = wrapm(Y, g.ny); Z = wrapm(Z, g.nz);
+    }
+    const float w = T.w[i];
+    s += w * in[(long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z)];
+    ws += w;
+  }
+  out[t] = s / ws;
+  if (W) W[t] = ws;
+}
+
+__global__ void __launch_bounds__(kBlock) k_filter_adj(Geom g, Taps T, const float* __restrict__ in,
+                                                       const float* __restrict__ W, float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float s = 0.f;
+  for (int i = 0; i < T.n; i++) {
+    int X = x - T.dx[i], Y = y - T.dy[i], Z = z - T.dz[i];
+    const bool inside = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz;
+    if (!inside) {
+      if (!T.periodic) continue;
+      X = wrapm(X, g.nx); Y = wrapm(Y, g.ny); Z = wrapm(Z, g.nz);
+    }
+    const long long j = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
+    s += T.w[i] * in[j] / W[j];
+  }
+  out[t] = s;
+}
+
+// kappa = k1 + (k0 - k1) rho^3 (+floor), dkappa = 3 (k0 - k1) rho^2
+// (Framework.cu:163 with penalty 3).
+__global__ void __launch_bounds__(kBlock) k_simp_fwd(long long n, const float* __restrict__ rho, float k0, float k1,
+                                                     float floor_, float* __restrict__ kappa,
+                                                     float* __restrict__ dkappa) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float p = rho[t];
+  const float d = k0 - k1;
+  kappa[t] = k1 + d * p * p * p + floor_;
+  if (dkappa) dkappa[t] = 3.f * d * p * p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mul(long long n, const float* __restrict__ a, const float* __restrict__ b,
+                                                float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = a[t] * b[t];
+}
+
+__global__ void __launch_bounds__(kBlock) k_oc_filter_sens(Geom g, Taps T, const float* __restrict__ sens,
+                                                           const float* __restrict__ rho, float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float s = 0.f, ws = 0.f;
+  for (int i = 0; i < T.n; i++) {
+    const int X = wrapm(x + T.dx[i], g.nx), Y = wrapm(y + T.dy[i], g.ny), Z = wrapm(z + T.dz[i], g.nz);
+    const long long j = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
+    const float w = T.w[i];
+    s += sens[j] * rho[j] * w;
+    ws += w;
+  }
+  out[t] = s / (ws * rho[t]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_maxabs(long long n, const float* __restrict__ a, unsigned int* out_bits) {
+  float m = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(a[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(m));
+}
+
+__device__ inline float oc_rule(float s, float rho, float g, float min_rho, float step, float damp) {
+  float B = -s / g;
+  if (B < 0.f) B = 0.01f;
+  float v = powf(B, damp) * rho;
+  if (v - rho < -step) v = rho - step;
+  if (v - rho > step) v = rho + step;
+  if (v < min_rho) v = min_rho;
+  if (v > 1.f) v = 1.f;
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_oc_apply_sum(long long n, const float* __restrict__ sens,
+                                                         const float* __restrict__ rho, float* __restrict__ rho_new,
+                                                         float g, float min_rho, float step, float damp, Reduce R) {
+  double acc[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = oc_rule(sens[i], rho[i], g, min_rho, step, damp);
+    rho_new[i] = v;
+    acc[0] += (double)v;
+  }
+  reduce_finish<1>(acc, R);
+}
+
+void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W) {
+  hipLaunchKernelGGL(k_filter_fwd, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, in, out, W);
+}
+void launch_filter_adj(hipStream_t s, const Geom& g, const Taps& T, const float* in, const float* W, float* out) {
+  hipLaunchKernelGGL(k_filter_adj, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, in, W, out);
+}
+void launch_simp_fwd(hipStream_t s, long long n, const float* rho, float k0, float k1, float floor_, float* kappa,
+                     float* dkappa) {
+  hipLaunchKernelGGL(k_simp_fwd, dim3(nblk(n)), dim3(kBlock), 0, s, n, rho, k0, k1, floor_, kappa, dkappa);
+}
+void launch_mul(hipStream_t s, long long n, const float* a, const float* b, float* out) {
+  hipLaunchKernelGGL(k_mul, dim3(nblk(n)), dim3(kBlock), 0, s, n, a, b, out);
+}
+void launch_oc_filter_sens(hipStream_t s, const Geom& g, const Taps& T, const float* sens, const float* rho,
+                           float* out) {
+  hipLaunchKernelGGL(k_oc_filter_sens, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, sens, rho, out);
+}
+void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out_bits) {
+  unsigned nb = nblk(n);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kBlock), 0, s, n, a, out_bits);
+}
+void launch_oc_apply_sum(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new, float g,
+                         float min_rho, float step, float damp, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_oc_apply_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, rho_new, g, min_rho, step,
+                     damp, R);
+}
+
+}  // namespace otm

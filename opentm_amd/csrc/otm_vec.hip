@@ -1,0 +1,130 @@
+// otm_vec.hip -- vector updates and FP64-accumulated reductions for the
+// lock-step three-right-hand-side preconditioned conjugate gradient.
+// The reference sums in FP32 through cub (culib/lib.cuh:595-760); here every
+// inner product accumulates in FP64 with a fixed summation order.
+#include "otm_kernels.hpp"
+
+namespace otm {
+
+static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__global__ void __launch_bounds__(kBlock) k_fill(float* a, long long n, float v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __restrict__ a,
+                                                 const float* __restrict__ b, Reduce R) {
+  double acc[OTM_NRHS] = {0, 0, 0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i] * (double)b[k * nn + i];
+  }
+  reduce_finish<OTM_NRHS>(acc, R);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __restrict__ a, Reduce R) {
+  double acc[OTM_NRHS] = {0, 0, 0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i];
+  }
+  reduce_finish<OTM_NRHS>(acc, R);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, const PcgScalars* __restrict__ P) {
+  const float m0 = (float)P->mean[0], m1 = (float)P->mean[1], m2 = (float)P->mean[2];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+    a[i] -= m0;
+    a[nn + i] -= m1;
+    a[2 * nn + i] -= m2;
+  }
+}
+
+// x += alpha p, r -= alpha q, partial r.r
+__global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restrict__ x, float* __restrict__ r,
+                                                   const float* __restrict__ p, const float* __restrict__ q,
+                                                   const PcgScalars* __restrict__ P, Reduce R) {
+  float al[OTM_NRHS];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) al[k] = (float)P->alpha[k];
+  double acc[OTM_NRHS] = {0, 0, 0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const long long j = k * nn + i;
+      x[j] += al[k] * p[j];
+      const float rv = r[j] - al[k] * q[j];
+      r[j] = rv;
+      acc[k] += (double)rv * (double)rv;
+    }
+  }
+  reduce_finish<OTM_NRHS>(acc, R);
+}
+
+// p = z + beta p
+__global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restrict__ p, const float* __restrict__ z,
+                                                  const PcgScalars* __restrict__ P) {
+  float be[OTM_NRHS];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) be[k] = (float)P->beta[k];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const long long j = k * nn + i;
+      p[j] = z[j] + be[k] * p[j];
+    }
+  }
+}
+
+// dst[v] = src[x + pitch*(y + ny*z)] + add (pitched lexicographic input)
+__global__ void __launch_bounds__(kBlock) k_copy_pitched(Geom g, const float* __restrict__ src, int pitch,
+                                                         float* __restrict__ dst, float add) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const long long x = t % g.nx, r = t / g.nx;
+  dst[t] = src[x + (long long)pitch * r] + add;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_to_pitched(Geom g, const float* __restrict__ src,
+                                                            float* __restrict__ dst, int pitch) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  const long long x = t % g.nx, r = t / g.nx;
+  dst[x + (long long)pitch * r] = src[t];
+}
+
+void launch_fill(hipStream_t s, float* a, long long n, float v) {
+  unsigned nb = nblk(n);
+  if (nb > 4096) nb = 4096;
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_fill, dim3(nb), dim3(kBlock), 0, s, a, n, v);
+}
+void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_dot3, dim3(nblocks), dim3(kBlock), 0, s, nn, a, b, R);
+}
+void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_sum3, dim3(nblocks), dim3(kBlock), 0, s, nn, a, R);
+}
+void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P) {
+  unsigned nb = nblk(nn);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_sub_mean3, dim3(nb), dim3(kBlock), 0, s, nn, a, P);
+}
+void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,
+                   const PcgScalars* P, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_pcg_xr, dim3(nblocks), dim3(kBlock), 0, s, nn, x, r, p, q, P, R);
+}
+void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P) {
+  unsigned nb = nblk(nn);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(kBlock), 0, s, nn, p, z, P);
+}
+void launch_copy_pitched(hipStream_t s, const Geom& g, const float* src, int pitch, float* dst, float add) {
+  hipLaunchKernelGGL(k_copy_pitched, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, src, pitch, dst, add);
+}
+void launch_copy_to_pitched(hipStream_t s, const Geom& g, const float* src, float* dst, int pitch) {
+  hipLaunchKernelGGL(k_copy_to_pitched, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, src, dst, pitch);
+}
+
+}  // namespace otm

@@ -1,0 +1,154 @@
+"""ctypes binding of the FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker. Arrays are float64 with x
+fastest, i.e. numpy shape (nz, ny, nx).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise ImportError("oracle/liboracle.so missing: run make -C oracle")
+        L = ctypes.CDLL(path)
+        i, d, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        L.orc_template.argtypes = [d, _dp, _dp, _dp]
+        L.orc_assemble.argtypes = [i, i, i, _dp, _dp, _dp]
+        L.orc_matvec.argtypes = [i, i, i, _dp, _dp, _dp]
+        L.orc_rhs.argtypes = [i, i, i, _dp, _dp, i, _dp]
+        L.orc_cg.argtypes = [i, i, i, _dp, _dp, _dp, d, i, i, ctypes.POINTER(d)]
+        L.orc_cg.restype = i
+        L.orc_homogenize.argtypes = [i, i, i, _dp, d, i, vp, _dp, vp]
+        L.orc_homogenize.restype = i
+        L.orc_element_energy.argtypes = [i, i, i, _dp, _dp]
+        L.orc_sensitivity.argtypes = [i, _dp, _dp, _dp]
+        L.orc_filter_fwd.argtypes = [i, i, i, i, d, i, _dp, _dp, vp]
+        L.orc_filter_adj.argtypes = [i, i, i, i, d, i, _dp, _dp]
+        L.orc_oc_filter_sens.argtypes = [i, i, i, d, _dp, _dp, _dp]
+        L.orc_oc_update.argtypes = [i, _dp, _dp, _dp, d, d, d, d, i, d, ctypes.POINTER(d)]
+        L.orc_oc_update.restype = i
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def template(n):
+    Ke, fe, disp = np.zeros(64), np.zeros(24), np.zeros(24)
+    lib().orc_template(0.5 / n, Ke, fe, disp)
+    return Ke.reshape(8, 8), fe.reshape(8, 3), disp.reshape(8, 3)
+
+
+def assemble(kappa_eff):
+    k = _d(kappa_eff)
+    nz, ny, nx = k.shape
+    Ke, _, _ = template(nx)
+    A = np.zeros(27 * k.size)
+    lib().orc_assemble(nx, ny, nz, k.reshape(-1), _d(Ke).reshape(-1), A)
+    return A.reshape(27, k.size)
+
+
+def matvec(shape, A, x):
+    nz, ny, nx = shape
+    y = np.zeros(nx * ny * nz)
+    lib().orc_matvec(nx, ny, nz, _d(A).reshape(-1), _d(x).reshape(-1), y)
+    return y
+
+
+def rhs(kappa_eff, k):
+    ke = _d(kappa_eff)
+    nz, ny, nx = ke.shape
+    _, fe, _ = template(nx)
+    f = np.zeros(ke.size)
+    lib().orc_rhs(nx, ny, nz, ke.reshape(-1), _d(fe).reshape(-1), k, f)
+    return f
+
+
+def cg(shape, A, b, rtol=1e-12, pin=True, maxit=20000):
+    """FP64 Jacobi-PCG on the ELL-27 operator; returns (x, iterations, relres)."""
+    nz, ny, nx = shape
+    x = np.zeros(nx * ny * nz)
+    rr = ctypes.c_double()
+    it = lib().orc_cg(nx, ny, nz, _d(A).reshape(-1), _d(b).reshape(-1), x, rtol, maxit, 1 if pin else 0,
+                      ctypes.byref(rr))
+    return x, it, rr.value
+
+
+def homogenize(kappa, rtol=1e-12, pin=True, want_chi=False, want_S=False):
+    """FP64 kappa^H of the SIMP-mapped conductivity `kappa` (1e-9 floor added
+    inside). Returns (C, chi or None, S or None, iterations)."""
+    k = _d(kappa)
+    nz, ny, nx = k.shape
+    n = k.size
+    C = np.zeros(9)
+    chi = np.zeros(3 * n) if want_chi else None
+    S = np.zeros(6 * n) if want_S else None
+    cp = chi.ctypes.data_as(ctypes.c_void_p) if want_chi else None
+    sp = S.ctypes.data_as(ctypes.c_void_p) if want_S else None
+    it = lib().orc_homogenize(nx, ny, nz, k.reshape(-1), rtol, 1 if pin else 0, cp, C, sp)
+    return (C.reshape(3, 3), None if chi is None else chi.reshape(3, n),
+            None if S is None else S.reshape(6, n), it)
+
+
+def element_energy(shape, chi):
+    nz, ny, nx = shape
+    S = np.zeros(6 * nx * ny * nz)
+    lib().orc_element_energy(nx, ny, nz, _d(chi).reshape(-1), S)
+    return S.reshape(6, -1)
+
+
+def sensitivity(S6, dC):
+    S = _d(S6)
+    n = S.shape[1]
+    out = np.zeros(n)
+    lib().orc_sensitivity(n, S.reshape(-1), _d(dC).reshape(-1), out)
+    return out
+
+
+def filter_fwd(x, kind=0, r=1.5, periodic=False, want_W=False):
+    a = _d(x)
+    nz, ny, nx = a.shape
+    out = np.zeros(a.size)
+    W = np.zeros(a.size) if want_W else None
+    lib().orc_filter_fwd(nx, ny, nz, kind, r, 1 if periodic else 0, a.reshape(-1), out,
+                         W.ctypes.data_as(ctypes.c_void_p) if want_W else None)
+    out = out.reshape(a.shape)
+    return (out, W.reshape(a.shape)) if want_W else out
+
+
+def filter_adj(x, kind=0, r=1.5, periodic=False):
+    a = _d(x)
+    nz, ny, nx = a.shape
+    out = np.zeros(a.size)
+    lib().orc_filter_adj(nx, ny, nz, kind, r, 1 if periodic else 0, a.reshape(-1), out)
+    return out.reshape(a.shape)
+
+
+def oc_filter_sens(rho, sens, r=2.0):
+    a = _d(rho)
+    nz, ny, nx = a.shape
+    out = np.zeros(a.size)
+    lib().orc_oc_filter_sens(nx, ny, nz, r, a.reshape(-1), _d(sens).reshape(-1), out)
+    return out.reshape(a.shape)
+
+
+def oc_update(sens, rho, volratio, min_rho=0.001, step=0.02, damp=0.5, max_steps=20, vol_tol=1e-4):
+    s = _d(sens).reshape(-1)
+    r = _d(rho).reshape(-1)
+    out = np.zeros(r.size)
+    g = ctypes.c_double()
+    steps = lib().orc_oc_update(r.size, s, r, out, volratio, min_rho, step, damp, max_steps, vol_tol,
+                                ctypes.byref(g))
+    return out, steps, g.value

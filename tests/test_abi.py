@@ -1,0 +1,71 @@
+"""C-ABI boundary checks that need no GPU: the engine library loads, exports
+every entry point include/opentm.h declares, and its host-only calls
+behave as documented."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import opentm_amd
+from opentm_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"OTM_API[^;(]*?\b(otm_\w+)\s*\(", txt))
+    return names
+
+
+def test_header_matches_exports():
+    names = _declared()
+    assert len(names) >= 30
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names == set(_capi.SIGNATURES), names ^ set(_capi.SIGNATURES)
+
+
+def test_config_defaults():
+    lib = _capi.load()
+    c = _capi.OtmConfig()
+    lib.otm_config_init(ctypes.byref(c))
+    assert list(c.reso) == [128, 128, 128]          # cmdline.cpp:6-8
+    assert list(c.heat_ratio) == [1.0, 0.0]          # cmdline.cpp:12-13
+    assert c.pre_sweeps == 2 and c.post_sweeps == 2
+    assert c.fem_rel_thres == pytest.approx(1e-6)
+    p = _capi.OtmDensityParams()
+    lib.otm_density_params_init(ctypes.byref(p))
+    assert p.filter_kind == 0 and p.filter_radius == 1.5 and p.filter_periodic == 0
+    assert lib.otm_version() >= 100
+
+
+def test_argument_errors_without_gpu():
+    lib = _capi.load()
+    ctx = ctypes.c_void_p()
+    assert lib.otm_create(None, ctypes.byref(ctx)) == _capi.OTM_E_ARG
+    assert b"null" in lib.otm_last_error()
+    assert lib.otm_heat_matrix(None, None) == _capi.OTM_E_ARG
+    assert lib.otm_oc_set_bisection(None, 0, 0.0) == _capi.OTM_E_ARG
+    assert lib.otm_destroy(None) == _capi.OTM_OK
+    with pytest.raises(opentm_amd.OtmError):
+        _capi.check(lib.otm_update(None, None, 8, 0), "otm_update")
+
+
+def test_workload_generators_are_seeded():
+    from opentm_amd import workloads as w
+    a = w.plain_density(16, 3)
+    b = w.plain_density(16, 3)
+    c = w.plain_density(16, 4)
+    assert a.dtype == np.float32 and a.shape == (16, 16, 16)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)
+    assert a.min() >= 1e-3 and a.max() <= 1.0
+    s = w.oc_start_density(16, 0)
+    assert abs(float(s.mean()) - 0.3) < 0.01 and float(np.abs(s - 0.3).max()) <= 0.05 + 1e-6

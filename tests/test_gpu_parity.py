@@ -1,0 +1,242 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerances (north_star): kappa^H within 1e-5 relative, per-element
+sensitivities within 1e-4 relative, OC densities within 1e-4 L-inf over
+10 iterations. Operator-level kernels are FP32 and are held to 2e-6
+relative (normwise) against the FP64 oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import opentm_amd as otm
+from opentm_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def make(n, rtol=1e-6, k1=1e-3):
+    cfg = otm.HomoConfig(n, (1.0, k1))
+    cfg.femRelThres = rtol
+    return otm.Homogenization_H(cfg)
+
+
+# ------------------------------------------------------------ operators
+@pytest.mark.parametrize("n", [8, 16])
+def test_fine_operator(orc, n):
+    say("fine operator", n)
+    rng = np.random.default_rng(n)
+    k = rng.uniform(1e-3, 1.0, (n, n, n)).astype(np.float32)
+    hom = make(n)
+    hom.update(k)
+    u = rng.standard_normal((3, n ** 3)).astype(np.float32)
+    q = hom.apply_level(0, u).reshape(3, -1)
+    A = orc.assemble(k.astype(np.float64) + 1e-9)
+    for c in range(3):
+        ref = orc.matvec(k.shape, A, u[c].astype(np.float64))
+        assert np.linalg.norm(q[c] - ref) / np.linalg.norm(ref) < 2e-6
+
+
+def _prolongation(n):
+    nc = n // 2
+    P = np.zeros((n ** 3, nc ** 3))
+    w1 = {0: 1.0, 1: 0.5, -1: 0.5}
+    for z, y, x in np.ndindex(n, n, n):
+        for Z, Y, X in np.ndindex(nc, nc, nc):
+            w = 1.0
+            for a, A in ((x, X), (y, Y), (z, Z)):
+                d = (a - 2 * A) % n
+                d = d - n if d > n // 2 else d
+                w *= w1.get(d, 0.0)
+            if w:
+                P[x + n * (y + n * z), X + nc * (Y + nc * Z)] = w
+    return P
+
+
+def _dense(A, n):
+    N = n ** 3
+    D = np.zeros((N, N))
+    for o in range(27):
+        dx, dy, dz = o % 3 - 1, o // 3 % 3 - 1, o // 9 - 1
+        for v in range(N):
+            x, y, z = v % n, v // n % n, v // (n * n)
+            D[v, (x + dx) % n + n * ((y + dy) % n + n * ((z + dz) % n))] += A[o, v]
+    return D
+
+
+def test_galerkin_coarse_operator(orc):
+    n = 8
+    say("galerkin", n)
+    rng = np.random.default_rng(11)
+    k = rng.uniform(1e-3, 1.0, (n, n, n)).astype(np.float32)
+    hom = make(n)
+    hom.update(k)
+    assert hom.levels()[:2] == [(8, 8, 8), (4, 4, 4)]
+    P = _prolongation(n)
+    Kc = P.T @ _dense(orc.assemble(k.astype(np.float64) + 1e-9), n) @ P
+    Gc = _dense(hom.stencil(1).astype(np.float64), n // 2)
+    assert rel(Gc, Kc) < 2e-6
+    u = rng.standard_normal((3, (n // 2) ** 3)).astype(np.float32)
+    q = hom.apply_level(1, u).reshape(3, -1)
+    for c in range(3):
+        ref = Kc @ u[c]
+        assert np.linalg.norm(q[c] - ref) / np.linalg.norm(ref) < 2e-6
+
+
+# ------------------------------------------------------------ homogenization
+@pytest.mark.parametrize("n,seed", [(16, 0), (32, 0), (32, 1), (32, 2)])
+def test_heat_matrix(orc, n, seed):
+    say("heat matrix", n, seed)
+    k = wl.simp(wl.plain_density(n, seed))
+    hom = make(n)
+    hom.update(k)
+    C = hom.heatMatrix()
+    its, rr = hom.solve_stats()
+    Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+    say("  its", its, "relres", rr, "err", rel(C, Co))
+    assert rel(C, Co) < 1e-5
+    assert max(rr) < 1e-5
+
+
+def test_sensitivity(orc):
+    n = 32
+    say("sensitivity", n)
+    k = wl.simp(wl.plain_density(n, 5))
+    hom = make(n)
+    hom.update(k)
+    hom.heatMatrix()
+    S = hom.element_energy()
+    Co, chi, So, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12, want_S=True)
+    for c in range(6):
+        assert rel(S[c], So[c]) < 1e-4, c
+    dC = np.array([[1.0, 0.3, -0.2], [0.1, -0.5, 0.4], [0.25, 0.0, 2.0]])
+    s = hom.Sensitivity(dC)
+    assert rel(s, orc.sensitivity(So, dC)) < 1e-4
+    # pitched output: rows padded to 40 floats, padding left at zero
+    sp = hom.Sensitivity(dC, pitchT=40).reshape(n * n, 40)
+    np.testing.assert_array_equal(sp[:, :n].reshape(-1), s)
+    assert not sp[:, n:].any()
+
+
+def test_warm_start_and_reset():
+    n = 32
+    say("warm start", n)
+    k = wl.simp(wl.plain_density(n, 7))
+    hom = make(n)
+    hom.update(k)
+    C1 = hom.heatMatrix()
+    its1, _ = hom.solve_stats()
+    C2 = hom.heatMatrix()
+    its2, _ = hom.solve_stats()
+    assert max(its2) <= 1 and min(its1) >= 3
+    assert rel(C2, C1) < 1e-6
+    hom.reset_displacement()
+    C3 = hom.heatMatrix()
+    its3, _ = hom.solve_stats()
+    assert its3 == its1
+    np.testing.assert_array_equal(C3, C1)
+
+
+def test_full_size_closed_forms():
+    n = 128
+    say("closed forms", n)
+    hom = make(n)
+    hom.update(np.full((n, n, n), 0.6, np.float32))
+    C = hom.heatMatrix()
+    assert rel(C, 0.6 * np.eye(3)) < 1e-6
+    vals = np.where(np.arange(n) % 16 < 5, 1.0, 0.02).astype(np.float32)
+    lam = np.broadcast_to(vals[None, None, :], (n, n, n)).copy()
+    hom.reset_displacement()
+    hom.update(lam)
+    C = hom.heatMatrix()
+    ve = vals.astype(np.float64) + 1e-9
+    harm, arith = 1.0 / np.mean(1.0 / ve), np.mean(ve)
+    assert C[1, 1] == pytest.approx(harm, rel=1e-5)
+    assert C[0, 0] == pytest.approx(arith, rel=1e-5)
+    assert C[2, 2] == pytest.approx(arith, rel=1e-5)
+
+
+def test_full_size_shift_invariance():
+    n = 128
+    say("shift invariance", n)
+    k = wl.simp(wl.plain_density(n, 3))
+    hom = make(n)
+    hom.update(k)
+    C = hom.heatMatrix()
+    hom.reset_displacement()
+    hom.update(np.ascontiguousarray(np.roll(k, (5, -9, 17), axis=(0, 1, 2))))
+    Cs = hom.heatMatrix()
+    assert rel(Cs, C) < 1e-5
+    np.testing.assert_allclose(Cs, Cs.T, atol=0)
+
+
+# ------------------------------------------------------------ filters and OC
+@pytest.mark.parametrize("kind,r,periodic", [(0, 1.5, False), (1, 2.0, True), (0, 2.5, True)])
+def test_conv_filter(orc, kind, r, periodic):
+    say("filter", kind, r, periodic)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0, 1, (10, 12, 14)).astype(np.float32)
+    y = otm.conv_filter(x, kind, r, periodic)
+    assert rel(y, orc.filter_fwd(x, kind, r, periodic)) < 2e-6
+    ya = otm.conv_filter(x, kind, r, periodic, adjoint=True)
+    assert rel(ya, orc.filter_adj(x, kind, r, periodic)) < 2e-6
+
+
+def test_oc_filter_and_update(orc):
+    n = 16
+    say("oc", n)
+    rng = np.random.default_rng(13)
+    rho = rng.uniform(0.05, 0.95, (n, n, n)).astype(np.float32)
+    sens = (-rng.uniform(0.0, 1.0, (n, n, n))).astype(np.float32)
+    oc = otm.OCOptimizer(n ** 3, 0.001, 0.2, 0.5, bisect_steps=40, bisect_tol=1e-6)
+    s = sens.reshape(-1).copy()
+    oc.filterSens(s, rho.reshape(-1), n, (n, n, n), 2.0)
+    so = orc.oc_filter_sens(rho, sens, 2.0).reshape(-1)
+    assert rel(s, so) < 2e-6
+    r = rho.reshape(-1).copy()
+    oc.update(s, r, 0.35)
+    ro, steps, g = orc.oc_update(so, rho, 0.35, 0.001, 0.2, 0.5, 40, 1e-6)
+    assert np.abs(r - ro).max() < 1e-4
+    assert abs(oc.last_volume - 0.35) <= 1e-6 + 1e-7
+
+
+def test_oc_loop_parity(orc):
+    """Ten OC iterations of the max-trace problem: design densities within
+    1e-4 L-inf of the FP64 oracle loop at every iteration."""
+    n = 16
+    say("oc loop", n)
+    k0, k1 = 1.0, 1e-3
+    rho0 = wl.oc_start_density(n, 0)
+    hom = make(n, rtol=1e-7)
+    hom.set_design(rho0)
+    oc = otm.OCOptimizer(n ** 3, 0.001, 0.2, 0.5, bisect_steps=40, bisect_tol=1e-6)
+    dC = -np.eye(3)
+    r = rho0.astype(np.float64)
+    for it in range(10):
+        C = hom.eval()
+        hom.backward(dC, want=False)
+        oc.step(hom, 2.0, 0.3)
+        rg = hom.get_design()
+        rf, W = orc.filter_fwd(r, 0, 1.5, False, want_W=True)
+        kap = k1 + (k0 - k1) * rf ** 3
+        Co, _, So, _ = orc.homogenize(kap, rtol=1e-12, want_S=True)
+        sk = orc.sensitivity(So, dC).reshape(r.shape)
+        g = orc.filter_adj(sk * 3 * (k0 - k1) * rf ** 2, 0, 1.5, False)
+        gs = orc.oc_filter_sens(r, g, 2.0)
+        r, steps, _ = orc.oc_update(gs, r, 0.3, 0.001, 0.2, 0.5, 40, 1e-6)
+        r = r.reshape(rho0.shape)
+        say("  it", it, "C err", rel(C, Co), "rho err", float(np.abs(rg.reshape(r.shape) - r).max()))
+        assert rel(C, Co) < 1e-5
+        assert float(np.abs(rg.reshape(r.shape) - r).max()) < 1e-4
