@@ -21,17 +21,10 @@ __device__ inline long long lin(const Geom& g, int x, int y, int z) {
   return (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
 }
 
-// GS update of one of 8 colours (parity of x, y, z), colour = cx + 2cy + 4cz.
-__global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __restrict__ st,
-                                                        const float* __restrict__ f, float* u, int cx, int cy,
-                                                        int cz) {
-  const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
-  const long long cnt = (long long)hx * hy * hz;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
-  const int x = 2 * (int)(t % hx) + cx;
-  const long long r = t / hx;
-  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+// ---- per-node operations of the stored-stencil levels ----
+// GS update at node (x,y,z): u = (f - sum_{o != 13} a_o u_o) / a_13.
+__device__ inline void st_gs_node(const Geom& g, const float* __restrict__ st, const float* __restrict__ f,
+                                  float* u, int x, int y, int z) {
   const long long v = lin(g, x, y, z);
   const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
   const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
@@ -53,11 +46,9 @@ __global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __r
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, const float* __restrict__ st,
-                                                        const float* __restrict__ f, const float* __restrict__ u,
-                                                        float* __restrict__ res) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= g.nn) return;
+// res = f - A u at node t.
+__device__ inline void st_resid_node(const Geom& g, const float* __restrict__ st, const float* __restrict__ f,
+                                     const float* u, float* res, long long t) {
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
   const int y = (int)(r % g.ny), z = (int)(r / g.ny);
@@ -79,10 +70,7 @@ __global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, const float* __r
 
 // f_c(I) = sum_{d in {-1,0,1}^3} w(d) res(2I + d), w = prod(d_i == 0 ? 1 : 1/2)
 // (restrict_residual_kernel_H, grid.cu:811-878, weights (2-|d|)/2 per axis).
-__global__ void __launch_bounds__(kBlock) k_restrict(Geom fg, Geom cg, const float* __restrict__ res,
-                                                     float* __restrict__ fc) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cg.nn) return;
+__device__ inline void restrict_node(const Geom& fg, const Geom& cg, const float* res, float* fc, long long t) {
   const int X = (int)(t % cg.nx);
   const long long r = t / cg.nx;
   const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
@@ -111,10 +99,7 @@ __global__ void __launch_bounds__(kBlock) k_restrict(Geom fg, Geom cg, const flo
 
 // u_f(i) += sum_I P(i, I) u_c(I), trilinear (prolongate_correction_kernel_H,
 // grid.cu:881-940).
-__global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const float* __restrict__ uc,
-                                                        float* __restrict__ uf) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= fg.nn) return;
+__device__ inline void prolong_node(const Geom& fg, const Geom& cg, const float* uc, float* uf, long long t) {
   const int x = (int)(t % fg.nx);
   const long long r = t / fg.nx;
   const int y = (int)(r % fg.ny), z = (int)(r / fg.ny);
@@ -141,6 +126,43 @@ __global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const 
         }
     uf[k * fg.nn + t] += s;
   }
+}
+
+// ---- per-level kernels ----
+// GS update of one of 8 colours (parity of x, y, z), colour = cx + 2cy + 4cz.
+__global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __restrict__ st,
+                                                        const float* __restrict__ f, float* u, int cx, int cy,
+                                                        int cz) {
+  const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
+  const long long cnt = (long long)hx * hy * hz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int x = 2 * (int)(t % hx) + cx;
+  const long long r = t / hx;
+  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+  st_gs_node(g, st, f, u, x, y, z);
+}
+
+__global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, const float* __restrict__ st,
+                                                        const float* __restrict__ f, const float* __restrict__ u,
+                                                        float* __restrict__ res) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  st_resid_node(g, st, f, u, res, t);
+}
+
+__global__ void __launch_bounds__(kBlock) k_restrict(Geom fg, Geom cg, const float* __restrict__ res,
+                                                     float* __restrict__ fc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cg.nn) return;
+  restrict_node(fg, cg, res, fc, t);
+}
+
+__global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const float* __restrict__ uc,
+                                                        float* __restrict__ uf) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.nn) return;
+  prolong_node(fg, cg, uc, uf, t);
 }
 
 // Level-1 element matrices from kappa: E(E) = sum_c kappa_c G[c].
@@ -349,6 +371,77 @@ __global__ void __launch_bounds__(kBlock) k_coarsest_solve(Geom g, const double*
   u[(long long)k * n + i] = (float)s;
 }
 
+// Brick smoother for stored-stencil levels whose edges are multiples of 8.
+// One 512-thread workgroup owns an 8^3 brick: its nodes' stencils sit in
+// registers, u of the brick plus a one-node halo sits in LDS. All colours
+// of `sweeps` Gauss-Seidel sweeps run inside the brick with workgroup
+// barriers; halo values stay frozen at their values on entry (block
+// Gauss-Seidel across bricks, exact multicolour Gauss-Seidel inside). The
+// result goes to a second buffer so bricks never read a neighbour's new
+// values. The post-smoother runs the colours in reverse order, which is the
+// adjoint of the pre-smoother: the V-cycle stays symmetric.
+constexpr int kBrick = 8;
+constexpr int kBrickThreads = kBrick * kBrick * kBrick;
+constexpr int kHalo = kBrick + 2;
+__global__ void __launch_bounds__(kBrickThreads) k_st_smooth_brick(Geom g, const float* __restrict__ st,
+                                                                   const float* __restrict__ f,
+                                                                   const float* __restrict__ uin,
+                                                                   float* __restrict__ uout, int sweeps,
+                                                                   int forward, int zero_in) {
+  __shared__ float su[OTM_NRHS][kHalo * kHalo * kHalo];
+  const int bx = g.nx / kBrick, by = g.ny / kBrick;
+  const int b = blockIdx.x;
+  const int ox = (b % bx) * kBrick, oy = ((b / bx) % by) * kBrick, oz = (b / (bx * by)) * kBrick;
+  const int t = threadIdx.x;
+  const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+  const int x = ox + lx, y = oy + ly, z = oz + lz;
+  const int nn = (int)g.nn;
+  const int v = x + g.nx * (y + g.ny * z);
+  // halo load (or zero initial guess)
+  for (int i = t; i < kHalo * kHalo * kHalo; i += kBrickThreads) {
+    const int hx = i % kHalo, hy = (i / kHalo) % kHalo, hz = i / (kHalo * kHalo);
+    if (zero_in) {
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
+    } else {
+      const int X = wrapm(ox + hx - 1, g.nx), Y = wrapm(oy + hy - 1, g.ny), Z = wrapm(oz + hz - 1, g.nz);
+      const int w = X + g.nx * (Y + g.ny * Z);
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = uin[k * (long long)nn + w];
+    }
+  }
+  float a[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) a[o] = st[o * (long long)nn + v];
+  float fv[OTM_NRHS];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) fv[k] = f[k * (long long)nn + v];
+  const float inv = 1.f / a[13];
+  const int mycol = (x & 1) + 2 * (y & 1) + 4 * (z & 1);
+  const int c0 = (lx + 1) + kHalo * ((ly + 1) + kHalo * (lz + 1));
+  __syncthreads();
+  for (int s = 0; s < sweeps; s++)
+    for (int i = 0; i < 8; i++) {
+      const int col = forward ? i : 7 - i;
+      if (col == mycol) {
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) {
+          float acc = 0.f;
+#pragma unroll
+          for (int o = 0; o < 27; o++) {
+            if (o == 13) continue;
+            const int d = (o % 3 - 1) + kHalo * (((o / 3) % 3 - 1) + kHalo * (o / 9 - 1));
+            acc += a[o] * su[k][c0 + d];
+          }
+          su[k][c0] = (fv[k] - acc) * inv;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) uout[k * (long long)nn + v] = su[k][c0];
+}
+
 // ---- launchers ----
 void launch_st_gs_color(hipStream_t s, const Geom& g, const float* st, const float* f, float* u, int color) {
   const long long cnt = (long long)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
@@ -376,6 +469,13 @@ void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, 
 }
 void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv) {
   hipLaunchKernelGGL(k_coarsest_factor, dim3(1), dim3(kFactorThreads), 0, s, g, st, B, X, Binv);
+}
+bool brick_ok(const Geom& g) { return g.nx % kBrick == 0 && g.ny % kBrick == 0 && g.nz % kBrick == 0; }
+void launch_st_smooth_brick(hipStream_t s, const Geom& g, const float* st, const float* f, const float* uin,
+                            float* uout, int sweeps, bool forward, bool zero_in) {
+  const unsigned nb = (unsigned)((g.nx / kBrick) * (g.ny / kBrick) * (g.nz / kBrick));
+  hipLaunchKernelGGL(k_st_smooth_brick, dim3(nb), dim3(kBrickThreads), 0, s, g, st, f, uin, uout, sweeps,
+                     forward ? 1 : 0, zero_in ? 1 : 0);
 }
 void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u) {
   hipLaunchKernelGGL(k_coarsest_solve, dim3(nblk(g.nn), OTM_NRHS), dim3(kBlock), 0, s, g, Binv, f, u);

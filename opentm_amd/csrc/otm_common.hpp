@@ -53,7 +53,7 @@ struct PcgScalars {
   double beta[OTM_NRHS];
   double mean[OTM_NRHS];   // scratch for mean removal
   int active[OTM_NRHS];    // 1 while the right-hand side is iterating
-  int fault;               // set when a reduction produced NaN/Inf
+  int fault;               // bit 0: NaN/Inf in a reduction; bit 1: preconditioner not positive
 };
 
 // Reduction finalisers run by the last block of a reduction kernel.
@@ -69,7 +69,6 @@ enum FinalMode : int {
 
 struct Reduce {
   double* partials;      // [nblocks * nv]
-  unsigned int* counter; // zero between launches; reset by the last block
   double* out;           // FIN_STORE target
   PcgScalars* pcg;       // for PCG modes
   double rtol2;          // squared relative tolerance for FIN_RR
@@ -78,6 +77,7 @@ struct Reduce {
 };
 
 constexpr int kBlock = 256;       // threads per block for streaming kernels
+constexpr int kMaxReduce = 9;     // values per reduction
 constexpr int kWaves = kBlock / 64;
 
 // Wave64 sum through cross-lane shuffles.
@@ -108,74 +108,80 @@ __device__ inline void block_sum(double (&v)[NV], double* sh) {
   }
 }
 
-// Deterministic two-level reduction: every block stores its partial sums;
-// the last block to finish (atomic ticket after a device-scope fence) sums
-// the partials in block order and applies the finaliser. Fixed summation
-// order makes results independent of scheduling.
+// Deterministic two-level reduction, stage 1: every block stores its NV
+// partial sums (FP64). Stage 2 (k_reduce_final, launched right after on
+// the same stream) sums the partials in block order and applies the
+// finaliser; the kernel boundary makes the partials visible without
+// device-scope fences inside the producing kernel.
 template <int NV>
-__device__ inline void reduce_finish(double (&v)[NV], const Reduce& R) {
+__device__ inline void reduce_block(double (&v)[NV], const Reduce& R) {
   __shared__ double sh[NV * kWaves];
-  __shared__ bool last;
   block_sum<NV>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < NV; i++) R.partials[(size_t)blockIdx.x * NV + i] = v[i];
-    __threadfence();
-    unsigned t = atomicAdd(R.counter, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const int nb = gridDim.x;
-  double acc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; i++) acc[i] = 0.0;
-  // each thread sums a contiguous strip of blocks, then a fixed-order block sum
-  for (int b = threadIdx.x; b < nb; b += kBlock) {
-#pragma unroll
-    for (int i = 0; i < NV; i++) acc[i] += R.partials[(size_t)b * NV + i];
-  }
-  __syncthreads();
-  block_sum<NV>(acc, sh);
-  if (threadIdx.x != 0) return;
-  *R.counter = 0u;
+}
+
+// Applies a finaliser to the summed values (one thread).
+__device__ inline void finalize(const double* acc, int nv, const Reduce& R) {
   PcgScalars* P = R.pcg;
   switch (R.mode) {
     case FIN_STORE:
-#pragma unroll
-      for (int i = 0; i < NV; i++) R.out[i] = acc[i];
+      for (int i = 0; i < nv; i++) R.out[i] = acc[i];
       break;
-    case FIN_BB_RR:
+    case FIN_BB_RR: {
+      // sums = {b.b x3, r.r x3, sum(r) x3}. The mean of r is projected out
+      // afterwards (a periodic cell's system is consistent only for
+      // mean-free right-hand sides), so r.r is reported for the projected r.
+      // A load vector that is zero up to rounding (e.g. a gradient along a
+      // laminate) is measured against the largest of the three: its
+      // reference norm is floored at 1e-3 of the largest norm (the
+      // reference floors the norm with + 1e-10, MG.cpp:100).
+      double bmax = 0.0;
+      for (int k = 0; k < OTM_NRHS; k++) bmax = acc[k] > bmax ? acc[k] : bmax;
       for (int k = 0; k < OTM_NRHS; k++) {
-        P->bb[k] = acc[k];
-        P->rr[k] = acc[OTM_NRHS + k];
-        double bb = acc[k] > 0 ? acc[k] : 1.0;
-        P->active[k] = (acc[OTM_NRHS + k] > R.rtol2 * bb) ? 1 : 0;
+        double bb = acc[k] > 1e-6 * bmax ? acc[k] : 1e-6 * bmax;
+        if (!(bb > 0.0)) bb = 1.0;
+        const double mean = acc[2 * OTM_NRHS + k] * R.inv_nn;
+        double rr = acc[OTM_NRHS + k] - mean * acc[2 * OTM_NRHS + k];
+        if (rr < 0.0) rr = 0.0;
+        P->mean[k] = mean;
+        P->bb[k] = bb;
+        P->rr[k] = rr;
+        if (!(acc[OTM_NRHS + k] - acc[OTM_NRHS + k] == 0.0)) P->fault |= 1;
+        P->active[k] = (rr > R.rtol2 * bb) ? 1 : 0;
         P->alpha[k] = 0.0;
         P->beta[k] = 0.0;
       }
       break;
+    }
     case FIN_RZ_FIRST:
-      for (int k = 0; k < OTM_NRHS; k++) { P->rz[k] = acc[k]; P->beta[k] = 0.0; }
+      for (int k = 0; k < OTM_NRHS; k++) {
+        P->rz[k] = acc[k];
+        P->beta[k] = 0.0;
+        if (P->active[k] && !(acc[k] > 0.0)) P->fault |= 2;  // preconditioner not positive
+      }
       break;
     case FIN_RZ:
       for (int k = 0; k < OTM_NRHS; k++) {
         P->beta[k] = (P->active[k] && P->rz[k] != 0.0) ? acc[k] / P->rz[k] : 0.0;
         P->rz[k] = acc[k];
+        if (P->active[k] && !(acc[k] > 0.0)) P->fault |= 2;
       }
       break;
     case FIN_PQ:
       for (int k = 0; k < OTM_NRHS; k++) {
         P->pq[k] = acc[k];
         P->alpha[k] = (P->active[k] && acc[k] > 0.0) ? P->rz[k] / acc[k] : 0.0;
+        if (P->active[k] && !(acc[k] > 0.0)) P->fault |= 2;
       }
       break;
     case FIN_RR:
       for (int k = 0; k < OTM_NRHS; k++) {
         P->rr[k] = acc[k];
         double bb = P->bb[k] > 0 ? P->bb[k] : 1.0;
-        if (!(acc[k] - acc[k] == 0.0)) P->fault = 1;  // NaN or Inf
+        if (!(acc[k] - acc[k] == 0.0)) P->fault |= 1;  // NaN or Inf
         if (!(acc[k] > R.rtol2 * bb)) P->active[k] = 0;
       }
       break;

@@ -13,6 +13,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/opentm.h"
@@ -177,6 +178,8 @@ struct Level {
   float* res = nullptr;  // residual before restriction
   float* st = nullptr;   // 27-point stencil (levels >= 1)
   float* E = nullptr;    // element matrices, 36 per element (levels >= 1)
+  float* u2 = nullptr;   // second correction buffer for the brick smoother
+  bool brick = false;    // smoothed with LDS bricks (edges multiples of 8)
   bool own_uf = true;
 };
 
@@ -198,7 +201,6 @@ struct otm_ctx {
   double *cB = nullptr, *cX = nullptr, *cBinv = nullptr;
   // reductions
   double* partials = nullptr;
-  unsigned* counter = nullptr;
   double* red_out = nullptr;
   PcgScalars* pcg = nullptr;
   PcgScalars* pcg_host = nullptr;  // pinned
@@ -219,6 +221,8 @@ struct otm_ctx {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
+  bool bricks_on = true;   // coarse levels use the LDS brick smoother
+  int fallbacks = 0;       // solves that fell back to exact Gauss-Seidel
   long long prof_launches[OTM_PROF_NCLASSES] = {};
   double prof_ms[OTM_PROF_NCLASSES] = {};
   double prof_bytes[OTM_PROF_NCLASSES] = {};
@@ -231,7 +235,6 @@ struct otm_ctx {
   Reduce reducer(int mode, double rtol2 = 0.0, double inv_nn = 0.0) const {
     Reduce R;
     R.partials = partials;
-    R.counter = counter;
     R.out = red_out;
     R.pcg = pcg;
     R.rtol2 = rtol2;
@@ -301,11 +304,9 @@ struct otm_oc {
   hipStream_t s = nullptr;
   float *newrho = nullptr, *newsens = nullptr, *a = nullptr, *b = nullptr;
   unsigned* maxbits = nullptr;
+  OcState* st = nullptr;
+  OcState* st_host = nullptr;  // pinned
   double* partials = nullptr;
-  unsigned* counter = nullptr;
-  double* red_out = nullptr;
-  double* host = nullptr;  // pinned
-  unsigned* host_bits = nullptr;
 };
 
 // ---- context lifetime --------------------------------------------------------
@@ -320,10 +321,11 @@ static void ctx_free(otm_ctx* c) {
     dfree(L.res);
     dfree(L.st);
     dfree(L.E);
+    dfree(L.u2);
   }
   dfree(c->kappa); dfree(c->x); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q);
   dfree(c->cB); dfree(c->cX); dfree(c->cBinv);
-  dfree(c->partials); dfree(c->counter); dfree(c->red_out); dfree(c->pcg);
+  dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
   if (c->out_host) (void)hipHostFree(c->out_host);
   dfree(c->rho); dfree(c->rho_f); dfree(c->W); dfree(c->kraw); dfree(c->dkappa);
@@ -380,6 +382,10 @@ static void build_hierarchy(otm_ctx* c) {
       L.f = dalloc<float>(3 * n);
       L.st = dalloc<float>(27 * n);
       L.E = dalloc<float>(36 * n);
+      if (l + 1 < geoms.size() && brick_ok(L.g) && cfg.pre_sweeps == cfg.post_sweeps) {
+        L.brick = true;
+        L.u2 = dalloc<float>(3 * n);
+      }
     }
     if (l + 1 < geoms.size()) L.res = dalloc<float>(3 * n);
     c->lv.push_back(L);
@@ -387,12 +393,10 @@ static void build_hierarchy(otm_ctx* c) {
   c->cB = dalloc<double>(ncoarse * ncoarse);
   c->cX = dalloc<double>(ncoarse * ncoarse);
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
-  c->partials = dalloc<double>(2048 * 8);
-  c->counter = dalloc<unsigned>(4);
-  hip_check(hipMemset(c->counter, 0, 4 * sizeof(unsigned)), "hipMemset counter");
+  c->partials = dalloc<double>(2048 * kMaxReduce);
   c->red_out = dalloc<double>(16);
   c->pcg = dalloc<PcgScalars>(1);
-  hip_check(hipMemset(c->pcg, 0, sizeof(PcgScalars)), "hipMemset pcg");
+  hip_check(hipMemsetAsync(c->pcg, 0, sizeof(PcgScalars), c->s), "hipMemset pcg");
   hip_check(hipHostMalloc((void**)&c->pcg_host, sizeof(PcgScalars), hipHostMallocDefault), "hipHostMalloc");
   hip_check(hipHostMalloc((void**)&c->out_host, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
   launch_fill(c->s, c->x, 3 * n0, 0.f);
@@ -448,11 +452,17 @@ static void vcycle(otm_ctx* c, size_t l) {
     launch_coarsest_solve(c->s, L.g, c->cBinv, L.f, L.u);
     return;
   }
-  {
-    ProfScope ps(c, OTM_PROF_VECTOR, 12.0 * nn);
-    launch_fill(c->s, L.u, 3 * L.g.nn, 0.f);
+  if (L.brick) {
+    // pre-smoothing from a zero guess straight into u
+    ProfScope ps(c, OTM_PROF_COARSE_GS, 132.0 * nn);  // stencil 108 + f 12 + u 12 bytes per node
+    launch_st_smooth_brick(c->s, L.g, L.st, L.f, nullptr, L.u, c->cfg.pre_sweeps, true, true);
+  } else {
+    {
+      ProfScope ps(c, OTM_PROF_VECTOR, 12.0 * nn);
+      launch_fill(c->s, L.u, 3 * L.g.nn, 0.f);
+    }
+    smooth(c, l, c->cfg.pre_sweeps, true);
   }
-  smooth(c, l, c->cfg.pre_sweeps, true);
   if (l == 0) {
     ProfScope ps(c, OTM_PROF_FINE_RESID, 40.0 * nn);
     launch_fine_residual(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.res);
@@ -470,25 +480,54 @@ static void vcycle(otm_ctx* c, size_t l) {
     ProfScope ps(c, OTM_PROF_TRANSFER, 24.0 * nn + 12.0 * (double)C.g.nn);
     launch_prolong_add(c->s, L.g, C.g, C.u, L.u);
   }
-  smooth(c, l, c->cfg.post_sweeps, false);
+  if (L.brick) {
+    // post-smoothing reads u (frozen halos) and writes u2; swap so u holds the result
+    ProfScope ps(c, OTM_PROF_COARSE_GS, 155.0 * nn);  // plus the u halo read, 12 * 10^3/8^3
+    launch_st_smooth_brick(c->s, L.g, L.st, L.f, L.u, L.u2, c->cfg.post_sweeps, false, false);
+    std::swap(L.u, L.u2);
+  } else {
+    smooth(c, l, c->cfg.post_sweeps, false);
+  }
 }
 
 // ---- PCG over the three load cases -------------------------------------------
-static void pcg_solve(otm_ctx* c) {
+// Three preconditioned CG recurrences in lock step, one per load case, with
+// the V-cycle as preconditioner (the reference iterates the V-cycle as a
+// stationary method instead, MG.cpp:80-142). The starting residual is
+// computed in FP64 and made mean-free; the true residual of the final
+// iterate is recomputed and the iteration restarts from it when the FP32
+// recurrence drifted above the tolerance (residual correction).
+// Returns false when the preconditioner turned out not to be positive.
+static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
   const long long n = c->g0.nn;
   const unsigned nb = red_blocks_for(n);
   const double rtol = c->cfg.fem_rel_thres;
   const double rtol2 = rtol * rtol;
+  const double inv_n = 1.0 / (double)n;
   int it = 0;
-  int its[3] = {0, 0, 0};
+  double prev_worst = 1e300;
   for (int restart = 0; restart < 3; restart++) {
     {
       ProfScope ps(c, OTM_PROF_FINE_RESID, 28.0 * (double)n);
-      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2), nb);
+      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2, inv_n), nb);
     }
     c->read_pcg();
-    if (c->pcg_host->fault) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+    if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
     if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
+    if (it >= c->cfg.max_cg_iter) break;
+    // restart only while the true residual still improves: with the iterate
+    // stored in FP32 it stalls near the FP32 resolution of the solution
+    double worst = 0.0;
+    for (int k = 0; k < 3; k++) {
+      const double q = c->pcg_host->rr[k] / c->pcg_host->bb[k];
+      worst = q > worst ? q : worst;
+    }
+    if (restart > 0 && worst > 0.25 * prev_worst) break;
+    prev_worst = worst;
+    {
+      ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n);
+      launch_sub_mean3(c->s, n, c->r, c->pcg);
+    }
     vcycle(c, 0);
     {
       ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 12.0 * (double)n + 36.0 * (double)n);
@@ -496,7 +535,6 @@ static void pcg_solve(otm_ctx* c) {
       launch_fill(c->s, c->p, 3 * n, 0.f);
       launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
     }
-    bool done = false;
     while (it < c->cfg.max_cg_iter) {
       {
         ProfScope ps(c, OTM_PROF_FINE_APPLY, 28.0 * (double)n);
@@ -510,11 +548,12 @@ static void pcg_solve(otm_ctx* c) {
       c->read_pcg();
       for (int k = 0; k < 3; k++)
         if (c->pcg_host->active[k]) its[k]++;
-      if (c->pcg_host->fault) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
-      if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) {
-        done = true;
-        break;
+      if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+      if (c->pcg_host->fault & 2) {
+        *total = it;
+        return false;
       }
+      if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
       vcycle(c, 0);
       {
         ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 36.0 * (double)n);
@@ -522,22 +561,51 @@ static void pcg_solve(otm_ctx* c) {
         launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
       }
     }
-    if (!done) break;
+  }
+  *total = it;
+  return true;
+}
+
+static void set_bricks(otm_ctx* c, bool on) {
+  for (size_t l = 1; l < c->lv.size(); l++) c->lv[l].brick = on && c->lv[l].u2 != nullptr;
+  c->bricks_on = on;
+}
+
+static void pcg_solve(otm_ctx* c) {
+  const long long n = c->g0.nn;
+  const unsigned nb = red_blocks_for(n);
+  int its[3] = {0, 0, 0};
+  int it = 0;
+  hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
+  if (!pcg_iterate(c, its, &it)) {
+    // The block-hybrid coarse smoother lost positivity on this operator:
+    // fall back to exact multicolour Gauss-Seidel on every level (always a
+    // positive preconditioner) for this context and solve again.
+    set_bricks(c, false);
+    c->fallbacks++;
+    hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
+    int it2 = 0;
+    if (!pcg_iterate(c, its, &it2)) fail(OTM_E_NUMERIC, "preconditioner not positive (MG numerical explode)");
+    it += it2;
   }
   // true residual of the final iterate
-  launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2), nb);
+  launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r,
+                            c->reducer(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n), nb);
+  c->read_pcg();
+  double relres[3];
+  for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
   // remove the constant component (translateForce, grid.cpp:654-664)
   launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
   launch_sub_mean3(c->s, n, c->x, c->pcg);
   c->read_pcg();
   for (int k = 0; k < 3; k++) {
-    const double bb = c->pcg_host->bb[k] > 0 ? c->pcg_host->bb[k] : 1.0;
-    c->relres[k] = sqrt(c->pcg_host->rr[k] / bb);
+    c->relres[k] = relres[k];
     c->iters[k] = its[k] + (its[k] < it ? 1 : 0);
   }
-  if (c->pcg_host->fault) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+  if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
   if (c->cfg.verbose)
-    fprintf(stderr, "[otm] PCG its=%d relres=(%.3e, %.3e, %.3e)\n", it, c->relres[0], c->relres[1], c->relres[2]);
+    fprintf(stderr, "[otm] PCG its=%d relres=(%.3e, %.3e, %.3e)%s\n", it, c->relres[0], c->relres[1], c->relres[2],
+            c->bricks_on ? "" : " (exact GS)");
 }
 
 static void kappa_h(otm_ctx* c, double C[9]) {
@@ -982,9 +1050,8 @@ static void oc_free(otm_oc* o) {
   if (!o) return;
   if (o->s) (void)hipStreamSynchronize(o->s);
   dfree(o->newrho); dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits);
-  dfree(o->partials); dfree(o->counter); dfree(o->red_out);
-  if (o->host) (void)hipHostFree(o->host);
-  if (o->host_bits) (void)hipHostFree(o->host_bits);
+  dfree(o->st); dfree(o->partials);
+  if (o->st_host) (void)hipHostFree(o->st_host);
   if (o->s) (void)hipStreamDestroy(o->s);
   delete o;
 }
@@ -1007,12 +1074,9 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
       o->a = dalloc<float>(ne);
       o->b = dalloc<float>(ne);
       o->maxbits = dalloc<unsigned>(1);
+      o->st = dalloc<OcState>(1);
       o->partials = dalloc<double>(2048);
-      o->counter = dalloc<unsigned>(1);
-      hip_check(hipMemset(o->counter, 0, sizeof(unsigned)), "memset");
-      o->red_out = dalloc<double>(4);
-      hip_check(hipHostMalloc((void**)&o->host, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-      hip_check(hipHostMalloc((void**)&o->host_bits, sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&o->st_host, sizeof(OcState), hipHostMallocDefault), "hipHostMalloc");
     } catch (...) {
       oc_free(o);
       throw;
@@ -1066,43 +1130,25 @@ OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pi
 
 static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volratio, int* steps, double* volume) {
   const long long n = o->ne;
-  hip_check(hipMemsetAsync(o->maxbits, 0, sizeof(unsigned), o->s), "memset");
-  launch_maxabs(o->s, n, sens, o->maxbits);
-  hip_check(hipMemcpyAsync(o->host_bits, o->maxbits, sizeof(unsigned), hipMemcpyDeviceToHost, o->s), "read max");
-  hip_check(hipStreamSynchronize(o->s), "sync");
-  float mx;
-  memcpy(&mx, o->host_bits, sizeof(float));
-  double lo = 0.0, hi = fabs((double)mx);
   const unsigned nb = red_blocks_for(n);
   Reduce R;
   R.partials = o->partials;
-  R.counter = o->counter;
-  R.out = o->red_out;
+  R.out = nullptr;
   R.pcg = nullptr;
   R.rtol2 = 0;
   R.inv_nn = 0;
   R.mode = FIN_STORE;
-  int it = 0;
-  double vol = 0.0;
-  for (it = 0; it < o->max_steps;) {
-    // the reference evaluates the midpoint in FP32 (OCOptimizer.cu:44-48)
-    const float gm = (float)((hi + lo) / 2.0);
-    launch_oc_apply_sum(o->s, n, sens, rho, o->newrho, gm, (float)o->min_rho, (float)o->step, (float)o->damp, R, nb);
-    hip_check(hipMemcpyAsync(o->host, o->red_out, sizeof(double), hipMemcpyDeviceToHost, o->s), "read vol");
-    hip_check(hipStreamSynchronize(o->s), "sync");
-    vol = o->host[0] / (double)n;
-    it++;
-    if (vol < volratio - o->vol_tol)
-      hi = gm;
-    else if (vol > volratio + o->vol_tol)
-      lo = gm;
-    else
-      break;
-  }
+  // the whole bisection is queued without host round trips; midpoints after
+  // the one that meets the tolerance are skipped on the device
+  launch_oc_begin(o->s, n, sens, o->maxbits, o->st);
+  for (int i = 0; i < o->max_steps; i++)
+    launch_oc_bisect_step(o->s, n, sens, rho, o->newrho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp,
+                          volratio, o->vol_tol, R, nb);
   hip_check(hipMemcpyAsync(rho, o->newrho, sizeof(float) * n, hipMemcpyDeviceToDevice, o->s), "copy rho");
+  hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
   hip_check(hipStreamSynchronize(o->s), "sync");
-  if (steps) *steps = it;
-  if (volume) *volume = vol;
+  if (steps) *steps = o->st_host->steps;
+  if (volume) *volume = o->st_host->vol;
 }
 
 OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volratio, int where, int* steps,

@@ -85,6 +85,33 @@ __device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems
   return s;
 }
 
+// FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
+__device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeElems& E,
+                                   const float* __restrict__ u) {
+  const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
+  const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
+  const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
+  const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
+  const double u_mm0 = u[rM0 + n.xm], u_pm0 = u[rM0 + n.xp];
+  const double u_mp0 = u[rP0 + n.xm], u_pp0 = u[rP0 + n.xp];
+  const double u_m0m = u[r0m + n.xm], u_p0m = u[r0m + n.xp];
+  const double u_m0p = u[r0p + n.xm], u_p0p = u[r0p + n.xp];
+  const double u_mmm = u[rMm + n.xm], u_0mm = u[rMm + n.x], u_pmm = u[rMm + n.xp];
+  const double u_mmp = u[rMp + n.xm], u_0mp = u[rMp + n.x], u_pmp = u[rMp + n.xp];
+  const double u_mpm = u[rPm + n.xm], u_0pm = u[rPm + n.x], u_ppm = u[rPm + n.xp];
+  const double u_mpp = u[rPp + n.xm], u_0pp = u[rPp + n.x], u_ppp = u[rPp + n.xp];
+  double s = 0.0;
+  s += (double)E.k[0] * (u_mm0 + u_m0m + u_0mm + u_mmm);
+  s += (double)E.k[1] * (u_pm0 + u_p0m + u_0mm + u_pmm);
+  s += (double)E.k[2] * (u_mp0 + u_m0m + u_0pm + u_mpm);
+  s += (double)E.k[3] * (u_pp0 + u_p0m + u_0pm + u_ppm);
+  s += (double)E.k[4] * (u_mm0 + u_m0p + u_0mp + u_mmp);
+  s += (double)E.k[5] * (u_pm0 + u_p0p + u_0mp + u_pmp);
+  s += (double)E.k[6] * (u_mp0 + u_m0p + u_0pp + u_mpp);
+  s += (double)E.k[7] * (u_pp0 + u_p0p + u_0pp + u_ppp);
+  return s;
+}
+
 // Load vector of case k at the node (grid.cu:1282-1342): sum_e kappa_e feMu[7-e][k].
 __device__ inline float load_vec(const NodeElems& E, const FineConsts& fc, int k) {
   float s = 0.f;
@@ -154,15 +181,18 @@ __global__ void __launch_bounds__(kBlock) k_fine_apply_dot(Geom g, FineConsts fc
       acc[k] += (double)pv * (double)kp;
     }
   }
-  reduce_finish<OTM_NRHS>(acc, R);
+  reduce_block<OTM_NRHS>(acc, R);
 }
 
-// r = b - K x with b the load vectors built on the fly; partial sums of
-// b.b and r.r (PCG start).
+// r = b - K x in FP64 with b the load vectors built on the fly (the FP64
+// sum of the eight element contributions cancels exactly wherever kappa
+// does not vary along the load direction); stored rounded to FP32. Partial
+// sums of b.b, r.r and sum(r) (PCG start and restarts).
 __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                                const float* __restrict__ x_, float* __restrict__ rvec,
                                                                Reduce R) {
-  double acc[2 * OTM_NRHS] = {0, 0, 0, 0, 0, 0};
+  double acc[3 * OTM_NRHS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double c = (double)fc.c;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
        t += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(t % g.nx);
@@ -170,18 +200,24 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConst
     const int y = (int)(r % g.ny), z = (int)(r / g.ny);
     const NodeIdx n = node_idx(g, x, y, z);
     const NodeElems E = load_elems(g, n, kappa);
+    double ksum = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ksum += (double)E.k[e];
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const float* xk = x_ + k * g.nn;
-      const float b = load_vec(E, fc, k);
-      const float kx = fc.c * (4.f * E.ksum * xk[n.v] - far_sum(g, n, E, xk));
-      const float rv = b - kx;
-      rvec[k * g.nn + n.v] = rv;
-      acc[k] += (double)b * (double)b;
-      acc[OTM_NRHS + k] += (double)rv * (double)rv;
+      double b = 0.0;
+#pragma unroll
+      for (int e = 0; e < 8; e++) b += (double)E.k[e] * (double)fc.femu[7 - e][k];
+      const double kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
+      const double rv = b - kx;
+      rvec[k * g.nn + n.v] = (float)rv;
+      acc[k] += b * b;
+      acc[OTM_NRHS + k] += rv * rv;
+      acc[2 * OTM_NRHS + k] += rv;
     }
   }
-  reduce_finish<2 * OTM_NRHS>(acc, R);
+  reduce_block<3 * OTM_NRHS>(acc, R);
 }
 
 // Element energies (homogenization.cu:13-153 and :255-412). For element e
@@ -235,7 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, 
 #pragma unroll
     for (int c = 0; c < 6; c++) acc[c] += ke * (double)S[c];
   }
-  reduce_finish<6>(acc, R);
+  reduce_block<6>(acc, R);
 }
 
 // sens[x + pitch*(y + ny*z)] = sum_c w_c S_c (contracted sensitivity with
@@ -285,16 +321,19 @@ void launch_fine_residual(hipStream_t s, const Geom& g, const FineConsts& fc, co
 void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* p,
                            float* q, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_fine_apply_dot, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, p, q, R);
+  launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const float* x, float* r, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_fine_init_residual, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
+  launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
                          const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_fine_kappa_h, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, chi, R);
+  launch_reduce_final(s, R, 6, nblocks);
 }
 
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],

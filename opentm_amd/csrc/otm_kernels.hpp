@@ -32,7 +32,13 @@ void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, 
 void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv);
 void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u);
 
+// Brick smoother (8^3 bricks, frozen halo, result in uout).
+bool brick_ok(const Geom& g);
+void launch_st_smooth_brick(hipStream_t s, const Geom& g, const float* st, const float* f, const float* uin,
+                            float* uout, int sweeps, bool forward, bool zero_in);
+
 // ---- vectors and reductions (otm_vec.hip) ----
+void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblocks);
 void launch_fill(hipStream_t s, float* a, long long n, float v);
 void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks);
@@ -58,8 +64,19 @@ void launch_simp_fwd(hipStream_t s, long long n, const float* rho, float k0, flo
 void launch_mul(hipStream_t s, long long n, const float* a, const float* b, float* out);
 void launch_oc_filter_sens(hipStream_t s, const Geom& g, const Taps& T, const float* sens, const float* rho,
                            float* out);
+// Device-side OC bisection state (OCOptimizer::update, OCOptimizer.cu:38-67).
+struct OcState {
+  float lo, hi;   // bracket of the Lagrange multiplier (FP32 as in the reference)
+  float g;        // last evaluated midpoint
+  int done;       // volume within tolerance
+  int steps;      // midpoints evaluated
+  int pad;
+  double vol;     // volume fraction at the last midpoint
+};
 void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out_bits);
-void launch_oc_apply_sum(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new, float g,
-                         float min_rho, float step, float damp, const Reduce& R, unsigned nblocks);
+void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st);
+void launch_oc_bisect_step(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new,
+                           OcState* st, float min_rho, float step, float damp, double volratio, double vol_tol,
+                           const Reduce& R, unsigned nblocks);
 
 }  // namespace otm

@@ -113,16 +113,54 @@ __device__ inline float oc_rule(float s, float rho, float g, float min_rho, floa
   return v;
 }
 
+__global__ void k_oc_zero(unsigned int* bits) { *bits = 0u; }
+
+__global__ void k_oc_init(const unsigned int* __restrict__ bits, OcState* st) {
+  st->lo = 0.f;
+  st->hi = fabsf(__uint_as_float(*bits));
+  st->g = 0.f;
+  st->done = 0;
+  st->steps = 0;
+  st->vol = 0.0;
+}
+
+// One bisection midpoint: rho_new from the update rule and partial volume
+// sums. Skipped entirely once an earlier midpoint met the tolerance.
 __global__ void __launch_bounds__(kBlock) k_oc_apply_sum(long long n, const float* __restrict__ sens,
                                                          const float* __restrict__ rho, float* __restrict__ rho_new,
-                                                         float g, float min_rho, float step, float damp, Reduce R) {
+                                                         const OcState* __restrict__ st, float min_rho, float step,
+                                                         float damp, Reduce R) {
+  if (st->done) return;
+  const float g = 0.5f * (st->hi + st->lo);
   double acc[1] = {0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float v = oc_rule(sens[i], rho[i], g, min_rho, step, damp);
     rho_new[i] = v;
     acc[0] += (double)v;
   }
-  reduce_finish<1>(acc, R);
+  reduce_block<1>(acc, R);
+}
+
+// Bracket update after a midpoint (OCOptimizer.cu:50-61).
+__global__ void __launch_bounds__(kBlock) k_oc_final(OcState* st, Reduce R, int nblocks, double inv_n,
+                                                     double volratio, double tol) {
+  if (st->done) return;
+  __shared__ double sh[kWaves];
+  double acc[1] = {0.0};
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc[0] += R.partials[b];
+  block_sum<1>(acc, sh);
+  if (threadIdx.x != 0) return;
+  const float g = 0.5f * (st->hi + st->lo);
+  const double vol = acc[0] * inv_n;
+  st->g = g;
+  st->vol = vol;
+  st->steps += 1;
+  if (vol < volratio - tol)
+    st->hi = g;
+  else if (vol > volratio + tol)
+    st->lo = g;
+  else
+    st->done = 1;
 }
 
 void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W) {
@@ -147,10 +185,18 @@ void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out
   if (nb > 2048) nb = 2048;
   hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kBlock), 0, s, n, a, out_bits);
 }
-void launch_oc_apply_sum(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new, float g,
-                         float min_rho, float step, float damp, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_oc_apply_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, rho_new, g, min_rho, step,
+void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st) {
+  hipLaunchKernelGGL(k_oc_zero, dim3(1), dim3(1), 0, s, bits);
+  launch_maxabs(s, n, sens, bits);
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, s, bits, st);
+}
+void launch_oc_bisect_step(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new,
+                           OcState* st, float min_rho, float step, float damp, double volratio, double vol_tol,
+                           const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_oc_apply_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, rho_new, st, min_rho, step,
                      damp, R);
+  hipLaunchKernelGGL(k_oc_final, dim3(1), dim3(kBlock), 0, s, st, R, (int)nblocks, 1.0 / (double)n, volratio,
+                     vol_tol);
 }
 
 }  // namespace otm

@@ -20,7 +20,7 @@ __global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __re
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i] * (double)b[k * nn + i];
   }
-  reduce_finish<OTM_NRHS>(acc, R);
+  reduce_block<OTM_NRHS>(acc, R);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __restrict__ a, Reduce R) {
@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __re
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i];
   }
-  reduce_finish<OTM_NRHS>(acc, R);
+  reduce_block<OTM_NRHS>(acc, R);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, const PcgScalars* __restrict__ P) {
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restri
       acc[k] += (double)rv * (double)rv;
     }
   }
-  reduce_finish<OTM_NRHS>(acc, R);
+  reduce_block<OTM_NRHS>(acc, R);
 }
 
 // p = z + beta p
@@ -94,6 +94,21 @@ __global__ void __launch_bounds__(kBlock) k_copy_to_pitched(Geom g, const float*
   dst[x + (long long)pitch * r] = src[t];
 }
 
+// Stage 2 of every reduction: sums the per-block partials in block order and
+// applies the finaliser (one block).
+__global__ void __launch_bounds__(kBlock) k_reduce_final(Reduce R, int nv, int nblocks) {
+  __shared__ double sh[kMaxReduce * kWaves];
+  double acc[kMaxReduce] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+    for (int i = 0; i < nv; i++) acc[i] += R.partials[(size_t)b * nv + i];
+  block_sum<kMaxReduce>(acc, sh);
+  if (threadIdx.x == 0) finalize(acc, nv, R);
+}
+
+void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblocks) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s, R, nv, (int)nblocks);
+}
+
 void launch_fill(hipStream_t s, float* a, long long n, float v) {
   unsigned nb = nblk(n);
   if (nb > 4096) nb = 4096;
@@ -102,9 +117,11 @@ void launch_fill(hipStream_t s, float* a, long long n, float v) {
 }
 void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_dot3, dim3(nblocks), dim3(kBlock), 0, s, nn, a, b, R);
+  launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_sum3, dim3(nblocks), dim3(kBlock), 0, s, nn, a, R);
+  launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P) {
   unsigned nb = nblk(nn);
@@ -114,6 +131,7 @@ void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P
 void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,
                    const PcgScalars* P, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_pcg_xr, dim3(nblocks), dim3(kBlock), 0, s, nn, x, r, p, q, P, R);
+  launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P) {
   unsigned nb = nblk(nn);

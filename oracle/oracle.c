@@ -210,7 +210,8 @@ int orc_cg(int nx, int ny, int nz, const double* A, const double* b, double* x,
   long n = (long)nx * ny * nz;
   double* r = (double*)malloc(sizeof(double) * n);
   double* z = (double*)malloc(sizeof(double) * n);
-  double* p = (double*)malloc(sizeof(double) * n);
+  /* p starts at zero: the first update p = z + 0 * p must not see garbage */
+  double* p = (double*)calloc((size_t)n, sizeof(double));
   double* q = (double*)malloc(sizeof(double) * n);
   const double* diag = A + 13 * n;
   double bn = sqrt(dot(n, b, b));
