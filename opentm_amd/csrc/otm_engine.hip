@@ -197,6 +197,7 @@ struct otm_ctx {
   float* kappa = nullptr;  // per element, with the 1e-9 floor
   float* x = nullptr;      // chi (warm start)
   float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  float* z2 = nullptr;  // second fine correction buffer (fine brick smoother)
   // coarsest dense solve
   double *cB = nullptr, *cX = nullptr, *cBinv = nullptr;
   // reductions
@@ -321,9 +322,9 @@ static void ctx_free(otm_ctx* c) {
     dfree(L.res);
     dfree(L.st);
     dfree(L.E);
-    dfree(L.u2);
+    if (L.own_uf) dfree(L.u2);
   }
-  dfree(c->kappa); dfree(c->x); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q);
+  dfree(c->kappa); dfree(c->x); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
   dfree(c->cB); dfree(c->cX); dfree(c->cBinv);
   dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
@@ -377,6 +378,11 @@ static void build_hierarchy(otm_ctx* c) {
       L.u = c->z;
       L.f = c->r;
       L.own_uf = false;
+      if (fine_brick_x(L.g) && cfg.pre_sweeps == cfg.post_sweeps) {
+        L.brick = true;
+        c->z2 = dalloc<float>(3 * n);
+        L.u2 = c->z2;
+      }
     } else {
       L.u = dalloc<float>(3 * n);
       L.f = dalloc<float>(3 * n);
@@ -452,7 +458,11 @@ static void vcycle(otm_ctx* c, size_t l) {
     launch_coarsest_solve(c->s, L.g, c->cBinv, L.f, L.u);
     return;
   }
-  if (L.brick) {
+  if (L.brick && l == 0) {
+    // kappa 4 * 33*9*9/(32*8*8) + f 12 + u 12 bytes per node, zero start
+    ProfScope ps(c, OTM_PROF_FINE_GS, 29.2 * nn);
+    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, nullptr, L.u, c->cfg.pre_sweeps, true, true);
+  } else if (L.brick) {
     // pre-smoothing from a zero guess straight into u
     ProfScope ps(c, OTM_PROF_COARSE_GS, 132.0 * nn);  // stencil 108 + f 12 + u 12 bytes per node
     launch_st_smooth_brick(c->s, L.g, L.st, L.f, nullptr, L.u, c->cfg.pre_sweeps, true, true);
@@ -480,7 +490,13 @@ static void vcycle(otm_ctx* c, size_t l) {
     ProfScope ps(c, OTM_PROF_TRANSFER, 24.0 * nn + 12.0 * (double)C.g.nn);
     launch_prolong_add(c->s, L.g, C.g, C.u, L.u);
   }
-  if (L.brick) {
+  if (L.brick && l == 0) {
+    // plus the u halo read, 12 * 34*10*10/(32*8*8)
+    ProfScope ps(c, OTM_PROF_FINE_GS, 49.1 * nn);
+    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, c->cfg.post_sweeps, false, false);
+    std::swap(L.u, L.u2);
+    std::swap(c->z, c->z2);
+  } else if (L.brick) {
     // post-smoothing reads u (frozen halos) and writes u2; swap so u holds the result
     ProfScope ps(c, OTM_PROF_COARSE_GS, 155.0 * nn);  // plus the u halo read, 12 * 10^3/8^3
     launch_st_smooth_brick(c->s, L.g, L.st, L.f, L.u, L.u2, c->cfg.post_sweeps, false, false);
@@ -567,7 +583,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
 }
 
 static void set_bricks(otm_ctx* c, bool on) {
-  for (size_t l = 1; l < c->lv.size(); l++) c->lv[l].brick = on && c->lv[l].u2 != nullptr;
+  for (size_t l = 0; l + 1 < c->lv.size(); l++) c->lv[l].brick = on && c->lv[l].u2 != nullptr;
   c->bricks_on = on;
 }
 

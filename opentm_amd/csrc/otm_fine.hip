@@ -304,6 +304,122 @@ __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, 
   for (int c = 0; c < 6; c++) S6[c * g.nn + t] = S[c];
 }
 
+// Fine-level brick smoother: a workgroup owns BX x 8 x 8 nodes and stages
+// the kappa of every element touching them plus u of the brick and a
+// one-node halo in LDS, then runs `sweeps` x-row Gauss-Seidel sweeps (4
+// colours) with workgroup barriers. Halo values stay frozen at their entry
+// values and the result goes to a separate buffer (block Gauss-Seidel across
+// bricks, exact colour order inside); the post-smoother runs the colours in
+// reverse. Each thread updates two nodes of the active colour per step.
+template <int BX>
+__global__ void __launch_bounds__(BX * 8) k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                               const float* __restrict__ f,
+                                                               const float* __restrict__ uin, float* __restrict__ uout,
+                                                               int sweeps, int forward, int zero_in) {
+  constexpr int BY = 8, BZ = 8;
+  constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
+  constexpr int KX = BX + 1, KY = BY + 1, KZ = BZ + 1;
+  constexpr int NT = BX * 8;
+  __shared__ float su[OTM_NRHS][HZ * HY * HX];
+  __shared__ float sk[KZ * KY * KX];
+  const int bx = g.nx / BX, by = g.ny / BY;
+  const int b = blockIdx.x;
+  const int ox = (b % bx) * BX, oy = ((b / bx) % by) * BY, oz = (b / (bx * by)) * BZ;
+  const int t = threadIdx.x;
+  for (int i = t; i < KX * KY * KZ; i += NT) {
+    const int ix = i % KX, iy = (i / KX) % KY, iz = i / (KX * KY);
+    const int X = wrapm(ox - 1 + ix, g.nx), Y = wrapm(oy - 1 + iy, g.ny), Z = wrapm(oz - 1 + iz, g.nz);
+    sk[i] = kappa[X + (long long)g.nx * (Y + (long long)g.ny * Z)];
+  }
+  for (int i = t; i < HX * HY * HZ; i += NT) {
+    if (zero_in) {
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
+    } else {
+      const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
+      const int X = wrapm(ox - 1 + hx, g.nx), Y = wrapm(oy - 1 + hy, g.ny), Z = wrapm(oz - 1 + hz, g.nz);
+      const long long w = X + (long long)g.nx * (Y + (long long)g.ny * Z);
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = uin[k * g.nn + w];
+    }
+  }
+  // node ownership: x = t % BX, and for colour (cy, cz) the rows
+  // y = cy + 2*(j%4), z = cz + 2*(j/4) and z + 4, j = t / BX in [0, 8)
+  const int lx = t % BX, j = t / BX;
+  const int ry = 2 * (j & 3), rz = 2 * (j >> 2);
+  float fv[4][2][OTM_NRHS];
+#pragma unroll
+  for (int col = 0; col < 4; col++)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
+      const long long v = (ox + lx) + (long long)g.nx * ((oy + ly) + (long long)g.ny * (oz + lz));
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) fv[col][h][k] = f[k * g.nn + v];
+    }
+  __syncthreads();
+  const float c = fc.c;
+  for (int s = 0; s < sweeps; s++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int col = forward ? i : 3 - i;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
+        // kappa of the 8 elements around the node: element (lx-1+ax, ...) at sk(lx+ax, ly+ay, lz+az)
+        float ke[8];
+        float ks = 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          ke[e] = sk[(lx + (e & 1)) + KX * ((ly + ((e >> 1) & 1)) + KY * (lz + (e >> 2)))];
+          ks += ke[e];
+        }
+        const float inv = 1.f / (4.f * c * ks);
+        const int c0 = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) {
+          const float* U = su[k];
+          float acc = 0.f;
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            const int sx = (e & 1) ? 1 : -1, sy = ((e >> 1) & 1) ? 1 : -1, sz = (e >> 2) ? 1 : -1;
+            const float T = U[c0 + sx + HX * sy] + U[c0 + sx + HX * HY * sz] + U[c0 + HX * sy + HX * HY * sz] +
+                            U[c0 + sx + HX * sy + HX * HY * sz];
+            acc += ke[e] * T;
+          }
+          su[k][c0] = (fv[col][h][k] + c * acc) * inv;
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = t; i < BX * BY * BZ; i += NT) {
+    const int lxx = i % BX, lyy = (i / BX) % BY, lzz = i / (BX * BY);
+    const long long v = (ox + lxx) + (long long)g.nx * ((oy + lyy) + (long long)g.ny * (oz + lzz));
+    const int c0 = (lxx + 1) + HX * ((lyy + 1) + HY * (lzz + 1));
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) uout[k * g.nn + v] = su[k][c0];
+  }
+}
+
+int fine_brick_x(const Geom& g) {
+  if (g.ny % 8 || g.nz % 8) return 0;
+  if (g.nx % 32 == 0) return 32;
+  if (g.nx % 16 == 0) return 16;
+  return 0;
+}
+
+void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
+                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in) {
+  const int bxw = fine_brick_x(g);
+  const unsigned nb = (unsigned)((g.nx / bxw) * (g.ny / 8) * (g.nz / 8));
+  if (bxw == 32)
+    hipLaunchKernelGGL(k_fine_smooth_brick<32>, dim3(nb), dim3(256), 0, s, g, fc, kappa, f, uin, uout, sweeps,
+                       forward ? 1 : 0, zero_in ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_fine_smooth_brick<16>, dim3(nb), dim3(128), 0, s, g, fc, kappa, f, uin, uout, sweeps,
+                       forward ? 1 : 0, zero_in ? 1 : 0);
+}
+
 // ---- launchers ---------------------------------------------------------
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 

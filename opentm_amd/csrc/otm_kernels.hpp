@@ -19,6 +19,11 @@ void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, con
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
                              float* sens, int pitch);
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6);
+// Fine brick smoother (BX x 8 x 8 bricks, frozen halo, result in uout);
+// fine_brick_x returns BX (32 or 16) or 0 when the cell does not tile.
+int fine_brick_x(const Geom& g);
+void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
+                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in);
 
 // ---- coarse levels (stored 27-point stencils, otm_coarse.hip) ----
 void launch_st_gs_color(hipStream_t s, const Geom& g, const float* st, const float* f, float* u, int color);
