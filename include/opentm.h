@@ -61,6 +61,7 @@ typedef struct otm_config {
   int min_coarse;           /* stop coarsening once a level edge would fall below this */
   int device;               /* HIP device ordinal */
   int verbose;              /* print per-solve statistics to stderr */
+  int coarse_sweeps;        /* sweeps on levels >= 1 (0: same as the fine level) */
 } otm_config;
 
 /* cfg::HomoConfig::init (cmdline.cpp:4-35) defaults where they apply;
@@ -188,7 +189,9 @@ enum {
   OTM_PROF_SETUP = 9,        /* Galerkin assembly and coarsest factorisation */
   OTM_PROF_NCLASSES = 10
 };
-/* enable != 0 starts timing (and clears the totals); 0 stops it. */
+/* enable is a bit mask of classes to time (bit k = class k; 0 stops
+ * timing); enabling clears the totals. Timed launches carry one event
+ * pair each, so time only the classes of interest. */
 OTM_API int otm_profile_enable(otm_ctx* ctx, int enable);
 /* Launch count, summed kernel time (ms) and algorithmic bytes moved by the
  * launches of one class since timing was enabled. */

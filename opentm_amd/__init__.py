@@ -112,6 +112,7 @@ class HomoConfig:
         self.pre_sweeps = 2
         self.post_sweeps = 2
         self.min_coarse = 4
+        self.coarse_sweeps = 0
         self.device = 0
         self.verbose = 0
 
@@ -126,6 +127,7 @@ class HomoConfig:
         c.pre_sweeps = self.pre_sweeps
         c.post_sweeps = self.post_sweeps
         c.min_coarse = self.min_coarse
+        c.coarse_sweeps = self.coarse_sweeps
         c.device = self.device
         c.verbose = self.verbose
         return c
@@ -296,8 +298,14 @@ class Homogenization_H:
         _capi.check(_capi.load().otm_synchronize(self._ctx), "otm_synchronize")
 
     # -- live kernel timing --------------------------------------------
-    def profile_enable(self, on=True):
-        _capi.check(_capi.load().otm_profile_enable(self._ctx, 1 if on else 0), "otm_profile_enable")
+    def profile_enable(self, on=True, classes=None):
+        """Time kernel classes with HIP events (all classes by default)."""
+        mask = 0
+        if on:
+            names = _capi.PROF_CLASSES if classes is None else classes
+            for nm in names:
+                mask |= 1 << _capi.PROF_CLASSES.index(nm)
+        _capi.check(_capi.load().otm_profile_enable(self._ctx, mask), "otm_profile_enable")
 
     def profile_read(self):
         """{class: (launches, total_ms, algorithmic_bytes)} since enabling."""

@@ -43,6 +43,7 @@ class OtmConfig(ctypes.Structure):
         ("min_coarse", _c.c_int),
         ("device", _c.c_int),
         ("verbose", _c.c_int),
+        ("coarse_sweeps", _c.c_int),
     ]
 
 

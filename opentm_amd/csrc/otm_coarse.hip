@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __r
                                                         int cz) {
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
   const long long cnt = (long long)hx * hy * hz;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= cnt) return;
   const int x = 2 * (int)(t % hx) + cx;
   const long long r = t / hx;
@@ -146,21 +146,21 @@ __global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __r
 __global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, const float* __restrict__ st,
                                                         const float* __restrict__ f, const float* __restrict__ u,
                                                         float* __restrict__ res) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   st_resid_node(g, st, f, u, res, t);
 }
 
 __global__ void __launch_bounds__(kBlock) k_restrict(Geom fg, Geom cg, const float* __restrict__ res,
                                                      float* __restrict__ fc) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= cg.nn) return;
   restrict_node(fg, cg, res, fc, t);
 }
 
 __global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const float* __restrict__ uc,
                                                         float* __restrict__ uf) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= fg.nn) return;
   prolong_node(fg, cg, uc, uf, t);
 }
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const 
 // Level-1 element matrices from kappa: E(E) = sum_c kappa_c G[c].
 __global__ void __launch_bounds__(kBlock) k_galerkin_first(Geom fg, Geom cg, GalerkinConsts G,
                                                            const float* __restrict__ kappa, float* __restrict__ E) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= cg.nn) return;
   const int X = (int)(t % cg.nx);
   const long long r = t / cg.nx;
@@ -194,7 +194,7 @@ __device__ constexpr float w1d(int p, int M) {
 // Deeper levels: Eout(E) = sum_c Q_c^T Ein(child c) Q_c.
 __global__ void __launch_bounds__(kBlock) k_galerkin_next(Geom fg, Geom cg, const float* __restrict__ Ein,
                                                           float* __restrict__ Eout) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= cg.nn) return;
   const int X = (int)(t % cg.nx);
   const long long r = t / cg.nx;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) k_galerkin_next(Geom fg, Geom cg, cons
 // of the off-diagonal couplings so that constants stay in the null space.
 __global__ void __launch_bounds__(kBlock) k_stencil_from_elements(Geom g, const float* __restrict__ E,
                                                                   float* __restrict__ st) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -277,31 +277,32 @@ __global__ void __launch_bounds__(kBlock) k_stencil_from_elements(Geom g, const 
 }
 
 // Coarsest level: dense FP64 matrix from the stencil, symmetrised, zero row
-// sums, plus (tr/n^2) * 1 1^T to remove the constant null space, Cholesky
-// factor L, X = L^{-1}, Binv = X^T X. One 1024-thread workgroup.
+// sums, plus (tr/n^2) * 1 1^T to remove the constant null space; Cholesky
+// factor L (right-looking), X = L^{-1} (row by row), Binv = X^T X. One
+// 1024-thread workgroup; the n x n work matrices live in LDS when n <= 128
+// (W points into dynamic shared memory) and in global memory otherwise.
 constexpr int kFactorThreads = 1024;
-__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, const float* __restrict__ st, double* B,
-                                                                    double* Xm, double* Binv) {
+__device__ inline void coarsest_factor_body(const Geom& g, const float* __restrict__ st, double* B, double* Xm,
+                                            double* Binv, double* red, double* alpha_sh) {
   const int n = (int)g.nn;
   const int tid = threadIdx.x, bs = blockDim.x;
-  __shared__ double red[kFactorThreads / 64];
-  __shared__ double alpha_sh;
-  for (long long i = tid; i < (long long)n * n; i += bs) B[i] = 0.0;
+  for (int i = tid; i < n * n; i += bs) B[i] = 0.0;
   __syncthreads();
   for (int v = tid; v < n; v += bs) {
     const int x = v % g.nx, y = (v / g.nx) % g.ny, z = v / (g.nx * g.ny);
     for (int o = 0; o < 27; o++) {
-      const int w = (int)lin(g, wrapm(x + o % 3 - 1, g.nx), wrapm(y + (o / 3) % 3 - 1, g.ny), wrapm(z + o / 9 - 1, g.nz));
-      B[(long long)v * n + w] += (double)st[o * g.nn + v];
+      const int w = wrapm(x + o % 3 - 1, g.nx) + g.nx * (wrapm(y + (o / 3) % 3 - 1, g.ny) +
+                                                         g.ny * wrapm(z + o / 9 - 1, g.nz));
+      B[v * n + w] += (double)st[o * n + v];
     }
   }
   __syncthreads();
-  for (long long i = tid; i < (long long)n * n; i += bs) {
-    const int a = (int)(i / n), b = (int)(i % n);
+  for (int i = tid; i < n * n; i += bs) {
+    const int a = i / n, b = i - a * n;
     if (a < b) {
-      const double m = 0.5 * (B[(long long)a * n + b] + B[(long long)b * n + a]);
-      B[(long long)a * n + b] = m;
-      B[(long long)b * n + a] = m;
+      const double m = 0.5 * (B[a * n + b] + B[b * n + a]);
+      B[a * n + b] = m;
+      B[b * n + a] = m;
     }
   }
   __syncthreads();
@@ -309,8 +310,8 @@ __global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, cons
   for (int a = tid; a < n; a += bs) {
     double s = 0.0;
     for (int b = 0; b < n; b++)
-      if (b != a) s += B[(long long)a * n + b];
-    B[(long long)a * n + a] = -s;
+      if (b != a) s += B[a * n + b];
+    B[a * n + a] = -s;
     tr += -s;
   }
   tr = wave_sum(tr);
@@ -319,43 +320,59 @@ __global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, cons
   if (tid == 0) {
     double s = 0.0;
     for (int w = 0; w < bs / 64; w++) s += red[w];
-    alpha_sh = s / ((double)n * (double)n);
+    *alpha_sh = s / ((double)n * (double)n);
   }
   __syncthreads();
-  const double alpha = alpha_sh;
-  for (long long i = tid; i < (long long)n * n; i += bs) B[i] += alpha;
+  const double alpha = *alpha_sh;
+  for (int i = tid; i < n * n; i += bs) B[i] += alpha;
   __syncthreads();
   // right-looking Cholesky, lower triangle in place
   for (int k = 0; k < n; k++) {
-    if (tid == 0) B[(long long)k * n + k] = sqrt(B[(long long)k * n + k]);
+    if (tid == 0) B[k * n + k] = sqrt(B[k * n + k]);
     __syncthreads();
-    const double dk = B[(long long)k * n + k];
-    for (int i = k + 1 + tid; i < n; i += bs) B[(long long)i * n + k] /= dk;
+    const double dk = B[k * n + k];
+    for (int i = k + 1 + tid; i < n; i += bs) B[i * n + k] /= dk;
     __syncthreads();
     const int m = n - k - 1;
-    for (long long q = tid; q < (long long)m * m; q += bs) {
-      const int i = k + 1 + (int)(q / m), j = k + 1 + (int)(q % m);
-      if (j <= i) B[(long long)i * n + j] -= B[(long long)i * n + k] * B[(long long)j * n + k];
+    for (int q = tid; q < m * m; q += bs) {
+      const int i = k + 1 + q / m, j = k + 1 + q % m;
+      if (j <= i) B[i * n + j] -= B[i * n + k] * B[j * n + k];
     }
     __syncthreads();
   }
-  // X = L^{-1}, one column per thread
-  for (int j = tid; j < n; j += bs) {
-    for (int i = 0; i < j; i++) Xm[(long long)i * n + j] = 0.0;
-    for (int i = j; i < n; i++) {
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int m2 = j; m2 < i; m2++) s -= B[(long long)i * n + m2] * Xm[(long long)m2 * n + j];
-      Xm[(long long)i * n + j] = s / B[(long long)i * n + i];
+  // X = L^{-1}: row k from rows < k, columns in parallel
+  for (int k = 0; k < n; k++) {
+    for (int j = tid; j < n; j += bs) {
+      double s = (j == k) ? 1.0 : 0.0;
+      if (j < k)
+        for (int m2 = j; m2 < k; m2++) s -= B[k * n + m2] * Xm[m2 * n + j];
+      Xm[k * n + j] = j <= k ? s / B[k * n + k] : 0.0;
     }
+    __syncthreads();
   }
-  __syncthreads();
-  for (long long q = tid; q < (long long)n * n; q += bs) {
-    const int i = (int)(q / n), j = (int)(q % n);
+  for (int q = tid; q < n * n; q += bs) {
+    const int i = q / n, j = q - i * n;
     const int m0 = i > j ? i : j;
     double s = 0.0;
-    for (int m2 = m0; m2 < n; m2++) s += Xm[(long long)m2 * n + i] * Xm[(long long)m2 * n + j];
+    for (int m2 = m0; m2 < n; m2++) s += Xm[m2 * n + i] * Xm[m2 * n + j];
     Binv[q] = s;
   }
+}
+
+__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor_lds(Geom g, const float* __restrict__ st,
+                                                                        double* Binv) {
+  extern __shared__ double wsh[];
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  const int n = (int)g.nn;
+  coarsest_factor_body(g, st, wsh, wsh + n * n, Binv, red, &alpha_sh);
+}
+
+__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, const float* __restrict__ st, double* B,
+                                                                    double* Xm, double* Binv) {
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  coarsest_factor_body(g, st, B, Xm, Binv, red, &alpha_sh);
 }
 
 // u = Binv f per right-hand side, FP64 accumulation.
@@ -390,7 +407,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_st_smooth_brick(Geom g, const
                                                                    int forward, int zero_in) {
   __shared__ float su[OTM_NRHS][kHalo * kHalo * kHalo];
   const int bx = g.nx / kBrick, by = g.ny / kBrick;
-  const int b = blockIdx.x;
+  const int b = xcd_block();
   const int ox = (b % bx) * kBrick, oy = ((b / bx) % by) * kBrick, oz = (b / (bx * by)) * kBrick;
   const int t = threadIdx.x;
   const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
@@ -468,7 +485,23 @@ void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, 
   hipLaunchKernelGGL(k_stencil_from_elements, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, E, st);
 }
 void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv) {
-  hipLaunchKernelGGL(k_coarsest_factor, dim3(1), dim3(kFactorThreads), 0, s, g, st, B, X, Binv);
+  const int n = (int)g.nn;
+  const size_t lds = (size_t)2 * n * n * sizeof(double);
+  // one thread per row: a single wavefront for n <= 64 keeps the ~4n
+  // barriers of the factorisation cheap
+  int nt = ((n + 63) / 64) * 64;
+  if (nt > kFactorThreads) nt = kFactorThreads;
+  if (n <= 90) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)k_coarsest_factor_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(2 * 90 * 90 * sizeof(double)));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(k_coarsest_factor_lds, dim3(1), dim3(nt), lds, s, g, st, Binv);
+  } else {
+    hipLaunchKernelGGL(k_coarsest_factor, dim3(1), dim3(nt), 0, s, g, st, B, X, Binv);
+  }
 }
 bool brick_ok(const Geom& g) { return g.nx % kBrick == 0 && g.ny % kBrick == 0 && g.nz % kBrick == 0; }
 void launch_st_smooth_brick(hipStream_t s, const Geom& g, const float* st, const float* f, const float* uin,

@@ -80,6 +80,35 @@ constexpr int kBlock = 256;       // threads per block for streaming kernels
 constexpr int kMaxReduce = 9;     // values per reduction
 constexpr int kWaves = kBlock / 64;
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, and
+// each XCD has its own L2. The logical block index below gives every XCD a
+// contiguous range of blocks, so neighbouring tiles -- and the halo cache
+// lines they share -- stay on one L2. The mapping is a bijection for any
+// grid size.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int x = b % kXcds, local = b / kXcds;
+  const int start = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return start + local;
+}
+
+// Global thread index of one-item-per-thread kernels, XCD-ordered.
+__device__ inline long long gtid() { return (long long)xcd_block() * blockDim.x + threadIdx.x; }
+
+// Contiguous item range [b, e) of this block for fixed-grid streaming loops.
+struct BlockRange {
+  long long b, e;
+};
+__device__ inline BlockRange block_range(long long n) {
+  const long long per = ((n + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
+  long long s = (long long)xcd_block() * per, e = s + per;
+  if (s > n) s = n;
+  if (e > n) e = n;
+  return {s, e};
+}
+
 // Wave64 sum through cross-lane shuffles.
 __device__ inline double wave_sum(double v) {
 #pragma unroll

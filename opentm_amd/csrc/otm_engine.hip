@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -166,7 +167,7 @@ Taps make_taps(int kind, double r, int periodic) {
 
 unsigned red_blocks_for(long long n) {
   long long b = (n + kBlock - 1) / kBlock;
-  if (b > 2048) b = 2048;
+  if (b > 1024) b = 1024;
   if (b < 1) b = 1;
   return (unsigned)b;
 }
@@ -215,6 +216,7 @@ struct otm_ctx {
   int taps_periodic = -1;
   // live kernel timing (otm_profile_enable)
   bool prof_on = false;
+  int prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
   struct Pending {
     int cls;
@@ -283,7 +285,7 @@ struct ProfScope {
   double bytes;
   hipEvent_t a = nullptr;
   ProfScope(otm_ctx* c_, int cls_, double bytes_) : c(c_), cls(cls_), bytes(bytes_) {
-    if (c->prof_on) {
+    if (c->prof_on && ((c->prof_mask >> cls) & 1)) {
       a = c->take_event();
       hip_check(hipEventRecord(a, c->s), "hipEventRecord");
     }
@@ -453,25 +455,26 @@ static void smooth(otm_ctx* c, size_t l, int sweeps, bool forward) {
 static void vcycle(otm_ctx* c, size_t l) {
   Level& L = c->lv[l];
   const double nn = (double)L.g.nn;
+  const int nsw = (l > 0 && c->cfg.coarse_sweeps > 0) ? c->cfg.coarse_sweeps : c->cfg.pre_sweeps;
   if (l + 1 == c->lv.size()) {
     ProfScope ps(c, OTM_PROF_COARSEST, 8.0 * nn * nn + 24.0 * nn);
     launch_coarsest_solve(c->s, L.g, c->cBinv, L.f, L.u);
     return;
   }
   if (L.brick && l == 0) {
-    // kappa 4 * 33*9*9/(32*8*8) + f 12 + u 12 bytes per node, zero start
-    ProfScope ps(c, OTM_PROF_FINE_GS, 29.2 * nn);
-    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, nullptr, L.u, c->cfg.pre_sweeps, true, true);
+    // compulsory bytes per node: kappa 4 + f 12 + u written 12 (zero start)
+    ProfScope ps(c, OTM_PROF_FINE_GS, 28.0 * nn);
+    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, nullptr, L.u, nsw, true, true);
   } else if (L.brick) {
     // pre-smoothing from a zero guess straight into u
     ProfScope ps(c, OTM_PROF_COARSE_GS, 132.0 * nn);  // stencil 108 + f 12 + u 12 bytes per node
-    launch_st_smooth_brick(c->s, L.g, L.st, L.f, nullptr, L.u, c->cfg.pre_sweeps, true, true);
+    launch_st_smooth_brick(c->s, L.g, L.st, L.f, nullptr, L.u, nsw, true, true);
   } else {
     {
       ProfScope ps(c, OTM_PROF_VECTOR, 12.0 * nn);
       launch_fill(c->s, L.u, 3 * L.g.nn, 0.f);
     }
-    smooth(c, l, c->cfg.pre_sweeps, true);
+    smooth(c, l, nsw, true);
   }
   if (l == 0) {
     ProfScope ps(c, OTM_PROF_FINE_RESID, 40.0 * nn);
@@ -491,18 +494,18 @@ static void vcycle(otm_ctx* c, size_t l) {
     launch_prolong_add(c->s, L.g, C.g, C.u, L.u);
   }
   if (L.brick && l == 0) {
-    // plus the u halo read, 12 * 34*10*10/(32*8*8)
-    ProfScope ps(c, OTM_PROF_FINE_GS, 49.1 * nn);
-    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, c->cfg.post_sweeps, false, false);
+    // compulsory bytes per node: kappa 4 + u read 12 + f 12 + u written 12
+    ProfScope ps(c, OTM_PROF_FINE_GS, 40.0 * nn);
+    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, nsw, false, false);
     std::swap(L.u, L.u2);
     std::swap(c->z, c->z2);
   } else if (L.brick) {
     // post-smoothing reads u (frozen halos) and writes u2; swap so u holds the result
     ProfScope ps(c, OTM_PROF_COARSE_GS, 155.0 * nn);  // plus the u halo read, 12 * 10^3/8^3
-    launch_st_smooth_brick(c->s, L.g, L.st, L.f, L.u, L.u2, c->cfg.post_sweeps, false, false);
+    launch_st_smooth_brick(c->s, L.g, L.st, L.f, L.u, L.u2, nsw, false, false);
     std::swap(L.u, L.u2);
   } else {
-    smooth(c, l, c->cfg.post_sweeps, false);
+    smooth(c, l, nsw, false);
   }
 }
 
@@ -676,6 +679,7 @@ OTM_API void otm_config_init(otm_config* cfg) {
   cfg->min_coarse = 4;
   cfg->device = 0;
   cfg->verbose = 0;
+  cfg->coarse_sweeps = 0;
 }
 
 OTM_API const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -695,6 +699,7 @@ OTM_API int otm_create(const otm_config* cfg, otm_ctx** out) {
       fail(OTM_E_ARG, "need at least one smoothing sweep");
     if (cfg->pre_sweeps != cfg->post_sweeps)
       fail(OTM_E_ARG, "pre and post sweeps must match (symmetric preconditioner for CG)");
+    if (cfg->coarse_sweeps < 0) fail(OTM_E_ARG, "coarse_sweeps must be >= 0");
     if (!(cfg->fem_rel_thres > 0)) fail(OTM_E_ARG, "fem_rel_thres must be positive");
     otm_ctx* c = new otm_ctx();
     c->cfg = *cfg;
@@ -1154,12 +1159,22 @@ static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volra
   R.rtol2 = 0;
   R.inv_nn = 0;
   R.mode = FIN_STORE;
-  // the whole bisection is queued without host round trips; midpoints after
-  // the one that meets the tolerance are skipped on the device
+  // midpoints are queued in chunks without host round trips; midpoints after
+  // the one that meets the tolerance are skipped on the device, and the
+  // host checks the state once per chunk
   launch_oc_begin(o->s, n, sens, o->maxbits, o->st);
-  for (int i = 0; i < o->max_steps; i++)
-    launch_oc_bisect_step(o->s, n, sens, rho, o->newrho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp,
-                          volratio, o->vol_tol, R, nb);
+  const int chunk = 24;
+  for (int done = 0; done < o->max_steps;) {
+    const int m = std::min(chunk, o->max_steps - done);
+    for (int i = 0; i < m; i++)
+      launch_oc_bisect_step(o->s, n, sens, rho, o->newrho, o->st, (float)o->min_rho, (float)o->step,
+                            (float)o->damp, volratio, o->vol_tol, R, nb);
+    done += m;
+    if (done >= o->max_steps) break;
+    hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
+    hip_check(hipStreamSynchronize(o->s), "sync");
+    if (o->st_host->done) break;
+  }
   hip_check(hipMemcpyAsync(rho, o->newrho, sizeof(float) * n, hipMemcpyDeviceToDevice, o->s), "copy rho");
   hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
   hip_check(hipStreamSynchronize(o->s), "sync");
@@ -1206,6 +1221,7 @@ OTM_API int otm_profile_enable(otm_ctx* ctx, int enable) {
     if (!ctx) fail(OTM_E_ARG, "null context");
     ctx->sync();
     ctx->prof_on = enable != 0;
+    ctx->prof_mask = enable;
     if (ctx->prof_on) {
       for (int k = 0; k < OTM_PROF_NCLASSES; k++) {
         ctx->prof_launches[k] = 0;

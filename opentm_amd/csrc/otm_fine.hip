@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_gs_color(Geom g, FineConsts fc,
                                                           const float* __restrict__ f, float* u, int cy, int cz) {
   const int hy = g.ny >> 1, hz = g.nz >> 1;
   const long long cnt = (long long)g.nx * hy * hz;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= cnt) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_gs_color(Geom g, FineConsts fc,
 __global__ void __launch_bounds__(kBlock) k_fine_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                           const float* __restrict__ f, const float* __restrict__ u,
                                                           float* __restrict__ res) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -165,8 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_apply_dot(Geom g, FineConsts fc
                                                            const float* __restrict__ p, float* __restrict__ q,
                                                            Reduce R) {
   double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
-       t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = block_range(g.nn).b + threadIdx.x, t_end = block_range(g.nn).e; t < t_end; t += blockDim.x) {
     const int x = (int)(t % g.nx);
     const long long r = t / g.nx;
     const int y = (int)(r % g.ny), z = (int)(r / g.ny);
@@ -193,8 +192,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConst
                                                                Reduce R) {
   double acc[3 * OTM_NRHS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const double c = (double)fc.c;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
-       t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = block_range(g.nn).b + threadIdx.x, t_end = block_range(g.nn).e; t < t_end; t += blockDim.x) {
     const int x = (int)(t % g.nx);
     const long long r = t / g.nx;
     const int y = (int)(r % g.ny), z = (int)(r / g.ny);
@@ -260,8 +258,7 @@ __device__ inline void element_S(const Geom& g, const FineConsts& fc, const floa
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                          const float* __restrict__ chi, Reduce R) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.nn;
-       t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = block_range(g.nn).b + threadIdx.x, t_end = block_range(g.nn).e; t < t_end; t += blockDim.x) {
     const int x = (int)(t % g.nx);
     const long long r = t / g.nx;
     const int y = (int)(r % g.ny), z = (int)(r / g.ny);
@@ -279,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, 
 __global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, const float* __restrict__ chi,
                                                              float w0, float w1, float w2, float w3, float w4, float w5,
                                                              float* __restrict__ sens, int pitch) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -293,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts 
 // S6[c*nn + e]: the six element energies, for callers that keep them.
 __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, const float* __restrict__ chi,
                                                          float* __restrict__ S6) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -323,7 +320,7 @@ __global__ void __launch_bounds__(BX * 8) k_fine_smooth_brick(Geom g, FineConsts
   __shared__ float su[OTM_NRHS][HZ * HY * HX];
   __shared__ float sk[KZ * KY * KX];
   const int bx = g.nx / BX, by = g.ny / BY;
-  const int b = blockIdx.x;
+  const int b = xcd_block();
   const int ox = (b % bx) * BX, oy = ((b / bx) % by) * BY, oz = (b / (bx * by)) * BZ;
   const int t = threadIdx.x;
   for (int i = t; i < KX * KY * KZ; i += NT) {

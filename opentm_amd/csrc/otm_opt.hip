@@ -13,7 +13,7 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / 
 
 __global__ void __launch_bounds__(kBlock) k_filter_fwd(Geom g, Taps T, const float* __restrict__ in,
                                                        float* __restrict__ out, float* __restrict__ W) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_fwd(Geom g, Taps T, const flo
 
 __global__ void __launch_bounds__(kBlock) k_filter_adj(Geom g, Taps T, const float* __restrict__ in,
                                                        const float* __restrict__ W, float* __restrict__ out) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_adj(Geom g, Taps T, const flo
 __global__ void __launch_bounds__(kBlock) k_simp_fwd(long long n, const float* __restrict__ rho, float k0, float k1,
                                                      float floor_, float* __restrict__ kappa,
                                                      float* __restrict__ dkappa) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= n) return;
   const float p = rho[t];
   const float d = k0 - k1;
@@ -70,14 +70,14 @@ __global__ void __launch_bounds__(kBlock) k_simp_fwd(long long n, const float* _
 
 __global__ void __launch_bounds__(kBlock) k_mul(long long n, const float* __restrict__ a, const float* __restrict__ b,
                                                 float* __restrict__ out) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= n) return;
   out[t] = a[t] * b[t];
 }
 
 __global__ void __launch_bounds__(kBlock) k_oc_filter_sens(Geom g, Taps T, const float* __restrict__ sens,
                                                            const float* __restrict__ rho, float* __restrict__ out) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
@@ -95,11 +95,17 @@ __global__ void __launch_bounds__(kBlock) k_oc_filter_sens(Geom g, Taps T, const
 
 __global__ void __launch_bounds__(kBlock) k_maxabs(long long n, const float* __restrict__ a, unsigned int* out_bits) {
   float m = 0.f;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x)
     m = fmaxf(m, fabsf(a[i]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(m));
+  __shared__ float wm[kWaves];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; w++) m = fmaxf(m, wm[w]);
+    atomicMax(out_bits, __float_as_uint(m));
+  }
 }
 
 __device__ inline float oc_rule(float s, float rho, float g, float min_rho, float step, float damp) {
@@ -133,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_oc_apply_sum(long long n, const floa
   if (st->done) return;
   const float g = 0.5f * (st->hi + st->lo);
   double acc[1] = {0.0};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x) {
     const float v = oc_rule(sens[i], rho[i], g, min_rho, step, damp);
     rho_new[i] = v;
     acc[0] += (double)v;
@@ -182,7 +188,7 @@ void launch_oc_filter_sens(hipStream_t s, const Geom& g, const Taps& T, const fl
 }
 void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out_bits) {
   unsigned nb = nblk(n);
-  if (nb > 2048) nb = 2048;
+  if (nb > 512) nb = 512;
   hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kBlock), 0, s, n, a, out_bits);
 }
 void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st) {

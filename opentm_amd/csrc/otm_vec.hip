@@ -9,14 +9,14 @@ namespace otm {
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 __global__ void __launch_bounds__(kBlock) k_fill(float* a, long long n, float v) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x)
     a[i] = v;
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __restrict__ a,
                                                  const float* __restrict__ b, Reduce R) {
   double acc[OTM_NRHS] = {0, 0, 0};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i] * (double)b[k * nn + i];
   }
@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __re
 
 __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __restrict__ a, Reduce R) {
   double acc[OTM_NRHS] = {0, 0, 0};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i];
   }
@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __re
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, const PcgScalars* __restrict__ P) {
   const float m0 = (float)P->mean[0], m1 = (float)P->mean[1], m2 = (float)P->mean[2];
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
     a[i] -= m0;
     a[nn + i] -= m1;
     a[2 * nn + i] -= m2;
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restri
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) al[k] = (float)P->alpha[k];
   double acc[OTM_NRHS] = {0, 0, 0};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const long long j = k * nn + i;
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restric
   float be[OTM_NRHS];
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) be[k] = (float)P->beta[k];
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const long long j = k * nn + i;
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restric
 // dst[v] = src[x + pitch*(y + ny*z)] + add (pitched lexicographic input)
 __global__ void __launch_bounds__(kBlock) k_copy_pitched(Geom g, const float* __restrict__ src, int pitch,
                                                          float* __restrict__ dst, float add) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const long long x = t % g.nx, r = t / g.nx;
   dst[t] = src[x + (long long)pitch * r] + add;
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_pitched(Geom g, const float* __
 
 __global__ void __launch_bounds__(kBlock) k_copy_to_pitched(Geom g, const float* __restrict__ src,
                                                             float* __restrict__ dst, int pitch) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = gtid();
   if (t >= g.nn) return;
   const long long x = t % g.nx, r = t / g.nx;
   dst[x + (long long)pitch * r] = src[t];

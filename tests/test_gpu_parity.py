@@ -123,7 +123,7 @@ def test_vcycle_symmetric_positive(n):
 
 
 # ------------------------------------------------------------ homogenization
-@pytest.mark.parametrize("n,seed", [(16, 0), (32, 0), (32, 1), (32, 2)])
+@pytest.mark.parametrize("n,seed", [(16, 0), (32, 0), (32, 1), (32, 2), (40, 1), (48, 0)])
 def test_heat_matrix(orc, n, seed):
     say("heat matrix", n, seed)
     k = wl.simp(wl.plain_density(n, seed))
@@ -135,6 +135,24 @@ def test_heat_matrix(orc, n, seed):
     say("  its", its, "relres", rr, "err", rel(C, Co))
     assert rel(C, Co) < 1e-5
     assert max(rr) < 1e-5
+
+
+def test_non_cubic_cell(orc):
+    """reso = (32, 24, 16): element side 1/reso[0] as in the reference
+    (homogenization.cpp:47); kappa^H and sensitivities against the oracle."""
+    say("non-cubic", (32, 24, 16))
+    rng = np.random.default_rng(15)
+    k = rng.uniform(1e-3, 1.0, (16, 24, 32)).astype(np.float32)
+    cfg = otm.HomoConfig((32, 24, 16), (1.0, 1e-3))
+    hom = otm.Homogenization_H(cfg)
+    assert hom.levels()[0] == (32, 24, 16)
+    hom.update(k)
+    C = hom.heatMatrix()
+    Co, _, So, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12, want_S=True)
+    assert rel(C, Co) < 1e-5
+    S = hom.element_energy()
+    for c in range(6):
+        assert rel(S[c], So[c]) < 1e-4
 
 
 def test_sensitivity(orc):
