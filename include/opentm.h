@@ -174,6 +174,26 @@ OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volr
 OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double volratio, int* steps,
                         double* volume);
 
+/* homo::MMAOptimizer (mmaOptimizer.h:12-35): Svanberg's MMA in FP64 on the
+ * device (mmaOpt.cu:834-1019 with the interior-point subsolver
+ * mmaOpt.cu:429-830). m <= 4 constraints, n design variables. */
+typedef struct otm_mma otm_mma;
+/* MMAOptimizer(int m, int n, double a0, double a, double c, double d) */
+OTM_API int otm_mma_create(int m, int n, double a0, double a, double c, double d, int device, otm_mma** out);
+OTM_API int otm_mma_destroy(otm_mma* mma);
+/* MMAOptimizer::setBound(float xmin, float xmax) and the array overload */
+OTM_API int otm_mma_set_bounds(otm_mma* mma, double xmin, double xmax);
+OTM_API int otm_mma_set_bounds_array(otm_mma* mma, const float* xmin, const float* xmax, int where);
+/* Move limit of the subproblem (MMAOptimizer::update passes 1). */
+OTM_API int otm_mma_set_move(otm_mma* mma, double move);
+/* MMAOptimizer::update(int itn, float* x, float* dfdx, float* gval,
+ * float** dgdx) (mmaOptimizer.cu:61-77): x is updated in place; dfdx is the
+ * objective gradient, gval the m constraint values (host), dgdx m pointers
+ * to constraint gradients. x, dfdx and dgdx[i] are `where` memory.
+ * *newton receives the number of interior-point Newton steps. */
+OTM_API int otm_mma_update(otm_mma* mma, int itn, float* x, const float* dfdx, const double* gval,
+                           const float* const* dgdx, int where, int* newton);
+
 /* Live kernel timing with HIP events recorded on the context's stream
  * around every launch of a kernel class. Classes: */
 enum {

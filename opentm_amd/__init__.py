@@ -21,7 +21,7 @@ from . import _capi
 from ._capi import OTM_DEVICE, OTM_HOST, OtmError
 
 __all__ = [
-    "HomoConfig", "Homogenization_H", "OCOptimizer", "DeviceBuffer", "DensityParams",
+    "HomoConfig", "Homogenization_H", "OCOptimizer", "MMAOptimizer", "DeviceBuffer", "DensityParams",
     "conv_filter", "OtmError", "load_library",
 ]
 
@@ -386,6 +386,66 @@ class OCOptimizer:
         _capi.check(_capi.load().otm_oc_step(hom._ctx, self._h, float(filter_radius), float(volratio),
                                              ctypes.byref(st), ctypes.byref(vol)), "otm_oc_step")
         self.last_steps, self.last_volume = st.value, vol.value
+
+
+class MMAOptimizer:
+    """homo::MMAOptimizer(int m, int n, double a0, double a, double c,
+    double d) with setBound and update (mmaOptimizer.h:12-35)."""
+
+    def __init__(self, m, n, a0=1.0, a=0.0, c=1e3, d=1.0, device=0):
+        lib = _capi.load()
+        self.m, self.n = int(m), int(n)
+        h = ctypes.c_void_p()
+        _capi.check(lib.otm_mma_create(self.m, self.n, a0, a, c, d, device, ctypes.byref(h)), "otm_mma_create")
+        self._h = h
+        self.last_newton = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.load().otm_mma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setBound(self, xmin, xmax):
+        if np.isscalar(xmin) and np.isscalar(xmax):
+            _capi.check(_capi.load().otm_mma_set_bounds(self._h, float(xmin), float(xmax)), "otm_mma_set_bounds")
+            return
+        a = _f32(xmin, self.n)
+        b = _f32(xmax, self.n)
+        _capi.check(_capi.load().otm_mma_set_bounds_array(self._h, _ptr(a), _ptr(b), OTM_HOST),
+                    "otm_mma_set_bounds_array")
+
+    def set_move(self, move):
+        _capi.check(_capi.load().otm_mma_set_move(self._h, float(move)), "otm_mma_set_move")
+
+    def update(self, itn, x, dfdx, gval, dgdx):
+        """In place on x (float32 array or DeviceBuffer); dgdx is a sequence
+        of m arrays (all host or all device, like x)."""
+        px, wx, kx = _arg(x, self.n)
+        pf, wf, kf = _arg(dfdx, self.n)
+        if isinstance(x, np.ndarray) and (x.dtype != np.float32 or not x.flags.c_contiguous):
+            raise ValueError("x must be a contiguous float32 array (updated in place)")
+        keep = []
+        ptrs = (ctypes.c_void_p * self.m)()
+        for i in range(self.m):
+            p, w, k = _arg(dgdx[i], self.n)
+            if w != wx:
+                raise ValueError("all arrays must be host or all device")
+            ptrs[i] = p
+            keep.append(k)
+        if wf != wx:
+            raise ValueError("all arrays must be host or all device")
+        g = (ctypes.c_double * self.m)(*[float(v) for v in np.asarray(gval).reshape(-1)[:self.m]])
+        newton = ctypes.c_int()
+        _capi.check(_capi.load().otm_mma_update(self._h, int(itn), px, pf, g, ptrs, wx, ctypes.byref(newton)),
+                    "otm_mma_update")
+        self.last_newton = newton.value
+        return x
 
 
 def conv_filter(x, kind=0, radius=1.5, periodic=False, adjoint=False):

@@ -87,6 +87,12 @@ SIGNATURES = {
     "otm_oc_filter_sens": (_i, [_p, _p, _p, _i, _ip, _d, _i]),
     "otm_oc_update": (_i, [_p, _p, _p, _d, _i, _ip, _dp]),
     "otm_oc_step": (_i, [_p, _p, _d, _d, _ip, _dp]),
+    "otm_mma_create": (_i, [_i, _i, _d, _d, _d, _d, _i, _c.POINTER(_p)]),
+    "otm_mma_destroy": (_i, [_p]),
+    "otm_mma_set_bounds": (_i, [_p, _d, _d]),
+    "otm_mma_set_bounds_array": (_i, [_p, _p, _p, _i]),
+    "otm_mma_set_move": (_i, [_p, _d]),
+    "otm_mma_update": (_i, [_p, _i, _p, _p, _dp, _c.POINTER(_p), _i, _ip]),
     "otm_profile_enable": (_i, [_p, _i]),
     "otm_profile_read": (_i, [_p, _i, _c.POINTER(_c.c_longlong), _dp, _dp]),
     "otm_device_alloc": (_i, [_i, _c.c_size_t, _c.POINTER(_p)]),

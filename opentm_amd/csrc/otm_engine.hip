@@ -1216,6 +1216,144 @@ OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double v
   });
 }
 
+// ---- MMA optimizer ------------------------------------------------------------------
+struct otm_mma {
+  int m = 0, device = 0;
+  long long n = 0;
+  double a0 = 1, a = 0, c = 1e3, d = 1, move = 1.0;
+  hipStream_t s = nullptr;
+  MmaArrays A;
+  double* gval = nullptr;
+  void* small = nullptr;
+  void* small_host = nullptr;  // pinned
+  float* xstage = nullptr;     // FP32 staging for host callers
+  std::vector<double*> owned;
+};
+
+static void mma_free(otm_mma* o) {
+  if (!o) return;
+  if (o->s) (void)hipStreamSynchronize(o->s);
+  for (double* p : o->owned) dfree(p);
+  dfree(o->gval);
+  dfree(o->small);
+  dfree(o->xstage);
+  if (o->small_host) (void)hipHostFree(o->small_host);
+  if (o->s) (void)hipStreamDestroy(o->s);
+  delete o;
+}
+
+OTM_API int otm_mma_create(int m, int n, double a0, double a, double c, double d, int device, otm_mma** out) {
+  return guarded([&] {
+    if (!out || n <= 0) fail(OTM_E_ARG, "invalid argument");
+    if (m < 1 || m > kMmaMaxM) fail(OTM_E_UNSUPPORTED, "MMA supports 1..4 constraints");
+    *out = nullptr;
+    otm_mma* o = new otm_mma();
+    o->m = m;
+    o->n = n;
+    o->a0 = a0;
+    o->a = a;
+    o->c = c;
+    o->d = d;
+    o->device = device;
+    try {
+      hip_check(hipSetDevice(device), "hipSetDevice");
+      hip_check(hipStreamCreateWithFlags(&o->s, hipStreamNonBlocking), "hipStreamCreate");
+      auto vec = [&](long long cnt) {
+        double* p = dalloc<double>(cnt);
+        o->owned.push_back(p);
+        return p;
+      };
+      MmaArrays& A = o->A;
+      A.n = n;
+      A.xvar = vec(n); A.xmin = vec(n); A.xmax = vec(n); A.xold1 = vec(n); A.xold2 = vec(n);
+      A.low = vec(n); A.upp = vec(n); A.alfa = vec(n); A.beta = vec(n); A.p0 = vec(n); A.q0 = vec(n);
+      A.P = vec((long long)m * n); A.Q = vec((long long)m * n);
+      A.df0dx = vec(n); A.dgdx = vec((long long)m * n);
+      A.xs = vec(n); A.xsi = vec(n); A.eta = vec(n); A.dx = vec(n);
+      A.xs_t = vec(n); A.xsi_t = vec(n); A.eta_t = vec(n);
+      A.partials = vec(1024 * 32);
+      o->gval = dalloc<double>(kMmaMaxM);
+      o->small = dalloc<char>(mma_small_bytes());
+      o->xstage = dalloc<float>(n);
+      hip_check(hipHostMalloc(&o->small_host, mma_small_bytes(), hipHostMallocDefault), "hipHostMalloc");
+      launch_fill_d(o->s, A.xmin, n, 0.0);
+      launch_fill_d(o->s, A.xmax, n, 1.0);
+      launch_fill_d(o->s, A.xold1, n, 0.0);
+      launch_fill_d(o->s, A.xold2, n, 0.0);
+      hip_check(hipStreamSynchronize(o->s), "sync");
+    } catch (...) {
+      mma_free(o);
+      throw;
+    }
+    *out = o;
+  });
+}
+
+OTM_API int otm_mma_destroy(otm_mma* mma) {
+  return guarded([&] { mma_free(mma); });
+}
+
+OTM_API int otm_mma_set_bounds(otm_mma* o, double xmin, double xmax) {
+  return guarded([&] {
+    if (!o || !(xmax > xmin)) fail(OTM_E_ARG, "invalid bounds");
+    hip_check(hipSetDevice(o->device), "hipSetDevice");
+    launch_fill_d(o->s, o->A.xmin, o->n, xmin);
+    launch_fill_d(o->s, o->A.xmax, o->n, xmax);
+    hip_check(hipStreamSynchronize(o->s), "sync");
+  });
+}
+
+static const float* mma_stage(otm_mma* o, const float* p, int where) {
+  if (where == OTM_DEVICE) return p;
+  hip_check(hipMemcpyAsync(o->xstage, p, sizeof(float) * o->n, hipMemcpyHostToDevice, o->s), "upload");
+  return o->xstage;
+}
+
+OTM_API int otm_mma_set_bounds_array(otm_mma* o, const float* xmin, const float* xmax, int where) {
+  return guarded([&] {
+    if (!o || !xmin || !xmax) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    hip_check(hipSetDevice(o->device), "hipSetDevice");
+    launch_f2d(o->s, o->n, mma_stage(o, xmin, where), o->A.xmin);
+    launch_f2d(o->s, o->n, mma_stage(o, xmax, where), o->A.xmax);
+    hip_check(hipStreamSynchronize(o->s), "sync");
+  });
+}
+
+OTM_API int otm_mma_set_move(otm_mma* o, double move) {
+  return guarded([&] {
+    if (!o || !(move > 0)) fail(OTM_E_ARG, "invalid move");
+    o->move = move;
+  });
+}
+
+OTM_API int otm_mma_update(otm_mma* o, int itn, float* x, const float* dfdx, const double* gval,
+                           const float* const* dgdx, int where, int* newton) {
+  return guarded([&] {
+    if (!o || !x || !dfdx || !gval || !dgdx) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    hip_check(hipSetDevice(o->device), "hipSetDevice");
+    const long long n = o->n;
+    // FP32 -> FP64 (mmaOptimizer.cu:63-68)
+    launch_f2d(o->s, n, mma_stage(o, x, where), o->A.xvar);
+    launch_f2d(o->s, n, mma_stage(o, dfdx, where), o->A.df0dx);
+    for (int i = 0; i < o->m; i++) {
+      if (!dgdx[i]) fail(OTM_E_ARG, "null constraint gradient");
+      launch_f2d(o->s, n, mma_stage(o, dgdx[i], where), o->A.dgdx + (long long)i * n);
+    }
+    hip_check(hipMemcpyAsync(o->gval, gval, sizeof(double) * o->m, hipMemcpyHostToDevice, o->s), "gval");
+    float* xout = where == OTM_DEVICE ? x : o->xstage;
+    const int rc = mma_update(o->s, o->m, o->A, o->small, o->small_host, o->gval, itn, o->move, o->a0, o->a, o->c,
+                              o->d, xout);
+    if (rc == -2) fail(OTM_E_NUMERIC, "MMA subsolver produced non-finite values");
+    if (rc < 0) fail(OTM_E_HIP, "MMA update failed");
+    if (where == OTM_HOST)
+      hip_check(hipMemcpyAsync(x, o->xstage, sizeof(float) * n, hipMemcpyDeviceToHost, o->s), "download x");
+    hip_check(hipStreamSynchronize(o->s), "sync");
+    if (newton) *newton = rc;
+  });
+}
+
 OTM_API int otm_profile_enable(otm_ctx* ctx, int enable) {
   return guarded([&] {
     if (!ctx) fail(OTM_E_ARG, "null context");

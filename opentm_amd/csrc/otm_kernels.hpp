@@ -84,4 +84,28 @@ void launch_oc_bisect_step(hipStream_t s, long long n, const float* sens, const 
                            OcState* st, float min_rho, float step, float damp, double volratio, double vol_tol,
                            const Reduce& R, unsigned nblocks);
 
+// ---- MMA (otm_mma.hip) ----
+constexpr int kMmaMaxM = 4;  // constraints supported by the device subsolver
+struct MmaArrays {
+  long long n;
+  double *xvar, *xmin, *xmax, *xold1, *xold2, *low, *upp;  // persistent state
+  double *alfa, *beta, *p0, *q0, *P, *Q;                   // subproblem
+  double *df0dx, *dgdx;                                    // inputs (dgdx: m x n)
+  double *xs, *xsi, *eta, *dx, *xs_t, *xsi_t, *eta_t;      // interior point
+  double* partials;                                        // [blocks * 32]
+};
+inline unsigned red_blocks_for_mma(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+size_t mma_small_bytes();
+// One MMA update on device arrays; returns Newton steps or a negative code
+// (-1 HIP error, -2 non-finite values, -3 unsupported m).
+int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
+               int itn, double move, double a0, double av, double cv, double dv, float* xout);
+void launch_f2d(hipStream_t s, long long n, const float* in, double* out);
+void launch_fill_d(hipStream_t s, double* a, long long n, double v);
+
 }  // namespace otm

@@ -109,6 +109,21 @@ void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblock
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s, R, nv, (int)nblocks);
 }
 
+__global__ void __launch_bounds__(kBlock) k_f2d(long long n, const float* __restrict__ in, double* __restrict__ out) {
+  const long long t = gtid();
+  if (t < n) out[t] = (double)in[t];
+}
+__global__ void __launch_bounds__(kBlock) k_fill_d(double* a, long long n, double v) {
+  const long long t = gtid();
+  if (t < n) a[t] = v;
+}
+void launch_f2d(hipStream_t s, long long n, const float* in, double* out) {
+  hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(kBlock), 0, s, n, in, out);
+}
+void launch_fill_d(hipStream_t s, double* a, long long n, double v) {
+  hipLaunchKernelGGL(k_fill_d, dim3(nblk(n)), dim3(kBlock), 0, s, a, n, v);
+}
+
 void launch_fill(hipStream_t s, float* a, long long n, float v) {
   unsigned nb = nblk(n);
   if (nb > 4096) nb = 4096;

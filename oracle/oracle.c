@@ -473,3 +473,295 @@ int orc_oc_update(int n, const double* sens, const double* rho, double* rho_new,
   if (g_out) *g_out = g;
   return it;
 }
+
+/* ---- MMA -------------------------------------------------------------------
+ * Restates mmasub_ker (mmaOpt.cu:834-1019) and subsolv_g (mmaOpt.cu:429-830):
+ * asymptote update with asyinit 0.5, asyincr 1.2, asydecr 0.7 and the
+ * [0.01, 10] (xmax-xmin) clamps; alfa/beta with albefa 0.1 and the move
+ * limit; p0/q0/P/Q with 1e-3 and raa0 = 1e-5 regularisation; then the
+ * interior point with epsi from 1 down to epsimin = 1e-7, Newton steps on the
+ * (m+1) system [Alam a; a^T -zet/z] (solveLinearHost, mmaOpt.cpp:48-73),
+ * step length from the 1.01 fraction-to-boundary rule and a halving line
+ * search on the residual norm (at most 50 halvings, 200 Newton steps per
+ * epsi level). */
+static void small_solve(int k, double* A, double* b) {
+  /* Gaussian elimination with partial pivoting, A k x k row-major */
+  for (int c = 0; c < k; c++) {
+    int p = c;
+    for (int r = c + 1; r < k; r++)
+      if (fabs(A[r * k + c]) > fabs(A[p * k + c])) p = r;
+    if (p != c) {
+      for (int j = 0; j < k; j++) { double t = A[c * k + j]; A[c * k + j] = A[p * k + j]; A[p * k + j] = t; }
+      double t = b[c]; b[c] = b[p]; b[p] = t;
+    }
+    for (int r = c + 1; r < k; r++) {
+      double f = A[r * k + c] / A[c * k + c];
+      for (int j = c; j < k; j++) A[r * k + j] -= f * A[c * k + j];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = k - 1; r >= 0; r--) {
+    double s = b[r];
+    for (int j = r + 1; j < k; j++) s -= A[r * k + j] * b[j];
+    b[r] = s / A[r * k + r];
+  }
+}
+
+typedef struct {
+  int m, n;
+  const double *low, *upp, *alfa, *beta, *p0, *q0, *P, *Q, *a, *b, *c, *d;
+  double a0;
+} mma_sub;
+
+/* residual of the perturbed KKT system at (x..s); returns the norm, *mx the max */
+static double mma_residual(const mma_sub* S, double epsi, const double* x, const double* y, double z,
+                           const double* lam, const double* xsi, const double* eta, const double* mu,
+                           double zet, const double* s, double* mx) {
+  int m = S->m, n = S->n;
+  double ss = 0, mm = 0;
+  double gvec[16] = {0};
+  for (int j = 0; j < n; j++) {
+    double ux1 = S->upp[j] - x[j], xl1 = x[j] - S->low[j];
+    double plam = S->p0[j], qlam = S->q0[j];
+    for (int i = 0; i < m; i++) {
+      plam += lam[i] * S->P[(long)i * n + j];
+      qlam += lam[i] * S->Q[(long)i * n + j];
+      gvec[i] += S->P[(long)i * n + j] / ux1 + S->Q[(long)i * n + j] / xl1;
+    }
+    double dpsidx = plam / (ux1 * ux1) - qlam / (xl1 * xl1);
+    double rex = dpsidx - xsi[j] + eta[j];
+    double rexsi = xsi[j] * (x[j] - S->alfa[j]) - epsi;
+    double reeta = eta[j] * (S->beta[j] - x[j]) - epsi;
+    ss += rex * rex + rexsi * rexsi + reeta * reeta;
+    if (fabs(rex) > mm) mm = fabs(rex);
+    if (fabs(rexsi) > mm) mm = fabs(rexsi);
+    if (fabs(reeta) > mm) mm = fabs(reeta);
+  }
+  double alam = 0;
+  for (int i = 0; i < m; i++) alam += S->a[i] * lam[i];
+  double rez = S->a0 - zet - alam, rezet = zet * z - epsi;
+  ss += rez * rez + rezet * rezet;
+  if (fabs(rez) > mm) mm = fabs(rez);
+  if (fabs(rezet) > mm) mm = fabs(rezet);
+  for (int i = 0; i < m; i++) {
+    double rey = S->c[i] + S->d[i] * y[i] - mu[i] - lam[i];
+    double relam = gvec[i] - S->a[i] * z - y[i] + s[i] - S->b[i];
+    double remu = mu[i] * y[i] - epsi;
+    double res = lam[i] * s[i] - epsi;
+    ss += rey * rey + relam * relam + remu * remu + res * res;
+    if (fabs(rey) > mm) mm = fabs(rey);
+    if (fabs(relam) > mm) mm = fabs(relam);
+    if (fabs(remu) > mm) mm = fabs(remu);
+    if (fabs(res) > mm) mm = fabs(res);
+  }
+  *mx = mm;
+  return sqrt(ss);
+}
+
+static int mma_subsolv(const mma_sub* S, double epsimin, double* xout) {
+  int m = S->m, n = S->n;
+  double* x = (double*)malloc(sizeof(double) * n);
+  double* xsi = (double*)malloc(sizeof(double) * n);
+  double* eta = (double*)malloc(sizeof(double) * n);
+  double* dx = (double*)malloc(sizeof(double) * n);
+  double* xo = (double*)malloc(sizeof(double) * n);
+  double* xsio = (double*)malloc(sizeof(double) * n);
+  double* etao = (double*)malloc(sizeof(double) * n);
+  double* dxsi = (double*)malloc(sizeof(double) * n);
+  double* deta = (double*)malloc(sizeof(double) * n);
+  double* delx = (double*)malloc(sizeof(double) * n);
+  double* diagx = (double*)malloc(sizeof(double) * n);
+  double* GG = (double*)malloc(sizeof(double) * (size_t)m * n);
+  double y[16], lam[16], mu[16], s[16], z = 1, zet = 1;
+  for (int j = 0; j < n; j++) {
+    x[j] = 0.5 * (S->alfa[j] + S->beta[j]);
+    xsi[j] = 1.0 / (x[j] - S->alfa[j]); if (xsi[j] < 1) xsi[j] = 1;
+    eta[j] = 1.0 / (S->beta[j] - x[j]); if (eta[j] < 1) eta[j] = 1;
+  }
+  for (int i = 0; i < m; i++) {
+    y[i] = 1; lam[i] = 1; s[i] = 1;
+    mu[i] = 0.5 * S->c[i] > 1 ? 0.5 * S->c[i] : 1;
+  }
+  int total = 0;
+  double epsi = 1;
+  while (epsi > epsimin) {
+    double residumax;
+    double residunorm = mma_residual(S, epsi, x, y, z, lam, xsi, eta, mu, zet, s, &residumax);
+    int ittt = 0;
+    while (residumax > 0.9 * epsi && ittt < 200) {
+      ittt++;
+      total++;
+      double gvec[16] = {0}, GGdd[16] = {0}, Alam[256] = {0};
+      for (int j = 0; j < n; j++) {
+        double ux1 = S->upp[j] - x[j], xl1 = x[j] - S->low[j];
+        double ux2 = ux1 * ux1, xl2 = xl1 * xl1, ux3 = ux1 * ux2, xl3 = xl1 * xl2;
+        double plam = S->p0[j], qlam = S->q0[j];
+        for (int i = 0; i < m; i++) {
+          double Pi = S->P[(long)i * n + j], Qi = S->Q[(long)i * n + j];
+          plam += lam[i] * Pi;
+          qlam += lam[i] * Qi;
+          gvec[i] += Pi / ux1 + Qi / xl1;
+          GG[(long)i * n + j] = Pi / ux2 - Qi / xl2;
+        }
+        double dpsidx = plam / ux2 - qlam / xl2;
+        delx[j] = dpsidx - epsi / (x[j] - S->alfa[j]) + epsi / (S->beta[j] - x[j]);
+        diagx[j] = 2 * (plam / ux3 + qlam / xl3) + xsi[j] / (x[j] - S->alfa[j]) + eta[j] / (S->beta[j] - x[j]);
+        for (int i = 0; i < m; i++) {
+          GGdd[i] += GG[(long)i * n + j] * delx[j] / diagx[j];
+          for (int k = 0; k < m; k++) Alam[i * m + k] += GG[(long)i * n + j] * GG[(long)k * n + j] / diagx[j];
+        }
+      }
+      double alam = 0;
+      for (int i = 0; i < m; i++) alam += S->a[i] * lam[i];
+      double delz = S->a0 - alam - epsi / z;
+      int K = m + 1;
+      double AA[289], bb[17];
+      for (int i = 0; i < m; i++) {
+        double dely = S->c[i] + S->d[i] * y[i] - lam[i] - epsi / y[i];
+        double dellam = gvec[i] - S->a[i] * z - y[i] - S->b[i] + epsi / lam[i];
+        double diagy = S->d[i] + mu[i] / y[i];
+        double diaglamyi = s[i] / lam[i] + 1.0 / diagy;
+        bb[i] = dellam + dely / diagy - GGdd[i];
+        for (int k = 0; k < m; k++) AA[i * K + k] = Alam[i * m + k] + (i == k ? diaglamyi : 0.0);
+        AA[i * K + m] = S->a[i];
+        AA[m * K + i] = S->a[i];
+      }
+      AA[m * K + m] = -zet / z;
+      bb[m] = delz;
+      small_solve(K, AA, bb);
+      double dlam[16], dy[16], dmu[16], ds[16];
+      double dz = bb[m];
+      for (int i = 0; i < m; i++) dlam[i] = bb[i];
+      for (int j = 0; j < n; j++) {
+        double g = 0;
+        for (int i = 0; i < m; i++) g += GG[(long)i * n + j] * dlam[i];
+        dx[j] = -delx[j] / diagx[j] - g / diagx[j];
+        dxsi[j] = -xsi[j] + epsi / (x[j] - S->alfa[j]) - xsi[j] * dx[j] / (x[j] - S->alfa[j]);
+        deta[j] = -eta[j] + epsi / (S->beta[j] - x[j]) + eta[j] * dx[j] / (S->beta[j] - x[j]);
+      }
+      for (int i = 0; i < m; i++) {
+        double dely = S->c[i] + S->d[i] * y[i] - lam[i] - epsi / y[i];
+        double diagy = S->d[i] + mu[i] / y[i];
+        dy[i] = -dely / diagy + dlam[i] / diagy;
+        dmu[i] = -mu[i] + epsi / y[i] - mu[i] * dy[i] / y[i];
+        ds[i] = -s[i] + epsi / lam[i] - s[i] * dlam[i] / lam[i];
+      }
+      double dzet = -zet + epsi / z - zet * dz / z;
+      /* fraction-to-boundary step */
+      double stm = 1.0;
+      for (int i = 0; i < m; i++) {
+        double v;
+        v = -1.01 * dy[i] / y[i]; if (v > stm) stm = v;
+        v = -1.01 * dlam[i] / lam[i]; if (v > stm) stm = v;
+        v = -1.01 * dmu[i] / mu[i]; if (v > stm) stm = v;
+        v = -1.01 * ds[i] / s[i]; if (v > stm) stm = v;
+      }
+      { double v = -1.01 * dz / z; if (v > stm) stm = v; v = -1.01 * dzet / zet; if (v > stm) stm = v; }
+      for (int j = 0; j < n; j++) {
+        double v;
+        v = -1.01 * dxsi[j] / xsi[j]; if (v > stm) stm = v;
+        v = -1.01 * deta[j] / eta[j]; if (v > stm) stm = v;
+        v = -1.01 * dx[j] / (x[j] - S->alfa[j]); if (v > stm) stm = v;
+        v = 1.01 * dx[j] / (S->beta[j] - x[j]); if (v > stm) stm = v;
+      }
+      double steg = 1.0 / stm;
+      memcpy(xo, x, sizeof(double) * n);
+      memcpy(xsio, xsi, sizeof(double) * n);
+      memcpy(etao, eta, sizeof(double) * n);
+      double yo[16], lamo[16], muo[16], so[16], zo = z, zeto = zet;
+      memcpy(yo, y, sizeof(double) * m); memcpy(lamo, lam, sizeof(double) * m);
+      memcpy(muo, mu, sizeof(double) * m); memcpy(so, s, sizeof(double) * m);
+      int itto = 0;
+      double resinew = 2 * residunorm, rmax = residumax;
+      while (resinew > residunorm && itto < 50) {
+        itto++;
+        for (int j = 0; j < n; j++) {
+          x[j] = xo[j] + steg * dx[j];
+          xsi[j] = xsio[j] + steg * dxsi[j];
+          eta[j] = etao[j] + steg * deta[j];
+        }
+        for (int i = 0; i < m; i++) {
+          y[i] = yo[i] + steg * dy[i];
+          lam[i] = lamo[i] + steg * dlam[i];
+          mu[i] = muo[i] + steg * dmu[i];
+          s[i] = so[i] + steg * ds[i];
+        }
+        z = zo + steg * dz;
+        zet = zeto + steg * dzet;
+        resinew = mma_residual(S, epsi, x, y, z, lam, xsi, eta, mu, zet, s, &rmax);
+        steg /= 2;
+      }
+      residunorm = resinew;
+      residumax = rmax;
+    }
+    epsi *= 0.1;
+  }
+  memcpy(xout, x, sizeof(double) * n);
+  free(x); free(xsi); free(eta); free(dx); free(xo); free(xsio); free(etao);
+  free(dxsi); free(deta); free(delx); free(diagx); free(GG);
+  return total;
+}
+
+int orc_mmasub(int m, int n, int itn, double* x, const double* xmin, const double* xmax,
+               double* xold1, double* xold2, const double* df0dx, const double* gval,
+               const double* dgdx, double* low, double* upp, double a0, const double* a,
+               const double* c, const double* d, double move, double epsimin) {
+  const double raa0 = 1e-5, albefa = 0.1;
+  const double asyinit = 0.5, asyincr = 1.2, asydecr = 0.7;
+  if (m < 1 || m > 15) return -1;
+  double* alfa = (double*)malloc(sizeof(double) * n);
+  double* beta = (double*)malloc(sizeof(double) * n);
+  double* p0 = (double*)malloc(sizeof(double) * n);
+  double* q0 = (double*)malloc(sizeof(double) * n);
+  double* P = (double*)malloc(sizeof(double) * (size_t)m * n);
+  double* Q = (double*)malloc(sizeof(double) * (size_t)m * n);
+  double b[16] = {0};
+  for (int j = 0; j < n; j++) {
+    double dxm = xmax[j] - xmin[j];
+    if (itn <= 2) {
+      low[j] = x[j] - asyinit * dxm;
+      upp[j] = x[j] + asyinit * dxm;
+    } else {
+      double zzz = (x[j] - xold1[j]) * (xold1[j] - xold2[j]);
+      double f = zzz > 0 ? asyincr : (zzz < 0 ? asydecr : 1.0);
+      low[j] = x[j] - f * (xold1[j] - low[j]);
+      upp[j] = x[j] + f * (upp[j] - xold1[j]);
+      double lmin = x[j] - 10 * dxm, lmax = x[j] - 0.01 * dxm;
+      double umin = x[j] + 0.01 * dxm, umax = x[j] + 10 * dxm;
+      if (low[j] < lmin) low[j] = lmin;
+      if (low[j] > lmax) low[j] = lmax;
+      if (upp[j] > umax) upp[j] = umax;
+      if (upp[j] < umin) upp[j] = umin;
+    }
+    double z1 = low[j] + albefa * (x[j] - low[j]), z2 = x[j] - move * dxm;
+    double al = z1 > z2 ? z1 : z2;
+    alfa[j] = al > xmin[j] ? al : xmin[j];
+    double z3 = upp[j] - albefa * (upp[j] - x[j]), z4 = x[j] + move * dxm;
+    double be = z3 < z4 ? z3 : z4;
+    beta[j] = be < xmax[j] ? be : xmax[j];
+    double xmami = dxm > 1e-5 ? dxm : 1e-5;
+    double ux1 = upp[j] - x[j], xl1 = x[j] - low[j];
+    double ux2 = ux1 * ux1, xl2 = xl1 * xl1;
+    double dp = df0dx[j] > 0 ? df0dx[j] : 0, dq = -df0dx[j] > 0 ? -df0dx[j] : 0;
+    double pq0 = 1e-3 * (dp + dq) + raa0 / xmami;
+    p0[j] = (dp + pq0) * ux2;
+    q0[j] = (dq + pq0) * xl2;
+    for (int i = 0; i < m; i++) {
+      double g = dgdx[(long)i * n + j];
+      double pp = g > 0 ? g : 0, qq = -g > 0 ? -g : 0;
+      double pq = 1e-3 * (pp + qq) + raa0 / xmami;
+      P[(long)i * n + j] = (pp + pq) * ux2;
+      Q[(long)i * n + j] = (qq + pq) * xl2;
+      b[i] += P[(long)i * n + j] / ux1 + Q[(long)i * n + j] / xl1;
+    }
+  }
+  for (int i = 0; i < m; i++) b[i] -= gval[i];
+  mma_sub S = {m, n, low, upp, alfa, beta, p0, q0, P, Q, a, b, c, d, a0};
+  double* xnew = (double*)malloc(sizeof(double) * n);
+  int its = mma_subsolv(&S, epsimin, xnew);
+  memcpy(xold2, xold1, sizeof(double) * n);
+  memcpy(xold1, x, sizeof(double) * n);
+  memcpy(x, xnew, sizeof(double) * n);
+  free(alfa); free(beta); free(p0); free(q0); free(P); free(Q); free(xnew);
+  return its;
+}

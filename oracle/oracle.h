@@ -75,6 +75,17 @@ int orc_oc_update(int n, const double* sens, const double* rho, double* rho_new,
                   double volratio, double min_rho, double step_limit, double damp,
                   int max_steps, double vol_tol, double* g_out);
 
+/* One MMA update (mmaOpt.cu:834-1019 mmasub_ker with subsolv_g :429-830,
+ * Svanberg's method-of-moving-asymptotes subproblem solved by a primal-dual
+ * interior point). m constraints, n variables; dgdx is m x n row-major.
+ * x is updated in place; xold1/xold2/low/upp carry state between calls.
+ * epsimin is the last interior-point barrier level (1e-7 in the reference,
+ * mmaOpt.cu:846). Returns the number of Newton iterations. */
+int orc_mmasub(int m, int n, int itn, double* x, const double* xmin, const double* xmax,
+               double* xold1, double* xold2, const double* df0dx, const double* gval,
+               const double* dgdx, double* low, double* upp, double a0, const double* a,
+               const double* c, const double* d, double move, double epsimin);
+
 #ifdef __cplusplus
 }
 #endif

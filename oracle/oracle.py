@@ -37,6 +37,8 @@ def lib():
         L.orc_oc_filter_sens.argtypes = [i, i, i, d, _dp, _dp, _dp]
         L.orc_oc_update.argtypes = [i, _dp, _dp, _dp, d, d, d, d, i, d, ctypes.POINTER(d)]
         L.orc_oc_update.restype = i
+        L.orc_mmasub.argtypes = [i, i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, _dp, _dp, _dp, d, d]
+        L.orc_mmasub.restype = i
         _LIB = L
     return _LIB
 
@@ -152,3 +154,30 @@ def oc_update(sens, rho, volratio, min_rho=0.001, step=0.02, damp=0.5, max_steps
     steps = lib().orc_oc_update(r.size, s, r, out, volratio, min_rho, step, damp, max_steps, vol_tol,
                                 ctypes.byref(g))
     return out, steps, g.value
+
+
+class MMA:
+    """State of the oracle's MMA (xold1, xold2, low, upp), like MMAContext."""
+
+    def __init__(self, m, n, a0=1.0, a=0.0, c=1e3, d=1.0, xmin=0.0, xmax=1.0):
+        self.m, self.n = m, n
+        self.a0 = a0
+        self.a = np.full(m, float(a))
+        self.c = np.full(m, float(c))
+        self.d = np.full(m, float(d))
+        self.xmin = np.broadcast_to(np.asarray(xmin, np.float64), (n,)).copy()
+        self.xmax = np.broadcast_to(np.asarray(xmax, np.float64), (n,)).copy()
+        self.xold1 = np.zeros(n)
+        self.xold2 = np.zeros(n)
+        self.low = np.zeros(n)
+        self.upp = np.zeros(n)
+        self.move = 1.0
+        self.epsimin = 1e-7  # the reference's last barrier level (mmaOpt.cu:846)
+
+    def update(self, itn, x, df0dx, gval, dgdx):
+        """Returns the new x (float64) and the Newton step count."""
+        xv = _d(x).reshape(-1).copy()
+        its = lib().orc_mmasub(self.m, self.n, itn, xv, self.xmin, self.xmax, self.xold1, self.xold2,
+                               _d(df0dx).reshape(-1), _d(gval).reshape(-1), _d(dgdx).reshape(-1),
+                               self.low, self.upp, self.a0, self.a, self.c, self.d, self.move, self.epsimin)
+        return xv, its

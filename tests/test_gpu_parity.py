@@ -294,3 +294,29 @@ def test_oc_loop_parity(orc):
         say("  it", it, "C err (same design)", rel(C, Cg), "rho err", drho, "stats", hom.solve_stats())
         assert rel(C, Cg) < 1e-5
         assert drho < 1e-4
+
+
+@pytest.mark.parametrize("m", [1, 3])
+def test_mma_matches_oracle(orc, m):
+    """Six consecutive MMA updates (asymptote initialisation and adaptation)
+    on the GPU and in the oracle from the same FP32 inputs."""
+    say("mma", m)
+    rng = np.random.default_rng(30 + m)
+    n = 4096
+    t = rng.uniform(0, 1, n)
+    W = rng.uniform(0.5, 1.5, (m, n))
+    bounds = np.linspace(0.2, 0.35, m)
+    gpu = otm.MMAOptimizer(m, n, 1.0, 0.0, 1e3, 1.0)
+    gpu.setBound(1e-3, 1.0)
+    ref = orc.MMA(m, n, 1.0, 0.0, 1e3, 1.0, 1e-3, 1.0)
+    xg = np.full(n, 0.3, np.float32)
+    xo = xg.astype(np.float64)
+    for it in range(6):
+        x32 = xg.astype(np.float64)
+        df = (2 * (x32 - t)).astype(np.float32)
+        g = (W * x32).mean(axis=1) - bounds
+        dg = (W / n).astype(np.float32)
+        gpu.update(it, xg, df, g, [dg[i] for i in range(m)])
+        xo, its = ref.update(it, x32, df.astype(np.float64), g, dg.astype(np.float64))
+        say("  it", it, "newton gpu", gpu.last_newton, "oracle", its, "max diff", float(np.abs(xg - xo).max()))
+        assert np.abs(xg - xo).max() < 2e-6

@@ -282,3 +282,88 @@ def test_oc_filter_sens_loop(orc):
             ws += w
         ref[z, y, x] = s / (ws * rho[z, y, x])
     np.testing.assert_allclose(out, ref, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- MMA
+def _mma_subproblem(x, xmin, xmax, df0, g, dg, move=1.0):
+    """Approximation built at iteration 0 (asymptotes x -+ 0.5 (xmax - xmin))."""
+    dxm = xmax - xmin
+    low, upp = x - 0.5 * dxm, x + 0.5 * dxm
+    alfa = np.maximum(np.maximum(low + 0.1 * (x - low), x - move * dxm), xmin)
+    beta = np.minimum(np.minimum(upp - 0.1 * (upp - x), x + move * dxm), xmax)
+    inv = 1.0 / np.maximum(dxm, 1e-5)
+    ux1, xl1 = upp - x, x - low
+
+    def pq(gr):
+        p, q = np.maximum(gr, 0), np.maximum(-gr, 0)
+        r = 1e-3 * (p + q) + 1e-5 * inv
+        return (p + r) * ux1 ** 2, (q + r) * xl1 ** 2
+    p0, q0 = pq(df0)
+    P, Q = zip(*[pq(dg[i]) for i in range(dg.shape[0])])
+    P, Q = np.array(P), np.array(Q)
+    b = (P / ux1 + Q / xl1).sum(axis=1) - g
+    return low, upp, alfa, beta, p0, q0, P, Q, b
+
+
+def test_mma_subproblem_against_dual_solver(orc):
+    """One oracle MMA update equals the minimiser of the MMA subproblem found
+    independently by maximising its concave dual over lambda >= 0."""
+    from scipy.optimize import minimize
+    rng = np.random.default_rng(21)
+    n, m = 300, 2
+    c, d = 1e3, 1.0
+    x = rng.uniform(0.2, 0.8, n)
+    df0 = rng.standard_normal(n)
+    dg = rng.standard_normal((m, n)) / n + 1.0 / n
+    g = np.array([0.05, -0.02])
+    low, upp, alfa, beta, p0, q0, P, Q, b = _mma_subproblem(x, 0.0, 1.0, df0, g, dg)
+
+    def xstar(lam):
+        A = p0 + lam @ P
+        B = q0 + lam @ Q
+        xs = (low * np.sqrt(A) + upp * np.sqrt(B)) / (np.sqrt(A) + np.sqrt(B))
+        return np.clip(xs, alfa, beta)
+
+    def negdual(lam):
+        lam = np.maximum(lam, 0)
+        xs = xstar(lam)
+        A = p0 + lam @ P
+        B = q0 + lam @ Q
+        y = np.maximum(0, (lam - c) / d)
+        W = np.sum(A / (upp - xs) + B / (xs - low)) + np.sum(c * y + 0.5 * d * y * y - lam * y) - lam @ b
+        grad = (P / (upp - xs) + Q / (xs - low)).sum(axis=1) - y - b
+        return -W, -grad
+    res = minimize(negdual, np.ones(m), jac=True, method="L-BFGS-B", bounds=[(0, None)] * m,
+                   options={"gtol": 1e-13, "ftol": 1e-16, "maxiter": 5000})
+    x_dual = xstar(np.maximum(res.x, 0))
+    # driven to a tiny barrier the interior point lands on the dual solution
+    tight = orc.MMA(m, n, 1.0, 0.0, c, d, 0.0, 1.0)
+    tight.epsimin = 1e-13
+    x_tight, _ = tight.update(0, x, df0, g, dg)
+    np.testing.assert_allclose(x_tight, x_dual, atol=1e-8)
+    # at the reference's last barrier level (1e-7) it stops within 1e-4
+    mma = orc.MMA(m, n, 1.0, 0.0, c, d, 0.0, 1.0)
+    x_new, its = mma.update(0, x, df0, g, dg)
+    assert its > 0
+    assert np.abs(x_new - x_dual).max() < 1e-4
+
+
+def test_mma_converges_to_constrained_optimum(orc):
+    """min sum (x-t)^2 s.t. mean(x) <= 0.3, mean(w x) <= 0.25 on [0,1]^n: the
+    MMA iterates reach the optimum computed by SLSQP."""
+    from scipy.optimize import minimize
+    rng = np.random.default_rng(22)
+    n = 40
+    t = rng.uniform(0, 1, n)
+    w = rng.uniform(0.5, 1.5, n)
+    cons = [{"type": "ineq", "fun": lambda x: 0.3 - x.mean(), "jac": lambda x: -np.ones(n) / n},
+            {"type": "ineq", "fun": lambda x: 0.25 - (w * x).mean(), "jac": lambda x: -w / n}]
+    ref = minimize(lambda x: ((x - t) ** 2).sum(), np.full(n, 0.2), jac=lambda x: 2 * (x - t),
+                   bounds=[(0, 1)] * n, constraints=cons, method="SLSQP", options={"ftol": 1e-14, "maxiter": 500})
+    mma = orc.MMA(2, n, 1.0, 0.0, 1e3, 1.0, 0.0, 1.0)
+    x = np.full(n, 0.2)
+    for it in range(60):
+        g = np.array([x.mean() - 0.3, (w * x).mean() - 0.25])
+        dg = np.stack([np.ones(n) / n, w / n])
+        x, _ = mma.update(it, x, 2 * (x - t), g, dg)
+    assert np.abs(x - ref.x).max() < 2e-3
