@@ -194,6 +194,46 @@ OTM_API int otm_mma_set_move(otm_mma* mma, double move);
 OTM_API int otm_mma_update(otm_mma* mma, int itn, float* x, const float* dfdx, const double* gval,
                            const float* const* dgdx, int where, int* newton);
 
+/* Density initialisers of initDensity (Framework.cu:61-145), indexed like
+ * cfg::InitWay: 0 random, 1 randcenter, 2 noise, 3 manual, 4 interp,
+ * 5 rep_randcenter, 6 P, 7 G, 8 D, 9 IWP, 10 example. The TPMS fields P, G,
+ * D, IWP and `example` follow the reference's closed forms; the random ones
+ * are seeded here (the reference seeds cuRAND from the clock); manual/interp
+ * read OpenVDB files and are not supported. The result is clamped to
+ * [0.1, 1] as in Framework.cu:144 and becomes the context's design. */
+OTM_API int otm_init_density(otm_ctx* ctx, int initway, int seed, double volratio);
+
+/* Mean design density and mean SIMP-mapped conductivity of the last
+ * otm_eval (rho_H.sum()/ne and rhop_H.sum()/ne in runCustom). */
+OTM_API int otm_design_stats(otm_ctx* ctx, double* mean_rho, double* mean_kappa);
+
+/* runCustom (Framework.cu:147-244): tensor-matching design loop, objective
+ * sum over (00,11,22,01,21,02) of (C_ij - target)^2 - 1e-4, with OC
+ * (VolumeGovernor + ConvergeChecker) or MMA (minimise volume subject to the
+ * tensor error). */
+enum { OTM_MODEL_MMA = 0, OTM_MODEL_OC = 1 };
+typedef struct otm_custom_params {
+  otm_config base;     /* resolution, heat ratios, solver */
+  int model;           /* OTM_MODEL_MMA or OTM_MODEL_OC (HomoConfig.model) */
+  int initway;         /* HomoConfig.winit, see otm_init_density */
+  int seed;            /* seed of the random initialisers */
+  int max_iter;        /* HomoConfig.max_iter */
+  double finthres;     /* HomoConfig.finthres */
+  double volratio;     /* HomoConfig.volRatio (noise initialiser) */
+  double target[6];    /* HomoConfig.target_tensor: xx yy zz xy yz xz */
+} otm_custom_params;
+typedef struct otm_custom_report {
+  int iterations;
+  int converged;
+  double objective;    /* last objective value */
+  double volume;       /* mean design density */
+  double C[9];         /* last homogenized tensor */
+} otm_custom_report;
+OTM_API void otm_custom_params_init(otm_custom_params* p);
+/* rho0: optional host initial density (reso^3 floats); rho_out: host
+ * reso^3 floats, the final design; report may be NULL. */
+OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float* rho_out, otm_custom_report* report);
+
 /* Live kernel timing with HIP events recorded on the context's stream
  * around every launch of a kernel class. Classes: */
 enum {

@@ -22,7 +22,7 @@ from ._capi import OTM_DEVICE, OTM_HOST, OtmError
 
 __all__ = [
     "HomoConfig", "Homogenization_H", "OCOptimizer", "MMAOptimizer", "DeviceBuffer", "DensityParams",
-    "conv_filter", "OtmError", "load_library",
+    "conv_filter", "OtmError", "load_library", "run_custom",
 ]
 
 
@@ -460,3 +460,40 @@ def conv_filter(x, kind=0, radius=1.5, periodic=False, adjoint=False):
                                              1 if adjoint else 0, _ptr(a), _ptr(out), OTM_HOST),
                 "otm_conv_filter")
     return out
+
+
+_INITWAYS = ["random", "randcenter", "noise", "manual", "interp", "rep_randcenter", "P", "G", "D", "IWP", "example"]
+
+
+def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
+               target=(0.3, 0.2, 0.1, 0.0, 0.0, 0.0), heat_ratio=(1.0, 0.0), fem_rel_thres=1e-6,
+               finthres=5e-4, volratio=0.3, coarse_sweeps=0):
+    """runCustom (Framework.cu:147-244) on the engine: returns the final
+    design density (x fastest) and a report dict."""
+    lib = _capi.load()
+    p = _capi.OtmCustomParams()
+    lib.otm_custom_params_init(ctypes.byref(p))
+    r = (reso, reso, reso) if np.isscalar(reso) else tuple(reso)
+    for a in range(3):
+        p.base.reso[a] = int(r[a])
+    p.base.heat_ratio[0], p.base.heat_ratio[1] = float(heat_ratio[0]), float(heat_ratio[1])
+    p.base.fem_rel_thres = fem_rel_thres
+    p.base.coarse_sweeps = coarse_sweeps
+    p.model = 0 if model == "mma" else 1
+    p.initway = _INITWAYS.index(initway)
+    p.seed = int(seed)
+    p.max_iter = int(max_iter)
+    p.finthres = finthres
+    p.volratio = volratio
+    for i in range(6):
+        p.target[i] = float(target[i])
+    n = r[0] * r[1] * r[2]
+    out = np.empty(n, np.float32)
+    rep = _capi.OtmCustomReport()
+    r0 = None
+    if rho0 is not None:
+        r0a = _f32(rho0, n)
+        r0 = _ptr(r0a)
+    _capi.check(lib.otm_run_custom(ctypes.byref(p), r0, _ptr(out), ctypes.byref(rep)), "otm_run_custom")
+    return out, {"iterations": rep.iterations, "converged": bool(rep.converged), "objective": rep.objective,
+                 "volume": rep.volume, "C": np.array(rep.C[:]).reshape(3, 3)}

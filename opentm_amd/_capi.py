@@ -56,6 +56,29 @@ class OtmDensityParams(ctypes.Structure):
     ]
 
 
+class OtmCustomParams(ctypes.Structure):
+    _fields_ = [
+        ("base", OtmConfig),
+        ("model", _c.c_int),
+        ("initway", _c.c_int),
+        ("seed", _c.c_int),
+        ("max_iter", _c.c_int),
+        ("finthres", _c.c_double),
+        ("volratio", _c.c_double),
+        ("target", _c.c_double * 6),
+    ]
+
+
+class OtmCustomReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", _c.c_int),
+        ("converged", _c.c_int),
+        ("objective", _c.c_double),
+        ("volume", _c.c_double),
+        ("C", _c.c_double * 9),
+    ]
+
+
 SIGNATURES = {
     "otm_config_init": (None, [_c.POINTER(OtmConfig)]),
     "otm_last_error": (_c.c_char_p, []),
@@ -93,6 +116,10 @@ SIGNATURES = {
     "otm_mma_set_bounds_array": (_i, [_p, _p, _p, _i]),
     "otm_mma_set_move": (_i, [_p, _d]),
     "otm_mma_update": (_i, [_p, _i, _p, _p, _dp, _c.POINTER(_p), _i, _ip]),
+    "otm_init_density": (_i, [_p, _i, _i, _d]),
+    "otm_design_stats": (_i, [_p, _dp, _dp]),
+    "otm_custom_params_init": (None, [_c.POINTER(OtmCustomParams)]),
+    "otm_run_custom": (_i, [_c.POINTER(OtmCustomParams), _p, _p, _c.POINTER(OtmCustomReport)]),
     "otm_profile_enable": (_i, [_p, _i]),
     "otm_profile_read": (_i, [_p, _i, _c.POINTER(_c.c_longlong), _dp, _dp]),
     "otm_device_alloc": (_i, [_i, _c.c_size_t, _c.POINTER(_p)]),

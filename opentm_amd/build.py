@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libopentm.so")
-SOURCES = ["otm_fine.hip", "otm_coarse.hip", "otm_vec.hip", "otm_opt.hip", "otm_mma.hip", "otm_engine.hip"]
+SOURCES = ["otm_fine.hip", "otm_coarse.hip", "otm_vec.hip", "otm_opt.hip", "otm_mma.hip", "otm_engine.hip", "otm_framework.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [
@@ -38,7 +38,7 @@ def build_engine(verbose=False):
     jobs = []
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         for src in SOURCES:
-            obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+            obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
             objs.append(obj)
             cmd = [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
             jobs.append(ex.submit(_run, cmd))
@@ -52,6 +52,23 @@ def build_engine(verbose=False):
     return LIB
 
 
+def build_pybind(verbose=False):
+    """The pybind11 module openTM (g++), linked against libopentm.so."""
+    import sysconfig
+    import pybind11
+    out = os.path.join(HERE, "openTM" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-fvisibility=hidden",
+           "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
+           "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "openTM_pybind.cpp"), "-L" + HERE, "-lopentm",
+           "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_oracle(verbose=False):
     _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
     if verbose:
@@ -61,6 +78,7 @@ def build_oracle(verbose=False):
 def build_all(verbose=False):
     build_oracle(verbose)
     build_engine(verbose)
+    build_pybind(verbose)
 
 
 if __name__ == "__main__":

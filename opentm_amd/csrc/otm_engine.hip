@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -649,6 +650,7 @@ static void ensure_design_buffers(otm_ctx* c) {
   c->rho_f = dalloc<float>(n);
   c->W = dalloc<float>(n);
   c->kraw = dalloc<float>(n);
+  launch_fill(c->s, c->kraw, n, 0.f);
   c->dkappa = dalloc<float>(n);
   c->sens = dalloc<float>(n);
   c->tmp = dalloc<float>(n);
@@ -1012,6 +1014,60 @@ OTM_API int otm_get_design(otm_ctx* ctx, float* rho, int where) {
     hip_check(hipMemcpyAsync(rho, ctx->rho, sizeof(float) * ctx->g0.nn, kind_of(where, OTM_DEVICE), ctx->s),
               "copy rho");
     ctx->sync();
+  });
+}
+
+OTM_API int otm_init_density(otm_ctx* ctx, int initway, int seed, double volratio) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    if (initway == 3 || initway == 4) fail(OTM_E_UNSUPPORTED, "OpenVDB initial densities are not supported");
+    if (initway < 0 || initway > 10) fail(OTM_E_ARG, "unknown initialisation");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ensure_design_buffers(ctx);
+    InitModes M;
+    memset(&M, 0, sizeof(M));
+    M.mean = (float)volratio;
+    if (initway <= 5) {
+      // 8 seeded cosine modes, |k| <= 3, amplitude N(0,1)/8 scaled to 0.25
+      std::mt19937_64 gen((uint64_t)seed * 0x9E3779B97F4A7C15ull + 12345u);
+      std::normal_distribution<double> nd(0.0, 1.0);
+      std::uniform_real_distribution<double> ud(0.0, 6.283185307179586);
+      std::uniform_int_distribution<int> kd(-3, 3);
+      M.count = 8;
+      for (int m = 0; m < M.count; m++) {
+        int k0, k1, k2;
+        do {
+          k0 = kd(gen);
+          k1 = kd(gen);
+          k2 = kd(gen);
+        } while (k0 * k0 + k1 * k1 + k2 * k2 == 0 || k0 * k0 + k1 * k1 + k2 * k2 > 9);
+        M.k[m][0] = (float)k0;
+        M.k[m][1] = (float)k1;
+        M.k[m][2] = (float)k2;
+        M.amp[m] = (float)(0.25 * nd(gen) / 8.0);
+        M.phase[m] = (float)ud(gen);
+      }
+    }
+    launch_init_density(ctx->s, ctx->g0, initway, M, ctx->rho);
+    hip_check(hipGetLastError(), "init density");
+    ctx->sync();
+    ctx->has_drho = false;
+  });
+}
+
+OTM_API int otm_design_stats(otm_ctx* ctx, double* mean_rho, double* mean_kappa) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    if (!ctx->rho) fail(OTM_E_STATE, "no design density set");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    const unsigned nb = red_blocks_for(n);
+    launch_sum2(ctx->s, n, ctx->rho, ctx->kraw, ctx->reducer(FIN_STORE), nb);
+    hip_check(hipMemcpyAsync(ctx->out_host, ctx->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->s),
+              "read stats");
+    ctx->sync();
+    if (mean_rho) *mean_rho = ctx->out_host[0] / (double)n;
+    if (mean_kappa) *mean_kappa = ctx->out_host[1] / (double)n;
   });
 }
 

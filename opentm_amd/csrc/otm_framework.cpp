@@ -1,0 +1,156 @@
+// otm_framework.cpp -- runCustom (Framework.cu:147-244) over the engine's
+// public C ABI: the tensor-matching design loop with OC (VolumeGovernor +
+// ConvergeChecker) or MMA. Everything stays on the device between
+// iterations; only the 3x3 tensor, the volume statistics and the MMA
+// constraint value come back to the host.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/opentm.h"
+#include "../../include/opentm/opentm.hpp"
+
+namespace {
+
+struct Fail {
+  int code;
+};
+inline void ck(int status) {
+  if (status != OTM_OK) throw Fail{status};
+}
+
+// objective of runCustom (Framework.cu:168-170) and its gradient wrt C
+double tensor_objective(const double C[9], const double t[6], double dC[9]) {
+  static const int ii[6] = {0, 1, 2, 0, 2, 0}, jj[6] = {0, 1, 2, 1, 1, 2};
+  double f = 0.0;
+  for (int q = 0; q < 9; q++) dC[q] = 0.0;
+  for (int q = 0; q < 6; q++) {
+    const double e = C[ii[q] * 3 + jj[q]] - t[q];
+    f += e * e;
+    dC[ii[q] * 3 + jj[q]] += 2.0 * e;
+  }
+  return f - 1e-4;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf(int device, size_t bytes) { ck(otm_device_alloc(device, bytes, &p)); }
+  ~DevBuf() { otm_device_free(p); }
+  float* f() { return static_cast<float*>(p); }
+};
+
+struct Ctx {
+  otm_ctx* c = nullptr;
+  ~Ctx() { otm_destroy(c); }
+};
+
+}  // namespace
+
+extern "C" {
+
+OTM_API void otm_custom_params_init(otm_custom_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  otm_config_init(&p->base);
+  p->model = OTM_MODEL_OC;
+  p->initway = 9;  // IWP (cmdline.cpp:10)
+  p->seed = 0;
+  p->max_iter = 500;         // cmdline.cpp:22
+  p->finthres = 5e-4;        // cmdline.cpp:24
+  p->volratio = 0.3;         // cmdline.cpp:11
+  const double t[6] = {0.3, 0.2, 0.1, 0.0, 0.0, 0.0};  // cmdline.cpp:29-34
+  for (int i = 0; i < 6; i++) p->target[i] = t[i];
+}
+
+OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float* rho_out, otm_custom_report* report) {
+  try {
+    if (!p || !rho_out) return OTM_E_ARG;
+    const int* reso = p->base.reso;
+    const long long ne = (long long)reso[0] * reso[1] * reso[2];
+    const int dev = p->base.device;
+    Ctx hom;
+    ck(otm_create(&p->base, &hom.c));
+    if (rho0)
+      ck(otm_set_design(hom.c, rho0, OTM_HOST));
+    else
+      ck(otm_init_density(hom.c, p->initway, p->seed, p->volratio));
+    otm_density_params dp;
+    otm_density_params_init(&dp);  // Spline4 r = 1.5, non-periodic (Framework.cu:163)
+    homo::ConvergeChecker criteria(p->finthres);
+    double C[9] = {0}, dC[9], obj = 0.0;
+    int itn = 0;
+    bool converged = false;
+    if (p->model == OTM_MODEL_MMA) {
+      // MMAOptimizer mma(1, ne, 1, 0, 1e7, 1); setBound(1e-4, 1) (Framework.cu:172-173):
+      // the volume is the objective and the tensor error the constraint
+      otm_mma* mma = nullptr;
+      ck(otm_mma_create(1, (int)ne, 1.0, 0.0, 1e7, 1.0, dev, &mma));
+      struct MmaGuard {
+        otm_mma* m;
+        ~MmaGuard() { otm_mma_destroy(m); }
+      } guard{mma};
+      ck(otm_mma_set_bounds(mma, 1e-4, 1.0));
+      DevBuf rho(dev, sizeof(float) * ne), dfdx(dev, sizeof(float) * ne), dvdx(dev, sizeof(float) * ne);
+      std::vector<float> vol((size_t)ne, (float)(1.0 / (double)ne));
+      ck(otm_memcpy(dvdx.p, vol.data(), sizeof(float) * ne, OTM_DEVICE, OTM_HOST));
+      for (itn = 0; itn < p->max_iter; itn++) {
+        ck(otm_eval(hom.c, &dp, C));
+        obj = tensor_objective(C, p->target, dC);
+        ck(otm_backward(hom.c, dC, dfdx.f(), OTM_DEVICE));
+        if (criteria.is_converge(itn, obj)) {
+          converged = true;
+          break;
+        }
+        ck(otm_get_design(hom.c, rho.f(), OTM_DEVICE));
+        const float* dg[1] = {dfdx.f()};
+        const double gval[1] = {obj};
+        ck(otm_mma_update(mma, itn, rho.f(), dvdx.f(), gval, dg, OTM_DEVICE, nullptr));
+        ck(otm_set_design(hom.c, rho.f(), OTM_DEVICE));
+      }
+    } else {
+      // OCOptimizer oc(ne, 0.001, 0.02, 0.5) with the adaptive volume bound
+      // (Framework.cu:206-231)
+      otm_oc* oc = nullptr;
+      ck(otm_oc_create((int)ne, 0.001, 0.02, 0.5, dev, &oc));
+      struct OcGuard {
+        otm_oc* o;
+        ~OcGuard() { otm_oc_destroy(o); }
+      } guard{oc};
+      homo::VolumeGovernor governor;
+      for (itn = 0; itn < p->max_iter; itn++) {
+        ck(otm_eval(hom.c, &dp, C));
+        obj = tensor_objective(C, p->target, dC);
+        double vf = 0.0, lower = 0.0;
+        ck(otm_design_stats(hom.c, &vf, &lower));
+        governor.volume_check((float)obj, (float)lower, (float)vf, itn);
+        ck(otm_backward(hom.c, dC, nullptr, OTM_DEVICE));
+        if (criteria.is_converge(itn, obj) && governor.get_current_decrease() < 1e-4) {
+          converged = true;
+          break;
+        }
+        ck(otm_oc_step(hom.c, oc, 2.0, governor.get_volume_bound(), nullptr, nullptr));
+      }
+    }
+    ck(otm_get_design(hom.c, rho_out, OTM_HOST));
+    if (report) {
+      report->iterations = itn;
+      report->converged = converged ? 1 : 0;
+      report->objective = obj;
+      double vf = 0.0;
+      ck(otm_design_stats(hom.c, &vf, nullptr));
+      report->volume = vf;
+      for (int q = 0; q < 9; q++) report->C[q] = C[q];
+    }
+    return OTM_OK;
+  } catch (const Fail& f) {
+    return f.code;  // the failing engine call has set otm_last_error
+  } catch (const std::exception&) {
+    return OTM_E_ARG;
+  }
+}
+
+}  // extern "C"

@@ -48,6 +48,7 @@ void launch_fill(hipStream_t s, float* a, long long n, float v);
 void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks);
 void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P);
+void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
 void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,
                    const PcgScalars* P, const Reduce& R, unsigned nblocks);
 void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P);
@@ -79,6 +80,15 @@ struct OcState {
   double vol;     // volume fraction at the last midpoint
 };
 void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out_bits);
+// seeded cosine-mode field for the random initialisers
+struct InitModes {
+  int count;
+  float mean;
+  float k[16][3];
+  float amp[16];
+  float phase[16];
+};
+void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes& M, float* rho);
 void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st);
 void launch_oc_bisect_step(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new,
                            OcState* st, float min_rho, float step, float damp, double volratio, double vol_tol,

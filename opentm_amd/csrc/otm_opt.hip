@@ -169,6 +169,68 @@ __global__ void __launch_bounds__(kBlock) k_oc_final(OcState* st, Reduce R, int 
     st->done = 1;
 }
 
+// Density initialisers (Framework.cu:61-145). tanproj(v, 20, 0.5) is the
+// smooth projection of lib.cuh:1372; TPMS fields are clamped to
+// [0.001, 1], then everything to [0.1, 1] (Framework.cu:144).
+__device__ inline float tanproj(float v, float beta) {
+  const float t = tanhf(beta * 0.5f);
+  return (t + tanhf(beta * (v - 0.5f))) / (t + tanhf(beta * 0.5f));
+}
+
+__global__ void __launch_bounds__(kBlock) k_init_density(Geom g, int kind, InitModes M, float* __restrict__ rho) {
+  const long long t = gtid();
+  if (t >= g.nn) return;
+  const int i = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+  const float pi = 3.1415926f;
+  const float x = float(i) / g.nx, y = float(j) / g.ny, z = float(k) / g.nz;
+  float v = 0.f;
+  switch (kind) {
+    case 6: {  // P
+      const float val = cosf(2 * pi * x) + cosf(2 * pi * y) + cosf(2 * pi * z);
+      v = fmaxf(fminf(tanproj(-val, 20.f), 1.f), 0.001f);
+      break;
+    }
+    case 7: {  // G
+      const float val = sinf(2 * pi * x) * cosf(2 * pi * y) + sinf(2 * pi * z) * cosf(2 * pi * x) +
+                        sinf(2 * pi * y) * cosf(2 * pi * z);
+      v = fmaxf(fminf(tanproj(val, 20.f), 1.f), 0.001f);
+      break;
+    }
+    case 8: {  // D
+      const float val = cosf(2 * pi * x) * cosf(2 * pi * y) * cosf(2 * pi * z) -
+                        sinf(2 * pi * x) * sinf(2 * pi * y) * sinf(2 * pi * z);
+      v = fmaxf(fminf(tanproj(val, 20.f), 1.f), 0.001f);
+      break;
+    }
+    case 9: {  // IWP
+      const float val = 2 * (cosf(2 * pi * x) * cosf(2 * pi * y) + cosf(2 * pi * y) * cosf(2 * pi * z) +
+                             cosf(2 * pi * z) * cosf(2 * pi * x)) -
+                        (cosf(4 * pi * x) + cosf(4 * pi * y) + cosf(4 * pi * z));
+      v = fmaxf(fminf(tanproj(val, 20.f), 1.f), 0.001f);
+      break;
+    }
+    case 10: {  // example: a void sphere of radius min(reso)/6 in a 0.3 matrix
+      const float dx = i - g.nx / 2.f + 0.5f, dy = j - g.ny / 2.f + 0.5f, dz = k - g.nz / 2.f + 0.5f;
+      const float rad = fminf(fminf((float)g.nx, (float)g.ny), (float)g.nz) / 6.f;
+      v = sqrtf(dx * dx + dy * dy + dz * dz) < rad ? 0.f : 0.3f;
+      break;
+    }
+    default: {  // seeded periodic cosine modes around the volume ratio
+      float s = 0.f;
+      for (int m = 0; m < M.count; m++)
+        s += M.amp[m] * cosf(2 * pi * (M.k[m][0] * x + M.k[m][1] * y + M.k[m][2] * z) + M.phase[m]);
+      v = M.mean + s;
+    }
+  }
+  rho[t] = fminf(fmaxf(v, 0.1f), 1.f);
+}
+
+void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes& M, float* rho) {
+  hipLaunchKernelGGL(k_init_density, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, kind, M, rho);
+}
+
 void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W) {
   hipLaunchKernelGGL(k_filter_fwd, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, in, out, W);
 }

@@ -32,6 +32,17 @@ __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __re
   reduce_block<OTM_NRHS>(acc, R);
 }
 
+__global__ void __launch_bounds__(kBlock) k_sum2(long long nn, const float* __restrict__ a, const float* __restrict__ b,
+                                                 Reduce R) {
+  double acc[2] = {0, 0};
+  const BlockRange br = block_range(nn);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+    acc[0] += (double)a[i];
+    acc[1] += (double)b[i];
+  }
+  reduce_block<2>(acc, R);
+}
+
 __global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, const PcgScalars* __restrict__ P) {
   const float m0 = (float)P->mean[0], m1 = (float)P->mean[1], m2 = (float)P->mean[2];
   for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
@@ -137,6 +148,10 @@ void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, co
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_sum3, dim3(nblocks), dim3(kBlock), 0, s, nn, a, R);
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
+}
+void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_sum2, dim3(nblocks), dim3(kBlock), 0, s, nn, a, b, R);
+  launch_reduce_final(s, R, 2, nblocks);
 }
 void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P) {
   unsigned nb = nblk(nn);

@@ -19,6 +19,9 @@ def _ensure_built():
         build.build_oracle()
     if not os.path.exists(build.LIB):
         build.build_engine()
+    import glob
+    if not glob.glob(os.path.join(ROOT, "opentm_amd", "openTM*.so")):
+        build.build_pybind()
 
 
 _ensure_built()

@@ -320,3 +320,59 @@ def test_mma_matches_oracle(orc, m):
         xo, its = ref.update(it, x32, df.astype(np.float64), g, dg.astype(np.float64))
         say("  it", it, "newton gpu", gpu.last_newton, "oracle", its, "max diff", float(np.abs(xg - xo).max()))
         assert np.abs(xg - xo).max() < 2e-6
+
+
+def test_run_custom_mma_matches_oracle_loop(orc):
+    """runCustom's MMA branch (Framework.cu:171-201): minimise the volume
+    subject to the tensor error, three updates, against the FP64 oracle loop
+    from the same start design."""
+    n = 16
+    say("runCustom mma", n)
+    k0, k1 = 1.0, 1e-4
+    t = np.array([0.3, 0.2, 0.1, 0.0, 0.0, 0.0])
+    rho0 = wl.oc_start_density(n, 3, mean=0.5, amplitude=0.2)
+    iters = 3
+    rg, rep = otm.run_custom(n, "mma", iters, rho0=rho0, target=t, heat_ratio=(k0, k1), fem_rel_thres=1e-7)
+    assert rep["iterations"] == iters
+    ne = n ** 3
+    x = rho0.astype(np.float64).reshape(-1)
+    mma = orc.MMA(1, ne, 1.0, 0.0, 1e7, 1.0, 1e-4, 1.0)
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (2, 1), (0, 2)]
+    for itn in range(iters):
+        xf = x.reshape(rho0.shape)
+        rf = orc.filter_fwd(xf, 0, 1.5, False)
+        C, _, S, _ = orc.homogenize(k1 + (k0 - k1) * rf ** 3, rtol=1e-12, want_S=True)
+        dC = np.zeros((3, 3))
+        obj = -1e-4
+        for q, (i, j) in enumerate(pairs):
+            obj += (C[i, j] - t[q]) ** 2
+            dC[i, j] += 2 * (C[i, j] - t[q])
+        g = orc.filter_adj(orc.sensitivity(S, dC).reshape(rf.shape) * 3 * (k0 - k1) * rf ** 2, 0, 1.5, False)
+        x, _ = mma.update(itn, x, np.full(ne, 1.0 / ne), np.array([obj]), g.reshape(1, -1))
+    err = float(np.abs(rg - x).max())
+    say("  rho err", err)
+    assert err < 1e-4
+
+
+def test_run_custom_oc_and_openTM():
+    """runCustom's OC branch with VolumeGovernor/ConvergeChecker through the
+    C ABI and through the pybind openTM module (python/testpyd.py usage)."""
+    n = 16
+    say("runCustom oc", n)
+    rho, rep = otm.run_custom(n, "oc", 40, initway="IWP", target=(0.5, 0.4, 0.3, 0, 0, 0), heat_ratio=(1.0, 1e-4))
+    assert rho.shape == (n ** 3,) and rho.min() >= 1e-3 - 1e-7 and rho.max() <= 1.0
+    assert 1 <= rep["iterations"] <= 40
+    first, _ = otm.run_custom(n, "oc", 1, initway="IWP", target=(0.5, 0.4, 0.3, 0, 0, 0), heat_ratio=(1.0, 1e-4))
+    say("  objective after", rep["iterations"], rep["objective"])
+    from opentm_amd import openTM
+    h = openTM.homo()
+    h.setConfig(n, [1.0, 1e-4], [0.5, 0.4, 0.3, 0.0, 0.0, 0.0])
+    arr = np.full((n, n, n), 0.5)
+    arr[np.sqrt(((np.indices((n, n, n)) - 7.5) ** 2).sum(axis=0)) <= n / 6] = 0.2
+    h.setDensity(arr.reshape(-1).tolist())
+    h.setMaxIter(5)
+    out = np.asarray(h.optimize(), np.float32)
+    assert out.shape == (n ** 3,) and out.min() >= 1e-3 - 1e-7 and out.max() <= 1.0
+    out2 = np.asarray(openTM.runInstance(n, [1.0, 1e-4], [0.5, 0.4, 0.3, 0.0, 0.0, 0.0], openTM.InitWay.IWP,
+                                         openTM.Model.oc), np.float32)
+    assert out2.shape == (n ** 3,)
