@@ -277,13 +277,16 @@ __global__ void __launch_bounds__(kBlock) k_stencil_from_elements(Geom g, const 
 }
 
 // Coarsest level: dense FP64 matrix from the stencil, symmetrised, zero row
-// sums, plus (tr/n^2) * 1 1^T to remove the constant null space; Cholesky
-// factor L (right-looking), X = L^{-1} (row by row), Binv = X^T X. One
-// 1024-thread workgroup; the n x n work matrices live in LDS when n <= 128
-// (W points into dynamic shared memory) and in global memory otherwise.
+// sums, plus (tr/n^2) * 1 1^T to remove the constant null space (the
+// reference solves its coarsest system on the host with Eigen and a
+// null-space projection, grid.cpp:272-310, :432-510). The result is its
+// explicit inverse Binv, so every V-cycle's coarsest solve is one small
+// dense matrix-vector product on the device.
 constexpr int kFactorThreads = 1024;
-__device__ inline void coarsest_factor_body(const Geom& g, const float* __restrict__ st, double* B, double* Xm,
-                                            double* Binv, double* red, double* alpha_sh) {
+constexpr int kInvertLdsMax = 128;  // n*n doubles in LDS up to n = 128 (128 KiB)
+
+__device__ inline void coarsest_assemble(const Geom& g, const float* __restrict__ st, double* B, double* red,
+                                         double* alpha_sh) {
   const int n = (int)g.nn;
   const int tid = threadIdx.x, bs = blockDim.x;
   for (int i = tid; i < n * n; i += bs) B[i] = 0.0;
@@ -319,14 +322,59 @@ __device__ inline void coarsest_factor_body(const Geom& g, const float* __restri
   __syncthreads();
   if (tid == 0) {
     double s = 0.0;
-    for (int w = 0; w < bs / 64; w++) s += red[w];
+    for (int w = 0; w < (bs + 63) / 64; w++) s += red[w];
     *alpha_sh = s / ((double)n * (double)n);
   }
   __syncthreads();
   const double alpha = *alpha_sh;
   for (int i = tid; i < n * n; i += bs) B[i] += alpha;
   __syncthreads();
-  // right-looking Cholesky, lower triangle in place
+}
+
+// n <= 128: in-place Gauss-Jordan inversion in LDS. The matrix is SPD, so no
+// pivoting is needed; every elimination step updates all n^2 entries in
+// parallel between two barriers (2n barriers in total instead of the long
+// dependent chains of a triangular inverse).
+__global__ void __launch_bounds__(kFactorThreads) k_coarsest_invert_lds(Geom g, const float* __restrict__ st,
+                                                                        double* __restrict__ Binv) {
+  extern __shared__ double A[];
+  __shared__ double colk[kInvertLdsMax], rowk[kInvertLdsMax];
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  const int n = (int)g.nn;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  coarsest_assemble(g, st, A, red, &alpha_sh);
+  const int rstep = bs / n;  // rows updated per pass (n <= bs)
+  const int j = tid % n, i0 = tid / n;
+  for (int k = 0; k < n; k++) {
+    const double ip = 1.0 / A[k * n + k];
+    for (int i = tid; i < n; i += bs) {
+      colk[i] = A[i * n + k];
+      rowk[i] = (i == k ? 1.0 : A[k * n + i]) * ip;
+    }
+    __syncthreads();
+    if (i0 < rstep)
+      for (int i = i0; i < n; i += rstep) {
+        const int q = i * n + j;
+        A[q] = (i == k) ? rowk[j] : ((j == k ? 0.0 : A[q]) - colk[i] * rowk[j]);
+      }
+    __syncthreads();
+  }
+  for (int q = tid; q < n * n; q += bs) {
+    const int a = q / n, b = q - a * n;
+    Binv[q] = 0.5 * (A[a * n + b] + A[b * n + a]);
+  }
+}
+
+// Larger coarsest levels (n <= 4096, work matrices in global memory):
+// right-looking Cholesky B = L L^T, X = L^{-1} row by row, Binv = X^T X.
+__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, const float* __restrict__ st, double* B,
+                                                                    double* Xm, double* Binv) {
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  const int n = (int)g.nn;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  coarsest_assemble(g, st, B, red, &alpha_sh);
   for (int k = 0; k < n; k++) {
     if (tid == 0) B[k * n + k] = sqrt(B[k * n + k]);
     __syncthreads();
@@ -335,44 +383,27 @@ __device__ inline void coarsest_factor_body(const Geom& g, const float* __restri
     __syncthreads();
     const int m = n - k - 1;
     for (int q = tid; q < m * m; q += bs) {
-      const int i = k + 1 + q / m, j = k + 1 + q % m;
-      if (j <= i) B[i * n + j] -= B[i * n + k] * B[j * n + k];
+      const int i = k + 1 + q / m, jj = k + 1 + q % m;
+      if (jj <= i) B[i * n + jj] -= B[i * n + k] * B[jj * n + k];
     }
     __syncthreads();
   }
-  // X = L^{-1}: row k from rows < k, columns in parallel
   for (int k = 0; k < n; k++) {
-    for (int j = tid; j < n; j += bs) {
-      double s = (j == k) ? 1.0 : 0.0;
-      if (j < k)
-        for (int m2 = j; m2 < k; m2++) s -= B[k * n + m2] * Xm[m2 * n + j];
-      Xm[k * n + j] = j <= k ? s / B[k * n + k] : 0.0;
+    for (int jj = tid; jj < n; jj += bs) {
+      double s = (jj == k) ? 1.0 : 0.0;
+      if (jj < k)
+        for (int m2 = jj; m2 < k; m2++) s -= B[k * n + m2] * Xm[m2 * n + jj];
+      Xm[k * n + jj] = jj <= k ? s / B[k * n + k] : 0.0;
     }
     __syncthreads();
   }
   for (int q = tid; q < n * n; q += bs) {
-    const int i = q / n, j = q - i * n;
-    const int m0 = i > j ? i : j;
+    const int i = q / n, jj = q - i * n;
+    const int m0 = i > jj ? i : jj;
     double s = 0.0;
-    for (int m2 = m0; m2 < n; m2++) s += Xm[m2 * n + i] * Xm[m2 * n + j];
+    for (int m2 = m0; m2 < n; m2++) s += Xm[m2 * n + i] * Xm[m2 * n + jj];
     Binv[q] = s;
   }
-}
-
-__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor_lds(Geom g, const float* __restrict__ st,
-                                                                        double* Binv) {
-  extern __shared__ double wsh[];
-  __shared__ double red[kFactorThreads / 64];
-  __shared__ double alpha_sh;
-  const int n = (int)g.nn;
-  coarsest_factor_body(g, st, wsh, wsh + n * n, Binv, red, &alpha_sh);
-}
-
-__global__ void __launch_bounds__(kFactorThreads) k_coarsest_factor(Geom g, const float* __restrict__ st, double* B,
-                                                                    double* Xm, double* Binv) {
-  __shared__ double red[kFactorThreads / 64];
-  __shared__ double alpha_sh;
-  coarsest_factor_body(g, st, B, Xm, Binv, red, &alpha_sh);
 }
 
 // u = Binv f per right-hand side, FP64 accumulation.
@@ -486,21 +517,15 @@ void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, 
 }
 void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv) {
   const int n = (int)g.nn;
-  const size_t lds = (size_t)2 * n * n * sizeof(double);
-  // one thread per row: a single wavefront for n <= 64 keeps the ~4n
-  // barriers of the factorisation cheap
-  int nt = ((n + 63) / 64) * 64;
-  if (nt > kFactorThreads) nt = kFactorThreads;
-  if (n <= 90) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)k_coarsest_factor_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(2 * 90 * 90 * sizeof(double)));
-      attr_set = true;
-    }
-    hipLaunchKernelGGL(k_coarsest_factor_lds, dim3(1), dim3(nt), lds, s, g, st, Binv);
+  if (n <= kInvertLdsMax) {
+    // per device and cheap; set on every call (once per design update)
+    (void)hipFuncSetAttribute((const void*)k_coarsest_invert_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(kInvertLdsMax * kInvertLdsMax * sizeof(double)));
+    int nt = ((n * n + 63) / 64) * 64;
+    if (nt > kFactorThreads) nt = kFactorThreads;
+    hipLaunchKernelGGL(k_coarsest_invert_lds, dim3(1), dim3(nt), (size_t)n * n * sizeof(double), s, g, st, Binv);
   } else {
-    hipLaunchKernelGGL(k_coarsest_factor, dim3(1), dim3(nt), 0, s, g, st, B, X, Binv);
+    hipLaunchKernelGGL(k_coarsest_factor, dim3(1), dim3(kFactorThreads), 0, s, g, st, B, X, Binv);
   }
 }
 bool brick_ok(const Geom& g) { return g.nx % kBrick == 0 && g.ny % kBrick == 0 && g.nz % kBrick == 0; }

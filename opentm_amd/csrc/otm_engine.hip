@@ -306,7 +306,7 @@ struct otm_oc {
   int max_steps = 20;
   double vol_tol = 1e-4;
   hipStream_t s = nullptr;
-  float *newrho = nullptr, *newsens = nullptr, *a = nullptr, *b = nullptr;
+  float *newsens = nullptr, *a = nullptr, *b = nullptr;
   unsigned* maxbits = nullptr;
   OcState* st = nullptr;
   OcState* st_host = nullptr;  // pinned
@@ -1126,7 +1126,7 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
 static void oc_free(otm_oc* o) {
   if (!o) return;
   if (o->s) (void)hipStreamSynchronize(o->s);
-  dfree(o->newrho); dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits);
+  dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits);
   dfree(o->st); dfree(o->partials);
   if (o->st_host) (void)hipHostFree(o->st_host);
   if (o->s) (void)hipStreamDestroy(o->s);
@@ -1146,13 +1146,12 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
     try {
       hip_check(hipSetDevice(device), "hipSetDevice");
       hip_check(hipStreamCreateWithFlags(&o->s, hipStreamNonBlocking), "hipStreamCreate");
-      o->newrho = dalloc<float>(ne);
       o->newsens = dalloc<float>(ne);
       o->a = dalloc<float>(ne);
       o->b = dalloc<float>(ne);
       o->maxbits = dalloc<unsigned>(1);
       o->st = dalloc<OcState>(1);
-      o->partials = dalloc<double>(2048);
+      o->partials = dalloc<double>((size_t)1024 * kOcCand);
       hip_check(hipHostMalloc((void**)&o->st_host, sizeof(OcState), hipHostMallocDefault), "hipHostMalloc");
     } catch (...) {
       oc_free(o);
@@ -1215,23 +1214,24 @@ static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volra
   R.rtol2 = 0;
   R.inv_nn = 0;
   R.mode = FIN_STORE;
-  // midpoints are queued in chunks without host round trips; midpoints after
-  // the one that meets the tolerance are skipped on the device, and the
-  // host checks the state once per chunk
+  // passes of kOcTree bisection levels are queued without host round trips
+  // (a pass after the tolerance is met returns at once); the host checks the
+  // state every few passes, then the update is applied at the final midpoint
   launch_oc_begin(o->s, n, sens, o->maxbits, o->st);
-  const int chunk = 24;
-  for (int done = 0; done < o->max_steps;) {
-    const int m = std::min(chunk, o->max_steps - done);
+  const int passes = (o->max_steps + kOcTree - 1) / kOcTree;
+  const int per_check = 4;
+  for (int done = 0; done < passes;) {
+    const int m = std::min(per_check, passes - done);
     for (int i = 0; i < m; i++)
-      launch_oc_bisect_step(o->s, n, sens, rho, o->newrho, o->st, (float)o->min_rho, (float)o->step,
-                            (float)o->damp, volratio, o->vol_tol, R, nb);
+      launch_oc_tree_pass(o->s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, volratio,
+                          o->vol_tol, o->max_steps, R, nb);
     done += m;
-    if (done >= o->max_steps) break;
+    if (done >= passes) break;
     hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
     hip_check(hipStreamSynchronize(o->s), "sync");
     if (o->st_host->done) break;
   }
-  hip_check(hipMemcpyAsync(rho, o->newrho, sizeof(float) * n, hipMemcpyDeviceToDevice, o->s), "copy rho");
+  launch_oc_apply(o->s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp);
   hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
   hip_check(hipStreamSynchronize(o->s), "sync");
   if (steps) *steps = o->st_host->steps;

@@ -308,8 +308,20 @@ __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, 
 // values and the result goes to a separate buffer (block Gauss-Seidel across
 // bricks, exact colour order inside); the post-smoother runs the colours in
 // reverse. Each thread updates two nodes of the active colour per step.
+//
+// A node couples to its 12 face-diagonal neighbours with the summed
+// conductivity of the two elements they share and to its 8 body-diagonal
+// neighbours with one element's conductivity (one-axis neighbours have a
+// zero coefficient), so an update is 20 LDS reads and 20 FMAs per load case
+// plus 12 adds shared by the three. Staging reads whole x-rows of the brick
+// (coalesced) and the two halo columns separately, with no per-element
+// index division.
 template <int BX>
-__global__ void __launch_bounds__(BX * 8) k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa,
+// The occupancy hint keeps the fully unrolled colour loop from holding
+// every node's loads in flight at once (256 VGPRs, one wave per SIMD); the
+// LDS footprint allows three workgroups per CU anyway.
+__global__ void __launch_bounds__(BX * 8) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                                const float* __restrict__ f,
                                                                const float* __restrict__ uin, float* __restrict__ uout,
                                                                int sweeps, int forward, int zero_in) {
@@ -323,39 +335,59 @@ __global__ void __launch_bounds__(BX * 8) k_fine_smooth_brick(Geom g, FineConsts
   const int b = xcd_block();
   const int ox = (b % bx) * BX, oy = ((b / bx) % by) * BY, oz = (b / (bx * by)) * BZ;
   const int t = threadIdx.x;
-  for (int i = t; i < KX * KY * KZ; i += NT) {
-    const int ix = i % KX, iy = (i / KX) % KY, iz = i / (KX * KY);
-    const int X = wrapm(ox - 1 + ix, g.nx), Y = wrapm(oy - 1 + iy, g.ny), Z = wrapm(oz - 1 + iz, g.nz);
-    sk[i] = kappa[X + (long long)g.nx * (Y + (long long)g.ny * Z)];
+  const int tx = t % BX, ty = t / BX;  // ty in [0, 8)
+  const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
+  const int xm = wrapm(ox - 1, g.nx), xp = wrapm(ox + BX, g.nx);
+  // kappa of element (ox-1+kx, oy-1+ky, oz-1+kz) at sk[kx + KX*(ky + KY*kz)]
+  for (int r = ty; r < KY * KZ; r += 8) {
+    const int ky = r % KY, kz = r / KY;
+    const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
+    sk[r * KX + 1 + tx] = kappa[row + ox + tx];
   }
-  for (int i = t; i < HX * HY * HZ; i += NT) {
-    if (zero_in) {
+  for (int r = t; r < KY * KZ; r += NT) {
+    const int ky = r % KY, kz = r / KY;
+    const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
+    sk[r * KX] = kappa[row + xm];
+  }
+  if (zero_in) {
+    for (int i = t; i < HX * HY * HZ; i += NT) {
 #pragma unroll
       for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
-    } else {
-      const int hx = i % HX, hy = (i / HX) % HY, hz = i / (HX * HY);
-      const int X = wrapm(ox - 1 + hx, g.nx), Y = wrapm(oy - 1 + hy, g.ny), Z = wrapm(oz - 1 + hz, g.nz);
-      const long long w = X + (long long)g.nx * (Y + (long long)g.ny * Z);
+    }
+  } else {
+    for (int r = ty; r < HY * HZ; r += 8) {
+      const int hy = r % HY, hz = r / HY;
+      const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
 #pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = uin[k * g.nn + w];
+      for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + 1 + tx] = uin[k * g.nn + row + ox + tx];
+    }
+    for (int i = t; i < 2 * HY * HZ; i += NT) {
+      const int r = i >> 1, side = i & 1;
+      const int hy = r % HY, hz = r / HY;
+      const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
+      const int hx = side ? HX - 1 : 0;
+      const long long w = row + (side ? xp : xm);
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + hx] = uin[k * g.nn + w];
     }
   }
-  // node ownership: x = t % BX, and for colour (cy, cz) the rows
-  // y = cy + 2*(j%4), z = cz + 2*(j/4) and z + 4, j = t / BX in [0, 8)
-  const int lx = t % BX, j = t / BX;
-  const int ry = 2 * (j & 3), rz = 2 * (j >> 2);
+  // node ownership: x = tx, and for colour (cy, cz) the rows
+  // y = cy + 2*(ty%4), z = cz + 2*(ty/4) and z + 4
+  const int lx = tx;
+  const int ry = 2 * (ty & 3), rz = 2 * (ty >> 2);
   float fv[4][2][OTM_NRHS];
 #pragma unroll
   for (int col = 0; col < 4; col++)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
-      const long long v = (ox + lx) + (long long)g.nx * ((oy + ly) + (long long)g.ny * (oz + lz));
+      const long long v = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
 #pragma unroll
       for (int k = 0; k < OTM_NRHS; k++) fv[col][h][k] = f[k * g.nn + v];
     }
   __syncthreads();
   const float c = fc.c;
+  constexpr int DY = HX, DZ = HX * HY;
   for (int s = 0; s < sweeps; s++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -363,36 +395,48 @@ __global__ void __launch_bounds__(BX * 8) k_fine_smooth_brick(Geom g, FineConsts
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
-        // kappa of the 8 elements around the node: element (lx-1+ax, ...) at sk(lx+ax, ly+ay, lz+az)
+        // kappa of the 8 elements around the node, e = ax + 2ay + 4az: element
+        // (x-1+ax, y-1+ay, z-1+az) at sk(lx+ax, ly+ay, lz+az)
+        const int kb = lx + KX * (ly + KY * lz);
         float ke[8];
-        float ks = 0.f;
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-          ke[e] = sk[(lx + (e & 1)) + KX * ((ly + ((e >> 1) & 1)) + KY * (lz + (e >> 2)))];
-          ks += ke[e];
+        for (int e = 0; e < 8; e++) ke[e] = sk[kb + (e & 1) + KX * (((e >> 1) & 1) + KY * (e >> 2))];
+        float wxy[4], wxz[4], wyz[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int a0 = q & 1, a1 = q >> 1;
+          wxy[q] = ke[a0 + 2 * a1] + ke[a0 + 2 * a1 + 4];  // neighbour (+-1, +-1, 0)
+          wxz[q] = ke[a0 + 4 * a1] + ke[a0 + 2 + 4 * a1];  // neighbour (+-1, 0, +-1)
+          wyz[q] = ke[2 * a0 + 4 * a1] + ke[1 + 2 * a0 + 4 * a1];  // neighbour (0, +-1, +-1)
         }
-        const float inv = 1.f / (4.f * c * ks);
-        const int c0 = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
+        const float ks = (wxy[0] + wxy[1]) + (wxy[2] + wxy[3]);
+        const float inv = __builtin_amdgcn_rcpf(4.f * c * ks);
+        const int c0 = (lx + 1) + DY * (ly + 1) + DZ * (lz + 1);
 #pragma unroll
         for (int k = 0; k < OTM_NRHS; k++) {
           const float* U = su[k];
           float acc = 0.f;
 #pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int s0 = 2 * (q & 1) - 1, s1 = 2 * (q >> 1) - 1;
+            acc += wxy[q] * U[c0 + s0 + DY * s1];
+            acc += wxz[q] * U[c0 + s0 + DZ * s1];
+            acc += wyz[q] * U[c0 + DY * s0 + DZ * s1];
+          }
+#pragma unroll
           for (int e = 0; e < 8; e++) {
-            const int sx = (e & 1) ? 1 : -1, sy = ((e >> 1) & 1) ? 1 : -1, sz = (e >> 2) ? 1 : -1;
-            const float T = U[c0 + sx + HX * sy] + U[c0 + sx + HX * HY * sz] + U[c0 + HX * sy + HX * HY * sz] +
-                            U[c0 + sx + HX * sy + HX * HY * sz];
-            acc += ke[e] * T;
+            const int sx = 2 * (e & 1) - 1, sy = 2 * ((e >> 1) & 1) - 1, sz = 2 * (e >> 2) - 1;
+            acc += ke[e] * U[c0 + sx + DY * sy + DZ * sz];
           }
           su[k][c0] = (fv[col][h][k] + c * acc) * inv;
         }
       }
       __syncthreads();
     }
-  for (int i = t; i < BX * BY * BZ; i += NT) {
-    const int lxx = i % BX, lyy = (i / BX) % BY, lzz = i / (BX * BY);
-    const long long v = (ox + lxx) + (long long)g.nx * ((oy + lyy) + (long long)g.ny * (oz + lzz));
-    const int c0 = (lxx + 1) + HX * ((lyy + 1) + HY * (lzz + 1));
+  for (int r = ty; r < BY * BZ; r += 8) {
+    const int ly = r % BY, lz = r / BY;
+    const long long v = (ox + tx) + nx * (oy + ly) + nxy * (oz + lz);
+    const int c0 = (tx + 1) + DY * (ly + 1) + DZ * (lz + 1);
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) uout[k * g.nn + v] = su[k][c0];
   }

@@ -90,9 +90,13 @@ struct InitModes {
 };
 void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes& M, float* rho);
 void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st);
-void launch_oc_bisect_step(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new,
-                           OcState* st, float min_rho, float step, float damp, double volratio, double vol_tol,
-                           const Reduce& R, unsigned nblocks);
+constexpr int kOcTree = 5;                   // bisection levels per pass over the design
+constexpr int kOcCand = (1 << kOcTree) - 1;  // midpoints evaluated per pass
+void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const float* rho, OcState* st,
+                         float min_rho, float step, float damp, double volratio, double vol_tol, int max_steps,
+                         const Reduce& R, unsigned nblocks);
+void launch_oc_apply(hipStream_t s, long long n, const float* sens, float* rho, const OcState* st, float min_rho,
+                     float step, float damp);
 
 // ---- MMA (otm_mma.hip) ----
 constexpr int kMmaMaxM = 4;  // constraints supported by the device subsolver

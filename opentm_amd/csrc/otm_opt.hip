@@ -108,16 +108,6 @@ __global__ void __launch_bounds__(kBlock) k_maxabs(long long n, const float* __r
   }
 }
 
-__device__ inline float oc_rule(float s, float rho, float g, float min_rho, float step, float damp) {
-  float B = -s / g;
-  if (B < 0.f) B = 0.01f;
-  float v = powf(B, damp) * rho;
-  if (v - rho < -step) v = rho - step;
-  if (v - rho > step) v = rho + step;
-  if (v < min_rho) v = min_rho;
-  if (v > 1.f) v = 1.f;
-  return v;
-}
 
 __global__ void k_oc_zero(unsigned int* bits) { *bits = 0u; }
 
@@ -130,43 +120,115 @@ __global__ void k_oc_init(const unsigned int* __restrict__ bits, OcState* st) {
   st->vol = 0.0;
 }
 
-// One bisection midpoint: rho_new from the update rule and partial volume
-// sums. Skipped entirely once an earlier midpoint met the tolerance.
-__global__ void __launch_bounds__(kBlock) k_oc_apply_sum(long long n, const float* __restrict__ sens,
-                                                         const float* __restrict__ rho, float* __restrict__ rho_new,
-                                                         const OcState* __restrict__ st, float min_rho, float step,
-                                                         float damp, Reduce R) {
-  if (st->done) return;
-  const float g = 0.5f * (st->hi + st->lo);
-  double acc[1] = {0.0};
-  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x) {
-    const float v = oc_rule(sens[i], rho[i], g, min_rho, step, damp);
-    rho_new[i] = v;
-    acc[0] += (double)v;
+// Speculative bisection: one pass over the design evaluates the midpoints of
+// the next kOcTree bisection levels at once (heap order: node i has children
+// 2i+1, "volume too small, hi = g", and 2i+2, "volume too large, lo = g"),
+// and a one-block kernel then walks the tree with the bracket rule of
+// OCOptimizer.cu:50-61. The midpoints are the FP32 midpoints the sequential
+// bisection would take, and each volume is the same FP64 reduction, so the
+// walk takes the sequential bisection's path kOcTree levels per pass.
+//
+// Update rule of OCOptimizer.cu:12-34 per element: B = -s/g (0.01 when
+// negative), rho' = clamp(B^damp * rho, rho -+ step, [min_rho, 1]). B^damp is
+// evaluated as 2^(damp (log2(-s) - log2 g)) so the logarithm of the
+// sensitivity is taken once per element for all midpoints.
+__device__ inline void oc_tree_mids(const OcState* st, float* g) {
+  float lo[kOcCand], hi[kOcCand];
+  lo[0] = st->lo;
+  hi[0] = st->hi;
+  for (int i = 0; i < kOcCand; i++) {
+    g[i] = 0.5f * (hi[i] + lo[i]);
+    if (2 * i + 2 < kOcCand) {
+      lo[2 * i + 1] = lo[i];
+      hi[2 * i + 1] = g[i];
+      lo[2 * i + 2] = g[i];
+      hi[2 * i + 2] = hi[i];
+    }
   }
-  reduce_block<1>(acc, R);
 }
 
-// Bracket update after a midpoint (OCOptimizer.cu:50-61).
-__global__ void __launch_bounds__(kBlock) k_oc_final(OcState* st, Reduce R, int nblocks, double inv_n,
-                                                     double volratio, double tol) {
-  if (st->done) return;
-  __shared__ double sh[kWaves];
-  double acc[1] = {0.0};
-  for (int b = threadIdx.x; b < nblocks; b += kBlock) acc[0] += R.partials[b];
-  block_sum<1>(acc, sh);
+__device__ inline float oc_rule_log(float s, float ls, float rho, float lg, float c01, float min_rho, float step,
+                                    float damp) {
+  float v = (s > 0.f ? c01 : __builtin_amdgcn_exp2f(damp * (ls - lg))) * rho;
+  if (v - rho < -step) v = rho - step;
+  if (v - rho > step) v = rho + step;
+  if (v < min_rho) v = min_rho;
+  if (v > 1.f) v = 1.f;
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_oc_tree_sum(long long n, const float* __restrict__ sens,
+                                                        const float* __restrict__ rho,
+                                                        const OcState* __restrict__ st, float min_rho, float step,
+                                                        float damp, float c01, int max_steps, Reduce R) {
+  if (st->done || st->steps >= max_steps) return;
+  __shared__ float slg[kOcCand];
+  if (threadIdx.x == 0) {
+    float g[kOcCand];
+    oc_tree_mids(st, g);
+    for (int c = 0; c < kOcCand; c++) slg[c] = __builtin_amdgcn_logf(g[c]);
+  }
+  __syncthreads();
+  double acc[kOcCand];
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) acc[c] = 0.0;
+  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x) {
+    const float sv = sens[i], r = rho[i];
+    const float ls = __builtin_amdgcn_logf(-sv);
+#pragma unroll
+    for (int c = 0; c < kOcCand; c++) acc[c] += (double)oc_rule_log(sv, ls, r, slg[c], c01, min_rho, step, damp);
+  }
+  reduce_block<kOcCand>(acc, R);
+}
+
+__global__ void __launch_bounds__(kBlock) k_oc_tree_final(OcState* st, Reduce R, int nblocks, double inv_n,
+                                                          double volratio, double tol, int max_steps) {
+  if (st->done || st->steps >= max_steps) return;
+  __shared__ double sh[kOcCand * kWaves];
+  __shared__ float g[kOcCand];
+  if (threadIdx.x == 0) oc_tree_mids(st, g);
+  double acc[kOcCand];
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) acc[c] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+#pragma unroll
+    for (int c = 0; c < kOcCand; c++) acc[c] += R.partials[(size_t)b * kOcCand + c];
+  block_sum<kOcCand>(acc, sh);  // barrier inside also publishes g
   if (threadIdx.x != 0) return;
-  const float g = 0.5f * (st->hi + st->lo);
-  const double vol = acc[0] * inv_n;
-  st->g = g;
-  st->vol = vol;
-  st->steps += 1;
-  if (vol < volratio - tol)
-    st->hi = g;
-  else if (vol > volratio + tol)
-    st->lo = g;
-  else
-    st->done = 1;
+  float lo = st->lo, hi = st->hi;
+  int node = 0, steps = st->steps;
+  for (int lev = 0; lev < kOcTree && steps < max_steps; lev++) {
+    const float gm = g[node];
+    const double vol = acc[node] * inv_n;
+    st->g = gm;
+    st->vol = vol;
+    steps++;
+    if (vol < volratio - tol) {
+      hi = gm;
+      node = 2 * node + 1;
+    } else if (vol > volratio + tol) {
+      lo = gm;
+      node = 2 * node + 2;
+    } else {
+      st->done = 1;
+      break;
+    }
+  }
+  st->lo = lo;
+  st->hi = hi;
+  st->steps = steps;
+}
+
+// rho <- update rule at the final multiplier (in place).
+__global__ void __launch_bounds__(kBlock) k_oc_apply(long long n, const float* __restrict__ sens, float* rho,
+                                                     const OcState* __restrict__ st, float min_rho, float step,
+                                                     float damp, float c01) {
+  if (st->steps == 0) return;
+  const float lg = __builtin_amdgcn_logf(st->g);
+  for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float sv = sens[i];
+    rho[i] = oc_rule_log(sv, __builtin_amdgcn_logf(-sv), rho[i], lg, c01, min_rho, step, damp);
+  }
 }
 
 // Density initialisers (Framework.cu:61-145). tanproj(v, 20, 0.5) is the
@@ -258,13 +320,21 @@ void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int
   launch_maxabs(s, n, sens, bits);
   hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, s, bits, st);
 }
-void launch_oc_bisect_step(hipStream_t s, long long n, const float* sens, const float* rho, float* rho_new,
-                           OcState* st, float min_rho, float step, float damp, double volratio, double vol_tol,
-                           const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_oc_apply_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, rho_new, st, min_rho, step,
-                     damp, R);
-  hipLaunchKernelGGL(k_oc_final, dim3(1), dim3(kBlock), 0, s, st, R, (int)nblocks, 1.0 / (double)n, volratio,
-                     vol_tol);
+void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const float* rho, OcState* st,
+                         float min_rho, float step, float damp, double volratio, double vol_tol, int max_steps,
+                         const Reduce& R, unsigned nblocks) {
+  const float c01 = powf(0.01f, damp);
+  hipLaunchKernelGGL(k_oc_tree_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01,
+                     max_steps, R);
+  hipLaunchKernelGGL(k_oc_tree_final, dim3(1), dim3(kBlock), 0, s, st, R, (int)nblocks, 1.0 / (double)n, volratio,
+                     vol_tol, max_steps);
+}
+void launch_oc_apply(hipStream_t s, long long n, const float* sens, float* rho, const OcState* st, float min_rho,
+                     float step, float damp) {
+  const float c01 = powf(0.01f, damp);
+  unsigned nb = nblk(n);
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(k_oc_apply, dim3(nb), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01);
 }
 
 }  // namespace otm
