@@ -62,6 +62,8 @@ typedef struct otm_config {
   int device;               /* HIP device ordinal */
   int verbose;              /* print per-solve statistics to stderr */
   int coarse_sweeps;        /* sweeps on levels >= 1 (0: same as the fine level) */
+  int fp64_refine;          /* 1: FP64 iterate with FP32 inner PCG solves (iterative refinement) */
+  int fp16_stencils;        /* 1: coarse stencils stored in FP16 with a per-level scale */
 } otm_config;
 
 /* cfg::HomoConfig::init (cmdline.cpp:4-35) defaults where they apply;

@@ -45,6 +45,8 @@ struct HomoConfig {
   int initperiod;
   int seed = 0;          // seed of the random initialisers (the reference seeds from the clock)
   int coarse_sweeps = 0;  // Gauss-Seidel sweeps on coarse levels (0: as on the fine level)
+  int fp64_refine = 0;    // FP64 iterate with FP32 inner solves (iterative refinement)
+  int fp16_stencils = 0;  // coarse stencils stored in FP16 with a per-level scale
   int device = 0;
   void init() {
     reso[0] = reso[1] = reso[2] = 128;
@@ -71,6 +73,8 @@ struct HomoConfig {
     target_tensor[3] = target_tensor[4] = target_tensor[5] = 0.;
     seed = 0;
     coarse_sweeps = 0;
+    fp64_refine = 0;
+    fp16_stencils = 0;
     device = 0;
   }
   otm_config engine() const {
@@ -81,6 +85,8 @@ struct HomoConfig {
     c.heat_ratio[1] = heatRatio[1];
     c.fem_rel_thres = femRelThres;
     c.coarse_sweeps = coarse_sweeps;
+    c.fp64_refine = fp64_refine;
+    c.fp16_stencils = fp16_stencils;
     c.device = device;
     return c;
   }

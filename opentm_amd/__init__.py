@@ -113,6 +113,8 @@ class HomoConfig:
         self.post_sweeps = 2
         self.min_coarse = 4
         self.coarse_sweeps = 0
+        self.fp64_refine = 0
+        self.fp16_stencils = 0
         self.device = 0
         self.verbose = 0
 
@@ -128,6 +130,8 @@ class HomoConfig:
         c.post_sweeps = self.post_sweeps
         c.min_coarse = self.min_coarse
         c.coarse_sweeps = self.coarse_sweeps
+        c.fp64_refine = self.fp64_refine
+        c.fp16_stencils = self.fp16_stencils
         c.device = self.device
         c.verbose = self.verbose
         return c

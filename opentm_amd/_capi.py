@@ -44,6 +44,8 @@ class OtmConfig(ctypes.Structure):
         ("device", _c.c_int),
         ("verbose", _c.c_int),
         ("coarse_sweeps", _c.c_int),
+        ("fp64_refine", _c.c_int),
+        ("fp16_stencils", _c.c_int),
     ]
 
 

@@ -22,16 +22,37 @@ __device__ inline long long lin(const Geom& g, int x, int y, int z) {
 }
 
 // ---- per-node operations of the stored-stencil levels ----
+// Stencil entries of node v. FP16 storage (HALF) keeps the 26 couplings
+// divided by a per-level scale; the diagonal is rebuilt as minus their sum,
+// so rows keep a zero sum and the constants stay in the null space.
+template <bool HALF>
+__device__ inline void load_stencil(const StRef& S, long long nn, long long v, float a[27]) {
+  if constexpr (HALF) {
+    const float sc = *S.scale;
+    float off = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+      if (o == 13) continue;
+      a[o] = (float)S.f16[o * nn + v] * sc;
+      off += a[o];
+    }
+    a[13] = -off;
+  } else {
+#pragma unroll
+    for (int o = 0; o < 27; o++) a[o] = S.f32[o * nn + v];
+  }
+}
+
 // GS update at node (x,y,z): u = (f - sum_{o != 13} a_o u_o) / a_13.
-__device__ inline void st_gs_node(const Geom& g, const float* __restrict__ st, const float* __restrict__ f,
-                                  float* u, int x, int y, int z) {
+template <bool HALF>
+__device__ inline void st_gs_node(const Geom& g, const StRef& S, const float* __restrict__ f, float* u, int x, int y,
+                                  int z) {
   const long long v = lin(g, x, y, z);
   const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
   const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
   const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
   float a[27];
-#pragma unroll
-  for (int o = 0; o < 27; o++) a[o] = st[o * g.nn + v];
+  load_stencil<HALF>(S, g.nn, v, a);
   const float inv = 1.f / a[13];
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) {
@@ -47,8 +68,9 @@ __device__ inline void st_gs_node(const Geom& g, const float* __restrict__ st, c
 }
 
 // res = f - A u at node t.
-__device__ inline void st_resid_node(const Geom& g, const float* __restrict__ st, const float* __restrict__ f,
-                                     const float* u, float* res, long long t) {
+template <bool HALF>
+__device__ inline void st_resid_node(const Geom& g, const StRef& S, const float* __restrict__ f, const float* u,
+                                     float* res, long long t) {
   const int x = (int)(t % g.nx);
   const long long r = t / g.nx;
   const int y = (int)(r % g.ny), z = (int)(r / g.ny);
@@ -56,8 +78,7 @@ __device__ inline void st_resid_node(const Geom& g, const float* __restrict__ st
   const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
   const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
   float a[27];
-#pragma unroll
-  for (int o = 0; o < 27; o++) a[o] = st[o * g.nn + t];
+  load_stencil<HALF>(S, g.nn, t, a);
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) {
     const float* uk = u + k * g.nn;
@@ -130,9 +151,9 @@ __device__ inline void prolong_node(const Geom& fg, const Geom& cg, const float*
 
 // ---- per-level kernels ----
 // GS update of one of 8 colours (parity of x, y, z), colour = cx + 2cy + 4cz.
-__global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __restrict__ st,
-                                                        const float* __restrict__ f, float* u, int cx, int cy,
-                                                        int cz) {
+template <bool HALF>
+__global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, StRef S, const float* __restrict__ f, float* u, int cx,
+                                                        int cy, int cz) {
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
   const long long cnt = (long long)hx * hy * hz;
   const long long t = gtid();
@@ -140,15 +161,61 @@ __global__ void __launch_bounds__(kBlock) k_st_gs_color(Geom g, const float* __r
   const int x = 2 * (int)(t % hx) + cx;
   const long long r = t / hx;
   const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
-  st_gs_node(g, st, f, u, x, y, z);
+  st_gs_node<HALF>(g, S, f, u, x, y, z);
 }
 
-__global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, const float* __restrict__ st,
-                                                        const float* __restrict__ f, const float* __restrict__ u,
-                                                        float* __restrict__ res) {
+template <bool HALF>
+__global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, StRef S, const float* __restrict__ f,
+                                                        const float* __restrict__ u, float* __restrict__ res) {
   const long long t = gtid();
   if (t >= g.nn) return;
-  st_resid_node(g, st, f, u, res, t);
+  st_resid_node<HALF>(g, S, f, u, res, t);
+}
+
+// Largest |coupling| of a level (as float bits: non-negative floats order
+// like their bit patterns), for the FP16 scale.
+__global__ void __launch_bounds__(kBlock) k_stencil_absmax(Geom g, const float* __restrict__ st,
+                                                           unsigned int* __restrict__ bits) {
+  float m = 0.f;
+  for (long long v = gtid(); v < g.nn; v += (long long)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int o = 0; o < 27; o++)
+      if (o != 13) m = fmaxf(m, fabsf(st[o * g.nn + v]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+  __shared__ float wm[kWaves];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; w++) m = fmaxf(m, wm[w]);
+    atomicMax(bits, __float_as_uint(m));
+  }
+}
+
+// FP16 copy of a level's couplings divided by the level scale. Coupling
+// (v, v+d) for d before the centre in offset order is taken from the
+// neighbour's entry (v+d, v), so the stored operator is exactly symmetric.
+__global__ void __launch_bounds__(kBlock) k_stencil_pack(Geom g, const float* __restrict__ st,
+                                                         const float* __restrict__ scale, _Float16* __restrict__ st16) {
+  const long long t = gtid();
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  const float sc = *scale;
+  const float inv = sc > 0.f ? 1.f / sc : 0.f;
+#pragma unroll
+  for (int o = 0; o < 27; o++) {
+    float val = 0.f;
+    if (o > 13) {
+      val = st[o * g.nn + t];
+    } else if (o < 13) {
+      const long long w = lin(g, wrapm(x + o % 3 - 1, g.nx), wrapm(y + (o / 3) % 3 - 1, g.ny),
+                              wrapm(z + o / 9 - 1, g.nz));
+      val = st[(26 - o) * g.nn + w];
+    }
+    st16[o * g.nn + t] = (_Float16)(val * inv);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_restrict(Geom fg, Geom cg, const float* __restrict__ res,
@@ -431,8 +498,8 @@ __global__ void __launch_bounds__(kBlock) k_coarsest_solve(Geom g, const double*
 constexpr int kBrick = 8;
 constexpr int kBrickThreads = kBrick * kBrick * kBrick;
 constexpr int kHalo = kBrick + 2;
-__global__ void __launch_bounds__(kBrickThreads) k_st_smooth_brick(Geom g, const float* __restrict__ st,
-                                                                   const float* __restrict__ f,
+template <bool HALF>
+__global__ void __launch_bounds__(kBrickThreads) k_st_smooth_brick(Geom g, StRef S, const float* __restrict__ f,
                                                                    const float* __restrict__ uin,
                                                                    float* __restrict__ uout, int sweeps,
                                                                    int forward, int zero_in) {
@@ -459,8 +526,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_st_smooth_brick(Geom g, const
     }
   }
   float a[27];
-#pragma unroll
-  for (int o = 0; o < 27; o++) a[o] = st[o * (long long)nn + v];
+  load_stencil<HALF>(S, nn, v, a);
   float fv[OTM_NRHS];
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) fv[k] = f[k * (long long)nn + v];
@@ -491,13 +557,28 @@ __global__ void __launch_bounds__(kBrickThreads) k_st_smooth_brick(Geom g, const
 }
 
 // ---- launchers ----
-void launch_st_gs_color(hipStream_t s, const Geom& g, const float* st, const float* f, float* u, int color) {
+void launch_st_gs_color(hipStream_t s, const Geom& g, const StRef& S, const float* f, float* u, int color) {
   const long long cnt = (long long)(g.nx / 2) * (g.ny / 2) * (g.nz / 2);
-  hipLaunchKernelGGL(k_st_gs_color, dim3(nblk(cnt)), dim3(kBlock), 0, s, g, st, f, u, color & 1, (color >> 1) & 1,
-                     color >> 2);
+  if (S.f16)
+    hipLaunchKernelGGL(k_st_gs_color<true>, dim3(nblk(cnt)), dim3(kBlock), 0, s, g, S, f, u, color & 1,
+                       (color >> 1) & 1, color >> 2);
+  else
+    hipLaunchKernelGGL(k_st_gs_color<false>, dim3(nblk(cnt)), dim3(kBlock), 0, s, g, S, f, u, color & 1,
+                       (color >> 1) & 1, color >> 2);
 }
-void launch_st_residual(hipStream_t s, const Geom& g, const float* st, const float* f, const float* u, float* res) {
-  hipLaunchKernelGGL(k_st_residual, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, st, f, u, res);
+void launch_st_residual(hipStream_t s, const Geom& g, const StRef& S, const float* f, const float* u, float* res) {
+  if (S.f16)
+    hipLaunchKernelGGL(k_st_residual<true>, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, S, f, u, res);
+  else
+    hipLaunchKernelGGL(k_st_residual<false>, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, S, f, u, res);
+}
+void launch_stencil_pack(hipStream_t s, const Geom& g, const float* st, float* scale, _Float16* st16) {
+  unsigned int* bits = reinterpret_cast<unsigned int*>(scale);
+  launch_fill(s, scale, 1, 0.f);
+  unsigned nb = nblk(g.nn);
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(k_stencil_absmax, dim3(nb), dim3(kBlock), 0, s, g, st, bits);
+  hipLaunchKernelGGL(k_stencil_pack, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, st, scale, st16);
 }
 void launch_restrict(hipStream_t s, const Geom& fine, const Geom& coarse, const float* res, float* fc) {
   hipLaunchKernelGGL(k_restrict, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, res, fc);
@@ -529,11 +610,15 @@ void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, doubl
   }
 }
 bool brick_ok(const Geom& g) { return g.nx % kBrick == 0 && g.ny % kBrick == 0 && g.nz % kBrick == 0; }
-void launch_st_smooth_brick(hipStream_t s, const Geom& g, const float* st, const float* f, const float* uin,
+void launch_st_smooth_brick(hipStream_t s, const Geom& g, const StRef& S, const float* f, const float* uin,
                             float* uout, int sweeps, bool forward, bool zero_in) {
   const unsigned nb = (unsigned)((g.nx / kBrick) * (g.ny / kBrick) * (g.nz / kBrick));
-  hipLaunchKernelGGL(k_st_smooth_brick, dim3(nb), dim3(kBrickThreads), 0, s, g, st, f, uin, uout, sweeps,
-                     forward ? 1 : 0, zero_in ? 1 : 0);
+  if (S.f16)
+    hipLaunchKernelGGL(k_st_smooth_brick<true>, dim3(nb), dim3(kBrickThreads), 0, s, g, S, f, uin, uout, sweeps,
+                       forward ? 1 : 0, zero_in ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_st_smooth_brick<false>, dim3(nb), dim3(kBrickThreads), 0, s, g, S, f, uin, uout, sweeps,
+                       forward ? 1 : 0, zero_in ? 1 : 0);
 }
 void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u) {
   hipLaunchKernelGGL(k_coarsest_solve, dim3(nblk(g.nn), OTM_NRHS), dim3(kBlock), 0, s, g, Binv, f, u);

@@ -52,6 +52,7 @@ struct PcgScalars {
   double alpha[OTM_NRHS];
   double beta[OTM_NRHS];
   double mean[OTM_NRHS];   // scratch for mean removal
+  double thr[OTM_NRHS];    // r.r below which the current recurrence stops
   int active[OTM_NRHS];    // 1 while the right-hand side is iterating
   int fault;               // bit 0: NaN/Inf in a reduction; bit 1: preconditioner not positive
 };
@@ -63,7 +64,7 @@ enum FinalMode : int {
   FIN_RZ_FIRST = 2,  // sums = r.z x3: rz = sum, beta = 0
   FIN_RZ = 3,        // sums = r.z x3: beta = sum/rz, rz = sum
   FIN_PQ = 4,        // sums = p.q x3: alpha = rz/pq
-  FIN_RR = 5,        // sums = r.r x3: update activity against rtol
+  FIN_RR = 5,        // sums = r.r x3: update activity against thr
   FIN_MEAN = 6,      // sums = sum(x) x3: mean = sum / nn
 };
 
@@ -73,6 +74,7 @@ struct Reduce {
   PcgScalars* pcg;       // for PCG modes
   double rtol2;          // squared relative tolerance for FIN_RR
   double inv_nn;         // 1/nn for FIN_MEAN
+  double inner2 = 0.0;   // FIN_BB_RR: also stop once r.r falls by this factor (refinement)
   int mode;
 };
 
@@ -180,6 +182,10 @@ __device__ inline void finalize(const double* acc, int nv, const Reduce& R) {
         P->rr[k] = rr;
         if (!(acc[OTM_NRHS + k] - acc[OTM_NRHS + k] == 0.0)) P->fault |= 1;
         P->active[k] = (rr > R.rtol2 * bb) ? 1 : 0;
+        // an inner solve of the FP64 iterative refinement stops at a fixed
+        // reduction of its own starting residual; a plain solve at rtol
+        const double t_abs = R.rtol2 * bb, t_rel = R.inner2 * rr;
+        P->thr[k] = t_abs > t_rel ? t_abs : t_rel;
         P->alpha[k] = 0.0;
         P->beta[k] = 0.0;
       }
@@ -209,9 +215,8 @@ __device__ inline void finalize(const double* acc, int nv, const Reduce& R) {
     case FIN_RR:
       for (int k = 0; k < OTM_NRHS; k++) {
         P->rr[k] = acc[k];
-        double bb = P->bb[k] > 0 ? P->bb[k] : 1.0;
         if (!(acc[k] - acc[k] == 0.0)) P->fault |= 1;  // NaN or Inf
-        if (!(acc[k] > R.rtol2 * bb)) P->active[k] = 0;
+        if (!(acc[k] > P->thr[k])) P->active[k] = 0;
       }
       break;
     case FIN_MEAN:

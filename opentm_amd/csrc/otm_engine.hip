@@ -181,9 +181,13 @@ struct Level {
   float* st = nullptr;   // 27-point stencil (levels >= 1)
   float* E = nullptr;    // element matrices, 36 per element (levels >= 1)
   float* u2 = nullptr;   // second correction buffer for the brick smoother
+  _Float16* st16 = nullptr;  // FP16 couplings (fp16_stencils)
+  float* scale = nullptr;    // their per-level scale (device scalar)
   bool brick = false;    // smoothed with LDS bricks (edges multiples of 8)
   bool own_uf = true;
 };
+
+inline StRef stref(const Level& L) { return StRef{L.st, L.st16, L.scale}; }
 
 }  // namespace
 
@@ -197,7 +201,8 @@ struct otm_ctx {
   std::vector<Level> lv;
   // fine-level PCG state, 3 right-hand sides each
   float* kappa = nullptr;  // per element, with the 1e-9 floor
-  float* x = nullptr;      // chi (warm start)
+  float* x = nullptr;      // chi (warm start); the FP32 correction while refining
+  double* x64 = nullptr;   // FP64 chi of the iterative refinement (fp64_refine)
   float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float* z2 = nullptr;  // second fine correction buffer (fine brick smoother)
   // coarsest dense solve
@@ -325,9 +330,11 @@ static void ctx_free(otm_ctx* c) {
     dfree(L.res);
     dfree(L.st);
     dfree(L.E);
+    dfree(L.st16);
+    dfree(L.scale);
     if (L.own_uf) dfree(L.u2);
   }
-  dfree(c->kappa); dfree(c->x); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
+  dfree(c->kappa); dfree(c->x); dfree(c->x64); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
   dfree(c->cB); dfree(c->cX); dfree(c->cBinv);
   dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
@@ -391,6 +398,10 @@ static void build_hierarchy(otm_ctx* c) {
       L.f = dalloc<float>(3 * n);
       L.st = dalloc<float>(27 * n);
       L.E = dalloc<float>(36 * n);
+      if (cfg.fp16_stencils && l + 1 < geoms.size()) {
+        L.st16 = dalloc<_Float16>(27 * n);
+        L.scale = dalloc<float>(1);
+      }
       if (l + 1 < geoms.size() && brick_ok(L.g) && cfg.pre_sweeps == cfg.post_sweeps) {
         L.brick = true;
         L.u2 = dalloc<float>(3 * n);
@@ -410,6 +421,10 @@ static void build_hierarchy(otm_ctx* c) {
   hip_check(hipHostMalloc((void**)&c->out_host, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
   launch_fill(c->s, c->x, 3 * n0, 0.f);
   launch_fill(c->s, c->p, 3 * n0, 0.f);
+  if (cfg.fp64_refine) {
+    c->x64 = dalloc<double>(3 * n0);
+    launch_fill_d(c->s, c->x64, 3 * n0, 0.0);
+  }
   hip_check(hipGetLastError(), "initial fill");
   const double h = 1.0 / cfg.reso[0];  // element side (homogenization.cpp:47 passes h/2 as half-length)
   c->fc = make_fine_consts(h);
@@ -427,6 +442,7 @@ static void build_operators(otm_ctx* c) {
     else
       launch_galerkin_next(c->s, c->lv[l - 1].g, L.g, c->lv[l - 1].E, L.E);
     launch_stencil_from_elements(c->s, L.g, L.E, L.st);
+    if (L.st16) launch_stencil_pack(c->s, L.g, L.st, L.scale, L.st16);
   }
   const Level& Lc = c->lv.back();
   launch_coarsest_factor(c->s, Lc.g, Lc.st, c->cB, c->cX, c->cBinv);
@@ -448,7 +464,7 @@ static void smooth(otm_ctx* c, size_t l, int sweeps, bool forward) {
       } else {
         // stencil rows of the colour 27*4/8 + other colours 3*4*7/8 + f and u 2*3*4/8
         ProfScope ps(c, OTM_PROF_COARSE_GS, 27.0 * nn);
-        launch_st_gs_color(c->s, L.g, L.st, L.f, L.u, col);
+        launch_st_gs_color(c->s, L.g, stref(L), L.f, L.u, col);
       }
     }
 }
@@ -469,7 +485,7 @@ static void vcycle(otm_ctx* c, size_t l) {
   } else if (L.brick) {
     // pre-smoothing from a zero guess straight into u
     ProfScope ps(c, OTM_PROF_COARSE_GS, 132.0 * nn);  // stencil 108 + f 12 + u 12 bytes per node
-    launch_st_smooth_brick(c->s, L.g, L.st, L.f, nullptr, L.u, nsw, true, true);
+    launch_st_smooth_brick(c->s, L.g, stref(L), L.f, nullptr, L.u, nsw, true, true);
   } else {
     {
       ProfScope ps(c, OTM_PROF_VECTOR, 12.0 * nn);
@@ -482,7 +498,7 @@ static void vcycle(otm_ctx* c, size_t l) {
     launch_fine_residual(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.res);
   } else {
     ProfScope ps(c, OTM_PROF_COARSE_RESID, 144.0 * nn);
-    launch_st_residual(c->s, L.g, L.st, L.f, L.u, L.res);
+    launch_st_residual(c->s, L.g, stref(L), L.f, L.u, L.res);
   }
   Level& C = c->lv[l + 1];
   {
@@ -503,7 +519,7 @@ static void vcycle(otm_ctx* c, size_t l) {
   } else if (L.brick) {
     // post-smoothing reads u (frozen halos) and writes u2; swap so u holds the result
     ProfScope ps(c, OTM_PROF_COARSE_GS, 155.0 * nn);  // plus the u halo read, 12 * 10^3/8^3
-    launch_st_smooth_brick(c->s, L.g, L.st, L.f, L.u, L.u2, nsw, false, false);
+    launch_st_smooth_brick(c->s, L.g, stref(L), L.f, L.u, L.u2, nsw, false, false);
     std::swap(L.u, L.u2);
   } else {
     smooth(c, l, nsw, false);
@@ -513,23 +529,43 @@ static void vcycle(otm_ctx* c, size_t l) {
 // ---- PCG over the three load cases -------------------------------------------
 // Three preconditioned CG recurrences in lock step, one per load case, with
 // the V-cycle as preconditioner (the reference iterates the V-cycle as a
-// stationary method instead, MG.cpp:80-142). The starting residual is
-// computed in FP64 and made mean-free; the true residual of the final
-// iterate is recomputed and the iteration restarts from it when the FP32
-// recurrence drifted above the tolerance (residual correction).
+// stationary method instead, MG.cpp:80-142). Each restart computes the true
+// residual in FP64 and makes it mean-free:
+// - plain solve: the iterate x is FP32 and the iteration restarts from the
+//   recomputed residual while it still improves (residual correction);
+// - fp64_refine: the iterate is kept in FP64 (x64) and each restart solves
+//   for an FP32 correction (in x) down to a fixed reduction of its own start
+//   residual, then adds it to x64 (iterative refinement), which reaches
+//   relative residuals far below the FP32 resolution of the solution.
 // Returns false when the preconditioner turned out not to be positive.
+constexpr double kRefineInner = 1e-4;  // inner reduction per refinement step
 static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
   const long long n = c->g0.nn;
   const unsigned nb = red_blocks_for(n);
   const double rtol = c->cfg.fem_rel_thres;
   const double rtol2 = rtol * rtol;
   const double inv_n = 1.0 / (double)n;
+  const bool refine = c->x64 != nullptr;
+  const int max_restarts = refine ? 8 : 3;
   int it = 0;
   double prev_worst = 1e300;
-  for (int restart = 0; restart < 3; restart++) {
+  bool pending = false;  // a correction in x not yet added to x64
+  for (int restart = 0; restart < max_restarts; restart++) {
     {
-      ProfScope ps(c, OTM_PROF_FINE_RESID, 28.0 * (double)n);
-      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2, inv_n), nb);
+      ProfScope ps(c, OTM_PROF_FINE_RESID, (refine ? 56.0 : 28.0) * (double)n);
+      if (refine) {
+        if (pending)
+          launch_refine_add(c->s, 3 * n, c->x64, c->x);
+        else
+          launch_fill(c->s, c->x, 3 * n, 0.f);
+        pending = false;
+        Reduce R = c->reducer(FIN_BB_RR, rtol2, inv_n);
+        R.inner2 = kRefineInner * kRefineInner;
+        launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, R, nb);
+      } else {
+        launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2, inv_n),
+                                  nb);
+      }
     }
     c->read_pcg();
     if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
@@ -555,6 +591,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
       launch_fill(c->s, c->p, 3 * n, 0.f);
       launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
     }
+    pending = refine;
     while (it < c->cfg.max_cg_iter) {
       {
         ProfScope ps(c, OTM_PROF_FINE_APPLY, 28.0 * (double)n);
@@ -571,6 +608,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
       if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
       if (c->pcg_host->fault & 2) {
         *total = it;
+        if (pending) launch_fill(c->s, c->x, 3 * n, 0.f);  // drop the partial correction
         return false;
       }
       if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
@@ -582,6 +620,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
       }
     }
   }
+  if (pending) launch_refine_add(c->s, 3 * n, c->x64, c->x);
   *total = it;
   return true;
 }
@@ -609,14 +648,22 @@ static void pcg_solve(otm_ctx* c) {
     it += it2;
   }
   // true residual of the final iterate
-  launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r,
-                            c->reducer(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n), nb);
+  const Reduce Rf = c->reducer(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n);
+  if (c->x64)
+    launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, Rf, nb);
+  else
+    launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, Rf, nb);
   c->read_pcg();
   double relres[3];
   for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
   // remove the constant component (translateForce, grid.cpp:654-664)
-  launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
-  launch_sub_mean3(c->s, n, c->x, c->pcg);
+  if (c->x64) {
+    launch_sum3(c->s, n, c->x64, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sub_mean_cast(c->s, n, c->x64, c->x, c->pcg);
+  } else {
+    launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sub_mean3(c->s, n, c->x, c->pcg);
+  }
   c->read_pcg();
   for (int k = 0; k < 3; k++) {
     c->relres[k] = relres[k];
@@ -682,6 +729,8 @@ OTM_API void otm_config_init(otm_config* cfg) {
   cfg->device = 0;
   cfg->verbose = 0;
   cfg->coarse_sweeps = 0;
+  cfg->fp64_refine = 0;
+  cfg->fp16_stencils = 0;
 }
 
 OTM_API const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -881,6 +930,7 @@ OTM_API int otm_set_chi(otm_ctx* ctx, const float* chi, int where) {
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     hip_check(hipMemcpyAsync(ctx->x, chi, sizeof(float) * 3 * ctx->g0.nn, kind_of(OTM_DEVICE, where), ctx->s),
               "copy chi");
+    if (ctx->x64) launch_f2d(ctx->s, 3 * ctx->g0.nn, ctx->x, ctx->x64);
     ctx->sync();
   });
 }
@@ -890,6 +940,7 @@ OTM_API int otm_reset_displacement(otm_ctx* ctx) {
     if (!ctx) fail(OTM_E_ARG, "null context");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     launch_fill(ctx->s, ctx->x, 3 * ctx->g0.nn, 0.f);
+    if (ctx->x64) launch_fill_d(ctx->s, ctx->x64, 3 * ctx->g0.nn, 0.0);
     ctx->sync();
   });
 }
@@ -913,7 +964,7 @@ OTM_API int otm_apply_level(otm_ctx* ctx, int level, const float* u, float* q, i
       if (level == 0)
         launch_fine_residual(ctx->s, L.g, ctx->fc, ctx->kappa, dz, du, dq);
       else
-        launch_st_residual(ctx->s, L.g, L.st, dz, du, dq);
+        launch_st_residual(ctx->s, L.g, stref(L), dz, du, dq);
       std::vector<float> h((size_t)(3 * n));
       hip_check(hipMemcpyAsync(h.data(), dq, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, ctx->s), "copy q");
       ctx->sync();

@@ -86,8 +86,9 @@ __device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems
 }
 
 // FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
+template <typename XT>
 __device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeElems& E,
-                                   const float* __restrict__ u) {
+                                   const XT* __restrict__ u) {
   const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
   const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
   const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
@@ -186,9 +187,11 @@ __global__ void __launch_bounds__(kBlock) k_fine_apply_dot(Geom g, FineConsts fc
 // r = b - K x in FP64 with b the load vectors built on the fly (the FP64
 // sum of the eight element contributions cancels exactly wherever kappa
 // does not vary along the load direction); stored rounded to FP32. Partial
-// sums of b.b, r.r and sum(r) (PCG start and restarts).
+// sums of b.b, r.r and sum(r) (PCG start and restarts). XT is the iterate's
+// type: FP32, or FP64 for the iterative refinement.
+template <typename XT>
 __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
-                                                               const float* __restrict__ x_, float* __restrict__ rvec,
+                                                               const XT* __restrict__ x_, float* __restrict__ rvec,
                                                                Reduce R) {
   double acc[3 * OTM_NRHS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const double c = (double)fc.c;
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConst
     for (int e = 0; e < 8; e++) ksum += (double)E.k[e];
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
-      const float* xk = x_ + k * g.nn;
+      const XT* xk = x_ + k * g.nn;
       double b = 0.0;
 #pragma unroll
       for (int e = 0; e < 8; e++) b += (double)E.k[e] * (double)fc.femu[7 - e][k];
@@ -483,7 +486,13 @@ void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, c
 
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const float* x, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
+  hipLaunchKernelGGL(k_fine_init_residual<float>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
+  launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
+}
+
+void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
+                               const double* x, float* r, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_fine_init_residual<double>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 

@@ -14,6 +14,8 @@ void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, c
                            float* q, const Reduce& R, unsigned nblocks);
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const float* x, float* r, const Reduce& R, unsigned nblocks);
+void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
+                               const double* x, float* r, const Reduce& R, unsigned nblocks);
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
                          const Reduce& R, unsigned nblocks);
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
@@ -26,8 +28,16 @@ void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc
                               const float* uin, float* uout, int sweeps, bool forward, bool zero_in);
 
 // ---- coarse levels (stored 27-point stencils, otm_coarse.hip) ----
-void launch_st_gs_color(hipStream_t s, const Geom& g, const float* st, const float* f, float* u, int color);
-void launch_st_residual(hipStream_t s, const Geom& g, const float* st, const float* f, const float* u, float* res);
+// Stored stencil of a coarse level: FP32 (f16 == nullptr) or FP16 couplings
+// scaled by a per-level factor held on the device.
+struct StRef {
+  const float* f32;
+  const _Float16* f16;
+  const float* scale;
+};
+void launch_stencil_pack(hipStream_t s, const Geom& g, const float* st, float* scale, _Float16* st16);
+void launch_st_gs_color(hipStream_t s, const Geom& g, const StRef& S, const float* f, float* u, int color);
+void launch_st_residual(hipStream_t s, const Geom& g, const StRef& S, const float* f, const float* u, float* res);
 void launch_restrict(hipStream_t s, const Geom& fine, const Geom& coarse, const float* res, float* fc);
 void launch_prolong_add(hipStream_t s, const Geom& fine, const Geom& coarse, const float* uc, float* uf);
 void launch_galerkin_first(hipStream_t s, const Geom& fine, const Geom& coarse, const GalerkinConsts& G,
@@ -39,7 +49,7 @@ void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, con
 
 // Brick smoother (8^3 bricks, frozen halo, result in uout).
 bool brick_ok(const Geom& g);
-void launch_st_smooth_brick(hipStream_t s, const Geom& g, const float* st, const float* f, const float* uin,
+void launch_st_smooth_brick(hipStream_t s, const Geom& g, const StRef& S, const float* f, const float* uin,
                             float* uout, int sweeps, bool forward, bool zero_in);
 
 // ---- vectors and reductions (otm_vec.hip) ----
@@ -47,6 +57,9 @@ void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblock
 void launch_fill(hipStream_t s, float* a, long long n, float v);
 void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks);
+void launch_sum3(hipStream_t s, long long nn, const double* a, const Reduce& R, unsigned nblocks);
+void launch_refine_add(hipStream_t s, long long n, double* x64, float* d);
+void launch_sub_mean_cast(hipStream_t s, long long nn, double* x64, float* x32, const PcgScalars* P);
 void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P);
 void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
 void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,

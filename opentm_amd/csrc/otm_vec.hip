@@ -23,13 +23,36 @@ __global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __re
   reduce_block<OTM_NRHS>(acc, R);
 }
 
-__global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const float* __restrict__ a, Reduce R) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const T* __restrict__ a, Reduce R) {
   double acc[OTM_NRHS] = {0, 0, 0};
   for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i];
   }
   reduce_block<OTM_NRHS>(acc, R);
+}
+
+// Iterative refinement: x64 += d (the FP32 correction), d = 0.
+__global__ void __launch_bounds__(kBlock) k_refine_add(long long n, double* __restrict__ x64, float* __restrict__ d) {
+  for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
+    x64[i] += (double)d[i];
+    d[i] = 0.f;
+  }
+}
+
+// x64 -= mean (per load case), x32 = x64 rounded.
+__global__ void __launch_bounds__(kBlock) k_sub_mean_cast(long long nn, double* __restrict__ x64,
+                                                          float* __restrict__ x32, const PcgScalars* __restrict__ P) {
+  const double m[OTM_NRHS] = {P->mean[0], P->mean[1], P->mean[2]};
+  for (long long i = gtid(); i < nn; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const double v = x64[k * nn + i] - m[k];
+      x64[k * nn + i] = v;
+      x32[k * nn + i] = (float)v;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_sum2(long long nn, const float* __restrict__ a, const float* __restrict__ b,
@@ -146,8 +169,22 @@ void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, co
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_sum3, dim3(nblocks), dim3(kBlock), 0, s, nn, a, R);
+  hipLaunchKernelGGL(k_sum3<float>, dim3(nblocks), dim3(kBlock), 0, s, nn, a, R);
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
+}
+void launch_sum3(hipStream_t s, long long nn, const double* a, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_sum3<double>, dim3(nblocks), dim3(kBlock), 0, s, nn, a, R);
+  launch_reduce_final(s, R, OTM_NRHS, nblocks);
+}
+static inline unsigned stream_blocks(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  return (unsigned)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+void launch_refine_add(hipStream_t s, long long n, double* x64, float* d) {
+  hipLaunchKernelGGL(k_refine_add, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, x64, d);
+}
+void launch_sub_mean_cast(hipStream_t s, long long nn, double* x64, float* x32, const PcgScalars* P) {
+  hipLaunchKernelGGL(k_sub_mean_cast, dim3(stream_blocks(nn)), dim3(kBlock), 0, s, nn, x64, x32, P);
 }
 void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_sum2, dim3(nblocks), dim3(kBlock), 0, s, nn, a, b, R);

@@ -40,6 +40,7 @@ def test_config_defaults():
     assert list(c.heat_ratio) == [1.0, 0.0]          # cmdline.cpp:12-13
     assert c.pre_sweeps == 2 and c.post_sweeps == 2
     assert c.fem_rel_thres == pytest.approx(1e-6)
+    assert c.fp64_refine == 0 and c.fp16_stencils == 0 and c.coarse_sweeps == 0
     p = _capi.OtmDensityParams()
     lib.otm_density_params_init(ctypes.byref(p))
     assert p.filter_kind == 0 and p.filter_radius == 1.5 and p.filter_periodic == 0
@@ -96,3 +97,15 @@ def test_openTM_module_surface():
     for name in ("setDensity", "setConfig", "optimize"):
         assert hasattr(h, name)
     assert callable(openTM.runInstance)
+
+
+def test_struct_layouts_match_header():
+    """Field names and order of the ctypes mirrors equal the header's."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "opentm.h")).read()
+    for cname, py in [("otm_config", _capi.OtmConfig), ("otm_density_params", _capi.OtmDensityParams),
+                      ("otm_custom_params", _capi.OtmCustomParams), ("otm_custom_report", _capi.OtmCustomReport)]:
+        body = re.search(r"typedef struct\s*\w*\s*\{([^}]*)\}\s*" + cname + r"\s*;", hdr).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        names = [re.split(r"[\s\[;]", d.strip().split()[-1])[0] for d in body.split(";") if d.strip()]
+        assert names == [f[0] for f in py._fields_], cname

@@ -376,3 +376,80 @@ def test_run_custom_oc_and_openTM():
     out2 = np.asarray(openTM.runInstance(n, [1.0, 1e-4], [0.5, 0.4, 0.3, 0.0, 0.0, 0.0], openTM.InitWay.IWP,
                                          openTM.Model.oc), np.float32)
     assert out2.shape == (n ** 3,)
+
+
+# ------------------------------------------------------------ precision form
+def make_precise(n, rtol=1e-8, refine=1, fp16=1, k1=1e-3):
+    cfg = otm.HomoConfig(n, (1.0, k1))
+    cfg.femRelThres = rtol
+    cfg.fp64_refine = refine
+    cfg.fp16_stencils = fp16
+    return otm.Homogenization_H(cfg)
+
+
+@pytest.mark.parametrize("n,fp16", [(32, 0), (32, 1), (64, 1)])
+def test_refinement_reaches_1e8(orc, n, fp16):
+    """FP64 iterative refinement: the true relative residual of every load
+    case falls to 1e-8 (far below the FP32 iterate's ~5e-6 floor), with FP32
+    or FP16-stored coarse stencils; kappa^H and element energies against the
+    FP64 oracle."""
+    say("refine", n, "fp16" if fp16 else "fp32")
+    k = wl.simp(wl.plain_density(n, 3))
+    hom = make_precise(n, fp16=fp16)
+    hom.update(k)
+    C = hom.heatMatrix()
+    its, rr = hom.solve_stats()
+    Co, _, So, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12, want_S=True)
+    S = hom.element_energy()
+    errC = rel(C, Co)
+    errS = max(rel(S[c], So[c]) for c in range(6))
+    say("  its", its, "relres", rr, "kappaH err", errC, "energy err", errS)
+    assert max(rr) <= 1e-8
+    assert errC < 1e-6
+    assert errS < 1e-4
+
+
+def test_fp16_stencils_plain_tolerance(orc):
+    """FP16 coarse stencils with the plain FP32 iterate at rtol 1e-6: same
+    kappa^H bar as the FP32 stencils, and no more than a few extra PCG
+    iterations."""
+    n = 64
+    say("fp16 plain", n)
+    k = wl.simp(wl.plain_density(n, 4))
+    its = {}
+    for fp16 in (0, 1):
+        hom = make_precise(n, rtol=1e-6, refine=0, fp16=fp16)
+        hom.update(k)
+        C = hom.heatMatrix()
+        its[fp16] = max(hom.solve_stats()[0])
+        Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+        say("  fp16", fp16, "its", its[fp16], "err", rel(C, Co))
+        assert rel(C, Co) < 1e-5
+    assert its[1] <= its[0] + 3
+
+
+def test_fp16_coarse_operator_symmetric():
+    """The FP16 coarse operator is exactly symmetric (each coupling is stored
+    once) and keeps constants in its null space."""
+    n = 32
+    say("fp16 operator", n)
+    k = wl.simp(wl.plain_density(n, 6))
+    hom = make_precise(n, rtol=1e-6, refine=0, fp16=1)
+    hom.update(k)
+    ref = make_precise(n, rtol=1e-6, refine=0, fp16=0)
+    ref.update(k)
+    rng = np.random.default_rng(2)
+    for level in (1, 2):
+        m = int(np.prod(hom.levels()[level]))
+        u = rng.standard_normal(3 * m).astype(np.float32)
+        v = rng.standard_normal(3 * m).astype(np.float32)
+        Au = hom.apply_level(level, u).astype(np.float64)
+        Av = hom.apply_level(level, v).astype(np.float64)
+        scale = np.abs(Au).max() * np.abs(v).max() * m
+        asym = abs(float(v @ Au - u @ Av)) / scale
+        A1 = hom.apply_level(level, np.ones(3 * m, np.float32))
+        Ar = ref.apply_level(level, u)
+        say("  level", level, "asym", asym, "A1", np.abs(A1).max(), "vs fp32", rel(Au, Ar))
+        assert asym < 1e-6
+        assert np.abs(A1).max() < 1e-5 * np.abs(Au).max()
+        assert rel(Au, Ar) < 5e-3
