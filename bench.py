@@ -12,6 +12,13 @@ MG-PCG solve (FP32, FP64 reductions, relative residual 1e-6, warm start)
 homogenization + sensitivity solve, so value = solves/s.
 
 Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
+
+--workload precision runs the precision form instead: a 256^3 cell
+(cos-mode density at volume 0.25), MMA maximising C00 with C11, C22 <= 0.05
+and volume <= 0.25, solved with FP64 iterative refinement to 1e-8 and FP16
+coarse stencils; the same loop is then timed on the plain FP32 path, and
+--oracle adds the FP64 oracle's kappa^H and sensitivity errors on the
+first design (minutes of host time at 256^3).
 """
 import argparse
 import json
@@ -68,10 +75,15 @@ def main():
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel class with HIP events (adds event overhead to the step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["oc", "precision"], default="oc")
+    ap.add_argument("--oracle", action="store_true", help="precision workload: FP64 oracle errors (slow)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "precision":
+        precision_main(args)
+        return
 
     import opentm_amd as otm
     from opentm_amd import workloads as wl
@@ -153,6 +165,63 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (FP64 oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, args.seed, args.rtol)
+    print(json.dumps(out))
+
+
+def precision_main(args):
+    from opentm_amd import precision as pf
+    from opentm_amd import workloads as wl
+    n = args.reso if args.reso != 128 else 256
+    iters = args.steps if args.steps != 10 else 30
+    rho0 = wl.plain_density(n, args.seed, mean=0.25)
+
+    def run(refine, fp16, rtol, label):
+        loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=fp16)
+        t0 = time.perf_counter()
+        for i in range(iters):
+            h = loop.step()
+            print(f"[bench] {label} itn {i} {1e3 * h['seconds']:.1f} ms C00 {h['C'][0, 0]:.6f} "
+                  f"g {np.round(h['g'], 4).tolist()} its {h['pcg_iters']} relres {max(h['relres']):.2e} "
+                  f"newton {h['newton']}", file=sys.stderr, flush=True)
+        total = time.perf_counter() - t0
+        secs = [h["seconds"] for h in loop.history]
+        timed = secs[args.warmup:] if len(secs) > args.warmup else secs
+        res = {"iters": iters, "total_s": total, "ms_per_iter": 1e3 * float(np.mean(timed)),
+               "p50_ms": 1e3 * float(np.percentile(timed, 50)), "p99_ms": 1e3 * float(np.percentile(timed, 99)),
+               "pcg_iters_mean": float(np.mean([max(h["pcg_iters"]) for h in loop.history])),
+               "relres_max": float(max(max(h["relres"]) for h in loop.history)),
+               "C_first": loop.history[0]["C"].reshape(-1).tolist(),
+               "C_final": loop.history[-1]["C"].reshape(-1).tolist(),
+               "g_final": loop.history[-1]["g"].tolist()}
+        return loop, res
+
+    loop_p, prec = run(True, True, 1e-8, "precision")
+    del loop_p
+    loop_f, plain = run(False, False, args.rtol, "plain")
+    del loop_f
+    out = {"metric": "MMA design iterations/s (one homogenization + 3 sensitivity contractions each)",
+           "value": 1e3 / prec["ms_per_iter"], "unit": "iterations/s", "n_gpus": 1, "steps": iters,
+           "warmup": args.warmup, "ms_per_step": prec["ms_per_iter"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp16 stencils+fp64 refinement",
+           "data": "synthetic",
+           "config": {"workload": f"{n}^3 precision form: MMA max C00 s.t. C11, C22 <= 0.05, volume <= 0.25",
+                      "reso": n, "seed": args.seed, "mma": "m=3, move 0.2, c=1e3"},
+           "precision": prec, "plain_fp32": plain,
+           "time_vs_plain": prec["ms_per_iter"] / plain["ms_per_iter"]}
+    if args.oracle:
+        import oracle.oracle as orc
+        print("[bench] FP64 oracle on the first design ...", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        rf = orc.filter_fwd(rho0.astype(np.float64), 0, 1.5, False)
+        Co, _, _, _ = orc.homogenize(1e-3 + (1 - 1e-3) * rf ** 3, rtol=1e-10)
+        dt = time.perf_counter() - t0
+
+        def relerr(c):
+            c = np.asarray(c).reshape(3, 3)
+            return float(np.abs(c - Co).max() / np.abs(Co).max())
+        out["oracle"] = {"rtol": 1e-10, "seconds": dt, "C": Co.reshape(-1).tolist(),
+                         "kappaH_rel_err_precision": relerr(prec["C_first"]),
+                         "kappaH_rel_err_plain": relerr(plain["C_first"])}
     print(json.dumps(out))
 
 

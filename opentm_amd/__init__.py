@@ -277,7 +277,11 @@ class Homogenization_H:
         p, where, keep = _arg(rho, self.ne)
         _capi.check(_capi.load().otm_set_design(self._ctx, p, where), "otm_set_design")
 
-    def get_design(self):
+    def get_design(self, out=None):
+        """Design density (host array, or written into a DeviceBuffer)."""
+        if isinstance(out, DeviceBuffer):
+            _capi.check(_capi.load().otm_get_design(self._ctx, out.ptr, OTM_DEVICE), "otm_get_design")
+            return out
         out = np.empty(self.ne, np.float32)
         _capi.check(_capi.load().otm_get_design(self._ctx, _ptr(out), OTM_HOST), "otm_get_design")
         return out
@@ -289,14 +293,26 @@ class Homogenization_H:
         _capi.check(_capi.load().otm_eval(self._ctx, ctypes.byref(p), C), "otm_eval")
         return np.array(C[:], dtype=np.float64).reshape(3, 3)
 
-    def backward(self, dC, want=True):
+    def backward(self, dC, want=True, out=None):
+        """d(sum dC_ij C_ij)/d rho through the SIMP map and the filter:
+        returned as a host array, written into a DeviceBuffer `out`, or only
+        kept on the device (want=False, for the OC step)."""
         dCa = (ctypes.c_double * 9)(*np.asarray(dC, dtype=np.float64).reshape(9).tolist())
         if not want:
             _capi.check(_capi.load().otm_backward(self._ctx, dCa, None, OTM_HOST), "otm_backward")
             return None
+        if isinstance(out, DeviceBuffer):
+            _capi.check(_capi.load().otm_backward(self._ctx, dCa, out.ptr, OTM_DEVICE), "otm_backward")
+            return out
         out = np.empty(self.ne, np.float32)
         _capi.check(_capi.load().otm_backward(self._ctx, dCa, _ptr(out), OTM_HOST), "otm_backward")
         return out
+
+    def design_stats(self):
+        """(mean design density, mean SIMP conductivity) on the device."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _capi.check(_capi.load().otm_design_stats(self._ctx, ctypes.byref(a), ctypes.byref(b)), "otm_design_stats")
+        return a.value, b.value
 
     def synchronize(self):
         _capi.check(_capi.load().otm_synchronize(self._ctx), "otm_synchronize")
