@@ -6,7 +6,7 @@
  * C++ classes homo::Homogenization_H, homo::MG_H, homo::OCOptimizer and the
  * (commented-out) pybind module openTM. Each entry point below names the
  * reference interface it replaces. C++ classes with the reference's names
- * sit on top of this ABI in include/opentm/*.hpp.
+ * sit on top of this ABI in include/opentm/opentm.hpp.
  *
  * Conventions
  *  - Scalar fields are lexicographic with x fastest; a row pitch (in
@@ -235,6 +235,78 @@ OTM_API void otm_custom_params_init(otm_custom_params* p);
 /* rho0: optional host initial density (reso^3 floats); rho_out: host
  * reso^3 floats, the final design; report may be NULL. */
 OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float* rho_out, otm_custom_report* report);
+
+/* ---- batched form: many cells per launch -------------------------------
+ * No counterpart in the reference (one hierarchy per process). A batch
+ * solves the heatMatrix of many contexts together: every launch covers the
+ * same operation for all member cells (one fine smoother launch for all
+ * fine levels, one for all levels l, one per reduction), so small cells fill
+ * the GPU together. Each cell keeps its own arithmetic: results are the
+ * same whatever else is in the batch. The contexts must be on the batch's
+ * device and updated (otm_update / otm_eval's SIMP map); a batch call
+ * synchronises their streams first. */
+typedef struct otm_batch otm_batch;
+OTM_API int otm_batch_create(int device, otm_batch** out);
+OTM_API int otm_batch_destroy(otm_batch* batch);
+/* Homogenization_H::update (homogenization.cpp:76-81) of n contexts in one
+ * launch sequence: kappa[i] is cell i's SIMP-mapped conductivity (row pitch
+ * `pitch` floats, <= 0: reso[0]; host or device per `where`); every cell's
+ * Galerkin hierarchy and coarsest inverse are rebuilt together. */
+OTM_API int otm_batch_update(otm_batch* batch, otm_ctx* const* cells, int n, const float* const* kappa, int pitch,
+                             int where);
+/* heatMatrix (homogenization.cu:180-252) of n contexts: C receives 9
+ * doubles per cell (row-major). status (may be NULL) receives a status code
+ * per cell; without it the first failure is returned. */
+OTM_API int otm_batch_heat_matrix(otm_batch* batch, otm_ctx* const* cells, int n, double* C, int* status);
+/* Sensitivity (homogenization.cu:415-479) of n solved contexts: dC holds 9
+ * doubles per cell, sens[i] receives cell i's field with row pitch `pitch`
+ * floats (<= 0: reso[0]), host or device pointers per `where`. */
+OTM_API int otm_batch_sensitivity(otm_batch* batch, otm_ctx* const* cells, int n, const double* dC,
+                                  float* const* sens, int pitch, int where);
+/* Launches issued and PCG steps taken by the batch so far. */
+OTM_API int otm_batch_stats(const otm_batch* batch, long long* launches, int* steps);
+
+/* ---- serving form: continuous batching -----------------------------------
+ * No counterpart in the reference. A server owns a worker thread and one
+ * batch. Submitted requests are admitted between PCG steps while the
+ * member count (max_active) and the HBM budget (bytes of resident contexts,
+ * active and cached) allow; hierarchies stay resident and are reused by
+ * later requests of the same shape. Results do not depend on what else was
+ * in flight. */
+typedef struct otm_server otm_server;
+typedef struct otm_request {
+  otm_config cfg;           /* cell shape and solver settings (device must be the server's) */
+  const float* kappa;       /* host, reso[0]*reso[1]*reso[2] SIMP-mapped conductivities (copied at submit) */
+  double dC[9];             /* objective gradient for the sensitivity */
+  int want_sens;            /* 1: compute the contracted sensitivity */
+} otm_request;
+typedef struct otm_result {
+  int status;               /* OTM_OK or the failure code */
+  double C[9];              /* kappa^H */
+  int iters[3];             /* PCG iterations per load case */
+  double relres[3];         /* final true relative residuals */
+  double queue_ms;          /* submit -> admission */
+  double latency_ms;        /* submit -> result */
+} otm_result;
+typedef struct otm_server_counters {
+  long long completed;         /* requests finished */
+  long long steps;             /* batched PCG steps run */
+  long long contexts_created;  /* hierarchies allocated */
+  long long contexts_evicted;  /* idle hierarchies freed to fit the budget */
+  int active_peak;             /* most requests on the GPU at once */
+  int queued;                  /* requests waiting for admission */
+  double resident_bytes;       /* device bytes of held hierarchies */
+  double budget_bytes;
+} otm_server_counters;
+OTM_API void otm_request_init(otm_request* r);
+OTM_API int otm_server_create(int device, double hbm_budget_bytes, int max_active, otm_server** out);
+OTM_API int otm_server_destroy(otm_server* server);
+OTM_API int otm_server_submit(otm_server* server, const otm_request* req, long long* id);
+/* Blocks until request `id` is done; sens (host, may be NULL) receives the
+ * sensitivity field (x fastest) when it was requested. Returns the
+ * request's status. */
+OTM_API int otm_server_wait(otm_server* server, long long id, otm_result* res, float* sens);
+OTM_API int otm_server_stats(otm_server* server, otm_server_counters* st);
 
 /* Live kernel timing with HIP events recorded on the context's stream
  * around every launch of a kernel class. Classes: */

@@ -22,7 +22,7 @@ from ._capi import OTM_DEVICE, OTM_HOST, OtmError
 
 __all__ = [
     "HomoConfig", "Homogenization_H", "OCOptimizer", "MMAOptimizer", "DeviceBuffer", "DensityParams",
-    "conv_filter", "OtmError", "load_library", "run_custom",
+    "conv_filter", "OtmError", "load_library", "run_custom", "Batch", "Server",
 ]
 
 
@@ -517,3 +517,144 @@ def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
     _capi.check(lib.otm_run_custom(ctypes.byref(p), r0, _ptr(out), ctypes.byref(rep)), "otm_run_custom")
     return out, {"iterations": rep.iterations, "converged": bool(rep.converged), "objective": rep.objective,
                  "volume": rep.volume, "C": np.array(rep.C[:]).reshape(3, 3)}
+
+
+class Batch:
+    """Batched solves of many cells (otm_batch_*): every launch covers one
+    operation for all member cells, so small cells fill the GPU together.
+    Members are Homogenization_H objects on the batch's device."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _capi.check(_capi.load().otm_batch_create(int(device), ctypes.byref(h)), "otm_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.load().otm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ctxs(homs):
+        arr = (ctypes.c_void_p * len(homs))()
+        for i, h in enumerate(homs):
+            arr[i] = h._ctx
+        return arr
+
+    def update(self, homs, kappas, pitchT=-1):
+        """Homogenization_H::update of every member together (host arrays or
+        DeviceBuffers, all of one kind)."""
+        n = len(homs)
+        ptrs = (ctypes.c_void_p * max(n, 1))()
+        keep, wheres = [], set()
+        for i, (h, k) in enumerate(zip(homs, kappas)):
+            pitch = h.reso[0] if pitchT is None or pitchT < 0 else int(pitchT)
+            p, w, kk = _arg(k, pitch * h.reso[1] * h.reso[2])
+            ptrs[i] = p
+            keep.append(kk)
+            wheres.add(w)
+        if len(wheres) > 1:
+            raise ValueError("conductivities must be all host arrays or all device buffers")
+        where = wheres.pop() if wheres else OTM_HOST
+        _capi.check(_capi.load().otm_batch_update(self._h, self._ctxs(homs), n, ptrs, int(pitchT), where),
+                    "otm_batch_update")
+
+    def heat_matrix(self, homs, status=False):
+        """kappa^H of every member, shape (n, 3, 3); with status=True also the
+        per-cell status codes (failed cells hold NaN)."""
+        n = len(homs)
+        C = (ctypes.c_double * (9 * max(n, 1)))()
+        st = (ctypes.c_int * max(n, 1))()
+        _capi.check(_capi.load().otm_batch_heat_matrix(self._h, self._ctxs(homs), n, C, st if status else None),
+                    "otm_batch_heat_matrix")
+        out = np.array(C[:9 * n], dtype=np.float64).reshape(n, 3, 3)
+        return (out, list(st[:n])) if status else out
+
+    def sensitivity(self, homs, dCs, outs=None):
+        """Contracted sensitivities of every member (host arrays, or written
+        into the DeviceBuffers of `outs`)."""
+        n = len(homs)
+        dC = (ctypes.c_double * (9 * max(n, 1)))(*np.asarray(dCs, dtype=np.float64).reshape(-1).tolist())
+        ptrs = (ctypes.c_void_p * max(n, 1))()
+        keep = []
+        where = OTM_DEVICE if outs is not None else OTM_HOST
+        for i, h in enumerate(homs):
+            if outs is not None:
+                ptrs[i] = outs[i].ptr
+            else:
+                a = np.empty(h.ne, np.float32)
+                keep.append(a)
+                ptrs[i] = a.ctypes.data
+        _capi.check(_capi.load().otm_batch_sensitivity(self._h, self._ctxs(homs), n, dC, ptrs, -1, where),
+                    "otm_batch_sensitivity")
+        return outs if outs is not None else keep
+
+    def stats(self):
+        L, S = ctypes.c_longlong(), ctypes.c_int()
+        _capi.check(_capi.load().otm_batch_stats(self._h, ctypes.byref(L), ctypes.byref(S)), "otm_batch_stats")
+        return L.value, S.value
+
+
+class Server:
+    """Serving form (otm_server_*): a worker thread admits submitted
+    requests into a running batch between PCG steps, within an HBM budget,
+    and keeps hierarchies resident for later requests of the same shape."""
+
+    def __init__(self, hbm_budget_gb=64.0, max_active=64, device=0):
+        h = ctypes.c_void_p()
+        _capi.check(_capi.load().otm_server_create(int(device), float(hbm_budget_gb) * 2 ** 30, int(max_active),
+                                                   ctypes.byref(h)), "otm_server_create")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.load().otm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, kappa, cfg=None, dC=None, want_sens=True):
+        """Queues one homogenization (+ sensitivity) of the SIMP-mapped
+        conductivity `kappa` (shape (nz, ny, nx)); returns the request id."""
+        k = _f32(kappa)
+        req = _capi.OtmRequest()
+        _capi.load().otm_request_init(ctypes.byref(req))
+        if cfg is None:
+            nz, ny, nx = k.shape
+            cfg = HomoConfig((nx, ny, nz), (1.0, 1e-3))
+        req.cfg = cfg._c()
+        req.cfg.device = self.device
+        req.kappa = k.ctypes.data
+        d = np.zeros(9) if dC is None else np.asarray(dC, dtype=np.float64).reshape(9)
+        for i in range(9):
+            req.dC[i] = float(d[i])
+        req.want_sens = 1 if want_sens else 0
+        rid = ctypes.c_longlong()
+        _capi.check(_capi.load().otm_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)), "otm_server_submit")
+        return rid.value
+
+    def wait(self, rid, ne=None):
+        """(kappa^H (3, 3), sensitivity or None, info dict) of request `rid`."""
+        res = _capi.OtmResult()
+        sens = np.empty(ne, np.float32) if ne else None
+        _capi.check(_capi.load().otm_server_wait(self._h, ctypes.c_longlong(rid), ctypes.byref(res),
+                                                 _ptr(sens) if sens is not None else None), "otm_server_wait")
+        info = {"iters": list(res.iters), "relres": list(res.relres), "queue_ms": res.queue_ms,
+                "latency_ms": res.latency_ms}
+        return np.array(res.C[:]).reshape(3, 3), sens, info
+
+    def stats(self):
+        st = _capi.OtmServerCounters()
+        _capi.check(_capi.load().otm_server_stats(self._h, ctypes.byref(st)), "otm_server_stats")
+        return {f[0]: getattr(st, f[0]) for f in st._fields_}

@@ -71,6 +71,39 @@ class OtmCustomParams(ctypes.Structure):
     ]
 
 
+class OtmRequest(ctypes.Structure):
+    _fields_ = [
+        ("cfg", OtmConfig),
+        ("kappa", _c.c_void_p),
+        ("dC", _c.c_double * 9),
+        ("want_sens", _c.c_int),
+    ]
+
+
+class OtmResult(ctypes.Structure):
+    _fields_ = [
+        ("status", _c.c_int),
+        ("C", _c.c_double * 9),
+        ("iters", _c.c_int * 3),
+        ("relres", _c.c_double * 3),
+        ("queue_ms", _c.c_double),
+        ("latency_ms", _c.c_double),
+    ]
+
+
+class OtmServerCounters(ctypes.Structure):
+    _fields_ = [
+        ("completed", _c.c_longlong),
+        ("steps", _c.c_longlong),
+        ("contexts_created", _c.c_longlong),
+        ("contexts_evicted", _c.c_longlong),
+        ("active_peak", _c.c_int),
+        ("queued", _c.c_int),
+        ("resident_bytes", _c.c_double),
+        ("budget_bytes", _c.c_double),
+    ]
+
+
 class OtmCustomReport(ctypes.Structure):
     _fields_ = [
         ("iterations", _c.c_int),
@@ -128,6 +161,18 @@ SIGNATURES = {
     "otm_device_free": (_i, [_p]),
     "otm_memcpy": (_i, [_p, _p, _c.c_size_t, _i, _i]),
     "otm_synchronize": (_i, [_p]),
+    "otm_batch_create": (_i, [_i, _c.POINTER(_p)]),
+    "otm_batch_destroy": (_i, [_p]),
+    "otm_batch_update": (_i, [_p, _c.POINTER(_p), _i, _c.POINTER(_p), _i, _i]),
+    "otm_batch_heat_matrix": (_i, [_p, _c.POINTER(_p), _i, _c.POINTER(_c.c_double), _c.POINTER(_i)]),
+    "otm_batch_sensitivity": (_i, [_p, _c.POINTER(_p), _i, _c.POINTER(_c.c_double), _c.POINTER(_p), _i, _i]),
+    "otm_batch_stats": (_i, [_p, _c.POINTER(_c.c_longlong), _c.POINTER(_i)]),
+    "otm_request_init": (None, [_c.POINTER(OtmRequest)]),
+    "otm_server_create": (_i, [_i, _c.c_double, _i, _c.POINTER(_p)]),
+    "otm_server_destroy": (_i, [_p]),
+    "otm_server_submit": (_i, [_p, _c.POINTER(OtmRequest), _c.POINTER(_c.c_longlong)]),
+    "otm_server_wait": (_i, [_p, _c.c_longlong, _c.POINTER(OtmResult), _p]),
+    "otm_server_stats": (_i, [_p, _c.POINTER(OtmServerCounters)]),
 }
 
 _lib = None

@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libopentm.so")
-SOURCES = ["otm_fine.hip", "otm_coarse.hip", "otm_vec.hip", "otm_opt.hip", "otm_mma.hip", "otm_engine.hip", "otm_framework.cpp"]
+SOURCES = ["otm_fine.hip", "otm_coarse.hip", "otm_vec.hip", "otm_opt.hip", "otm_mma.hip", "otm_engine.hip", "otm_batch.hip",
+           "otm_framework.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [

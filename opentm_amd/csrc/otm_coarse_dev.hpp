@@ -1,0 +1,546 @@
+// otm_coarse_dev.hpp -- device bodies of the coarse multigrid levels
+// (stored 27-point stencils, transfers, coarsest solve), shared by the
+// single-cell launches (otm_coarse.hip) and the batched multi-cell launches
+// (otm_batch.hip).
+//
+// Stencil layout: st[o*nn + v] couples node v to v + off(o) with
+// off(o) = (o%3-1, o/3%3-1, o/9-1); o = 13 is the diagonal (the reference
+// keeps 27 SoA arrays the same way, grid.cpp:189-220, diagonal at index 13,
+// grid.cu:707-809).
+//
+// Galerkin coarse operators A_{l+1} = P^T A_l P with trilinear P
+// (grid.cu:811-940, grid_ex.cu:16-227) are formed element by element: a
+// coarse element's 8x8 matrix is the sum over its 8 children c of
+// Q_c^T E_c Q_c, where Q_c interpolates the coarse element's corners to the
+// child's corners. This equals the stencil-level triple product exactly.
+#pragma once
+#include "otm_kernels.hpp"
+
+namespace otm {
+
+__device__ inline long long lin(const Geom& g, int x, int y, int z) {
+  return (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+}
+
+// ---- per-node operations of the stored-stencil levels ----
+// Stencil entries of node v. FP16 storage (HALF) keeps the 26 couplings
+// divided by a per-level scale; the diagonal is rebuilt as minus their sum,
+// so rows keep a zero sum and the constants stay in the null space.
+template <bool HALF>
+__device__ inline void load_stencil(const StRef& S, long long nn, long long v, float a[27]) {
+  if constexpr (HALF) {
+    const float sc = *S.scale;
+    float off = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+      if (o == 13) continue;
+      a[o] = (float)S.f16[o * nn + v] * sc;
+      off += a[o];
+    }
+    a[13] = -off;
+  } else {
+#pragma unroll
+    for (int o = 0; o < 27; o++) a[o] = S.f32[o * nn + v];
+  }
+}
+
+// GS update at node (x,y,z): u = (f - sum_{o != 13} a_o u_o) / a_13.
+template <bool HALF>
+__device__ inline void st_gs_node(const Geom& g, const StRef& S, const float* __restrict__ f, float* u, int x, int y,
+                                  int z) {
+  const long long v = lin(g, x, y, z);
+  const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+  const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+  const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+  float a[27];
+  load_stencil<HALF>(S, g.nn, v, a);
+  const float inv = 1.f / a[13];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * g.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+      if (o == 13) continue;
+      s += a[o] * uk[lin(g, xs[o % 3], ys[(o / 3) % 3], zs[o / 9])];
+    }
+    u[k * g.nn + v] = (f[k * g.nn + v] - s) * inv;
+  }
+}
+
+// res = f - A u at node t.
+template <bool HALF>
+__device__ inline void st_resid_node(const Geom& g, const StRef& S, const float* __restrict__ f, const float* u,
+                                     float* res, long long t) {
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+  const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+  const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+  float a[27];
+  load_stencil<HALF>(S, g.nn, t, a);
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * g.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) s += a[o] * uk[lin(g, xs[o % 3], ys[(o / 3) % 3], zs[o / 9])];
+    res[k * g.nn + t] = f[k * g.nn + t] - s;
+  }
+}
+
+// f_c(I) = sum_{d in {-1,0,1}^3} w(d) res(2I + d), w = prod(d_i == 0 ? 1 : 1/2)
+// (restrict_residual_kernel_H, grid.cu:811-878, weights (2-|d|)/2 per axis).
+__device__ inline void restrict_node(const Geom& fg, const Geom& cg, const float* res, float* fc, long long t) {
+  const int X = (int)(t % cg.nx);
+  const long long r = t / cg.nx;
+  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  const int x = 2 * X, y = 2 * Y, z = 2 * Z;
+  const int xs[3] = {x == 0 ? fg.nx - 1 : x - 1, x, x + 1};
+  const int ys[3] = {y == 0 ? fg.ny - 1 : y - 1, y, y + 1};
+  const int zs[3] = {z == 0 ? fg.nz - 1 : z - 1, z, z + 1};
+  const float w1[3] = {0.5f, 1.f, 0.5f};
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* rk = res + k * fg.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        float row = 0.f;
+        const long long rb = (long long)fg.nx * ((long long)ys[b] + (long long)fg.ny * zs[c]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) row += w1[a] * rk[rb + xs[a]];
+        s += w1[b] * w1[c] * row;
+      }
+    fc[k * cg.nn + t] = s;
+  }
+}
+
+// u_f(i) += sum_I P(i, I) u_c(I), trilinear (prolongate_correction_kernel_H,
+// grid.cu:881-940).
+__device__ inline void prolong_node(const Geom& fg, const Geom& cg, const float* uc, float* uf, long long t) {
+  const int x = (int)(t % fg.nx);
+  const long long r = t / fg.nx;
+  const int y = (int)(r % fg.ny), z = (int)(r / fg.ny);
+  int X[2], Y[2], Z[2];
+  float wx[2], wy[2], wz[2];
+  X[0] = x >> 1; X[1] = (x & 1) ? ((x >> 1) + 1 == cg.nx ? 0 : (x >> 1) + 1) : X[0];
+  wx[0] = (x & 1) ? 0.5f : 1.f; wx[1] = (x & 1) ? 0.5f : 0.f;
+  Y[0] = y >> 1; Y[1] = (y & 1) ? ((y >> 1) + 1 == cg.ny ? 0 : (y >> 1) + 1) : Y[0];
+  wy[0] = (y & 1) ? 0.5f : 1.f; wy[1] = (y & 1) ? 0.5f : 0.f;
+  Z[0] = z >> 1; Z[1] = (z & 1) ? ((z >> 1) + 1 == cg.nz ? 0 : (z >> 1) + 1) : Z[0];
+  wz[0] = (z & 1) ? 0.5f : 1.f; wz[1] = (z & 1) ? 0.5f : 0.f;
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* ck = uc + k * cg.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          const float w = wx[a] * wy[b] * wz[c];
+          if (w != 0.f) s += w * ck[lin(cg, X[a], Y[b], Z[c])];
+        }
+    uf[k * fg.nn + t] += s;
+  }
+}
+
+// ---- per-level bodies ----
+// GS update of one of 8 colours (parity of x, y, z), colour = cx + 2cy + 4cz.
+template <bool HALF>
+__device__ inline void st_gs_color_body(const Geom& g, const StRef& S, const float* __restrict__ f, float* u, int cx,
+                                        int cy, int cz, long long t) {
+  const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
+  const long long cnt = (long long)hx * hy * hz;
+  if (t >= cnt) return;
+  const int x = 2 * (int)(t % hx) + cx;
+  const long long r = t / hx;
+  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+  st_gs_node<HALF>(g, S, f, u, x, y, z);
+}
+
+template <bool HALF>
+__device__ inline void st_residual_body(const Geom& g, const StRef& S, const float* __restrict__ f,
+                                        const float* __restrict__ u, float* __restrict__ res, long long t) {
+  if (t >= g.nn) return;
+  st_resid_node<HALF>(g, S, f, u, res, t);
+}
+
+__device__ inline void restrict_body(const Geom& fg, const Geom& cg, const float* __restrict__ res,
+                                     float* __restrict__ fc, long long t) {
+  if (t >= cg.nn) return;
+  restrict_node(fg, cg, res, fc, t);
+}
+
+__device__ inline void prolong_body(const Geom& fg, const Geom& cg, const float* __restrict__ uc,
+                                    float* __restrict__ uf, long long t) {
+  if (t >= fg.nn) return;
+  prolong_node(fg, cg, uc, uf, t);
+}
+
+// u = Binv f for right-hand side k, row i, FP64 accumulation.
+__device__ inline void coarsest_solve_body(const Geom& g, const double* __restrict__ Binv,
+                                           const float* __restrict__ f, float* __restrict__ u, int i, int k) {
+  const int n = (int)g.nn;
+  if (i >= n) return;
+  const float* fk = f + (long long)k * n;
+  double s = 0.0;
+  for (int j = 0; j < n; j++) s += Binv[(long long)i * n + j] * (double)fk[j];
+  u[(long long)k * n + i] = (float)s;
+}
+
+// Brick smoother for stored-stencil levels. One 512-thread workgroup owns an
+// 8^3 brick (cut at the level's edge when an edge is not a multiple of 8):
+// its nodes' stencils sit in registers, u of the brick plus a one-node halo
+// sits in LDS. All colours of `sweeps` Gauss-Seidel sweeps run inside the
+// brick with workgroup barriers; halo values stay frozen at their values on
+// entry (block Gauss-Seidel across bricks, exact multicolour Gauss-Seidel
+// inside; a level smaller than a brick is smoothed exactly except across
+// its periodic seam). The result goes to a second buffer so bricks never
+// read a neighbour's new values. The post-smoother runs the colours in
+// reverse order, which is the adjoint of the pre-smoother: the V-cycle
+// stays symmetric.
+constexpr int kBrick = 8;
+constexpr int kBrickThreads = kBrick * kBrick * kBrick;
+constexpr int kHalo = kBrick + 2;
+__host__ __device__ inline int brick_count(const Geom& g) {
+  return ((g.nx + kBrick - 1) / kBrick) * ((g.ny + kBrick - 1) / kBrick) * ((g.nz + kBrick - 1) / kBrick);
+}
+__device__ inline int wrap_any(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+template <bool HALF>
+__device__ inline void st_brick_body(const Geom& g, const StRef& S, const float* __restrict__ f,
+                                     const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward,
+                                     int zero_in, int b) {
+  __shared__ float su[OTM_NRHS][kHalo * kHalo * kHalo];
+  const int bx = (g.nx + kBrick - 1) / kBrick, by = (g.ny + kBrick - 1) / kBrick;
+  const int ox = (b % bx) * kBrick, oy = ((b / bx) % by) * kBrick, oz = (b / (bx * by)) * kBrick;
+  const int t = threadIdx.x;
+  const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+  const int x = ox + lx, y = oy + ly, z = oz + lz;
+  const bool active = x < g.nx && y < g.ny && z < g.nz;
+  const long long nn = g.nn;
+  const long long v = active ? (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z) : 0;
+  // halo load (or zero initial guess)
+  for (int i = t; i < kHalo * kHalo * kHalo; i += kBrickThreads) {
+    const int hx = i % kHalo, hy = (i / kHalo) % kHalo, hz = i / (kHalo * kHalo);
+    if (zero_in) {
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
+    } else {
+      const int X = wrap_any(ox + hx - 1, g.nx), Y = wrap_any(oy + hy - 1, g.ny), Z = wrap_any(oz + hz - 1, g.nz);
+      const long long w = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = uin[k * nn + w];
+    }
+  }
+  float a[27];
+  float fv[OTM_NRHS];
+  float inv = 0.f;
+  if (active) {
+    load_stencil<HALF>(S, nn, v, a);
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) fv[k] = f[k * nn + v];
+    inv = 1.f / a[13];
+  }
+  const int mycol = active ? (x & 1) + 2 * (y & 1) + 4 * (z & 1) : -1;
+  const int c0 = (lx + 1) + kHalo * ((ly + 1) + kHalo * (lz + 1));
+  __syncthreads();
+  for (int s = 0; s < sweeps; s++)
+    for (int i = 0; i < 8; i++) {
+      const int col = forward ? i : 7 - i;
+      if (col == mycol) {
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) {
+          float acc = 0.f;
+#pragma unroll
+          for (int o = 0; o < 27; o++) {
+            if (o == 13) continue;
+            const int d = (o % 3 - 1) + kHalo * (((o / 3) % 3 - 1) + kHalo * (o / 9 - 1));
+            acc += a[o] * su[k][c0 + d];
+          }
+          su[k][c0] = (fv[k] - acc) * inv;
+        }
+      }
+      __syncthreads();
+    }
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) uout[k * nn + v] = su[k][c0];
+  }
+}
+
+// ---- setup bodies (Galerkin hierarchy, FP16 packing, coarsest inverse) ----
+// Level-1 element matrices from kappa: E(E) = sum_c kappa_c G[c].
+__device__ inline void galerkin_first_body(const Geom& fg, const Geom& cg, const GalerkinConsts& G,
+                                           const float* __restrict__ kappa, float* __restrict__ E, long long t) {
+  if (t >= cg.nn) return;
+  const int X = (int)(t % cg.nx);
+  const long long r = t / cg.nx;
+  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  float kc[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    kc[c] = kappa[lin(fg, 2 * X + (c & 1), 2 * Y + ((c >> 1) & 1), 2 * Z + (c >> 2))];
+#pragma unroll
+  for (int q = 0; q < 36; q++) {
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) s += kc[c] * G.G[c][q];
+    E[q * cg.nn + t] = s;
+  }
+}
+
+// 1-D interpolation weight of a fine point p (0..2) from coarse corner M (0/1).
+__device__ constexpr float w1d(int p, int M) {
+  return (p - 2 * M == 0) ? 1.f : ((p - 2 * M == 1 || p - 2 * M == -1) ? 0.5f : 0.f);
+}
+
+// Deeper levels: Eout(E) = sum_c Q_c^T Ein(child c) Q_c.
+__device__ inline void galerkin_next_body(const Geom& fg, const Geom& cg, const float* __restrict__ Ein,
+                                          float* __restrict__ Eout, long long t) {
+  if (t >= cg.nn) return;
+  const int X = (int)(t % cg.nx);
+  const long long r = t / cg.nx;
+  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  float acc[36];
+#pragma unroll
+  for (int q = 0; q < 36; q++) acc[q] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+    const long long child = lin(fg, 2 * X + cx, 2 * Y + cy, 2 * Z + cz);
+    float e[36];
+#pragma unroll
+    for (int q = 0; q < 36; q++) e[q] = Ein[q * fg.nn + child];
+    // T = E Q  (8 x 8)
+    float T[8][8];
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        float s = 0.f;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+          const float qnb = w1d(cx + (n & 1), b & 1) * w1d(cy + ((n >> 1) & 1), (b >> 1) & 1) *
+                            w1d(cz + (n >> 2), b >> 2);
+          if (qnb != 0.f) s += e[tri8(m, n)] * qnb;
+        }
+        T[m][b] = s;
+      }
+    // acc(a,b) += sum_m Q(m,a) T(m,b)
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int b = a; b < 8; b++) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+          const float qma = w1d(cx + (m & 1), a & 1) * w1d(cy + ((m >> 1) & 1), (a >> 1) & 1) *
+                            w1d(cz + (m >> 2), a >> 2);
+          if (qma != 0.f) s += qma * T[m][b];
+        }
+        acc[tri8(a, b)] += s;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < 36; q++) Eout[q * cg.nn + t] = acc[q];
+}
+
+// Node stencil from element matrices; the diagonal is reset to minus the sum
+// of the off-diagonal couplings so that constants stay in the null space.
+__device__ inline void stencil_from_elements_body(const Geom& g, const float* __restrict__ E, float* __restrict__ st,
+                                                  long long t) {
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  float a[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) a[o] = 0.f;
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    const int ax = e & 1, ay = (e >> 1) & 1, az = e >> 2;
+    const int ex = wrapm(x - 1 + ax, g.nx), ey = wrapm(y - 1 + ay, g.ny), ez = wrapm(z - 1 + az, g.nz);
+    const long long eid = lin(g, ex, ey, ez);
+    const int lv = (1 - ax) + 2 * (1 - ay) + 4 * (1 - az);
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+      const int o = (ax + (m & 1)) + 3 * (ay + ((m >> 1) & 1)) + 9 * (az + (m >> 2));
+      a[o] += E[tri8(lv, m) * g.nn + eid];
+    }
+  }
+  double off = 0.0;
+#pragma unroll
+  for (int o = 0; o < 27; o++)
+    if (o != 13) off += (double)a[o];
+  a[13] = (float)(-off);
+#pragma unroll
+  for (int o = 0; o < 27; o++) st[o * g.nn + t] = a[o];
+}
+
+// Largest |coupling| of a level (as float bits: non-negative floats order
+// like their bit patterns), for the FP16 scale; block lb of nb.
+__device__ inline void stencil_absmax_body(const Geom& g, const float* __restrict__ st, unsigned int* __restrict__ bits,
+                                           int lb, int nb) {
+  float m = 0.f;
+  for (long long v = (long long)lb * blockDim.x + threadIdx.x; v < g.nn; v += (long long)nb * blockDim.x)
+#pragma unroll
+    for (int o = 0; o < 27; o++)
+      if (o != 13) m = fmaxf(m, fabsf(st[o * g.nn + v]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+  __shared__ float wm[kWaves];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; w++) m = fmaxf(m, wm[w]);
+    atomicMax(bits, __float_as_uint(m));
+  }
+}
+
+// FP16 copy of a level's couplings divided by the level scale. Coupling
+// (v, v+d) for d before the centre in offset order is taken from the
+// neighbour's entry (v+d, v), so the stored operator is exactly symmetric.
+__device__ inline void stencil_pack_body(const Geom& g, const float* __restrict__ st, const float* __restrict__ scale,
+                                         _Float16* __restrict__ st16, long long t) {
+  if (t >= g.nn) return;
+  const int x = (int)(t % g.nx);
+  const long long r = t / g.nx;
+  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  const float sc = *scale;
+  const float inv = sc > 0.f ? 1.f / sc : 0.f;
+#pragma unroll
+  for (int o = 0; o < 27; o++) {
+    float val = 0.f;
+    if (o > 13) {
+      val = st[o * g.nn + t];
+    } else if (o < 13) {
+      const long long w = lin(g, wrapm(x + o % 3 - 1, g.nx), wrapm(y + (o / 3) % 3 - 1, g.ny),
+                              wrapm(z + o / 9 - 1, g.nz));
+      val = st[(26 - o) * g.nn + w];
+    }
+    st16[o * g.nn + t] = (_Float16)(val * inv);
+  }
+}
+
+// Coarsest level: dense FP64 matrix from the stencil, symmetrised, zero row
+// sums, plus (tr/n^2) * 1 1^T to remove the constant null space (the
+// reference solves its coarsest system on the host with Eigen and a
+// null-space projection, grid.cpp:272-310, :432-510). The result is its
+// explicit inverse Binv, so every V-cycle's coarsest solve is one small
+// dense matrix-vector product on the device.
+constexpr int kFactorThreads = 1024;
+constexpr int kInvertLdsMax = 128;  // n*n doubles in LDS up to n = 128 (128 KiB)
+
+__device__ inline void coarsest_assemble(const Geom& g, const float* __restrict__ st, double* B, double* red,
+                                         double* alpha_sh) {
+  const int n = (int)g.nn;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int i = tid; i < n * n; i += bs) B[i] = 0.0;
+  __syncthreads();
+  for (int v = tid; v < n; v += bs) {
+    const int x = v % g.nx, y = (v / g.nx) % g.ny, z = v / (g.nx * g.ny);
+    for (int o = 0; o < 27; o++) {
+      const int w = wrapm(x + o % 3 - 1, g.nx) + g.nx * (wrapm(y + (o / 3) % 3 - 1, g.ny) +
+                                                         g.ny * wrapm(z + o / 9 - 1, g.nz));
+      B[v * n + w] += (double)st[o * n + v];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < n * n; i += bs) {
+    const int a = i / n, b = i - a * n;
+    if (a < b) {
+      const double m = 0.5 * (B[a * n + b] + B[b * n + a]);
+      B[a * n + b] = m;
+      B[b * n + a] = m;
+    }
+  }
+  __syncthreads();
+  double tr = 0.0;
+  for (int a = tid; a < n; a += bs) {
+    double s = 0.0;
+    for (int b = 0; b < n; b++)
+      if (b != a) s += B[a * n + b];
+    B[a * n + a] = -s;
+    tr += -s;
+  }
+  tr = wave_sum(tr);
+  if ((tid & 63) == 0) red[tid >> 6] = tr;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (bs + 63) / 64; w++) s += red[w];
+    *alpha_sh = s / ((double)n * (double)n);
+  }
+  __syncthreads();
+  const double alpha = *alpha_sh;
+  for (int i = tid; i < n * n; i += bs) B[i] += alpha;
+  __syncthreads();
+}
+
+// In-place Gauss-Jordan inversion of the SPD matrix A (n x n, LDS or global
+// memory): no pivoting is needed; every elimination step updates all n^2
+// entries in parallel between two barriers (2n barriers in total instead of
+// the long dependent chains of a triangular inverse). colk/rowk hold n
+// doubles each. Binv gets the symmetrised inverse.
+__device__ inline void gauss_jordan_invert(int n, double* A, double* colk, double* rowk, double* __restrict__ Binv) {
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int k = 0; k < n; k++) {
+    const double ip = 1.0 / A[k * n + k];
+    for (int i = tid; i < n; i += bs) {
+      colk[i] = A[i * n + k];
+      rowk[i] = (i == k ? 1.0 : A[k * n + i]) * ip;
+    }
+    __syncthreads();
+    if (n <= bs) {
+      const int rstep = bs / n;  // rows updated per pass
+      const int j = tid % n, i0 = tid / n;
+      if (i0 < rstep)
+        for (int i = i0; i < n; i += rstep) {
+          const int q = i * n + j;
+          A[q] = (i == k) ? rowk[j] : ((j == k ? 0.0 : A[q]) - colk[i] * rowk[j]);
+        }
+    } else {
+      for (long long q = tid; q < (long long)n * n; q += bs) {
+        const int i = (int)(q / n), j = (int)(q - (long long)i * n);
+        A[q] = (i == k) ? rowk[j] : ((j == k ? 0.0 : A[q]) - colk[i] * rowk[j]);
+      }
+    }
+    __syncthreads();
+  }
+  for (long long q = tid; q < (long long)n * n; q += bs) {
+    const int a = (int)(q / n), b = (int)(q - (long long)a * n);
+    Binv[q] = 0.5 * (A[(long long)a * n + b] + A[(long long)b * n + a]);
+  }
+}
+
+// n <= kInvertLdsMax: the work matrix lives in (dynamic) LDS.
+__device__ inline void coarsest_invert_lds_body(const Geom& g, const float* __restrict__ st, double* A,
+                                                double* __restrict__ Binv) {
+  __shared__ double colk[kInvertLdsMax], rowk[kInvertLdsMax];
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  coarsest_assemble(g, st, A, red, &alpha_sh);
+  gauss_jordan_invert((int)g.nn, A, colk, rowk, Binv);
+}
+
+// Larger coarsest levels (n <= 4096): work matrix B in global memory,
+// colk/rowk in the 2n doubles after it.
+__device__ inline void coarsest_invert_global_body(const Geom& g, const float* __restrict__ st, double* B,
+                                                   double* __restrict__ Binv) {
+  __shared__ double red[kFactorThreads / 64];
+  __shared__ double alpha_sh;
+  const int n = (int)g.nn;
+  coarsest_assemble(g, st, B, red, &alpha_sh);
+  gauss_jordan_invert(n, B, B + (long long)n * n, B + (long long)n * n + n, Binv);
+}
+
+}  // namespace otm

@@ -99,17 +99,19 @@ __device__ inline int xcd_block() {
 // Global thread index of one-item-per-thread kernels, XCD-ordered.
 __device__ inline long long gtid() { return (long long)xcd_block() * blockDim.x + threadIdx.x; }
 
-// Contiguous item range [b, e) of this block for fixed-grid streaming loops.
+// Contiguous item range [b, e) of block lb of nb blocks for fixed-grid
+// streaming loops (block_range: this launch's XCD-ordered block).
 struct BlockRange {
   long long b, e;
 };
-__device__ inline BlockRange block_range(long long n) {
-  const long long per = ((n + gridDim.x - 1) / gridDim.x + blockDim.x - 1) / blockDim.x * blockDim.x;
-  long long s = (long long)xcd_block() * per, e = s + per;
+__device__ inline BlockRange block_range_v(long long n, int lb, int nb) {
+  const long long per = ((n + nb - 1) / nb + blockDim.x - 1) / blockDim.x * blockDim.x;
+  long long s = (long long)lb * per, e = s + per;
   if (s > n) s = n;
   if (e > n) e = n;
   return {s, e};
 }
+__device__ inline BlockRange block_range(long long n) { return block_range_v(n, xcd_block(), gridDim.x); }
 
 // Wave64 sum through cross-lane shuffles.
 __device__ inline double wave_sum(double v) {
@@ -145,13 +147,19 @@ __device__ inline void block_sum(double (&v)[NV], double* sh) {
 // finaliser; the kernel boundary makes the partials visible without
 // device-scope fences inside the producing kernel.
 template <int NV>
-__device__ inline void reduce_block(double (&v)[NV], const Reduce& R) {
+__device__ inline void reduce_block_at(double (&v)[NV], const Reduce& R, int slot) {
   __shared__ double sh[NV * kWaves];
   block_sum<NV>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) R.partials[(size_t)blockIdx.x * NV + i] = v[i];
+    for (int i = 0; i < NV; i++) R.partials[(size_t)slot * NV + i] = v[i];
   }
+}
+// The partial's slot is the block's range index, so the finaliser adds the
+// partials in item order whatever the launch shape.
+template <int NV>
+__device__ inline void reduce_block(double (&v)[NV], const Reduce& R) {
+  reduce_block_at<NV>(v, R, xcd_block());
 }
 
 // Applies a finaliser to the summed values (one thread).

@@ -19,65 +19,15 @@
 #include <vector>
 
 #include "../../include/opentm.h"
-#include "otm_kernels.hpp"
+#include "otm_ctx.hpp"
 
 using namespace otm;
 
 // ---- error plumbing --------------------------------------------------------
 static thread_local std::string g_last_error;
+void otm::set_last_error(const std::string& msg) { g_last_error = msg; }
 
 namespace {
-struct OtmError {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw OtmError{code, msg}; }
-
-inline void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess) fail(OTM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-int guarded(F&& f) {
-  try {
-    f();
-    return OTM_OK;
-  } catch (const OtmError& e) {
-    g_last_error = e.msg;
-    return e.code;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return OTM_E_ARG;
-  }
-}
-
-template <typename T>
-T* dalloc(size_t count) {
-  void* p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc(&p, count * sizeof(T));
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    fail(OTM_E_OOM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
-  }
-  return static_cast<T*>(p);
-}
-
-inline void dfree(void* p) {
-  if (p) (void)hipFree(p);
-}
-
-inline hipMemcpyKind kind_of(int dst_where, int src_where) {
-  if (dst_where == OTM_DEVICE && src_where == OTM_DEVICE) return hipMemcpyDeviceToDevice;
-  if (dst_where == OTM_DEVICE) return hipMemcpyHostToDevice;
-  if (src_where == OTM_DEVICE) return hipMemcpyDeviceToHost;
-  return hipMemcpyHostToHost;
-}
-
-void check_where(int w) {
-  if (w != OTM_HOST && w != OTM_DEVICE) fail(OTM_E_ARG, "where must be OTM_HOST or OTM_DEVICE");
-}
 
 // Element constants (closed forms of templateMatrix.cpp:21-87 after the node
 // remap {1,3,2,0,5,7,6,4}): load case k is a unit gradient along grid axis
@@ -166,143 +116,7 @@ Taps make_taps(int kind, double r, int periodic) {
   return T;
 }
 
-unsigned red_blocks_for(long long n) {
-  long long b = (n + kBlock - 1) / kBlock;
-  if (b > 1024) b = 1024;
-  if (b < 1) b = 1;
-  return (unsigned)b;
-}
-
-struct Level {
-  Geom g;
-  float* u = nullptr;    // correction (level 0: the PCG z vector)
-  float* f = nullptr;    // right-hand side (level 0: the PCG residual)
-  float* res = nullptr;  // residual before restriction
-  float* st = nullptr;   // 27-point stencil (levels >= 1)
-  float* E = nullptr;    // element matrices, 36 per element (levels >= 1)
-  float* u2 = nullptr;   // second correction buffer for the brick smoother
-  _Float16* st16 = nullptr;  // FP16 couplings (fp16_stencils)
-  float* scale = nullptr;    // their per-level scale (device scalar)
-  bool brick = false;    // smoothed with LDS bricks (edges multiples of 8)
-  bool own_uf = true;
-};
-
-inline StRef stref(const Level& L) { return StRef{L.st, L.st16, L.scale}; }
-
 }  // namespace
-
-struct otm_ctx {
-  otm_config cfg;
-  int device = 0;
-  hipStream_t s = nullptr;
-  Geom g0;
-  FineConsts fc;
-  GalerkinConsts G;
-  std::vector<Level> lv;
-  // fine-level PCG state, 3 right-hand sides each
-  float* kappa = nullptr;  // per element, with the 1e-9 floor
-  float* x = nullptr;      // chi (warm start); the FP32 correction while refining
-  double* x64 = nullptr;   // FP64 chi of the iterative refinement (fp64_refine)
-  float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
-  float* z2 = nullptr;  // second fine correction buffer (fine brick smoother)
-  // coarsest dense solve
-  double *cB = nullptr, *cX = nullptr, *cBinv = nullptr;
-  // reductions
-  double* partials = nullptr;
-  double* red_out = nullptr;
-  PcgScalars* pcg = nullptr;
-  PcgScalars* pcg_host = nullptr;  // pinned
-  double* out_host = nullptr;      // pinned, 8 doubles
-  // design pipeline
-  float *rho = nullptr, *rho_f = nullptr, *W = nullptr, *kraw = nullptr, *dkappa = nullptr;
-  float *sens = nullptr, *tmp = nullptr, *drho = nullptr;
-  Taps taps;
-  int taps_kind = -2;
-  double taps_r = 0;
-  int taps_periodic = -1;
-  // live kernel timing (otm_profile_enable)
-  bool prof_on = false;
-  int prof_mask = 0;
-  std::vector<hipEvent_t> ev_pool;
-  struct Pending {
-    int cls;
-    double bytes;
-    hipEvent_t a, b;
-  };
-  std::vector<Pending> pending;
-  bool bricks_on = true;   // coarse levels use the LDS brick smoother
-  int fallbacks = 0;       // solves that fell back to exact Gauss-Seidel
-  long long prof_launches[OTM_PROF_NCLASSES] = {};
-  double prof_ms[OTM_PROF_NCLASSES] = {};
-  double prof_bytes[OTM_PROF_NCLASSES] = {};
-  // status
-  bool has_kappa = false, has_solution = false, has_drho = false;
-  int iters[3] = {0, 0, 0};
-  double relres[3] = {0, 0, 0};
-  double C[9];
-
-  Reduce reducer(int mode, double rtol2 = 0.0, double inv_nn = 0.0) const {
-    Reduce R;
-    R.partials = partials;
-    R.out = red_out;
-    R.pcg = pcg;
-    R.rtol2 = rtol2;
-    R.inv_nn = inv_nn;
-    R.mode = mode;
-    return R;
-  }
-  void sync() {
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    if (!pending.empty()) collect();
-  }
-  hipEvent_t take_event() {
-    if (ev_pool.empty()) {
-      hipEvent_t e;
-      hip_check(hipEventCreate(&e), "hipEventCreate");
-      return e;
-    }
-    hipEvent_t e = ev_pool.back();
-    ev_pool.pop_back();
-    return e;
-  }
-  void collect() {
-    for (auto& p : pending) {
-      float ms = 0.f;
-      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
-      prof_ms[p.cls] += ms;
-      prof_launches[p.cls] += 1;
-      prof_bytes[p.cls] += p.bytes;
-      ev_pool.push_back(p.a);
-      ev_pool.push_back(p.b);
-    }
-    pending.clear();
-  }
-  void read_pcg() {
-    hip_check(hipMemcpyAsync(pcg_host, pcg, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), "read pcg scalars");
-    sync();
-  }
-};
-
-// Records a start/stop event pair around the launches in its scope when
-// timing is on; `bytes` is the algorithmic traffic of those launches.
-struct ProfScope {
-  otm_ctx* c;
-  int cls;
-  double bytes;
-  hipEvent_t a = nullptr;
-  ProfScope(otm_ctx* c_, int cls_, double bytes_) : c(c_), cls(cls_), bytes(bytes_) {
-    if (c->prof_on && ((c->prof_mask >> cls) & 1)) {
-      a = c->take_event();
-      hip_check(hipEventRecord(a, c->s), "hipEventRecord");
-    }
-  }
-  ~ProfScope() {
-    if (a) {
-      hipEvent_t b = c->take_event();
-      if (hipEventRecord(b, c->s) == hipSuccess) c->pending.push_back({cls, bytes, a, b});
-    }
-  }
-};
 
 struct otm_oc {
   int ne = 0;
@@ -335,7 +149,7 @@ static void ctx_free(otm_ctx* c) {
     if (L.own_uf) dfree(L.u2);
   }
   dfree(c->kappa); dfree(c->x); dfree(c->x64); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
-  dfree(c->cB); dfree(c->cX); dfree(c->cBinv);
+  dfree(c->cB); dfree(c->cBinv);
   dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
   if (c->out_host) (void)hipHostFree(c->out_host);
@@ -410,8 +224,7 @@ static void build_hierarchy(otm_ctx* c) {
     if (l + 1 < geoms.size()) L.res = dalloc<float>(3 * n);
     c->lv.push_back(L);
   }
-  c->cB = dalloc<double>(ncoarse * ncoarse);
-  c->cX = dalloc<double>(ncoarse * ncoarse);
+  c->cB = dalloc<double>(ncoarse * ncoarse + 2 * ncoarse);  // Gauss-Jordan work matrix + pivot row/column
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
   c->partials = dalloc<double>(2048 * kMaxReduce);
   c->red_out = dalloc<double>(16);
@@ -445,7 +258,7 @@ static void build_operators(otm_ctx* c) {
     if (L.st16) launch_stencil_pack(c->s, L.g, L.st, L.scale, L.st16);
   }
   const Level& Lc = c->lv.back();
-  launch_coarsest_factor(c->s, Lc.g, Lc.st, c->cB, c->cX, c->cBinv);
+  launch_coarsest_factor(c->s, Lc.g, Lc.st, c->cB, c->cBinv);
   hip_check(hipGetLastError(), "operator build");
 }
 
@@ -1516,3 +1329,10 @@ OTM_API int otm_synchronize(otm_ctx* ctx) {
 }
 
 }  // extern "C"
+
+// Solve with exact multicolour Gauss-Seidel on every level (the batched
+// runtime's fallback for a cell whose brick smoother lost positivity).
+void otm::ctx_solve_exact(otm_ctx* c, double C[9]) {
+  set_bricks(c, false);
+  do_heat_matrix(c, C);
+}

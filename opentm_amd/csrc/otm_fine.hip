@@ -1,448 +1,58 @@
-// otm_fine.hip -- matrix-free fine-level kernels (gfx950, wave64).
-//
-// The fine operator is the SIMP-weighted trilinear conductivity stencil
-// assembled on the fly from per-element kappa (reference: grid.cu:318-471
-// smoother, grid.cu:1027-1164 residual, grid.cu:1282-1342 load vectors).
-// With the element matrix Ke = c*[4 diag, 0 one-axis neighbours, -1 else],
-// node v couples only to its 12 edge and 8 corner neighbours:
-//   (K u)_v = c * sum_{e ni v} kappa_e * (4 u_v - T_e(u)),
-// where T_e sums u over the three face-diagonal and the body-diagonal nodes
-// of v inside element e. Because one-axis neighbours do not couple, the
-// reference's 8-colour Gauss-Seidel (colour = parity of x,y,z, in order
-// 0..7) is exactly a 4-colour sweep over whole x-rows with colour
-// (y%2) + 2*(z%2): colours 2k and 2k+1 of the 8-colour order never couple.
-#include "otm_kernels.hpp"
+// otm_fine.hip -- single-cell launches of the matrix-free fine-level
+// kernels; the device bodies are in otm_fine_dev.hpp.
+#include "otm_fine_dev.hpp"
 
 namespace otm {
 
-// kappa of the 8 elements around node (x,y,z) and their sum.
-struct NodeElems {
-  float k[8];  // index ax + 2 ay + 4 az: element at (x-1+ax, y-1+ay, z-1+az)
-  float ksum;
-};
-
-struct NodeIdx {
-  int x, y, z, xm, xp, ym, yp, zm, zp;
-  long long v;
-};
-
-__device__ inline NodeIdx node_idx(const Geom& g, int x, int y, int z) {
-  NodeIdx n;
-  n.x = x; n.y = y; n.z = z;
-  n.xm = x == 0 ? g.nx - 1 : x - 1;
-  n.xp = x == g.nx - 1 ? 0 : x + 1;
-  n.ym = y == 0 ? g.ny - 1 : y - 1;
-  n.yp = y == g.ny - 1 ? 0 : y + 1;
-  n.zm = z == 0 ? g.nz - 1 : z - 1;
-  n.zp = z == g.nz - 1 ? 0 : z + 1;
-  n.v = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
-  return n;
-}
-
-__device__ inline long long rowb(const Geom& g, int y, int z) {
-  return (long long)g.nx * ((long long)y + (long long)g.ny * z);
-}
-
-__device__ inline NodeElems load_elems(const Geom& g, const NodeIdx& n, const float* __restrict__ kappa) {
-  NodeElems E;
-  const long long r00 = rowb(g, n.ym, n.zm), r10 = rowb(g, n.y, n.zm);
-  const long long r01 = rowb(g, n.ym, n.z), r11 = rowb(g, n.y, n.z);
-  E.k[0] = kappa[r00 + n.xm]; E.k[1] = kappa[r00 + n.x];
-  E.k[2] = kappa[r10 + n.xm]; E.k[3] = kappa[r10 + n.x];
-  E.k[4] = kappa[r01 + n.xm]; E.k[5] = kappa[r01 + n.x];
-  E.k[6] = kappa[r11 + n.xm]; E.k[7] = kappa[r11 + n.x];
-  E.ksum = ((E.k[0] + E.k[1]) + (E.k[2] + E.k[3])) + ((E.k[4] + E.k[5]) + (E.k[6] + E.k[7]));
-  return E;
-}
-
-// sum_e kappa_e T_e(u) for one right-hand side.
-__device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems& E,
-                                const float* __restrict__ u) {
-  // rows touched: (y+-1, z), (y, z+-1), (y+-1, z+-1)
-  const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
-  const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
-  const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
-  const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
-  // U(dx,dy,dz) naming: first letter x (m/0/p), then y, then z
-  const float u_mm0 = u[rM0 + n.xm], u_pm0 = u[rM0 + n.xp];
-  const float u_mp0 = u[rP0 + n.xm], u_pp0 = u[rP0 + n.xp];
-  const float u_m0m = u[r0m + n.xm], u_p0m = u[r0m + n.xp];
-  const float u_m0p = u[r0p + n.xm], u_p0p = u[r0p + n.xp];
-  const float u_mmm = u[rMm + n.xm], u_0mm = u[rMm + n.x], u_pmm = u[rMm + n.xp];
-  const float u_mmp = u[rMp + n.xm], u_0mp = u[rMp + n.x], u_pmp = u[rMp + n.xp];
-  const float u_mpm = u[rPm + n.xm], u_0pm = u[rPm + n.x], u_ppm = u[rPm + n.xp];
-  const float u_mpp = u[rPp + n.xm], u_0pp = u[rPp + n.x], u_ppp = u[rPp + n.xp];
-  // element (ax,ay,az) lies on side s = 2a-1; far nodes (sx,sy,0) (sx,0,sz) (0,sy,sz) (sx,sy,sz)
-  float s = 0.f;
-  s += E.k[0] * (u_mm0 + u_m0m + u_0mm + u_mmm);
-  s += E.k[1] * (u_pm0 + u_p0m + u_0mm + u_pmm);
-  s += E.k[2] * (u_mp0 + u_m0m + u_0pm + u_mpm);
-  s += E.k[3] * (u_pp0 + u_p0m + u_0pm + u_ppm);
-  s += E.k[4] * (u_mm0 + u_m0p + u_0mp + u_mmp);
-  s += E.k[5] * (u_pm0 + u_p0p + u_0mp + u_pmp);
-  s += E.k[6] * (u_mp0 + u_m0p + u_0pp + u_mpp);
-  s += E.k[7] * (u_pp0 + u_p0p + u_0pp + u_ppp);
-  return s;
-}
-
-// FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
-template <typename XT>
-__device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeElems& E,
-                                   const XT* __restrict__ u) {
-  const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
-  const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
-  const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
-  const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
-  const double u_mm0 = u[rM0 + n.xm], u_pm0 = u[rM0 + n.xp];
-  const double u_mp0 = u[rP0 + n.xm], u_pp0 = u[rP0 + n.xp];
-  const double u_m0m = u[r0m + n.xm], u_p0m = u[r0m + n.xp];
-  const double u_m0p = u[r0p + n.xm], u_p0p = u[r0p + n.xp];
-  const double u_mmm = u[rMm + n.xm], u_0mm = u[rMm + n.x], u_pmm = u[rMm + n.xp];
-  const double u_mmp = u[rMp + n.xm], u_0mp = u[rMp + n.x], u_pmp = u[rMp + n.xp];
-  const double u_mpm = u[rPm + n.xm], u_0pm = u[rPm + n.x], u_ppm = u[rPm + n.xp];
-  const double u_mpp = u[rPp + n.xm], u_0pp = u[rPp + n.x], u_ppp = u[rPp + n.xp];
-  double s = 0.0;
-  s += (double)E.k[0] * (u_mm0 + u_m0m + u_0mm + u_mmm);
-  s += (double)E.k[1] * (u_pm0 + u_p0m + u_0mm + u_pmm);
-  s += (double)E.k[2] * (u_mp0 + u_m0m + u_0pm + u_mpm);
-  s += (double)E.k[3] * (u_pp0 + u_p0m + u_0pm + u_ppm);
-  s += (double)E.k[4] * (u_mm0 + u_m0p + u_0mp + u_mmp);
-  s += (double)E.k[5] * (u_pm0 + u_p0p + u_0mp + u_pmp);
-  s += (double)E.k[6] * (u_mp0 + u_m0p + u_0pp + u_mpp);
-  s += (double)E.k[7] * (u_pp0 + u_p0p + u_0pp + u_ppp);
-  return s;
-}
-
-// Load vector of case k at the node (grid.cu:1282-1342): sum_e kappa_e feMu[7-e][k].
-__device__ inline float load_vec(const NodeElems& E, const FineConsts& fc, int k) {
-  float s = 0.f;
-#pragma unroll
-  for (int e = 0; e < 8; e++) s += E.k[e] * fc.femu[7 - e][k];
-  return s;
-}
-
-// One colour of the x-row Gauss-Seidel sweep: u_v = (f_v + c*far)/(4c*ksum).
 __global__ void __launch_bounds__(kBlock) k_fine_gs_color(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                           const float* __restrict__ f, float* u, int cy, int cz) {
-  const int hy = g.ny >> 1, hz = g.nz >> 1;
-  const long long cnt = (long long)g.nx * hy * hz;
-  const long long t = gtid();
-  if (t >= cnt) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
-  const NodeIdx n = node_idx(g, x, y, z);
-  const NodeElems E = load_elems(g, n, kappa);
-  const float inv = 1.f / (4.f * fc.c * E.ksum);
-#pragma unroll
-  for (int k = 0; k < OTM_NRHS; k++) {
-    const float* uk = u + k * g.nn;
-    const float fs = far_sum(g, n, E, uk);
-    u[k * g.nn + n.v] = (f[k * g.nn + n.v] + fc.c * fs) * inv;
-  }
+  fine_gs_color_body(g, fc, kappa, f, u, cy, cz, gtid());
 }
 
-// res = f - K u on every node.
 __global__ void __launch_bounds__(kBlock) k_fine_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                           const float* __restrict__ f, const float* __restrict__ u,
                                                           float* __restrict__ res) {
-  const long long t = gtid();
-  if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-  const NodeIdx n = node_idx(g, x, y, z);
-  const NodeElems E = load_elems(g, n, kappa);
-#pragma unroll
-  for (int k = 0; k < OTM_NRHS; k++) {
-    const float* uk = u + k * g.nn;
-    const float ku = fc.c * (4.f * E.ksum * uk[n.v] - far_sum(g, n, E, uk));
-    res[k * g.nn + n.v] = f[k * g.nn + n.v] - ku;
-  }
+  fine_residual_body(g, fc, kappa, f, u, res, gtid());
 }
 
-// q = K p and partial sums of p.q (PCG step).
 __global__ void __launch_bounds__(kBlock) k_fine_apply_dot(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                            const float* __restrict__ p, float* __restrict__ q,
                                                            Reduce R) {
-  double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
-  for (long long t = block_range(g.nn).b + threadIdx.x, t_end = block_range(g.nn).e; t < t_end; t += blockDim.x) {
-    const int x = (int)(t % g.nx);
-    const long long r = t / g.nx;
-    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-    const NodeIdx n = node_idx(g, x, y, z);
-    const NodeElems E = load_elems(g, n, kappa);
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) {
-      const float* pk = p + k * g.nn;
-      const float pv = pk[n.v];
-      const float kp = fc.c * (4.f * E.ksum * pv - far_sum(g, n, E, pk));
-      q[k * g.nn + n.v] = kp;
-      acc[k] += (double)pv * (double)kp;
-    }
-  }
-  reduce_block<OTM_NRHS>(acc, R);
+  fine_apply_dot_body(g, fc, kappa, p, q, R, xcd_block(), gridDim.x);
 }
 
-// r = b - K x in FP64 with b the load vectors built on the fly (the FP64
-// sum of the eight element contributions cancels exactly wherever kappa
-// does not vary along the load direction); stored rounded to FP32. Partial
-// sums of b.b, r.r and sum(r) (PCG start and restarts). XT is the iterate's
-// type: FP32, or FP64 for the iterative refinement.
 template <typename XT>
 __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                                const XT* __restrict__ x_, float* __restrict__ rvec,
                                                                Reduce R) {
-  double acc[3 * OTM_NRHS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const double c = (double)fc.c;
-  for (long long t = block_range(g.nn).b + threadIdx.x, t_end = block_range(g.nn).e; t < t_end; t += blockDim.x) {
-    const int x = (int)(t % g.nx);
-    const long long r = t / g.nx;
-    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-    const NodeIdx n = node_idx(g, x, y, z);
-    const NodeElems E = load_elems(g, n, kappa);
-    double ksum = 0.0;
-#pragma unroll
-    for (int e = 0; e < 8; e++) ksum += (double)E.k[e];
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) {
-      const XT* xk = x_ + k * g.nn;
-      double b = 0.0;
-#pragma unroll
-      for (int e = 0; e < 8; e++) b += (double)E.k[e] * (double)fc.femu[7 - e][k];
-      const double kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
-      const double rv = b - kx;
-      rvec[k * g.nn + n.v] = (float)rv;
-      acc[k] += b * b;
-      acc[OTM_NRHS + k] += rv * rv;
-      acc[2 * OTM_NRHS + k] += rv;
-    }
-  }
-  reduce_block<3 * OTM_NRHS>(acc, R);
+  fine_init_residual_body<XT>(g, fc, kappa, x_, rvec, R, xcd_block(), gridDim.x);
 }
 
-// Element energies (homogenization.cu:13-153 and :255-412). For element e
-// with nodal fields d_k = disp[:,k] - chi_k(nodes): S_ij = d_i^T Ke d_j,
-// using (Ke d)_a = c (5 d_a + d_{a^1} + d_{a^2} + d_{a^4} - sum d).
-__device__ inline void element_S(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
-                                 int x, int y, int z, float S[6]) {
-  const int xp = x == g.nx - 1 ? 0 : x + 1;
-  const int yp = y == g.ny - 1 ? 0 : y + 1;
-  const int zp = z == g.nz - 1 ? 0 : z + 1;
-  const long long r0 = rowb(g, y, z), r1 = rowb(g, yp, z), r2 = rowb(g, y, zp), r3 = rowb(g, yp, zp);
-  const long long nid[8] = {r0 + x, r0 + xp, r1 + x, r1 + xp, r2 + x, r2 + xp, r3 + x, r3 + xp};
-  float d[3][8], kd[3][8];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float tot = 0.f;
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-      d[k][m] = fc.disp[m][k] - chi[k * g.nn + nid[m]];
-      tot += d[k][m];
-    }
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-      kd[k][a] = fc.c * (5.f * d[k][a] + d[k][a ^ 1] + d[k][a ^ 2] + d[k][a ^ 4] - tot);
-  }
-  int c = 0;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = i; j < 3; j++) {
-      float s = 0.f;
-#pragma unroll
-      for (int a = 0; a < 8; a++) s += d[i][a] * kd[j][a];
-      S[c++] = s;
-    }
-}
-
-// kappa^H_ij = sum_e (kappa_e) S_e(i,j) with FP64 accumulation; kappa
-// already holds the 1e-9 floor.
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                          const float* __restrict__ chi, Reduce R) {
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (long long t = block_range(g.nn).b + threadIdx.x, t_end = block_range(g.nn).e; t < t_end; t += blockDim.x) {
-    const int x = (int)(t % g.nx);
-    const long long r = t / g.nx;
-    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-    float S[6];
-    element_S(g, fc, chi, x, y, z, S);
-    const double ke = (double)kappa[t];
-#pragma unroll
-    for (int c = 0; c < 6; c++) acc[c] += ke * (double)S[c];
-  }
-  reduce_block<6>(acc, R);
+  fine_kappa_h_body(g, fc, kappa, chi, R, xcd_block(), gridDim.x);
 }
 
-// sens[x + pitch*(y + ny*z)] = sum_c w_c S_c (contracted sensitivity with
-// respect to kappa; the penalty exponent is 1, homogenization.cu:344-346).
 __global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, const float* __restrict__ chi,
                                                              float w0, float w1, float w2, float w3, float w4, float w5,
                                                              float* __restrict__ sens, int pitch) {
-  const long long t = gtid();
-  if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-  float S[6];
-  element_S(g, fc, chi, x, y, z, S);
-  const float s = S[0] * w0 + S[1] * w1 + S[2] * w2 + S[3] * w3 + S[4] * w4 + S[5] * w5;
-  sens[(long long)x + (long long)pitch * ((long long)y + (long long)g.ny * z)] = s;
+  const float w[6] = {w0, w1, w2, w3, w4, w5};
+  fine_sensitivity_body(g, fc, chi, w, sens, pitch, gtid());
 }
 
-// S6[c*nn + e]: the six element energies, for callers that keep them.
 __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, const float* __restrict__ chi,
                                                          float* __restrict__ S6) {
-  const long long t = gtid();
-  if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
-  float S[6];
-  element_S(g, fc, chi, x, y, z, S);
-#pragma unroll
-  for (int c = 0; c < 6; c++) S6[c * g.nn + t] = S[c];
+  fine_energy6_body(g, fc, chi, S6, gtid());
 }
 
-// Fine-level brick smoother: a workgroup owns BX x 8 x 8 nodes and stages
-// the kappa of every element touching them plus u of the brick and a
-// one-node halo in LDS, then runs `sweeps` x-row Gauss-Seidel sweeps (4
-// colours) with workgroup barriers. Halo values stay frozen at their entry
-// values and the result goes to a separate buffer (block Gauss-Seidel across
-// bricks, exact colour order inside); the post-smoother runs the colours in
-// reverse. Each thread updates two nodes of the active colour per step.
-//
-// A node couples to its 12 face-diagonal neighbours with the summed
-// conductivity of the two elements they share and to its 8 body-diagonal
-// neighbours with one element's conductivity (one-axis neighbours have a
-// zero coefficient), so an update is 20 LDS reads and 20 FMAs per load case
-// plus 12 adds shared by the three. Staging reads whole x-rows of the brick
-// (coalesced) and the two halo columns separately, with no per-element
-// index division.
-template <int BX>
 // The occupancy hint keeps the fully unrolled colour loop from holding
 // every node's loads in flight at once (256 VGPRs, one wave per SIMD); the
 // LDS footprint allows three workgroups per CU anyway.
+template <int BX>
 __global__ void __launch_bounds__(BX * 8) __attribute__((amdgpu_waves_per_eu(3, 8)))
-k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa,
-                                                               const float* __restrict__ f,
-                                                               const float* __restrict__ uin, float* __restrict__ uout,
-                                                               int sweeps, int forward, int zero_in) {
-  constexpr int BY = 8, BZ = 8;
-  constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
-  constexpr int KX = BX + 1, KY = BY + 1, KZ = BZ + 1;
-  constexpr int NT = BX * 8;
-  __shared__ float su[OTM_NRHS][HZ * HY * HX];
-  __shared__ float sk[KZ * KY * KX];
-  const int bx = g.nx / BX, by = g.ny / BY;
-  const int b = xcd_block();
-  const int ox = (b % bx) * BX, oy = ((b / bx) % by) * BY, oz = (b / (bx * by)) * BZ;
-  const int t = threadIdx.x;
-  const int tx = t % BX, ty = t / BX;  // ty in [0, 8)
-  const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
-  const int xm = wrapm(ox - 1, g.nx), xp = wrapm(ox + BX, g.nx);
-  // kappa of element (ox-1+kx, oy-1+ky, oz-1+kz) at sk[kx + KX*(ky + KY*kz)]
-  for (int r = ty; r < KY * KZ; r += 8) {
-    const int ky = r % KY, kz = r / KY;
-    const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
-    sk[r * KX + 1 + tx] = kappa[row + ox + tx];
-  }
-  for (int r = t; r < KY * KZ; r += NT) {
-    const int ky = r % KY, kz = r / KY;
-    const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
-    sk[r * KX] = kappa[row + xm];
-  }
-  if (zero_in) {
-    for (int i = t; i < HX * HY * HZ; i += NT) {
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
-    }
-  } else {
-    for (int r = ty; r < HY * HZ; r += 8) {
-      const int hy = r % HY, hz = r / HY;
-      const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + 1 + tx] = uin[k * g.nn + row + ox + tx];
-    }
-    for (int i = t; i < 2 * HY * HZ; i += NT) {
-      const int r = i >> 1, side = i & 1;
-      const int hy = r % HY, hz = r / HY;
-      const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
-      const int hx = side ? HX - 1 : 0;
-      const long long w = row + (side ? xp : xm);
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + hx] = uin[k * g.nn + w];
-    }
-  }
-  // node ownership: x = tx, and for colour (cy, cz) the rows
-  // y = cy + 2*(ty%4), z = cz + 2*(ty/4) and z + 4
-  const int lx = tx;
-  const int ry = 2 * (ty & 3), rz = 2 * (ty >> 2);
-  float fv[4][2][OTM_NRHS];
-#pragma unroll
-  for (int col = 0; col < 4; col++)
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
-      const long long v = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) fv[col][h][k] = f[k * g.nn + v];
-    }
-  __syncthreads();
-  const float c = fc.c;
-  constexpr int DY = HX, DZ = HX * HY;
-  for (int s = 0; s < sweeps; s++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int col = forward ? i : 3 - i;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
-        // kappa of the 8 elements around the node, e = ax + 2ay + 4az: element
-        // (x-1+ax, y-1+ay, z-1+az) at sk(lx+ax, ly+ay, lz+az)
-        const int kb = lx + KX * (ly + KY * lz);
-        float ke[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) ke[e] = sk[kb + (e & 1) + KX * (((e >> 1) & 1) + KY * (e >> 2))];
-        float wxy[4], wxz[4], wyz[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int a0 = q & 1, a1 = q >> 1;
-          wxy[q] = ke[a0 + 2 * a1] + ke[a0 + 2 * a1 + 4];  // neighbour (+-1, +-1, 0)
-          wxz[q] = ke[a0 + 4 * a1] + ke[a0 + 2 + 4 * a1];  // neighbour (+-1, 0, +-1)
-          wyz[q] = ke[2 * a0 + 4 * a1] + ke[1 + 2 * a0 + 4 * a1];  // neighbour (0, +-1, +-1)
-        }
-        const float ks = (wxy[0] + wxy[1]) + (wxy[2] + wxy[3]);
-        const float inv = __builtin_amdgcn_rcpf(4.f * c * ks);
-        const int c0 = (lx + 1) + DY * (ly + 1) + DZ * (lz + 1);
-#pragma unroll
-        for (int k = 0; k < OTM_NRHS; k++) {
-          const float* U = su[k];
-          float acc = 0.f;
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int s0 = 2 * (q & 1) - 1, s1 = 2 * (q >> 1) - 1;
-            acc += wxy[q] * U[c0 + s0 + DY * s1];
-            acc += wxz[q] * U[c0 + s0 + DZ * s1];
-            acc += wyz[q] * U[c0 + DY * s0 + DZ * s1];
-          }
-#pragma unroll
-          for (int e = 0; e < 8; e++) {
-            const int sx = 2 * (e & 1) - 1, sy = 2 * ((e >> 1) & 1) - 1, sz = 2 * (e >> 2) - 1;
-            acc += ke[e] * U[c0 + sx + DY * sy + DZ * sz];
-          }
-          su[k][c0] = (fv[col][h][k] + c * acc) * inv;
-        }
-      }
-      __syncthreads();
-    }
-  for (int r = ty; r < BY * BZ; r += 8) {
-    const int ly = r % BY, lz = r / BY;
-    const long long v = (ox + tx) + nx * (oy + ly) + nxy * (oz + lz);
-    const int c0 = (tx + 1) + DY * (ly + 1) + DZ * (lz + 1);
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) uout[k * g.nn + v] = su[k][c0];
-  }
+k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa, const float* __restrict__ f,
+                    const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward, int zero_in) {
+  fine_brick_body<BX>(g, fc, kappa, f, uin, uout, sweeps, forward, zero_in, xcd_block());
 }
 
 int fine_brick_x(const Geom& g) {

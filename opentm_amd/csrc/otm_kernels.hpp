@@ -44,7 +44,7 @@ void launch_galerkin_first(hipStream_t s, const Geom& fine, const Geom& coarse, 
                            const float* kappa, float* E);
 void launch_galerkin_next(hipStream_t s, const Geom& fine, const Geom& coarse, const float* Ein, float* Eout);
 void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, float* st);
-void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* X, double* Binv);
+void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* Binv);
 void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u);
 
 // Brick smoother (8^3 bricks, frozen halo, result in uout).

@@ -1,58 +1,32 @@
-// otm_vec.hip -- vector updates and FP64-accumulated reductions for the
-// lock-step three-right-hand-side preconditioned conjugate gradient.
-// The reference sums in FP32 through cub (culib/lib.cuh:595-760); here every
-// inner product accumulates in FP64 with a fixed summation order.
-#include "otm_kernels.hpp"
+// otm_vec.hip -- single-cell launches of the PCG vector updates and FP64
+// reductions (device bodies in otm_vec_dev.hpp), plus copies and fills.
+#include "otm_vec_dev.hpp"
 
 namespace otm {
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 __global__ void __launch_bounds__(kBlock) k_fill(float* a, long long n, float v) {
-  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x)
-    a[i] = v;
+  fill_body(a, n, v, xcd_block(), gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __restrict__ a,
                                                  const float* __restrict__ b, Reduce R) {
-  double acc[OTM_NRHS] = {0, 0, 0};
-  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i] * (double)b[k * nn + i];
-  }
-  reduce_block<OTM_NRHS>(acc, R);
+  dot3_body(nn, a, b, R, xcd_block(), gridDim.x);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const T* __restrict__ a, Reduce R) {
-  double acc[OTM_NRHS] = {0, 0, 0};
-  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i];
-  }
-  reduce_block<OTM_NRHS>(acc, R);
+  sum3_body<T>(nn, a, R, xcd_block(), gridDim.x);
 }
 
-// Iterative refinement: x64 += d (the FP32 correction), d = 0.
 __global__ void __launch_bounds__(kBlock) k_refine_add(long long n, double* __restrict__ x64, float* __restrict__ d) {
-  for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
-    x64[i] += (double)d[i];
-    d[i] = 0.f;
-  }
+  refine_add_body(n, x64, d, xcd_block(), gridDim.x);
 }
 
-// x64 -= mean (per load case), x32 = x64 rounded.
 __global__ void __launch_bounds__(kBlock) k_sub_mean_cast(long long nn, double* __restrict__ x64,
                                                           float* __restrict__ x32, const PcgScalars* __restrict__ P) {
-  const double m[OTM_NRHS] = {P->mean[0], P->mean[1], P->mean[2]};
-  for (long long i = gtid(); i < nn; i += (long long)gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) {
-      const double v = x64[k * nn + i] - m[k];
-      x64[k * nn + i] = v;
-      x32[k * nn + i] = (float)v;
-    }
-  }
+  sub_mean_cast_body(nn, x64, x32, P, xcd_block(), gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sum2(long long nn, const float* __restrict__ a, const float* __restrict__ b,
@@ -67,48 +41,18 @@ __global__ void __launch_bounds__(kBlock) k_sum2(long long nn, const float* __re
 }
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, const PcgScalars* __restrict__ P) {
-  const float m0 = (float)P->mean[0], m1 = (float)P->mean[1], m2 = (float)P->mean[2];
-  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
-    a[i] -= m0;
-    a[nn + i] -= m1;
-    a[2 * nn + i] -= m2;
-  }
+  sub_mean3_body(nn, a, P, xcd_block(), gridDim.x);
 }
 
-// x += alpha p, r -= alpha q, partial r.r
 __global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restrict__ x, float* __restrict__ r,
                                                    const float* __restrict__ p, const float* __restrict__ q,
                                                    const PcgScalars* __restrict__ P, Reduce R) {
-  float al[OTM_NRHS];
-#pragma unroll
-  for (int k = 0; k < OTM_NRHS; k++) al[k] = (float)P->alpha[k];
-  double acc[OTM_NRHS] = {0, 0, 0};
-  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) {
-      const long long j = k * nn + i;
-      x[j] += al[k] * p[j];
-      const float rv = r[j] - al[k] * q[j];
-      r[j] = rv;
-      acc[k] += (double)rv * (double)rv;
-    }
-  }
-  reduce_block<OTM_NRHS>(acc, R);
+  pcg_xr_body(nn, x, r, p, q, P, R, xcd_block(), gridDim.x);
 }
 
-// p = z + beta p
 __global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restrict__ p, const float* __restrict__ z,
                                                   const PcgScalars* __restrict__ P) {
-  float be[OTM_NRHS];
-#pragma unroll
-  for (int k = 0; k < OTM_NRHS; k++) be[k] = (float)P->beta[k];
-  for (long long i = block_range(nn).b + threadIdx.x, i_end = block_range(nn).e; i < i_end; i += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) {
-      const long long j = k * nn + i;
-      p[j] = z[j] + be[k] * p[j];
-    }
-  }
+  pcg_p_body(nn, p, z, P, xcd_block(), gridDim.x);
 }
 
 // dst[v] = src[x + pitch*(y + ny*z)] + add (pitched lexicographic input)
@@ -128,15 +72,8 @@ __global__ void __launch_bounds__(kBlock) k_copy_to_pitched(Geom g, const float*
   dst[x + (long long)pitch * r] = src[t];
 }
 
-// Stage 2 of every reduction: sums the per-block partials in block order and
-// applies the finaliser (one block).
 __global__ void __launch_bounds__(kBlock) k_reduce_final(Reduce R, int nv, int nblocks) {
-  __shared__ double sh[kMaxReduce * kWaves];
-  double acc[kMaxReduce] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < nblocks; b += kBlock)
-    for (int i = 0; i < nv; i++) acc[i] += R.partials[(size_t)b * nv + i];
-  block_sum<kMaxReduce>(acc, sh);
-  if (threadIdx.x == 0) finalize(acc, nv, R);
+  reduce_final_body(R, nv, nblocks);
 }
 
 void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblocks) {

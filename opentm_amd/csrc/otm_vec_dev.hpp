@@ -1,0 +1,123 @@
+// otm_vec_dev.hpp -- device bodies of the vector updates and FP64
+// reductions of the lock-step three-right-hand-side preconditioned
+// conjugate gradient, shared by the single-cell launches (otm_vec.hip) and
+// the batched multi-cell launches (otm_batch.hip). Each body works on block
+// lb of the nb blocks assigned to one cell.
+// The reference sums in FP32 through cub (culib/lib.cuh:595-760); here every
+// inner product accumulates in FP64 with a fixed summation order.
+#pragma once
+#include "otm_kernels.hpp"
+
+namespace otm {
+
+__device__ inline void fill_body(float* a, long long n, float v, int lb, int nb) {
+  const BlockRange br = block_range_v(n, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) a[i] = v;
+}
+
+__device__ inline void dot3_body(long long nn, const float* __restrict__ a, const float* __restrict__ b,
+                                 const Reduce& R, int lb, int nb) {
+  double acc[OTM_NRHS] = {0, 0, 0};
+  const BlockRange br = block_range_v(nn, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i] * (double)b[k * nn + i];
+  }
+  reduce_block_at<OTM_NRHS>(acc, R, lb);
+}
+
+template <typename T>
+__device__ inline void sum3_body(long long nn, const T* __restrict__ a, const Reduce& R, int lb, int nb) {
+  double acc[OTM_NRHS] = {0, 0, 0};
+  const BlockRange br = block_range_v(nn, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)a[k * nn + i];
+  }
+  reduce_block_at<OTM_NRHS>(acc, R, lb);
+}
+
+// Iterative refinement: x64 += d (the FP32 correction), d = 0.
+__device__ inline void refine_add_body(long long n, double* __restrict__ x64, float* __restrict__ d, int lb, int nb) {
+  const BlockRange br = block_range_v(n, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+    x64[i] += (double)d[i];
+    d[i] = 0.f;
+  }
+}
+
+// x64 -= mean (per load case), x32 = x64 rounded.
+__device__ inline void sub_mean_cast_body(long long nn, double* __restrict__ x64, float* __restrict__ x32,
+                                          const PcgScalars* __restrict__ P, int lb, int nb) {
+  const double m[OTM_NRHS] = {P->mean[0], P->mean[1], P->mean[2]};
+  const BlockRange br = block_range_v(nn, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const double v = x64[k * nn + i] - m[k];
+      x64[k * nn + i] = v;
+      x32[k * nn + i] = (float)v;
+    }
+  }
+}
+
+__device__ inline void sub_mean3_body(long long nn, float* a, const PcgScalars* __restrict__ P, int lb, int nb) {
+  const float m0 = (float)P->mean[0], m1 = (float)P->mean[1], m2 = (float)P->mean[2];
+  const BlockRange br = block_range_v(nn, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+    a[i] -= m0;
+    a[nn + i] -= m1;
+    a[2 * nn + i] -= m2;
+  }
+}
+
+// x += alpha p, r -= alpha q, partial r.r
+__device__ inline void pcg_xr_body(long long nn, float* __restrict__ x, float* __restrict__ r,
+                                   const float* __restrict__ p, const float* __restrict__ q,
+                                   const PcgScalars* __restrict__ P, const Reduce& R, int lb, int nb) {
+  float al[OTM_NRHS];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) al[k] = (float)P->alpha[k];
+  double acc[OTM_NRHS] = {0, 0, 0};
+  const BlockRange br = block_range_v(nn, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const long long j = k * nn + i;
+      x[j] += al[k] * p[j];
+      const float rv = r[j] - al[k] * q[j];
+      r[j] = rv;
+      acc[k] += (double)rv * (double)rv;
+    }
+  }
+  reduce_block_at<OTM_NRHS>(acc, R, lb);
+}
+
+// p = z + beta p
+__device__ inline void pcg_p_body(long long nn, float* __restrict__ p, const float* __restrict__ z,
+                                  const PcgScalars* __restrict__ P, int lb, int nb) {
+  float be[OTM_NRHS];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) be[k] = (float)P->beta[k];
+  const BlockRange br = block_range_v(nn, lb, nb);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const long long j = k * nn + i;
+      p[j] = z[j] + be[k] * p[j];
+    }
+  }
+}
+
+// Stage 2 of every reduction: sums the per-block partials in block order and
+// applies the finaliser (one block).
+__device__ inline void reduce_final_body(const Reduce& R, int nv, int nblocks) {
+  __shared__ double sh[kMaxReduce * kWaves];
+  double acc[kMaxReduce] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+    for (int i = 0; i < nv; i++) acc[i] += R.partials[(size_t)b * nv + i];
+  block_sum<kMaxReduce>(acc, sh);
+  if (threadIdx.x == 0) finalize(acc, nv, R);
+}
+
+}  // namespace otm

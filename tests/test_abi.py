@@ -104,7 +104,9 @@ def test_struct_layouts_match_header():
     import re
     hdr = open(os.path.join(ROOT, "include", "opentm.h")).read()
     for cname, py in [("otm_config", _capi.OtmConfig), ("otm_density_params", _capi.OtmDensityParams),
-                      ("otm_custom_params", _capi.OtmCustomParams), ("otm_custom_report", _capi.OtmCustomReport)]:
+                      ("otm_custom_params", _capi.OtmCustomParams), ("otm_custom_report", _capi.OtmCustomReport),
+                      ("otm_request", _capi.OtmRequest), ("otm_result", _capi.OtmResult),
+                      ("otm_server_counters", _capi.OtmServerCounters)]:
         body = re.search(r"typedef struct\s*\w*\s*\{([^}]*)\}\s*" + cname + r"\s*;", hdr).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = [re.split(r"[\s\[;]", d.strip().split()[-1])[0] for d in body.split(";") if d.strip()]

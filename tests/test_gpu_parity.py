@@ -492,3 +492,163 @@ def test_precision_mma_loop_matches_oracle(orc):
     err = float(np.abs(rg - x).max())
     say("  rho err", err)
     assert err < 1e-4
+
+
+# ------------------------------------------------------------ batched form
+def _batch_cells(specs, k1=1e-3, rtol=1e-6, refine=0, fp16=0):
+    homs, kaps = [], []
+    for reso, seed in specs:
+        nx, ny, nz = (reso, reso, reso) if np.isscalar(reso) else reso
+        cfg = otm.HomoConfig((nx, ny, nz), (1.0, k1))
+        cfg.femRelThres = rtol
+        cfg.fp64_refine = refine
+        cfg.fp16_stencils = fp16
+        rng = np.random.default_rng(seed)
+        if nx == ny == nz:
+            k = wl.simp(wl.plain_density(nx, seed))
+        else:
+            k = rng.uniform(1e-3, 1.0, (nz, ny, nx)).astype(np.float32)
+        h = otm.Homogenization_H(cfg)
+        h.update(k)
+        homs.append(h)
+        kaps.append(k)
+    return homs, kaps
+
+
+BATCH_SPECS = [(32, 1), (48, 2), (16, 3), ((32, 24, 16), 4), (64, 5), (32, 6)]
+
+
+def test_batch_matches_single_and_oracle(orc):
+    """Batched heatMatrix and sensitivities of cells of different sizes equal
+    the single-cell solves of the same contexts (same per-cell arithmetic)
+    and match the FP64 oracle."""
+    say("batch vs single", BATCH_SPECS)
+    homs, kaps = _batch_cells(BATCH_SPECS)
+    ref, _ = _batch_cells(BATCH_SPECS)
+    b = otm.Batch()
+    C = b.heat_matrix(homs)
+    dCs = [np.array([[1.0, 0.3, -0.2], [0.1, -0.5, 0.4], [0.25, 0.0, 2.0]]) * (i + 1) for i in range(len(homs))]
+    sens = b.sensitivity(homs, dCs)
+    launches, steps = b.stats()
+    say("  launches", launches, "steps", steps)
+    for i, (h, r, k) in enumerate(zip(homs, ref, kaps)):
+        Cs = r.heatMatrix()
+        ss = r.Sensitivity(dCs[i])
+        d = rel(C[i], Cs)
+        ds = rel(sens[i], ss)
+        its_b, rr_b = h.solve_stats()
+        its_s, rr_s = r.solve_stats()
+        say("  cell", i, BATCH_SPECS[i][0], "its", its_b, its_s, "C diff", d, "sens diff", ds)
+        assert d < 1e-12 and ds < 1e-6
+        assert its_b == its_s
+        if k.size <= 48 ** 3:
+            Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+            assert rel(C[i], Co) < 1e-5
+
+
+def test_batch_result_same_alone_and_in_mix():
+    """A cell's result does not depend on what else is in the batch:
+    bitwise-identical kappa^H and sensitivity alone and in a mix."""
+    say("batch alone vs mix")
+    spec = [(48, 11)]
+    alone, _ = _batch_cells(spec)
+    mix, _ = _batch_cells([(32, 12), (48, 11), (16, 13), (64, 14)])
+    b = otm.Batch()
+    Ca = b.heat_matrix(alone)
+    Cm = b.heat_matrix(mix)
+    dC = np.diag([1.0, 2.0, 3.0])
+    sa = b.sensitivity(alone, [dC])[0]
+    sm = b.sensitivity(mix, [dC] * 4)[1]
+    say("  alone", Ca[0].reshape(-1)[:3], "mix", Cm[1].reshape(-1)[:3])
+    np.testing.assert_array_equal(Ca[0], Cm[1])
+    np.testing.assert_array_equal(sa, sm)
+
+
+def test_batch_precision_cells(orc):
+    """Cells with FP64 refinement and FP16 coarse stencils in a batch: true
+    relative residual <= 1e-8 and kappa^H against the oracle."""
+    specs = [(32, 21), (48, 22), (16, 23)]
+    say("batch precision", specs)
+    homs, kaps = _batch_cells(specs, rtol=1e-8, refine=1, fp16=1)
+    C = otm.Batch().heat_matrix(homs)
+    for i, (h, k) in enumerate(zip(homs, kaps)):
+        its, rr = h.solve_stats()
+        Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+        say("  cell", i, "its", its, "relres", max(rr), "err", rel(C[i], Co))
+        assert max(rr) <= 1e-8
+        assert rel(C[i], Co) < 1e-6
+
+
+# ------------------------------------------------------------ serving form
+def test_server_results_same_alone_and_in_mix(orc):
+    """Continuous batching: a request's kappa^H and sensitivity are bitwise
+    the same when it is served alone and when it is admitted into a running
+    batch of other requests; kappa^H matches the FP64 oracle."""
+    say("server alone vs mix")
+    k_x = wl.simp(wl.plain_density(48, 31))
+    dC = np.diag([1.0, -2.0, 0.5])
+    sv = otm.Server(hbm_budget_gb=8.0, max_active=16)
+    rid = sv.submit(k_x, dC=dC)
+    Ca, sa, ia = sv.wait(rid, k_x.size)
+    # a stream of other requests, with the probe submitted in the middle
+    others = [wl.simp(wl.plain_density(n, 40 + i)) for i, n in enumerate([32, 64, 16, 48, 32, 64, 32, 16])]
+    ids = [sv.submit(k, dC=dC) for k in others[:4]]
+    rid2 = sv.submit(k_x, dC=dC)
+    ids += [sv.submit(k, dC=dC) for k in others[4:]]
+    Cm, sm, im = sv.wait(rid2, k_x.size)
+    for i, r in zip(range(len(ids)), ids):
+        C_o, _, info = sv.wait(r, None)
+        assert max(info["relres"]) < 1e-5
+    st = sv.stats()
+    say("  alone", ia, "mix", im, "stats", st)
+    np.testing.assert_array_equal(Ca, Cm)
+    np.testing.assert_array_equal(sa, sm)
+    Co, _, So, _ = orc.homogenize(k_x.astype(np.float64), rtol=1e-12, want_S=True)
+    assert rel(Ca, Co) < 1e-5
+    assert rel(sa, orc.sensitivity(So, dC)) < 1e-4
+    assert st["completed"] == 2 + len(ids)
+    assert st["contexts_created"] < st["completed"]  # hierarchies are reused
+    sv.close()
+
+
+def test_server_budget_admission():
+    """Admission by HBM budget: with room for about two 64^3 hierarchies the
+    server still completes a burst of requests (evicting idle hierarchies as
+    needed) and never holds more than its budget."""
+    say("server budget")
+    sv = otm.Server(hbm_budget_gb=0.12, max_active=8)
+    ks = [wl.simp(wl.plain_density(n, 60 + i)) for i, n in enumerate([64, 32, 64, 48, 64, 32])]
+    ids = [sv.submit(k, want_sens=False) for k in ks]
+    for r in ids:
+        C, _, info = sv.wait(r)
+        assert np.isfinite(C).all() and max(info["relres"]) < 1e-5
+    st = sv.stats()
+    say("  stats", st)
+    assert st["resident_bytes"] <= st["budget_bytes"]
+    assert st["completed"] == len(ks)
+    sv.close()
+
+
+def test_batch_update_matches_single_update():
+    """Batched setup (one launch per Galerkin level and one coarsest inverse
+    per cell, all cells together) builds the same operators as the
+    single-cell update: identical stencils and identical solves."""
+    specs = [(32, 71), (48, 72), (16, 73), (64, 74)]
+    say("batch update", specs)
+    single, kaps = _batch_cells(specs)
+    homs = []
+    for (reso, seed), k in zip(specs, kaps):
+        cfg = otm.HomoConfig(reso, (1.0, 1e-3))
+        homs.append(otm.Homogenization_H(cfg))
+    b = otm.Batch()
+    b.update(homs, [otm.DeviceBuffer.from_numpy(k) for k in kaps])
+    for i, (h, r) in enumerate(zip(homs, single)):
+        for l in range(1, len(h.levels())):
+            np.testing.assert_array_equal(h.stencil(l), r.stencil(l))
+    C1 = b.heat_matrix(homs)
+    C2 = b.heat_matrix(single)
+    np.testing.assert_array_equal(C1, C2)
+    b.update(homs, kaps)  # host conductivities, cold start again
+    for h in homs:
+        h.reset_displacement()
+    np.testing.assert_array_equal(b.heat_matrix(homs), C1)
