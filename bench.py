@@ -75,7 +75,11 @@ def main():
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel class with HIP events (adds event overhead to the step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["oc", "precision", "batch"], default="oc")
+    ap.add_argument("--workload", choices=["oc", "precision", "batch", "serve"], default="oc")
+    ap.add_argument("--qps", default="100,200,300,400", help="serve workload: arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=2000, help="serve workload: requests per arrival rate")
+    ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve workload: HBM budget of the server")
+    ap.add_argument("--max-active", type=int, default=96, help="serve workload: requests on the GPU at once")
     ap.add_argument("--cells", type=int, default=96, help="batch workload: number of cells")
     ap.add_argument("--oracle", action="store_true", help="precision workload: FP64 oracle errors (slow)")
     args = ap.parse_args()
@@ -87,6 +91,9 @@ def main():
         return
     if args.workload == "batch":
         batch_main(args)
+        return
+    if args.workload == "serve":
+        serve_main(args)
         return
 
     import opentm_amd as otm
@@ -312,6 +319,98 @@ def batch_main(args):
            "pcg_iters_mean": {"batched": float(np.mean(its_bat)), "looped": float(np.mean(its_loop))},
            "max_rel_diff_batched_vs_looped": float(np.max(np.abs(C_b - C_l)) / np.max(np.abs(C_l))),
            "batch_launches_total": launches, "batch_steps_total": steps}
+    print(json.dumps(out))
+
+
+def serve_main(args):
+    """Open-loop Poisson arrivals of homogenization + sensitivity requests
+    (edges log-uniform over 32..192 in multiples of 16, thresholded periodic
+    Gaussian random fields with correlation length U[4,12] voxels, volume
+    U[0.1,0.5], projection beta 8), FP16 coarse stencils and FP64
+    refinement; p50/p99 latency (arrival -> result) and solves/s per rate."""
+    import threading
+    import opentm_amd as otm
+    from opentm_amd import workloads as wl
+    rng = np.random.Generator(np.random.Philox(int(args.seed) + 104729))
+    edges = list(range(32, 193, 16))
+    pool_per_edge = 2
+    t0 = time.perf_counter()
+    pool = {}
+    for e in edges:
+        pool[e] = []
+        for j in range(pool_per_edge):
+            corr = float(rng.uniform(4.0, 12.0))
+            vol = float(rng.uniform(0.1, 0.5))
+            rho = wl.grf_density(e, int(rng.integers(0, 2 ** 31 - 1)), corr, vol)
+            pool[e].append(wl.simp(rho))
+    print(f"[bench] density pool ({len(edges)} edges x {pool_per_edge}) in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+
+    def draw_edge():
+        v = float(np.exp(rng.uniform(np.log(32.0), np.log(192.0))))
+        return int(min(192, max(32, 16 * round(v / 16.0))))
+
+    def cfg_for(e):
+        c = otm.HomoConfig(e, (1.0, 1e-3))
+        c.femRelThres = args.rtol
+        c.coarse_sweeps = args.coarse_sweeps
+        c.fp16_stencils = 1
+        c.fp64_refine = 1
+        return c
+
+    cfgs = {e: cfg_for(e) for e in edges}
+    dC = -np.eye(3)
+    sv = otm.Server(hbm_budget_gb=args.hbm_gb, max_active=args.max_active)
+    # warm the hierarchy cache with one request per shape (not timed)
+    for r in [sv.submit(pool[e][0], cfg=cfgs[e], dC=dC) for e in edges]:
+        sv.wait(r)
+    levels = []
+    for qps in [float(q) for q in args.qps.split(",") if q]:
+        n = args.requests
+        gaps = rng.exponential(1.0 / qps, size=n)
+        picks = [(draw_edge(), int(rng.integers(0, pool_per_edge))) for _ in range(n)]
+        ids = [None] * n
+        t_start = time.perf_counter()
+
+        def submitter():
+            t_next = t_start
+            for i in range(n):
+                t_next += gaps[i]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                e, j = picks[i]
+                ids[i] = sv.submit(pool[e][j], cfg=cfgs[e], dC=dC)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, worst_rr = [], 0.0
+        for i in range(n):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            C, _, info = sv.wait(ids[i])
+            lat.append(info["latency_ms"])
+            worst_rr = max(worst_rr, max(info["relres"]))
+        t_end = time.perf_counter()
+        th.join()
+        lv = {"qps_offered": qps, "requests": n, "solves_per_s": n / (t_end - t_start),
+              "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+              "mean_ms": float(np.mean(lat)), "relres_max": worst_rr}
+        levels.append(lv)
+        print(f"[bench] qps {qps:g}: {lv['solves_per_s']:.1f} solves/s p50 {lv['p50_ms']:.1f} ms "
+              f"p99 {lv['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+    st = sv.stats()
+    sv.close()
+    best = max(levels, key=lambda l: l["solves_per_s"])
+    out = {"metric": "homog. solves/s + p50/p99 latency (arrival -> result), serving form", "value": best["solves_per_s"],
+           "unit": "solves/s", "n_gpus": 1, "steps": sum(l["requests"] for l in levels), "warmup": len(edges),
+           "ms_per_step": 1e3 / best["solves_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32+fp16 stencils+fp64 refinement", "data": "synthetic",
+           "config": {"workload": "open-loop Poisson arrivals, edges 32..192 step 16 log-uniform, thresholded periodic "
+                                  "GRF densities (corr U[4,12], vol U[0.1,0.5], beta 8), kappaH + sensitivity",
+                      "rtol": args.rtol, "seed": args.seed, "hbm_budget_gb": args.hbm_gb,
+                      "max_active": args.max_active},
+           "levels": levels, "server": st}
     print(json.dumps(out))
 
 

@@ -837,6 +837,72 @@ void finish(otm_batch* b) {
   }
 }
 
+// Plans the setup of the contexts in `tmp` (slots already assigned):
+// conductivity copy from src[i] (row pitch `pitch`, <= 0: reso[0]), every
+// Galerkin level, FP16 packing and the coarsest inverse, all cells per
+// launch; `cold` also zeroes the iterates. The descriptors are uploaded to
+// the members' slots. Leaves b->members alone, so it can run between the
+// steps of a live batch.
+struct OpZeroX {
+  static constexpr int kThreads = kBlock;
+  __device__ void operator()(const CellDev& C, const Ent&, int lb, int nb) const {
+    fill_body(C.x, 3 * C.g[0].nn, 0.f, lb, nb);
+    if (C.x64) {
+      const BlockRange br = block_range_v(3 * C.g[0].nn, lb, nb);
+      for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) C.x64[i] = 0.0;
+    }
+  }
+};
+
+void plan_setup(otm_batch* b, std::vector<Member>& tmp, const std::vector<const float*>& src, int pitch, bool cold) {
+  std::vector<Member*> all;
+  int maxlev = 0;
+  size_t lds_max = 0;
+  std::vector<Member*> inv_lds, inv_glob;
+  for (size_t i = 0; i < tmp.size(); i++) {
+    Member& m = tmp[i];
+    otm_ctx* c = m.c;
+    CellDev d = describe(m);
+    d.ksrc = src[i];
+    d.kpitch = pitch > 0 ? pitch : c->g0.nx;
+    if ((int)b->hcells.size() <= m.slot) b->hcells.resize(m.slot + 1);
+    b->hcells[m.slot] = d;
+    hip_check(hipMemcpyAsync(b->dcells + m.slot, &b->hcells[m.slot], sizeof(CellDev), hipMemcpyHostToDevice, b->s),
+              "upload cell");
+    all.push_back(&m);
+    maxlev = std::max(maxlev, m.nlev);
+    const long long nc = c->lv.back().g.nn;
+    if (nc <= kInvertLdsMax) {
+      inv_lds.push_back(&m);
+      lds_max = std::max(lds_max, (size_t)(nc * nc) * sizeof(double));
+    } else {
+      inv_glob.push_back(&m);
+    }
+  }
+  add_op(b, all, OpCopyKappa{}, [](const Member& m) { return (long long)nblk(m.c->g0.nn); });
+  if (cold) add_op(b, all, OpZeroX{}, [](const Member& m) { return std::min<long long>(8192, nblk(3 * m.c->g0.nn)); });
+  for (int l = 1; l < maxlev; l++) {
+    std::vector<Member*> S, P;
+    for (Member* m : all)
+      if (l < m->nlev) {
+        S.push_back(m);
+        if (m->c->lv[l].st16) P.push_back(m);
+      }
+    add_op(b, S, OpGalerkin{l, 0}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
+    add_op(b, S, OpGalerkin{l, 1}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
+    add_op(b, P, OpPackFp16{l, 0}, one);
+    add_op(b, P, OpPackFp16{l, 1},
+           [l](const Member& m) { return std::min<long long>(1024, (long long)nblk(m.c->lv[l].g.nn)); });
+    add_op(b, P, OpPackFp16{l, 2}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
+  }
+  if (!inv_lds.empty())
+    hip_check(hipFuncSetAttribute((const void*)kb_run<OpCoarsestInvert>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(kInvertLdsMax * kInvertLdsMax * sizeof(double))),
+              "hipFuncSetAttribute");
+  add_op_lds(b, inv_lds, OpCoarsestInvert{1}, one, lds_max);
+  add_op(b, inv_glob, OpCoarsestInvert{0}, one);
+}
+
 // Adds a context as a member in slot `slot` (its CellDev is uploaded at
 // once); the caller keeps slots unique and within the capacity.
 Member& add_member(otm_batch* b, otm_ctx* c, int slot, int out) {
@@ -852,7 +918,10 @@ Member& add_member(otm_batch* b, otm_ctx* c, int slot, int out) {
   for (size_t l = 1; l + 1 < c->lv.size(); l++) bricks = bricks && c->lv[l].brick;
   if (!bricks) m.phase = Member::FALLBACK;
   if ((int)b->hcells.size() <= slot) b->hcells.resize(slot + 1);
-  b->hcells[slot] = describe(m);
+  CellDev d = describe(m);
+  d.ksrc = b->hcells[slot].ksrc;  // keep what a pending setup launch reads
+  d.kpitch = b->hcells[slot].kpitch;
+  b->hcells[slot] = d;
   hip_check(hipMemcpyAsync(b->dcells + slot, &b->hcells[slot], sizeof(CellDev), hipMemcpyHostToDevice, b->s),
             "upload cell");
   hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), b->s), "clear fault");
@@ -1064,68 +1133,28 @@ OTM_API int otm_batch_update(otm_batch* b, otm_ctx* const* cells, int n, const f
     }
     if (n == 0) return;
     ensure_cells(b, n);
-    b->members.assign(n, Member());
-    b->hcells.resize(n);
-    std::vector<Member*> all;
-    int maxlev = 0;
-    size_t lds_max = 0;
-    std::vector<Member*> inv_lds, inv_glob;
+    std::vector<Member> tmp(n);
+    std::vector<const float*> src(n);
     for (int i = 0; i < n; i++) {
-      Member& m = b->members[i];
       otm_ctx* c = cells[i];
-      m.c = c;
-      m.slot = i;
-      m.nlev = (int)c->lv.size();
-      CellDev d = describe(m);
+      tmp[i].c = c;
+      tmp[i].slot = i;
+      tmp[i].nlev = (int)c->lv.size();
       const int pt = pitch > 0 ? pitch : c->g0.nx;
       if (where == OTM_HOST) {
         // staged through the context's q vector, which the PCG rewrites before use
         hip_check(hipMemcpyAsync(c->q, kappa[i], sizeof(float) * pt * c->g0.ny * c->g0.nz, hipMemcpyHostToDevice,
                                  b->s),
                   "upload kappa");
-        d.ksrc = c->q;
+        src[i] = c->q;
       } else {
-        d.ksrc = kappa[i];
-      }
-      d.kpitch = pt;
-      b->hcells[i] = d;
-      all.push_back(&m);
-      maxlev = std::max(maxlev, m.nlev);
-      const long long nc = c->lv.back().g.nn;
-      if (nc <= kInvertLdsMax) {
-        inv_lds.push_back(&m);
-        lds_max = std::max(lds_max, (size_t)(nc * nc) * sizeof(double));
-      } else {
-        inv_glob.push_back(&m);
+        src[i] = kappa[i];
       }
     }
-    hip_check(hipMemcpyAsync(b->dcells, b->hcells.data(), sizeof(CellDev) * n, hipMemcpyHostToDevice, b->s),
-              "upload cells");
-    add_op(b, all, OpCopyKappa{}, [](const Member& m) { return (long long)nblk(m.c->g0.nn); });
-    for (int l = 1; l < maxlev; l++) {
-      std::vector<Member*> S, P;
-      for (Member* m : all)
-        if (l < m->nlev) {
-          S.push_back(m);
-          if (m->c->lv[l].st16) P.push_back(m);
-        }
-      add_op(b, S, OpGalerkin{l, 0}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
-      add_op(b, S, OpGalerkin{l, 1}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
-      add_op(b, P, OpPackFp16{l, 0}, one);
-      add_op(b, P, OpPackFp16{l, 1},
-             [l](const Member& m) { return std::min<long long>(1024, (long long)nblk(m.c->lv[l].g.nn)); });
-      add_op(b, P, OpPackFp16{l, 2}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
-    }
-    if (!inv_lds.empty())
-      hip_check(hipFuncSetAttribute((const void*)kb_run<OpCoarsestInvert>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(kInvertLdsMax * kInvertLdsMax * sizeof(double))),
-                "hipFuncSetAttribute");
-    add_op_lds(b, inv_lds, OpCoarsestInvert{1}, one, lds_max);
-    add_op(b, inv_glob, OpCoarsestInvert{0}, one);
+    plan_setup(b, tmp, src, pitch, false);
     flush(b);
     hip_check(hipStreamSynchronize(b->s), "hipStreamSynchronize");
     for (int i = 0; i < n; i++) cells[i]->has_kappa = true;
-    b->members.clear();
   });
 }
 
@@ -1310,6 +1339,10 @@ void server_loop(otm_server* sv) {
         admit.push_back(r);
       }
     }
+    // contexts and slots for the admitted requests, then one batched setup
+    // (conductivity upload, Galerkin hierarchy, coarsest inverse, cold
+    // start) for all of them on the batch's stream
+    std::vector<std::shared_ptr<ServerReq>> ready;
     for (auto& r : admit) {
       try {
         if (!r->ctx) {
@@ -1318,9 +1351,6 @@ void server_loop(otm_server* sv) {
           std::lock_guard<std::mutex> lk(sv->mu);
           sv->created++;
         }
-        if (otm_update(r->ctx, r->kappa.data(), r->cfg.reso[0], OTM_HOST) != OTM_OK) fail(OTM_E_ARG, otm_last_error());
-        if (otm_reset_displacement(r->ctx) != OTM_OK) fail(OTM_E_HIP, otm_last_error());
-        r->t_admit = Clock::now();
         int slot;
         if (!free_slots.empty()) {
           slot = free_slots.back();
@@ -1336,12 +1366,43 @@ void server_loop(otm_server* sv) {
           }
         }
         r->slot = slot;
-        add_member(b, r->ctx, slot, slot);
-        active[slot] = r;
+        ready.push_back(r);
       } catch (const OtmError& e) {
         std::lock_guard<std::mutex> lk(sv->mu);
         release_ctx(sv, *r, false);
         finish_request(sv, *r, e.code, e.msg);
+        sv->cv_done.notify_all();
+      }
+    }
+    if (!ready.empty()) {
+      try {
+        std::vector<Member> tmp(ready.size());
+        std::vector<const float*> src(ready.size());
+        for (size_t i = 0; i < ready.size(); i++) {
+          otm_ctx* c = ready[i]->ctx;
+          tmp[i].c = c;
+          tmp[i].slot = ready[i]->slot;
+          tmp[i].nlev = (int)c->lv.size();
+          hip_check(hipMemcpyAsync(c->q, ready[i]->kappa.data(), sizeof(float) * c->g0.nn, hipMemcpyHostToDevice,
+                                   b->s),
+                    "upload kappa");
+          src[i] = c->q;
+        }
+        plan_setup(b, tmp, src, -1, true);
+        flush(b);
+        for (auto& r : ready) {
+          r->ctx->has_kappa = true;
+          r->t_admit = Clock::now();
+          add_member(b, r->ctx, r->slot, r->slot);
+          active[r->slot] = r;
+        }
+      } catch (const OtmError& e) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (auto& r : ready) {
+          free_slots.push_back(r->slot);
+          release_ctx(sv, *r, false);
+          finish_request(sv, *r, e.code, e.msg);
+        }
         sv->cv_done.notify_all();
       }
     }

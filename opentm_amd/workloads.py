@@ -55,3 +55,27 @@ def simp(rho, k0=1.0, k1=1e-3, penal=3):
     """kappa = k1 + (k0 - k1) rho^p (BASELINE material law)."""
     r = np.asarray(rho, dtype=np.float64)
     return (k1 + (k0 - k1) * r ** penal).astype(np.float32)
+
+
+def grf_density(n, seed, corr_len, vol, beta=8.0):
+    """Serving-form density: a periodic Gaussian random field with Gaussian
+    correlation of length `corr_len` voxels (unit variance), passed through
+    the smooth projection 1 / (1 + exp(-beta (phi - t))) with the threshold
+    t bisected so the mean density is `vol`; clamped to [1e-3, 1]."""
+    rng = _rng(seed)
+    white = rng.standard_normal((n, n, n))
+    k = 2.0 * np.pi * np.fft.fftfreq(n)
+    kr = 2.0 * np.pi * np.fft.rfftfreq(n)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kr[None, None, :] ** 2
+    phi = np.fft.irfftn(np.fft.rfftn(white) * np.exp(-0.5 * k2 * corr_len ** 2), s=(n, n, n), axes=(0, 1, 2))
+    phi = (phi / max(phi.std(), 1e-30)).astype(np.float32)
+    lo, hi = -10.0, 10.0
+    for _ in range(32):
+        t = 0.5 * (lo + hi)
+        m = np.mean(1.0 / (1.0 + np.exp(np.float32(-beta) * (phi - np.float32(t)))), dtype=np.float64)
+        if m > vol:
+            lo = t
+        else:
+            hi = t
+    rho = 1.0 / (1.0 + np.exp(np.float32(-beta) * (phi - np.float32(0.5 * (lo + hi)))))
+    return np.clip(rho, 1e-3, 1.0).astype(np.float32)
