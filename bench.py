@@ -75,7 +75,8 @@ def main():
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel class with HIP events (adds event overhead to the step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["oc", "precision", "batch", "serve"], default="oc")
+    ap.add_argument("--workload", choices=["oc", "precision", "batch", "serve", "copy"], default="oc",
+                    help="copy: three 256 MiB device copies, the calibration run for rocprofv3 byte counters")
     ap.add_argument("--qps", default="100,200,300,400", help="serve workload: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=2000, help="serve workload: requests per arrival rate")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve workload: HBM budget of the server")
@@ -94,6 +95,16 @@ def main():
         return
     if args.workload == "serve":
         serve_main(args)
+        return
+    if args.workload == "copy":
+        import opentm_amd as otm
+        from opentm_amd import _capi
+        n = 64 * 1024 * 1024  # floats: 256 MiB per buffer
+        a, b = otm.DeviceBuffer(n), otm.DeviceBuffer(n)
+        for _ in range(3):
+            _capi.check(_capi.load().otm_memcpy(b.ptr, a.ptr, 4 * n, 1, 1), "otm_memcpy")
+        print(json.dumps({"workload": "copy", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
+                          "copies": 3}))
         return
 
     import opentm_amd as otm

@@ -205,6 +205,22 @@ OTM_API int otm_mma_update(otm_mma* mma, int itn, float* x, const float* dfdx, c
  * [0.1, 1] as in Framework.cu:144 and becomes the context's design. */
 OTM_API int otm_init_density(otm_ctx* ctx, int initway, int seed, double volratio);
 
+/* cfg::Symmetry (cmdline.h:9), same order. */
+enum { OTM_SYM_REFLECT3 = 0, OTM_SYM_REFLECT6 = 1, OTM_SYM_ROTATE3 = 2, OTM_SYM_ROTATE2 = 3, OTM_SYM_NONE = 4 };
+/* symmetrizeField (Framework.cuh:221-235): replaces every value by the mean
+ * over its orbit under the symmetry group about the cell centre (reflect6
+ * and rotate3 need a cubic cell). In place, host or device field. */
+OTM_API int otm_symmetrize(const int reso[3], int sym, float* field, int where);
+/* The same on the context's design density and, after otm_backward, on its
+ * gradient (runCustom's OC branch, Framework.cu:223-224). */
+OTM_API int otm_symmetrize_design(otm_ctx* ctx, int sym);
+
+/* Grid_H::writeDensity binary format (grid.cpp:330-345): int32 reso[3],
+ * then reso[0]*reso[1]*reso[2] float32 values, x fastest. otm_read_density
+ * with rho == NULL only reads the resolution. */
+OTM_API int otm_write_density(const char* path, const int reso[3], const float* rho, int where);
+OTM_API int otm_read_density(const char* path, int reso[3], float* rho, int where);
+
 /* Mean design density and mean SIMP-mapped conductivity of the last
  * otm_eval (rho_H.sum()/ne and rhop_H.sum()/ne in runCustom). */
 OTM_API int otm_design_stats(otm_ctx* ctx, double* mean_rho, double* mean_kappa);
@@ -223,6 +239,7 @@ typedef struct otm_custom_params {
   double finthres;     /* HomoConfig.finthres */
   double volratio;     /* HomoConfig.volRatio (noise initialiser) */
   double target[6];    /* HomoConfig.target_tensor: xx yy zz xy yz xz */
+  int sym;             /* HomoConfig.sym: OTM_SYM_* (default OTM_SYM_NONE) */
 } otm_custom_params;
 typedef struct otm_custom_report {
   int iterations;

@@ -282,6 +282,8 @@ inline std::vector<float> runCustom(cfg::HomoConfig config, std::vector<float>* 
   p.seed = config.seed;
   p.max_iter = config.max_iter;
   p.finthres = config.finthres;
+  p.volratio = config.volRatio;
+  p.sym = (int)config.sym;
   for (int i = 0; i < 6; i++) p.target[i] = config.target_tensor[i];
   const size_t n = (size_t)config.reso[0] * config.reso[1] * config.reso[2];
   std::vector<float> rho(n);

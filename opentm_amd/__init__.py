@@ -22,7 +22,8 @@ from ._capi import OTM_DEVICE, OTM_HOST, OtmError
 
 __all__ = [
     "HomoConfig", "Homogenization_H", "OCOptimizer", "MMAOptimizer", "DeviceBuffer", "DensityParams",
-    "conv_filter", "OtmError", "load_library", "run_custom", "Batch", "Server",
+    "conv_filter", "OtmError", "load_library", "run_custom", "Batch", "Server", "symmetrize",
+    "write_density", "read_density",
 ]
 
 
@@ -485,9 +486,43 @@ def conv_filter(x, kind=0, radius=1.5, periodic=False, adjoint=False):
 _INITWAYS = ["random", "randcenter", "noise", "manual", "interp", "rep_randcenter", "P", "G", "D", "IWP", "example"]
 
 
+def write_density(path, rho):
+    """Grid_H::writeDensity binary format (grid.cpp:330-345): int32 reso[3]
+    then float32 values, x fastest; rho has shape (nz, ny, nx)."""
+    a = _f32(rho)
+    nz, ny, nx = a.shape
+    reso = (ctypes.c_int * 3)(nx, ny, nz)
+    _capi.check(_capi.load().otm_write_density(str(path).encode(), reso, _ptr(a), OTM_HOST), "otm_write_density")
+
+
+def read_density(path):
+    """Reads the binary density format; returns an array of shape (nz, ny, nx)."""
+    lib = _capi.load()
+    reso = (ctypes.c_int * 3)()
+    _capi.check(lib.otm_read_density(str(path).encode(), reso, None, OTM_HOST), "otm_read_density")
+    nx, ny, nz = reso[0], reso[1], reso[2]
+    out = np.empty((nz, ny, nx), np.float32)
+    _capi.check(lib.otm_read_density(str(path).encode(), reso, _ptr(out), OTM_HOST), "otm_read_density")
+    return out
+
+
+_SYMMETRIES = ["reflect3", "reflect6", "rotate3", "rotate2", "NONE"]
+
+
+def symmetrize(field, sym):
+    """symmetrizeField (Framework.cuh:221-235) on the GPU: the orbit mean
+    under the named symmetry group ("reflect3", "reflect6", "rotate3",
+    "rotate2"); field has shape (nz, ny, nx). Returns a new array."""
+    a = np.array(field, dtype=np.float32, order="C", copy=True)
+    nz, ny, nx = a.shape
+    reso = (ctypes.c_int * 3)(nx, ny, nz)
+    _capi.check(_capi.load().otm_symmetrize(reso, _SYMMETRIES.index(sym), _ptr(a), OTM_HOST), "otm_symmetrize")
+    return a
+
+
 def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
                target=(0.3, 0.2, 0.1, 0.0, 0.0, 0.0), heat_ratio=(1.0, 0.0), fem_rel_thres=1e-6,
-               finthres=5e-4, volratio=0.3, coarse_sweeps=0):
+               finthres=5e-4, volratio=0.3, coarse_sweeps=0, sym="NONE"):
     """runCustom (Framework.cu:147-244) on the engine: returns the final
     design density (x fastest) and a report dict."""
     lib = _capi.load()
@@ -505,6 +540,7 @@ def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
     p.max_iter = int(max_iter)
     p.finthres = finthres
     p.volratio = volratio
+    p.sym = _SYMMETRIES.index(sym)
     for i in range(6):
         p.target[i] = float(target[i])
     n = r[0] * r[1] * r[2]

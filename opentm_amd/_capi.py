@@ -68,6 +68,7 @@ class OtmCustomParams(ctypes.Structure):
         ("finthres", _c.c_double),
         ("volratio", _c.c_double),
         ("target", _c.c_double * 6),
+        ("sym", _c.c_int),
     ]
 
 
@@ -152,6 +153,10 @@ SIGNATURES = {
     "otm_mma_set_move": (_i, [_p, _d]),
     "otm_mma_update": (_i, [_p, _i, _p, _p, _dp, _c.POINTER(_p), _i, _ip]),
     "otm_init_density": (_i, [_p, _i, _i, _d]),
+    "otm_symmetrize": (_i, [_c.POINTER(_i), _i, _p, _i]),
+    "otm_write_density": (_i, [_c.c_char_p, _c.POINTER(_i), _p, _i]),
+    "otm_read_density": (_i, [_c.c_char_p, _c.POINTER(_i), _p, _i]),
+    "otm_symmetrize_design": (_i, [_p, _i]),
     "otm_design_stats": (_i, [_p, _dp, _dp]),
     "otm_custom_params_init": (None, [_c.POINTER(OtmCustomParams)]),
     "otm_run_custom": (_i, [_c.POINTER(OtmCustomParams), _p, _p, _c.POINTER(OtmCustomReport)]),

@@ -46,6 +46,7 @@ constexpr int kMaxLevels = 10;  // 1024 -> 2 is ten levels
 struct CellDev {
   Geom g[kMaxLevels];
   int nlev;
+  int tail;  // first level of the one-workgroup tail V-cycle
   int fine_bx;  // 32 or 16: fine brick width; 0: exact colour sweeps
   int sweeps_fine, sweeps_coarse;
   FineConsts fc;
@@ -208,13 +209,26 @@ struct OpProlong {
   }
 };
 
-struct OpCoarsest {  // blocks: 3 load cases x ceil(n / kBlock)
-  static constexpr int kThreads = kBlock;
-  __device__ void operator()(const CellDev& C, const Ent& e, int lb, int nb) const {
-    const int L = C.nlev - 1;
-    const int per = nb / OTM_NRHS;
-    const int k = lb / per, row = (lb % per) * kBlock + threadIdx.x;
-    coarsest_solve_body(C.g[L], C.Binv, C.f[L], ucur(C, e.par, L), row, k);
+struct OpTail {  // one workgroup per cell: the tail V-cycle from level C.tail down
+  static constexpr int kThreads = kTailThreads;
+  __device__ void operator()(const CellDev& C, const Ent& e, int, int) const {
+    __shared__ TailLevels T;
+    if (threadIdx.x == 0) {
+      T.n = C.nlev - C.tail;
+      T.sweeps = C.sweeps_coarse;
+      for (int i = 0; i < T.n; i++) {
+        const int l = C.tail + i;
+        T.g[i] = C.g[l];
+        T.f[i] = C.f[l];
+        T.res[i] = C.res[l];
+        T.st[i] = C.st[l];
+      }
+      T.f0 = C.f[C.tail];
+      T.Binv = C.Binv;
+      T.uout = ucur(C, e.par, C.tail);
+    }
+    __syncthreads();
+    tail_vcycle_body(T);
   }
 };
 
@@ -372,17 +386,11 @@ struct OpPackFp16 {  // what 0: clear scale, 1: |coupling| max, 2: pack
   }
 };
 
-struct OpCoarsestInvert {  // one workgroup per cell
+struct OpCoarsestInvert {  // one workgroup per cell, work matrix in dynamic LDS (n <= kInvertLdsMax)
   static constexpr int kThreads = kFactorThreads;
-  int lds;  // 1: work matrix in dynamic LDS (n <= kInvertLdsMax)
   __device__ void operator()(const CellDev& C, const Ent&, int, int) const {
-    const Geom& g = C.g[C.nlev - 1];
-    if (lds) {
-      extern __shared__ double A[];
-      coarsest_invert_lds_body(g, C.st_w[C.nlev - 1], A, C.Binv_w);
-    } else {
-      coarsest_invert_global_body(g, C.st_w[C.nlev - 1], C.cB, C.Binv_w);
-    }
+    extern __shared__ double A[];
+    coarsest_invert_lds_body(C.g[C.nlev - 1], C.st_w[C.nlev - 1], A, C.Binv_w);
   }
 };
 
@@ -496,6 +504,7 @@ CellDev describe(const Member& m) {
   CellDev d;
   memset(&d, 0, sizeof(d));
   d.nlev = (int)c->lv.size();
+  d.tail = c->tail;
   for (int l = 0; l < d.nlev; l++) {
     const Level& L = c->lv[l];
     d.g[l] = L.g;
@@ -601,12 +610,14 @@ long long one(const Member&) { return 1; }
 
 // V-cycle for the member set V (all levels, every member at its own depth).
 void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
+  // levels above each member's tail run as separate launches (all members
+  // per launch); every member's tail V-cycle runs in one launch at the bottom
   int maxlev = 0;
-  for (Member* m : V) maxlev = std::max(maxlev, m->nlev);
-  auto at = [&](int l, bool coarsest) {
+  for (Member* m : V) maxlev = std::max(maxlev, m->c->tail + 1);
+  auto at = [&](int l, bool) {
     std::vector<Member*> s;
     for (Member* m : V)
-      if (coarsest ? (m->nlev - 1 == l) : (l < m->nlev - 1)) s.push_back(m);
+      if (l < m->c->tail) s.push_back(m);
     return s;
   };
   // down
@@ -642,8 +653,8 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
     }
     add_op(b, S, OpRestrict{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l + 1].g.nn); });
   }
-  // bottom: every member's coarsest level
-  add_op(b, V, OpCoarsest{}, [](const Member& m) { return (long long)OTM_NRHS * nblk(m.c->lv.back().g.nn); });
+  // bottom: every member's tail
+  add_op(b, V, OpTail{}, one);
   // up
   for (int l = maxlev - 2; l >= 0; l--) {
     std::vector<Member*> S = at(l, false);
@@ -899,8 +910,17 @@ void plan_setup(otm_batch* b, std::vector<Member>& tmp, const std::vector<const 
     hip_check(hipFuncSetAttribute((const void*)kb_run<OpCoarsestInvert>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(kInvertLdsMax * kInvertLdsMax * sizeof(double))),
               "hipFuncSetAttribute");
-  add_op_lds(b, inv_lds, OpCoarsestInvert{1}, one, lds_max);
-  add_op(b, inv_glob, OpCoarsestInvert{0}, one);
+  add_op_lds(b, inv_lds, OpCoarsestInvert{}, one, lds_max);
+  // larger coarsest levels: the blocked multi-workgroup inverse, per cell
+  for (Member* m : inv_glob) {
+    const otm_ctx* c = m->c;
+    const Geom g = c->lv.back().g;
+    const float* st = c->lv.back().st;
+    double* B = c->cB;
+    double* Binv = c->cBinv;
+    hipStream_t s = b->s;
+    b->plan.push_back({[s, g, st, B, Binv](const Ent*) { launch_coarsest_blocked(s, g, st, B, Binv); }, 0});
+  }
 }
 
 // Adds a context as a member in slot `slot` (its CellDev is uploaded at

@@ -532,15 +532,132 @@ __device__ inline void coarsest_invert_lds_body(const Geom& g, const float* __re
   gauss_jordan_invert((int)g.nn, A, colk, rowk, Binv);
 }
 
-// Larger coarsest levels (n <= 4096): work matrix B in global memory,
-// colk/rowk in the 2n doubles after it.
-__device__ inline void coarsest_invert_global_body(const Geom& g, const float* __restrict__ st, double* B,
-                                                   double* __restrict__ Binv) {
-  __shared__ double red[kFactorThreads / 64];
-  __shared__ double alpha_sh;
-  const int n = (int)g.nn;
-  coarsest_assemble(g, st, B, red, &alpha_sh);
-  gauss_jordan_invert(n, B, B + (long long)n * n, B + (long long)n * n + n, Binv);
+
+
+// ---- one-workgroup tail V-cycle ---------------------------------------------
+// The last levels of a hierarchy (from the first level with at most
+// kTailMaxNodes nodes down to the coarsest) hold a few thousand nodes: as
+// separate launches their smoothers, residuals and transfers are pure
+// launch latency. One workgroup runs the whole tail V-cycle instead: the
+// corrections of every tail level stay in LDS, smoothing is exact 8-colour
+// Gauss-Seidel (forward before restriction, reverse after prolongation, so
+// the V-cycle stays symmetric), residuals and restricted right-hand sides go
+// through the level buffers in global memory (visible to the workgroup after
+// each barrier), and the coarsest level is solved with its explicit inverse.
+constexpr int kTailMaxLevels = 6;
+constexpr int kTailMaxNodes = 4096;
+constexpr int kTailStartNodes = 1;  // largest first tail level the hierarchies use (see build_hierarchy)
+constexpr int kTailThreads = 1024;
+constexpr int kTailLdsNodes = 4096 + 512 + 64 + 8 + 1 + 1;
+
+struct TailLevels {
+  int n;  // levels in the tail; the last is the coarsest
+  int sweeps;
+  Geom g[kTailMaxLevels];
+  const float* f0;                // right-hand side of the first tail level
+  float* f[kTailMaxLevels];       // right-hand sides of the deeper levels (written here)
+  float* res[kTailMaxLevels];     // residual scratch of the non-coarsest levels
+  StRef st[kTailMaxLevels];
+  const double* Binv;             // inverse of the coarsest level
+  float* uout;                    // correction of the first tail level (output)
+};
+
+void launch_tail_vcycle(hipStream_t s, const TailLevels& T);
+
+__device__ inline void tail_stencil(const StRef& S, long long nn, long long v, float a[27]) {
+  if (S.f16)
+    load_stencil<true>(S, nn, v, a);
+  else
+    load_stencil<false>(S, nn, v, a);
+}
+
+// `sweeps` exact 8-colour Gauss-Seidel sweeps on u (LDS) of one tail level.
+__device__ inline void tail_smooth(const Geom& g, const StRef& S, const float* __restrict__ f, float* u, int sweeps,
+                                   bool forward) {
+  const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
+  const int cnt = hx * hy * hz;
+  const int nn = (int)g.nn;
+  for (int s = 0; s < sweeps; s++)
+    for (int i = 0; i < 8; i++) {
+      const int col = forward ? i : 7 - i;
+      const int cx = col & 1, cy = (col >> 1) & 1, cz = col >> 2;
+      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+        const int x = 2 * (t % hx) + cx, r = t / hx;
+        const int y = 2 * (r % hy) + cy, z = 2 * (r / hy) + cz;
+        const int v = x + g.nx * (y + g.ny * z);
+        const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+        const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+        const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+        float a[27];
+        tail_stencil(S, nn, v, a);
+        const float inv = 1.f / a[13];
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) {
+          const float* uk = u + k * nn;
+          float acc = 0.f;
+#pragma unroll
+          for (int o = 0; o < 27; o++) {
+            if (o == 13) continue;
+            acc += a[o] * uk[xs[o % 3] + g.nx * (ys[(o / 3) % 3] + g.ny * zs[o / 9])];
+          }
+          u[k * nn + v] = (f[k * nn + v] - acc) * inv;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__device__ inline void tail_vcycle_body(const TailLevels& T) {
+  __shared__ float su[OTM_NRHS * kTailLdsNodes];
+  const int tid = threadIdx.x, bs = blockDim.x;
+  int off[kTailMaxLevels];
+  {
+    int o = 0;
+    for (int l = 0; l < T.n; l++) {
+      off[l] = o;
+      o += OTM_NRHS * (int)T.g[l].nn;
+    }
+  }
+  const int L = T.n - 1;
+  // down: smooth from zero, residual, restriction
+  for (int l = 0; l < L; l++) {
+    const Geom& g = T.g[l];
+    float* u = su + off[l];
+    const float* f = l == 0 ? T.f0 : T.f[l];
+    for (int i = tid; i < OTM_NRHS * (int)g.nn; i += bs) u[i] = 0.f;
+    __syncthreads();
+    tail_smooth(g, T.st[l], f, u, T.sweeps, true);
+    for (int t = tid; t < (int)g.nn; t += bs) {
+      if (T.st[l].f16)
+        st_resid_node<true>(g, T.st[l], f, u, T.res[l], t);
+      else
+        st_resid_node<false>(g, T.st[l], f, u, T.res[l], t);
+    }
+    __syncthreads();
+    for (int t = tid; t < (int)T.g[l + 1].nn; t += bs) restrict_node(g, T.g[l + 1], T.res[l], T.f[l + 1], t);
+    __syncthreads();
+  }
+  // coarsest: u = Binv f (FP64 accumulation)
+  {
+    const int n = (int)T.g[L].nn;
+    const float* f = L == 0 ? T.f0 : T.f[L];
+    for (int q = tid; q < OTM_NRHS * n; q += bs) {
+      const int k = q / n, i = q - k * n;
+      double s = 0.0;
+      for (int j = 0; j < n; j++) s += T.Binv[(long long)i * n + j] * (double)f[k * n + j];
+      su[off[L] + q] = (float)s;
+    }
+    __syncthreads();
+  }
+  // up: prolongation, reverse smoothing
+  for (int l = L - 1; l >= 0; l--) {
+    const Geom& g = T.g[l];
+    float* u = su + off[l];
+    for (int t = tid; t < (int)g.nn; t += bs) prolong_node(g, T.g[l + 1], su + off[l + 1], u, t);
+    __syncthreads();
+    tail_smooth(g, T.st[l], l == 0 ? T.f0 : T.f[l], u, T.sweeps, false);
+  }
+  for (int i = tid; i < OTM_NRHS * (int)T.g[0].nn; i += bs) T.uout[i] = su[i];
 }
 
 }  // namespace otm

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/opentm.h"
+#include "otm_coarse_dev.hpp"
 #include "otm_kernels.hpp"
 
 namespace otm {
@@ -134,6 +135,7 @@ struct otm_ctx {
   };
   std::vector<Pending> pending;
   bool bricks_on = true;   // coarse levels use the LDS brick smoother
+  int tail = -1;           // first level of the one-workgroup tail V-cycle
   int fallbacks = 0;       // solves that fell back to exact Gauss-Seidel
   long long prof_launches[OTM_PROF_NCLASSES] = {};
   double prof_ms[OTM_PROF_NCLASSES] = {};

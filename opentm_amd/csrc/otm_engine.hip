@@ -224,7 +224,19 @@ static void build_hierarchy(otm_ctx* c) {
     if (l + 1 < geoms.size()) L.res = dalloc<float>(3 * n);
     c->lv.push_back(L);
   }
-  c->cB = dalloc<double>(ncoarse * ncoarse + 2 * ncoarse);  // Gauss-Jordan work matrix + pivot row/column
+  c->cB = dalloc<double>(coarsest_work_doubles(ncoarse));  // Gauss-Jordan work space
+  // the tail V-cycle takes over from the first coarse level small enough
+  // for one workgroup (the coarsest level at the latest). Measured at 128^3,
+  // a tail that starts at 16^3 costs 131 us per V-cycle against about 100 us
+  // for the separate launches it replaces (each of its 40 barrier phases
+  // waits for its stencil loads), so the tail currently holds the coarsest
+  // level only.
+  c->tail = (int)c->lv.size() - 1;
+  for (size_t l = 1; l < c->lv.size(); l++)
+    if (c->lv[l].g.nn <= kTailStartNodes && (int)(c->lv.size() - l) <= kTailMaxLevels) {
+      c->tail = (int)l;
+      break;
+    }
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
   c->partials = dalloc<double>(2048 * kMaxReduce);
   c->red_out = dalloc<double>(16);
@@ -282,10 +294,33 @@ static void smooth(otm_ctx* c, size_t l, int sweeps, bool forward) {
     }
 }
 
+static TailLevels tail_levels(const otm_ctx* c, int nsw, float* uout) {
+  TailLevels T;
+  memset(&T, 0, sizeof(T));
+  T.n = (int)c->lv.size() - c->tail;
+  T.sweeps = nsw;
+  for (int i = 0; i < T.n; i++) {
+    const Level& L = c->lv[c->tail + i];
+    T.g[i] = L.g;
+    T.f[i] = L.f;
+    T.res[i] = L.res;
+    T.st[i] = stref(L);
+  }
+  T.f0 = c->lv[c->tail].f;
+  T.Binv = c->cBinv;
+  T.uout = uout;
+  return T;
+}
+
 static void vcycle(otm_ctx* c, size_t l) {
   Level& L = c->lv[l];
   const double nn = (double)L.g.nn;
   const int nsw = (l > 0 && c->cfg.coarse_sweeps > 0) ? c->cfg.coarse_sweeps : c->cfg.pre_sweeps;
+  if ((int)l == c->tail) {
+    ProfScope ps(c, OTM_PROF_COARSE_GS, 0.0);
+    launch_tail_vcycle(c->s, tail_levels(c, nsw, L.u));
+    return;
+  }
   if (l + 1 == c->lv.size()) {
     ProfScope ps(c, OTM_PROF_COARSEST, 8.0 * nn * nn + 24.0 * nn);
     launch_coarsest_solve(c->s, L.g, c->cBinv, L.f, L.u);
@@ -848,6 +883,87 @@ OTM_API int otm_conv_filter(const int reso[3], int kind, double radius, int peri
   });
 }
 
+// Symmetry groups of symmetrizeField (Framework.cuh:221-235) on the cell's
+// centre: reflect3 = the 8 axis reflections, reflect6 = reflections x axis
+// permutations (48), rotate3 = the 24 rotations of the cube (permutation
+// parity = parity of the number of reflected axes), rotate2 = the point
+// reflection (2).
+static SymGroup make_sym_group(int sym, const Geom& g) {
+  SymGroup G;
+  memset(&G, 0, sizeof(G));
+  const int perms[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};  // 3 even, 3 odd
+  auto add = [&](int p, int f) {
+    for (int j = 0; j < 3; j++) G.perm[G.n][j] = (signed char)perms[p][j];
+    G.flip[G.n] = (unsigned char)f;
+    G.n++;
+  };
+  const bool cubic = g.nx == g.ny && g.ny == g.nz;
+  switch (sym) {
+    case OTM_SYM_REFLECT3:
+      for (int f = 0; f < 8; f++) add(0, f);
+      break;
+    case OTM_SYM_REFLECT6:
+      if (!cubic) fail(OTM_E_ARG, "reflect6 symmetry needs a cubic cell");
+      for (int p = 0; p < 6; p++)
+        for (int f = 0; f < 8; f++) add(p, f);
+      break;
+    case OTM_SYM_ROTATE3:
+      if (!cubic) fail(OTM_E_ARG, "rotate3 symmetry needs a cubic cell");
+      for (int p = 0; p < 6; p++)
+        for (int f = 0; f < 8; f++)
+          if ((__builtin_popcount(f) & 1) == (p >= 3 ? 1 : 0)) add(p, f);
+      break;
+    case OTM_SYM_ROTATE2:
+      add(0, 0);
+      add(0, 7);
+      break;
+    default:
+      fail(OTM_E_ARG, "unknown symmetry");
+  }
+  return G;
+}
+
+OTM_API int otm_symmetrize(const int reso[3], int sym, float* field, int where) {
+  return guarded([&] {
+    if (!reso || !field) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    for (int a = 0; a < 3; a++)
+      if (reso[a] <= 0) fail(OTM_E_ARG, "reso must be positive");
+    if (sym == OTM_SYM_NONE) return;
+    Geom g{reso[0], reso[1], reso[2], (long long)reso[0] * reso[1] * reso[2]};
+    const SymGroup G = make_sym_group(sym, g);
+    float* din = dalloc<float>(g.nn);
+    float* dout = dalloc<float>(g.nn);
+    hipError_t e = hipMemcpy(din, field, sizeof(float) * g.nn, kind_of(OTM_DEVICE, where));
+    if (e == hipSuccess) {
+      launch_symmetrize(nullptr, g, G, din, dout);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(field, dout, sizeof(float) * g.nn, kind_of(where, OTM_DEVICE));
+    dfree(din);
+    dfree(dout);
+    hip_check(e, "symmetrize");
+  });
+}
+
+OTM_API int otm_symmetrize_design(otm_ctx* ctx, int sym) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    if (!ctx->rho) fail(OTM_E_STATE, "no design density set");
+    if (sym == OTM_SYM_NONE) return;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const SymGroup G = make_sym_group(sym, ctx->g0);
+    const long long n = ctx->g0.nn;
+    launch_symmetrize(ctx->s, ctx->g0, G, ctx->rho, ctx->tmp);
+    hip_check(hipMemcpyAsync(ctx->rho, ctx->tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->s), "copy");
+    if (ctx->has_drho) {
+      launch_symmetrize(ctx->s, ctx->g0, G, ctx->drho, ctx->tmp);
+      hip_check(hipMemcpyAsync(ctx->drho, ctx->tmp, sizeof(float) * n, hipMemcpyDeviceToDevice, ctx->s), "copy");
+    }
+    ctx->sync();
+  });
+}
+
 OTM_API void otm_density_params_init(otm_density_params* p) {
   if (!p) return;
   p->filter_kind = 0;       // Spline4 (Framework.cu:163)
@@ -1070,7 +1186,7 @@ OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pi
 
 static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volratio, int* steps, double* volume) {
   const long long n = o->ne;
-  const unsigned nb = red_blocks_for(n);
+  const unsigned nb = std::min(512u, red_blocks_for(n));  // fewer partials for the 31-way final sum
   Reduce R;
   R.partials = o->partials;
   R.out = nullptr;

@@ -323,24 +323,28 @@ __device__ inline void fine_energy6_body(const Geom& g, const FineConsts& fc, co
 // plus 12 adds shared by the three. Staging reads whole x-rows of the brick
 // (coalesced) and the two halo columns separately, with no per-element
 // index division.
-template <int BX>
+// BX x BY x BZ nodes per brick, TR thread rows of BX threads (TR divides the
+// (BY/2)(BZ/2) x-rows of one colour).
+template <int BX, int BY = 8, int BZ = 8, int TR = 8>
 __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
                                        const float* __restrict__ f, const float* __restrict__ uin,
                                        float* __restrict__ uout, int sweeps, int forward, int zero_in, int b) {
-  constexpr int BY = 8, BZ = 8;
   constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
   constexpr int KX = BX + 1, KY = BY + 1, KZ = BZ + 1;
-  constexpr int NT = BX * 8;
+  constexpr int NT = BX * TR;
+  constexpr int R = (BY / 2) * (BZ / 2);  // x-rows of one colour
+  constexpr int H = R / TR;               // of them per thread row
+  static_assert(R % TR == 0, "thread rows must divide the rows of a colour");
   __shared__ float su[OTM_NRHS][HZ * HY * HX];
   __shared__ float sk[KZ * KY * KX];
   const int bx = g.nx / BX, by = g.ny / BY;
   const int ox = (b % bx) * BX, oy = ((b / bx) % by) * BY, oz = (b / (bx * by)) * BZ;
   const int t = threadIdx.x;
-  const int tx = t % BX, ty = t / BX;  // ty in [0, 8)
+  const int tx = t % BX, ty = t / BX;  // ty in [0, TR)
   const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
   const int xm = wrapm(ox - 1, g.nx), xp = wrapm(ox + BX, g.nx);
   // kappa of element (ox-1+kx, oy-1+ky, oz-1+kz) at sk[kx + KX*(ky + KY*kz)]
-  for (int r = ty; r < KY * KZ; r += 8) {
+  for (int r = ty; r < KY * KZ; r += TR) {
     const int ky = r % KY, kz = r / KY;
     const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
     sk[r * KX + 1 + tx] = kappa[row + ox + tx];
@@ -356,7 +360,7 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
       for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
     }
   } else {
-    for (int r = ty; r < HY * HZ; r += 8) {
+    for (int r = ty; r < HY * HZ; r += TR) {
       const int hy = r % HY, hz = r / HY;
       const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
 #pragma unroll
@@ -372,16 +376,16 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
       for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + hx] = uin[k * g.nn + w];
     }
   }
-  // node ownership: x = tx, and for colour (cy, cz) the rows
-  // y = cy + 2*(ty%4), z = cz + 2*(ty/4) and z + 4
+  // node ownership: x = tx; of colour (cy, cz) the x-rows rr = ty + TR*h,
+  // y = cy + 2*(rr % (BY/2)), z = cz + 2*(rr / (BY/2))
   const int lx = tx;
-  const int ry = 2 * (ty & 3), rz = 2 * (ty >> 2);
-  float fv[4][2][OTM_NRHS];
+  float fv[4][H][OTM_NRHS];
 #pragma unroll
   for (int col = 0; col < 4; col++)
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
+    for (int h = 0; h < H; h++) {
+      const int rr = ty + TR * h;
+      const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
       const long long v = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
 #pragma unroll
       for (int k = 0; k < OTM_NRHS; k++) fv[col][h][k] = f[k * g.nn + v];
@@ -394,8 +398,9 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
     for (int i = 0; i < 4; i++) {
       const int col = forward ? i : 3 - i;
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int ly = ry + (col & 1), lz = rz + (col >> 1) + 4 * h;
+      for (int h = 0; h < H; h++) {
+        const int rr = ty + TR * h;
+        const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
         // kappa of the 8 elements around the node, e = ax + 2ay + 4az: element
         // (x-1+ax, y-1+ay, z-1+az) at sk(lx+ax, ly+ay, lz+az)
         const int kb = lx + KX * (ly + KY * lz);
@@ -434,7 +439,7 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
       }
       __syncthreads();
     }
-  for (int r = ty; r < BY * BZ; r += 8) {
+  for (int r = ty; r < BY * BZ; r += TR) {
     const int ly = r % BY, lz = r / BY;
     const long long v = (ox + tx) + nx * (oy + ly) + nxy * (oz + lz);
     const int c0 = (tx + 1) + DY * (ly + 1) + DZ * (lz + 1);

@@ -4,6 +4,7 @@
 // iterations; only the 3x3 tensor, the volume statistics and the MMA
 // constraint value come back to the host.
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -64,6 +65,55 @@ OTM_API void otm_custom_params_init(otm_custom_params* p) {
   p->volratio = 0.3;         // cmdline.cpp:11
   const double t[6] = {0.3, 0.2, 0.1, 0.0, 0.0, 0.0};  // cmdline.cpp:29-34
   for (int i = 0; i < 6; i++) p->target[i] = t[i];
+  p->sym = OTM_SYM_NONE;  // cmdline.cpp:5
+}
+
+// Binary density format of Grid_H::writeDensity (grid.cpp:330-345): three
+// int32 cell resolutions, then one float32 per element in lexicographic
+// order (x fastest).
+OTM_API int otm_write_density(const char* path, const int reso[3], const float* rho, int where) {
+  if (!path || !reso || !rho || (where != OTM_HOST && where != OTM_DEVICE)) return OTM_E_ARG;
+  const size_t n = (size_t)reso[0] * reso[1] * reso[2];
+  if (reso[0] <= 0 || reso[1] <= 0 || reso[2] <= 0) return OTM_E_ARG;
+  std::vector<float> host;
+  const float* src = rho;
+  if (where == OTM_DEVICE) {
+    host.resize(n);
+    const int st = otm_memcpy(host.data(), rho, sizeof(float) * n, OTM_HOST, OTM_DEVICE);
+    if (st != OTM_OK) return st;
+    src = host.data();
+  }
+  FILE* f = fopen(path, "wb");
+  if (!f) return OTM_E_ARG;
+  const int32_t hdr[3] = {reso[0], reso[1], reso[2]};
+  const bool ok = fwrite(hdr, sizeof(int32_t), 3, f) == 3 && fwrite(src, sizeof(float), n, f) == n;
+  return (fclose(f) == 0 && ok) ? OTM_OK : OTM_E_ARG;
+}
+
+OTM_API int otm_read_density(const char* path, int reso[3], float* rho, int where) {
+  if (!path || !reso || (where != OTM_HOST && where != OTM_DEVICE)) return OTM_E_ARG;
+  FILE* f = fopen(path, "rb");
+  if (!f) return OTM_E_ARG;
+  int32_t hdr[3];
+  if (fread(hdr, sizeof(int32_t), 3, f) != 3 || hdr[0] <= 0 || hdr[1] <= 0 || hdr[2] <= 0) {
+    fclose(f);
+    return OTM_E_ARG;
+  }
+  for (int a = 0; a < 3; a++) reso[a] = hdr[a];
+  const size_t n = (size_t)hdr[0] * hdr[1] * hdr[2];
+  int st = OTM_OK;
+  if (rho) {
+    std::vector<float> host(n);
+    if (fread(host.data(), sizeof(float), n, f) != n) st = OTM_E_ARG;
+    if (st == OTM_OK) {
+      if (where == OTM_DEVICE)
+        st = otm_memcpy(rho, host.data(), sizeof(float) * n, OTM_DEVICE, OTM_HOST);
+      else
+        memcpy(rho, host.data(), sizeof(float) * n);
+    }
+  }
+  fclose(f);
+  return st;
 }
 
 OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float* rho_out, otm_custom_report* report) {
@@ -74,10 +124,12 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
     const int dev = p->base.device;
     Ctx hom;
     ck(otm_create(&p->base, &hom.c));
-    if (rho0)
+    if (rho0) {
       ck(otm_set_design(hom.c, rho0, OTM_HOST));
-    else
+    } else {
       ck(otm_init_density(hom.c, p->initway, p->seed, p->volratio));
+      ck(otm_symmetrize_design(hom.c, p->sym));  // initDensity (Framework.cu:140-141)
+    }
     otm_density_params dp;
     otm_density_params_init(&dp);  // Spline4 r = 1.5, non-periodic (Framework.cu:163)
     homo::ConvergeChecker criteria(p->finthres);
@@ -132,6 +184,7 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
           converged = true;
           break;
         }
+        ck(otm_symmetrize_design(hom.c, p->sym));  // design and gradient (Framework.cu:223-224)
         ck(otm_oc_step(hom.c, oc, 2.0, governor.get_volume_bound(), nullptr, nullptr));
       }
     }

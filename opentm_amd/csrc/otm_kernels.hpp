@@ -44,6 +44,9 @@ void launch_galerkin_first(hipStream_t s, const Geom& fine, const Geom& coarse, 
                            const float* kappa, float* E);
 void launch_galerkin_next(hipStream_t s, const Geom& fine, const Geom& coarse, const float* Ein, float* Eout);
 void launch_stencil_from_elements(hipStream_t s, const Geom& g, const float* E, float* st);
+// Device doubles the coarsest factorisation needs as work space for n nodes.
+size_t coarsest_work_doubles(long long n);
+void launch_coarsest_blocked(hipStream_t s, const Geom& g, const float* st, double* B, double* Binv);
 void launch_coarsest_factor(hipStream_t s, const Geom& g, const float* st, double* B, double* Binv);
 void launch_coarsest_solve(hipStream_t s, const Geom& g, const double* Binv, const float* f, float* u);
 
@@ -134,5 +137,15 @@ int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_
                int itn, double move, double a0, double av, double cv, double dv, float* xout);
 void launch_f2d(hipStream_t s, long long n, const float* in, double* out);
 void launch_fill_d(hipStream_t s, double* a, long long n, double v);
+
+// Symmetry group acting on a periodic lattice: element i maps coordinate c
+// to c' with c'[j] = c[perm[i][j]], reflected (n - 1 - c'[j]) where bit j of
+// flip[i] is set. Permutations need a cubic lattice.
+struct SymGroup {
+  int n;
+  signed char perm[48][3];
+  unsigned char flip[48];
+};
+void launch_symmetrize(hipStream_t s, const Geom& g, const SymGroup& G, const float* in, float* out);
 
 }  // namespace otm

@@ -132,19 +132,21 @@ __global__ void k_oc_init(const unsigned int* __restrict__ bits, OcState* st) {
 // negative), rho' = clamp(B^damp * rho, rho -+ step, [min_rho, 1]). B^damp is
 // evaluated as 2^(damp (log2(-s) - log2 g)) so the logarithm of the
 // sensitivity is taken once per element for all midpoints.
-__device__ inline void oc_tree_mids(const OcState* st, float* g) {
-  float lo[kOcCand], hi[kOcCand];
-  lo[0] = st->lo;
-  hi[0] = st->hi;
-  for (int i = 0; i < kOcCand; i++) {
-    g[i] = 0.5f * (hi[i] + lo[i]);
-    if (2 * i + 2 < kOcCand) {
-      lo[2 * i + 1] = lo[i];
-      hi[2 * i + 1] = g[i];
-      lo[2 * i + 2] = g[i];
-      hi[2 * i + 2] = hi[i];
-    }
+// Midpoint of heap node i: the path from the root is the binary expansion
+// of i + 1 after its leading one (0: left child, hi = g; 1: right, lo = g),
+// with the bracket halved in FP32 exactly as the sequential bisection does.
+__device__ inline float oc_tree_mid(const OcState* st, int i) {
+  float lo = st->lo, hi = st->hi;
+  const unsigned p = (unsigned)i + 1u;
+  const int depth = 31 - __builtin_clz(p);
+  for (int j = depth - 1; j >= 0; j--) {
+    const float g = 0.5f * (hi + lo);
+    if ((p >> j) & 1u)
+      lo = g;
+    else
+      hi = g;
   }
+  return 0.5f * (hi + lo);
 }
 
 __device__ inline float oc_rule_log(float s, float ls, float rho, float lg, float c01, float min_rho, float step,
@@ -163,11 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_oc_tree_sum(long long n, const float
                                                         float damp, float c01, int max_steps, Reduce R) {
   if (st->done || st->steps >= max_steps) return;
   __shared__ float slg[kOcCand];
-  if (threadIdx.x == 0) {
-    float g[kOcCand];
-    oc_tree_mids(st, g);
-    for (int c = 0; c < kOcCand; c++) slg[c] = __builtin_amdgcn_logf(g[c]);
-  }
+  if (threadIdx.x < kOcCand) slg[threadIdx.x] = __builtin_amdgcn_logf(oc_tree_mid(st, threadIdx.x));
   __syncthreads();
   double acc[kOcCand];
 #pragma unroll
@@ -186,7 +184,7 @@ __global__ void __launch_bounds__(kBlock) k_oc_tree_final(OcState* st, Reduce R,
   if (st->done || st->steps >= max_steps) return;
   __shared__ double sh[kOcCand * kWaves];
   __shared__ float g[kOcCand];
-  if (threadIdx.x == 0) oc_tree_mids(st, g);
+  if (threadIdx.x < kOcCand) g[threadIdx.x] = oc_tree_mid(st, threadIdx.x);
   double acc[kOcCand];
 #pragma unroll
   for (int c = 0; c < kOcCand; c++) acc[c] = 0.0;
@@ -335,6 +333,52 @@ void launch_oc_apply(hipStream_t s, long long n, const float* sens, float* rho, 
   unsigned nb = nblk(n);
   if (nb > 8192) nb = 8192;
   hipLaunchKernelGGL(k_oc_apply, dim3(nb), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01);
+}
+
+// Orbit average under a symmetry group (symetrize_tensor_kernel,
+// TensorExpression.cuh:70-300, driven by symmetrizeField, Framework.cuh:221-235):
+// out[v] = mean of in over the orbit of v. Every member of an orbit sums
+// the orbit in the same order (group order applied to the orbit's smallest
+// index), so the members get bitwise the same value; reading `in` and
+// writing `out` keeps the elements independent.
+__global__ void __launch_bounds__(kBlock) k_symmetrize(Geom g, SymGroup G, const float* __restrict__ in,
+                                                       float* __restrict__ out) {
+  const long long t = gtid();
+  if (t >= g.nn) return;
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  const int c[3] = {(int)(t % g.nx), (int)((t / g.nx) % g.ny), (int)(t / ((long long)g.nx * g.ny))};
+  auto image = [&](int i, const int* p, int* q) {
+    for (int j = 0; j < 3; j++) {
+      const int v = p[G.perm[i][j]];
+      q[j] = ((G.flip[i] >> j) & 1) ? dims[j] - 1 - v : v;
+    }
+  };
+  auto lin3 = [&](const int* q) { return (long long)q[0] + (long long)g.nx * ((long long)q[1] + (long long)g.ny * q[2]); };
+  // canonical representative: the orbit member with the smallest index
+  int r[3] = {c[0], c[1], c[2]};
+  long long rbest = t;
+  for (int i = 0; i < G.n; i++) {
+    int q[3];
+    image(i, c, q);
+    const long long id = lin3(q);
+    if (id < rbest) {
+      rbest = id;
+      r[0] = q[0];
+      r[1] = q[1];
+      r[2] = q[2];
+    }
+  }
+  float sum = 0.f;
+  for (int i = 0; i < G.n; i++) {
+    int q[3];
+    image(i, r, q);
+    sum += in[lin3(q)];
+  }
+  out[t] = sum / (float)G.n;
+}
+
+void launch_symmetrize(hipStream_t s, const Geom& g, const SymGroup& G, const float* in, float* out) {
+  hipLaunchKernelGGL(k_symmetrize, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, G, in, out);
 }
 
 }  // namespace otm

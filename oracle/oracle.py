@@ -181,3 +181,47 @@ class MMA:
                                _d(df0dx).reshape(-1), _d(gval).reshape(-1), _d(dgdx).reshape(-1),
                                self.low, self.upp, self.a0, self.a, self.c, self.d, self.move, self.epsimin)
         return xv, its
+
+
+# ---- symmetrisation ---------------------------------------------------------
+# symmetrizeField (Framework.cuh:221-235) with symetrize_tensor_kernel
+# (TensorExpression.cuh:70-300): every value becomes the mean over its orbit
+# under a group of the cube's symmetries about the cell centre. A group
+# element (perm, flip) maps coordinate c to c' with c'[j] = c[perm[j]],
+# reflected to n - 1 - c'[j] when bit j of flip is set.
+_PERMS = [(0, 1, 2), (1, 2, 0), (2, 0, 1), (0, 2, 1), (2, 1, 0), (1, 0, 2)]  # 3 even, 3 odd
+
+
+def sym_group(sym):
+    """Group elements of 'reflect3' (8 reflections), 'reflect6' (48:
+    reflections x permutations), 'rotate3' (the 24 rotations: permutation
+    parity equals reflection-count parity) and 'rotate2' (point reflection)."""
+    if sym == "reflect3":
+        return [(_PERMS[0], f) for f in range(8)]
+    if sym == "reflect6":
+        return [(p, f) for p in _PERMS for f in range(8)]
+    if sym == "rotate3":
+        return [(p, f) for i, p in enumerate(_PERMS) for f in range(8) if bin(f).count("1") % 2 == (1 if i >= 3 else 0)]
+    if sym == "rotate2":
+        return [(_PERMS[0], 0), (_PERMS[0], 7)]
+    raise ValueError(sym)
+
+
+def sym_apply(x, elem):
+    """x o g for one group element: y[c] = x[g(c)], arrays (nz, ny, nx)."""
+    perm, flip = elem
+    nz, ny, nx = x.shape
+    dims = (nx, ny, nz)
+    c = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")  # c[j][x, y, z]
+    q = []
+    for j in range(3):
+        v = c[perm[j]]
+        q.append(dims[j] - 1 - v if (flip >> j) & 1 else v)
+    return np.transpose(x[q[2], q[1], q[0]], (2, 1, 0))
+
+
+def symmetrize(x, sym):
+    """Orbit mean of x (FP64)."""
+    x = np.asarray(x, np.float64)
+    G = sym_group(sym)
+    return sum(sym_apply(x, g) for g in G) / len(G)

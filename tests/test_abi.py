@@ -111,3 +111,19 @@ def test_struct_layouts_match_header():
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = [re.split(r"[\s\[;]", d.strip().split()[-1])[0] for d in body.split(";") if d.strip()]
         assert names == [f[0] for f in py._fields_], cname
+
+
+def test_density_file_round_trip(tmp_path):
+    """Grid_H::writeDensity binary layout: int32 reso[3] then float32, x
+    fastest; host-only calls need no GPU."""
+    rho = np.random.default_rng(1).uniform(0, 1, (4, 5, 6)).astype(np.float32)  # (nz, ny, nx)
+    path = tmp_path / "rho.bin"
+    opentm_amd.write_density(path, rho)
+    raw = path.read_bytes()
+    assert np.frombuffer(raw[:12], np.int32).tolist() == [6, 5, 4]
+    np.testing.assert_array_equal(np.frombuffer(raw[12:], np.float32), rho.reshape(-1))
+    np.testing.assert_array_equal(opentm_amd.read_density(path), rho)
+    bad = tmp_path / "short.bin"
+    bad.write_bytes(raw[:40])
+    with pytest.raises(opentm_amd.OtmError):
+        opentm_amd.read_density(bad)

@@ -95,7 +95,7 @@ def test_galerkin_coarse_operator(orc):
         assert np.linalg.norm(q[c] - ref) / np.linalg.norm(ref) < 2e-6
 
 
-@pytest.mark.parametrize("n", [32, 64])
+@pytest.mark.parametrize("n", [32, 64, 22])
 def test_vcycle_symmetric_positive(n):
     """The V-cycle (pre-smoother forward, post-smoother reversed, Galerkin
     coarse levels) must be a symmetric positive preconditioner for CG."""
@@ -652,3 +652,55 @@ def test_batch_update_matches_single_update():
     for h in homs:
         h.reset_displacement()
     np.testing.assert_array_equal(b.heat_matrix(homs), C1)
+
+
+# ------------------------------------------------------------ symmetry, density files
+@pytest.mark.parametrize("sym", ["reflect3", "reflect6", "rotate3", "rotate2"])
+def test_symmetrize_matches_oracle(orc, sym):
+    """symmetrizeField on the GPU against the oracle's orbit mean."""
+    say("symmetrize", sym)
+    rng = np.random.default_rng(8)
+    for shape in [(8, 8, 8), (12, 12, 12)] + ([(6, 8, 10)] if sym in ("reflect3", "rotate2") else []):
+        x = rng.uniform(0, 1, shape).astype(np.float32)
+        y = otm.symmetrize(x, sym)
+        assert rel(y, orc.symmetrize(x, sym)) < 1e-6
+        assert rel(otm.symmetrize(y, sym), y) < 1e-6
+        for elem in orc.sym_group(sym):  # members of an orbit share one value, bitwise
+            np.testing.assert_array_equal(orc.sym_apply(y, elem).astype(np.float32), y)
+
+
+def test_run_custom_symmetric_design():
+    """runCustom's OC branch with a symmetry: the design stays invariant."""
+    from opentm_amd import workloads  # noqa: F401
+    n = 16
+    say("runCustom reflect3", n)
+    rho, rep = otm.run_custom(n, "oc", 8, initway="IWP", target=(0.5, 0.4, 0.3, 0, 0, 0), heat_ratio=(1.0, 1e-4),
+                              sym="reflect3")
+    r = rho.reshape(n, n, n)
+    for axis in range(3):
+        assert np.abs(r - np.flip(r, axis)).max() < 1e-6
+
+
+@pytest.mark.parametrize("n", [14, 18, 22])
+def test_large_coarsest_blocked_inverse(orc, n):
+    """Cells whose coarsest level exceeds the LDS inverse (7^3, 9^3, 11^3
+    nodes) use the blocked multi-workgroup Gauss-Jordan inverse: kappa^H
+    against the oracle, few PCG iterations (exact coarse solve), and the
+    same in a batch."""
+    say("large coarsest", n)
+    k = wl.simp(wl.plain_density(n, 9))
+    hom = make(n)
+    hom.update(k)
+    levels = hom.levels()
+    C = hom.heatMatrix()
+    its, rr = hom.solve_stats()
+    Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+    say("  levels", levels, "its", its, "err", rel(C, Co))
+    assert int(np.prod(levels[-1])) > 128
+    assert rel(C, Co) < 1e-5
+    assert max(its) <= 12
+    b = otm.Batch()
+    homs, _ = _batch_cells([(n, 9), (16, 3)])
+    b.update(homs, [k, wl.simp(wl.plain_density(16, 3))])
+    Cb = b.heat_matrix(homs)
+    assert rel(Cb[0], Co) < 1e-5

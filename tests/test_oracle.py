@@ -367,3 +367,36 @@ def test_mma_converges_to_constrained_optimum(orc):
         dg = np.stack([np.ones(n) / n, w / n])
         x, _ = mma.update(it, x, 2 * (x - t), g, dg)
     assert np.abs(x - ref.x).max() < 2e-3
+
+
+def test_symmetrize_group_properties(orc):
+    """Orbit averages: group sizes, idempotence, invariance of the result
+    under every group element, mean preservation, symmetric input
+    unchanged, and the 1-D closed form of a reflection."""
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((6, 6, 6))
+    sizes = {"reflect3": 8, "reflect6": 48, "rotate3": 24, "rotate2": 2}
+    for sym, size in sizes.items():
+        G = orc.sym_group(sym)
+        assert len(G) == size and len(set(G)) == size
+        y = orc.symmetrize(x, sym)
+        np.testing.assert_allclose(orc.symmetrize(y, sym), y, atol=1e-12)
+        for g in G:
+            np.testing.assert_allclose(orc.sym_apply(y, g), y, atol=1e-12)
+        assert abs(y.mean() - x.mean()) < 1e-12
+    # rotate3 elements are rotations: determinant +1 of the signed permutation
+    for perm, flip in orc.sym_group("rotate3"):
+        M = np.zeros((3, 3))
+        for j in range(3):
+            M[j, perm[j]] = -1.0 if (flip >> j) & 1 else 1.0
+        assert abs(np.linalg.det(M) - 1.0) < 1e-12
+    # a field depending on x only: reflect3 gives (f(x) + f(n-1-x)) / 2
+    f = rng.standard_normal(6)
+    xf = np.broadcast_to(f[None, None, :], (4, 5, 6))
+    np.testing.assert_allclose(orc.symmetrize(xf, "reflect3"), np.broadcast_to(0.5 * (f + f[::-1]), (4, 5, 6)),
+                               atol=1e-12)
+    # a radially symmetric field about the centre is invariant under all groups
+    c = (np.arange(6) - 2.5) ** 2
+    r = c[:, None, None] + c[None, :, None] + c[None, None, :]
+    for sym in sizes:
+        np.testing.assert_allclose(orc.symmetrize(r, sym), r, atol=1e-12)
