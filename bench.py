@@ -33,6 +33,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "homog. solves/s + p50/p99 latency at κᴴ rel err ≤1e-5; stencil HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fine brick smoother from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs of this workload, scaled by a copy-kernel calibration
+# (profiles/traffic_fine_brick_128.md gives the commands and figures).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_fine_brick_128.json")
+
+
+def measured_traffic(kernel, reso):
+    """HBM bytes per launch of `kernel` at `reso`, from the committed counter
+    summary, or None when no summary matches this kernel and size."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or t.get("reso") != reso:
+        return None
+    return t.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(reso, seed, rtol):
@@ -72,6 +89,8 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--coarse-sweeps", type=int, default=1,
                     help="Gauss-Seidel sweeps per smoothing step on the coarse levels (fine level: 2)")
+    ap.add_argument("--min-coarse", type=int, default=None,
+                    help="coarsest level edge floor (otm_config.min_coarse); default: the library's")
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel class with HIP events (adds event overhead to the step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -114,6 +133,8 @@ def main():
     cfg = otm.HomoConfig(n, (1.0, 1e-3))
     cfg.femRelThres = args.rtol
     cfg.coarse_sweeps = args.coarse_sweeps
+    if args.min_coarse is not None:
+        cfg.min_coarse = args.min_coarse
     hom = otm.Homogenization_H(cfg)
     hom.set_design(wl.oc_start_density(n, args.seed))
     oc = otm.OCOptimizer(n ** 3, 0.001, 0.2, 0.5, bisect_steps=60, bisect_tol=1e-6)
@@ -181,7 +202,8 @@ def main():
         "kappaH_final": [float(v) for v in C.reshape(-1)],
         "roofline": {"bound": "hbm", "kernel": "k_fine_smooth_brick<32>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "bytes_per_launch": by / nl, "avg_launch_us": round(avg_s * 1e6, 3), "traffic": None},
+                     "bytes_per_launch": by / nl, "avg_launch_us": round(avg_s * 1e6, 3),
+                     "traffic": measured_traffic("k_fine_smooth_brick<32>", n)},
         "kernel_classes": rows,
     }
     if not args.no_cpu_baseline:

@@ -48,6 +48,7 @@ struct CellDev {
   int nlev;
   int tail;  // first level of the one-workgroup tail V-cycle
   int fine_bx;  // 32 or 16: fine brick width; 0: exact colour sweeps
+  int tile_bx;  // 32 or 16: LDS tile width of the fine residual and apply; 0: pointwise kernels
   int sweeps_fine, sweeps_coarse;
   FineConsts fc;
   const float* kappa;
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(Op::kThreads) kb_run(const Ent* __restrict__ e
 
 // Fine bricks keep the single-cell kernel's occupancy hint.
 template <int BX>
-__global__ void __launch_bounds__(BX * 8) __attribute__((amdgpu_waves_per_eu(3, 8)))
+__global__ void __launch_bounds__(BX * kFineTR) __attribute__((amdgpu_waves_per_eu(4, 8)))
 kb_fine_brick(const Ent* __restrict__ ents, int nents, const CellDev* __restrict__ cells, int post) {
   const int b = xcd_block();
   const int i = find_ent(ents, nents, b);
@@ -173,11 +174,16 @@ struct OpFillU {  // zero the current correction of level l (exact colour sweeps
   }
 };
 
-struct OpFineResid {
+struct OpFineResid {  // one LDS tile per block when the cell tiles
   static constexpr int kThreads = kBlock;
   __device__ void operator()(const CellDev& C, const Ent& e, int lb, int) const {
-    fine_residual_body(C.g[0], C.fc, C.kappa, C.f[0], ucur(C, e.par, 0), C.res[0],
-                       (long long)lb * kBlock + threadIdx.x);
+    if (C.tile_bx == 32)
+      fine_residual_tile_body<32, kBlock / 32>(C.g[0], C.fc, C.kappa, C.f[0], ucur(C, e.par, 0), C.res[0], lb);
+    else if (C.tile_bx == 16)
+      fine_residual_tile_body<16, kBlock / 16>(C.g[0], C.fc, C.kappa, C.f[0], ucur(C, e.par, 0), C.res[0], lb);
+    else
+      fine_residual_body(C.g[0], C.fc, C.kappa, C.f[0], ucur(C, e.par, 0), C.res[0],
+                         (long long)lb * kBlock + threadIdx.x);
   }
 };
 
@@ -250,11 +256,14 @@ struct OpInitResid {
 struct OpFinal {  // one block per cell
   static constexpr int kThreads = kBlock;
   int mode, nv;
-  double rtol2;  // < 0: the cell's own
+  double rtol2;   // < 0: the cell's own
+  int apply = 0;  // partials of the operator apply (tiled cells have their own count)
   __device__ void operator()(const CellDev& C, const Ent&, int, int) const {
     Reduce R = cell_reduce(C, mode);
     if (rtol2 >= 0.0) R.rtol2 = rtol2;
-    reduce_final_body(R, nv, (int)red_blocks_for_dev(C.g[0].nn));
+    const unsigned nb = (apply && C.tile_bx) ? fine_apply_tile_blocks(C.g[0], C.tile_bx)
+                                              : red_blocks_for_dev(C.g[0].nn);
+    reduce_final_body(R, nv, (int)nb);
   }
   __device__ static unsigned red_blocks_for_dev(long long n) {
     long long b = (n + kBlock - 1) / kBlock;
@@ -265,7 +274,12 @@ struct OpFinal {  // one block per cell
 struct OpApplyDot {
   static constexpr int kThreads = kBlock;
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int nb) const {
-    fine_apply_dot_body(C.g[0], C.fc, C.kappa, C.p, C.q, cell_reduce(C, FIN_PQ), lb, nb);
+    if (C.tile_bx == 32)
+      fine_apply_tile_body<32, kBlock / 32>(C.g[0], C.fc, C.kappa, C.p, C.q, cell_reduce(C, FIN_PQ), lb, nb);
+    else if (C.tile_bx == 16)
+      fine_apply_tile_body<16, kBlock / 16>(C.g[0], C.fc, C.kappa, C.p, C.q, cell_reduce(C, FIN_PQ), lb, nb);
+    else
+      fine_apply_dot_body(C.g[0], C.fc, C.kappa, C.p, C.q, cell_reduce(C, FIN_PQ), lb, nb);
   }
 };
 
@@ -516,6 +530,7 @@ CellDev describe(const Member& m) {
     d.ub[l] = L.u2 ? L.u2 : L.u;
   }
   d.fine_bx = c->lv[0].brick ? fine_brick_x(c->g0) : 0;
+  d.tile_bx = fine_brick_x(c->g0);
   d.sweeps_fine = c->cfg.pre_sweeps;
   d.sweeps_coarse = c->cfg.coarse_sweeps > 0 ? c->cfg.coarse_sweeps : c->cfg.pre_sweeps;
   d.fc = c->fc;
@@ -606,6 +621,15 @@ void flush(otm_batch* b) {
 
 long long nodes0(const Member& m) { return m.c->g0.nn; }
 long long redb(const Member& m) { return red_blocks_for(m.c->g0.nn); }
+// blocks of the operator apply and of the fine residual (LDS tiles when the cell tiles)
+long long applyb(const Member& m) {
+  const int bx = fine_brick_x(m.c->g0);
+  return bx ? (long long)fine_apply_tile_blocks(m.c->g0, bx) : (long long)red_blocks_for(m.c->g0.nn);
+}
+long long residb(const Member& m) {
+  const int bx = fine_brick_x(m.c->g0);
+  return bx ? (long long)fine_tiles(m.c->g0, bx) : (long long)nblk(m.c->g0.nn);
+}
 long long one(const Member&) { return 1; }
 
 // V-cycle for the member set V (all levels, every member at its own depth).
@@ -632,12 +656,14 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
       }
       CellDev* cells = b->dcells;
       hipStream_t s = b->s;
-      auto bricks = [](const Member& m) { return (long long)(m.c->g0.nn / (fine_brick_x(m.c->g0) * 64)); };
+      auto bricks = [](const Member& m) {
+        return (long long)(m.c->g0.nn / (fine_brick_x(m.c->g0) * kFineBY * kFineBZ));
+      };
       add_launch(b, b32, bricks, 0, [cells, s](const Ent* e, int n, unsigned grid) {
-        hipLaunchKernelGGL(kb_fine_brick<32>, dim3(grid), dim3(256), 0, s, e, n, cells, 0);
+        hipLaunchKernelGGL(kb_fine_brick<32>, dim3(grid), dim3(32 * kFineTR), 0, s, e, n, cells, 0);
       });
       add_launch(b, b16, bricks, 0, [cells, s](const Ent* e, int n, unsigned grid) {
-        hipLaunchKernelGGL(kb_fine_brick<16>, dim3(grid), dim3(128), 0, s, e, n, cells, 0);
+        hipLaunchKernelGGL(kb_fine_brick<16>, dim3(grid), dim3(16 * kFineTR), 0, s, e, n, cells, 0);
       });
       if (!col.empty()) {
         add_op(b, col, OpFillU{0}, [](const Member& m) { return (long long)nblk(3 * m.c->g0.nn); });
@@ -646,7 +672,7 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
             add_op(b, col, OpFineGsColor{cc & 1, cc >> 1},
                    [](const Member& m) { return (long long)nblk(m.c->g0.nx * (m.c->g0.ny / 2) * (m.c->g0.nz / 2)); });
       }
-      add_op(b, S, OpFineResid{}, [](const Member& m) { return (long long)nblk(m.c->g0.nn); });
+      add_op(b, S, OpFineResid{}, residb);
     } else {
       add_op(b, S, OpStBrick{l, 0}, [l](const Member& m) { return (long long)brick_count(m.c->lv[l].g); });
       add_op(b, S, OpStResid{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
@@ -659,7 +685,7 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
   for (int l = maxlev - 2; l >= 0; l--) {
     std::vector<Member*> S = at(l, false);
     if (S.empty()) continue;
-    add_op(b, S, OpProlong{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
+    add_op(b, S, OpProlong{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l + 1].g.nn); });
     if (l == 0) {
       std::vector<Member*> b32, b16, col;
       for (Member* m : S) {
@@ -668,12 +694,14 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
       }
       CellDev* cells = b->dcells;
       hipStream_t s = b->s;
-      auto bricks = [](const Member& m) { return (long long)(m.c->g0.nn / (fine_brick_x(m.c->g0) * 64)); };
+      auto bricks = [](const Member& m) {
+        return (long long)(m.c->g0.nn / (fine_brick_x(m.c->g0) * kFineBY * kFineBZ));
+      };
       add_launch(b, b32, bricks, 0, [cells, s](const Ent* e, int n, unsigned grid) {
-        hipLaunchKernelGGL(kb_fine_brick<32>, dim3(grid), dim3(256), 0, s, e, n, cells, 1);
+        hipLaunchKernelGGL(kb_fine_brick<32>, dim3(grid), dim3(32 * kFineTR), 0, s, e, n, cells, 1);
       });
       add_launch(b, b16, bricks, 0, [cells, s](const Ent* e, int n, unsigned grid) {
-        hipLaunchKernelGGL(kb_fine_brick<16>, dim3(grid), dim3(128), 0, s, e, n, cells, 1);
+        hipLaunchKernelGGL(kb_fine_brick<16>, dim3(grid), dim3(16 * kFineTR), 0, s, e, n, cells, 1);
       });
       for (Member* m : b32) m->par ^= 1;
       for (Member* m : b16) m->par ^= 1;
@@ -736,8 +764,8 @@ void step(otm_batch* b) {
   add_op(b, A32, OpInitResid<float>{-1.0}, redb);
   add_op(b, A64, OpInitResid<double>{-1.0}, redb);
   add_op(b, A, OpFinal{FIN_BB_RR, 3 * OTM_NRHS, -1.0}, one);
-  add_op(b, B, OpApplyDot{}, redb);
-  add_op(b, B, OpFinal{FIN_PQ, OTM_NRHS, -1.0}, one);
+  add_op(b, B, OpApplyDot{}, applyb);
+  add_op(b, B, OpFinal{FIN_PQ, OTM_NRHS, -1.0, 1}, one);
   add_op(b, B, OpXr{}, redb);
   add_op(b, B, OpFinal{FIN_RR, OTM_NRHS, -1.0}, one);
   std::vector<Member*> AB(A);

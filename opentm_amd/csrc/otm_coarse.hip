@@ -312,7 +312,7 @@ void launch_restrict(hipStream_t s, const Geom& fine, const Geom& coarse, const 
   hipLaunchKernelGGL(k_restrict, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, res, fc);
 }
 void launch_prolong_add(hipStream_t s, const Geom& fine, const Geom& coarse, const float* uc, float* uf) {
-  hipLaunchKernelGGL(k_prolong_add, dim3(nblk(fine.nn)), dim3(kBlock), 0, s, fine, coarse, uc, uf);
+  hipLaunchKernelGGL(k_prolong_add, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, uc, uf);
 }
 void launch_galerkin_first(hipStream_t s, const Geom& fine, const Geom& coarse, const GalerkinConsts& G,
                            const float* kappa, float* E) {

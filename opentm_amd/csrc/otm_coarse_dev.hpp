@@ -72,9 +72,8 @@ __device__ inline void st_gs_node(const Geom& g, const StRef& S, const float* __
 template <bool HALF>
 __device__ inline void st_resid_node(const Geom& g, const StRef& S, const float* __restrict__ f, const float* u,
                                      float* res, long long t) {
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
   const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
   const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
@@ -93,9 +92,8 @@ __device__ inline void st_resid_node(const Geom& g, const StRef& S, const float*
 // f_c(I) = sum_{d in {-1,0,1}^3} w(d) res(2I + d), w = prod(d_i == 0 ? 1 : 1/2)
 // (restrict_residual_kernel_H, grid.cu:811-878, weights (2-|d|)/2 per axis).
 __device__ inline void restrict_node(const Geom& fg, const Geom& cg, const float* res, float* fc, long long t) {
-  const int X = (int)(t % cg.nx);
-  const long long r = t / cg.nx;
-  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  int X, Y, Z;
+  split_node(cg.nx, cg.ny, t, X, Y, Z);
   const int x = 2 * X, y = 2 * Y, z = 2 * Z;
   const int xs[3] = {x == 0 ? fg.nx - 1 : x - 1, x, x + 1};
   const int ys[3] = {y == 0 ? fg.ny - 1 : y - 1, y, y + 1};
@@ -122,32 +120,11 @@ __device__ inline void restrict_node(const Geom& fg, const Geom& cg, const float
 // u_f(i) += sum_I P(i, I) u_c(I), trilinear (prolongate_correction_kernel_H,
 // grid.cu:881-940).
 __device__ inline void prolong_node(const Geom& fg, const Geom& cg, const float* uc, float* uf, long long t) {
-  const int x = (int)(t % fg.nx);
-  const long long r = t / fg.nx;
-  const int y = (int)(r % fg.ny), z = (int)(r / fg.ny);
-  int X[2], Y[2], Z[2];
-  float wx[2], wy[2], wz[2];
-  X[0] = x >> 1; X[1] = (x & 1) ? ((x >> 1) + 1 == cg.nx ? 0 : (x >> 1) + 1) : X[0];
-  wx[0] = (x & 1) ? 0.5f : 1.f; wx[1] = (x & 1) ? 0.5f : 0.f;
-  Y[0] = y >> 1; Y[1] = (y & 1) ? ((y >> 1) + 1 == cg.ny ? 0 : (y >> 1) + 1) : Y[0];
-  wy[0] = (y & 1) ? 0.5f : 1.f; wy[1] = (y & 1) ? 0.5f : 0.f;
-  Z[0] = z >> 1; Z[1] = (z & 1) ? ((z >> 1) + 1 == cg.nz ? 0 : (z >> 1) + 1) : Z[0];
-  wz[0] = (z & 1) ? 0.5f : 1.f; wz[1] = (z & 1) ? 0.5f : 0.f;
+  int x, y, z;
+  split_node(fg.nx, fg.ny, t, x, y, z);
+  const ProlongW P = prolong_weights(cg, x, y, z);
 #pragma unroll
-  for (int k = 0; k < OTM_NRHS; k++) {
-    const float* ck = uc + k * cg.nn;
-    float s = 0.f;
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-          const float w = wx[a] * wy[b] * wz[c];
-          if (w != 0.f) s += w * ck[lin(cg, X[a], Y[b], Z[c])];
-        }
-    uf[k * fg.nn + t] += s;
-  }
+  for (int k = 0; k < OTM_NRHS; k++) uf[k * fg.nn + t] += prolong_sum(cg, uc + k * cg.nn, P);
 }
 
 // ---- per-level bodies ----
@@ -158,9 +135,11 @@ __device__ inline void st_gs_color_body(const Geom& g, const StRef& S, const flo
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
   const long long cnt = (long long)hx * hy * hz;
   if (t >= cnt) return;
-  const int x = 2 * (int)(t % hx) + cx;
-  const long long r = t / hx;
-  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+  int x, y, z;
+  split_node(hx, hy, t, x, y, z);
+  x = 2 * x + cx;
+  y = 2 * y + cy;
+  z = 2 * z + cz;
   st_gs_node<HALF>(g, S, f, u, x, y, z);
 }
 
@@ -177,10 +156,55 @@ __device__ inline void restrict_body(const Geom& fg, const Geom& cg, const float
   restrict_node(fg, cg, res, fc, t);
 }
 
+// Prolongation by coarse cell: thread t owns coarse node (X, Y, Z) and the
+// eight fine nodes (2X+dx, 2Y+dy, 2Z+dz). It loads the eight coarse values
+// around the cell once per right-hand side and adds to every fine node the
+// terms of prolong_sum in its order (zero weights skipped), so the result
+// is prolong_node's bit for bit with an eighth of the coarse loads and
+// paired, coalesced fine accesses. The fine level is twice the coarse one
+// along every axis.
 __device__ inline void prolong_body(const Geom& fg, const Geom& cg, const float* __restrict__ uc,
                                     float* __restrict__ uf, long long t) {
-  if (t >= fg.nn) return;
-  prolong_node(fg, cg, uc, uf, t);
+  if (t >= cg.nn) return;
+  int X, Y, Z;
+  split_node(cg.nx, cg.ny, t, X, Y, Z);
+  const int Xs[2] = {X, X + 1 == cg.nx ? 0 : X + 1};
+  const int Ys[2] = {Y, Y + 1 == cg.ny ? 0 : Y + 1};
+  const int Zs[2] = {Z, Z + 1 == cg.nz ? 0 : Z + 1};
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* ck = uc + k * cg.nn;
+    float v[2][2][2];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+          v[c][b][a] = ck[(long long)Xs[a] + (long long)cg.nx * ((long long)Ys[b] + (long long)cg.ny * Zs[c])];
+    float* fk = uf + k * fg.nn;
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+          const ProlongW P = prolong_weights(cg, 2 * X + dx, 2 * Y + dy, 2 * Z + dz);
+          float s = 0.f;
+#pragma unroll
+          for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+              for (int a = 0; a < 2; a++) {
+                const float w = P.wx[a] * P.wy[b] * P.wz[c];
+                if (w != 0.f) s += w * v[c & dz][b & dy][a & dx];
+              }
+          const long long fi = (long long)(2 * X + dx) +
+                               (long long)fg.nx * ((long long)(2 * Y + dy) + (long long)fg.ny * (2 * Z + dz));
+          fk[fi] += s;
+        }
+  }
 }
 
 // u = Binv f for right-hand side k, row i, FP64 accumulation.
@@ -282,9 +306,8 @@ __device__ inline void st_brick_body(const Geom& g, const StRef& S, const float*
 __device__ inline void galerkin_first_body(const Geom& fg, const Geom& cg, const GalerkinConsts& G,
                                            const float* __restrict__ kappa, float* __restrict__ E, long long t) {
   if (t >= cg.nn) return;
-  const int X = (int)(t % cg.nx);
-  const long long r = t / cg.nx;
-  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  int X, Y, Z;
+  split_node(cg.nx, cg.ny, t, X, Y, Z);
   float kc[8];
 #pragma unroll
   for (int c = 0; c < 8; c++)
@@ -307,9 +330,8 @@ __device__ constexpr float w1d(int p, int M) {
 __device__ inline void galerkin_next_body(const Geom& fg, const Geom& cg, const float* __restrict__ Ein,
                                           float* __restrict__ Eout, long long t) {
   if (t >= cg.nn) return;
-  const int X = (int)(t % cg.nx);
-  const long long r = t / cg.nx;
-  const int Y = (int)(r % cg.ny), Z = (int)(r / cg.ny);
+  int X, Y, Z;
+  split_node(cg.nx, cg.ny, t, X, Y, Z);
   float acc[36];
 #pragma unroll
   for (int q = 0; q < 36; q++) acc[q] = 0.f;
@@ -359,9 +381,8 @@ __device__ inline void galerkin_next_body(const Geom& fg, const Geom& cg, const 
 __device__ inline void stencil_from_elements_body(const Geom& g, const float* __restrict__ E, float* __restrict__ st,
                                                   long long t) {
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   float a[27];
 #pragma unroll
   for (int o = 0; o < 27; o++) a[o] = 0.f;
@@ -412,9 +433,8 @@ __device__ inline void stencil_absmax_body(const Geom& g, const float* __restric
 __device__ inline void stencil_pack_body(const Geom& g, const float* __restrict__ st, const float* __restrict__ scale,
                                          _Float16* __restrict__ st16, long long t) {
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   const float sc = *scale;
   const float inv = sc > 0.f ? 1.f / sc : 0.f;
 #pragma unroll
@@ -535,20 +555,24 @@ __device__ inline void coarsest_invert_lds_body(const Geom& g, const float* __re
 
 
 // ---- one-workgroup tail V-cycle ---------------------------------------------
-// The last levels of a hierarchy (from the first level with at most
-// kTailMaxNodes nodes down to the coarsest) hold a few thousand nodes: as
+// The last levels of a hierarchy (from the first coarse level with at most
+// kTailStartNodes nodes down to the coarsest) hold a few hundred nodes: as
 // separate launches their smoothers, residuals and transfers are pure
-// launch latency. One workgroup runs the whole tail V-cycle instead: the
-// corrections of every tail level stay in LDS, smoothing is exact 8-colour
-// Gauss-Seidel (forward before restriction, reverse after prolongation, so
-// the V-cycle stays symmetric), residuals and restricted right-hand sides go
-// through the level buffers in global memory (visible to the workgroup after
-// each barrier), and the coarsest level is solved with its explicit inverse.
+// launch and memory latency. One workgroup runs the whole tail V-cycle
+// instead. The stencils of its non-coarsest levels are staged into LDS once
+// (kTailStNodes nodes at most) and the corrections of every tail level stay
+// in LDS; smoothing is exact 8-colour Gauss-Seidel (forward before
+// restriction, reverse after prolongation, so the V-cycle stays symmetric),
+// residuals and restricted right-hand sides go through the level buffers in
+// global memory (visible to the workgroup after each barrier), and the
+// coarsest level is solved with its explicit inverse. A hierarchy whose
+// coarse levels are all larger keeps only the coarsest level in the tail
+// (up to kTailLdsNodes nodes).
 constexpr int kTailMaxLevels = 6;
-constexpr int kTailMaxNodes = 4096;
-constexpr int kTailStartNodes = 1;  // largest first tail level the hierarchies use (see build_hierarchy)
-constexpr int kTailThreads = 1024;
-constexpr int kTailLdsNodes = 4096 + 512 + 64 + 8 + 1 + 1;
+constexpr int kTailStartNodes = 512;                 // largest first tail level (8^3)
+constexpr int kTailStNodes = 512 + 64 + 8 + 1;       // stencil nodes held in LDS
+constexpr int kTailThreads = 512;  // (1024 would cap registers at 128 and spill)
+constexpr int kTailLdsNodes = 4096;                  // correction nodes held in LDS
 
 struct TailLevels {
   int n;  // levels in the tail; the last is the coarsest
@@ -571,8 +595,9 @@ __device__ inline void tail_stencil(const StRef& S, long long nn, long long v, f
     load_stencil<false>(S, nn, v, a);
 }
 
-// `sweeps` exact 8-colour Gauss-Seidel sweeps on u (LDS) of one tail level.
-__device__ inline void tail_smooth(const Geom& g, const StRef& S, const float* __restrict__ f, float* u, int sweeps,
+// `sweeps` exact 8-colour Gauss-Seidel sweeps on u (LDS) of one tail level
+// with its stencil sa[o*nn + v] in LDS.
+__device__ inline void tail_smooth(const Geom& g, const float* sa, const float* __restrict__ f, float* u, int sweeps,
                                    bool forward) {
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
   const int cnt = hx * hy * hz;
@@ -589,7 +614,8 @@ __device__ inline void tail_smooth(const Geom& g, const StRef& S, const float* _
         const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
         const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
         float a[27];
-        tail_stencil(S, nn, v, a);
+#pragma unroll
+        for (int o = 0; o < 27; o++) a[o] = sa[o * nn + v];
         const float inv = 1.f / a[13];
 #pragma unroll
         for (int k = 0; k < OTM_NRHS; k++) {
@@ -607,32 +633,63 @@ __device__ inline void tail_smooth(const Geom& g, const StRef& S, const float* _
     }
 }
 
+// res = f - A u at node t of a tail level (st_resid_node's arithmetic with
+// the stencil from LDS).
+__device__ inline void tail_resid_node(const Geom& g, const float* sa, const float* __restrict__ f, const float* u,
+                                       float* res, int t) {
+  const int nn = (int)g.nn;
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
+  const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
+  const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
+  const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
+  float a[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++) a[o] = sa[o * nn + t];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* uk = u + k * nn;
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < 27; o++) s += a[o] * uk[xs[o % 3] + g.nx * (ys[(o / 3) % 3] + g.ny * zs[o / 9])];
+    res[k * nn + t] = f[k * nn + t] - s;
+  }
+}
+
+// Offsets of tail level l in the LDS corrections and in the LDS stencils
+// (every level before it stores OTM_NRHS, resp. 27, values per node).
+__device__ inline int tail_nodes_before(const TailLevels& T, int l) {
+  int o = 0;
+  for (int m = 0; m < l; m++) o += (int)T.g[m].nn;
+  return o;
+}
+
 __device__ inline void tail_vcycle_body(const TailLevels& T) {
   __shared__ float su[OTM_NRHS * kTailLdsNodes];
+  __shared__ float sst[27 * kTailStNodes];
   const int tid = threadIdx.x, bs = blockDim.x;
-  int off[kTailMaxLevels];
-  {
-    int o = 0;
-    for (int l = 0; l < T.n; l++) {
-      off[l] = o;
-      o += OTM_NRHS * (int)T.g[l].nn;
+  const int L = T.n - 1;
+  auto off = [&](int l) { return OTM_NRHS * tail_nodes_before(T, l); };
+  auto soff = [&](int l) { return 27 * tail_nodes_before(T, l); };
+  // stencils of the non-coarsest levels, the values load_stencil gives
+  for (int l = 0; l < L; l++) {
+    const int nn = (int)T.g[l].nn;
+    for (int v = tid; v < nn; v += bs) {
+      float a[27];
+      tail_stencil(T.st[l], nn, v, a);
+#pragma unroll
+      for (int o = 0; o < 27; o++) sst[soff(l) + o * nn + v] = a[o];
     }
   }
-  const int L = T.n - 1;
   // down: smooth from zero, residual, restriction
   for (int l = 0; l < L; l++) {
     const Geom& g = T.g[l];
-    float* u = su + off[l];
+    float* u = su + off(l);
     const float* f = l == 0 ? T.f0 : T.f[l];
     for (int i = tid; i < OTM_NRHS * (int)g.nn; i += bs) u[i] = 0.f;
     __syncthreads();
-    tail_smooth(g, T.st[l], f, u, T.sweeps, true);
-    for (int t = tid; t < (int)g.nn; t += bs) {
-      if (T.st[l].f16)
-        st_resid_node<true>(g, T.st[l], f, u, T.res[l], t);
-      else
-        st_resid_node<false>(g, T.st[l], f, u, T.res[l], t);
-    }
+    tail_smooth(g, sst + soff(l), f, u, T.sweeps, true);
+    for (int t = tid; t < (int)g.nn; t += bs) tail_resid_node(g, sst + soff(l), f, u, T.res[l], t);
     __syncthreads();
     for (int t = tid; t < (int)T.g[l + 1].nn; t += bs) restrict_node(g, T.g[l + 1], T.res[l], T.f[l + 1], t);
     __syncthreads();
@@ -645,17 +702,17 @@ __device__ inline void tail_vcycle_body(const TailLevels& T) {
       const int k = q / n, i = q - k * n;
       double s = 0.0;
       for (int j = 0; j < n; j++) s += T.Binv[(long long)i * n + j] * (double)f[k * n + j];
-      su[off[L] + q] = (float)s;
+      su[off(L) + q] = (float)s;
     }
     __syncthreads();
   }
   // up: prolongation, reverse smoothing
   for (int l = L - 1; l >= 0; l--) {
     const Geom& g = T.g[l];
-    float* u = su + off[l];
-    for (int t = tid; t < (int)g.nn; t += bs) prolong_node(g, T.g[l + 1], su + off[l + 1], u, t);
+    float* u = su + off(l);
+    for (int t = tid; t < (int)g.nn; t += bs) prolong_node(g, T.g[l + 1], su + off(l + 1), u, t);
     __syncthreads();
-    tail_smooth(g, T.st[l], l == 0 ? T.f0 : T.f[l], u, T.sweeps, false);
+    tail_smooth(g, sst + soff(l), l == 0 ? T.f0 : T.f[l], u, T.sweeps, false);
   }
   for (int i = tid; i < OTM_NRHS * (int)T.g[0].nn; i += bs) T.uout[i] = su[i];
 }

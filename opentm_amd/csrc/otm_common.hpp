@@ -22,6 +22,56 @@ struct Geom {
 
 __host__ __device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// (x, y, z) of lexicographic index t on an nx x ny x * grid, x fastest.
+// Indices stay below 2^31 (every axis is at most 1024), so two 32-bit
+// divisions replace the much longer 64-bit division sequences.
+__device__ inline void split_node(int nx, int ny, long long t, int& x, int& y, int& z) {
+  const unsigned u = (unsigned)t, r = u / (unsigned)nx, q = r / (unsigned)ny;
+  x = (int)(u - r * (unsigned)nx);
+  y = (int)(r - q * (unsigned)ny);
+  z = (int)q;
+}
+
+// Trilinear prolongation from a coarse grid of half the size
+// (prolongate_correction_kernel_H, grid.cu:881-940): fine node (x, y, z)
+// takes coarse nodes X[a], Y[b], Z[c] with weight wx[a] wy[b] wz[c]; on an
+// even coordinate the second weight is zero.
+struct ProlongW {
+  int X[2], Y[2], Z[2];
+  float wx[2], wy[2], wz[2];
+};
+__device__ inline ProlongW prolong_weights(const Geom& cg, int x, int y, int z) {
+  ProlongW P;
+  P.X[0] = x >> 1;
+  P.X[1] = (x & 1) ? ((x >> 1) + 1 == cg.nx ? 0 : (x >> 1) + 1) : P.X[0];
+  P.wx[0] = (x & 1) ? 0.5f : 1.f;
+  P.wx[1] = (x & 1) ? 0.5f : 0.f;
+  P.Y[0] = y >> 1;
+  P.Y[1] = (y & 1) ? ((y >> 1) + 1 == cg.ny ? 0 : (y >> 1) + 1) : P.Y[0];
+  P.wy[0] = (y & 1) ? 0.5f : 1.f;
+  P.wy[1] = (y & 1) ? 0.5f : 0.f;
+  P.Z[0] = z >> 1;
+  P.Z[1] = (z & 1) ? ((z >> 1) + 1 == cg.nz ? 0 : (z >> 1) + 1) : P.Z[0];
+  P.wz[0] = (z & 1) ? 0.5f : 1.f;
+  P.wz[1] = (z & 1) ? 0.5f : 0.f;
+  return P;
+}
+// sum_I P(i, I) u_c(I) for one right-hand side ck of the coarse grid.
+__device__ inline float prolong_sum(const Geom& cg, const float* __restrict__ ck, const ProlongW& P) {
+  float s = 0.f;
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const float w = P.wx[a] * P.wy[b] * P.wz[c];
+        if (w != 0.f)
+          s += w * ck[(long long)P.X[a] + (long long)cg.nx * ((long long)P.Y[b] + (long long)cg.ny * P.Z[c])];
+      }
+  return s;
+}
+
 // Element constants of the fine level (closed forms of the reference's
 // quadrature, templateMatrix.cpp:21-87): Ke = c * P with P = 4 on the
 // diagonal, 0 between nodes that differ along one axis, -1 otherwise,
@@ -145,7 +195,9 @@ __device__ inline void block_sum(double (&v)[NV], double* sh) {
 // partial sums (FP64). Stage 2 (k_reduce_final, launched right after on
 // the same stream) sums the partials in block order and applies the
 // finaliser; the kernel boundary makes the partials visible without
-// device-scope fences inside the producing kernel.
+// device-scope fences inside the producing kernel (on this GPU an
+// agent-scope release writes back the XCD's L2, which measured several
+// times the cost of the separate one-block launch).
 template <int NV>
 __device__ inline void reduce_block_at(double (&v)[NV], const Reduce& R, int slot) {
   __shared__ double sh[NV * kWaves];

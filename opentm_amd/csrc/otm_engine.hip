@@ -225,18 +225,26 @@ static void build_hierarchy(otm_ctx* c) {
     c->lv.push_back(L);
   }
   c->cB = dalloc<double>(coarsest_work_doubles(ncoarse));  // Gauss-Jordan work space
-  // the tail V-cycle takes over from the first coarse level small enough
-  // for one workgroup (the coarsest level at the latest). Measured at 128^3,
-  // a tail that starts at 16^3 costs 131 us per V-cycle against about 100 us
-  // for the separate launches it replaces (each of its 40 barrier phases
-  // waits for its stencil loads), so the tail currently holds the coarsest
-  // level only.
+  // the tail V-cycle takes over from the first coarse level with at most
+  // kTailStartNodes nodes whose levels fit the tail's LDS (the coarsest level
+  // alone otherwise). A tail from 16^3 with stencils re-read from global
+  // memory measured slower than the separate launches (131 against about
+  // 100 us per V-cycle at 128^3); from 8^3 with the stencils in LDS it
+  // replaces eleven launches.
   c->tail = (int)c->lv.size() - 1;
-  for (size_t l = 1; l < c->lv.size(); l++)
-    if (c->lv[l].g.nn <= kTailStartNodes && (int)(c->lv.size() - l) <= kTailMaxLevels) {
+  for (size_t l = 1; l + 1 < c->lv.size(); l++) {
+    long long st_nodes = 0, nodes = 0;
+    for (size_t m = l; m < c->lv.size(); m++) {
+      nodes += c->lv[m].g.nn;
+      if (m + 1 < c->lv.size()) st_nodes += c->lv[m].g.nn;
+    }
+    if (c->lv[l].g.nn <= kTailStartNodes && (int)(c->lv.size() - l) <= kTailMaxLevels &&
+        st_nodes <= kTailStNodes && nodes <= kTailLdsNodes) {
       c->tail = (int)l;
       break;
     }
+  }
+  if (c->lv.back().g.nn > kTailLdsNodes) fail(OTM_E_UNSUPPORTED, "coarsest level too large for the tail");
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
   c->partials = dalloc<double>(2048 * kMaxReduce);
   c->red_out = dalloc<double>(16);
@@ -387,7 +395,9 @@ static void vcycle(otm_ctx* c, size_t l) {
 //   relative residuals far below the FP32 resolution of the solution.
 // Returns false when the preconditioner turned out not to be positive.
 constexpr double kRefineInner = 1e-4;  // inner reduction per refinement step
-static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
+// *fresh tells whether the last true residual (pcg_host's rr and bb) was
+// evaluated at the final iterate, so the caller need not evaluate it again.
+static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
   const long long n = c->g0.nn;
   const unsigned nb = red_blocks_for(n);
   const double rtol = c->cfg.fem_rel_thres;
@@ -398,6 +408,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
   int it = 0;
   double prev_worst = 1e300;
   bool pending = false;  // a correction in x not yet added to x64
+  *fresh = false;
   for (int restart = 0; restart < max_restarts; restart++) {
     {
       ProfScope ps(c, OTM_PROF_FINE_RESID, (refine ? 56.0 : 28.0) * (double)n);
@@ -417,6 +428,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
     }
     c->read_pcg();
     if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+    *fresh = true;  // every exit below leaves x as just evaluated
     if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
     if (it >= c->cfg.max_cg_iter) break;
     // restart only while the true residual still improves: with the iterate
@@ -427,6 +439,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
       worst = q > worst ? q : worst;
     }
     if (restart > 0 && worst > 0.25 * prev_worst) break;
+    *fresh = false;
     prev_worst = worst;
     {
       ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n);
@@ -468,7 +481,10 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total) {
       }
     }
   }
-  if (pending) launch_refine_add(c->s, 3 * n, c->x64, c->x);
+  if (pending) {
+    launch_refine_add(c->s, 3 * n, c->x64, c->x);
+    *fresh = false;
+  }
   *total = it;
   return true;
 }
@@ -484,7 +500,8 @@ static void pcg_solve(otm_ctx* c) {
   int its[3] = {0, 0, 0};
   int it = 0;
   hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
-  if (!pcg_iterate(c, its, &it)) {
+  bool fresh = false;
+  if (!pcg_iterate(c, its, &it, &fresh)) {
     // The block-hybrid coarse smoother lost positivity on this operator:
     // fall back to exact multicolour Gauss-Seidel on every level (always a
     // positive preconditioner) for this context and solve again.
@@ -492,16 +509,19 @@ static void pcg_solve(otm_ctx* c) {
     c->fallbacks++;
     hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
     int it2 = 0;
-    if (!pcg_iterate(c, its, &it2)) fail(OTM_E_NUMERIC, "preconditioner not positive (MG numerical explode)");
+    if (!pcg_iterate(c, its, &it2, &fresh)) fail(OTM_E_NUMERIC, "preconditioner not positive (MG numerical explode)");
     it += it2;
   }
-  // true residual of the final iterate
-  const Reduce Rf = c->reducer(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n);
-  if (c->x64)
-    launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, Rf, nb);
-  else
-    launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, Rf, nb);
-  c->read_pcg();
+  // true residual of the final iterate (already at hand when the restart
+  // loop ended right after evaluating it)
+  if (!fresh) {
+    const Reduce Rf = c->reducer(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n);
+    if (c->x64)
+      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, Rf, nb);
+    else
+      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, Rf, nb);
+    c->read_pcg();
+  }
   double relres[3];
   for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
   // remove the constant component (translateForce, grid.cpp:654-664)

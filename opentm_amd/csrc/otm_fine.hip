@@ -21,6 +21,21 @@ __global__ void __launch_bounds__(kBlock) k_fine_apply_dot(Geom g, FineConsts fc
   fine_apply_dot_body(g, fc, kappa, p, q, R, xcd_block(), gridDim.x);
 }
 
+// LDS-tiled forms of the two (kBlock threads: BX x kBlock/BX per tile)
+template <int BX>
+__global__ void __launch_bounds__(kBlock) k_fine_residual_tile(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                               const float* __restrict__ f,
+                                                               const float* __restrict__ u, float* __restrict__ res) {
+  fine_residual_tile_body<BX, kBlock / BX>(g, fc, kappa, f, u, res, xcd_block());
+}
+
+template <int BX>
+__global__ void __launch_bounds__(kBlock) k_fine_apply_tile(Geom g, FineConsts fc, const float* __restrict__ kappa,
+                                                            const float* __restrict__ p, float* __restrict__ q,
+                                                            Reduce R) {
+  fine_apply_tile_body<BX, kBlock / BX>(g, fc, kappa, p, q, R, xcd_block(), gridDim.x);
+}
+
 template <typename XT>
 __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                                const XT* __restrict__ x_, float* __restrict__ rvec,
@@ -46,32 +61,27 @@ __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, 
 }
 
 // The occupancy hint keeps the fully unrolled colour loop from holding
-// every node's loads in flight at once (256 VGPRs, one wave per SIMD); the
-// LDS footprint allows three workgroups per CU anyway.
+// every node's loads in flight at once (which costs occupancy); the 30 KB
+// LDS footprint of a 32x4x8 brick allows five workgroups per CU.
 template <int BX>
-__global__ void __launch_bounds__(BX * 8) __attribute__((amdgpu_waves_per_eu(3, 8)))
+__global__ void __launch_bounds__(BX * kFineTR) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa, const float* __restrict__ f,
                     const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward, int zero_in) {
   fine_brick_body<BX>(g, fc, kappa, f, uin, uout, sweeps, forward, zero_in, xcd_block());
 }
 
-int fine_brick_x(const Geom& g) {
-  if (g.ny % 8 || g.nz % 8) return 0;
-  if (g.nx % 32 == 0) return 32;
-  if (g.nx % 16 == 0) return 16;
-  return 0;
-}
+int fine_brick_x(const Geom& g) { return fine_tile_x(g); }
 
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
                               const float* uin, float* uout, int sweeps, bool forward, bool zero_in) {
   const int bxw = fine_brick_x(g);
-  const unsigned nb = (unsigned)((g.nx / bxw) * (g.ny / 8) * (g.nz / 8));
+  const unsigned nb = (unsigned)((g.nx / bxw) * (g.ny / kFineBY) * (g.nz / kFineBZ));
   if (bxw == 32)
-    hipLaunchKernelGGL(k_fine_smooth_brick<32>, dim3(nb), dim3(256), 0, s, g, fc, kappa, f, uin, uout, sweeps,
-                       forward ? 1 : 0, zero_in ? 1 : 0);
+    hipLaunchKernelGGL(k_fine_smooth_brick<32>, dim3(nb), dim3(32 * kFineTR), 0, s, g, fc, kappa, f, uin, uout,
+                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_fine_smooth_brick<16>, dim3(nb), dim3(128), 0, s, g, fc, kappa, f, uin, uout, sweeps,
-                       forward ? 1 : 0, zero_in ? 1 : 0);
+    hipLaunchKernelGGL(k_fine_smooth_brick<16>, dim3(nb), dim3(16 * kFineTR), 0, s, g, fc, kappa, f, uin, uout,
+                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0);
 }
 
 // ---- launchers ---------------------------------------------------------
@@ -85,11 +95,29 @@ void launch_fine_gs_color(hipStream_t s, const Geom& g, const FineConsts& fc, co
 
 void launch_fine_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
                           const float* u, float* res) {
-  hipLaunchKernelGGL(k_fine_residual, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, kappa, f, u, res);
+  const int bx = fine_tile_x(g);
+  if (bx == 32)
+    hipLaunchKernelGGL(k_fine_residual_tile<32>, dim3(fine_tiles(g, 32)), dim3(kBlock), 0, s, g, fc, kappa, f, u, res);
+  else if (bx == 16)
+    hipLaunchKernelGGL(k_fine_residual_tile<16>, dim3(fine_tiles(g, 16)), dim3(kBlock), 0, s, g, fc, kappa, f, u, res);
+  else
+    hipLaunchKernelGGL(k_fine_residual, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, kappa, f, u, res);
 }
 
+// Levels that tile use the LDS-tiled kernel with its own workgroup count
+// (fine_apply_tile_blocks); the reduction's partial count follows it.
 void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* p,
                            float* q, const Reduce& R, unsigned nblocks) {
+  const int bx = fine_tile_x(g);
+  if (bx) {
+    const unsigned nb = fine_apply_tile_blocks(g, bx);
+    if (bx == 32)
+      hipLaunchKernelGGL(k_fine_apply_tile<32>, dim3(nb), dim3(kBlock), 0, s, g, fc, kappa, p, q, R);
+    else
+      hipLaunchKernelGGL(k_fine_apply_tile<16>, dim3(nb), dim3(kBlock), 0, s, g, fc, kappa, p, q, R);
+    launch_reduce_final(s, R, OTM_NRHS, nb);
+    return;
+  }
   hipLaunchKernelGGL(k_fine_apply_dot, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, p, q, R);
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }

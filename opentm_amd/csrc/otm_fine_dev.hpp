@@ -61,7 +61,40 @@ __device__ inline NodeElems load_elems(const Geom& g, const NodeIdx& n, const fl
   return E;
 }
 
-// sum_e kappa_e T_e(u) for one right-hand side.
+// The same from an LDS tile of element conductivities: element
+// (x-1+ax, y-1+ay, z-1+az) at sk[kb + ax + KX*(ay + KY*az)].
+__device__ inline NodeElems tile_elems(const float* sk, int kb, int KX, int KY) {
+  NodeElems E;
+#pragma unroll
+  for (int e = 0; e < 8; e++) E.k[e] = sk[kb + (e & 1) + KX * (((e >> 1) & 1) + KY * (e >> 2))];
+  E.ksum = ((E.k[0] + E.k[1]) + (E.k[2] + E.k[3])) + ((E.k[4] + E.k[5]) + (E.k[6] + E.k[7]));
+  return E;
+}
+
+// The 20 neighbours a node couples to (one-axis neighbours have a zero
+// coefficient), named by offset: first letter x (m/0/p), then y, then z.
+struct Far20 {
+  float mm0, pm0, mp0, pp0, m0m, p0m, m0p, p0p;
+  float mmm, _0mm, pmm, mmp, _0mp, pmp, mpm, _0pm, ppm, mpp, _0pp, ppp;
+};
+
+// sum_e kappa_e T_e(u): element (ax,ay,az) lies on side s = 2a-1 and
+// couples to far nodes (sx,sy,0) (sx,0,sz) (0,sy,sz) (sx,sy,sz). One
+// expression for every kernel, so each evaluates it in the same order.
+__device__ inline float far_combine(const NodeElems& E, const Far20& u) {
+  float s = 0.f;
+  s += E.k[0] * (u.mm0 + u.m0m + u._0mm + u.mmm);
+  s += E.k[1] * (u.pm0 + u.p0m + u._0mm + u.pmm);
+  s += E.k[2] * (u.mp0 + u.m0m + u._0pm + u.mpm);
+  s += E.k[3] * (u.pp0 + u.p0m + u._0pm + u.ppm);
+  s += E.k[4] * (u.mm0 + u.m0p + u._0mp + u.mmp);
+  s += E.k[5] * (u.pm0 + u.p0p + u._0mp + u.pmp);
+  s += E.k[6] * (u.mp0 + u.m0p + u._0pp + u.mpp);
+  s += E.k[7] * (u.pp0 + u.p0p + u._0pp + u.ppp);
+  return s;
+}
+
+// sum_e kappa_e T_e(u) for one right-hand side, neighbours from global memory.
 __device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems& E,
                                 const float* __restrict__ u) {
   // rows touched: (y+-1, z), (y, z+-1), (y+-1, z+-1)
@@ -69,26 +102,30 @@ __device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems
   const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
   const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
   const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
-  // U(dx,dy,dz) naming: first letter x (m/0/p), then y, then z
-  const float u_mm0 = u[rM0 + n.xm], u_pm0 = u[rM0 + n.xp];
-  const float u_mp0 = u[rP0 + n.xm], u_pp0 = u[rP0 + n.xp];
-  const float u_m0m = u[r0m + n.xm], u_p0m = u[r0m + n.xp];
-  const float u_m0p = u[r0p + n.xm], u_p0p = u[r0p + n.xp];
-  const float u_mmm = u[rMm + n.xm], u_0mm = u[rMm + n.x], u_pmm = u[rMm + n.xp];
-  const float u_mmp = u[rMp + n.xm], u_0mp = u[rMp + n.x], u_pmp = u[rMp + n.xp];
-  const float u_mpm = u[rPm + n.xm], u_0pm = u[rPm + n.x], u_ppm = u[rPm + n.xp];
-  const float u_mpp = u[rPp + n.xm], u_0pp = u[rPp + n.x], u_ppp = u[rPp + n.xp];
-  // element (ax,ay,az) lies on side s = 2a-1; far nodes (sx,sy,0) (sx,0,sz) (0,sy,sz) (sx,sy,sz)
-  float s = 0.f;
-  s += E.k[0] * (u_mm0 + u_m0m + u_0mm + u_mmm);
-  s += E.k[1] * (u_pm0 + u_p0m + u_0mm + u_pmm);
-  s += E.k[2] * (u_mp0 + u_m0m + u_0pm + u_mpm);
-  s += E.k[3] * (u_pp0 + u_p0m + u_0pm + u_ppm);
-  s += E.k[4] * (u_mm0 + u_m0p + u_0mp + u_mmp);
-  s += E.k[5] * (u_pm0 + u_p0p + u_0mp + u_pmp);
-  s += E.k[6] * (u_mp0 + u_m0p + u_0pp + u_mpp);
-  s += E.k[7] * (u_pp0 + u_p0p + u_0pp + u_ppp);
-  return s;
+  Far20 v;
+  v.mm0 = u[rM0 + n.xm]; v.pm0 = u[rM0 + n.xp];
+  v.mp0 = u[rP0 + n.xm]; v.pp0 = u[rP0 + n.xp];
+  v.m0m = u[r0m + n.xm]; v.p0m = u[r0m + n.xp];
+  v.m0p = u[r0p + n.xm]; v.p0p = u[r0p + n.xp];
+  v.mmm = u[rMm + n.xm]; v._0mm = u[rMm + n.x]; v.pmm = u[rMm + n.xp];
+  v.mmp = u[rMp + n.xm]; v._0mp = u[rMp + n.x]; v.pmp = u[rMp + n.xp];
+  v.mpm = u[rPm + n.xm]; v._0pm = u[rPm + n.x]; v.ppm = u[rPm + n.xp];
+  v.mpp = u[rPp + n.xm]; v._0pp = u[rPp + n.x]; v.ppp = u[rPp + n.xp];
+  return far_combine(E, v);
+}
+
+// The same from an LDS tile: node at U[c0], row strides DY (y) and DZ (z).
+__device__ inline float far_sum_tile(const NodeElems& E, const float* U, int c0, int DY, int DZ) {
+  Far20 v;
+  v.mm0 = U[c0 - 1 - DY]; v.pm0 = U[c0 + 1 - DY];
+  v.mp0 = U[c0 - 1 + DY]; v.pp0 = U[c0 + 1 + DY];
+  v.m0m = U[c0 - 1 - DZ]; v.p0m = U[c0 + 1 - DZ];
+  v.m0p = U[c0 - 1 + DZ]; v.p0p = U[c0 + 1 + DZ];
+  v.mmm = U[c0 - 1 - DY - DZ]; v._0mm = U[c0 - DY - DZ]; v.pmm = U[c0 + 1 - DY - DZ];
+  v.mmp = U[c0 - 1 - DY + DZ]; v._0mp = U[c0 - DY + DZ]; v.pmp = U[c0 + 1 - DY + DZ];
+  v.mpm = U[c0 - 1 + DY - DZ]; v._0pm = U[c0 + DY - DZ]; v.ppm = U[c0 + 1 + DY - DZ];
+  v.mpp = U[c0 - 1 + DY + DZ]; v._0pp = U[c0 + DY + DZ]; v.ppp = U[c0 + 1 + DY + DZ];
+  return far_combine(E, v);
 }
 
 // FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
@@ -133,9 +170,10 @@ __device__ inline void fine_gs_color_body(const Geom& g, const FineConsts& fc, c
   const int hy = g.ny >> 1, hz = g.nz >> 1;
   const long long cnt = (long long)g.nx * hy * hz;
   if (t >= cnt) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = 2 * (int)(r % hy) + cy, z = 2 * (int)(r / hy) + cz;
+  int x, y, z;
+  split_node(g.nx, hy, t, x, y, z);
+  y = 2 * y + cy;
+  z = 2 * z + cz;
   const NodeIdx n = node_idx(g, x, y, z);
   const NodeElems E = load_elems(g, n, kappa);
   const float inv = 1.f / (4.f * fc.c * E.ksum);
@@ -152,9 +190,8 @@ __device__ inline void fine_residual_body(const Geom& g, const FineConsts& fc, c
                                           const float* __restrict__ f, const float* __restrict__ u,
                                           float* __restrict__ res, long long t) {
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   const NodeIdx n = node_idx(g, x, y, z);
   const NodeElems E = load_elems(g, n, kappa);
 #pragma unroll
@@ -172,9 +209,8 @@ __device__ inline void fine_apply_dot_body(const Geom& g, const FineConsts& fc, 
   double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
   const BlockRange br = block_range_v(g.nn, lb, nb);
   for (long long t = br.b + threadIdx.x; t < br.e; t += blockDim.x) {
-    const int x = (int)(t % g.nx);
-    const long long r = t / g.nx;
-    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    int x, y, z;
+    split_node(g.nx, g.ny, t, x, y, z);
     const NodeIdx n = node_idx(g, x, y, z);
     const NodeElems E = load_elems(g, n, kappa);
 #pragma unroll
@@ -202,6 +238,9 @@ __device__ inline void fine_init_residual_body(const Geom& g, const FineConsts& 
   const double c = (double)fc.c;
   const BlockRange br = block_range_v(g.nn, lb, nb);
   for (long long t = br.b + threadIdx.x; t < br.e; t += blockDim.x) {
+    // 64-bit decode on purpose: this FP64-heavy loop measured twice as fast
+    // with it on gfx950 as with split_node (47 against 95 us per launch at
+    // 128^3), although the 32-bit decode is shorter code
     const int x = (int)(t % g.nx);
     const long long r = t / g.nx;
     const int y = (int)(r % g.ny), z = (int)(r / g.ny);
@@ -269,9 +308,8 @@ __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, co
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const BlockRange br = block_range_v(g.nn, lb, nb);
   for (long long t = br.b + threadIdx.x; t < br.e; t += blockDim.x) {
-    const int x = (int)(t % g.nx);
-    const long long r = t / g.nx;
-    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    int x, y, z;
+    split_node(g.nx, g.ny, t, x, y, z);
     float S[6];
     element_S(g, fc, chi, x, y, z, S);
     const double ke = (double)kappa[t];
@@ -286,9 +324,8 @@ __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, co
 __device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
                                              const float w[6], float* __restrict__ sens, int pitch, long long t) {
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   float S[6];
   element_S(g, fc, chi, x, y, z, S);
   const float s = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3] + S[4] * w[4] + S[5] * w[5];
@@ -299,82 +336,207 @@ __device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc
 __device__ inline void fine_energy6_body(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
                                          float* __restrict__ S6, long long t) {
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   float S[6];
   element_S(g, fc, chi, x, y, z, S);
 #pragma unroll
   for (int c = 0; c < 6; c++) S6[c * g.nn + t] = S[c];
 }
 
-// Fine-level brick smoother: a workgroup owns BX x 8 x 8 nodes and stages
-// the kappa of every element touching them plus u of the brick and a
-// one-node halo in LDS, then runs `sweeps` x-row Gauss-Seidel sweeps (4
-// colours) with workgroup barriers. Halo values stay frozen at their entry
-// values and the result goes to a separate buffer (block Gauss-Seidel across
-// bricks, exact colour order inside); the post-smoother runs the colours in
-// reverse. Each thread updates two nodes of the active colour per step.
+// ---- LDS tiles of the fine level ----------------------------------------------
+// A tile is BX x BY x BZ nodes; a workgroup of TR thread rows of BX threads
+// stages the conductivity of every element touching the tile and up to
+// three fields with a one-node halo. Staging reads whole x-rows of the tile
+// (coalesced) and the two halo columns separately, with no per-element
+// index division. The default shape, 32 x 4 x 8 nodes for 256 threads,
+// measured fastest for the smoother on gfx950 among the shapes tried
+// (32x8x8, 32x8x4, 32x4x8, 32x4x4, 32x2x8, 32x4x16, 64x4x4, 64x4x8,
+// 64x8x4, 64x8x8, 16x8x8, 16x4x8 at 128 to 512 threads): more resident
+// workgroups hide the staging latency better than the smaller halo share of
+// larger tiles saves.
+constexpr int kFineBY = 4, kFineBZ = 8, kFineTR = 8;
+
+template <int BX, int BY, int BZ, int TR>
+struct FineTile {
+  static constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;  // nodes with the halo
+  static constexpr int KX = BX + 1, KY = BY + 1, KZ = BZ + 1;  // elements touching the tile
+  static constexpr int NT = BX * TR;                           // threads
+  static constexpr int kU = HX * HY * HZ, kK = KX * KY * KZ;   // LDS floats per field / of kappa
+};
+
+// Tile width along x for a level (0: the level does not tile); tiles per level.
+__host__ __device__ inline int fine_tile_x(const Geom& g) {
+  if (g.ny % kFineBY || g.nz % kFineBZ) return 0;
+  if (g.nx % 32 == 0) return 32;
+  if (g.nx % 16 == 0) return 16;
+  return 0;
+}
+__host__ __device__ inline int fine_tiles(const Geom& g, int bx) {
+  return (g.nx / bx) * (g.ny / kFineBY) * (g.nz / kFineBZ);
+}
+// Workgroups of the tiled operator apply: one tile each up to the partial-sum
+// capacity of a reduction, several tiles each beyond.
+constexpr int kTileRedBlocks = 2048;
+__host__ __device__ inline unsigned fine_apply_tile_blocks(const Geom& g, int bx) {
+  const int t = fine_tiles(g, bx);
+  return (unsigned)(t > kTileRedBlocks ? kTileRedBlocks : t);
+}
+
+struct TileOrigin {
+  int ox, oy, oz;
+};
+template <int BX, int BY, int BZ>
+__device__ inline TileOrigin tile_origin(const Geom& g, int b) {
+  const int bx = g.nx / BX, by = g.ny / BY;
+  return {(b % bx) * BX, ((b / bx) % by) * BY, (b / (bx * by)) * BZ};
+}
+
+// kappa of element (ox-1+kx, oy-1+ky, oz-1+kz) at sk[kx + KX*(ky + KY*kz)]
+template <int BX, int BY, int BZ, int TR>
+__device__ inline void stage_tile_kappa(const Geom& g, const float* __restrict__ kappa, float* sk,
+                                        const TileOrigin& o) {
+  using T = FineTile<BX, BY, BZ, TR>;
+  const int t = threadIdx.x, tx = t % BX, ty = t / BX;
+  const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
+  const int xm = wrapm(o.ox - 1, g.nx);
+  for (int r = ty; r < T::KY * T::KZ; r += TR) {
+    const int ky = r % T::KY, kz = r / T::KY;
+    const long long row = nx * wrapm(o.oy - 1 + ky, g.ny) + nxy * wrapm(o.oz - 1 + kz, g.nz);
+    sk[r * T::KX + 1 + tx] = kappa[row + o.ox + tx];
+  }
+  for (int r = t; r < T::KY * T::KZ; r += T::NT) {
+    const int ky = r % T::KY, kz = r / T::KY;
+    const long long row = nx * wrapm(o.oy - 1 + ky, g.ny) + nxy * wrapm(o.oz - 1 + kz, g.nz);
+    sk[r * T::KX] = kappa[row + xm];
+  }
+}
+
+// field k (k < OTM_NRHS) of node (ox-1+hx, oy-1+hy, oz-1+hz) at
+// su[k*kU + hx + HX*(hy + HY*hz)]
+template <int BX, int BY, int BZ, int TR>
+__device__ inline void stage_tile_fields(const Geom& g, const float* __restrict__ src, float* su,
+                                         const TileOrigin& o) {
+  using T = FineTile<BX, BY, BZ, TR>;
+  const int t = threadIdx.x, tx = t % BX, ty = t / BX;
+  const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
+  const int xm = wrapm(o.ox - 1, g.nx), xp = wrapm(o.ox + BX, g.nx);
+  for (int r = ty; r < T::HY * T::HZ; r += TR) {
+    const int hy = r % T::HY, hz = r / T::HY;
+    const long long row = nx * wrapm(o.oy - 1 + hy, g.ny) + nxy * wrapm(o.oz - 1 + hz, g.nz);
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) su[k * T::kU + r * T::HX + 1 + tx] = src[k * g.nn + row + o.ox + tx];
+  }
+  for (int i = t; i < 2 * T::HY * T::HZ; i += T::NT) {
+    const int r = i >> 1, side = i & 1;
+    const int hy = r % T::HY, hz = r / T::HY;
+    const long long row = nx * wrapm(o.oy - 1 + hy, g.ny) + nxy * wrapm(o.oz - 1 + hz, g.nz);
+    const int hx = side ? T::HX - 1 : 0;
+    const long long w = row + (side ? xp : xm);
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) su[k * T::kU + r * T::HX + hx] = src[k * g.nn + w];
+  }
+}
+
+// q = K p on the tiles lb, lb + nb, ... and the block's partial sums of p.q
+// (slot lb). Each node's arithmetic is fine_apply_dot_body's.
+template <int BX, int TR>
+__device__ inline void fine_apply_tile_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
+                                            const float* __restrict__ p, float* __restrict__ q, const Reduce& R,
+                                            int lb, int nb) {
+  using T = FineTile<BX, kFineBY, kFineBZ, TR>;
+  __shared__ float su[OTM_NRHS * T::kU];
+  __shared__ float sk[T::kK];
+  const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+  const int tiles = fine_tiles(g, BX);
+  double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
+  for (int b = lb; b < tiles; b += nb) {
+    const TileOrigin o = tile_origin<BX, kFineBY, kFineBZ>(g, b);
+    stage_tile_kappa<BX, kFineBY, kFineBZ, TR>(g, kappa, sk, o);
+    stage_tile_fields<BX, kFineBY, kFineBZ, TR>(g, p, su, o);
+    __syncthreads();
+    for (int r = ty; r < kFineBY * kFineBZ; r += TR) {
+      const int ly = r % kFineBY, lz = r / kFineBY;
+      const NodeElems E = tile_elems(sk, tx + T::KX * (ly + T::KY * lz), T::KX, T::KY);
+      const int c0 = (tx + 1) + T::HX * ((ly + 1) + T::HY * (lz + 1));
+      const long long v = (long long)(o.ox + tx) + (long long)g.nx * ((long long)(o.oy + ly) + (long long)g.ny * (o.oz + lz));
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) {
+        const float* U = su + k * T::kU;
+        const float pv = U[c0];
+        const float kp = fc.c * (4.f * E.ksum * pv - far_sum_tile(E, U, c0, T::HX, T::HX * T::HY));
+        q[k * g.nn + v] = kp;
+        acc[k] += (double)pv * (double)kp;
+      }
+    }
+    __syncthreads();
+  }
+  reduce_block_at<OTM_NRHS>(acc, R, lb);
+}
+
+// res = f - K u on tile b; each node's arithmetic is fine_residual_body's.
+template <int BX, int TR>
+__device__ inline void fine_residual_tile_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
+                                               const float* __restrict__ f, const float* __restrict__ u,
+                                               float* __restrict__ res, int b) {
+  using T = FineTile<BX, kFineBY, kFineBZ, TR>;
+  __shared__ float su[OTM_NRHS * T::kU];
+  __shared__ float sk[T::kK];
+  const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
+  const TileOrigin o = tile_origin<BX, kFineBY, kFineBZ>(g, b);
+  stage_tile_kappa<BX, kFineBY, kFineBZ, TR>(g, kappa, sk, o);
+  stage_tile_fields<BX, kFineBY, kFineBZ, TR>(g, u, su, o);
+  __syncthreads();
+  for (int r = ty; r < kFineBY * kFineBZ; r += TR) {
+    const int ly = r % kFineBY, lz = r / kFineBY;
+    const NodeElems E = tile_elems(sk, tx + T::KX * (ly + T::KY * lz), T::KX, T::KY);
+    const int c0 = (tx + 1) + T::HX * ((ly + 1) + T::HY * (lz + 1));
+    const long long v = (long long)(o.ox + tx) + (long long)g.nx * ((long long)(o.oy + ly) + (long long)g.ny * (o.oz + lz));
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) {
+      const float* U = su + k * T::kU;
+      const float ku = fc.c * (4.f * E.ksum * U[c0] - far_sum_tile(E, U, c0, T::HX, T::HX * T::HY));
+      res[k * g.nn + v] = f[k * g.nn + v] - ku;
+    }
+  }
+}
+
+// Fine-level brick smoother on a tile: `sweeps` x-row Gauss-Seidel sweeps
+// (4 colours) in LDS with workgroup barriers. Halo values stay frozen at
+// their entry values and the result goes to a separate buffer (block
+// Gauss-Seidel across tiles, exact colour order inside); the post-smoother
+// runs the colours in reverse. Each thread updates (BY/2)(BZ/2)/TR nodes of
+// the active colour per step (one with the default tile and 256 threads).
 //
 // A node couples to its 12 face-diagonal neighbours with the summed
 // conductivity of the two elements they share and to its 8 body-diagonal
 // neighbours with one element's conductivity (one-axis neighbours have a
 // zero coefficient), so an update is 20 LDS reads and 20 FMAs per load case
-// plus 12 adds shared by the three. Staging reads whole x-rows of the brick
-// (coalesced) and the two halo columns separately, with no per-element
-// index division.
-// BX x BY x BZ nodes per brick, TR thread rows of BX threads (TR divides the
-// (BY/2)(BZ/2) x-rows of one colour).
-template <int BX, int BY = 8, int BZ = 8, int TR = 8>
+// plus 12 adds shared by the three.
+template <int BX, int BY = kFineBY, int BZ = kFineBZ, int TR = kFineTR>
 __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
                                        const float* __restrict__ f, const float* __restrict__ uin,
                                        float* __restrict__ uout, int sweeps, int forward, int zero_in, int b) {
-  constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
-  constexpr int KX = BX + 1, KY = BY + 1, KZ = BZ + 1;
-  constexpr int NT = BX * TR;
+  using T = FineTile<BX, BY, BZ, TR>;
+  constexpr int HX = T::HX, HY = T::HY, KX = T::KX, KY = T::KY;
   constexpr int R = (BY / 2) * (BZ / 2);  // x-rows of one colour
   constexpr int H = R / TR;               // of them per thread row
   static_assert(R % TR == 0, "thread rows must divide the rows of a colour");
-  __shared__ float su[OTM_NRHS][HZ * HY * HX];
-  __shared__ float sk[KZ * KY * KX];
-  const int bx = g.nx / BX, by = g.ny / BY;
-  const int ox = (b % bx) * BX, oy = ((b / bx) % by) * BY, oz = (b / (bx * by)) * BZ;
+  __shared__ float su[OTM_NRHS][T::kU];
+  __shared__ float sk[T::kK];
+  const TileOrigin o = tile_origin<BX, BY, BZ>(g, b);
+  const int ox = o.ox, oy = o.oy, oz = o.oz;
   const int t = threadIdx.x;
   const int tx = t % BX, ty = t / BX;  // ty in [0, TR)
   const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
-  const int xm = wrapm(ox - 1, g.nx), xp = wrapm(ox + BX, g.nx);
-  // kappa of element (ox-1+kx, oy-1+ky, oz-1+kz) at sk[kx + KX*(ky + KY*kz)]
-  for (int r = ty; r < KY * KZ; r += TR) {
-    const int ky = r % KY, kz = r / KY;
-    const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
-    sk[r * KX + 1 + tx] = kappa[row + ox + tx];
-  }
-  for (int r = t; r < KY * KZ; r += NT) {
-    const int ky = r % KY, kz = r / KY;
-    const long long row = nx * wrapm(oy - 1 + ky, g.ny) + nxy * wrapm(oz - 1 + kz, g.nz);
-    sk[r * KX] = kappa[row + xm];
-  }
+  stage_tile_kappa<BX, BY, BZ, TR>(g, kappa, sk, o);
   if (zero_in) {
-    for (int i = t; i < HX * HY * HZ; i += NT) {
+    for (int i = t; i < T::kU; i += T::NT) {
 #pragma unroll
       for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
     }
   } else {
-    for (int r = ty; r < HY * HZ; r += TR) {
-      const int hy = r % HY, hz = r / HY;
-      const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + 1 + tx] = uin[k * g.nn + row + ox + tx];
-    }
-    for (int i = t; i < 2 * HY * HZ; i += NT) {
-      const int r = i >> 1, side = i & 1;
-      const int hy = r % HY, hz = r / HY;
-      const long long row = nx * wrapm(oy - 1 + hy, g.ny) + nxy * wrapm(oz - 1 + hz, g.nz);
-      const int hx = side ? HX - 1 : 0;
-      const long long w = row + (side ? xp : xm);
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][r * HX + hx] = uin[k * g.nn + w];
-    }
+    stage_tile_fields<BX, BY, BZ, TR>(g, uin, &su[0][0], o);
   }
   // node ownership: x = tx; of colour (cy, cz) the x-rows rr = ty + TR*h,
   // y = cy + 2*(rr % (BY/2)), z = cz + 2*(rr / (BY/2))

@@ -21,7 +21,7 @@ void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, con
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
                              float* sens, int pitch);
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6);
-// Fine brick smoother (BX x 8 x 8 bricks, frozen halo, result in uout);
+// Fine brick smoother (BX x kFineBY x kFineBZ bricks, frozen halo, result in uout);
 // fine_brick_x returns BX (32 or 16) or 0 when the cell does not tile.
 int fine_brick_x(const Geom& g);
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,

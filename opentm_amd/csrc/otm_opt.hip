@@ -15,9 +15,8 @@ __global__ void __launch_bounds__(kBlock) k_filter_fwd(Geom g, Taps T, const flo
                                                        float* __restrict__ out, float* __restrict__ W) {
   const long long t = gtid();
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   float s = 0.f, ws = 0.f;
   for (int i = 0; i < T.n; i++) {
     int X = x + T.dx[i], Y = y + T.dy[i], Z = z + T.dz[i];
@@ -38,9 +37,8 @@ __global__ void __launch_bounds__(kBlock) k_filter_adj(Geom g, Taps T, const flo
                                                        const float* __restrict__ W, float* __restrict__ out) {
   const long long t = gtid();
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   float s = 0.f;
   for (int i = 0; i < T.n; i++) {
     int X = x - T.dx[i], Y = y - T.dy[i], Z = z - T.dz[i];
@@ -79,9 +77,8 @@ __global__ void __launch_bounds__(kBlock) k_oc_filter_sens(Geom g, Taps T, const
                                                            const float* __restrict__ rho, float* __restrict__ out) {
   const long long t = gtid();
   if (t >= g.nn) return;
-  const int x = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+  int x, y, z;
+  split_node(g.nx, g.ny, t, x, y, z);
   float s = 0.f, ws = 0.f;
   for (int i = 0; i < T.n; i++) {
     const int X = wrapm(x + T.dx[i], g.nx), Y = wrapm(y + T.dy[i], g.ny), Z = wrapm(z + T.dz[i], g.nz);
@@ -240,9 +237,8 @@ __device__ inline float tanproj(float v, float beta) {
 __global__ void __launch_bounds__(kBlock) k_init_density(Geom g, int kind, InitModes M, float* __restrict__ rho) {
   const long long t = gtid();
   if (t >= g.nn) return;
-  const int i = (int)(t % g.nx);
-  const long long r = t / g.nx;
-  const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+  int i, j, k;
+  split_node(g.nx, g.ny, t, i, j, k);
   const float pi = 3.1415926f;
   const float x = float(i) / g.nx, y = float(j) / g.ny, z = float(k) / g.nz;
   float v = 0.f;

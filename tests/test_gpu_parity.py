@@ -137,15 +137,19 @@ def test_heat_matrix(orc, n, seed):
     assert max(rr) < 1e-5
 
 
-def test_non_cubic_cell(orc):
-    """reso = (32, 24, 16): element side 1/reso[0] as in the reference
-    (homogenization.cpp:47); kappa^H and sensitivities against the oracle."""
-    say("non-cubic", (32, 24, 16))
+@pytest.mark.parametrize("reso", [(32, 24, 16), (32, 20, 24), (48, 12, 8)])
+def test_non_cubic_cell(orc, reso):
+    """Non-cubic cells, element side 1/reso[0] as in the reference
+    (homogenization.cpp:47); kappa^H and sensitivities against the oracle.
+    y-sizes 20 and 12 tile the 4-row fine bricks but not 8-row ones; 48 x 12 x 8
+    takes the 16-wide brick."""
+    say("non-cubic", reso)
+    nx, ny, nz = reso
     rng = np.random.default_rng(15)
-    k = rng.uniform(1e-3, 1.0, (16, 24, 32)).astype(np.float32)
-    cfg = otm.HomoConfig((32, 24, 16), (1.0, 1e-3))
+    k = rng.uniform(1e-3, 1.0, (nz, ny, nx)).astype(np.float32)
+    cfg = otm.HomoConfig(reso, (1.0, 1e-3))
     hom = otm.Homogenization_H(cfg)
-    assert hom.levels()[0] == (32, 24, 16)
+    assert hom.levels()[0] == reso
     hom.update(k)
     C = hom.heatMatrix()
     Co, _, So, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12, want_S=True)
