@@ -257,11 +257,11 @@ struct OpFinal {  // one block per cell
   static constexpr int kThreads = kBlock;
   int mode, nv;
   double rtol2;   // < 0: the cell's own
-  int apply = 0;  // partials of the operator apply (tiled cells have their own count)
+  int tiled = 0;  // partials of the tiled operator apply: its own count
   __device__ void operator()(const CellDev& C, const Ent&, int, int) const {
     Reduce R = cell_reduce(C, mode);
     if (rtol2 >= 0.0) R.rtol2 = rtol2;
-    const unsigned nb = (apply && C.tile_bx) ? fine_apply_tile_blocks(C.g[0], C.tile_bx)
+    const unsigned nb = (tiled && C.tile_bx) ? fine_apply_tile_blocks(C.g[0], C.tile_bx)
                                               : red_blocks_for_dev(C.g[0].nn);
     reduce_final_body(R, nv, (int)nb);
   }
@@ -621,7 +621,8 @@ void flush(otm_batch* b) {
 
 long long nodes0(const Member& m) { return m.c->g0.nn; }
 long long redb(const Member& m) { return red_blocks_for(m.c->g0.nn); }
-// blocks of the operator apply and of the fine residual (LDS tiles when the cell tiles)
+// blocks of the operator apply and of the fine residual (LDS tiles when the
+// cell tiles)
 long long applyb(const Member& m) {
   const int bx = fine_brick_x(m.c->g0);
   return bx ? (long long)fine_apply_tile_blocks(m.c->g0, bx) : (long long)red_blocks_for(m.c->g0.nn);
@@ -786,6 +787,8 @@ void step(otm_batch* b) {
     for (int k = 0; k < 3; k++) worst = std::max(worst, P.rr[k] / P.bb[k]);
     if (!any_active(P) || m->it >= m->c->cfg.max_cg_iter || (m->restart > 0 && worst > 0.25 * m->prev_worst) ||
         m->restart >= max_restarts) {
+      // the true residual was just evaluated at the final iterate
+      for (int k = 0; k < 3; k++) m->relres[k] = sqrt(P.rr[k] / P.bb[k]);
       m->phase = Member::FINISH;
       continue;
     }
@@ -832,31 +835,18 @@ void step(otm_batch* b) {
   b->steps++;
 }
 
-// True residual, mean removal and kappa^H for the members that finished.
+// Mean removal and kappa^H for the members that finished. A member finishes
+// right after its restart phase evaluated the true residual at its final
+// iterate (pending refinement corrections are added before that
+// evaluation), so its relative residuals are already recorded.
 void finish(otm_batch* b) {
-  std::vector<Member*> D, D32, D64, Dadd;
+  std::vector<Member*> D, D32, D64;
   for (Member& m : b->members)
     if (m.phase == Member::FINISH) {
       D.push_back(&m);
       (m.refine ? D64 : D32).push_back(&m);
-      if (m.refine && m.pending) Dadd.push_back(&m);
     }
   if (D.empty()) return;
-  add_op(b, Dadd, OpRefineAdd{}, [](const Member& m) { return (long long)std::min<long long>(8192, nblk(3 * m.c->g0.nn)); });
-  for (Member* m : Dadd) m->pending = false;
-  add_op(b, D32, OpInitResid<float>{-1.0}, redb);
-  add_op(b, D64, OpInitResid<double>{-1.0}, redb);
-  add_op(b, D, OpFinal{FIN_BB_RR, 3 * OTM_NRHS, -1.0}, one);
-  read_back(b, D);
-  for (Member* m : D) {
-    const PcgScalars& P = b->hread[m->slot].pcg;
-    for (int k = 0; k < 3; k++) m->relres[k] = sqrt(P.rr[k] / P.bb[k]);
-    if (P.fault & 1) {
-      m->phase = Member::FAILED;
-      m->fail_code = OTM_E_NUMERIC;
-      m->fail_msg = "non-finite residual (MG numerical explode)";
-    }
-  }
   add_op(b, D32, OpSum3<float>{}, redb);
   add_op(b, D64, OpSum3<double>{}, redb);
   add_op(b, D, OpFinal{FIN_MEAN, OTM_NRHS, -1.0}, one);

@@ -596,7 +596,9 @@ __device__ inline void tail_stencil(const StRef& S, long long nn, long long v, f
 }
 
 // `sweeps` exact 8-colour Gauss-Seidel sweeps on u (LDS) of one tail level
-// with its stencil sa[o*nn + v] in LDS.
+// with its stencil sa[o*nn + v] in LDS. Work items are (load case, node) of
+// the active colour, so a colour step of a small level keeps three times as
+// many lanes busy as one thread per node would.
 __device__ inline void tail_smooth(const Geom& g, const float* sa, const float* __restrict__ f, float* u, int sweeps,
                                    bool forward) {
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
@@ -606,28 +608,23 @@ __device__ inline void tail_smooth(const Geom& g, const float* sa, const float* 
     for (int i = 0; i < 8; i++) {
       const int col = forward ? i : 7 - i;
       const int cx = col & 1, cy = (col >> 1) & 1, cz = col >> 2;
-      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      for (int q = threadIdx.x; q < OTM_NRHS * cnt; q += blockDim.x) {
+        const int k = q / cnt, t = q - k * cnt;
         const int x = 2 * (t % hx) + cx, r = t / hx;
         const int y = 2 * (r % hy) + cy, z = 2 * (r / hy) + cz;
         const int v = x + g.nx * (y + g.ny * z);
         const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
         const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
         const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
-        float a[27];
+        const float inv = 1.f / sa[13 * nn + v];
+        const float* uk = u + k * nn;
+        float acc = 0.f;
 #pragma unroll
-        for (int o = 0; o < 27; o++) a[o] = sa[o * nn + v];
-        const float inv = 1.f / a[13];
-#pragma unroll
-        for (int k = 0; k < OTM_NRHS; k++) {
-          const float* uk = u + k * nn;
-          float acc = 0.f;
-#pragma unroll
-          for (int o = 0; o < 27; o++) {
-            if (o == 13) continue;
-            acc += a[o] * uk[xs[o % 3] + g.nx * (ys[(o / 3) % 3] + g.ny * zs[o / 9])];
-          }
-          u[k * nn + v] = (f[k * nn + v] - acc) * inv;
+        for (int o = 0; o < 27; o++) {
+          if (o == 13) continue;
+          acc += sa[o * nn + v] * uk[xs[o % 3] + g.nx * (ys[(o / 3) % 3] + g.ny * zs[o / 9])];
         }
+        u[k * nn + v] = (f[k * nn + v] - acc) * inv;
       }
       __syncthreads();
     }

@@ -129,6 +129,22 @@ __device__ inline float far_sum_tile(const NodeElems& E, const float* U, int c0,
 }
 
 // FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
+struct Far20d {
+  double mm0, pm0, mp0, pp0, m0m, p0m, m0p, p0p;
+  double mmm, _0mm, pmm, mmp, _0mp, pmp, mpm, _0pm, ppm, mpp, _0pp, ppp;
+};
+__device__ inline double far_combine_d(const NodeElems& E, const Far20d& u) {
+  double s = 0.0;
+  s += (double)E.k[0] * (u.mm0 + u.m0m + u._0mm + u.mmm);
+  s += (double)E.k[1] * (u.pm0 + u.p0m + u._0mm + u.pmm);
+  s += (double)E.k[2] * (u.mp0 + u.m0m + u._0pm + u.mpm);
+  s += (double)E.k[3] * (u.pp0 + u.p0m + u._0pm + u.ppm);
+  s += (double)E.k[4] * (u.mm0 + u.m0p + u._0mp + u.mmp);
+  s += (double)E.k[5] * (u.pm0 + u.p0p + u._0mp + u.pmp);
+  s += (double)E.k[6] * (u.mp0 + u.m0p + u._0pp + u.mpp);
+  s += (double)E.k[7] * (u.pp0 + u.p0p + u._0pp + u.ppp);
+  return s;
+}
 template <typename XT>
 __device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeElems& E,
                                    const XT* __restrict__ u) {
@@ -136,26 +152,17 @@ __device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeEl
   const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
   const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
   const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
-  const double u_mm0 = u[rM0 + n.xm], u_pm0 = u[rM0 + n.xp];
-  const double u_mp0 = u[rP0 + n.xm], u_pp0 = u[rP0 + n.xp];
-  const double u_m0m = u[r0m + n.xm], u_p0m = u[r0m + n.xp];
-  const double u_m0p = u[r0p + n.xm], u_p0p = u[r0p + n.xp];
-  const double u_mmm = u[rMm + n.xm], u_0mm = u[rMm + n.x], u_pmm = u[rMm + n.xp];
-  const double u_mmp = u[rMp + n.xm], u_0mp = u[rMp + n.x], u_pmp = u[rMp + n.xp];
-  const double u_mpm = u[rPm + n.xm], u_0pm = u[rPm + n.x], u_ppm = u[rPm + n.xp];
-  const double u_mpp = u[rPp + n.xm], u_0pp = u[rPp + n.x], u_ppp = u[rPp + n.xp];
-  double s = 0.0;
-  s += (double)E.k[0] * (u_mm0 + u_m0m + u_0mm + u_mmm);
-  s += (double)E.k[1] * (u_pm0 + u_p0m + u_0mm + u_pmm);
-  s += (double)E.k[2] * (u_mp0 + u_m0m + u_0pm + u_mpm);
-  s += (double)E.k[3] * (u_pp0 + u_p0m + u_0pm + u_ppm);
-  s += (double)E.k[4] * (u_mm0 + u_m0p + u_0mp + u_mmp);
-  s += (double)E.k[5] * (u_pm0 + u_p0p + u_0mp + u_pmp);
-  s += (double)E.k[6] * (u_mp0 + u_m0p + u_0pp + u_mpp);
-  s += (double)E.k[7] * (u_pp0 + u_p0p + u_0pp + u_ppp);
-  return s;
+  Far20d v;
+  v.mm0 = u[rM0 + n.xm]; v.pm0 = u[rM0 + n.xp];
+  v.mp0 = u[rP0 + n.xm]; v.pp0 = u[rP0 + n.xp];
+  v.m0m = u[r0m + n.xm]; v.p0m = u[r0m + n.xp];
+  v.m0p = u[r0p + n.xm]; v.p0p = u[r0p + n.xp];
+  v.mmm = u[rMm + n.xm]; v._0mm = u[rMm + n.x]; v.pmm = u[rMm + n.xp];
+  v.mmp = u[rMp + n.xm]; v._0mp = u[rMp + n.x]; v.pmp = u[rMp + n.xp];
+  v.mpm = u[rPm + n.xm]; v._0pm = u[rPm + n.x]; v.ppm = u[rPm + n.xp];
+  v.mpp = u[rPp + n.xm]; v._0pp = u[rPp + n.x]; v.ppp = u[rPp + n.xp];
+  return far_combine_d(E, v);
 }
-
 // Load vector of case k at the node (grid.cu:1282-1342): sum_e kappa_e feMu[7-e][k].
 __device__ inline float load_vec(const NodeElems& E, const FineConsts& fc, int k) {
   float s = 0.f;

@@ -156,23 +156,45 @@ __device__ inline float oc_rule_log(float s, float ls, float rho, float lg, floa
   return v;
 }
 
+// The bisection's volume sums factor the rule's power: (-s/g)^damp =
+// 2^(damp log2(-s)) * 2^(-damp log2 g), one exp2 per element and one per
+// candidate instead of one per (element, candidate). The final update
+// (k_oc_apply) evaluates the rule unfactored with the chosen multiplier.
+__device__ inline float oc_rule_fac(float s, float e, float kc, float rho, float c01, float min_rho, float step) {
+  float v = (s > 0.f ? c01 : e * kc) * rho;
+  if (v - rho < -step) v = rho - step;
+  if (v - rho > step) v = rho + step;
+  if (v < min_rho) v = min_rho;
+  if (v > 1.f) v = 1.f;
+  return v;
+}
+
+// Per-thread sums stay in FP32 over the thread's few elements (each term is
+// at most 1) and go to FP64 for the block and grid stages.
 __global__ void __launch_bounds__(kBlock) k_oc_tree_sum(long long n, const float* __restrict__ sens,
                                                         const float* __restrict__ rho,
                                                         const OcState* __restrict__ st, float min_rho, float step,
                                                         float damp, float c01, int max_steps, Reduce R) {
   if (st->done || st->steps >= max_steps) return;
-  __shared__ float slg[kOcCand];
-  if (threadIdx.x < kOcCand) slg[threadIdx.x] = __builtin_amdgcn_logf(oc_tree_mid(st, threadIdx.x));
+  __shared__ float sk[kOcCand];
+  if (threadIdx.x < kOcCand)
+    sk[threadIdx.x] = __builtin_amdgcn_exp2f(-damp * __builtin_amdgcn_logf(oc_tree_mid(st, threadIdx.x)));
   __syncthreads();
-  double acc[kOcCand];
+  float kc[kOcCand], accf[kOcCand];
 #pragma unroll
-  for (int c = 0; c < kOcCand; c++) acc[c] = 0.0;
+  for (int c = 0; c < kOcCand; c++) {
+    kc[c] = sk[c];
+    accf[c] = 0.f;
+  }
   for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x) {
     const float sv = sens[i], r = rho[i];
-    const float ls = __builtin_amdgcn_logf(-sv);
+    const float e = __builtin_amdgcn_exp2f(damp * __builtin_amdgcn_logf(-sv));
 #pragma unroll
-    for (int c = 0; c < kOcCand; c++) acc[c] += (double)oc_rule_log(sv, ls, r, slg[c], c01, min_rho, step, damp);
+    for (int c = 0; c < kOcCand; c++) accf[c] += oc_rule_fac(sv, e, kc[c], r, c01, min_rho, step);
   }
+  double acc[kOcCand];
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) acc[c] = (double)accf[c];
   reduce_block<kOcCand>(acc, R);
 }
 
