@@ -91,6 +91,7 @@ def main():
                     help="Gauss-Seidel sweeps per smoothing step on the coarse levels (fine level: 2)")
     ap.add_argument("--min-coarse", type=int, default=None,
                     help="coarsest level edge floor (otm_config.min_coarse); default: the library's")
+    ap.add_argument("--verbose", action="store_true", help="library prints per-solve PCG progress")
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel class with HIP events (adds event overhead to the step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -135,6 +136,7 @@ def main():
     cfg.coarse_sweeps = args.coarse_sweeps
     if args.min_coarse is not None:
         cfg.min_coarse = args.min_coarse
+    cfg.verbose = 1 if args.verbose else 0
     hom = otm.Homogenization_H(cfg)
     hom.set_design(wl.oc_start_density(n, args.seed))
     oc = otm.OCOptimizer(n ** 3, 0.001, 0.2, 0.5, bisect_steps=60, bisect_tol=1e-6)

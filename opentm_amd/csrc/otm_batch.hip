@@ -53,6 +53,7 @@ struct CellDev {
   FineConsts fc;
   const float* kappa;
   float *x, *r, *p, *q;
+  float* xlo;  // low part of chi as an FP32 pair (plain cells)
   double* x64;
   float* ua[kMaxLevels];  // correction buffer pair per level (level 0: z, z2)
   float* ub[kMaxLevels];
@@ -102,7 +103,7 @@ __device__ inline Reduce cell_reduce(const CellDev& C, int mode) {
   R.pcg = C.pcg;
   R.rtol2 = C.rtol2;
   R.inv_nn = C.inv_nn;
-  R.inner2 = 0.0;
+  R.inner2 = C.inner2;    // used by FIN_BB_RR only, as in the single-cell solve
   R.mode = mode;
   return R;
 }
@@ -245,11 +246,10 @@ struct OpInitResid {
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int nb) const {
     Reduce R = cell_reduce(C, FIN_BB_RR);
     if (rtol2 >= 0.0) R.rtol2 = rtol2;
-    R.inner2 = C.inner2;
     const XT* x;
     if constexpr (sizeof(XT) == 8) x = C.x64;
     else x = C.x;
-    fine_init_residual_body<XT>(C.g[0], C.fc, C.kappa, x, C.r, R, lb, nb);
+    fine_init_residual_body<XT>(C.g[0], C.fc, C.kappa, x, sizeof(XT) == 4 ? C.xlo : nullptr, C.r, R, lb, nb);
   }
 };
 
@@ -286,7 +286,7 @@ struct OpApplyDot {
 struct OpXr {
   static constexpr int kThreads = kBlock;
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int nb) const {
-    pcg_xr_body(C.g[0].nn, C.x, C.r, C.p, C.q, C.pcg, cell_reduce(C, FIN_RR), lb, nb);
+    pcg_xr_body(C.g[0].nn, C.x, C.xlo, C.r, C.p, C.q, C.pcg, cell_reduce(C, FIN_RR), lb, nb);
   }
 };
 
@@ -536,6 +536,7 @@ CellDev describe(const Member& m) {
   d.fc = c->fc;
   d.kappa = c->kappa;
   d.x = c->x;
+  d.xlo = c->xlo;
   d.r = c->r;
   d.p = c->p;
   d.q = c->q;
@@ -876,6 +877,7 @@ struct OpZeroX {
   static constexpr int kThreads = kBlock;
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int nb) const {
     fill_body(C.x, 3 * C.g[0].nn, 0.f, lb, nb);
+    if (C.xlo) fill_body(C.xlo, 3 * C.g[0].nn, 0.f, lb, nb);
     if (C.x64) {
       const BlockRange br = block_range_v(3 * C.g[0].nn, lb, nb);
       for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) C.x64[i] = 0.0;

@@ -107,6 +107,7 @@ struct otm_ctx {
   float* kappa = nullptr;  // per element, with the 1e-9 floor
   float* x = nullptr;      // chi (warm start); the FP32 correction while refining
   double* x64 = nullptr;   // FP64 chi of the iterative refinement (fp64_refine)
+  float* xlo = nullptr;    // low part of chi held as an FP32 pair (x + xlo) when not refining
   float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float* z2 = nullptr;  // second fine correction buffer (fine brick smoother)
   // coarsest dense solve

@@ -148,7 +148,7 @@ static void ctx_free(otm_ctx* c) {
     dfree(L.scale);
     if (L.own_uf) dfree(L.u2);
   }
-  dfree(c->kappa); dfree(c->x); dfree(c->x64); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
+  dfree(c->kappa); dfree(c->x); dfree(c->x64); dfree(c->xlo); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
   dfree(c->cB); dfree(c->cBinv);
   dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
@@ -257,6 +257,9 @@ static void build_hierarchy(otm_ctx* c) {
   if (cfg.fp64_refine) {
     c->x64 = dalloc<double>(3 * n0);
     launch_fill_d(c->s, c->x64, 3 * n0, 0.0);
+  } else {
+    c->xlo = dalloc<float>(3 * n0);
+    launch_fill(c->s, c->xlo, 3 * n0, 0.f);
   }
   hip_check(hipGetLastError(), "initial fill");
   const double h = 1.0 / cfg.reso[0];  // element side (homogenization.cpp:47 passes h/2 as half-length)
@@ -422,8 +425,8 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
         R.inner2 = kRefineInner * kRefineInner;
         launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, R, nb);
       } else {
-        launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, c->reducer(FIN_BB_RR, rtol2, inv_n),
-                                  nb);
+        launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->xlo, c->r,
+                                  c->reducer(FIN_BB_RR, rtol2, inv_n), nb);
       }
     }
     c->read_pcg();
@@ -438,6 +441,8 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
       const double q = c->pcg_host->rr[k] / c->pcg_host->bb[k];
       worst = q > worst ? q : worst;
     }
+    if (c->cfg.verbose)
+      fprintf(stderr, "[otm] restart %d after %d iterations: true relres %.3e\n", restart, it, sqrt(worst));
     if (restart > 0 && worst > 0.25 * prev_worst) break;
     *fresh = false;
     prev_worst = worst;
@@ -460,7 +465,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
       }
       {
         ProfScope ps(c, OTM_PROF_VECTOR, 72.0 * (double)n);
-        launch_pcg_xr(c->s, n, c->x, c->r, c->p, c->q, c->pcg, c->reducer(FIN_RR, rtol2), nb);
+        launch_pcg_xr(c->s, n, c->x, c->xlo, c->r, c->p, c->q, c->pcg, c->reducer(FIN_RR, rtol2), nb);
       }
       it++;
       c->read_pcg();
@@ -519,7 +524,7 @@ static void pcg_solve(otm_ctx* c) {
     if (c->x64)
       launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, Rf, nb);
     else
-      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->r, Rf, nb);
+      launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->xlo, c->r, Rf, nb);
     c->read_pcg();
   }
   double relres[3];
@@ -799,6 +804,7 @@ OTM_API int otm_set_chi(otm_ctx* ctx, const float* chi, int where) {
     hip_check(hipMemcpyAsync(ctx->x, chi, sizeof(float) * 3 * ctx->g0.nn, kind_of(OTM_DEVICE, where), ctx->s),
               "copy chi");
     if (ctx->x64) launch_f2d(ctx->s, 3 * ctx->g0.nn, ctx->x, ctx->x64);
+    if (ctx->xlo) launch_fill(ctx->s, ctx->xlo, 3 * ctx->g0.nn, 0.f);
     ctx->sync();
   });
 }
@@ -809,6 +815,7 @@ OTM_API int otm_reset_displacement(otm_ctx* ctx) {
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     launch_fill(ctx->s, ctx->x, 3 * ctx->g0.nn, 0.f);
     if (ctx->x64) launch_fill_d(ctx->s, ctx->x64, 3 * ctx->g0.nn, 0.0);
+    if (ctx->xlo) launch_fill(ctx->s, ctx->xlo, 3 * ctx->g0.nn, 0.f);
     ctx->sync();
   });
 }

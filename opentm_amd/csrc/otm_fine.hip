@@ -38,9 +38,9 @@ __global__ void __launch_bounds__(kBlock) k_fine_apply_tile(Geom g, FineConsts f
 
 template <typename XT>
 __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
-                                                               const XT* __restrict__ x_, float* __restrict__ rvec,
-                                                               Reduce R) {
-  fine_init_residual_body<XT>(g, fc, kappa, x_, rvec, R, xcd_block(), gridDim.x);
+                                                               const XT* __restrict__ x_, const float* __restrict__ xlo,
+                                                               float* __restrict__ rvec, Reduce R) {
+  fine_init_residual_body<XT>(g, fc, kappa, x_, xlo, rvec, R, xcd_block(), gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
@@ -125,14 +125,15 @@ void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, c
 // The FP64 residual stays pointwise: an LDS-tiled form (180 VGPRs, two
 // waves per SIMD) measured 25% slower than this one (three waves per SIMD).
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
-                               const float* x, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual<float>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
+                               const float* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks) {
+  hipLaunchKernelGGL(k_fine_init_residual<float>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, xlo, r, R);
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const double* x, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual<double>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, r, R);
+  hipLaunchKernelGGL(k_fine_init_residual<double>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x,
+                     (const float*)nullptr, r, R);
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 

@@ -163,6 +163,26 @@ __device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeEl
   v.mpp = u[rPp + n.xm]; v._0pp = u[rPp + n.x]; v.ppp = u[rPp + n.xp];
   return far_combine_d(E, v);
 }
+// The same for an iterate held as an FP32 pair (hi + lo).
+__device__ inline double far_sum_pair(const Geom& g, const NodeIdx& n, const NodeElems& E,
+                                      const float* __restrict__ u, const float* __restrict__ ul) {
+  const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
+  const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
+  const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
+  const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
+  auto at = [&](long long i) { return (double)u[i] + (double)ul[i]; };
+  Far20d v;
+  v.mm0 = at(rM0 + n.xm); v.pm0 = at(rM0 + n.xp);
+  v.mp0 = at(rP0 + n.xm); v.pp0 = at(rP0 + n.xp);
+  v.m0m = at(r0m + n.xm); v.p0m = at(r0m + n.xp);
+  v.m0p = at(r0p + n.xm); v.p0p = at(r0p + n.xp);
+  v.mmm = at(rMm + n.xm); v._0mm = at(rMm + n.x); v.pmm = at(rMm + n.xp);
+  v.mmp = at(rMp + n.xm); v._0mp = at(rMp + n.x); v.pmp = at(rMp + n.xp);
+  v.mpm = at(rPm + n.xm); v._0pm = at(rPm + n.x); v.ppm = at(rPm + n.xp);
+  v.mpp = at(rPp + n.xm); v._0pp = at(rPp + n.x); v.ppp = at(rPp + n.xp);
+  return far_combine_d(E, v);
+}
+
 // Load vector of case k at the node (grid.cu:1282-1342): sum_e kappa_e feMu[7-e][k].
 __device__ inline float load_vec(const NodeElems& E, const FineConsts& fc, int k) {
   float s = 0.f;
@@ -236,11 +256,11 @@ __device__ inline void fine_apply_dot_body(const Geom& g, const FineConsts& fc, 
 // sum of the eight element contributions cancels exactly wherever kappa
 // does not vary along the load direction); stored rounded to FP32. Partial
 // sums of b.b, r.r and sum(r) (PCG start and restarts). XT is the iterate's
-// type: FP32, or FP64 for the iterative refinement.
+// type: FP32 (with xlo: an FP32 pair), or FP64 for the iterative refinement.
 template <typename XT>
 __device__ inline void fine_init_residual_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
-                                               const XT* __restrict__ x_, float* __restrict__ rvec, const Reduce& R,
-                                               int lb, int nb) {
+                                               const XT* __restrict__ x_, const float* __restrict__ xlo,
+                                               float* __restrict__ rvec, const Reduce& R, int lb, int nb) {
   double acc[3 * OTM_NRHS] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const double c = (double)fc.c;
   const BlockRange br = block_range_v(g.nn, lb, nb);
@@ -262,7 +282,17 @@ __device__ inline void fine_init_residual_body(const Geom& g, const FineConsts& 
       double b = 0.0;
 #pragma unroll
       for (int e = 0; e < 8; e++) b += (double)E.k[e] * (double)fc.femu[7 - e][k];
-      const double kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
+      double kx;
+      if constexpr (sizeof(XT) == 4) {
+        if (xlo) {
+          const float* xl = xlo + k * g.nn;
+          kx = c * (4.0 * ksum * ((double)xk[n.v] + (double)xl[n.v]) - far_sum_pair(g, n, E, xk, xl));
+        } else {
+          kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
+        }
+      } else {
+        kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
+      }
       const double rv = b - kx;
       rvec[k * g.nn + n.v] = (float)rv;
       acc[k] += b * b;

@@ -12,8 +12,9 @@ void launch_fine_residual(hipStream_t s, const Geom& g, const FineConsts& fc, co
                           const float* u, float* res);
 void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* p,
                            float* q, const Reduce& R, unsigned nblocks);
+// x as an FP32 pair when xlo is given (nullptr: x alone)
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
-                               const float* x, float* r, const Reduce& R, unsigned nblocks);
+                               const float* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks);
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const double* x, float* r, const Reduce& R, unsigned nblocks);
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
@@ -65,7 +66,7 @@ void launch_refine_add(hipStream_t s, long long n, double* x64, float* d);
 void launch_sub_mean_cast(hipStream_t s, long long nn, double* x64, float* x32, const PcgScalars* P);
 void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P);
 void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
-void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,
+void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* xlo, float* r, const float* p, const float* q,
                    const PcgScalars* P, const Reduce& R, unsigned nblocks);
 void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P);
 void launch_copy_pitched(hipStream_t s, const Geom& g, const float* src, int pitch, float* dst, float add);

@@ -44,10 +44,11 @@ __global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, co
   sub_mean3_body(nn, a, P, xcd_block(), gridDim.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restrict__ x, float* __restrict__ r,
-                                                   const float* __restrict__ p, const float* __restrict__ q,
-                                                   const PcgScalars* __restrict__ P, Reduce R) {
-  pcg_xr_body(nn, x, r, p, q, P, R, xcd_block(), gridDim.x);
+__global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restrict__ x, float* __restrict__ xlo,
+                                                   float* __restrict__ r, const float* __restrict__ p,
+                                                   const float* __restrict__ q, const PcgScalars* __restrict__ P,
+                                                   Reduce R) {
+  pcg_xr_body(nn, x, xlo, r, p, q, P, R, xcd_block(), gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restrict__ p, const float* __restrict__ z,
@@ -132,9 +133,9 @@ void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P
   if (nb > 4096) nb = 4096;
   hipLaunchKernelGGL(k_sub_mean3, dim3(nb), dim3(kBlock), 0, s, nn, a, P);
 }
-void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* r, const float* p, const float* q,
+void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* xlo, float* r, const float* p, const float* q,
                    const PcgScalars* P, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_pcg_xr, dim3(nblocks), dim3(kBlock), 0, s, nn, x, r, p, q, P, R);
+  hipLaunchKernelGGL(k_pcg_xr, dim3(nblocks), dim3(kBlock), 0, s, nn, x, xlo, r, p, q, P, R);
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P) {

@@ -72,8 +72,26 @@ __device__ inline void sub_mean3_body(long long nn, float* a, const PcgScalars* 
 }
 
 // x += alpha p, r -= alpha q, partial r.r
-__device__ inline void pcg_xr_body(long long nn, float* __restrict__ x, float* __restrict__ r,
-                                   const float* __restrict__ p, const float* __restrict__ q,
+// x + a*p into the unevaluated FP32 pair (hi, lo): the rounding errors of
+// the product (exact through fma) and of the sum (TwoSum) go to lo, and the
+// pair is renormalised, so the iterate keeps about twice FP32's precision.
+// Explicitly rounded operations keep the compiler from contracting them.
+__device__ inline void pair_add_prod(float& hi, float& lo, float a, float p) {
+  const float b = __fmul_rn(a, p);
+  const float eb = fmaf(a, p, -b);
+  const float s = __fadd_rn(hi, b);
+  const float bv = __fsub_rn(s, hi);
+  const float es = __fadd_rn(__fsub_rn(hi, __fsub_rn(s, bv)), __fsub_rn(b, bv));
+  const float l = __fadd_rn(lo, __fadd_rn(es, eb));
+  const float h = __fadd_rn(s, l);
+  lo = __fsub_rn(l, __fsub_rn(h, s));
+  hi = h;
+}
+
+// x += alpha p (x as an FP32 pair when xlo is given), r -= alpha q, and the
+// partial sums of r.r.
+__device__ inline void pcg_xr_body(long long nn, float* __restrict__ x, float* __restrict__ xlo,
+                                   float* __restrict__ r, const float* __restrict__ p, const float* __restrict__ q,
                                    const PcgScalars* __restrict__ P, const Reduce& R, int lb, int nb) {
   float al[OTM_NRHS];
 #pragma unroll
@@ -84,7 +102,14 @@ __device__ inline void pcg_xr_body(long long nn, float* __restrict__ x, float* _
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const long long j = k * nn + i;
-      x[j] += al[k] * p[j];
+      if (xlo) {
+        float hi = x[j], lo = xlo[j];
+        pair_add_prod(hi, lo, al[k], p[j]);
+        x[j] = hi;
+        xlo[j] = lo;
+      } else {
+        x[j] += al[k] * p[j];
+      }
       const float rv = r[j] - al[k] * q[j];
       r[j] = rv;
       acc[k] += (double)rv * (double)rv;
