@@ -115,8 +115,9 @@ __device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems
 }
 
 // The same from an LDS tile: node at U[c0], row strides DY (y) and DZ (z).
-__device__ inline float far_sum_tile(const NodeElems& E, const float* U, int c0, int DY, int DZ) {
-  Far20 v;
+template <typename F20>
+__device__ inline F20 tile_far20(const float* U, int c0, int DY, int DZ) {
+  F20 v;
   v.mm0 = U[c0 - 1 - DY]; v.pm0 = U[c0 + 1 - DY];
   v.mp0 = U[c0 - 1 + DY]; v.pp0 = U[c0 + 1 + DY];
   v.m0m = U[c0 - 1 - DZ]; v.p0m = U[c0 + 1 - DZ];
@@ -125,7 +126,10 @@ __device__ inline float far_sum_tile(const NodeElems& E, const float* U, int c0,
   v.mmp = U[c0 - 1 - DY + DZ]; v._0mp = U[c0 - DY + DZ]; v.pmp = U[c0 + 1 - DY + DZ];
   v.mpm = U[c0 - 1 + DY - DZ]; v._0pm = U[c0 + DY - DZ]; v.ppm = U[c0 + 1 + DY - DZ];
   v.mpp = U[c0 - 1 + DY + DZ]; v._0pp = U[c0 + DY + DZ]; v.ppp = U[c0 + 1 + DY + DZ];
-  return far_combine(E, v);
+  return v;
+}
+__device__ inline float far_sum_tile(const NodeElems& E, const float* U, int c0, int DY, int DZ) {
+  return far_combine(E, tile_far20<Far20>(U, c0, DY, DZ));
 }
 
 // FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
@@ -240,11 +244,14 @@ __device__ inline void fine_apply_dot_body(const Geom& g, const FineConsts& fc, 
     split_node(g.nx, g.ny, t, x, y, z);
     const NodeIdx n = node_idx(g, x, y, z);
     const NodeElems E = load_elems(g, n, kappa);
+    double ksd = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ksd += (double)E.k[e];
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const float* pk = p + k * g.nn;
       const float pv = pk[n.v];
-      const float kp = fc.c * (4.f * E.ksum * pv - far_sum(g, n, E, pk));
+      const float kp = (float)((double)fc.c * (4.0 * ksd * (double)pv - far_sum_d(g, n, E, pk)));
       q[k * g.nn + n.v] = kp;
       acc[k] += (double)pv * (double)kp;
     }
@@ -476,7 +483,10 @@ __device__ inline void stage_tile_fields(const Geom& g, const float* __restrict_
 }
 
 // q = K p on the tiles lb, lb + nb, ... and the block's partial sums of p.q
-// (slot lb). Each node's arithmetic is fine_apply_dot_body's.
+// (slot lb). Each node's arithmetic is fine_apply_dot_body's: the stencil
+// sum is accumulated in FP64 and rounded once, because the difference of
+// the diagonal and neighbour terms cancels for smooth p, and an FP32 sum
+// would let the PCG residual recurrence drift from the true residual.
 template <int BX, int TR>
 __device__ inline void fine_apply_tile_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
                                             const float* __restrict__ p, float* __restrict__ q, const Reduce& R,
@@ -497,11 +507,15 @@ __device__ inline void fine_apply_tile_body(const Geom& g, const FineConsts& fc,
       const NodeElems E = tile_elems(sk, tx + T::KX * (ly + T::KY * lz), T::KX, T::KY);
       const int c0 = (tx + 1) + T::HX * ((ly + 1) + T::HY * (lz + 1));
       const long long v = (long long)(o.ox + tx) + (long long)g.nx * ((long long)(o.oy + ly) + (long long)g.ny * (o.oz + lz));
+      double ksd = 0.0;
+#pragma unroll
+      for (int e = 0; e < 8; e++) ksd += (double)E.k[e];
 #pragma unroll
       for (int k = 0; k < OTM_NRHS; k++) {
         const float* U = su + k * T::kU;
         const float pv = U[c0];
-        const float kp = fc.c * (4.f * E.ksum * pv - far_sum_tile(E, U, c0, T::HX, T::HX * T::HY));
+        const double far = far_combine_d(E, tile_far20<Far20d>(U, c0, T::HX, T::HX * T::HY));
+        const float kp = (float)((double)fc.c * (4.0 * ksd * (double)pv - far));
         q[k * g.nn + v] = kp;
         acc[k] += (double)pv * (double)kp;
       }
