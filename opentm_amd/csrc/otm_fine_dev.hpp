@@ -94,9 +94,8 @@ __device__ inline float far_combine(const NodeElems& E, const Far20& u) {
   return s;
 }
 
-// sum_e kappa_e T_e(u) for one right-hand side, neighbours from global memory.
-__device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems& E,
-                                const float* __restrict__ u) {
+// The 20 neighbour values of node n from global memory.
+__device__ inline Far20 load_far20(const Geom& g, const NodeIdx& n, const float* __restrict__ u) {
   // rows touched: (y+-1, z), (y, z+-1), (y+-1, z+-1)
   const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
   const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
@@ -111,7 +110,13 @@ __device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems
   v.mmp = u[rMp + n.xm]; v._0mp = u[rMp + n.x]; v.pmp = u[rMp + n.xp];
   v.mpm = u[rPm + n.xm]; v._0pm = u[rPm + n.x]; v.ppm = u[rPm + n.xp];
   v.mpp = u[rPp + n.xm]; v._0pp = u[rPp + n.x]; v.ppp = u[rPp + n.xp];
-  return far_combine(E, v);
+  return v;
+}
+
+// sum_e kappa_e T_e(u) for one right-hand side, neighbours from global memory.
+__device__ inline float far_sum(const Geom& g, const NodeIdx& n, const NodeElems& E,
+                                const float* __restrict__ u) {
+  return far_combine(E, load_far20(g, n, u));
 }
 
 // The same from an LDS tile: node at U[c0], row strides DY (y) and DZ (z).
@@ -130,6 +135,20 @@ __device__ inline F20 tile_far20(const float* U, int c0, int DY, int DZ) {
 }
 __device__ inline float far_sum_tile(const NodeElems& E, const float* U, int c0, int DY, int DZ) {
   return far_combine(E, tile_far20<Far20>(U, c0, DY, DZ));
+}
+
+// (K u)_v in difference form: the rows of K sum to zero, so
+// (K u)_v = c sum_e kappa_e sum_{j far in e} (u_v - u_j). For smooth u the
+// differences are small and exact (Sterbenz), so the FP32 result keeps FP32
+// precision relative to itself; the direct form c(4 ksum u_v - far) loses
+// it to cancellation.
+__device__ inline float apply_diff(const NodeElems& E, float c, float uv, Far20 v) {
+  v.mm0 = uv - v.mm0; v.pm0 = uv - v.pm0; v.mp0 = uv - v.mp0; v.pp0 = uv - v.pp0;
+  v.m0m = uv - v.m0m; v.p0m = uv - v.p0m; v.m0p = uv - v.m0p; v.p0p = uv - v.p0p;
+  v.mmm = uv - v.mmm; v._0mm = uv - v._0mm; v.pmm = uv - v.pmm; v.mmp = uv - v.mmp;
+  v._0mp = uv - v._0mp; v.pmp = uv - v.pmp; v.mpm = uv - v.mpm; v._0pm = uv - v._0pm;
+  v.ppm = uv - v.ppm; v.mpp = uv - v.mpp; v._0pp = uv - v._0pp; v.ppp = uv - v.ppp;
+  return c * far_combine(E, v);
 }
 
 // FP64 variant of far_sum for residuals that must resolve below FP32 rounding.
@@ -244,14 +263,11 @@ __device__ inline void fine_apply_dot_body(const Geom& g, const FineConsts& fc, 
     split_node(g.nx, g.ny, t, x, y, z);
     const NodeIdx n = node_idx(g, x, y, z);
     const NodeElems E = load_elems(g, n, kappa);
-    double ksd = 0.0;
-#pragma unroll
-    for (int e = 0; e < 8; e++) ksd += (double)E.k[e];
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const float* pk = p + k * g.nn;
       const float pv = pk[n.v];
-      const float kp = (float)((double)fc.c * (4.0 * ksd * (double)pv - far_sum_d(g, n, E, pk)));
+      const float kp = apply_diff(E, fc.c, pv, load_far20(g, n, pk));
       q[k * g.nn + n.v] = kp;
       acc[k] += (double)pv * (double)kp;
     }
@@ -483,10 +499,9 @@ __device__ inline void stage_tile_fields(const Geom& g, const float* __restrict_
 }
 
 // q = K p on the tiles lb, lb + nb, ... and the block's partial sums of p.q
-// (slot lb). Each node's arithmetic is fine_apply_dot_body's: the stencil
-// sum is accumulated in FP64 and rounded once, because the difference of
-// the diagonal and neighbour terms cancels for smooth p, and an FP32 sum
-// would let the PCG residual recurrence drift from the true residual.
+// (slot lb). Each node's arithmetic is fine_apply_dot_body's: the
+// difference form (apply_diff), because with the direct form the PCG
+// residual recurrence drifted from the true residual.
 template <int BX, int TR>
 __device__ inline void fine_apply_tile_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
                                             const float* __restrict__ p, float* __restrict__ q, const Reduce& R,
@@ -507,15 +522,11 @@ __device__ inline void fine_apply_tile_body(const Geom& g, const FineConsts& fc,
       const NodeElems E = tile_elems(sk, tx + T::KX * (ly + T::KY * lz), T::KX, T::KY);
       const int c0 = (tx + 1) + T::HX * ((ly + 1) + T::HY * (lz + 1));
       const long long v = (long long)(o.ox + tx) + (long long)g.nx * ((long long)(o.oy + ly) + (long long)g.ny * (o.oz + lz));
-      double ksd = 0.0;
-#pragma unroll
-      for (int e = 0; e < 8; e++) ksd += (double)E.k[e];
 #pragma unroll
       for (int k = 0; k < OTM_NRHS; k++) {
         const float* U = su + k * T::kU;
         const float pv = U[c0];
-        const double far = far_combine_d(E, tile_far20<Far20d>(U, c0, T::HX, T::HX * T::HY));
-        const float kp = (float)((double)fc.c * (4.0 * ksd * (double)pv - far));
+        const float kp = apply_diff(E, fc.c, pv, tile_far20<Far20>(U, c0, T::HX, T::HX * T::HY));
         q[k * g.nn + v] = kp;
         acc[k] += (double)pv * (double)kp;
       }
