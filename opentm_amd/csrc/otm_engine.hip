@@ -127,6 +127,7 @@ struct otm_oc {
   hipStream_t s = nullptr;
   float *newsens = nullptr, *a = nullptr, *b = nullptr;
   unsigned* maxbits = nullptr;
+  unsigned* counter = nullptr;  // finished-block counter of the one-launch bisection pass
   OcState* st = nullptr;
   OcState* st_host = nullptr;  // pinned
   double* partials = nullptr;
@@ -1133,7 +1134,7 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
 static void oc_free(otm_oc* o) {
   if (!o) return;
   if (o->s) (void)hipStreamSynchronize(o->s);
-  dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits);
+  dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits); dfree(o->counter);
   dfree(o->st); dfree(o->partials);
   if (o->st_host) (void)hipHostFree(o->st_host);
   if (o->s) (void)hipStreamDestroy(o->s);
@@ -1157,6 +1158,8 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
       o->a = dalloc<float>(ne);
       o->b = dalloc<float>(ne);
       o->maxbits = dalloc<unsigned>(1);
+      o->counter = dalloc<unsigned>(1);
+      hip_check(hipMemsetAsync(o->counter, 0, sizeof(unsigned), o->s), "clear counter");
       o->st = dalloc<OcState>(1);
       o->partials = dalloc<double>((size_t)1024 * kOcCand);
       hip_check(hipHostMalloc((void**)&o->st_host, sizeof(OcState), hipHostMallocDefault), "hipHostMalloc");
@@ -1231,7 +1234,7 @@ static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volra
     const int m = std::min(per_check, passes - done);
     for (int i = 0; i < m; i++)
       launch_oc_tree_pass(o->s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, volratio,
-                          o->vol_tol, o->max_steps, R, nb);
+                          o->vol_tol, o->max_steps, R, nb, o->counter);
     done += m;
     if (done >= passes) break;
     hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");

@@ -169,13 +169,62 @@ __device__ inline float oc_rule_fac(float s, float e, float kc, float rho, float
   return v;
 }
 
+// Adds one element's updated density at every candidate multiplier. With
+// the element's factors er = 2^(damp log2(-s)) rho and its clamp interval
+// [max(rho - step, min_rho), min(rho + step, 1)], each candidate costs a
+// multiply, a median-of-three and an add; an element with s > 0 takes the
+// multiplier-free value c01 rho (clamped) at every candidate (interval of
+// one point).
+__device__ inline void oc_tree_add(float sv, float r, float damp, float c01, float min_rho, float step,
+                                   const float (&kc)[kOcCand], float (&accf)[kOcCand]) {
+  float er = 0.f, lo, hi;
+  if (sv > 0.f) {
+    lo = hi = oc_rule_fac(sv, 0.f, 0.f, r, c01, min_rho, step);
+  } else {
+    er = __builtin_amdgcn_exp2f(damp * __builtin_amdgcn_logf(-sv)) * r;
+    lo = fmaxf(r - step, min_rho);
+    hi = fminf(r + step, 1.f);
+  }
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) accf[c] += __builtin_amdgcn_fmed3f(er * kc[c], lo, hi);
+}
+
+// Block stage of the candidate sums. The threads' FP32 sums go through LDS
+// (rows padded to 33 words: conflict-free writes and reads), thread t adds
+// candidate t % 32 over the 32 threads of group t / 32 in FP64, and the
+// eight group sums are added in group order. Wave 0 stores the block's
+// partials at the block's range index, so the grid stage adds them in item
+// order whatever the launch shape. (A shuffle tree per candidate costs six
+// cross-lane rounds for each of the 31 sums.)
+__device__ inline void oc_block_partials(const float (&accf)[kOcCand], const Reduce& R) {
+  constexpr int kRow = 33, kGroups = kBlock / 32;
+  __shared__ float rows[kBlock * kRow];
+  __shared__ double part[kGroups][32];
+#pragma unroll
+  for (int c = 0; c < kOcCand; c++) rows[threadIdx.x * kRow + c] = accf[c];
+  __syncthreads();
+  const int c = threadIdx.x & 31, j = threadIdx.x >> 5;
+  double a = 0.0;
+  if (c < kOcCand) {
+#pragma unroll 8
+    for (int k = 0; k < 32; k++) a += (double)rows[(j * 32 + k) * kRow + c];
+  }
+  part[j][c] = a;
+  __syncthreads();
+  if (threadIdx.x < kOcCand) {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < kGroups; q++) sum += part[q][threadIdx.x];
+    R.partials[(size_t)xcd_block() * kOcCand + threadIdx.x] = sum;
+  }
+}
+
 // Per-thread sums stay in FP32 over the thread's few elements (each term is
-// at most 1) and go to FP64 for the block and grid stages.
-__global__ void __launch_bounds__(kBlock) k_oc_tree_sum(long long n, const float* __restrict__ sens,
-                                                        const float* __restrict__ rho,
-                                                        const OcState* __restrict__ st, float min_rho, float step,
-                                                        float damp, float c01, int max_steps, Reduce R) {
-  if (st->done || st->steps >= max_steps) return;
+// at most 1) and go to FP64 for the block and grid stages. Four elements
+// per thread are loaded together so that their latencies overlap.
+__device__ inline void oc_tree_sum_body(long long n, const float* __restrict__ sens, const float* __restrict__ rho,
+                                        const OcState* __restrict__ st, float min_rho, float step, float damp,
+                                        float c01, const Reduce& R) {
   __shared__ float sk[kOcCand];
   if (threadIdx.x < kOcCand)
     sk[threadIdx.x] = __builtin_amdgcn_exp2f(-damp * __builtin_amdgcn_logf(oc_tree_mid(st, threadIdx.x)));
@@ -186,37 +235,71 @@ __global__ void __launch_bounds__(kBlock) k_oc_tree_sum(long long n, const float
     kc[c] = sk[c];
     accf[c] = 0.f;
   }
-  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x) {
-    const float sv = sens[i], r = rho[i];
-    const float e = __builtin_amdgcn_exp2f(damp * __builtin_amdgcn_logf(-sv));
+  const BlockRange rg = block_range(n);
+  long long i = rg.b + threadIdx.x;
+  for (; i + 3 * kBlock < rg.e; i += 4 * kBlock) {
+    float sv[4], r[4];
 #pragma unroll
-    for (int c = 0; c < kOcCand; c++) accf[c] += oc_rule_fac(sv, e, kc[c], r, c01, min_rho, step);
+    for (int k = 0; k < 4; k++) {
+      sv[k] = sens[i + k * kBlock];
+      r[k] = rho[i + k * kBlock];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) oc_tree_add(sv[k], r[k], damp, c01, min_rho, step, kc, accf);
   }
-  double acc[kOcCand];
-#pragma unroll
-  for (int c = 0; c < kOcCand; c++) acc[c] = (double)accf[c];
-  reduce_block<kOcCand>(acc, R);
+  for (; i < rg.e; i += kBlock) oc_tree_add(sens[i], rho[i], damp, c01, min_rho, step, kc, accf);
+  oc_block_partials(accf, R);
 }
 
-__global__ void __launch_bounds__(kBlock) k_oc_tree_final(OcState* st, Reduce R, int nblocks, double inv_n,
-                                                          double volratio, double tol, int max_steps) {
+__global__ void __launch_bounds__(kBlock) k_oc_tree_sum(long long n, const float* __restrict__ sens,
+                                                        const float* __restrict__ rho,
+                                                        const OcState* __restrict__ st, float min_rho, float step,
+                                                        float damp, float c01, int max_steps, Reduce R) {
   if (st->done || st->steps >= max_steps) return;
-  __shared__ double sh[kOcCand * kWaves];
+  oc_tree_sum_body(n, sens, rho, st, min_rho, step, damp, c01, R);
+}
+
+// Sums the partials of the kOcCand midpoints and walks the bisection tree
+// with them (thread 0): the state's bracket, multiplier and step count
+// advance by up to kOcTree levels. Thread t sums candidate t % 32 over the
+// blocks t / 32, t / 32 + 8, ... (coalesced reads, eight in flight), and the
+// eight groups are added in group order: a fixed order for every launch.
+__device__ inline void oc_tree_final_body(OcState* st, const Reduce& R, int nblocks, double inv_n, double volratio,
+                                          double tol, int max_steps) {
+  constexpr int kGroups = kBlock / 32;
+  __shared__ double part[kGroups][32];
+  __shared__ double vol_sum[kOcCand];
   __shared__ float g[kOcCand];
+  const int c = threadIdx.x & 31, j = threadIdx.x >> 5;
   if (threadIdx.x < kOcCand) g[threadIdx.x] = oc_tree_mid(st, threadIdx.x);
-  double acc[kOcCand];
+  double a = 0.0;
+  if (c < kOcCand) {
+    const double* src = R.partials + c;
+    int b = j;
+    for (; b + 7 * kGroups < nblocks; b += 8 * kGroups) {
+      double v[8];
 #pragma unroll
-  for (int c = 0; c < kOcCand; c++) acc[c] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+      for (int k = 0; k < 8; k++) v[k] = src[(size_t)(b + k * kGroups) * kOcCand];
 #pragma unroll
-    for (int c = 0; c < kOcCand; c++) acc[c] += R.partials[(size_t)b * kOcCand + c];
-  block_sum<kOcCand>(acc, sh);  // barrier inside also publishes g
+      for (int k = 0; k < 8; k++) a += v[k];
+    }
+    for (; b < nblocks; b += kGroups) a += src[(size_t)b * kOcCand];
+  }
+  part[j][c] = a;
+  __syncthreads();
+  if (threadIdx.x < kOcCand) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kGroups; q++) s += part[q][threadIdx.x];
+    vol_sum[threadIdx.x] = s;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   float lo = st->lo, hi = st->hi;
   int node = 0, steps = st->steps;
   for (int lev = 0; lev < kOcTree && steps < max_steps; lev++) {
     const float gm = g[node];
-    const double vol = acc[node] * inv_n;
+    const double vol = vol_sum[node] * inv_n;
     st->g = gm;
     st->vol = vol;
     steps++;
@@ -234,6 +317,35 @@ __global__ void __launch_bounds__(kBlock) k_oc_tree_final(OcState* st, Reduce R,
   st->lo = lo;
   st->hi = hi;
   st->steps = steps;
+}
+
+__global__ void __launch_bounds__(kBlock) k_oc_tree_final(OcState* st, Reduce R, int nblocks, double inv_n,
+                                                          double volratio, double tol, int max_steps) {
+  if (st->done || st->steps >= max_steps) return;
+  oc_tree_final_body(st, R, nblocks, inv_n, volratio, tol, max_steps);
+}
+
+// One pass in one launch: the tree sum's blocks only read the design and
+// write their partials, so the last block to finish can publish-and-count
+// cheaply (an agent-scope release writes back little) and run the final
+// stage itself, with the same code and order as k_oc_tree_final.
+__global__ void __launch_bounds__(kBlock) k_oc_tree_pass(long long n, const float* __restrict__ sens,
+                                                         const float* __restrict__ rho, OcState* st, float min_rho,
+                                                         float step, float damp, float c01, int max_steps,
+                                                         Reduce R, unsigned* counter, double inv_n, double volratio,
+                                                         double tol) {
+  if (st->done || st->steps >= max_steps) return;
+  oc_tree_sum_body(n, sens, rho, st, min_rho, step, damp, c01, R);
+  // the partials were stored by wave 0, the wave whose thread 0 releases them
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    const unsigned done = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = done + 1 == gridDim.x;
+  }
+  __syncthreads();
+  if (!last) return;
+  oc_tree_final_body(st, R, (int)gridDim.x, inv_n, volratio, tol, max_steps);
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // rho <- update rule at the final multiplier (in place).
@@ -338,8 +450,13 @@ void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int
 }
 void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const float* rho, OcState* st,
                          float min_rho, float step, float damp, double volratio, double vol_tol, int max_steps,
-                         const Reduce& R, unsigned nblocks) {
+                         const Reduce& R, unsigned nblocks, unsigned* counter) {
   const float c01 = powf(0.01f, damp);
+  if (counter) {
+    hipLaunchKernelGGL(k_oc_tree_pass, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01,
+                       max_steps, R, counter, 1.0 / (double)n, volratio, vol_tol);
+    return;
+  }
   hipLaunchKernelGGL(k_oc_tree_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01,
                      max_steps, R);
   hipLaunchKernelGGL(k_oc_tree_final, dim3(1), dim3(kBlock), 0, s, st, R, (int)nblocks, 1.0 / (double)n, volratio,
