@@ -118,6 +118,10 @@ struct otm_ctx {
   PcgScalars* pcg = nullptr;
   PcgScalars* pcg_host = nullptr;  // pinned
   double* out_host = nullptr;      // pinned, 8 doubles
+  HostMirror* hm = nullptr;        // coherent pinned block the finalisers publish to
+  HostMirror* hm_dev = nullptr;    // its device address
+  unsigned hm_tag = 0;             // tag of the last finaliser asked to publish
+  bool hm_pending = false;         // that finaliser's results not yet taken
   // design pipeline
   float *rho = nullptr, *rho_f = nullptr, *W = nullptr, *kraw = nullptr, *dkappa = nullptr;
   float *sens = nullptr, *tmp = nullptr, *drho = nullptr;
@@ -125,6 +129,7 @@ struct otm_ctx {
   int taps_kind = -2;
   double taps_r = 0;
   int taps_periodic = -1;
+  bool w_valid = false;  // W holds the weight sums of these taps (they depend on the taps alone)
   // live kernel timing (otm_profile_enable)
   bool prof_on = false;
   int prof_mask = 0;
@@ -161,6 +166,35 @@ struct otm_ctx {
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (!pending.empty()) collect();
   }
+  // A reducer whose finaliser also publishes its results to the host block
+  // (taken by wait_host); one such reduction in flight at a time.
+  Reduce reducer_host(int mode, double rtol2 = 0.0, double inv_nn = 0.0) {
+    Reduce R = reducer(mode, rtol2, inv_nn);
+    R.hm = hm_dev;
+    R.tag = ++hm_tag;
+    hm_pending = true;
+    return R;
+  }
+  // Waits until the last publishing finaliser has run, polling its tag; the
+  // stream keeps running the work queued after it. A stream that drains or
+  // fails without the tag is an error (e.g. a kernel fault).
+  void wait_host() {
+    unsigned spins = 0;
+    while (__atomic_load_n(&hm->seq, __ATOMIC_ACQUIRE) != hm_tag) {
+      if ((++spins & 255u) == 0) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) {
+          if (__atomic_load_n(&hm->seq, __ATOMIC_ACQUIRE) == hm_tag) break;
+          fail(OTM_E_HIP, "reduction results were not published");
+        }
+        if (e != hipErrorNotReady) hip_check(e, "hipStreamQuery");
+      }
+      __builtin_ia32_pause();
+    }
+    hm_pending = false;
+    // timing events are read at stream synchronisations; bound their number
+    if (pending.size() > 512) sync();
+  }
   hipEvent_t take_event() {
     if (ev_pool.empty()) {
       hipEvent_t e;
@@ -183,7 +217,14 @@ struct otm_ctx {
     }
     pending.clear();
   }
+  // PCG scalars after the last reduction: from the host block when that
+  // reduction published them, else by a copy and a synchronisation.
   void read_pcg() {
+    if (hm_pending) {
+      wait_host();
+      *pcg_host = hm->pcg;
+      return;
+    }
     hip_check(hipMemcpyAsync(pcg_host, pcg, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), "read pcg scalars");
     sync();
   }

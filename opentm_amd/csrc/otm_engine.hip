@@ -154,6 +154,7 @@ static void ctx_free(otm_ctx* c) {
   dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
   if (c->out_host) (void)hipHostFree(c->out_host);
+  if (c->hm) (void)hipHostFree(c->hm);
   dfree(c->rho); dfree(c->rho_f); dfree(c->W); dfree(c->kraw); dfree(c->dkappa);
   dfree(c->sens); dfree(c->tmp); dfree(c->drho);
   for (auto& p : c->pending) {
@@ -253,6 +254,9 @@ static void build_hierarchy(otm_ctx* c) {
   hip_check(hipMemsetAsync(c->pcg, 0, sizeof(PcgScalars), c->s), "hipMemset pcg");
   hip_check(hipHostMalloc((void**)&c->pcg_host, sizeof(PcgScalars), hipHostMallocDefault), "hipHostMalloc");
   hip_check(hipHostMalloc((void**)&c->out_host, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipHostMalloc((void**)&c->hm, sizeof(HostMirror), hipHostMallocCoherent), "hipHostMalloc");
+  memset(c->hm, 0, sizeof(HostMirror));
+  hip_check(hipHostGetDevicePointer((void**)&c->hm_dev, c->hm, 0), "hipHostGetDevicePointer");
   launch_fill(c->s, c->x, 3 * n0, 0.f);
   launch_fill(c->s, c->p, 3 * n0, 0.f);
   if (cfg.fp64_refine) {
@@ -422,12 +426,12 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
         else
           launch_fill(c->s, c->x, 3 * n, 0.f);
         pending = false;
-        Reduce R = c->reducer(FIN_BB_RR, rtol2, inv_n);
+        Reduce R = c->reducer_host(FIN_BB_RR, rtol2, inv_n);
         R.inner2 = kRefineInner * kRefineInner;
         launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, R, nb);
       } else {
         launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->xlo, c->r,
-                                  c->reducer(FIN_BB_RR, rtol2, inv_n), nb);
+                                  c->reducer_host(FIN_BB_RR, rtol2, inv_n), nb);
       }
     }
     c->read_pcg();
@@ -466,7 +470,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
       }
       {
         ProfScope ps(c, OTM_PROF_VECTOR, 72.0 * (double)n);
-        launch_pcg_xr(c->s, n, c->x, c->xlo, c->r, c->p, c->q, c->pcg, c->reducer(FIN_RR, rtol2), nb);
+        launch_pcg_xr(c->s, n, c->x, c->xlo, c->r, c->p, c->q, c->pcg, c->reducer_host(FIN_RR, rtol2), nb);
       }
       it++;
       c->read_pcg();
@@ -521,7 +525,7 @@ static void pcg_solve(otm_ctx* c) {
   // true residual of the final iterate (already at hand when the restart
   // loop ended right after evaluating it)
   if (!fresh) {
-    const Reduce Rf = c->reducer(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n);
+    const Reduce Rf = c->reducer_host(FIN_BB_RR, c->cfg.fem_rel_thres * c->cfg.fem_rel_thres, 1.0 / (double)n);
     if (c->x64)
       launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, Rf, nb);
     else
@@ -532,10 +536,10 @@ static void pcg_solve(otm_ctx* c) {
   for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
   // remove the constant component (translateForce, grid.cpp:654-664)
   if (c->x64) {
-    launch_sum3(c->s, n, c->x64, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sum3(c->s, n, c->x64, c->reducer_host(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
     launch_sub_mean_cast(c->s, n, c->x64, c->x, c->pcg);
   } else {
-    launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sum3(c->s, n, c->x, c->reducer_host(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
     launch_sub_mean3(c->s, n, c->x, c->pcg);
   }
   c->read_pcg();
@@ -553,11 +557,10 @@ static void kappa_h(otm_ctx* c, double C[9]) {
   const unsigned nb = red_blocks_for(c->g0.nn);
   {
     ProfScope ps(c, OTM_PROF_ENERGY, 16.0 * (double)c->g0.nn);
-    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->x, c->reducer(FIN_STORE), nb);
+    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->x, c->reducer_host(FIN_STORE), nb);
   }
-  hip_check(hipMemcpyAsync(c->out_host, c->red_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->s), "read C");
-  c->sync();
-  const double* s = c->out_host;
+  c->wait_host();
+  const double* s = c->hm->out;
   // order (00,01,02,11,12,22); |Y| = 1 (homogenization.cu:207)
   C[0] = s[0]; C[1] = s[1]; C[2] = s[2];
   C[3] = s[1]; C[4] = s[3]; C[5] = s[4];
@@ -1094,8 +1097,10 @@ OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
         ctx->taps_kind = p->filter_kind;
         ctx->taps_r = p->filter_radius;
         ctx->taps_periodic = p->filter_periodic;
+        ctx->w_valid = false;
       }
-      launch_filter_fwd(ctx->s, ctx->g0, ctx->taps, ctx->rho, ctx->rho_f, ctx->W);
+      launch_filter_fwd(ctx->s, ctx->g0, ctx->taps, ctx->rho, ctx->rho_f, ctx->W, ctx->w_valid);
+      ctx->w_valid = true;
       src = ctx->rho_f;
     }
     launch_simp_fwd(ctx->s, n, src, (float)ctx->cfg.heat_ratio[0], (float)ctx->cfg.heat_ratio[1], 0.f, ctx->kraw,

@@ -452,50 +452,176 @@ __device__ inline TileOrigin tile_origin(const Geom& g, int b) {
   return {(b % bx) * BX, ((b / bx) % by) * BY, (b / (bx * by)) * BZ};
 }
 
-// kappa of element (ox-1+kx, oy-1+ky, oz-1+kz) at sk[kx + KX*(ky + KY*kz)]
+// Staging runs in two phases: every global load of a tile goes into
+// registers before the first LDS store, so a workgroup waits for HBM once
+// per tile instead of once per loop trip (the loops' trip counts are fixed,
+// so they unroll with a predicate on the last trip).
+//
+// Address arithmetic: a wave holds 64 / BX thread rows, so the x-runs it
+// loads in one trip are a few rows known to the whole wave. Their offsets
+// (periodic wrap, row and plane strides) are computed once per wave on the
+// scalar unit and each lane picks its row's; loads then take a wave-uniform
+// base and a 32-bit byte offset (a lattice has at most 2^30 nodes). Done
+// per lane, this arithmetic was about 40% of the smoother's vector
+// instructions.
 template <int BX, int BY, int BZ, int TR>
-__device__ inline void stage_tile_kappa(const Geom& g, const float* __restrict__ kappa, float* sk,
-                                        const TileOrigin& o) {
+struct TileStage {
+  using T = FineTile<BX, BY, BZ, TR>;
+  static constexpr int WR = 64 / BX;                   // thread rows per wave
+  static constexpr int KR = T::KY * T::KZ;             // kappa rows (x-runs of KX elements)
+  static constexpr int KJ = (KR + TR - 1) / TR;        // interior runs per thread
+  static constexpr int KH = (KR + T::NT - 1) / T::NT;  // left-column elements per thread
+  static constexpr int UR = T::HY * T::HZ;             // field rows (x-runs of HX nodes)
+  static constexpr int UJ = (UR + TR - 1) / TR;
+  static constexpr int UH = (2 * UR + T::NT - 1) / T::NT;  // halo-column nodes per thread
+  float k[KJ], kh[KH];
+  float u[OTM_NRHS][UJ], uh[OTM_NRHS][UH];
+};
+
+// value at byte offset `off` from a wave-uniform base
+__device__ inline float ld_off(const float* __restrict__ base, unsigned off) {
+  return *(const float*)((const char*)base + off);
+}
+// first thread row of this wave (wave-uniform) and the lane's row within it
+template <int BX>
+__device__ inline int wave_first_row() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * (64 / BX);
+}
+template <int BX>
+__device__ inline int lane_row() {
+  return (int)(threadIdx.x & 63) / BX;
+}
+// node offset of row (y, z) of a cell, wrapped periodically
+__device__ inline unsigned wrapped_row(const Geom& g, int y, int z) {
+  return (unsigned)g.nx * (unsigned)wrapm(y, g.ny) + (unsigned)g.nx * (unsigned)g.ny * (unsigned)wrapm(z, g.nz);
+}
+// Byte offset of the x-run (row r of an H-wide y range starting at y0,
+// z0) of this lane's thread row: the wave's rows are evaluated as uniform
+// values and the lane selects its own. Rows past `rows` are clamped.
+template <int BX, int H>
+__device__ inline unsigned lane_run_offset(const Geom& g, int rb, int rows, int y0, int z0) {
+  unsigned off = 0u;
+  const int sub = lane_row<BX>();
+#pragma unroll
+  for (int q = 0; q < 64 / BX; q++) {
+    int r = rb + q;
+    r = r < rows ? r : rows - 1;
+    const unsigned o = wrapped_row(g, y0 + r % H, z0 + r / H);
+    off = sub == q ? o : off;
+  }
+  return 4u * off;
+}
+
+// kappa of element (ox-1+kx, oy-1+ky, oz-1+kz), for sk[kx + KX*(ky + KY*kz)]
+template <int BX, int BY, int BZ, int TR>
+__device__ inline void load_tile_kappa(const Geom& g, const float* __restrict__ kappa, const TileOrigin& o,
+                                       TileStage<BX, BY, BZ, TR>& st) {
+  using S = TileStage<BX, BY, BZ, TR>;
+  using T = FineTile<BX, BY, BZ, TR>;
+  const int t = threadIdx.x, tx = t % BX;
+  const int ty = wave_first_row<BX>() + lane_row<BX>();
+  const int wr0 = wave_first_row<BX>();
+#pragma unroll
+  for (int j = 0; j < S::KJ; j++) {
+    const int rb = wr0 + TR * j;  // wave-uniform
+    const unsigned off = lane_run_offset<BX, T::KY>(g, rb, S::KR, o.oy - 1, o.oz - 1);
+    if (ty + TR * j < S::KR) st.k[j] = ld_off(kappa, off + 4u * (unsigned)(o.ox + tx));
+  }
+  const int xm = wrapm(o.ox - 1, g.nx);
+#pragma unroll
+  for (int j = 0; j < S::KH; j++) {
+    const int r = t + T::NT * j;
+    if (r < S::KR) {
+      const int ky = r % T::KY, kz = r / T::KY;
+      st.kh[j] = ld_off(kappa, 4u * (wrapped_row(g, o.oy - 1 + ky, o.oz - 1 + kz) + (unsigned)xm));
+    }
+  }
+}
+template <int BX, int BY, int BZ, int TR>
+__device__ inline void store_tile_kappa(float* sk, const TileStage<BX, BY, BZ, TR>& st) {
+  using S = TileStage<BX, BY, BZ, TR>;
   using T = FineTile<BX, BY, BZ, TR>;
   const int t = threadIdx.x, tx = t % BX, ty = t / BX;
-  const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
-  const int xm = wrapm(o.ox - 1, g.nx);
-  for (int r = ty; r < T::KY * T::KZ; r += TR) {
-    const int ky = r % T::KY, kz = r / T::KY;
-    const long long row = nx * wrapm(o.oy - 1 + ky, g.ny) + nxy * wrapm(o.oz - 1 + kz, g.nz);
-    sk[r * T::KX + 1 + tx] = kappa[row + o.ox + tx];
+#pragma unroll
+  for (int j = 0; j < S::KJ; j++) {
+    const int r = ty + TR * j;
+    if (r < S::KR) sk[r * T::KX + 1 + tx] = st.k[j];
   }
-  for (int r = t; r < T::KY * T::KZ; r += T::NT) {
-    const int ky = r % T::KY, kz = r / T::KY;
-    const long long row = nx * wrapm(o.oy - 1 + ky, g.ny) + nxy * wrapm(o.oz - 1 + kz, g.nz);
-    sk[r * T::KX] = kappa[row + xm];
+#pragma unroll
+  for (int j = 0; j < S::KH; j++) {
+    const int r = t + T::NT * j;
+    if (r < S::KR) sk[r * T::KX] = st.kh[j];
   }
 }
 
-// field k (k < OTM_NRHS) of node (ox-1+hx, oy-1+hy, oz-1+hz) at
+// field k (k < OTM_NRHS) of node (ox-1+hx, oy-1+hy, oz-1+hz), for
 // su[k*kU + hx + HX*(hy + HY*hz)]
 template <int BX, int BY, int BZ, int TR>
-__device__ inline void stage_tile_fields(const Geom& g, const float* __restrict__ src, float* su,
-                                         const TileOrigin& o) {
+__device__ inline void load_tile_fields(const Geom& g, const float* __restrict__ src, const TileOrigin& o,
+                                        TileStage<BX, BY, BZ, TR>& st) {
+  using S = TileStage<BX, BY, BZ, TR>;
+  using T = FineTile<BX, BY, BZ, TR>;
+  const int t = threadIdx.x, tx = t % BX;
+  const int wr0 = wave_first_row<BX>();
+  const int ty = wr0 + lane_row<BX>();
+  const float* srck[OTM_NRHS];
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) srck[k] = src + (size_t)k * (size_t)g.nn;
+#pragma unroll
+  for (int j = 0; j < S::UJ; j++) {
+    const int rb = wr0 + TR * j;
+    const unsigned off = lane_run_offset<BX, T::HY>(g, rb, S::UR, o.oy - 1, o.oz - 1) + 4u * (unsigned)(o.ox + tx);
+    if (ty + TR * j < S::UR) {
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) st.u[k][j] = ld_off(srck[k], off);
+    }
+  }
+  const int xm = wrapm(o.ox - 1, g.nx), xp = wrapm(o.ox + BX, g.nx);
+#pragma unroll
+  for (int j = 0; j < S::UH; j++) {
+    const int i = t + T::NT * j;
+    if (i < 2 * S::UR) {
+      const int r = i >> 1, side = i & 1;
+      const int hy = r % T::HY, hz = r / T::HY;
+      const unsigned w = 4u * (wrapped_row(g, o.oy - 1 + hy, o.oz - 1 + hz) + (unsigned)(side ? xp : xm));
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) st.uh[k][j] = ld_off(srck[k], w);
+    }
+  }
+}
+template <int BX, int BY, int BZ, int TR>
+__device__ inline void store_tile_fields(float* su, const TileStage<BX, BY, BZ, TR>& st) {
+  using S = TileStage<BX, BY, BZ, TR>;
   using T = FineTile<BX, BY, BZ, TR>;
   const int t = threadIdx.x, tx = t % BX, ty = t / BX;
-  const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
-  const int xm = wrapm(o.ox - 1, g.nx), xp = wrapm(o.ox + BX, g.nx);
-  for (int r = ty; r < T::HY * T::HZ; r += TR) {
-    const int hy = r % T::HY, hz = r / T::HY;
-    const long long row = nx * wrapm(o.oy - 1 + hy, g.ny) + nxy * wrapm(o.oz - 1 + hz, g.nz);
 #pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) su[k * T::kU + r * T::HX + 1 + tx] = src[k * g.nn + row + o.ox + tx];
-  }
-  for (int i = t; i < 2 * T::HY * T::HZ; i += T::NT) {
-    const int r = i >> 1, side = i & 1;
-    const int hy = r % T::HY, hz = r / T::HY;
-    const long long row = nx * wrapm(o.oy - 1 + hy, g.ny) + nxy * wrapm(o.oz - 1 + hz, g.nz);
-    const int hx = side ? T::HX - 1 : 0;
-    const long long w = row + (side ? xp : xm);
+  for (int j = 0; j < S::UJ; j++) {
+    const int r = ty + TR * j;
+    if (r < S::UR) {
 #pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) su[k * T::kU + r * T::HX + hx] = src[k * g.nn + w];
+      for (int k = 0; k < OTM_NRHS; k++) su[k * T::kU + r * T::HX + 1 + tx] = st.u[k][j];
+    }
   }
+#pragma unroll
+  for (int j = 0; j < S::UH; j++) {
+    const int i = t + T::NT * j;
+    if (i < 2 * S::UR) {
+      const int r = i >> 1, hx = (i & 1) ? T::HX - 1 : 0;
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) su[k * T::kU + r * T::HX + hx] = st.uh[k][j];
+    }
+  }
+}
+
+// Both at once (the tiled apply and residual).
+template <int BX, int BY, int BZ, int TR>
+__device__ inline void stage_tile(const Geom& g, const float* __restrict__ kappa, float* sk,
+                                  const float* __restrict__ src, float* su, const TileOrigin& o) {
+  TileStage<BX, BY, BZ, TR> st;
+  load_tile_kappa<BX, BY, BZ, TR>(g, kappa, o, st);
+  load_tile_fields<BX, BY, BZ, TR>(g, src, o, st);
+  store_tile_kappa<BX, BY, BZ, TR>(sk, st);
+  store_tile_fields<BX, BY, BZ, TR>(su, st);
 }
 
 // q = K p on the tiles lb, lb + nb, ... and the block's partial sums of p.q
@@ -514,8 +640,7 @@ __device__ inline void fine_apply_tile_body(const Geom& g, const FineConsts& fc,
   double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
   for (int b = lb; b < tiles; b += nb) {
     const TileOrigin o = tile_origin<BX, kFineBY, kFineBZ>(g, b);
-    stage_tile_kappa<BX, kFineBY, kFineBZ, TR>(g, kappa, sk, o);
-    stage_tile_fields<BX, kFineBY, kFineBZ, TR>(g, p, su, o);
+    stage_tile<BX, kFineBY, kFineBZ, TR>(g, kappa, sk, p, su, o);
     __syncthreads();
     for (int r = ty; r < kFineBY * kFineBZ; r += TR) {
       const int ly = r % kFineBY, lz = r / kFineBY;
@@ -546,8 +671,7 @@ __device__ inline void fine_residual_tile_body(const Geom& g, const FineConsts& 
   __shared__ float sk[T::kK];
   const int tx = threadIdx.x % BX, ty = threadIdx.x / BX;
   const TileOrigin o = tile_origin<BX, kFineBY, kFineBZ>(g, b);
-  stage_tile_kappa<BX, kFineBY, kFineBZ, TR>(g, kappa, sk, o);
-  stage_tile_fields<BX, kFineBY, kFineBZ, TR>(g, u, su, o);
+  stage_tile<BX, kFineBY, kFineBZ, TR>(g, kappa, sk, u, su, o);
   __syncthreads();
   for (int r = ty; r < kFineBY * kFineBZ; r += TR) {
     const int ly = r % kFineBY, lz = r / kFineBY;
@@ -591,28 +715,45 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
   const int t = threadIdx.x;
   const int tx = t % BX, ty = t / BX;  // ty in [0, TR)
   const long long nx = g.nx, nxy = (long long)g.nx * g.ny;
-  stage_tile_kappa<BX, BY, BZ, TR>(g, kappa, sk, o);
-  if (zero_in) {
-    for (int i = t; i < T::kU; i += T::NT) {
+  // the tile's global loads (kappa, the entry field) are all issued before
+  // the first LDS store; the right-hand side's loads then fly during the
+  // stores and the barrier
+  {
+    TileStage<BX, BY, BZ, TR> st;
+    load_tile_kappa<BX, BY, BZ, TR>(g, kappa, o, st);
+    if (!zero_in) load_tile_fields<BX, BY, BZ, TR>(g, uin, o, st);
+    store_tile_kappa<BX, BY, BZ, TR>(sk, st);
+    if (zero_in) {
+      for (int i = t; i < T::kU; i += T::NT) {
 #pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
+        for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
+      }
+    } else {
+      store_tile_fields<BX, BY, BZ, TR>(&su[0][0], st);
     }
-  } else {
-    stage_tile_fields<BX, BY, BZ, TR>(g, uin, &su[0][0], o);
   }
   // node ownership: x = tx; of colour (cy, cz) the x-rows rr = ty + TR*h,
-  // y = cy + 2*(rr % (BY/2)), z = cz + 2*(rr / (BY/2))
+  // y = cy + 2*(rr % (BY/2)), z = cz + 2*(rr / (BY/2)); row offsets are
+  // evaluated per wave as in the staging
   const int lx = tx;
+  const int wr0 = wave_first_row<BX>(), sub = lane_row<BX>();
+  const unsigned nxu = (unsigned)g.nx, nxyu = (unsigned)g.nx * (unsigned)g.ny;
   float fv[4][H][OTM_NRHS];
 #pragma unroll
   for (int col = 0; col < 4; col++)
 #pragma unroll
     for (int h = 0; h < H; h++) {
-      const int rr = ty + TR * h;
-      const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
-      const long long v = (ox + lx) + nx * (oy + ly) + nxy * (oz + lz);
+      unsigned off = 0u;
 #pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) fv[col][h][k] = f[k * g.nn + v];
+      for (int q = 0; q < 64 / BX; q++) {
+        const int rr = wr0 + q + TR * h;
+        const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
+        const unsigned oq = nxu * (unsigned)(oy + ly) + nxyu * (unsigned)(oz + lz);
+        off = sub == q ? oq : off;
+      }
+      off = 4u * (off + (unsigned)(ox + lx));
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) fv[col][h][k] = ld_off(f + (size_t)k * (size_t)g.nn, off);
     }
   __syncthreads();
   const float c = fc.c;
@@ -663,12 +804,22 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
       }
       __syncthreads();
     }
-  for (int r = ty; r < BY * BZ; r += TR) {
+  static_assert((BY * BZ) % TR == 0, "thread rows must divide the tile's rows");
+#pragma unroll
+  for (int j = 0; j < BY * BZ / TR; j++) {
+    unsigned off = 0u;
+#pragma unroll
+    for (int q = 0; q < 64 / BX; q++) {
+      const int r = wr0 + q + TR * j;
+      const unsigned oq = nxu * (unsigned)(oy + r % BY) + nxyu * (unsigned)(oz + r / BY);
+      off = sub == q ? oq : off;
+    }
+    off = 4u * (off + (unsigned)(ox + tx));
+    const int r = ty + TR * j;
     const int ly = r % BY, lz = r / BY;
-    const long long v = (ox + tx) + nx * (oy + ly) + nxy * (oz + lz);
     const int c0 = (tx + 1) + DY * (ly + 1) + DZ * (lz + 1);
 #pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) uout[k * g.nn + v] = su[k][c0];
+    for (int k = 0; k < OTM_NRHS; k++) *(float*)((char*)(uout + (size_t)k * (size_t)g.nn) + off) = su[k][c0];
   }
 }
 

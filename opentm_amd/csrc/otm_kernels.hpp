@@ -80,7 +80,11 @@ struct Taps {
   signed char dx[kMaxTaps], dy[kMaxTaps], dz[kMaxTaps];
   float w[kMaxTaps];
 };
-void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W);
+// W: per-node weight sums, written (w_known false) or, when a previous call
+// with the same taps wrote them, read for the nodes near a non-periodic
+// cell's faces instead of being summed again.
+void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W,
+                       bool w_known = false);
 void launch_filter_adj(hipStream_t s, const Geom& g, const Taps& T, const float* in, const float* W, float* out);
 void launch_simp_fwd(hipStream_t s, long long n, const float* rho, float k0, float k1, float floor_, float* kappa,
                      float* dkappa);

@@ -5,52 +5,110 @@
 // periodic), adjoint out_j = sum_i w_ij in_i / W_i. OC: density-weighted
 // periodic sensitivity filter (OCOptimizer.cu:76-104) and the bisection
 // update rule (OCOptimizer.cu:12-34).
+#include <algorithm>
+#include <cstdlib>
+#include <stdexcept>
+
 #include "otm_kernels.hpp"
 
 namespace otm {
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-__global__ void __launch_bounds__(kBlock) k_filter_fwd(Geom g, Taps T, const float* __restrict__ in,
-                                                       float* __restrict__ out, float* __restrict__ W) {
-  const long long t = gtid();
-  if (t >= g.nn) return;
-  int x, y, z;
-  split_node(g.nx, g.ny, t, x, y, z);
-  float s = 0.f, ws = 0.f;
-  for (int i = 0; i < T.n; i++) {
-    int X = x + T.dx[i], Y = y + T.dy[i], Z = z + T.dz[i];
-    const bool inside = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz;
-    if (!inside) {
-      if (!T.periodic) continue;
-      X = wrapm(X, g.nx); Y = wrapm(Y, g.ny); Z = wrapm(Z, g.nz);
-    }
-    const float w = T.w[i];
-    s += w * in[(long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z)];
-    ws += w;
-  }
-  out[t] = s / ws;
-  if (W) W[t] = ws;
-}
+// ---- filters, LDS-tiled ----
+// A block computes a 32 x 8 x 4 box of outputs (thread (x, y), four z) from
+// an LDS copy of the box grown by the taps' reach h (at most 3) on every
+// side, so each input comes from HBM about twice (halo) instead of once per
+// tap. Positions outside a non-periodic cell hold 0, which leaves the sums
+// as if the tap were skipped. Taps become offsets into that copy.
+constexpr int kFtX = 32, kFtY = 8, kFtZ = 4, kFtH = 3;
+constexpr int kFtLds = (kFtX + 2 * kFtH) * (kFtY + 2 * kFtH) * (kFtZ + 2 * kFtH);
+static_assert(kFtX * kFtY == kBlock, "one thread per (x, y) column of a filter tile");
+enum { FILT_FWD = 0, FILT_ADJ = 1, FILT_SENS = 2 };
 
-__global__ void __launch_bounds__(kBlock) k_filter_adj(Geom g, Taps T, const float* __restrict__ in,
-                                                       const float* __restrict__ W, float* __restrict__ out) {
-  const long long t = gtid();
-  if (t >= g.nn) return;
-  int x, y, z;
-  split_node(g.nx, g.ny, t, x, y, z);
-  float s = 0.f;
-  for (int i = 0; i < T.n; i++) {
-    int X = x - T.dx[i], Y = y - T.dy[i], Z = z - T.dz[i];
-    const bool inside = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz;
-    if (!inside) {
-      if (!T.periodic) continue;
-      X = wrapm(X, g.nx); Y = wrapm(Y, g.ny); Z = wrapm(Z, g.nz);
-    }
-    const long long j = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
-    s += T.w[i] * in[j] / W[j];
+// FILT_FWD: out = sum w in / W and W = sum of the weights of the taps
+// inside the cell (all taps when periodic; W may be null).
+// FILT_ADJ: out_j = sum_i w in_i / W_i over the reversed taps (b = W).
+// FILT_SENS: out = sum w (sens rho) / (W rho), periodic (a = sens, b = rho).
+// wall is the sum of all tap weights in tap order (the W of an output whose
+// taps are all inside).
+// w_known (FILT_FWD): W already holds the weight sums (a previous call with
+// the same taps wrote them); nodes near a non-periodic cell's faces read
+// theirs instead of summing the taps again.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_filter_tile(Geom g, Taps T, int h, float wall, const float* __restrict__ a,
+                                                        const float* __restrict__ b, float* __restrict__ out,
+                                                        float* __restrict__ W, int w_known) {
+  __shared__ float tile[kFtLds];
+  __shared__ int toff[kMaxTaps];
+  __shared__ float tw[kMaxTaps];
+  const int SX = kFtX + 2 * h, SY = kFtY + 2 * h, SZ = kFtZ + 2 * h;
+  const int ntx = (g.nx + kFtX - 1) / kFtX, nty = (g.ny + kFtY - 1) / kFtY;
+  const int bx = (int)blockIdx.x % ntx, byz = (int)blockIdx.x / ntx;
+  const int x0 = bx * kFtX, y0 = (byz % nty) * kFtY, z0 = (byz / nty) * kFtZ;
+  const bool periodic = MODE == FILT_SENS || T.periodic;
+  for (int i = threadIdx.x; i < T.n; i += kBlock) {
+    const int o = T.dx[i] + SX * (T.dy[i] + SY * T.dz[i]);
+    toff[i] = MODE == FILT_ADJ ? -o : o;
+    tw[i] = T.w[i];
   }
-  out[t] = s;
+  for (int i = threadIdx.x; i < SX * SY * SZ; i += kBlock) {
+    const int lx = i % SX, rest = i / SX;
+    int X = x0 - h + lx, Y = y0 - h + rest % SY, Z = z0 - h + rest / SY;
+    float v = 0.f;
+    // positions past the cell by more than h feed no output of this tile
+    if (X < g.nx + h && Y < g.ny + h && Z < g.nz + h) {
+      const bool inside = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz;
+      if (inside || periodic) {
+        X = wrapm(X, g.nx);
+        Y = wrapm(Y, g.ny);
+        Z = wrapm(Z, g.nz);
+        const long long j = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
+        v = MODE == FILT_FWD ? a[j] : (MODE == FILT_ADJ ? a[j] / b[j] : a[j] * b[j]);
+      }
+    }
+    tile[i] = v;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % kFtX, ty = threadIdx.x / kFtX;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= g.nx || y >= g.ny) return;
+  const int plane = SX * SY;
+  const int c0 = (tx + h) + SX * ((ty + h) + SY * h);
+  float acc[kFtZ] = {0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < T.n; i++) {
+    const int o = c0 + toff[i];
+    const float w = tw[i];
+#pragma unroll
+    for (int k = 0; k < kFtZ; k++) acc[k] = fmaf(w, tile[o + k * plane], acc[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kFtZ; k++) {
+    const int z = z0 + k;
+    if (z >= g.nz) break;
+    const long long t = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    if (MODE == FILT_ADJ) {
+      out[t] = acc[k];
+    } else if (MODE == FILT_SENS) {
+      out[t] = acc[k] / (wall * b[t]);
+    } else {
+      float ws = wall;
+      const bool interior = x >= h && x + h < g.nx && y >= h && y + h < g.ny && z >= h && z + h < g.nz;
+      if (!periodic && !interior) {
+        if (w_known) {
+          ws = W[t];
+        } else {
+          ws = 0.f;
+          for (int i = 0; i < T.n; i++) {
+            const int X = x + T.dx[i], Y = y + T.dy[i], Z = z + T.dz[i];
+            if (X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz) ws += tw[i];
+          }
+        }
+      }
+      out[t] = acc[k] / ws;
+      if (W && !w_known) W[t] = ws;
+    }
+  }
 }
 
 // kappa = k1 + (k0 - k1) rho^3 (+floor), dkappa = 3 (k0 - k1) rho^2
@@ -71,23 +129,6 @@ __global__ void __launch_bounds__(kBlock) k_mul(long long n, const float* __rest
   const long long t = gtid();
   if (t >= n) return;
   out[t] = a[t] * b[t];
-}
-
-__global__ void __launch_bounds__(kBlock) k_oc_filter_sens(Geom g, Taps T, const float* __restrict__ sens,
-                                                           const float* __restrict__ rho, float* __restrict__ out) {
-  const long long t = gtid();
-  if (t >= g.nn) return;
-  int x, y, z;
-  split_node(g.nx, g.ny, t, x, y, z);
-  float s = 0.f, ws = 0.f;
-  for (int i = 0; i < T.n; i++) {
-    const int X = wrapm(x + T.dx[i], g.nx), Y = wrapm(y + T.dy[i], g.ny), Z = wrapm(z + T.dz[i], g.nz);
-    const long long j = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
-    const float w = T.w[i];
-    s += sens[j] * rho[j] * w;
-    ws += w;
-  }
-  out[t] = s / (ws * rho[t]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_maxabs(long long n, const float* __restrict__ a, unsigned int* out_bits) {
@@ -421,11 +462,36 @@ void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes
   hipLaunchKernelGGL(k_init_density, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, kind, M, rho);
 }
 
-void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W) {
-  hipLaunchKernelGGL(k_filter_fwd, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, in, out, W);
+// Reach of the taps (largest offset along any axis) and the sum of all
+// weights in tap order, in FP32 as the kernel adds them.
+static void tap_stats(const Taps& T, int& h, float& wall) {
+  h = 0;
+  wall = 0.f;
+  for (int i = 0; i < T.n; i++) {
+    h = std::max(h, std::max(std::abs((int)T.dx[i]), std::max(std::abs((int)T.dy[i]), std::abs((int)T.dz[i]))));
+    wall += T.w[i];
+  }
+}
+
+template <int MODE>
+static void launch_filter_tile(hipStream_t s, const Geom& g, const Taps& T, const float* a, const float* b,
+                               float* out, float* W, bool w_known = false) {
+  int h;
+  float wall;
+  tap_stats(T, h, wall);
+  if (h > kFtH || T.n > kMaxTaps) throw std::invalid_argument("filter taps reach beyond 3 cells");
+  if (h > g.nx || h > g.ny || h > g.nz) throw std::invalid_argument("filter reach exceeds the cell");
+  const unsigned nb = (unsigned)(((g.nx + kFtX - 1) / kFtX) * ((g.ny + kFtY - 1) / kFtY) * ((g.nz + kFtZ - 1) / kFtZ));
+  hipLaunchKernelGGL(k_filter_tile<MODE>, dim3(nb), dim3(kBlock), 0, s, g, T, h, wall, a, b, out, W,
+                     (int)(w_known && W));
+}
+
+void launch_filter_fwd(hipStream_t s, const Geom& g, const Taps& T, const float* in, float* out, float* W,
+                       bool w_known) {
+  launch_filter_tile<FILT_FWD>(s, g, T, in, nullptr, out, W, w_known);
 }
 void launch_filter_adj(hipStream_t s, const Geom& g, const Taps& T, const float* in, const float* W, float* out) {
-  hipLaunchKernelGGL(k_filter_adj, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, in, W, out);
+  launch_filter_tile<FILT_ADJ>(s, g, T, in, W, out, nullptr);
 }
 void launch_simp_fwd(hipStream_t s, long long n, const float* rho, float k0, float k1, float floor_, float* kappa,
                      float* dkappa) {
@@ -436,7 +502,7 @@ void launch_mul(hipStream_t s, long long n, const float* a, const float* b, floa
 }
 void launch_oc_filter_sens(hipStream_t s, const Geom& g, const Taps& T, const float* sens, const float* rho,
                            float* out) {
-  hipLaunchKernelGGL(k_oc_filter_sens, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, T, sens, rho, out);
+  launch_filter_tile<FILT_SENS>(s, g, T, sens, rho, out, nullptr);
 }
 void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out_bits) {
   unsigned nb = nblk(n);

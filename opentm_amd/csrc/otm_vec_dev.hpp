@@ -142,7 +142,10 @@ __device__ inline void reduce_final_body(const Reduce& R, int nv, int nblocks) {
   for (int b = threadIdx.x; b < nblocks; b += kBlock)
     for (int i = 0; i < nv; i++) acc[i] += R.partials[(size_t)b * nv + i];
   block_sum<kMaxReduce>(acc, sh);
-  if (threadIdx.x == 0) finalize(acc, nv, R);
+  if (threadIdx.x == 0) {
+    finalize(acc, nv, R);
+    publish_host(acc, nv, R);
+  }
 }
 
 }  // namespace otm

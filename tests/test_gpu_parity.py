@@ -254,11 +254,14 @@ def test_full_size_shift_invariance():
 
 
 # ------------------------------------------------------------ filters and OC
-@pytest.mark.parametrize("kind,r,periodic", [(0, 1.5, False), (1, 2.0, True), (0, 2.5, True)])
-def test_conv_filter(orc, kind, r, periodic):
-    say("filter", kind, r, periodic)
+@pytest.mark.parametrize("kind,r,periodic,shape", [
+    (0, 1.5, False, (10, 12, 14)), (1, 2.0, True, (10, 12, 14)), (0, 2.5, True, (10, 12, 14)),
+    # reach 3 (the widest) and several 32 x 8 x 4 tiles per axis, ragged
+    (1, 3.4, False, (9, 19, 70)), (0, 3.4, True, (9, 19, 70)), (1, 2.0, True, (3, 5, 40))])
+def test_conv_filter(orc, kind, r, periodic, shape):
+    say("filter", kind, r, periodic, shape)
     rng = np.random.default_rng(12)
-    x = rng.uniform(0, 1, (10, 12, 14)).astype(np.float32)
+    x = rng.uniform(0, 1, shape).astype(np.float32)
     y = otm.conv_filter(x, kind, r, periodic)
     assert rel(y, orc.filter_fwd(x, kind, r, periodic)) < 2e-6
     ya = otm.conv_filter(x, kind, r, periodic, adjoint=True)
