@@ -68,6 +68,25 @@ inline void check_where(int w) {
   if (w != OTM_HOST && w != OTM_DEVICE) fail(OTM_E_ARG, "where must be OTM_HOST or OTM_DEVICE");
 }
 
+// Polls a tag that a kernel on stream s publishes to coherent host memory
+// with a system-scope release, until it equals `tag`. The stream keeps
+// running what is queued after that kernel. A stream that drains or fails
+// without the tag is an error (e.g. a kernel fault).
+inline void wait_tag(hipStream_t s, const unsigned* seq, unsigned tag) {
+  unsigned spins = 0;
+  while (__atomic_load_n(seq, __ATOMIC_ACQUIRE) != tag) {
+    if ((++spins & 255u) == 0) {
+      const hipError_t e = hipStreamQuery(s);
+      if (e == hipSuccess) {
+        if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == tag) return;
+        fail(OTM_E_HIP, "kernel results were not published");
+      }
+      if (e != hipErrorNotReady) hip_check(e, "hipStreamQuery");
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 inline unsigned red_blocks_for(long long n) {
   long long b = (n + kBlock - 1) / kBlock;
   if (b > 1024) b = 1024;
@@ -175,22 +194,9 @@ struct otm_ctx {
     hm_pending = true;
     return R;
   }
-  // Waits until the last publishing finaliser has run, polling its tag; the
-  // stream keeps running the work queued after it. A stream that drains or
-  // fails without the tag is an error (e.g. a kernel fault).
+  // Waits until the last publishing finaliser has run (wait_tag).
   void wait_host() {
-    unsigned spins = 0;
-    while (__atomic_load_n(&hm->seq, __ATOMIC_ACQUIRE) != hm_tag) {
-      if ((++spins & 255u) == 0) {
-        const hipError_t e = hipStreamQuery(s);
-        if (e == hipSuccess) {
-          if (__atomic_load_n(&hm->seq, __ATOMIC_ACQUIRE) == hm_tag) break;
-          fail(OTM_E_HIP, "reduction results were not published");
-        }
-        if (e != hipErrorNotReady) hip_check(e, "hipStreamQuery");
-      }
-      __builtin_ia32_pause();
-    }
+    wait_tag(s, &hm->seq, hm_tag);
     hm_pending = false;
     // timing events are read at stream synchronisations; bound their number
     if (pending.size() > 512) sync();

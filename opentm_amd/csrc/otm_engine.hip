@@ -129,7 +129,9 @@ struct otm_oc {
   unsigned* maxbits = nullptr;
   unsigned* counter = nullptr;  // finished-block counter of the one-launch bisection pass
   OcState* st = nullptr;
-  OcState* st_host = nullptr;  // pinned
+  OcHost* hm = nullptr;      // coherent pinned block the apply kernel publishes to
+  OcHost* hm_dev = nullptr;  // its device address
+  unsigned tag = 0;
   double* partials = nullptr;
 };
 
@@ -534,15 +536,15 @@ static void pcg_solve(otm_ctx* c) {
   }
   double relres[3];
   for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
-  // remove the constant component (translateForce, grid.cpp:654-664)
+  // remove the constant component (translateForce, grid.cpp:654-664); in
+  // stream order with what follows (no result the host needs)
   if (c->x64) {
-    launch_sum3(c->s, n, c->x64, c->reducer_host(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sum3(c->s, n, c->x64, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
     launch_sub_mean_cast(c->s, n, c->x64, c->x, c->pcg);
   } else {
-    launch_sum3(c->s, n, c->x, c->reducer_host(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
     launch_sub_mean3(c->s, n, c->x, c->pcg);
   }
-  c->read_pcg();
   for (int k = 0; k < 3; k++) {
     c->relres[k] = relres[k];
     c->iters[k] = its[k] + (its[k] < it ? 1 : 0);
@@ -1129,9 +1131,9 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
     if (drho) {
       check_where(where);
       hip_check(hipMemcpyAsync(drho, ctx->drho, sizeof(float) * n, kind_of(where, OTM_DEVICE), ctx->s), "copy drho");
+      ctx->sync();
     }
-    ctx->sync();
-    ctx->has_drho = true;
+    ctx->has_drho = true;  // on the device, in stream order
   });
 }
 
@@ -1141,7 +1143,7 @@ static void oc_free(otm_oc* o) {
   if (o->s) (void)hipStreamSynchronize(o->s);
   dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits); dfree(o->counter);
   dfree(o->st); dfree(o->partials);
-  if (o->st_host) (void)hipHostFree(o->st_host);
+  if (o->hm) (void)hipHostFree(o->hm);
   if (o->s) (void)hipStreamDestroy(o->s);
   delete o;
 }
@@ -1167,7 +1169,10 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
       hip_check(hipMemsetAsync(o->counter, 0, sizeof(unsigned), o->s), "clear counter");
       o->st = dalloc<OcState>(1);
       o->partials = dalloc<double>((size_t)1024 * kOcCand);
-      hip_check(hipHostMalloc((void**)&o->st_host, sizeof(OcState), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&o->hm, sizeof(OcHost), hipHostMallocCoherent), "hipHostMalloc");
+      memset(o->hm, 0, sizeof(OcHost));
+      hip_check(hipHostGetDevicePointer((void**)&o->hm_dev, o->hm, 0), "hipHostGetDevicePointer");
+      hip_check(hipStreamSynchronize(o->s), "sync");  // counter cleared before any stream uses it
     } catch (...) {
       oc_free(o);
       throw;
@@ -1188,10 +1193,14 @@ OTM_API int otm_oc_set_bisection(otm_oc* oc, int max_steps, double vol_tol) {
   });
 }
 
-static void oc_filter_dev(otm_oc* o, float* sens, const float* rho, const Geom& g, double radius) {
+// Filtered sensitivities into o->newsens (on stream s); copied back over
+// sens when copy_back is set (filterSens works in place).
+static void oc_filter_dev(otm_oc* o, hipStream_t s, float* sens, const float* rho, const Geom& g, double radius,
+                          bool copy_back) {
   Taps T = make_taps(1, radius, 1);  // OCOptimizer.cu:143: Linear, periodic
-  launch_oc_filter_sens(o->s, g, T, sens, rho, o->newsens);
-  hip_check(hipMemcpyAsync(sens, o->newsens, sizeof(float) * g.nn, hipMemcpyDeviceToDevice, o->s), "copy sens");
+  launch_oc_filter_sens(s, g, T, sens, rho, o->newsens);
+  if (copy_back)
+    hip_check(hipMemcpyAsync(sens, o->newsens, sizeof(float) * g.nn, hipMemcpyDeviceToDevice, s), "copy sens");
 }
 
 OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pitch, const int reso[3],
@@ -1211,7 +1220,7 @@ OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pi
     hip_check(hipMemcpy2DAsync(oc->b, sizeof(float) * reso[0], rho, sizeof(float) * pitch, sizeof(float) * reso[0],
                                rows, kind_of(OTM_DEVICE, where), oc->s),
               "gather rho");
-    oc_filter_dev(oc, oc->a, oc->b, g, radius);
+    oc_filter_dev(oc, oc->s, oc->a, oc->b, g, radius, true);
     hip_check(hipMemcpy2DAsync(sens, sizeof(float) * pitch, oc->a, sizeof(float) * reso[0], sizeof(float) * reso[0],
                                rows, kind_of(where, OTM_DEVICE), oc->s),
               "scatter sens");
@@ -1219,7 +1228,10 @@ OTM_API int otm_oc_filter_sens(otm_oc* oc, float* sens, const float* rho, int pi
   });
 }
 
-static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volratio, int* steps, double* volume) {
+// The OC update on stream s. With final_sync the call returns once rho is
+// updated; without, rho is updated in stream order on s.
+static void oc_update_dev(otm_oc* o, hipStream_t s, const float* sens, float* rho, double volratio, int* steps,
+                          double* volume, bool final_sync = true) {
   const long long n = o->ne;
   const unsigned nb = std::min(512u, red_blocks_for(n));  // fewer partials for the 31-way final sum
   Reduce R;
@@ -1230,27 +1242,27 @@ static void oc_update_dev(otm_oc* o, const float* sens, float* rho, double volra
   R.inv_nn = 0;
   R.mode = FIN_STORE;
   // passes of kOcTree bisection levels are queued without host round trips
-  // (a pass after the tolerance is met returns at once); the host checks the
-  // state every few passes, then the update is applied at the final midpoint
-  launch_oc_begin(o->s, n, sens, o->maxbits, o->st);
+  // (a pass after the tolerance is met returns at once), each group followed
+  // by an apply that updates rho only once the bisection has ended and
+  // publishes the state: one host round trip per group, normally one
+  launch_oc_begin(s, n, sens, o->maxbits, o->st);
   const int passes = (o->max_steps + kOcTree - 1) / kOcTree;
   const int per_check = 4;
   for (int done = 0; done < passes;) {
     const int m = std::min(per_check, passes - done);
     for (int i = 0; i < m; i++)
-      launch_oc_tree_pass(o->s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, volratio,
+      launch_oc_tree_pass(s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, volratio,
                           o->vol_tol, o->max_steps, R, nb, o->counter);
     done += m;
-    if (done >= passes) break;
-    hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
-    hip_check(hipStreamSynchronize(o->s), "sync");
-    if (o->st_host->done) break;
+    const unsigned tag = ++o->tag;
+    launch_oc_apply(s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, o->max_steps,
+                    done >= passes ? 1 : 0, o->hm_dev, tag);
+    wait_tag(s, &o->hm->seq, tag);
+    if (o->hm->applied || o->hm->st.steps == 0) break;
   }
-  launch_oc_apply(o->s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp);
-  hip_check(hipMemcpyAsync(o->st_host, o->st, sizeof(OcState), hipMemcpyDeviceToHost, o->s), "read state");
-  hip_check(hipStreamSynchronize(o->s), "sync");
-  if (steps) *steps = o->st_host->steps;
-  if (volume) *volume = o->st_host->vol;
+  if (final_sync) hip_check(hipStreamSynchronize(s), "sync");  // rho updated
+  if (steps) *steps = o->hm->st.steps;
+  if (volume) *volume = o->hm->st.vol;
 }
 
 OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volratio, int where, int* steps,
@@ -1261,11 +1273,11 @@ OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volr
     hip_check(hipSetDevice(oc->device), "hipSetDevice");
     const long long n = oc->ne;
     if (where == OTM_DEVICE) {
-      oc_update_dev(oc, sens, rho, volratio, steps, volume);
+      oc_update_dev(oc, oc->s, sens, rho, volratio, steps, volume);
     } else {
       hip_check(hipMemcpyAsync(oc->a, sens, sizeof(float) * n, hipMemcpyHostToDevice, oc->s), "upload sens");
       hip_check(hipMemcpyAsync(oc->b, rho, sizeof(float) * n, hipMemcpyHostToDevice, oc->s), "upload rho");
-      oc_update_dev(oc, oc->a, oc->b, volratio, steps, volume);
+      oc_update_dev(oc, oc->s, oc->a, oc->b, volratio, steps, volume);
       hip_check(hipMemcpyAsync(rho, oc->b, sizeof(float) * n, hipMemcpyDeviceToHost, oc->s), "download rho");
       hip_check(hipStreamSynchronize(oc->s), "sync");
     }
@@ -1280,9 +1292,14 @@ OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double v
     if (oc->ne != ctx->g0.nn) fail(OTM_E_ARG, "optimizer size does not match the cell");
     if (oc->device != ctx->device) fail(OTM_E_ARG, "optimizer and context on different devices");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-    ctx->sync();
-    if (filter_radius > 0) oc_filter_dev(oc, ctx->drho, ctx->rho, ctx->g0, filter_radius);
-    oc_update_dev(oc, ctx->drho, ctx->rho, volratio, steps, volume);
+    // on the context's stream, in order with the gradient before and the
+    // next evaluation after: no synchronisation but the bisection's own
+    const float* g = ctx->drho;
+    if (filter_radius > 0) {
+      oc_filter_dev(oc, ctx->s, ctx->drho, ctx->rho, ctx->g0, filter_radius, false);
+      g = oc->newsens;
+    }
+    oc_update_dev(oc, ctx->s, g, ctx->rho, volratio, steps, volume, false);
     ctx->has_drho = false;
   });
 }

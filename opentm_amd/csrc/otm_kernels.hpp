@@ -118,8 +118,17 @@ constexpr int kOcCand = (1 << kOcTree) - 1;  // midpoints evaluated per pass
 void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const float* rho, OcState* st,
                          float min_rho, float step, float damp, double volratio, double vol_tol, int max_steps,
                          const Reduce& R, unsigned nblocks, unsigned* counter = nullptr);
+// The OC state as published to the host by k_oc_apply (coherent pinned memory).
+struct OcHost {
+  OcState st;
+  int applied;    // the update was applied
+  unsigned seq;   // tag of the apply launch that published this
+};
+// Applies the update at the state's multiplier when the bisection is done,
+// has used max_steps midpoints, or force is set; otherwise leaves rho. Then
+// publishes the state and whether it applied to hm with `tag`.
 void launch_oc_apply(hipStream_t s, long long n, const float* sens, float* rho, const OcState* st, float min_rho,
-                     float step, float damp);
+                     float step, float damp, int max_steps, int force, OcHost* hm, unsigned tag);
 
 // ---- MMA (otm_mma.hip) ----
 constexpr int kMmaMaxM = 4;  // constraints supported by the device subsolver

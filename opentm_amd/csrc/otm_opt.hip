@@ -392,8 +392,15 @@ __global__ void __launch_bounds__(kBlock) k_oc_tree_pass(long long n, const floa
 // rho <- update rule at the final multiplier (in place).
 __global__ void __launch_bounds__(kBlock) k_oc_apply(long long n, const float* __restrict__ sens, float* rho,
                                                      const OcState* __restrict__ st, float min_rho, float step,
-                                                     float damp, float c01) {
-  if (st->steps == 0) return;
+                                                     float damp, float c01, int max_steps, int force, OcHost* hm,
+                                                     unsigned tag) {
+  const bool apply = st->steps > 0 && (st->done || st->steps >= max_steps || force);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hm->st = *st;
+    hm->applied = apply ? 1 : 0;
+    __hip_atomic_store(&hm->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  if (!apply) return;
   const float lg = __builtin_amdgcn_logf(st->g);
   for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
     const float sv = sens[i];
@@ -529,11 +536,12 @@ void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const fl
                      vol_tol, max_steps);
 }
 void launch_oc_apply(hipStream_t s, long long n, const float* sens, float* rho, const OcState* st, float min_rho,
-                     float step, float damp) {
+                     float step, float damp, int max_steps, int force, OcHost* hm, unsigned tag) {
   const float c01 = powf(0.01f, damp);
   unsigned nb = nblk(n);
   if (nb > 8192) nb = 8192;
-  hipLaunchKernelGGL(k_oc_apply, dim3(nb), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01);
+  hipLaunchKernelGGL(k_oc_apply, dim3(nb), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01, max_steps,
+                     force, hm, tag);
 }
 
 // Orbit average under a symmetry group (symetrize_tensor_kernel,
