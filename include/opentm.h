@@ -154,7 +154,10 @@ OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]);
 
 /* objective.backward() (homoExpression.h:106-120): sensitivity with dC,
  * chained through SIMP and the filter adjoint into d obj / d rho, kept in
- * the context; copied to drho if it is non-NULL. */
+ * the context; copied to drho if it is non-NULL (the call then returns
+ * with drho written). With drho NULL the gradient stays in the context's
+ * stream order for the calls that use it (otm_oc_step,
+ * otm_symmetrize_design) and the call returns without waiting. */
 OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int where);
 
 /* homo::OCOptimizer (OCOptimizer.h:7-20). */
@@ -172,7 +175,10 @@ OTM_API int otm_oc_update(otm_oc* oc, const float* sens, float* rho, double volr
                           double* volume);
 
 /* One OC design iteration on the context's design density, using its
- * current d obj / d rho (after otm_backward): filterSens + update. */
+ * current d obj / d rho (after otm_backward): filterSens + update, on the
+ * context's stream (the optimizer's buffers are used there; calls on one
+ * optimizer must not overlap). Returns once the bisection has ended; the
+ * updated density is in the context's stream order for the next call. */
 OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double volratio, int* steps,
                         double* volume);
 
