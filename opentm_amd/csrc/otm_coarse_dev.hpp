@@ -239,6 +239,34 @@ __device__ inline int wrap_any(int a, int n) {
   a %= n;
   return a < 0 ? a + n : a;
 }
+// Sum over aligned groups of 8 lanes; every lane of a group receives the
+// same sum (IEEE addition commutes). Cross-lane moves through DPP: swap
+// pairs (quad_perm 1,0,3,2), swap pairs of pairs (quad_perm 2,3,0,1), then
+// the mirrored half row (lane i of 8 takes lane 7 - i).
+__device__ inline float group8_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+  return v;
+}
+
+// Coupling o of the stored stencil at node v (FP16: scaled back to FP32).
+template <bool HALF>
+__device__ inline float stencil_coef(const StRef& S, long long nn, long long v, int o) {
+  if constexpr (HALF)
+    return (float)S.f16[o * nn + v] * *S.scale;
+  else
+    return S.f32[o * nn + v];
+}
+
+// Work split of a colour step: the 64 nodes of one colour in a brick are
+// updated by 64 groups of 8 lanes. Lane j of a group holds the couplings
+// o = j, j + 8, j + 16, j + 24 (o != 13, o < 27: 2 to 4 of the 26) of its
+// node for every colour, forms their products with the three load cases'
+// values, and the group adds its eight partial sums (group8_sum); lanes
+// 0-2 then write load cases 0-2. With one node per thread
+// only 16 lanes of half the waves worked in a colour step, each through 78
+// dependent multiply-adds.
 template <bool HALF>
 __device__ inline void st_brick_body(const Geom& g, const StRef& S, const float* __restrict__ f,
                                      const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward,
@@ -247,55 +275,96 @@ __device__ inline void st_brick_body(const Geom& g, const StRef& S, const float*
   const int bx = (g.nx + kBrick - 1) / kBrick, by = (g.ny + kBrick - 1) / kBrick;
   const int ox = (b % bx) * kBrick, oy = ((b / bx) % by) * kBrick, oz = (b / (bx * by)) * kBrick;
   const int t = threadIdx.x;
+  const long long nn = g.nn;
+  // halo load (or zero initial guess): loads first, then the LDS stores
+  {
+    constexpr int kH3 = kHalo * kHalo * kHalo;
+    constexpr int kTrips = (kH3 + kBrickThreads - 1) / kBrickThreads;
+    float hv[kTrips][OTM_NRHS];
+#pragma unroll
+    for (int r = 0; r < kTrips; r++) {
+      const int i = t + kBrickThreads * r;
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) hv[r][k] = 0.f;
+      if (i < kH3 && !zero_in) {
+        const int hx = i % kHalo, hy = (i / kHalo) % kHalo, hz = i / (kHalo * kHalo);
+        const int X = wrap_any(ox + hx - 1, g.nx), Y = wrap_any(oy + hy - 1, g.ny), Z = wrap_any(oz + hz - 1, g.nz);
+        const long long w = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) hv[r][k] = uin[k * nn + w];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kTrips; r++) {
+      const int i = t + kBrickThreads * r;
+      if (i < kH3) {
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) su[k][i] = hv[r][k];
+      }
+    }
+  }
+  const int m = t >> 3, j = t & 7;
+  int doff[4];
+  bool slot[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int o = j + 8 * q;
+    slot[q] = o < 27 && o != 13;
+    doff[q] = slot[q] ? (o % 3 - 1) + kHalo * (((o / 3) % 3 - 1) + kHalo * (o / 9 - 1)) : 0;
+  }
+  float cf[8][4], inv[8], fv[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int x = ox + 2 * (m & 3) + (c & 1), y = oy + 2 * ((m >> 2) & 3) + ((c >> 1) & 1),
+              z = oz + 2 * (m >> 4) + (c >> 2);
+    const bool act = x < g.nx && y < g.ny && z < g.nz;
+    const long long v = act ? (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z) : 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) cf[c][q] = act && slot[q] ? stencil_coef<HALF>(S, nn, v, j + 8 * q) : 0.f;
+    if constexpr (!HALF) inv[c] = act ? 1.f / S.f32[13 * nn + v] : 0.f;
+    fv[c] = act && j < OTM_NRHS ? f[j * nn + v] : 0.f;
+  }
+  if constexpr (HALF) {
+    // the diagonal is minus the sum of the stored couplings (rows sum to zero)
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const float sd = group8_sum((cf[c][0] + cf[c][1]) + (cf[c][2] + cf[c][3]));
+      inv[c] = sd != 0.f ? -1.f / sd : 0.f;
+    }
+  }
+  __syncthreads();
+  auto step = [&](const int c) {
+    const int c0 = (2 * (m & 3) + (c & 1) + 1) + kHalo * ((2 * ((m >> 2) & 3) + ((c >> 1) & 1) + 1) +
+                                                          kHalo * (2 * (m >> 4) + (c >> 2) + 1));
+    float a[OTM_NRHS] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int p = c0 + doff[q];
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) a[k] += cf[c][q] * su[k][p];
+    }
+#pragma unroll
+    for (int k = 0; k < OTM_NRHS; k++) a[k] = group8_sum(a[k]);
+    if (j < OTM_NRHS && inv[c] != 0.f) {
+      const float acc = j == 0 ? a[0] : (j == 1 ? a[1] : a[2]);
+      su[j][c0] = (fv[c] - acc) * inv[c];
+    }
+    __syncthreads();
+  };
+  for (int s = 0; s < sweeps; s++) {
+    if (forward) {
+#pragma unroll
+      for (int c = 0; c < 8; c++) step(c);
+    } else {
+#pragma unroll
+      for (int c = 7; c >= 0; c--) step(c);
+    }
+  }
   const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
   const int x = ox + lx, y = oy + ly, z = oz + lz;
-  const bool active = x < g.nx && y < g.ny && z < g.nz;
-  const long long nn = g.nn;
-  const long long v = active ? (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z) : 0;
-  // halo load (or zero initial guess)
-  for (int i = t; i < kHalo * kHalo * kHalo; i += kBrickThreads) {
-    const int hx = i % kHalo, hy = (i / kHalo) % kHalo, hz = i / (kHalo * kHalo);
-    if (zero_in) {
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = 0.f;
-    } else {
-      const int X = wrap_any(ox + hx - 1, g.nx), Y = wrap_any(oy + hy - 1, g.ny), Z = wrap_any(oz + hz - 1, g.nz);
-      const long long w = (long long)X + (long long)g.nx * ((long long)Y + (long long)g.ny * Z);
-#pragma unroll
-      for (int k = 0; k < OTM_NRHS; k++) su[k][i] = uin[k * nn + w];
-    }
-  }
-  float a[27];
-  float fv[OTM_NRHS];
-  float inv = 0.f;
-  if (active) {
-    load_stencil<HALF>(S, nn, v, a);
-#pragma unroll
-    for (int k = 0; k < OTM_NRHS; k++) fv[k] = f[k * nn + v];
-    inv = 1.f / a[13];
-  }
-  const int mycol = active ? (x & 1) + 2 * (y & 1) + 4 * (z & 1) : -1;
-  const int c0 = (lx + 1) + kHalo * ((ly + 1) + kHalo * (lz + 1));
-  __syncthreads();
-  for (int s = 0; s < sweeps; s++)
-    for (int i = 0; i < 8; i++) {
-      const int col = forward ? i : 7 - i;
-      if (col == mycol) {
-#pragma unroll
-        for (int k = 0; k < OTM_NRHS; k++) {
-          float acc = 0.f;
-#pragma unroll
-          for (int o = 0; o < 27; o++) {
-            if (o == 13) continue;
-            const int d = (o % 3 - 1) + kHalo * (((o / 3) % 3 - 1) + kHalo * (o / 9 - 1));
-            acc += a[o] * su[k][c0 + d];
-          }
-          su[k][c0] = (fv[k] - acc) * inv;
-        }
-      }
-      __syncthreads();
-    }
-  if (active) {
+  if (x < g.nx && y < g.ny && z < g.nz) {
+    const long long v = (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z);
+    const int c0 = (lx + 1) + kHalo * ((ly + 1) + kHalo * (lz + 1));
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) uout[k * nn + v] = su[k][c0];
   }
