@@ -127,7 +127,6 @@ struct otm_oc {
   hipStream_t s = nullptr;
   float *newsens = nullptr, *a = nullptr, *b = nullptr;
   unsigned* maxbits = nullptr;
-  unsigned* counter = nullptr;  // finished-block counter of the one-launch bisection pass
   OcState* st = nullptr;
   OcHost* hm = nullptr;      // coherent pinned block the apply kernel publishes to
   OcHost* hm_dev = nullptr;  // its device address
@@ -1141,7 +1140,7 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
 static void oc_free(otm_oc* o) {
   if (!o) return;
   if (o->s) (void)hipStreamSynchronize(o->s);
-  dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits); dfree(o->counter);
+  dfree(o->newsens); dfree(o->a); dfree(o->b); dfree(o->maxbits);
   dfree(o->st); dfree(o->partials);
   if (o->hm) (void)hipHostFree(o->hm);
   if (o->s) (void)hipStreamDestroy(o->s);
@@ -1165,14 +1164,11 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
       o->a = dalloc<float>(ne);
       o->b = dalloc<float>(ne);
       o->maxbits = dalloc<unsigned>(1);
-      o->counter = dalloc<unsigned>(1);
-      hip_check(hipMemsetAsync(o->counter, 0, sizeof(unsigned), o->s), "clear counter");
       o->st = dalloc<OcState>(1);
       o->partials = dalloc<double>((size_t)1024 * kOcCand);
       hip_check(hipHostMalloc((void**)&o->hm, sizeof(OcHost), hipHostMallocCoherent), "hipHostMalloc");
       memset(o->hm, 0, sizeof(OcHost));
       hip_check(hipHostGetDevicePointer((void**)&o->hm_dev, o->hm, 0), "hipHostGetDevicePointer");
-      hip_check(hipStreamSynchronize(o->s), "sync");  // counter cleared before any stream uses it
     } catch (...) {
       oc_free(o);
       throw;
@@ -1252,7 +1248,7 @@ static void oc_update_dev(otm_oc* o, hipStream_t s, const float* sens, float* rh
     const int m = std::min(per_check, passes - done);
     for (int i = 0; i < m; i++)
       launch_oc_tree_pass(s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, volratio,
-                          o->vol_tol, o->max_steps, R, nb, o->counter);
+                          o->vol_tol, o->max_steps, R, nb);
     done += m;
     const unsigned tag = ++o->tag;
     launch_oc_apply(s, n, sens, rho, o->st, (float)o->min_rho, (float)o->step, (float)o->damp, o->max_steps,

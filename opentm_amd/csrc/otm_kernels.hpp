@@ -113,11 +113,11 @@ void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes
 void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st);
 constexpr int kOcTree = 5;                   // bisection levels per pass over the design
 constexpr int kOcCand = (1 << kOcTree) - 1;  // midpoints evaluated per pass
-// With counter (a device counter at zero), the pass is one launch whose last
-// block runs the final stage; otherwise the final stage is a second launch.
+// One pass: the candidate sums, then the one-block final stage that walks
+// the tree (a second launch).
 void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const float* rho, OcState* st,
                          float min_rho, float step, float damp, double volratio, double vol_tol, int max_steps,
-                         const Reduce& R, unsigned nblocks, unsigned* counter = nullptr);
+                         const Reduce& R, unsigned nblocks);
 // The OC state as published to the host by k_oc_apply (coherent pinned memory).
 struct OcHost {
   OcState st;

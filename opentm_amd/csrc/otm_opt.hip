@@ -366,29 +366,6 @@ __global__ void __launch_bounds__(kBlock) k_oc_tree_final(OcState* st, Reduce R,
   oc_tree_final_body(st, R, nblocks, inv_n, volratio, tol, max_steps);
 }
 
-// One pass in one launch: the tree sum's blocks only read the design and
-// write their partials, so the last block to finish can publish-and-count
-// cheaply (an agent-scope release writes back little) and run the final
-// stage itself, with the same code and order as k_oc_tree_final.
-__global__ void __launch_bounds__(kBlock) k_oc_tree_pass(long long n, const float* __restrict__ sens,
-                                                         const float* __restrict__ rho, OcState* st, float min_rho,
-                                                         float step, float damp, float c01, int max_steps,
-                                                         Reduce R, unsigned* counter, double inv_n, double volratio,
-                                                         double tol) {
-  if (st->done || st->steps >= max_steps) return;
-  oc_tree_sum_body(n, sens, rho, st, min_rho, step, damp, c01, R);
-  // the partials were stored by wave 0, the wave whose thread 0 releases them
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    const unsigned done = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = done + 1 == gridDim.x;
-  }
-  __syncthreads();
-  if (!last) return;
-  oc_tree_final_body(st, R, (int)gridDim.x, inv_n, volratio, tol, max_steps);
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // rho <- update rule at the final multiplier (in place).
 __global__ void __launch_bounds__(kBlock) k_oc_apply(long long n, const float* __restrict__ sens, float* rho,
                                                      const OcState* __restrict__ st, float min_rho, float step,
@@ -523,13 +500,8 @@ void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int
 }
 void launch_oc_tree_pass(hipStream_t s, long long n, const float* sens, const float* rho, OcState* st,
                          float min_rho, float step, float damp, double volratio, double vol_tol, int max_steps,
-                         const Reduce& R, unsigned nblocks, unsigned* counter) {
+                         const Reduce& R, unsigned nblocks) {
   const float c01 = powf(0.01f, damp);
-  if (counter) {
-    hipLaunchKernelGGL(k_oc_tree_pass, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01,
-                       max_steps, R, counter, 1.0 / (double)n, volratio, vol_tol);
-    return;
-  }
   hipLaunchKernelGGL(k_oc_tree_sum, dim3(nblocks), dim3(kBlock), 0, s, n, sens, rho, st, min_rho, step, damp, c01,
                      max_steps, R);
   hipLaunchKernelGGL(k_oc_tree_final, dim3(1), dim3(kBlock), 0, s, st, R, (int)nblocks, 1.0 / (double)n, volratio,
