@@ -642,6 +642,8 @@ constexpr int kTailStartNodes = 512;                 // largest first tail level
 constexpr int kTailStNodes = 512 + 64 + 8 + 1;       // stencil nodes held in LDS
 constexpr int kTailThreads = 512;  // (1024 would cap registers at 128 and spill)
 constexpr int kTailLdsNodes = 4096;                  // correction nodes held in LDS
+constexpr int kTailBinvNodes = 64;                   // coarsest inverse held in LDS up to this size
+static_assert(kTailStartNodes / 8 <= kTailThreads / 8, "a tail colour step has one node per group of 8 lanes");
 
 struct TailLevels {
   int n;  // levels in the tail; the last is the coarsest
@@ -664,36 +666,49 @@ __device__ inline void tail_stencil(const StRef& S, long long nn, long long v, f
     load_stencil<false>(S, nn, v, a);
 }
 
-// `sweeps` exact 8-colour Gauss-Seidel sweeps on u (LDS) of one tail level
-// with its stencil sa[o*nn + v] in LDS. Work items are (load case, node) of
-// the active colour, so a colour step of a small level keeps three times as
-// many lanes busy as one thread per node would.
-__device__ inline void tail_smooth(const Geom& g, const float* sa, const float* __restrict__ f, float* u, int sweeps,
+// `sweeps` exact 8-colour Gauss-Seidel sweeps on u of one tail level with
+// its stencil sa[o*nn + v] and right-hand side f (all in LDS). A node of
+// the active colour is updated by a group of 8 lanes as in the coarse brick
+// smoother (lane j takes the couplings o = j + 8q, the group adds with
+// group8_sum, lanes 0-2 write the three load cases). A tail level has at
+// most kTailStartNodes nodes, so each group owns one node position per
+// colour: its coordinates are decoded once per call, not per colour step
+// (the runtime divisions and the FP32 division per step made a step 2700
+// cycles long).
+__device__ inline void tail_smooth(const Geom& g, const float* sa, const float* f, float* u, int sweeps,
                                    bool forward) {
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
-  const int cnt = hx * hy * hz;
+  const int cnt = hx * hy * hz;  // nodes per colour, at most blockDim.x / 8
   const int nn = (int)g.nn;
+  const int grp = (int)threadIdx.x >> 3, j = (int)threadIdx.x & 7;
+  const bool act = grp < cnt;
+  const int tx = act ? grp % hx : 0, tr = act ? grp / hx : 0;
+  const int ty = tr % hy, tz = tr / hy;
   for (int s = 0; s < sweeps; s++)
     for (int i = 0; i < 8; i++) {
       const int col = forward ? i : 7 - i;
-      const int cx = col & 1, cy = (col >> 1) & 1, cz = col >> 2;
-      for (int q = threadIdx.x; q < OTM_NRHS * cnt; q += blockDim.x) {
-        const int k = q / cnt, t = q - k * cnt;
-        const int x = 2 * (t % hx) + cx, r = t / hx;
-        const int y = 2 * (r % hy) + cy, z = 2 * (r / hy) + cz;
+      if (act) {
+        const int x = 2 * tx + (col & 1), y = 2 * ty + ((col >> 1) & 1), z = 2 * tz + (col >> 2);
         const int v = x + g.nx * (y + g.ny * z);
-        const int xs[3] = {x == 0 ? g.nx - 1 : x - 1, x, x == g.nx - 1 ? 0 : x + 1};
-        const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
-        const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
-        const float inv = 1.f / sa[13 * nn + v];
-        const float* uk = u + k * nn;
-        float acc = 0.f;
+        float a[OTM_NRHS] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int o = 0; o < 27; o++) {
-          if (o == 13) continue;
-          acc += sa[o * nn + v] * uk[xs[o % 3] + g.nx * (ys[(o / 3) % 3] + g.ny * zs[o / 9])];
+        for (int q = 0; q < 4; q++) {
+          const int o = j + 8 * q;
+          if (o < 27 && o != 13) {
+            const int X = wrapm(x + o % 3 - 1, g.nx), Y = wrapm(y + (o / 3) % 3 - 1, g.ny),
+                      Z = wrapm(z + o / 9 - 1, g.nz);
+            const int w = X + g.nx * (Y + g.ny * Z);
+            const float c = sa[o * nn + v];
+#pragma unroll
+            for (int k = 0; k < OTM_NRHS; k++) a[k] += c * u[k * nn + w];
+          }
         }
-        u[k * nn + v] = (f[k * nn + v] - acc) * inv;
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) a[k] = group8_sum(a[k]);
+        if (j < OTM_NRHS) {
+          const float acc = j == 0 ? a[0] : (j == 1 ? a[1] : a[2]);
+          u[j * nn + v] = (f[j * nn + v] - acc) * __builtin_amdgcn_rcpf(sa[13 * nn + v]);
+        }
       }
       __syncthreads();
     }
@@ -733,11 +748,19 @@ __device__ inline int tail_nodes_before(const TailLevels& T, int l) {
 __device__ inline void tail_vcycle_body(const TailLevels& T) {
   __shared__ float su[OTM_NRHS * kTailLdsNodes];
   __shared__ float sst[27 * kTailStNodes];
+  __shared__ float sf[OTM_NRHS * kTailStNodes];        // right-hand sides of the tail levels
+  __shared__ float sres[OTM_NRHS * kTailStartNodes];   // residual of the level being restricted
+  __shared__ double sbinv[kTailBinvNodes * kTailBinvNodes];
   const int tid = threadIdx.x, bs = blockDim.x;
   const int L = T.n - 1;
+  const int nc = (int)T.g[L].nn;
   auto off = [&](int l) { return OTM_NRHS * tail_nodes_before(T, l); };
   auto soff = [&](int l) { return 27 * tail_nodes_before(T, l); };
-  // stencils of the non-coarsest levels, the values load_stencil gives
+  // Stencils of the non-coarsest levels (the values load_stencil gives), the
+  // first level's right-hand side and a small coarsest inverse: everything
+  // the cycle reads more than once comes from LDS. (With the right-hand
+  // sides, residuals and inverse in global memory, every colour step and
+  // the coarsest product waited for L2: 33 us per cycle at 128^3.)
   for (int l = 0; l < L; l++) {
     const int nn = (int)T.g[l].nn;
     for (int v = tid; v < nn; v += bs) {
@@ -747,27 +770,51 @@ __device__ inline void tail_vcycle_body(const TailLevels& T) {
       for (int o = 0; o < 27; o++) sst[soff(l) + o * nn + v] = a[o];
     }
   }
-  // down: smooth from zero, residual, restriction
+  if (L > 0)
+    for (int i = tid; i < OTM_NRHS * (int)T.g[0].nn; i += bs) sf[i] = T.f0[i];
+  const bool binv_lds = nc <= kTailBinvNodes;
+  if (binv_lds) {
+    // all loads in flight before the stores (a fixed trip count)
+    constexpr int kTrips = (kTailBinvNodes * kTailBinvNodes + kTailThreads - 1) / kTailThreads;
+    double bv[kTrips];
+#pragma unroll
+    for (int r = 0; r < kTrips; r++) {
+      const int i = tid + kTailThreads * r;
+      bv[r] = i < nc * nc ? T.Binv[i] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kTrips; r++) {
+      const int i = tid + kTailThreads * r;
+      if (i < nc * nc) sbinv[i] = bv[r];
+    }
+  }
+  // down: smooth from zero, residual, restriction (into the next level's
+  // right-hand side in LDS)
   for (int l = 0; l < L; l++) {
     const Geom& g = T.g[l];
     float* u = su + off(l);
-    const float* f = l == 0 ? T.f0 : T.f[l];
+    const float* f = sf + off(l);
     for (int i = tid; i < OTM_NRHS * (int)g.nn; i += bs) u[i] = 0.f;
     __syncthreads();
     tail_smooth(g, sst + soff(l), f, u, T.sweeps, true);
-    for (int t = tid; t < (int)g.nn; t += bs) tail_resid_node(g, sst + soff(l), f, u, T.res[l], t);
+    for (int t = tid; t < (int)g.nn; t += bs) tail_resid_node(g, sst + soff(l), f, u, sres, t);
     __syncthreads();
-    for (int t = tid; t < (int)T.g[l + 1].nn; t += bs) restrict_node(g, T.g[l + 1], T.res[l], T.f[l + 1], t);
+    for (int t = tid; t < (int)T.g[l + 1].nn; t += bs) restrict_node(g, T.g[l + 1], sres, sf + off(l + 1), t);
     __syncthreads();
   }
-  // coarsest: u = Binv f (FP64 accumulation)
+  // coarsest: u = Binv f (FP64 accumulation). From LDS, row i is read as
+  // column i (Binv is symmetric): neighbouring lanes read neighbouring
+  // entries, no bank conflicts.
   {
-    const int n = (int)T.g[L].nn;
-    const float* f = L == 0 ? T.f0 : T.f[L];
-    for (int q = tid; q < OTM_NRHS * n; q += bs) {
-      const int k = q / n, i = q - k * n;
+    const float* f = L == 0 ? T.f0 : sf + off(L);
+    for (int q = tid; q < OTM_NRHS * nc; q += bs) {
+      const int k = q / nc, i = q - k * nc;
       double s = 0.0;
-      for (int j = 0; j < n; j++) s += T.Binv[(long long)i * n + j] * (double)f[k * n + j];
+      if (binv_lds) {
+        for (int j = 0; j < nc; j++) s += sbinv[j * nc + i] * (double)f[k * nc + j];
+      } else {
+        for (int j = 0; j < nc; j++) s += T.Binv[(long long)i * nc + j] * (double)f[k * nc + j];
+      }
       su[off(L) + q] = (float)s;
     }
     __syncthreads();
@@ -778,7 +825,7 @@ __device__ inline void tail_vcycle_body(const TailLevels& T) {
     float* u = su + off(l);
     for (int t = tid; t < (int)g.nn; t += bs) prolong_node(g, T.g[l + 1], su + off(l + 1), u, t);
     __syncthreads();
-    tail_smooth(g, sst + soff(l), l == 0 ? T.f0 : T.f[l], u, T.sweeps, false);
+    tail_smooth(g, sst + soff(l), sf + off(l), u, T.sweeps, false);
   }
   for (int i = tid; i < OTM_NRHS * (int)T.g[0].nn; i += bs) T.uout[i] = su[i];
 }
