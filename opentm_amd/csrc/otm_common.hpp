@@ -175,11 +175,31 @@ __device__ inline BlockRange block_range_v(long long n, int lb, int nb) {
 }
 __device__ inline BlockRange block_range(long long n) { return block_range_v(n, xcd_block(), gridDim.x); }
 
-// Wave64 sum through cross-lane shuffles.
+// Lane i's value moved through a DPP pattern (both halves of the double).
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double readlane_f64(double v, int lane) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Wave64 sum (every lane gets it; a fixed order). Inside each row of 16
+// lanes through DPP: swap pairs (quad_perm 1,0,3,2), pairs of pairs
+// (quad_perm 2,3,0,1), the mirrored half row, the mirrored row; then the
+// four row sums are read from lanes 0, 16, 32 and 48. (Cross-lane shuffles
+// went through the LDS crossbar, one round trip per step.)
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // Block-wide sum of NV values; result valid in thread 0. `sh` holds

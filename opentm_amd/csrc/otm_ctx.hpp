@@ -161,6 +161,8 @@ struct otm_ctx {
   std::vector<Pending> pending;
   bool bricks_on = true;   // coarse levels use the LDS brick smoother
   int tail = -1;           // first level of the one-workgroup tail V-cycle
+  int post_dot_mode = -1;  // >= 0: the next V-cycle's fine post-smoother also reduces r.z (that finaliser)
+  bool post_dot_done = false;
   int fallbacks = 0;       // solves that fell back to exact Gauss-Seidel
   long long prof_launches[OTM_PROF_NCLASSES] = {};
   double prof_ms[OTM_PROF_NCLASSES] = {};

@@ -376,9 +376,16 @@ static void vcycle(otm_ctx* c, size_t l) {
     launch_prolong_add(c->s, L.g, C.g, C.u, L.u);
   }
   if (L.brick && l == 0) {
-    // compulsory bytes per node: kappa 4 + u read 12 + f 12 + u written 12
-    ProfScope ps(c, OTM_PROF_FINE_GS, 40.0 * nn);
-    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, nsw, false, false);
+    // r.z of the PCG from the post-smoother's own values when asked for
+    const bool dot = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g);
+    const Reduce RD = c->reducer(dot ? c->post_dot_mode : FIN_STORE);
+    {
+      // compulsory bytes per node: kappa 4 + u read 12 + f 12 + u written 12
+      ProfScope ps(c, OTM_PROF_FINE_GS, 40.0 * nn);
+      launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, nsw, false, false, dot ? &RD : nullptr);
+    }
+    if (dot) launch_fine_brick_dot_final(c->s, L.g, RD);
+    c->post_dot_done = dot;
     std::swap(L.u, L.u2);
     std::swap(c->z, c->z2);
   } else if (L.brick) {
@@ -456,10 +463,13 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
       ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n);
       launch_sub_mean3(c->s, n, c->r, c->pcg);
     }
+    c->post_dot_mode = FIN_RZ_FIRST;
+    c->post_dot_done = false;
     vcycle(c, 0);
+    c->post_dot_mode = -1;
     {
       ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 12.0 * (double)n + 36.0 * (double)n);
-      launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ_FIRST), nb);
+      if (!c->post_dot_done) launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ_FIRST), nb);
       launch_fill(c->s, c->p, 3 * n, 0.f);
       launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
     }
@@ -484,10 +494,13 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
         return false;
       }
       if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
+      c->post_dot_mode = FIN_RZ;
+      c->post_dot_done = false;
       vcycle(c, 0);
+      c->post_dot_mode = -1;
       {
         ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 36.0 * (double)n);
-        launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ), nb);
+        if (!c->post_dot_done) launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ), nb);
         launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
       }
     }
@@ -510,6 +523,7 @@ static void pcg_solve(otm_ctx* c) {
   const unsigned nb = red_blocks_for(n);
   int its[3] = {0, 0, 0};
   int it = 0;
+  c->post_dot_mode = -1;
   hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
   bool fresh = false;
   if (!pcg_iterate(c, its, &it, &fresh)) {

@@ -1,5 +1,7 @@
 // otm_fine.hip -- single-cell launches of the matrix-free fine-level
 // kernels; the device bodies are in otm_fine_dev.hpp.
+#include <stdexcept>
+
 #include "otm_fine_dev.hpp"
 
 namespace otm {
@@ -66,22 +68,32 @@ __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, 
 template <int BX>
 __global__ void __launch_bounds__(BX * kFineTR) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa, const float* __restrict__ f,
-                    const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward, int zero_in) {
-  fine_brick_body<BX>(g, fc, kappa, f, uin, uout, sweeps, forward, zero_in, xcd_block());
+                    const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward, int zero_in,
+                    int dot, Reduce RD) {
+  fine_brick_body<BX>(g, fc, kappa, f, uin, uout, sweeps, forward, zero_in, xcd_block(), dot, RD);
 }
 
 int fine_brick_x(const Geom& g) { return fine_tile_x(g); }
 
+bool fine_brick_dot_ok(const Geom& g) { return fine_brick_x(g) == 32 && 32 * kFineTR == kBlock; }
+
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
-                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in) {
+                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in,
+                              const Reduce* dot) {
   const int bxw = fine_brick_x(g);
   const unsigned nb = (unsigned)((g.nx / bxw) * (g.ny / kFineBY) * (g.nz / kFineBZ));
+  if (dot && !fine_brick_dot_ok(g)) throw std::invalid_argument("r.z in the smoother needs 32-wide tiles");
+  const Reduce RD = dot ? *dot : Reduce{};
   if (bxw == 32)
     hipLaunchKernelGGL(k_fine_smooth_brick<32>, dim3(nb), dim3(32 * kFineTR), 0, s, g, fc, kappa, f, uin, uout,
-                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0);
+                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0, dot ? 1 : 0, RD);
   else
     hipLaunchKernelGGL(k_fine_smooth_brick<16>, dim3(nb), dim3(16 * kFineTR), 0, s, g, fc, kappa, f, uin, uout,
-                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0);
+                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0, 0, RD);
+}
+
+void launch_fine_brick_dot_final(hipStream_t s, const Geom& g, const Reduce& R) {
+  launch_reduce_final(s, R, OTM_NRHS, (unsigned)((g.nx / 32) * (g.ny / kFineBY) * (g.nz / kFineBZ)));
 }
 
 // ---- launchers ---------------------------------------------------------

@@ -699,10 +699,16 @@ __device__ inline void fine_residual_tile_body(const Geom& g, const FineConsts& 
 // neighbours with one element's conductivity (one-axis neighbours have a
 // zero coefficient), so an update is 20 LDS reads and 20 FMAs per load case
 // plus 12 adds shared by the three.
+//
+// With dot set (post-smoother of a PCG preconditioner application, 256
+// threads), the block also forms its partial sums of f.u over its nodes
+// with the final u (r.z: f is the PCG residual, u the preconditioned
+// residual), from the values it already holds, into slot b of RD.
 template <int BX, int BY = kFineBY, int BZ = kFineBZ, int TR = kFineTR>
 __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
                                        const float* __restrict__ f, const float* __restrict__ uin,
-                                       float* __restrict__ uout, int sweeps, int forward, int zero_in, int b) {
+                                       float* __restrict__ uout, int sweeps, int forward, int zero_in, int b,
+                                       int dot = 0, const Reduce& RD = Reduce{}) {
   using T = FineTile<BX, BY, BZ, TR>;
   constexpr int HX = T::HX, HY = T::HY, KX = T::KX, KY = T::KY;
   constexpr int R = (BY / 2) * (BZ / 2);  // x-rows of one colour
@@ -782,28 +788,49 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
         }
         const float ks = (wxy[0] + wxy[1]) + (wxy[2] + wxy[3]);
         const float inv = __builtin_amdgcn_rcpf(4.f * c * ks);
-        const int c0 = (lx + 1) + DY * (ly + 1) + DZ * (lz + 1);
+        // The neighbours are addressed from the lowest one (cb) so that
+        // every offset, load cases included, is a non-negative constant
+        // that the LDS instructions take as an immediate; the empty asm
+        // keeps the compiler from folding cb back into c0 and negative
+        // offsets (each of which cost an address add per colour step).
+        constexpr int LO = 1 + DY + DZ;
+        int cb = (lx + 1) + DY * (ly + 1) + DZ * (lz + 1) - LO;
+        asm volatile("" : "+v"(cb));
 #pragma unroll
         for (int k = 0; k < OTM_NRHS; k++) {
-          const float* U = su[k];
+          const float* U = &su[k][0] + cb;
           float acc = 0.f;
 #pragma unroll
           for (int q = 0; q < 4; q++) {
             const int s0 = 2 * (q & 1) - 1, s1 = 2 * (q >> 1) - 1;
-            acc += wxy[q] * U[c0 + s0 + DY * s1];
-            acc += wxz[q] * U[c0 + s0 + DZ * s1];
-            acc += wyz[q] * U[c0 + DY * s0 + DZ * s1];
+            acc += wxy[q] * U[LO + s0 + DY * s1];
+            acc += wxz[q] * U[LO + s0 + DZ * s1];
+            acc += wyz[q] * U[LO + DY * s0 + DZ * s1];
           }
 #pragma unroll
           for (int e = 0; e < 8; e++) {
             const int sx = 2 * (e & 1) - 1, sy = 2 * ((e >> 1) & 1) - 1, sz = 2 * (e >> 2) - 1;
-            acc += ke[e] * U[c0 + sx + DY * sy + DZ * sz];
+            acc += ke[e] * U[LO + sx + DY * sy + DZ * sz];
           }
-          su[k][c0] = (fv[col][h][k] + c * acc) * inv;
+          (&su[k][0] + cb)[LO] = (fv[col][h][k] + c * acc) * inv;
         }
       }
       __syncthreads();
     }
+  if (dot) {
+    double acc[OTM_NRHS] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int col = 0; col < 4; col++)
+#pragma unroll
+      for (int h = 0; h < H; h++) {
+        const int rr = ty + TR * h;
+        const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
+        const int c0 = (lx + 1) + DY * (ly + 1) + DZ * (lz + 1);
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) acc[k] += (double)fv[col][h][k] * (double)su[k][c0];
+      }
+    reduce_block_at<OTM_NRHS>(acc, RD, b);
+  }
   static_assert((BY * BZ) % TR == 0, "thread rows must divide the tile's rows");
 #pragma unroll
   for (int j = 0; j < BY * BZ / TR; j++) {

@@ -24,9 +24,15 @@ void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc,
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6);
 // Fine brick smoother (BX x kFineBY x kFineBZ bricks, frozen halo, result in uout);
 // fine_brick_x returns BX (32 or 16) or 0 when the cell does not tile.
+// With dot (fine_brick_dot_ok cells), the blocks also store their partial
+// sums of f.uout for that reducer (the PCG's r.z after a preconditioner
+// application); launch_fine_brick_dot_final then finishes the reduction.
 int fine_brick_x(const Geom& g);
+bool fine_brick_dot_ok(const Geom& g);
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
-                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in);
+                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in,
+                              const Reduce* dot = nullptr);
+void launch_fine_brick_dot_final(hipStream_t s, const Geom& g, const Reduce& R);
 
 // ---- coarse levels (stored 27-point stencils, otm_coarse.hip) ----
 // Stored stencil of a coarse level: FP32 (f16 == nullptr) or FP16 couplings
