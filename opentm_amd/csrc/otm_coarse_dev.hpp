@@ -267,11 +267,23 @@ __device__ inline float stencil_coef(const StRef& S, long long nn, long long v, 
 // 0-2 then write load cases 0-2. With one node per thread
 // only 16 lanes of half the waves worked in a colour step, each through 78
 // dependent multiply-adds.
+// LDS of the brick smoother, one allocation for both stencil precisions
+// (a batched launch instantiates both bodies in one kernel).
+struct StBrickLds {
+  float su[OTM_NRHS][kHalo * kHalo * kHalo];  // u of the brick and its halo
+  float scf[kBrickThreads * 27];              // stencils, colour-major node order
+};
+__device__ inline StBrickLds& st_brick_lds() {
+  __shared__ StBrickLds lds;
+  return lds;
+}
+
 template <bool HALF>
 __device__ inline void st_brick_body(const Geom& g, const StRef& S, const float* __restrict__ f,
                                      const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward,
                                      int zero_in, int b) {
-  __shared__ float su[OTM_NRHS][kHalo * kHalo * kHalo];
+  auto& su = st_brick_lds().su;
+  float* scf = st_brick_lds().scf;
   const int bx = (g.nx + kBrick - 1) / kBrick, by = (g.ny + kBrick - 1) / kBrick;
   const int ox = (b % bx) * kBrick, oy = ((b / bx) % by) * kBrick, oz = (b / (bx * by)) * kBrick;
   const int t = threadIdx.x;
@@ -312,27 +324,45 @@ __device__ inline void st_brick_body(const Geom& g, const StRef& S, const float*
     slot[q] = o < 27 && o != 13;
     doff[q] = slot[q] ? (o % 3 - 1) + kHalo * (((o / 3) % 3 - 1) + kHalo * (o / 9 - 1)) : 0;
   }
+  // Stencils: each thread loads its own node's 27 entries (neighbouring
+  // threads read neighbouring entries; load_stencil's values, FP16 diagonal
+  // included) into LDS in colour-major node order, from where each lane
+  // takes its slots for every colour. (Loaded straight into the slots, a
+  // wave's 64 loads touched 8 stencil arrays at stride-2 nodes.)
   float cf[8][4], inv[8], fv[8];
+  {
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    const int x = ox + lx, y = oy + ly, z = oz + lz;
+    const bool act = x < g.nx && y < g.ny && z < g.nz;
+    float a[27];
+    if (act) {
+      load_stencil<HALF>(S, nn, (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z), a);
+    } else {
+#pragma unroll
+      for (int o = 0; o < 27; o++) a[o] = 0.f;
+    }
+    const int c = (lx & 1) + 2 * (ly & 1) + 4 * (lz & 1), mm = (lx >> 1) + 4 * (ly >> 1) + 16 * (lz >> 1);
+    float* d = scf + (c * 64 + mm) * 27;
+#pragma unroll
+    for (int o = 0; o < 27; o++) d[o] = a[o];
+  }
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     const int x = ox + 2 * (m & 3) + (c & 1), y = oy + 2 * ((m >> 2) & 3) + ((c >> 1) & 1),
               z = oz + 2 * (m >> 4) + (c >> 2);
     const bool act = x < g.nx && y < g.ny && z < g.nz;
     const long long v = act ? (long long)x + (long long)g.nx * ((long long)y + (long long)g.ny * z) : 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) cf[c][q] = act && slot[q] ? stencil_coef<HALF>(S, nn, v, j + 8 * q) : 0.f;
-    if constexpr (!HALF) inv[c] = act ? 1.f / S.f32[13 * nn + v] : 0.f;
     fv[c] = act && j < OTM_NRHS ? f[j * nn + v] : 0.f;
   }
-  if constexpr (HALF) {
-    // the diagonal is minus the sum of the stored couplings (rows sum to zero)
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const float sd = group8_sum((cf[c][0] + cf[c][1]) + (cf[c][2] + cf[c][3]));
-      inv[c] = sd != 0.f ? -1.f / sd : 0.f;
-    }
-  }
   __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const float* d = scf + (c * 64 + m) * 27;
+#pragma unroll
+    for (int q = 0; q < 4; q++) cf[c][q] = slot[q] ? d[j + 8 * q] : 0.f;
+    const float dg = d[13];
+    inv[c] = dg != 0.f ? 1.f / dg : 0.f;  // zero for nodes past the level's edge
+  }
   auto step = [&](const int c) {
     const int c0 = (2 * (m & 3) + (c & 1) + 1) + kHalo * ((2 * ((m >> 2) & 3) + ((c >> 1) & 1) + 1) +
                                                           kHalo * (2 * (m >> 4) + (c >> 2) + 1));
