@@ -142,6 +142,7 @@ struct Reduce {
 
 constexpr int kBlock = 256;       // threads per block for streaming kernels
 constexpr int kMaxReduce = 9;     // values per reduction
+constexpr int kMaxReduceBlocks = 2048;  // partial-sum slots of a reduction (the largest grid that reduces)
 constexpr int kWaves = kBlock / 64;
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, and

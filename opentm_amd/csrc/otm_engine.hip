@@ -249,7 +249,7 @@ static void build_hierarchy(otm_ctx* c) {
   }
   if (c->lv.back().g.nn > kTailLdsNodes) fail(OTM_E_UNSUPPORTED, "coarsest level too large for the tail");
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
-  c->partials = dalloc<double>(2048 * kMaxReduce);
+  c->partials = dalloc<double>((size_t)kMaxReduceBlocks * kMaxReduce);
   c->red_out = dalloc<double>(16);
   c->pcg = dalloc<PcgScalars>(1);
   hip_check(hipMemsetAsync(c->pcg, 0, sizeof(PcgScalars), c->s), "hipMemset pcg");

@@ -75,7 +75,11 @@ k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa, cons
 
 int fine_brick_x(const Geom& g) { return fine_tile_x(g); }
 
-bool fine_brick_dot_ok(const Geom& g) { return fine_brick_x(g) == 32 && 32 * kFineTR == kBlock; }
+// r.z in the smoother: 256-thread tiles, one partial-sum slot per tile
+// (the reduction buffers hold kTileRedBlocks slots)
+bool fine_brick_dot_ok(const Geom& g) {
+  return fine_brick_x(g) == 32 && 32 * kFineTR == kBlock && fine_tiles(g, 32) <= kTileRedBlocks;
+}
 
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
                               const float* uin, float* uout, int sweeps, bool forward, bool zero_in,

@@ -437,7 +437,7 @@ __host__ __device__ inline int fine_tiles(const Geom& g, int bx) {
 }
 // Workgroups of the tiled operator apply: one tile each up to the partial-sum
 // capacity of a reduction, several tiles each beyond.
-constexpr int kTileRedBlocks = 2048;
+constexpr int kTileRedBlocks = kMaxReduceBlocks;
 __host__ __device__ inline unsigned fine_apply_tile_blocks(const Geom& g, int bx) {
   const int t = fine_tiles(g, bx);
   return (unsigned)(t > kTileRedBlocks ? kTileRedBlocks : t);

@@ -219,6 +219,33 @@ def test_full_size_plain_reaches_tolerance():
     assert max(its2) <= max(its1)
 
 
+@pytest.mark.parametrize("refine,fp16", [(0, 0), (1, 1)])
+def test_repeated_cell_same_tensor(refine, fp16):
+    """Periodicity: two copies of a 128^3 cell stacked along z (4096 fine
+    tiles, more than the reduction slots of the fused r.z, so the separate
+    dot product runs) give twice the tensor of one copy: the same periodic
+    solution, and the element side stays 1/reso[0] while the cell holds
+    twice the elements (|Y| = 1, homogenization.cu:207)."""
+    n = 128
+    say("repeated cell", n, "refine", refine, "fp16", fp16)
+    k = wl.simp(wl.plain_density(n, 11))
+    rtol = 1e-8 if refine else 1e-6
+    hs = []
+    for shape in [(n, n, n), (n, n, 2 * n)]:
+        cfg = otm.HomoConfig(shape, (1.0, 1e-3))
+        cfg.femRelThres = rtol
+        cfg.fp64_refine = refine
+        cfg.fp16_stencils = fp16
+        hs.append(otm.Homogenization_H(cfg))
+    hs[0].update(k)
+    hs[1].update(np.ascontiguousarray(np.concatenate([k, k], axis=0)))
+    C1, C2 = hs[0].heatMatrix(), hs[1].heatMatrix()
+    its, rr = hs[1].solve_stats()
+    say("  stacked its", its, "relres", rr, "diff", rel(C2, 2.0 * C1))
+    assert np.all(np.isfinite(C2)) and max(rr) <= rtol
+    assert rel(C2, 2.0 * C1) < (1e-7 if refine else 2e-5)
+
+
 def test_full_size_closed_forms():
     n = 128
     say("closed forms", n)
