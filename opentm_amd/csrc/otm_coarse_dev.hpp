@@ -78,13 +78,25 @@ __device__ inline void st_resid_node(const Geom& g, const StRef& S, const float*
   const int ys[3] = {y == 0 ? g.ny - 1 : y - 1, y, y == g.ny - 1 ? 0 : y + 1};
   const int zs[3] = {z == 0 ? g.nz - 1 : z - 1, z, z == g.nz - 1 ? 0 : z + 1};
   float a[27];
-  load_stencil<HALF>(S, g.nn, t, a);
+  if constexpr (HALF) {
+    load_stencil<HALF>(S, g.nn, t, a);
+  } else {
+#pragma unroll
+    for (int o = 0; o < 27; o++) a[o] = ld_off(S.f32 + (size_t)o * (size_t)g.nn, 4u * (unsigned)t);
+  }
+  // neighbour byte offsets from wave-uniform bases in 32 bits (64-bit
+  // per-neighbour addresses were half of this kernel's instructions)
+  const unsigned nx = (unsigned)g.nx, nxy = (unsigned)g.nx * (unsigned)g.ny;
+  unsigned off[27];
+#pragma unroll
+  for (int o = 0; o < 27; o++)
+    off[o] = 4u * ((unsigned)xs[o % 3] + nx * (unsigned)ys[(o / 3) % 3] + nxy * (unsigned)zs[o / 9]);
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) {
-    const float* uk = u + k * g.nn;
+    const float* uk = u + (size_t)k * (size_t)g.nn;
     float s = 0.f;
 #pragma unroll
-    for (int o = 0; o < 27; o++) s += a[o] * uk[lin(g, xs[o % 3], ys[(o / 3) % 3], zs[o / 9])];
+    for (int o = 0; o < 27; o++) s += a[o] * ld_off(uk, off[o]);
     res[k * g.nn + t] = f[k * g.nn + t] - s;
   }
 }
@@ -99,18 +111,23 @@ __device__ inline void restrict_node(const Geom& fg, const Geom& cg, const float
   const int ys[3] = {y == 0 ? fg.ny - 1 : y - 1, y, y + 1};
   const int zs[3] = {z == 0 ? fg.nz - 1 : z - 1, z, z + 1};
   const float w1[3] = {0.5f, 1.f, 0.5f};
+  // byte offsets in 32 bits from wave-uniform bases
+  unsigned rb[9];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) rb[b + 3 * c] = (unsigned)fg.nx * ((unsigned)ys[b] + (unsigned)fg.ny * (unsigned)zs[c]);
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) {
-    const float* rk = res + k * fg.nn;
+    const float* rk = res + (size_t)k * (size_t)fg.nn;
     float s = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
       for (int b = 0; b < 3; b++) {
         float row = 0.f;
-        const long long rb = (long long)fg.nx * ((long long)ys[b] + (long long)fg.ny * zs[c]);
 #pragma unroll
-        for (int a = 0; a < 3; a++) row += w1[a] * rk[rb + xs[a]];
+        for (int a = 0; a < 3; a++) row += w1[a] * ld_off(rk, 4u * (rb[b + 3 * c] + (unsigned)xs[a]));
         s += w1[b] * w1[c] * row;
       }
     fc[k * cg.nn + t] = s;

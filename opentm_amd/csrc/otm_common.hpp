@@ -22,6 +22,13 @@ struct Geom {
 
 __host__ __device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Value at byte offset `off` from a wave-uniform base: the load takes the
+// base in scalar registers and one 32-bit offset register (a lattice has at
+// most 2^30 nodes, so a field's byte offsets fit 32 bits).
+__device__ inline float ld_off(const float* __restrict__ base, unsigned off) {
+  return *(const float*)((const char*)base + off);
+}
+
 // (x, y, z) of lexicographic index t on an nx x ny x * grid, x fastest.
 // Indices stay below 2^31 (every axis is at most 1024), so two 32-bit
 // divisions replace the much longer 64-bit division sequences.
@@ -67,7 +74,7 @@ __device__ inline float prolong_sum(const Geom& cg, const float* __restrict__ ck
       for (int a = 0; a < 2; a++) {
         const float w = P.wx[a] * P.wy[b] * P.wz[c];
         if (w != 0.f)
-          s += w * ck[(long long)P.X[a] + (long long)cg.nx * ((long long)P.Y[b] + (long long)cg.ny * P.Z[c])];
+          s += w * ld_off(ck, 4u * ((unsigned)P.X[a] + (unsigned)cg.nx * ((unsigned)P.Y[b] + (unsigned)cg.ny * (unsigned)P.Z[c])));
       }
   return s;
 }

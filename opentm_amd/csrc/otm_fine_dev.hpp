@@ -478,10 +478,6 @@ struct TileStage {
   float u[OTM_NRHS][UJ], uh[OTM_NRHS][UH];
 };
 
-// value at byte offset `off` from a wave-uniform base
-__device__ inline float ld_off(const float* __restrict__ base, unsigned off) {
-  return *(const float*)((const char*)base + off);
-}
 // first thread row of this wave (wave-uniform) and the lane's row within it
 template <int BX>
 __device__ inline int wave_first_row() {
