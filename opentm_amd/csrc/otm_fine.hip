@@ -45,6 +45,16 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConst
   fine_init_residual_body<XT>(g, fc, kappa, x_, xlo, rvec, R, xcd_block(), gridDim.x);
 }
 
+// grid (nblocks, 3): one load case per block row
+template <typename XT>
+__global__ void __launch_bounds__(kBlock) k_fine_init_residual_case(Geom g, FineConsts fc,
+                                                                    const float* __restrict__ kappa,
+                                                                    const XT* __restrict__ x_,
+                                                                    const float* __restrict__ xlo,
+                                                                    float* __restrict__ rvec, Reduce R) {
+  fine_init_residual_case_body<XT>(g, fc, kappa, x_, xlo, rvec, R, xcd_block(), gridDim.x, (int)blockIdx.y);
+}
+
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                          const float* __restrict__ chi, Reduce R) {
   fine_kappa_h_body(g, fc, kappa, chi, R, xcd_block(), gridDim.x);
@@ -142,7 +152,8 @@ void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, c
 // waves per SIMD) measured 25% slower than this one (three waves per SIMD).
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const float* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual<float>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x, xlo, r, R);
+  hipLaunchKernelGGL(k_fine_init_residual_case<float>, dim3(nblocks, OTM_NRHS), dim3(kBlock), 0, s, g, fc, kappa, x,
+                     xlo, r, R);
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 

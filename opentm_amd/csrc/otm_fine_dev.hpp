@@ -186,23 +186,27 @@ __device__ inline double far_sum_d(const Geom& g, const NodeIdx& n, const NodeEl
   v.mpp = u[rPp + n.xm]; v._0pp = u[rPp + n.x]; v.ppp = u[rPp + n.xp];
   return far_combine_d(E, v);
 }
-// The same for an iterate held as an FP32 pair (hi + lo).
+// The same for an iterate held as an FP32 pair (hi + lo); both arrays are
+// read at the same 32-bit byte offsets from wave-uniform bases.
 __device__ inline double far_sum_pair(const Geom& g, const NodeIdx& n, const NodeElems& E,
                                       const float* __restrict__ u, const float* __restrict__ ul) {
-  const long long rMm = rowb(g, n.ym, n.zm), rMp = rowb(g, n.ym, n.zp);
-  const long long rPm = rowb(g, n.yp, n.zm), rPp = rowb(g, n.yp, n.zp);
-  const long long rM0 = rowb(g, n.ym, n.z), rP0 = rowb(g, n.yp, n.z);
-  const long long r0m = rowb(g, n.y, n.zm), r0p = rowb(g, n.y, n.zp);
-  auto at = [&](long long i) { return (double)u[i] + (double)ul[i]; };
+  const unsigned nx = (unsigned)g.nx, nxy = (unsigned)g.nx * (unsigned)g.ny;
+  auto row = [&](int y, int z) { return 4u * (nx * (unsigned)y + nxy * (unsigned)z); };
+  const unsigned rMm = row(n.ym, n.zm), rMp = row(n.ym, n.zp);
+  const unsigned rPm = row(n.yp, n.zm), rPp = row(n.yp, n.zp);
+  const unsigned rM0 = row(n.ym, n.z), rP0 = row(n.yp, n.z);
+  const unsigned r0m = row(n.y, n.zm), r0p = row(n.y, n.zp);
+  const unsigned xm = 4u * (unsigned)n.xm, xp = 4u * (unsigned)n.xp, x0 = 4u * (unsigned)n.x;
+  auto at = [&](unsigned i) { return (double)ld_off(u, i) + (double)ld_off(ul, i); };
   Far20d v;
-  v.mm0 = at(rM0 + n.xm); v.pm0 = at(rM0 + n.xp);
-  v.mp0 = at(rP0 + n.xm); v.pp0 = at(rP0 + n.xp);
-  v.m0m = at(r0m + n.xm); v.p0m = at(r0m + n.xp);
-  v.m0p = at(r0p + n.xm); v.p0p = at(r0p + n.xp);
-  v.mmm = at(rMm + n.xm); v._0mm = at(rMm + n.x); v.pmm = at(rMm + n.xp);
-  v.mmp = at(rMp + n.xm); v._0mp = at(rMp + n.x); v.pmp = at(rMp + n.xp);
-  v.mpm = at(rPm + n.xm); v._0pm = at(rPm + n.x); v.ppm = at(rPm + n.xp);
-  v.mpp = at(rPp + n.xm); v._0pp = at(rPp + n.x); v.ppp = at(rPp + n.xp);
+  v.mm0 = at(rM0 + xm); v.pm0 = at(rM0 + xp);
+  v.mp0 = at(rP0 + xm); v.pp0 = at(rP0 + xp);
+  v.m0m = at(r0m + xm); v.p0m = at(r0m + xp);
+  v.m0p = at(r0p + xm); v.p0p = at(r0p + xp);
+  v.mmm = at(rMm + xm); v._0mm = at(rMm + x0); v.pmm = at(rMm + xp);
+  v.mmp = at(rMp + xm); v._0mp = at(rMp + x0); v.pmp = at(rMp + xp);
+  v.mpm = at(rPm + xm); v._0pm = at(rPm + x0); v.ppm = at(rPm + xp);
+  v.mpp = at(rPp + xm); v._0pp = at(rPp + x0); v.ppp = at(rPp + xp);
   return far_combine_d(E, v);
 }
 
@@ -324,6 +328,61 @@ __device__ inline void fine_init_residual_body(const Geom& g, const FineConsts& 
     }
   }
   reduce_block_at<3 * OTM_NRHS>(acc, R, lb);
+}
+
+// fine_init_residual_body for load case k alone (a grid of nb x 3 blocks):
+// block (lb, k) covers the nodes of block lb of the one-dimensional form
+// in the same order, so r and the partial sums carry the same bits; it
+// writes values k, 3 + k and 6 + k of partial slot lb. With one load case
+// per thread, the gathered neighbours of three cases are not live together
+// (222 registers and two waves per SIMD for the three-case form).
+template <typename XT>
+__device__ inline void fine_init_residual_case_body(const Geom& g, const FineConsts& fc,
+                                                    const float* __restrict__ kappa, const XT* __restrict__ x_,
+                                                    const float* __restrict__ xlo, float* __restrict__ rvec,
+                                                    const Reduce& R, int lb, int nb, int k) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const double c = (double)fc.c;
+  float fe[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) fe[e] = k == 0 ? fc.femu[7 - e][0] : (k == 1 ? fc.femu[7 - e][1] : fc.femu[7 - e][2]);
+  const XT* xk = x_ + (size_t)k * (size_t)g.nn;
+  const float* xl = xlo ? xlo + (size_t)k * (size_t)g.nn : nullptr;
+  const BlockRange br = block_range_v(g.nn, lb, nb);
+  for (long long t = br.b + threadIdx.x; t < br.e; t += blockDim.x) {
+    // 64-bit decode as in fine_init_residual_body
+    const int x = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    const int y = (int)(r % g.ny), z = (int)(r / g.ny);
+    const NodeIdx n = node_idx(g, x, y, z);
+    const NodeElems E = load_elems(g, n, kappa);
+    double ksum = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ksum += (double)E.k[e];
+    double b = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) b += (double)E.k[e] * (double)fe[e];
+    double kx;
+    if constexpr (sizeof(XT) == 4) {
+      if (xl)
+        kx = c * (4.0 * ksum * ((double)xk[n.v] + (double)xl[n.v]) - far_sum_pair(g, n, E, xk, xl));
+      else
+        kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
+    } else {
+      kx = c * (4.0 * ksum * (double)xk[n.v] - far_sum_d(g, n, E, xk));
+    }
+    const double rv = b - kx;
+    rvec[(size_t)k * (size_t)g.nn + n.v] = (float)rv;
+    acc[0] += b * b;
+    acc[1] += rv * rv;
+    acc[2] += rv;
+  }
+  __shared__ double sh[3 * kWaves];
+  block_sum<3>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) R.partials[(size_t)lb * (3 * OTM_NRHS) + k + OTM_NRHS * i] = acc[i];
+  }
 }
 
 // Element energies (homogenization.cu:13-153 and :255-412). For element e
