@@ -722,8 +722,8 @@ __device__ inline void tail_stencil(const StRef& S, long long nn, long long v, f
 // colour: its coordinates are decoded once per call, not per colour step
 // (the runtime divisions and the FP32 division per step made a step 2700
 // cycles long).
-__device__ inline void tail_smooth(const Geom& g, const float* sa, const float* f, float* u, int sweeps,
-                                   bool forward) {
+template <bool FORWARD>
+__device__ inline void tail_smooth(const Geom& g, const float* sa, const float* f, float* u, int sweeps) {
   const int hx = g.nx >> 1, hy = g.ny >> 1, hz = g.nz >> 1;
   const int cnt = hx * hy * hz;  // nodes per colour, at most blockDim.x / 8
   const int nn = (int)g.nn;
@@ -732,8 +732,9 @@ __device__ inline void tail_smooth(const Geom& g, const float* sa, const float* 
   const int tx = act ? grp % hx : 0, tr = act ? grp / hx : 0;
   const int ty = tr % hy, tz = tr / hy;
   for (int s = 0; s < sweeps; s++)
+#pragma unroll
     for (int i = 0; i < 8; i++) {
-      const int col = forward ? i : 7 - i;
+      const int col = FORWARD ? i : 7 - i;
       if (act) {
         const int x = 2 * tx + (col & 1), y = 2 * ty + ((col >> 1) & 1), z = 2 * tz + (col >> 2);
         const int v = x + g.nx * (y + g.ny * z);
@@ -843,7 +844,7 @@ __device__ inline void tail_vcycle_body(const TailLevels& T) {
     const float* f = sf + off(l);
     for (int i = tid; i < OTM_NRHS * (int)g.nn; i += bs) u[i] = 0.f;
     __syncthreads();
-    tail_smooth(g, sst + soff(l), f, u, T.sweeps, true);
+    tail_smooth<true>(g, sst + soff(l), f, u, T.sweeps);
     for (int t = tid; t < (int)g.nn; t += bs) tail_resid_node(g, sst + soff(l), f, u, sres, t);
     __syncthreads();
     for (int t = tid; t < (int)T.g[l + 1].nn; t += bs) restrict_node(g, T.g[l + 1], sres, sf + off(l + 1), t);
@@ -872,7 +873,7 @@ __device__ inline void tail_vcycle_body(const TailLevels& T) {
     float* u = su + off(l);
     for (int t = tid; t < (int)g.nn; t += bs) prolong_node(g, T.g[l + 1], su + off(l + 1), u, t);
     __syncthreads();
-    tail_smooth(g, sst + soff(l), sf + off(l), u, T.sweeps, false);
+    tail_smooth<false>(g, sst + soff(l), sf + off(l), u, T.sweeps);
   }
   for (int i = tid; i < OTM_NRHS * (int)T.g[0].nn; i += bs) T.uout[i] = su[i];
 }
