@@ -195,6 +195,8 @@ struct OpStResid {
     const long long t = (long long)lb * kBlock + threadIdx.x;
     if (C.st[l].f16)
       st_residual_body<true>(C.g[l], C.st[l], C.f[l], ucur(C, e.par, l), C.res[l], t);
+    else if (st_resid8(C.g[l], false))  // 8 lanes per node, as the single-cell launch (8 x the blocks)
+      st_residual8_body(C.g[l], C.st[l].f32, C.f[l], ucur(C, e.par, l), C.res[l], t);
     else
       st_residual_body<false>(C.g[l], C.st[l], C.f[l], ucur(C, e.par, l), C.res[l], t);
   }
@@ -677,7 +679,10 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
       add_op(b, S, OpFineResid{}, residb);
     } else {
       add_op(b, S, OpStBrick{l, 0}, [l](const Member& m) { return (long long)brick_count(m.c->lv[l].g); });
-      add_op(b, S, OpStResid{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l].g.nn); });
+      add_op(b, S, OpStResid{l}, [l](const Member& m) {
+        const Level& L = m.c->lv[l];
+        return (long long)nblk(st_resid8(L.g, L.st16 != nullptr) ? 8 * L.g.nn : L.g.nn);
+      });
     }
     add_op(b, S, OpRestrict{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l + 1].g.nn); });
   }

@@ -33,6 +33,12 @@ __global__ void __launch_bounds__(kBlock) k_st_residual(Geom g, StRef S, const f
   st_residual_body<HALF>(g, S, f, u, res, gtid());
 }
 
+__global__ void __launch_bounds__(kBlock) k_st_residual8(Geom g, const float* __restrict__ st,
+                                                         const float* __restrict__ f, const float* __restrict__ u,
+                                                         float* __restrict__ res) {
+  st_residual8_body(g, st, f, u, res, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 __global__ void __launch_bounds__(kBlock) k_stencil_absmax(Geom g, const float* __restrict__ st,
                                                            unsigned int* __restrict__ bits) {
   stencil_absmax_body(g, st, bits, xcd_block(), gridDim.x);
@@ -297,6 +303,8 @@ void launch_st_gs_color(hipStream_t s, const Geom& g, const StRef& S, const floa
 void launch_st_residual(hipStream_t s, const Geom& g, const StRef& S, const float* f, const float* u, float* res) {
   if (S.f16)
     hipLaunchKernelGGL(k_st_residual<true>, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, S, f, u, res);
+  else if (st_resid8(g, false))
+    hipLaunchKernelGGL(k_st_residual8, dim3(nblk(8 * g.nn)), dim3(kBlock), 0, s, g, S.f32, f, u, res);
   else
     hipLaunchKernelGGL(k_st_residual<false>, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, S, f, u, res);
 }

@@ -160,6 +160,46 @@ __device__ inline void st_gs_color_body(const Geom& g, const StRef& S, const flo
   st_gs_node<HALF>(g, S, f, u, x, y, z);
 }
 
+// Levels up to this many nodes take the 8-lanes-per-node residual (latency
+// bound: 10 -> 3.4-5.4 us at 16^3 and 32^3); larger ones one node per thread
+// (at 64^3 the 8-lane form's stencil reads coalesce worse: 12.7 -> 26 us).
+constexpr long long kResid8MaxNodes = 65536;
+__host__ __device__ inline bool st_resid8(const Geom& g, bool half) { return !half && g.nn <= kResid8MaxNodes; }
+
+// res = f - A u with 8 lanes per node (FP32 stencils): lane j of a group
+// takes the entries o = j, j + 8, j + 16, j + 24 (o < 27) with their three
+// load cases' neighbours, the group adds its partial sums (group8_sum) and
+// lanes 0-2 write. Thread t serves node t / 8 (a whole group is past the
+// level when its node is). One node per thread left small levels waiting
+// on its 111 loads one after another (10 us at 16^3 and 32^3).
+__device__ inline void st_residual8_body(const Geom& g, const float* __restrict__ st, const float* __restrict__ f,
+                                         const float* __restrict__ u, float* __restrict__ res, long long t) {
+  const long long node = t >> 3;
+  const int j = (int)(t & 7);
+  if (node >= g.nn) return;
+  int x, y, z;
+  split_node(g.nx, g.ny, node, x, y, z);
+  const unsigned nx = (unsigned)g.nx, nxy = (unsigned)g.nx * (unsigned)g.ny;
+  float a[OTM_NRHS] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int o = j + 8 * q;
+    if (o < 27) {
+      const float c = ld_off(st + (size_t)o * (size_t)g.nn, 4u * (unsigned)node);
+      const unsigned w = 4u * ((unsigned)wrapm(x + o % 3 - 1, g.nx) + nx * (unsigned)wrapm(y + (o / 3) % 3 - 1, g.ny) +
+                               nxy * (unsigned)wrapm(z + o / 9 - 1, g.nz));
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) a[k] += c * ld_off(u + (size_t)k * (size_t)g.nn, w);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) a[k] = group8_sum(a[k]);
+  if (j < OTM_NRHS) {
+    const float s = j == 0 ? a[0] : (j == 1 ? a[1] : a[2]);
+    res[(size_t)j * (size_t)g.nn + node] = f[(size_t)j * (size_t)g.nn + node] - s;
+  }
+}
+
 template <bool HALF>
 __device__ inline void st_residual_body(const Geom& g, const StRef& S, const float* __restrict__ f,
                                         const float* __restrict__ u, float* __restrict__ res, long long t) {
@@ -256,17 +296,6 @@ __device__ inline int wrap_any(int a, int n) {
   a %= n;
   return a < 0 ? a + n : a;
 }
-// Sum over aligned groups of 8 lanes; every lane of a group receives the
-// same sum (IEEE addition commutes). Cross-lane moves through DPP: swap
-// pairs (quad_perm 1,0,3,2), swap pairs of pairs (quad_perm 2,3,0,1), then
-// the mirrored half row (lane i of 8 takes lane 7 - i).
-__device__ inline float group8_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
-  return v;
-}
-
 // Coupling o of the stored stencil at node v (FP16: scaled back to FP32).
 template <bool HALF>
 __device__ inline float stencil_coef(const StRef& S, long long nn, long long v, int o) {

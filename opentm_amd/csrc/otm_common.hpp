@@ -183,6 +183,17 @@ __device__ inline BlockRange block_range_v(long long n, int lb, int nb) {
 }
 __device__ inline BlockRange block_range(long long n) { return block_range_v(n, xcd_block(), gridDim.x); }
 
+// Sum over aligned groups of 8 lanes; every lane of a group receives the
+// same sum (IEEE addition commutes). Cross-lane moves through DPP: swap
+// pairs (quad_perm 1,0,3,2), swap pairs of pairs (quad_perm 2,3,0,1), then
+// the mirrored half row (lane i of 8 takes lane 7 - i).
+__device__ inline float group8_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+  return v;
+}
+
 // Lane i's value moved through a DPP pattern (both halves of the double).
 template <int CTRL>
 __device__ inline double dpp_f64(double v) {
