@@ -206,7 +206,11 @@ struct OpRestrict {
   static constexpr int kThreads = kBlock;
   int l;
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int) const {
-    restrict_body(C.g[l], C.g[l + 1], C.res[l], C.f[l + 1], (long long)lb * kBlock + threadIdx.x);
+    const long long t = (long long)lb * kBlock + threadIdx.x;
+    if (restrict8(C.g[l + 1]))  // as the single-cell launch (8 x the blocks)
+      restrict8_body(C.g[l], C.g[l + 1], C.res[l], C.f[l + 1], t);
+    else
+      restrict_body(C.g[l], C.g[l + 1], C.res[l], C.f[l + 1], t);
   }
 };
 
@@ -684,7 +688,10 @@ void plan_vcycle(otm_batch* b, std::vector<Member*>& V) {
         return (long long)nblk(st_resid8(L.g, L.st16 != nullptr) ? 8 * L.g.nn : L.g.nn);
       });
     }
-    add_op(b, S, OpRestrict{l}, [l](const Member& m) { return (long long)nblk(m.c->lv[l + 1].g.nn); });
+    add_op(b, S, OpRestrict{l}, [l](const Member& m) {
+      const Geom& cg = m.c->lv[l + 1].g;
+      return (long long)nblk(restrict8(cg) ? 8 * cg.nn : cg.nn);
+    });
   }
   // bottom: every member's tail
   add_op(b, V, OpTail{}, one);

@@ -59,6 +59,11 @@ __global__ void __launch_bounds__(kBlock) k_prolong_add(Geom fg, Geom cg, const 
   prolong_body(fg, cg, uc, uf, gtid());
 }
 
+__global__ void __launch_bounds__(kBlock) k_restrict8(Geom fg, Geom cg, const float* __restrict__ res,
+                                                      float* __restrict__ fc) {
+  restrict8_body(fg, cg, res, fc, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 __global__ void __launch_bounds__(kBlock) k_galerkin_first(Geom fg, Geom cg, GalerkinConsts G,
                                                            const float* __restrict__ kappa, float* __restrict__ E) {
   galerkin_first_body(fg, cg, G, kappa, E, gtid());
@@ -317,7 +322,10 @@ void launch_stencil_pack(hipStream_t s, const Geom& g, const float* st, float* s
   hipLaunchKernelGGL(k_stencil_pack, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, st, scale, st16);
 }
 void launch_restrict(hipStream_t s, const Geom& fine, const Geom& coarse, const float* res, float* fc) {
-  hipLaunchKernelGGL(k_restrict, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, res, fc);
+  if (restrict8(coarse))
+    hipLaunchKernelGGL(k_restrict8, dim3(nblk(8 * coarse.nn)), dim3(kBlock), 0, s, fine, coarse, res, fc);
+  else
+    hipLaunchKernelGGL(k_restrict, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, res, fc);
 }
 void launch_prolong_add(hipStream_t s, const Geom& fine, const Geom& coarse, const float* uc, float* uf) {
   hipLaunchKernelGGL(k_prolong_add, dim3(nblk(coarse.nn)), dim3(kBlock), 0, s, fine, coarse, uc, uf);

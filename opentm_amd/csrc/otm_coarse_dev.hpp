@@ -134,6 +134,37 @@ __device__ inline void restrict_node(const Geom& fg, const Geom& cg, const float
   }
 }
 
+// Restriction with 8 lanes per coarse node for small coarse levels (the
+// same weights; lane j takes the fine offsets o = j + 8q, the group adds
+// with group8_sum, lanes 0-2 write). Thread t serves coarse node t / 8.
+constexpr long long kRestrict8MaxNodes = 4096;
+__host__ __device__ inline bool restrict8(const Geom& cg) { return cg.nn <= kRestrict8MaxNodes; }
+__device__ inline void restrict8_body(const Geom& fg, const Geom& cg, const float* __restrict__ res,
+                                      float* __restrict__ fc, long long t) {
+  const long long node = t >> 3;
+  const int j = (int)(t & 7);
+  if (node >= cg.nn) return;
+  int X, Y, Z;
+  split_node(cg.nx, cg.ny, node, X, Y, Z);
+  const int x = 2 * X, y = 2 * Y, z = 2 * Z;
+  float a[OTM_NRHS] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int o = j + 8 * q;
+    if (o < 27) {
+      const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+      const float w = (dx ? 0.5f : 1.f) * (dy ? 0.5f : 1.f) * (dz ? 0.5f : 1.f);
+      const unsigned off = 4u * ((unsigned)wrapm(x + dx, fg.nx) +
+                                 (unsigned)fg.nx * ((unsigned)wrapm(y + dy, fg.ny) + (unsigned)fg.ny * (unsigned)wrapm(z + dz, fg.nz)));
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) a[k] += w * ld_off(res + (size_t)k * (size_t)fg.nn, off);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) a[k] = group8_sum(a[k]);
+  if (j < OTM_NRHS) fc[(size_t)j * (size_t)cg.nn + node] = j == 0 ? a[0] : (j == 1 ? a[1] : a[2]);
+}
+
 // u_f(i) += sum_I P(i, I) u_c(I), trilinear (prolongate_correction_kernel_H,
 // grid.cu:881-940).
 __device__ inline void prolong_node(const Geom& fg, const Geom& cg, const float* uc, float* uf, long long t) {
