@@ -135,8 +135,40 @@ __device__ inline void pcg_p_body(long long nn, float* __restrict__ p, const flo
 }
 
 // Stage 2 of every reduction: sums the per-block partials in block order and
-// applies the finaliser (one block).
+// applies the finaliser (one block). With the value count a compile-time
+// constant the trips over the partials unroll, so all of a thread's loads
+// are in flight together (the runtime-count loop waited for each load in
+// turn); each thread still adds its blocks tid, tid + 256, ... in order.
+template <int NV>
+__device__ inline void reduce_final_nv(const Reduce& R, int nblocks) {
+  constexpr int kTrips = kMaxReduceBlocks / kBlock;
+  __shared__ double sh[NV * kWaves];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) acc[i] = 0.0;
+#pragma unroll
+  for (int r = 0; r < kTrips; r++) {
+    const int b = threadIdx.x + kBlock * r;
+    if (b < nblocks) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) acc[i] += R.partials[(size_t)b * NV + i];
+    }
+  }
+  block_sum<NV>(acc, sh);
+  if (threadIdx.x == 0) {
+    finalize(acc, NV, R);
+    publish_host(acc, NV, R);
+  }
+}
+
 __device__ inline void reduce_final_body(const Reduce& R, int nv, int nblocks) {
+  switch (nv) {
+    case 2: reduce_final_nv<2>(R, nblocks); return;
+    case 3: reduce_final_nv<3>(R, nblocks); return;
+    case 6: reduce_final_nv<6>(R, nblocks); return;
+    case 9: reduce_final_nv<9>(R, nblocks); return;
+    default: break;
+  }
   __shared__ double sh[kMaxReduce * kWaves];
   double acc[kMaxReduce] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblocks; b += kBlock)
