@@ -468,10 +468,9 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
     vcycle(c, 0);
     c->post_dot_mode = -1;
     {
-      ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 12.0 * (double)n + 36.0 * (double)n);
+      ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 24.0 * (double)n);
       if (!c->post_dot_done) launch_dot3(c->s, n, c->r, c->z, c->reducer(FIN_RZ_FIRST), nb);
-      launch_fill(c->s, c->p, 3 * n, 0.f);
-      launch_pcg_p(c->s, n, c->p, c->z, c->pcg);
+      launch_pcg_p(c->s, n, c->p, c->z, c->pcg, true);
     }
     pending = refine;
     while (it < c->cfg.max_cg_iter) {

@@ -74,7 +74,8 @@ void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P
 void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);
 void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* xlo, float* r, const float* p, const float* q,
                    const PcgScalars* P, const Reduce& R, unsigned nblocks);
-void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P);
+// p = z + beta p, or p = z when first (a PCG start or restart)
+void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P, bool first = false);
 void launch_copy_pitched(hipStream_t s, const Geom& g, const float* src, int pitch, float* dst, float add);
 void launch_copy_to_pitched(hipStream_t s, const Geom& g, const float* src, float* dst, int pitch);
 

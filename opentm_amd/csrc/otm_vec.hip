@@ -52,8 +52,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restri
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restrict__ p, const float* __restrict__ z,
-                                                  const PcgScalars* __restrict__ P) {
-  pcg_p_body(nn, p, z, P, xcd_block(), gridDim.x);
+                                                  const PcgScalars* __restrict__ P, int first) {
+  pcg_p_body(nn, p, z, P, xcd_block(), gridDim.x, first != 0);
 }
 
 // dst[v] = src[x + pitch*(y + ny*z)] + add (pitched lexicographic input)
@@ -138,10 +138,10 @@ void launch_pcg_xr(hipStream_t s, long long nn, float* x, float* xlo, float* r, 
   hipLaunchKernelGGL(k_pcg_xr, dim3(nblocks), dim3(kBlock), 0, s, nn, x, xlo, r, p, q, P, R);
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
-void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P) {
+void launch_pcg_p(hipStream_t s, long long nn, float* p, const float* z, const PcgScalars* P, bool first) {
   unsigned nb = nblk(nn);
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(kBlock), 0, s, nn, p, z, P);
+  hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(kBlock), 0, s, nn, p, z, P, first ? 1 : 0);
 }
 void launch_copy_pitched(hipStream_t s, const Geom& g, const float* src, int pitch, float* dst, float add) {
   hipLaunchKernelGGL(k_copy_pitched, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, src, pitch, dst, add);

@@ -119,8 +119,9 @@ __device__ inline void pcg_xr_body(long long nn, float* __restrict__ x, float* _
 }
 
 // p = z + beta p
+// p = z + beta p; first: p = z (a restart: the old p is not read).
 __device__ inline void pcg_p_body(long long nn, float* __restrict__ p, const float* __restrict__ z,
-                                  const PcgScalars* __restrict__ P, int lb, int nb) {
+                                  const PcgScalars* __restrict__ P, int lb, int nb, bool first = false) {
   float be[OTM_NRHS];
 #pragma unroll
   for (int k = 0; k < OTM_NRHS; k++) be[k] = (float)P->beta[k];
@@ -129,7 +130,7 @@ __device__ inline void pcg_p_body(long long nn, float* __restrict__ p, const flo
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const long long j = k * nn + i;
-      p[j] = z[j] + be[k] * p[j];
+      p[j] = first ? z[j] : z[j] + be[k] * p[j];
     }
   }
 }
