@@ -293,7 +293,10 @@ __device__ inline void finalize(const double* acc, int nv, const Reduce& R) {
       // laminate) is measured against the largest of the three: its
       // reference norm is floored at 1e-3 of the largest norm (the
       // reference floors the norm with + 1e-10, MG.cpp:100).
+      // A (re)start of the PCG begins here, so the fault flags start over
+      // from this residual alone.
       double bmax = 0.0;
+      int fault = 0;
       for (int k = 0; k < OTM_NRHS; k++) bmax = acc[k] > bmax ? acc[k] : bmax;
       for (int k = 0; k < OTM_NRHS; k++) {
         double bb = acc[k] > 1e-6 * bmax ? acc[k] : 1e-6 * bmax;
@@ -304,7 +307,7 @@ __device__ inline void finalize(const double* acc, int nv, const Reduce& R) {
         P->mean[k] = mean;
         P->bb[k] = bb;
         P->rr[k] = rr;
-        if (!(acc[OTM_NRHS + k] - acc[OTM_NRHS + k] == 0.0)) P->fault |= 1;
+        if (!(acc[OTM_NRHS + k] - acc[OTM_NRHS + k] == 0.0)) fault = 1;
         P->active[k] = (rr > R.rtol2 * bb) ? 1 : 0;
         // an inner solve of the FP64 iterative refinement stops at a fixed
         // reduction of its own starting residual; a plain solve at rtol
@@ -313,6 +316,7 @@ __device__ inline void finalize(const double* acc, int nv, const Reduce& R) {
         P->alpha[k] = 0.0;
         P->beta[k] = 0.0;
       }
+      P->fault = fault;
       break;
     }
     case FIN_RZ_FIRST:

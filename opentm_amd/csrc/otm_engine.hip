@@ -523,7 +523,7 @@ static void pcg_solve(otm_ctx* c) {
   int its[3] = {0, 0, 0};
   int it = 0;
   c->post_dot_mode = -1;
-  hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
+  // the fault flags are cleared by the finaliser of each start residual
   bool fresh = false;
   if (!pcg_iterate(c, its, &it, &fresh)) {
     // The block-hybrid coarse smoother lost positivity on this operator:
@@ -531,7 +531,6 @@ static void pcg_solve(otm_ctx* c) {
     // positive preconditioner) for this context and solve again.
     set_bricks(c, false);
     c->fallbacks++;
-    hip_check(hipMemsetAsync(&c->pcg->fault, 0, sizeof(int), c->s), "clear fault");
     int it2 = 0;
     if (!pcg_iterate(c, its, &it2, &fresh)) fail(OTM_E_NUMERIC, "preconditioner not positive (MG numerical explode)");
     it += it2;
@@ -1177,6 +1176,7 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
       o->a = dalloc<float>(ne);
       o->b = dalloc<float>(ne);
       o->maxbits = dalloc<unsigned>(1);
+      hip_check(hipMemsetAsync(o->maxbits, 0, sizeof(unsigned), o->s), "hipMemsetAsync");
       o->st = dalloc<OcState>(1);
       o->partials = dalloc<double>((size_t)1024 * kOcCand);
       hip_check(hipHostMalloc((void**)&o->hm, sizeof(OcHost), hipHostMallocCoherent), "hipHostMalloc");

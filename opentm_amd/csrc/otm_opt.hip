@@ -147,11 +147,13 @@ __global__ void __launch_bounds__(kBlock) k_maxabs(long long n, const float* __r
 }
 
 
-__global__ void k_oc_zero(unsigned int* bits) { *bits = 0u; }
-
-__global__ void k_oc_init(const unsigned int* __restrict__ bits, OcState* st) {
+// Starts the bisection from [0, max|sens|] and clears the maximum for the
+// next call (the word is zeroed once when the optimiser is created).
+__global__ void k_oc_init(unsigned int* __restrict__ bits, OcState* st) {
+  const unsigned int b = *bits;
+  *bits = 0u;
   st->lo = 0.f;
-  st->hi = fabsf(__uint_as_float(*bits));
+  st->hi = fabsf(__uint_as_float(b));
   st->g = 0.f;
   st->done = 0;
   st->steps = 0;
@@ -494,7 +496,6 @@ void launch_maxabs(hipStream_t s, long long n, const float* a, unsigned int* out
   hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(kBlock), 0, s, n, a, out_bits);
 }
 void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st) {
-  hipLaunchKernelGGL(k_oc_zero, dim3(1), dim3(1), 0, s, bits);
   launch_maxabs(s, n, sens, bits);
   hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(1), 0, s, bits, st);
 }
