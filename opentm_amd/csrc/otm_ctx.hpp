@@ -169,6 +169,11 @@ struct otm_ctx {
   double prof_bytes[OTM_PROF_NCLASSES] = {};
   // status
   bool has_kappa = false, has_solution = false, has_drho = false;
+  // Start residual of the next solve launched on s_side while the coarse
+  // operators are rebuilt on s (otm_eval); s waits for it through ev_side.
+  hipStream_t s_side = nullptr;
+  hipEvent_t ev_main = nullptr, ev_side = nullptr;
+  bool start_res_ready = false;
   int iters[3] = {0, 0, 0};
   double relres[3] = {0, 0, 0};
   double C[9];

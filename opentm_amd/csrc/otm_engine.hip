@@ -163,6 +163,10 @@ static void ctx_free(otm_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->s_side) (void)hipStreamSynchronize(c->s_side);
+  if (c->ev_main) (void)hipEventDestroy(c->ev_main);
+  if (c->ev_side) (void)hipEventDestroy(c->ev_side);
+  if (c->s_side) (void)hipStreamDestroy(c->s_side);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
 }
@@ -275,7 +279,12 @@ static void build_hierarchy(otm_ctx* c) {
 
 // Galerkin coarse operators + coarsest factorisation (MG_H::updateStencils,
 // MG.cpp:144-150).
-static void build_operators(otm_ctx* c) {
+// start_res: launch the next solve's FP64 start residual (it needs only
+// kappa and the warm-start iterate) on the side stream while the coarsest
+// inverse, one workgroup on one CU, is formed; the main stream waits for it
+// afterwards. Overlapping it with the bandwidth-bound Galerkin kernels
+// instead saved nothing.
+static void build_operators(otm_ctx* c, bool start_res = false) {
   ProfScope ps(c, OTM_PROF_SETUP, 0.0);
   for (size_t l = 1; l < c->lv.size(); l++) {
     Level& L = c->lv[l];
@@ -287,7 +296,24 @@ static void build_operators(otm_ctx* c) {
     if (L.st16) launch_stencil_pack(c->s, L.g, L.st, L.scale, L.st16);
   }
   const Level& Lc = c->lv.back();
+  if (start_res) {
+    if (!c->s_side) {
+      hip_check(hipStreamCreateWithFlags(&c->s_side, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming), "hipEventCreate");
+      hip_check(hipEventCreateWithFlags(&c->ev_side, hipEventDisableTiming), "hipEventCreate");
+    }
+    const double rtol2 = c->cfg.fem_rel_thres * c->cfg.fem_rel_thres;
+    hip_check(hipEventRecord(c->ev_main, c->s), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->s_side, c->ev_main, 0), "hipStreamWaitEvent");
+    launch_fine_init_residual(c->s_side, c->g0, c->fc, c->kappa, c->x, c->xlo, c->r,
+                              c->reducer_host(FIN_BB_RR, rtol2, 1.0 / (double)c->g0.nn), red_blocks_for(c->g0.nn));
+    hip_check(hipEventRecord(c->ev_side, c->s_side), "hipEventRecord");
+  }
   launch_coarsest_factor(c->s, Lc.g, Lc.st, c->cB, c->cBinv);
+  if (start_res) {
+    hip_check(hipStreamWaitEvent(c->s, c->ev_side, 0), "hipStreamWaitEvent");
+    c->start_res_ready = true;
+  }
   hip_check(hipGetLastError(), "operator build");
 }
 
@@ -437,6 +463,8 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
         Reduce R = c->reducer_host(FIN_BB_RR, rtol2, inv_n);
         R.inner2 = kRefineInner * kRefineInner;
         launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, R, nb);
+      } else if (c->start_res_ready) {
+        c->start_res_ready = false;  // launched by do_update, same arguments
       } else {
         launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->xlo, c->r,
                                   c->reducer_host(FIN_BB_RR, rtol2, inv_n), nb);
@@ -683,7 +711,12 @@ OTM_API int otm_level_reso(const otm_ctx* ctx, int l, int reso[3]) {
   });
 }
 
-static void do_update(otm_ctx* c, const float* kappa, int pitch, int where) {
+// overlap_start: launch the next solve's FP64 start residual (it needs only
+// kappa and the warm-start iterate) on the side stream, concurrently with
+// the coarse operator rebuild, whose coarsest inverse occupies one CU; the
+// main stream waits for it after the rebuild. Only for a caller that solves
+// next with nothing in between (otm_eval), on the plain path.
+static void do_update(otm_ctx* c, const float* kappa, int pitch, int where, bool overlap_start = false) {
   if (kappa) {
     check_where(where);
     if (pitch < c->g0.nx) fail(OTM_E_ARG, "pitch smaller than reso[0]");
@@ -712,7 +745,8 @@ static void do_update(otm_ctx* c, const float* kappa, int pitch, int where) {
     c->has_kappa = true;
   }
   if (!c->has_kappa) fail(OTM_E_STATE, "otm_update without a conductivity field");
-  build_operators(c);
+  c->start_res_ready = false;
+  build_operators(c, overlap_start && c->x64 == nullptr);
 }
 
 OTM_API int otm_update(otm_ctx* ctx, const float* kappa, int pitch, int where) {
@@ -1118,7 +1152,7 @@ OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
     }
     launch_simp_fwd(ctx->s, n, src, (float)ctx->cfg.heat_ratio[0], (float)ctx->cfg.heat_ratio[1], 0.f, ctx->kraw,
                     ctx->dkappa);
-    do_update(ctx, ctx->kraw, ctx->g0.nx, OTM_DEVICE);
+    do_update(ctx, ctx->kraw, ctx->g0.nx, OTM_DEVICE, true);
     do_heat_matrix(ctx, C);
     ctx->has_drho = false;
     ctx->taps_kind = p->filter_kind >= 0 ? ctx->taps_kind : -1;
