@@ -47,7 +47,7 @@ typedef enum otm_status {
   OTM_E_OOM = -8           /* device allocation failed */
 } otm_status;
 
-enum { OTM_HOST = 0, OTM_DEVICE = 1 };
+enum { OTM_HOST = 0, OTM_DEVICE = 1, OTM_ANY = 2 /* otm_memcpy only: host, device or managed, found from the pointer */ };
 
 /* Solver configuration. Mirrors cfg::HomoConfig (cmdline.h:12-33) for the
  * fields the solve uses, plus the multigrid knobs the reference hard-codes. */
@@ -64,6 +64,9 @@ typedef struct otm_config {
   int coarse_sweeps;        /* sweeps on levels >= 1 (0: same as the fine level) */
   int fp64_refine;          /* 1: FP64 iterate with FP32 inner PCG solves (iterative refinement) */
   int fp16_stencils;        /* 1: coarse stencils stored in FP16 with a per-level scale */
+  int exact_smoother;       /* 1: exact multicolour Gauss-Seidel on every level instead of the
+                               LDS brick smoother (the fallback a solve takes by itself when
+                               the brick preconditioner loses positivity) */
 } otm_config;
 
 /* cfg::HomoConfig::init (cmdline.cpp:4-35) defaults where they apply;
@@ -94,7 +97,11 @@ OTM_API int otm_update(otm_ctx* ctx, const float* kappa, int pitch, int where);
 
 /* Homogenization_H::heatMatrix(double C[3][3]) (homogenization.cu:180-252):
  * solves the three load cases (warm-started from the previous solution)
- * and returns the symmetric 3x3 homogenized conductivity, row-major. */
+ * and returns the symmetric 3x3 homogenized conductivity, row-major.
+ * OTM_E_NOCONV: the solve ended above fem_rel_thres (max_cg_iter reached
+ * or the true residual stopped improving); C, the fluctuation fields and
+ * otm_solve_stats are still written (the reference reports "Failed to
+ * converge" and carries on, MG.cpp:101-118). OTM_E_NUMERIC: NaN/Inf. */
 OTM_API int otm_heat_matrix(otm_ctx* ctx, double C[9]);
 
 /* PCG iterations and achieved relative residual of the last solve. */
@@ -124,8 +131,48 @@ OTM_API int otm_reset_displacement(otm_ctx* ctx);
  * preconditioner on the fine level; MG_H::v_cycle, MG.cpp:46-71). */
 OTM_API int otm_apply_level(otm_ctx* ctx, int level, const float* u, float* q, int where);
 OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where);
+/* Grid_H::getDensity (grid.cpp:330-345 writes it): the element conductivity
+ * last given to otm_update (to one FP32 rounding), lexicographic, x fastest. */
+OTM_API int otm_get_kappa(otm_ctx* ctx, float* kappa, int where);
 /* Level-l 27-point stencil, 27*nodes_l floats (offset major), l >= 1. */
 OTM_API int otm_get_stencil(otm_ctx* ctx, int level, float* st, int where);
+
+/* MG_H::solveEquation(double tol, bool with_guess) (MG.cpp:80-142) for the
+ * three load cases together: MG-preconditioned CG to relative residual
+ * rtol (<= 0: the context's fem_rel_thres) from the current fluctuation
+ * fields (warm != 0) or from zero, without kappa^H. relres (may be NULL)
+ * receives the final true relative residuals. OTM_E_NOCONV as for
+ * otm_heat_matrix. */
+OTM_API int otm_solve(otm_ctx* ctx, double rtol, int warm, double relres[3]);
+
+/* MG_H::v_cycle(w, pre, post) (MG.cpp:46-71) iterated the way the
+ * reference iterates it (MG_H::solveEquation's loop): `cycles` stationary
+ * steps chi += M^{-1}(b - K chi) of the three load cases, with this
+ * engine's V-cycle as M^{-1}. relres (may be NULL) receives the relative
+ * residual measured at the start of the last cycle (the reference's check
+ * lags one cycle the same way, MG.cpp:100). */
+OTM_API int otm_mg_cycles(otm_ctx* ctx, int cycles, double relres[3]);
+
+/* Grid_H::restrict_residual / prolongate_correction (grid.cu:811-940):
+ * direction 0 restricts three level-(level-1) vectors `in` onto level
+ * `level` (out = P^T in); direction 1 adds the prolongation of three
+ * level-`level` vectors `in` to the level-(level-1) vectors `out`
+ * (out += P in). Trilinear P, 1 <= level < levels. */
+OTM_API int otm_transfer(otm_ctx* ctx, int level, int direction, const float* in, float* out, int where);
+
+/* Grid_H::gs_relaxation (grid.cu:943-987) on level `level` (< levels-1)
+ * for three right-hand sides f, in place on u (3*nodes_l floats each).
+ * mode 0: exact multicolour Gauss-Seidel, colours in this engine's forward
+ * order (the reverse of the reference's colour order 0..7); mode 1: the
+ * reference's order; mode 2 / 3: the LDS brick smoother in those orders
+ * (exact inside each brick, neighbours outside it frozen at the entry u),
+ * the one the preconditioner uses. */
+OTM_API int otm_smooth_level(otm_ctx* ctx, int level, const float* f, float* u, int sweeps, int mode, int where);
+
+/* Grid_H::solveHostEquation (grid.cpp:272-286): u = B^{-1} f on the
+ * coarsest level for three right-hand sides, with B the coarsest Galerkin
+ * operator plus the rank-one term that fixes its constant null space. */
+OTM_API int otm_coarsest_solve(otm_ctx* ctx, const float* f, float* u, int where);
 
 /* Radial convolution filter (conv_tsexp_t, TensorExpression.h:1095-1185;
  * kernels TensorExpression.cuh:303-384). kind 0 = Spline4, 1 = Linear;
@@ -253,6 +300,7 @@ typedef struct otm_custom_report {
   double objective;    /* last objective value */
   double volume;       /* mean design density */
   double C[9];         /* last homogenized tensor */
+  int unconverged_solves;  /* solves that ended above fem_rel_thres (the loop carried on) */
 } otm_custom_report;
 OTM_API void otm_custom_params_init(otm_custom_params* p);
 /* rho0: optional host initial density (reso^3 floats); rho_out: host
@@ -357,6 +405,9 @@ OTM_API int otm_profile_read(otm_ctx* ctx, int kclass, long long* launches, doub
 /* Device memory helpers for callers without their own HIP allocator. */
 OTM_API int otm_device_alloc(int device, size_t bytes, void** ptr);
 OTM_API int otm_device_free(void* ptr);
+/* dst_where / src_where: OTM_HOST, OTM_DEVICE, or OTM_ANY when the caller
+ * does not know (e.g. a pointer handed over by reference code that may be
+ * managed memory); any OTM_ANY lets the runtime find out from the pointers. */
 OTM_API int otm_memcpy(void* dst, const void* src, size_t bytes, int dst_where, int src_where);
 OTM_API int otm_synchronize(otm_ctx* ctx);
 

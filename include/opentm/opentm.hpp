@@ -1,13 +1,17 @@
 // opentm.hpp -- C++ classes with the reference's names over the C ABI of
 // opentm.h, so that code written against quanyuchen2000/OPENTM's
-// homo::Homogenization_H / OCOptimizer / MMAOptimizer, cfg::HomoConfig,
+// homo::Homogenization_H / MG_H / Grid_H (subset) / OCOptimizer /
+// MMAOptimizer, cfg::HomoConfig,
 // runCustom, runInstance and wrapper_homo compiles against this engine.
 // Errors throw std::runtime_error with the engine's message (the reference
 // throws std::runtime_error too, e.g. "MG numerical explode", MG.cpp:116).
 #pragma once
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -99,29 +103,146 @@ inline void check(int status) {
   if (status != OTM_OK) throw std::runtime_error(otm_last_error());
 }
 
+// grid.h:89
+enum VoxelIOFormat { binary, openVDB };
+
+// Grid_H (grid.h:104-421), the subset a caller of Homogenization_H / MG_H
+// touches, as a view of one level of an engine context. The reference's
+// per-grid device buffers (padded Gauss-Seidel order) have no counterpart:
+// the engine keeps every level lexicographic and unpadded.
+struct Grid_H {
+  otm_ctx* ctx = nullptr;
+  int level = 0;
+  std::array<int, 3> cellReso{{0, 0, 0}};
+  Grid_H(otm_ctx* c, int l) : ctx(c), level(l) { check(otm_level_reso(c, l, cellReso.data())); }
+  int n_cells() const { return cellReso[0] * cellReso[1] * cellReso[2]; }
+  // nodes of the level (the engine stores no periodic images or padding)
+  int n_gsvertices() const { return n_cells(); }
+  int n_gscells() const { return n_cells(); }
+  // Grid_H::update (root level): SIMP-mapped conductivity, device pointer
+  void update(float* rho, int pitchT = -1, bool lexiOrder = true) {
+    if (level != 0) throw std::runtime_error("Grid_H::update applies to the root grid");
+    if (!lexiOrder) throw std::runtime_error("only lexicographic conductivities are accepted");
+    check(otm_update(ctx, rho, pitchT < 0 ? cellReso[0] : pitchT, OTM_DEVICE));
+  }
+  void reset_displacement() {
+    if (level != 0) throw std::runtime_error("only the root grid's displacement is held between solves");
+    check(otm_reset_displacement(ctx));
+  }
+  // Grid_H::getDensity (root level): lexicographic conductivity on the host
+  void getDensity(std::vector<float>& rho, bool lexiOrder = true) {
+    if (level != 0) throw std::runtime_error("Grid_H::getDensity applies to the root grid");
+    (void)lexiOrder;  // the engine's order is lexicographic
+    rho.resize((size_t)n_cells());
+    check(otm_get_kappa(ctx, rho.data(), OTM_HOST));
+  }
+  // Grid_H::writeDensity (grid.cpp:330-345); OpenVDB is not available here
+  void writeDensity(const std::string& fname, VoxelIOFormat frmat) {
+    if (frmat != binary) throw std::runtime_error("OpenVDB output is not supported; use VoxelIOFormat::binary");
+    std::vector<float> rho;
+    getDensity(rho);
+    check(otm_write_density(fname.c_str(), cellReso.data(), rho.data(), OTM_HOST));
+  }
+};
+
+// MGConfig / MG_H (MG.h:8-32) over an engine context. MG_H::build makes
+// a context of its own; the MG_H of a Homogenization_H shares that one's.
+struct MGConfig {
+  int reso[3];
+  bool enableManagedMem = true;
+  std::string namePrefix;
+};
+
+struct MG_H {
+  std::vector<std::shared_ptr<Grid_H>> grids;  // fine to coarse
+  MGConfig mgConfig;
+  otm_ctx* ctx = nullptr;
+  bool owns = false;
+  MG_H() = default;
+  MG_H(const MG_H&) = delete;
+  MG_H& operator=(const MG_H&) = delete;
+  ~MG_H() { release(); }
+  void release() {
+    grids.clear();
+    if (owns) otm_destroy(ctx);
+    ctx = nullptr;
+    owns = false;
+  }
+  // view of a context owned elsewhere (Homogenization_H::mg_)
+  void attach(otm_ctx* c) {
+    release();
+    ctx = c;
+    int nl = 0;
+    check(otm_levels(c, &nl));
+    for (int l = 0; l < nl; l++) grids.push_back(std::make_shared<Grid_H>(c, l));
+    for (int a = 0; a < 3; a++) mgConfig.reso[a] = grids[0]->cellReso[a];
+  }
+  void build(MGConfig config) {
+    otm_config e;
+    otm_config_init(&e);
+    for (int a = 0; a < 3; a++) e.reso[a] = config.reso[a];
+    otm_ctx* c = nullptr;
+    check(otm_create(&e, &c));
+    attach(c);
+    owns = true;
+    mgConfig = config;
+  }
+  std::shared_ptr<Grid_H> getRootGrid() { return grids.at(0); }
+  // one stationary V-cycle of the three load cases (pre/post as in the
+  // reference: ignored; the engine's smoothing schedule is fixed per context)
+  void v_cycle(float w_SOR = 1.f, int pre = 1, int post = 1) {
+    (void)w_SOR;
+    (void)pre;
+    (void)post;
+    check(otm_mg_cycles(ctx, 1, nullptr));
+  }
+  void reset_displacement() { check(otm_reset_displacement(ctx)); }
+  // MG-preconditioned CG of the three load cases to `tol`; returns the
+  // largest relative residual reached (the reference returns its residual)
+  double solveEquation(double tol = 1e-2, bool with_guess = true) {
+    double rr[3] = {0, 0, 0};
+    const int st = otm_solve(ctx, tol, with_guess ? 1 : 0, rr);
+    if (st != OTM_OK && st != OTM_E_NOCONV) check(st);  // the reference reports and carries on
+    return std::max(rr[0], std::max(rr[1], rr[2]));
+  }
+  void updateStencils() { check(otm_update(ctx, nullptr, mgConfig.reso[0], OTM_DEVICE)); }
+};
+
 // homogenization.h:15-32. Pointers are device pointers, as in the reference.
 struct Homogenization_H {
   cfg::HomoConfig config;
   otm_ctx* ctx = nullptr;
+  std::shared_ptr<Grid_H> grid;  // root grid (homogenization.h:21)
+  std::unique_ptr<MG_H> mg_;     // the hierarchy (homogenization.h:22)
   float power_penal = 1;
   float diag_strength = 1e6;
   explicit Homogenization_H(cfg::HomoConfig c) { build(c); }
   Homogenization_H(const Homogenization_H&) = delete;
   Homogenization_H& operator=(const Homogenization_H&) = delete;
-  ~Homogenization_H() { otm_destroy(ctx); }
+  ~Homogenization_H() {
+    mg_.reset();
+    grid.reset();
+    otm_destroy(ctx);
+  }
   void build(cfg::HomoConfig c) {
+    mg_.reset();
+    grid.reset();
     if (ctx) otm_destroy(ctx);
     ctx = nullptr;
     config = c;
     otm_config e = c.engine();
     check(otm_create(&e, &ctx));
+    mg_.reset(new MG_H());
+    mg_->attach(ctx);
+    grid = mg_->getRootGrid();
   }
   void update(float* rho = nullptr, int pitchT = -1) {
     check(otm_update(ctx, rho, pitchT < 0 ? config.reso[0] : pitchT, OTM_DEVICE));
   }
   void heatMatrix(double C[3][3]) {
     double c[9];
-    check(otm_heat_matrix(ctx, c));
+    const int st = otm_heat_matrix(ctx, c);
+    if (st != OTM_OK && st != OTM_E_NOCONV) check(st);  // NOCONV: C is written, as the reference carries on
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) C[i][j] = c[3 * i + j];
   }
@@ -176,97 +297,105 @@ struct MMAOptimizer {
   ~MMAOptimizer() { otm_mma_destroy(h); }
   void setBound(float xmin, float xmax) { check(otm_mma_set_bounds(h, xmin, xmax)); }
   void setBound(float* xmin, float* xmax) { check(otm_mma_set_bounds_array(h, xmin, xmax, OTM_DEVICE)); }
+  // x, dfdx, gval and dgdx[i] are device (or managed) float arrays, as the
+  // reference's callers pass them (mmaOptimizer.cu:61-69 casts gval on the
+  // device); the m constraint values are read back before the update
   void update(int itn, float* x, float* dfdx, float* gval, float** dgdx) {
-    std::vector<double> g(m);
-    for (int i = 0; i < m; i++) g[i] = gval[i];  // gval is host memory here
+    std::vector<float> gf((size_t)m);
+    check(otm_memcpy(gf.data(), gval, sizeof(float) * (size_t)m, OTM_HOST, OTM_ANY));
+    std::vector<double> g(gf.begin(), gf.end());
     check(otm_mma_update(h, itn, x, dfdx, g.data(), dgdx, OTM_DEVICE, nullptr));
   }
 };
 
-// Framework.cuh:312-343: converged when the relative objective change stays
-// below the threshold for three consecutive iterations after iteration 11.
+// Convergence test of runCustom (Framework.cuh:312-343). Behaviour: every
+// iteration's objective is recorded; up to and including iteration 11 the
+// answer is "not converged"; afterwards the loop has converged when each of
+// the last three steps (obj[it-1] -> obj[it], obj[it-2] -> obj[it-1],
+// obj[it-3] -> obj[it-2]) changed the objective by at most `thres` relative
+// to the earlier value. A comparison that cannot be made (NaN) counts as
+// within the threshold, as in the reference.
 struct ConvergeChecker {
- private:
-  static constexpr int circleLen = 10;
-  double thres = 1e-3;
-  double Obj[circleLen] = {};
-  int p_cir = 0;
-  int cirId(int id) const {
-    while (id < 0) id += circleLen;
-    return id % circleLen;
-  }
-  double get(int id) const { return Obj[cirId(id)]; }
-  bool checkSeq() const {
-    bool out = false;
-    for (int i = 0; i < 3; i++)
-      out = out || std::fabs(get(p_cir - i) - get(p_cir - i - 1)) / std::fabs(get(p_cir - i - 1)) > thres;
-    return !out;
+  explicit ConvergeChecker(double thres_ = 1e-3) : tol_(thres_) {}
+  bool is_converge(int iter, double obj) {
+    // the last four objectives, newest first
+    for (int i = 3; i > 0; i--) recent_[i] = recent_[i - 1];
+    recent_[0] = obj;
+    if (iter <= 11) return false;
+    for (int step = 0; step < 3; step++) {
+      const double now = recent_[step], before = recent_[step + 1];
+      const double change = std::fabs(now - before) / std::fabs(before);
+      if (change > tol_) return false;
+    }
+    return true;
   }
 
- public:
-  explicit ConvergeChecker(double thres_ = 1e-3) : thres(thres_) {}
-  bool is_converge(int iter, double obj) {
-    const int id = cirId(iter);
-    p_cir = id;
-    Obj[id] = obj;
-    if (iter <= 11) return false;
-    return checkSeq();
-  }
+ private:
+  double tol_;
+  double recent_[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
-// Framework.cuh:345-411: adaptive volume bound for the OC tensor-matching loop.
-struct VolumeGovernor {
- private:
-  float lowBound = 0;
-  float volume_bound = 1.0f;
-  float decrease = 0;
-  float decrease_factor = 1.f;
-  float val_last = 1;
-  float vibrate = 0;
-  int vibrate_count = 0;
-  int count = 0;
-  void shrink() {
-    decrease = volume_bound - lowBound;
-    volume_bound = volume_bound - decrease_factor * decrease;
-    if (decrease_factor > 0.05f) decrease_factor *= 0.8f;
-  }
-  void expand() { volume_bound += 0.3f * decrease * decrease_factor; }
-
+// Adaptive volume bound of runCustom's OC branch (Framework.cuh:345-411),
+// as a small state machine over: the bound B (starts at 1), the lower
+// limit L given by the caller, the last cut D and the cut factor F (starts
+// at 1), the previous objective and its previous change, an oscillation
+// counter and a stall counter. Per check(value v, lower, volfrac, itn):
+//  1. L = lower. If v < 0 within 1e-4 (v + 1e-4 < 1e-4) or itn == 100, cut:
+//     D = B - L, B -= F D, and F shrinks by 0.8 while above 0.05. If F is
+//     then at most 0.05, report 1 (stop); otherwise reset the stall counter.
+//  2. "reached" means |B - volfrac| < 1e-2.
+//  3. With c = previous objective - v: a sign change of c against the
+//     previous change counts one oscillation, anything else resets the count.
+//     Eight oscillations in a row: widen (B += 0.3 D F, count reset) when
+//     reached, otherwise cut as in 1 (count kept).
+//  4. A step with c < 1e-4, v > 1e-4 and reached counts one stall (else the
+//     stall count resets); five stalls widen the bound by 0.3 D F and reset.
+//  5. Remember v and c; report 0.
+// The state is FP32 and the constants are double literals, evaluated in the
+// reference's precision (float state, double arithmetic where it mixes).
+class VolumeGovernor {
  public:
-  float get_volume_bound() const { return volume_bound; }
-  float get_current_decrease() const { return decrease_factor * decrease; }
+  float get_volume_bound() const { return bound_; }
+  float get_current_decrease() const { return factor_ * cut_; }
   int volume_check(float value, float lowBound_, float volfrac, int itn) {
-    lowBound = lowBound_;
-    if (value + 0.0001f < 1e-4f || itn == 100) {
-      shrink();
-      if (decrease_factor <= 0.05f) return 1;
-      count = 0;
+    lower_ = lowBound_;
+    if ((double)value + 0.0001 < 1e-4 || itn == 100) {
+      cut();
+      if ((double)factor_ <= 0.05) return 1;
+      stalls_ = 0;
     }
-    const bool reach = std::fabs(volume_bound - volfrac) < 1e-2f;
-    if (vibrate * (val_last - value) < 0)
-      vibrate_count++;
-    else
-      vibrate_count = 0;
-    if (vibrate_count >= 8) {
-      if (reach) {
-        expand();
-        vibrate_count = 0;
+    const bool reached = std::fabs(bound_ - volfrac) < 1e-2;
+    const float change = last_value_ - value;
+    oscillations_ = (last_change_ * change < 0) ? oscillations_ + 1 : 0;
+    if (oscillations_ >= 8) {
+      if (reached) {
+        widen();
+        oscillations_ = 0;
       } else {
-        shrink();
+        cut();
       }
     }
-    vibrate = val_last - value;
-    if (val_last - value < 1e-4f && value > 1e-4f && reach)
-      count++;
-    else
-      count = 0;
-    if (count >= 5) {
-      volume_bound += 0.3f * decrease * decrease_factor;
-      count = 0;
+    last_change_ = change;
+    const bool stalled = (double)change < 1e-4 && (double)value > 1e-4 && reached;
+    stalls_ = stalled ? stalls_ + 1 : 0;
+    if (stalls_ >= 5) {
+      widen();
+      stalls_ = 0;
     }
-    val_last = value;
+    last_value_ = value;
     return 0;
   }
+
+ private:
+  void cut() {
+    cut_ = bound_ - lower_;
+    bound_ = bound_ - factor_ * cut_;
+    if ((double)factor_ > 0.05) factor_ = (float)((double)factor_ * 0.8);
+  }
+  void widen() { bound_ = (float)((double)bound_ + 0.3 * (double)cut_ * (double)factor_); }
+  float bound_ = 1.0f, lower_ = 0.0f, cut_ = 0.0f, factor_ = 1.0f;
+  float last_value_ = 1.0f, last_change_ = 0.0f;
+  int oscillations_ = 0, stalls_ = 0;
 };
 
 }  // namespace homo

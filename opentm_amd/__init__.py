@@ -52,6 +52,7 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         _capi.check(lib.otm_device_alloc(device, 4 * self.n, ctypes.byref(p)), "otm_device_alloc")
         self.ptr = p
+        _capi.track(self)
 
     @classmethod
     def from_numpy(cls, a, device=0):
@@ -74,7 +75,11 @@ class DeviceBuffer:
             _capi.load().otm_device_free(self.ptr)
             self.ptr = None
 
+    close = free
+
     def __del__(self):
+        if _capi.shutting_down():
+            return
         try:
             self.free()
         except Exception:
@@ -116,6 +121,7 @@ class HomoConfig:
         self.coarse_sweeps = 0
         self.fp64_refine = 0
         self.fp16_stencils = 0
+        self.exact_smoother = 0
         self.device = 0
         self.verbose = 0
 
@@ -133,6 +139,7 @@ class HomoConfig:
         c.coarse_sweeps = self.coarse_sweeps
         c.fp64_refine = self.fp64_refine
         c.fp16_stencils = self.fp16_stencils
+        c.exact_smoother = self.exact_smoother
         c.device = self.device
         c.verbose = self.verbose
         return c
@@ -171,6 +178,7 @@ class Homogenization_H:
         cfg = config._c()
         _capi.check(lib.otm_create(ctypes.byref(cfg), ctypes.byref(ctx)), "otm_create")
         self._ctx = ctx
+        _capi.track(self)
         self.reso = tuple(config.reso)
         self.ne = self.reso[0] * self.reso[1] * self.reso[2]
 
@@ -180,6 +188,8 @@ class Homogenization_H:
             self._ctx = None
 
     def __del__(self):
+        if _capi.shutting_down():
+            return
         try:
             self.close()
         except Exception:
@@ -273,6 +283,59 @@ class Homogenization_H:
         _capi.check(_capi.load().otm_get_stencil(self._ctx, level, _ptr(st), OTM_HOST), "otm_get_stencil")
         return st.reshape(27, n)
 
+    def _level_nodes(self, level):
+        n = 1
+        for v in self.levels()[level]:
+            n *= v
+        return n
+
+    def solve(self, rtol=0.0, warm=True):
+        """MG_H::solveEquation for the three load cases (no kappa^H); returns
+        the three final relative residuals."""
+        rr = (ctypes.c_double * 3)()
+        _capi.check(_capi.load().otm_solve(self._ctx, float(rtol), 1 if warm else 0, rr), "otm_solve")
+        return list(rr)
+
+    def mg_cycles(self, cycles=1):
+        """MG_H::v_cycle iterated as a stationary method; returns the relative
+        residual measured at the start of the last cycle."""
+        rr = (ctypes.c_double * 3)()
+        _capi.check(_capi.load().otm_mg_cycles(self._ctx, int(cycles), rr), "otm_mg_cycles")
+        return list(rr)
+
+    def restrict(self, level, r_fine):
+        """P^T r onto `level` from three level-(level-1) vectors."""
+        nf, nc = self._level_nodes(level - 1), self._level_nodes(level)
+        a = _f32(r_fine, 3 * nf)
+        out = np.empty(3 * nc, np.float32)
+        _capi.check(_capi.load().otm_transfer(self._ctx, level, 0, _ptr(a), _ptr(out), OTM_HOST), "otm_transfer")
+        return out
+
+    def prolong_add(self, level, u_coarse, u_fine):
+        """u_fine + P u_coarse (returns a new array)."""
+        nf, nc = self._level_nodes(level - 1), self._level_nodes(level)
+        a = _f32(u_coarse, 3 * nc)
+        out = np.array(_f32(u_fine, 3 * nf), copy=True)
+        _capi.check(_capi.load().otm_transfer(self._ctx, level, 1, _ptr(a), _ptr(out), OTM_HOST), "otm_transfer")
+        return out
+
+    def smooth_level(self, level, f, u, sweeps=1, mode=0):
+        """Gauss-Seidel sweeps on a level (mode 0/1 exact in forward/reference
+        colour order, 2/3 the brick smoother); returns the new u."""
+        n = self._level_nodes(level)
+        fa = _f32(f, 3 * n)
+        out = np.array(_f32(u, 3 * n), copy=True)
+        _capi.check(_capi.load().otm_smooth_level(self._ctx, level, _ptr(fa), _ptr(out), int(sweeps), int(mode),
+                                                  OTM_HOST), "otm_smooth_level")
+        return out
+
+    def coarsest_solve(self, f):
+        n = self._level_nodes(len(self.levels()) - 1)
+        fa = _f32(f, 3 * n)
+        out = np.empty(3 * n, np.float32)
+        _capi.check(_capi.load().otm_coarsest_solve(self._ctx, _ptr(fa), _ptr(out), OTM_HOST), "otm_coarsest_solve")
+        return out
+
     # -- density pipeline (AutoDiff parts of runCustom) ------------------
     def set_design(self, rho):
         p, where, keep = _arg(rho, self.ne)
@@ -354,6 +417,7 @@ class OCOptimizer:
         _capi.check(lib.otm_oc_create(self.ne, min_density, stepLimit, dampExponent, device, ctypes.byref(h)),
                     "otm_oc_create")
         self._h = h
+        _capi.track(self)
         _capi.check(lib.otm_oc_set_bisection(h, int(bisect_steps), float(bisect_tol)), "otm_oc_set_bisection")
         self.last_steps = 0
         self.last_volume = 0.0
@@ -364,6 +428,8 @@ class OCOptimizer:
             self._h = None
 
     def __del__(self):
+        if _capi.shutting_down():
+            return
         try:
             self.close()
         except Exception:
@@ -419,6 +485,7 @@ class MMAOptimizer:
         h = ctypes.c_void_p()
         _capi.check(lib.otm_mma_create(self.m, self.n, a0, a, c, d, device, ctypes.byref(h)), "otm_mma_create")
         self._h = h
+        _capi.track(self)
         self.last_newton = 0
 
     def close(self):
@@ -427,6 +494,8 @@ class MMAOptimizer:
             self._h = None
 
     def __del__(self):
+        if _capi.shutting_down():
+            return
         try:
             self.close()
         except Exception:
@@ -552,7 +621,8 @@ def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
         r0 = _ptr(r0a)
     _capi.check(lib.otm_run_custom(ctypes.byref(p), r0, _ptr(out), ctypes.byref(rep)), "otm_run_custom")
     return out, {"iterations": rep.iterations, "converged": bool(rep.converged), "objective": rep.objective,
-                 "volume": rep.volume, "C": np.array(rep.C[:]).reshape(3, 3)}
+                 "volume": rep.volume, "C": np.array(rep.C[:]).reshape(3, 3),
+                 "unconverged_solves": rep.unconverged_solves}
 
 
 class Batch:
@@ -564,6 +634,7 @@ class Batch:
         h = ctypes.c_void_p()
         _capi.check(_capi.load().otm_batch_create(int(device), ctypes.byref(h)), "otm_batch_create")
         self._h = h
+        _capi.track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -571,6 +642,8 @@ class Batch:
             self._h = None
 
     def __del__(self):
+        if _capi.shutting_down():
+            return
         try:
             self.close()
         except Exception:
@@ -647,6 +720,7 @@ class Server:
         _capi.check(_capi.load().otm_server_create(int(device), float(hbm_budget_gb) * 2 ** 30, int(max_active),
                                                    ctypes.byref(h)), "otm_server_create")
         self._h = h
+        _capi.track(self)
         self.device = device
 
     def close(self):
@@ -655,6 +729,8 @@ class Server:
             self._h = None
 
     def __del__(self):
+        if _capi.shutting_down():
+            return
         try:
             self.close()
         except Exception:

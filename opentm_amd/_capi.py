@@ -20,6 +20,7 @@ OTM_E_STATE = -7
 OTM_E_OOM = -8
 OTM_HOST = 0
 OTM_DEVICE = 1
+OTM_ANY = 2
 PROF_CLASSES = ["fine_gs", "fine_apply", "fine_resid", "coarse_gs", "coarse_resid", "transfer",
                 "coarsest", "vector", "energy", "setup"]
 
@@ -46,6 +47,7 @@ class OtmConfig(ctypes.Structure):
         ("coarse_sweeps", _c.c_int),
         ("fp64_refine", _c.c_int),
         ("fp16_stencils", _c.c_int),
+        ("exact_smoother", _c.c_int),
     ]
 
 
@@ -112,6 +114,7 @@ class OtmCustomReport(ctypes.Structure):
         ("objective", _c.c_double),
         ("volume", _c.c_double),
         ("C", _c.c_double * 9),
+        ("unconverged_solves", _c.c_int),
     ]
 
 
@@ -134,6 +137,12 @@ SIGNATURES = {
     "otm_apply_level": (_i, [_p, _i, _p, _p, _i]),
     "otm_vcycle": (_i, [_p, _p, _p, _i]),
     "otm_get_stencil": (_i, [_p, _i, _p, _i]),
+    "otm_get_kappa": (_i, [_p, _p, _i]),
+    "otm_solve": (_i, [_p, _d, _i, _dp]),
+    "otm_mg_cycles": (_i, [_p, _i, _dp]),
+    "otm_transfer": (_i, [_p, _i, _i, _p, _p, _i]),
+    "otm_smooth_level": (_i, [_p, _i, _p, _p, _i, _i, _i]),
+    "otm_coarsest_solve": (_i, [_p, _p, _p, _i]),
     "otm_conv_filter": (_i, [_ip, _i, _d, _i, _i, _p, _p, _i]),
     "otm_density_params_init": (None, [_c.POINTER(OtmDensityParams)]),
     "otm_set_design": (_i, [_p, _p, _i]),
@@ -187,6 +196,44 @@ class OtmError(RuntimeError):
     def __init__(self, code, func, msg):
         super().__init__(f"{func} failed with status {code}: {msg}")
         self.code = code
+
+
+# Engine handles still open, closed in order by an exit hook while the HIP
+# runtime is alive; after it ran, finalisers make no engine calls (the
+# runtime's own teardown may already have started).
+_live = []
+_closed_at_exit = False
+
+
+def track(obj):
+    import weakref
+    _live.append(weakref.ref(obj))
+
+
+def shutting_down():
+    return _closed_at_exit
+
+
+def _close_all():
+    global _closed_at_exit
+    # servers and batches hold contexts: close them first
+    objs = [r() for r in _live]
+    objs = [o for o in objs if o is not None]
+    order = {"Server": 0, "Batch": 1, "OCOptimizer": 2, "MMAOptimizer": 3, "Homogenization_H": 4,
+             "DeviceBuffer": 5}
+    objs.sort(key=lambda o: order.get(type(o).__name__, 6))
+    for o in objs:
+        try:
+            o.close()
+        except Exception:
+            pass
+    _live.clear()
+    _closed_at_exit = True
+
+
+import atexit as _atexit  # noqa: E402
+
+_atexit.register(_close_all)
 
 
 def load():

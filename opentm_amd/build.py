@@ -70,6 +70,24 @@ def build_pybind(verbose=False):
     return out
 
 
+DROPIN = os.path.join(ROOT, "tests", "cxx", "dropin_gpu")
+
+
+def build_dropin(verbose=False):
+    """The C++ drop-in driver the GPU tests run (tests/cxx/dropin_gpu.cpp):
+    host code only, against include/opentm/opentm.hpp, libopentm.so and the
+    HIP runtime."""
+    cmd = ["g++", "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "cxx", "dropin_gpu.cpp"),
+           "-L" + HERE, "-lopentm", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../opentm_amd", "-Wl,-rpath,/opt/rocm/lib", "-o", DROPIN + ".tmp"]
+    _run(cmd)
+    os.replace(DROPIN + ".tmp", DROPIN)
+    if verbose:
+        print("built", DROPIN)
+    return DROPIN
+
+
 def build_oracle(verbose=False):
     _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
     if verbose:
@@ -80,6 +98,7 @@ def build_all(verbose=False):
     build_oracle(verbose)
     build_engine(verbose)
     build_pybind(verbose)
+    build_dropin(verbose)
 
 
 if __name__ == "__main__":

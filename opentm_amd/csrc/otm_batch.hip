@@ -992,7 +992,7 @@ int conclude(Member& m) {
   if (m.phase == Member::FALLBACK) {
     c->fallbacks++;
     try {
-      ctx_solve_exact(c, m.C);
+      (void)ctx_solve_exact(c, m.C);  // the target is checked below, as for every member
       for (int k = 0; k < 3; k++) m.relres[k] = c->relres[k];
       m.phase = Member::DONE;
     } catch (const OtmError& e) {
@@ -1008,7 +1008,19 @@ int conclude(Member& m) {
     memcpy(c->C, m.C, sizeof(c->C));
     c->has_solution = true;
   }
-  if (m.phase == Member::DONE) return OTM_OK;
+  if (m.phase == Member::DONE) {
+    // C and the statistics are published either way; a solve that ended
+    // above its target (iteration cap, stalled residual) says so
+    for (int k = 0; k < 3; k++)
+      if (!(m.relres[k] <= c->cfg.fem_rel_thres)) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "solve ended above the relative residual target %.3g: (%.3g, %.3g, %.3g)",
+                 c->cfg.fem_rel_thres, m.relres[0], m.relres[1], m.relres[2]);
+        m.fail_msg = buf;
+        return OTM_E_NOCONV;
+      }
+    return OTM_OK;
+  }
   return m.fail_code ? m.fail_code : OTM_E_NUMERIC;
 }
 
@@ -1079,10 +1091,11 @@ OTM_API int otm_batch_heat_matrix(otm_batch* b, otm_ctx* const* cells, int n, do
     for (Member& m : b->members) {
       const int st = conclude(m);
       if (status) status[m.out] = st;
-      if (st == OTM_OK)
+      if (st == OTM_OK || st == OTM_E_NOCONV)  // published also when the target was missed
         memcpy(C + 9 * m.out, m.C, sizeof(m.C));
-      else {
-        for (int q = 0; q < 9; q++) C[9 * m.out + q] = NAN;
+      if (st != OTM_OK) {
+        if (st != OTM_E_NOCONV)
+          for (int q = 0; q < 9; q++) C[9 * m.out + q] = NAN;
         if (first_fail == OTM_OK) {
           first_fail = st;
           msg = m.fail_msg;
@@ -1314,8 +1327,17 @@ struct otm_server {
 
 namespace {
 
+// Returns a request's context to the idle cache (keep) or frees it. A
+// request whose context was never created still holds the budget reserved
+// for it at admission; that reservation is given back here.
 void release_ctx(otm_server* sv, ServerReq& r, bool keep) {
-  if (!r.ctx) return;
+  if (!r.ctx) {
+    sv->resident -= r.bytes;
+    r.bytes = 0;
+    dfree(r.dsens);
+    r.dsens = nullptr;
+    return;
+  }
   if (keep) {
     sv->idle.emplace(shape_key(r.cfg), IdleCtx{r.ctx, r.dsens, r.bytes, sv->use_clock++});
   } else {
@@ -1325,6 +1347,7 @@ void release_ctx(otm_server* sv, ServerReq& r, bool keep) {
   }
   r.ctx = nullptr;
   r.dsens = nullptr;
+  r.bytes = 0;
 }
 
 bool evict_one(otm_server* sv) {
@@ -1398,7 +1421,8 @@ void server_loop(otm_server* sv) {
     for (auto& r : admit) {
       try {
         if (!r->ctx) {
-          if (otm_create(&r->cfg, &r->ctx) != OTM_OK) fail(OTM_E_OOM, otm_last_error());
+          const int cst = otm_create(&r->cfg, &r->ctx);
+          if (cst != OTM_OK) fail(cst, otm_last_error());
           r->dsens = dalloc<float>((size_t)r->cfg.reso[0] * r->cfg.reso[1] * r->cfg.reso[2]);
           std::lock_guard<std::mutex> lk(sv->mu);
           sv->created++;
@@ -1494,7 +1518,8 @@ void server_loop(otm_server* sv) {
         r->iters[k] = r->ctx->iters[k];
         r->relres[k] = m.relres[k];
       }
-      if (st == OTM_OK && r->want_sens) {
+      const bool solved = st == OTM_OK || st == OTM_E_NOCONV;  // NOCONV: results published, status says so
+      if (solved && r->want_sens) {
         CellDev d = describe(m);
         const double* q = r->dC;
         d.sw[0] = (float)q[0];
@@ -1535,7 +1560,7 @@ void server_loop(otm_server* sv) {
       std::lock_guard<std::mutex> lk(sv->mu);
       for (Member& m : left) {
         auto r = active[m.slot];
-        release_ctx(sv, *r, r->status == OTM_OK);
+        release_ctx(sv, *r, r->status == OTM_OK || r->status == OTM_E_NOCONV);
         finish_request(sv, *r, r->status, r->err);
         free_slots.push_back(m.slot);
         active.erase(m.slot);
@@ -1600,8 +1625,7 @@ OTM_API int otm_server_submit(otm_server* sv, const otm_request* req, long long*
   return guarded([&] {
     if (!sv || !req || !id || !req->kappa) fail(OTM_E_ARG, "null argument");
     const otm_config& c = req->cfg;
-    for (int a = 0; a < 3; a++)
-      if (c.reso[a] < 8 || (c.reso[a] & 1) || c.reso[a] > 1024) fail(OTM_E_UNSUPPORTED, "unsupported resolution");
+    (void)plan_levels(c);  // the shapes otm_create accepts, checked before anything is reserved
     if (c.device != sv->device) fail(OTM_E_ARG, "request for another device than the server");
     auto r = std::make_shared<ServerReq>();
     r->cfg = c;
@@ -1646,7 +1670,7 @@ OTM_API int otm_server_wait(otm_server* sv, long long id, otm_result* res, float
     };
     res->latency_ms = ms(r->t_submit, r->t_done);
     res->queue_ms = r->t_admit.time_since_epoch().count() ? ms(r->t_submit, r->t_admit) : res->latency_ms;
-    if (sens && r->status == OTM_OK && !r->sens.empty()) memcpy(sens, r->sens.data(), sizeof(float) * r->sens.size());
+    if (sens && (r->status == OTM_OK || r->status == OTM_E_NOCONV) && !r->sens.empty()) memcpy(sens, r->sens.data(), sizeof(float) * r->sens.size());
     if (r->status != OTM_OK) fail(r->status, r->err);
   });
 }

@@ -171,8 +171,10 @@ static void ctx_free(otm_ctx* c) {
   delete c;
 }
 
-static void build_hierarchy(otm_ctx* c) {
-  const otm_config& cfg = c->cfg;
+// Level sizes of the hierarchy a configuration gets (fine to coarse); throws
+// OTM_E_UNSUPPORTED for shapes the engine does not solve. Shared by context
+// creation and the server's admission check.
+std::vector<Geom> otm::plan_levels(const otm_config& cfg) {
   for (int a = 0; a < 3; a++) {
     if (cfg.reso[a] < 8 || (cfg.reso[a] & 1)) fail(OTM_E_UNSUPPORTED, "each resolution must be even and >= 8");
     if (cfg.reso[a] > 1024) fail(OTM_E_UNSUPPORTED, "resolution exceeds 1024 (grid.cpp:23-25 limit)");
@@ -193,6 +195,14 @@ static void build_hierarchy(otm_ctx* c) {
   const long long ncoarse = geoms.back().nn;
   if (ncoarse > 4096) fail(OTM_E_UNSUPPORTED, "coarsest level has " + std::to_string(ncoarse) +
                                                   " nodes (> 4096); use resolutions with more factors of two");
+  if (ncoarse > kTailLdsNodes) fail(OTM_E_UNSUPPORTED, "coarsest level too large for the tail");
+  return geoms;
+}
+
+static void build_hierarchy(otm_ctx* c) {
+  const otm_config& cfg = c->cfg;
+  const std::vector<Geom> geoms = plan_levels(cfg);
+  const long long ncoarse = geoms.back().nn;
   c->g0 = geoms[0];
   const long long n0 = c->g0.nn;
   c->kappa = dalloc<float>(n0);
@@ -251,7 +261,6 @@ static void build_hierarchy(otm_ctx* c) {
       break;
     }
   }
-  if (c->lv.back().g.nn > kTailLdsNodes) fail(OTM_E_UNSUPPORTED, "coarsest level too large for the tail");
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
   c->partials = dalloc<double>((size_t)kMaxReduceBlocks * kMaxReduce);
   c->red_out = dalloc<double>(16);
@@ -545,7 +554,10 @@ static void set_bricks(otm_ctx* c, bool on) {
   c->bricks_on = on;
 }
 
-static void pcg_solve(otm_ctx* c) {
+// Returns false when the solve ended above the relative residual target
+// (iteration cap, or the true residual stopped improving); the iterate and
+// its statistics are kept either way.
+static bool pcg_solve(otm_ctx* c) {
   const long long n = c->g0.nn;
   const unsigned nb = red_blocks_for(n);
   int its[3] = {0, 0, 0};
@@ -592,6 +604,17 @@ static void pcg_solve(otm_ctx* c) {
   if (c->cfg.verbose)
     fprintf(stderr, "[otm] PCG its=%d relres=(%.3e, %.3e, %.3e)%s\n", it, c->relres[0], c->relres[1], c->relres[2],
             c->bricks_on ? "" : " (exact GS)");
+  bool met = true;
+  for (int k = 0; k < 3; k++) met = met && relres[k] <= c->cfg.fem_rel_thres;
+  return met;
+}
+
+static std::string noconv_message(const otm_ctx* c) {
+  char buf[200];
+  snprintf(buf, sizeof(buf), "solve ended above the relative residual target %.3g: (%.3g, %.3g, %.3g) after %d PCG iterations",
+           c->cfg.fem_rel_thres, c->relres[0], c->relres[1], c->relres[2],
+           std::max(c->iters[0], std::max(c->iters[1], c->iters[2])));
+  return buf;
 }
 
 static void kappa_h(otm_ctx* c, double C[9]) {
@@ -649,6 +672,7 @@ OTM_API void otm_config_init(otm_config* cfg) {
   cfg->coarse_sweeps = 0;
   cfg->fp64_refine = 0;
   cfg->fp16_stencils = 0;
+  cfg->exact_smoother = 0;
 }
 
 OTM_API const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -677,6 +701,7 @@ OTM_API int otm_create(const otm_config* cfg, otm_ctx** out) {
       hip_check(hipSetDevice(c->device), "hipSetDevice");
       hip_check(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking), "hipStreamCreate");
       build_hierarchy(c);
+      if (cfg->exact_smoother) set_bricks(c, false);
       c->sync();
     } catch (...) {
       ctx_free(c);
@@ -758,19 +783,23 @@ OTM_API int otm_update(otm_ctx* ctx, const float* kappa, int pitch, int where) {
   });
 }
 
-static void do_heat_matrix(otm_ctx* c, double C[9]) {
+// Solve + kappa^H. Returns false when the solve ended above its relative
+// residual target; C, the iterate and the statistics are published anyway
+// (the reference prints "Failed to converge" and carries on, MG.cpp:101-118).
+static bool do_heat_matrix(otm_ctx* c, double C[9]) {
   if (!c->has_kappa) fail(OTM_E_STATE, "otm_heat_matrix before otm_update");
-  pcg_solve(c);
+  const bool met = pcg_solve(c);
   kappa_h(c, C);
   memcpy(c->C, C, sizeof(c->C));
   c->has_solution = true;
+  return met;
 }
 
 OTM_API int otm_heat_matrix(otm_ctx* ctx, double C[9]) {
   return guarded([&] {
     if (!ctx || !C) fail(OTM_E_ARG, "null argument");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-    do_heat_matrix(ctx, C);
+    if (!do_heat_matrix(ctx, C)) fail(OTM_E_NOCONV, noconv_message(ctx));
   });
 }
 
@@ -915,6 +944,181 @@ OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where) {
     vcycle(ctx, 0);
     hip_check(hipMemcpyAsync(z, ctx->z, sizeof(float) * 3 * n, kind_of(where, OTM_DEVICE), ctx->s), "copy z");
     ctx->sync();
+  });
+}
+
+// ---- MG_H entry points and level operators ---------------------------------------
+static void stationary_cycle(otm_ctx* c, double relres[3]) {
+  const long long n = c->g0.nn;
+  const unsigned nb = red_blocks_for(n);
+  const Reduce R = c->reducer_host(FIN_BB_RR, 0.0, 1.0 / (double)n);
+  if (c->x64)
+    launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x64, c->r, R, nb);
+  else
+    launch_fine_init_residual(c->s, c->g0, c->fc, c->kappa, c->x, c->xlo, c->r, R, nb);
+  c->read_pcg();
+  if (c->pcg_host->fault & 1) fail(OTM_E_NUMERIC, "non-finite residual (MG numerical explode)");
+  for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
+  launch_sub_mean3(c->s, n, c->r, c->pcg);
+  c->post_dot_mode = -1;
+  vcycle(c, 0);
+  if (c->x64)
+    launch_refine_add(c->s, 3 * n, c->x64, c->z);
+  else
+    launch_add_pair(c->s, 3 * n, c->x, c->xlo, c->z);
+}
+
+OTM_API int otm_solve(otm_ctx* ctx, double rtol, int warm, double relres[3]) {
+  return guarded([&] {
+    if (!ctx) fail(OTM_E_ARG, "null context");
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_solve before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    if (!warm) {
+      launch_fill(ctx->s, ctx->x, 3 * n, 0.f);
+      if (ctx->x64) launch_fill_d(ctx->s, ctx->x64, 3 * n, 0.0);
+      if (ctx->xlo) launch_fill(ctx->s, ctx->xlo, 3 * n, 0.f);
+    }
+    const double keep = ctx->cfg.fem_rel_thres;
+    if (rtol > 0) ctx->cfg.fem_rel_thres = rtol;
+    bool met = false;
+    std::string msg;
+    try {
+      met = pcg_solve(ctx);
+      if (!met) msg = noconv_message(ctx);
+    } catch (...) {
+      ctx->cfg.fem_rel_thres = keep;
+      throw;
+    }
+    ctx->cfg.fem_rel_thres = keep;
+    ctx->sync();
+    if (relres)
+      for (int k = 0; k < 3; k++) relres[k] = ctx->relres[k];
+    if (!met) fail(OTM_E_NOCONV, msg);
+  });
+}
+
+OTM_API int otm_mg_cycles(otm_ctx* ctx, int cycles, double relres[3]) {
+  return guarded([&] {
+    if (!ctx || cycles < 0) fail(OTM_E_ARG, "invalid argument");
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_mg_cycles before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    double rr[3] = {0, 0, 0};
+    for (int i = 0; i < cycles; i++) stationary_cycle(ctx, rr);
+    ctx->sync();
+    if (relres)
+      for (int k = 0; k < 3; k++) relres[k] = rr[k];
+  });
+}
+
+// Host or device arrays of `count` floats staged into a fresh device buffer.
+struct Staged {
+  float* d = nullptr;
+  ~Staged() { dfree(d); }
+};
+static float* stage_in(otm_ctx* c, Staged& st, const float* src, long long count, int where) {
+  st.d = dalloc<float>(count);
+  hip_check(hipMemcpyAsync(st.d, src, sizeof(float) * count, kind_of(OTM_DEVICE, where), c->s), "copy in");
+  return st.d;
+}
+static void stage_out(otm_ctx* c, float* dst, const float* src, long long count, int where) {
+  hip_check(hipMemcpyAsync(dst, src, sizeof(float) * count, kind_of(where, OTM_DEVICE), c->s), "copy out");
+  c->sync();
+}
+
+OTM_API int otm_transfer(otm_ctx* ctx, int level, int direction, const float* in, float* out, int where) {
+  return guarded([&] {
+    if (!ctx || !in || !out) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (level < 1 || level >= (int)ctx->lv.size()) fail(OTM_E_ARG, "level must be in 1..levels-1");
+    if (direction != 0 && direction != 1) fail(OTM_E_ARG, "direction must be 0 (restrict) or 1 (prolong)");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const Geom& F = ctx->lv[level - 1].g;
+    const Geom& C = ctx->lv[level].g;
+    Staged a, b;
+    if (direction == 0) {
+      const float* r = stage_in(ctx, a, in, 3 * F.nn, where);
+      b.d = dalloc<float>(3 * C.nn);
+      launch_restrict(ctx->s, F, C, r, b.d);
+      stage_out(ctx, out, b.d, 3 * C.nn, where);
+    } else {
+      const float* uc = stage_in(ctx, a, in, 3 * C.nn, where);
+      float* uf = stage_in(ctx, b, out, 3 * F.nn, where);
+      launch_prolong_add(ctx->s, F, C, uc, uf);
+      stage_out(ctx, out, uf, 3 * F.nn, where);
+    }
+  });
+}
+
+OTM_API int otm_smooth_level(otm_ctx* ctx, int level, const float* f, float* u, int sweeps, int mode, int where) {
+  return guarded([&] {
+    if (!ctx || !f || !u || sweeps < 1) fail(OTM_E_ARG, "invalid argument");
+    check_where(where);
+    if (level < 0 || level + 1 >= (int)ctx->lv.size()) fail(OTM_E_ARG, "level must be in 0..levels-2");
+    if (mode < 0 || mode > 3) fail(OTM_E_ARG, "mode must be 0..3");
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_smooth_level before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const Level& L = ctx->lv[level];
+    const long long n = L.g.nn;
+    const bool brick = mode >= 2, forward = (mode & 1) == 0;
+    if (brick && !L.u2) fail(OTM_E_UNSUPPORTED, "this level has no brick smoother");
+    Staged a, b, c2;
+    const float* df = stage_in(ctx, a, f, 3 * n, where);
+    float* du = stage_in(ctx, b, u, 3 * n, where);
+    if (brick) {
+      c2.d = dalloc<float>(3 * n);
+      if (level == 0)
+        launch_fine_smooth_brick(ctx->s, L.g, ctx->fc, ctx->kappa, df, du, c2.d, sweeps, forward, false);
+      else
+        launch_st_smooth_brick(ctx->s, L.g, stref(L), df, du, c2.d, sweeps, forward, false);
+      stage_out(ctx, u, c2.d, 3 * n, where);
+    } else {
+      const int ncol = level == 0 ? 4 : 8;
+      for (int s = 0; s < sweeps; s++)
+        for (int i = 0; i < ncol; i++) {
+          const int col = forward ? i : ncol - 1 - i;
+          if (level == 0)
+            launch_fine_gs_color(ctx->s, L.g, ctx->fc, ctx->kappa, df, du, col);
+          else
+            launch_st_gs_color(ctx->s, L.g, stref(L), df, du, col);
+        }
+      stage_out(ctx, u, du, 3 * n, where);
+    }
+  });
+}
+
+OTM_API int otm_coarsest_solve(otm_ctx* ctx, const float* f, float* u, int where) {
+  return guarded([&] {
+    if (!ctx || !f || !u) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_coarsest_solve before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const Level& L = ctx->lv.back();
+    const long long n = L.g.nn;
+    Staged a, b;
+    const float* df = stage_in(ctx, a, f, 3 * n, where);
+    b.d = dalloc<float>(3 * n);
+    launch_coarsest_solve(ctx->s, L.g, ctx->cBinv, df, b.d);
+    stage_out(ctx, u, b.d, 3 * n, where);
+  });
+}
+
+OTM_API int otm_get_kappa(otm_ctx* ctx, float* kappa, int where) {
+  return guarded([&] {
+    if (!ctx || !kappa) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_get_kappa before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    Staged a;
+    float* dst = kappa;
+    if (where == OTM_HOST) dst = a.d = dalloc<float>(n);
+    // the context holds kappa + 1e-9 (the floor of grid.cu:405)
+    launch_copy_pitched(ctx->s, ctx->g0, ctx->kappa, ctx->g0.nx, dst, -1e-9f);
+    if (where == OTM_HOST)
+      stage_out(ctx, kappa, dst, n, OTM_HOST);
+    else
+      ctx->sync();
   });
 }
 
@@ -1153,9 +1357,9 @@ OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
     launch_simp_fwd(ctx->s, n, src, (float)ctx->cfg.heat_ratio[0], (float)ctx->cfg.heat_ratio[1], 0.f, ctx->kraw,
                     ctx->dkappa);
     do_update(ctx, ctx->kraw, ctx->g0.nx, OTM_DEVICE, true);
-    do_heat_matrix(ctx, C);
     ctx->has_drho = false;
     ctx->taps_kind = p->filter_kind >= 0 ? ctx->taps_kind : -1;
+    if (!do_heat_matrix(ctx, C)) fail(OTM_E_NOCONV, noconv_message(ctx));
   });
 }
 
@@ -1216,6 +1420,9 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
       hip_check(hipHostMalloc((void**)&o->hm, sizeof(OcHost), hipHostMallocCoherent), "hipHostMalloc");
       memset(o->hm, 0, sizeof(OcHost));
       hip_check(hipHostGetDevicePointer((void**)&o->hm_dev, o->hm, 0), "hipHostGetDevicePointer");
+      // otm_oc_step runs the optimiser's kernels on a context's stream: the
+      // cleared word must be in memory before this call returns
+      hip_check(hipStreamSynchronize(o->s), "hipStreamSynchronize");
     } catch (...) {
       oc_free(o);
       throw;
@@ -1526,6 +1733,12 @@ OTM_API int otm_device_free(void* ptr) {
 
 OTM_API int otm_memcpy(void* dst, const void* src, size_t bytes, int dst_where, int src_where) {
   return guarded([&] {
+    if (dst_where == OTM_ANY || src_where == OTM_ANY) {
+      if (dst_where != OTM_ANY) check_where(dst_where);
+      if (src_where != OTM_ANY) check_where(src_where);
+      hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDefault), "hipMemcpy");
+      return;
+    }
     check_where(dst_where);
     check_where(src_where);
     hip_check(hipMemcpy(dst, src, bytes, kind_of(dst_where, src_where)), "hipMemcpy");
@@ -1543,7 +1756,7 @@ OTM_API int otm_synchronize(otm_ctx* ctx) {
 
 // Solve with exact multicolour Gauss-Seidel on every level (the batched
 // runtime's fallback for a cell whose brick smoother lost positivity).
-void otm::ctx_solve_exact(otm_ctx* c, double C[9]) {
+bool otm::ctx_solve_exact(otm_ctx* c, double C[9]) {
   set_bricks(c, false);
-  do_heat_matrix(c, C);
+  return do_heat_matrix(c, C);
 }

@@ -24,6 +24,18 @@ inline void ck(int status) {
   if (status != OTM_OK) throw Fail{status};
 }
 
+// otm_eval inside the loop: a solve that ends above its tolerance is
+// reported and the loop carries on with its result, as the reference's
+// MG_H::solveEquation prints "Failed to converge" and returns (MG.cpp:101-118).
+inline void eval_counted(otm_ctx* c, const otm_density_params* dp, double C[9], int* unconverged) {
+  const int st = otm_eval(c, dp, C);
+  if (st == OTM_E_NOCONV) {
+    (*unconverged)++;
+    return;
+  }
+  ck(st);
+}
+
 // objective of runCustom (Framework.cu:168-170) and its gradient wrt C
 double tensor_objective(const double C[9], const double t[6], double dC[9]) {
   static const int ii[6] = {0, 1, 2, 0, 2, 0}, jj[6] = {0, 1, 2, 1, 1, 2};
@@ -135,6 +147,7 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
     homo::ConvergeChecker criteria(p->finthres);
     double C[9] = {0}, dC[9], obj = 0.0;
     int itn = 0;
+    int unconverged = 0;
     bool converged = false;
     if (p->model == OTM_MODEL_MMA) {
       // MMAOptimizer mma(1, ne, 1, 0, 1e7, 1); setBound(1e-4, 1) (Framework.cu:172-173):
@@ -150,16 +163,17 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
       std::vector<float> vol((size_t)ne, (float)(1.0 / (double)ne));
       ck(otm_memcpy(dvdx.p, vol.data(), sizeof(float) * ne, OTM_DEVICE, OTM_HOST));
       for (itn = 0; itn < p->max_iter; itn++) {
-        ck(otm_eval(hom.c, &dp, C));
+        eval_counted(hom.c, &dp, C, &unconverged);
         obj = tensor_objective(C, p->target, dC);
+        const float f0val = (float)obj;  // the reference's objective value is a float (Framework.cu:181)
         ck(otm_backward(hom.c, dC, dfdx.f(), OTM_DEVICE));
-        if (criteria.is_converge(itn, obj)) {
+        if (criteria.is_converge(itn, f0val)) {
           converged = true;
           break;
         }
         ck(otm_get_design(hom.c, rho.f(), OTM_DEVICE));
         const float* dg[1] = {dfdx.f()};
-        const double gval[1] = {obj};
+        const double gval[1] = {f0val};
         ck(otm_mma_update(mma, itn, rho.f(), dvdx.f(), gval, dg, OTM_DEVICE, nullptr));
         ck(otm_set_design(hom.c, rho.f(), OTM_DEVICE));
       }
@@ -174,13 +188,14 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
       } guard{oc};
       homo::VolumeGovernor governor;
       for (itn = 0; itn < p->max_iter; itn++) {
-        ck(otm_eval(hom.c, &dp, C));
+        eval_counted(hom.c, &dp, C, &unconverged);
         obj = tensor_objective(C, p->target, dC);
+        const float val = (float)obj;  // a float in the reference (Framework.cu:211)
         double vf = 0.0, lower = 0.0;
         ck(otm_design_stats(hom.c, &vf, &lower));
-        governor.volume_check((float)obj, (float)lower, (float)vf, itn);
+        governor.volume_check(val, (float)lower, (float)vf, itn);
         ck(otm_backward(hom.c, dC, nullptr, OTM_DEVICE));
-        if (criteria.is_converge(itn, obj) && governor.get_current_decrease() < 1e-4) {
+        if (criteria.is_converge(itn, val) && governor.get_current_decrease() < 1e-4) {
           converged = true;
           break;
         }
@@ -197,6 +212,7 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
       ck(otm_design_stats(hom.c, &vf, nullptr));
       report->volume = vf;
       for (int q = 0; q < 9; q++) report->C[q] = C[q];
+      report->unconverged_solves = unconverged;
     }
     return OTM_OK;
   } catch (const Fail& f) {

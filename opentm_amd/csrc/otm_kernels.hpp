@@ -69,6 +69,8 @@ void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, co
 void launch_sum3(hipStream_t s, long long nn, const float* a, const Reduce& R, unsigned nblocks);
 void launch_sum3(hipStream_t s, long long nn, const double* a, const Reduce& R, unsigned nblocks);
 void launch_refine_add(hipStream_t s, long long n, double* x64, float* d);
+// x += d (x as an FP32 pair when xlo is given), then d = 0
+void launch_add_pair(hipStream_t s, long long n, float* x, float* xlo, float* d);
 void launch_sub_mean_cast(hipStream_t s, long long nn, double* x64, float* x32, const PcgScalars* P);
 void launch_sub_mean3(hipStream_t s, long long nn, float* a, const PcgScalars* P);
 void launch_sum2(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);

@@ -81,6 +81,27 @@ void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblock
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s, R, nv, (int)nblocks);
 }
 
+// x += d (x an FP32 pair when xlo is given), d cleared.
+__global__ void __launch_bounds__(kBlock) k_add_pair(long long n, float* __restrict__ x, float* __restrict__ xlo,
+                                                     float* __restrict__ d) {
+  for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
+    if (xlo) {
+      float hi = x[i], lo = xlo[i];
+      pair_add_prod(hi, lo, 1.f, d[i]);
+      x[i] = hi;
+      xlo[i] = lo;
+    } else {
+      x[i] += d[i];
+    }
+    d[i] = 0.f;
+  }
+}
+void launch_add_pair(hipStream_t s, long long n, float* x, float* xlo, float* d) {
+  unsigned nb = nblk(n);
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(k_add_pair, dim3(nb), dim3(kBlock), 0, s, n, x, xlo, d);
+}
+
 __global__ void __launch_bounds__(kBlock) k_f2d(long long n, const float* __restrict__ in, double* __restrict__ out) {
   const long long t = gtid();
   if (t < n) out[t] = (double)in[t];

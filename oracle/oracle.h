@@ -86,6 +86,30 @@ int orc_mmasub(int m, int n, int itn, double* x, const double* xmin, const doubl
                const double* dgdx, double* low, double* upp, double a0, const double* a,
                const double* c, const double* d, double move, double epsimin);
 
+/* ---- multigrid pieces (mg_impl.h; orc_ = FP64, orc32_ = FP32 build) ----
+ * Operators are ELL-27 as from orc_assemble; vectors hold nrhs right-hand
+ * sides, load case major. */
+/* Exact 8-colour Gauss-Seidel sweeps in the reference's colour order
+ * (grid.cu:943-987, colours of grid.cpp:520-545; reverse = 1: 7..0). */
+void orc_gs_ell(int nx, int ny, int nz, const double* A, const double* f, double* u, int sweeps, int reverse,
+                int nrhs);
+/* Brick (block) Gauss-Seidel: bricks of bx*by*bz nodes from the origin,
+ * cut at the edges, exact inside, couplings leaving a brick frozen at uin. */
+void orc_brick_gs_ell(int nx, int ny, int nz, const double* A, const double* f, const double* uin, double* uout,
+                      int bx, int by, int bz, int sweeps, int reverse, int nrhs);
+/* P^T A P onto the lattice of half the edges (trilinear P). */
+void orc_galerkin_ell(int nx, int ny, int nz, const double* Af, double* Ac);
+/* kappa^H by MG-preconditioned CG (same system as orc_homogenize). */
+int orc_mg_homogenize(int nx, int ny, int nz, const double* kappa, double rtol, double* chi, double C[9], double* S,
+                      int its[3], double relres[3]);
+int orc32_mg_homogenize(int nx, int ny, int nz, const double* kappa, double rtol, double* chi, double C[9],
+                        double* S, int its[3], double relres[3]);
+void orc32_gs_ell(int nx, int ny, int nz, const float* A, const float* f, float* u, int sweeps, int reverse,
+                  int nrhs);
+void orc32_brick_gs_ell(int nx, int ny, int nz, const float* A, const float* f, const float* uin, float* uout, int bx,
+                        int by, int bz, int sweeps, int reverse, int nrhs);
+void orc32_galerkin_ell(int nx, int ny, int nz, const float* Af, float* Ac);
+
 #ifdef __cplusplus
 }
 #endif

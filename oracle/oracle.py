@@ -39,6 +39,14 @@ def lib():
         L.orc_oc_update.restype = i
         L.orc_mmasub.argtypes = [i, i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, _dp, _dp, _dp, d, d]
         L.orc_mmasub.restype = i
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.orc_gs_ell.argtypes = [i, i, i, _dp, _dp, _dp, i, i, i]
+        L.orc_brick_gs_ell.argtypes = [i, i, i, _dp, _dp, _dp, _dp, i, i, i, i, i, i]
+        L.orc_galerkin_ell.argtypes = [i, i, i, _dp, _dp]
+        for nm in ("orc_mg_homogenize", "orc32_mg_homogenize"):
+            f = getattr(L, nm)
+            f.argtypes = [i, i, i, _dp, d, vp, _dp, vp, ip, _dp]
+            f.restype = i
         _LIB = L
     return _LIB
 
@@ -102,6 +110,59 @@ def homogenize(kappa, rtol=1e-12, pin=True, want_chi=False, want_S=False):
     it = lib().orc_homogenize(nx, ny, nz, k.reshape(-1), rtol, 1 if pin else 0, cp, C, sp)
     return (C.reshape(3, 3), None if chi is None else chi.reshape(3, n),
             None if S is None else S.reshape(6, n), it)
+
+
+def gs(shape, A, f, u, sweeps=1, reverse=False):
+    """Exact multicolour Gauss-Seidel in the reference's colour order (reverse:
+    7..0) on the ELL-27 operator A of a lattice of `shape` (nz, ny, nx); f and
+    u hold nrhs right-hand sides (nrhs, n). Returns the new u."""
+    nz, ny, nx = shape
+    fv = _d(f).reshape(-1, nx * ny * nz)
+    out = _d(u).reshape(fv.shape).copy()
+    lib().orc_gs_ell(nx, ny, nz, _d(A).reshape(-1), fv.reshape(-1), out.reshape(-1), int(sweeps),
+                     1 if reverse else 0, fv.shape[0])
+    return out
+
+
+def brick_gs(shape, A, f, u, brick, sweeps=1, reverse=False):
+    """Brick Gauss-Seidel (bricks of brick = (bx, by, bz) nodes, couplings
+    leaving a brick frozen at the entry u); returns the new u."""
+    nz, ny, nx = shape
+    fv = _d(f).reshape(-1, nx * ny * nz)
+    uin = _d(u).reshape(fv.shape)
+    out = np.zeros_like(uin)
+    bx, by, bz = brick
+    lib().orc_brick_gs_ell(nx, ny, nz, _d(A).reshape(-1), fv.reshape(-1), uin.reshape(-1), out.reshape(-1),
+                           bx, by, bz, int(sweeps), 1 if reverse else 0, fv.shape[0])
+    return out
+
+
+def galerkin(shape, A):
+    """P^T A P of an ELL-27 operator onto the lattice of half the edges."""
+    nz, ny, nx = shape
+    Ac = np.zeros(27 * (nx // 2) * (ny // 2) * (nz // 2))
+    lib().orc_galerkin_ell(nx, ny, nz, _d(A).reshape(-1), Ac)
+    return Ac.reshape(27, -1)
+
+
+def mg_homogenize(kappa, rtol=1e-10, want_chi=False, want_S=False, fp32=False):
+    """kappa^H by MG-preconditioned CG (FP64; fp32=True: the FP32 CPU port).
+    Returns (C, chi or None, S or None, its[3], relres[3])."""
+    k = _d(kappa)
+    nz, ny, nx = k.shape
+    n = k.size
+    C = np.zeros(9)
+    chi = np.zeros(3 * n) if want_chi else None
+    S = np.zeros(6 * n) if want_S else None
+    its = np.zeros(3, np.int32)
+    rr = np.zeros(3)
+    f = lib().orc32_mg_homogenize if fp32 else lib().orc_mg_homogenize
+    st = f(nx, ny, nz, k.reshape(-1), rtol, chi.ctypes.data_as(ctypes.c_void_p) if want_chi else None, C,
+           S.ctypes.data_as(ctypes.c_void_p) if want_S else None, its, rr)
+    if st != 0:
+        raise RuntimeError(f"orc_mg_homogenize failed ({st})")
+    return (C.reshape(3, 3), None if chi is None else chi.reshape(3, n), None if S is None else S.reshape(6, n),
+            its.tolist(), rr.tolist())
 
 
 def element_energy(shape, chi):

@@ -85,6 +85,8 @@ def test_cxx_dropin_header_compiles_and_reports_no_device(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    if "constructed with a GPU" in r.stdout:
+        pytest.skip("a GPU is present: the GPU path is covered by the gpu-marked drop-in test")
     assert "runtime_error" in r.stdout and "device" in r.stdout
 
 
