@@ -554,6 +554,21 @@ static void set_bricks(otm_ctx* c, bool on) {
   c->bricks_on = on;
 }
 
+// Removes the constant component of the iterate (translateForce,
+// grid.cpp:654-664) and refreshes its FP32 copy; in stream order with what
+// follows (no result the host needs).
+static void remove_constant(otm_ctx* c) {
+  const long long n = c->g0.nn;
+  const unsigned nb = red_blocks_for(n);
+  if (c->x64) {
+    launch_sum3(c->s, n, c->x64, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sub_mean_cast(c->s, n, c->x64, c->x, c->pcg);
+  } else {
+    launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
+    launch_sub_mean3(c->s, n, c->x, c->pcg);
+  }
+}
+
 // Returns false when the solve ended above the relative residual target
 // (iteration cap, or the true residual stopped improving); the iterate and
 // its statistics are kept either way.
@@ -587,15 +602,7 @@ static bool pcg_solve(otm_ctx* c) {
   }
   double relres[3];
   for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
-  // remove the constant component (translateForce, grid.cpp:654-664); in
-  // stream order with what follows (no result the host needs)
-  if (c->x64) {
-    launch_sum3(c->s, n, c->x64, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
-    launch_sub_mean_cast(c->s, n, c->x64, c->x, c->pcg);
-  } else {
-    launch_sum3(c->s, n, c->x, c->reducer(FIN_MEAN, 0.0, 1.0 / (double)n), nb);
-    launch_sub_mean3(c->s, n, c->x, c->pcg);
-  }
+  remove_constant(c);
   for (int k = 0; k < 3; k++) {
     c->relres[k] = relres[k];
     c->iters[k] = its[k] + (its[k] < it ? 1 : 0);
@@ -1005,6 +1012,7 @@ OTM_API int otm_mg_cycles(otm_ctx* ctx, int cycles, double relres[3]) {
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     double rr[3] = {0, 0, 0};
     for (int i = 0; i < cycles; i++) stationary_cycle(ctx, rr);
+    remove_constant(ctx);
     ctx->sync();
     if (relres)
       for (int k = 0; k < 3; k++) relres[k] = rr[k];

@@ -33,6 +33,7 @@
  * inner products.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -351,6 +352,10 @@ static void FN(mgpcg)(FN(Hier) * H, const REAL* b, REAL* x, double rtol, int max
         }
         its[k]++;
         const double rr = sqrt(FN(dotk)(n, rk, rk)) / bn[k];
+        if (getenv("ORC_VERBOSE")) {
+          fprintf(stderr, "[oracle mg] it %d load case %d relres %.3e\n", it + 1, k, rr);
+          fflush(stderr);
+        }
         if (rr <= rtol) active[k] = 0;
         still |= active[k];
       }

@@ -129,3 +129,46 @@ def test_density_file_round_trip(tmp_path):
     bad.write_bytes(raw[:40])
     with pytest.raises(opentm_amd.OtmError):
         opentm_amd.read_density(bad)
+
+
+def test_governor_and_checker_sequences(tmp_path):
+    """The drop-in ConvergeChecker / VolumeGovernor (Framework.cuh:312-411)
+    against decisions computed by hand from the reference's rules."""
+    import subprocess
+    exe = tmp_path / "gov"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cxx", "governor_check.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60).stdout.splitlines()
+    rows = {}
+    for line in out:
+        tag, *vals = line.split()
+        rows.setdefault(tag, []).append(vals)
+    # ConvergeChecker: "not converged" through iteration 11; then three
+    # consecutive relative changes <= 1e-3
+    assert rows["A"][0] == ["0"] * 12 + ["1", "1"]
+    assert rows["B"][0] == ["0"] * 15 + ["1"]
+    assert rows["D"][0] == ["0"] * 12 + ["1"]
+    # VolumeGovernor, scenario 1: value < 0 cuts B = 1 - 1 * (1 - 0.2) = 0.2
+    # and F = 0.8 (decrease F * D = 0.64); then value 0.3 with the bound
+    # reached stalls (change < 1e-4 from the second step on, and the first
+    # change -0.8 < 1e-4 counts too): the fifth stall widens B by
+    # 0.3 * D * F = 0.192
+    g1 = [(int(a), float(b), float(c)) for a, b, c in rows["G1"]]
+    assert g1[0][0] == 0 and g1[0][1] == pytest.approx(0.2, abs=1e-6) and g1[0][2] == pytest.approx(0.64, abs=1e-6)
+    for r_, b, _ in g1[1:5]:
+        assert r_ == 0 and b == pytest.approx(0.2, abs=1e-6)
+    assert g1[5][1] == pytest.approx(0.392, abs=1e-6)
+    # scenario 2: the change alternates sign from the second check on; the
+    # ninth check (eighth sign change) is not near the bound, so it cuts
+    # B = 1 - (1 - 0.1) = 0.1; the tenth cuts again with D = 0 (F = 0.64)
+    g2 = [(int(a), float(b), float(c)) for a, b, c in rows["G2"]]
+    for r_, b, _ in g2[:8]:
+        assert r_ == 0 and b == 1.0
+    assert g2[8][1] == pytest.approx(0.1, abs=1e-6) and g2[8][2] == pytest.approx(0.8 * 0.9, abs=1e-6)
+    assert g2[9][1] == pytest.approx(0.1, abs=1e-6) and g2[9][2] == pytest.approx(0.0, abs=1e-7)
+    # scenario 3: F = 0.8^k after k cuts while above 0.05; 0.8^14 < 0.05
+    assert [int(v[0]) for v in rows["G3"]] == [0] * 13 + [1]
+    # scenario 4: itn == 100 cuts B = 1 - (1 - 0.25) = 0.25
+    assert float(rows["G4"][0][0]) == pytest.approx(0.25, abs=1e-6)

@@ -1,0 +1,227 @@
+"""GPU tests of the drop-in boundary, the error paths and the element-wise
+sensitivity bar, through the C ABI (and the C++ classes for the drop-in)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import opentm_amd as otm
+from opentm_amd import _capi
+from opentm_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def make(n, rtol=1e-6, k1=1e-3, **kw):
+    cfg = otm.HomoConfig(n, (1.0, k1))
+    cfg.femRelThres = rtol
+    for key, v in kw.items():
+        setattr(cfg, key, v)
+    return otm.Homogenization_H(cfg)
+
+
+# ------------------------------------------------------------ sensitivities
+@pytest.mark.parametrize("dC", ["minus_identity", "general"])
+def test_sensitivity_elementwise(orc, dC):
+    """Per element: |s - s_oracle| <= 1e-4 * max(|s_oracle|, floor) with the
+    floor 1e-3 * max|s_oracle| (elements whose sensitivity is three orders
+    below the largest are held to 1e-7 of the largest in absolute terms)."""
+    n = 32
+    say("elementwise sensitivity", dC)
+    k = wl.simp(wl.plain_density(n, 5))
+    hom = make(n)
+    hom.update(k)
+    hom.heatMatrix()
+    D = -np.eye(3) if dC == "minus_identity" else np.array([[1.0, 0.3, -0.2], [0.1, -0.5, 0.4], [0.25, 0.0, 2.0]])
+    s = hom.Sensitivity(D).astype(np.float64)
+    _, _, So, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12, want_S=True)
+    so = orc.sensitivity(So, D)
+    floor = 1e-3 * np.abs(so).max()
+    err = np.abs(s - so) / np.maximum(np.abs(so), floor)
+    say("  max elementwise rel err", float(err.max()), "elements below floor", int((np.abs(so) < floor).sum()))
+    assert err.max() <= 1e-4
+
+
+# ------------------------------------------------------------ error paths
+def test_nan_kappa_is_numeric_and_context_recovers(orc):
+    """NaN in the conductivity: OTM_E_NUMERIC (the reference's "MG numerical
+    explode", MG.cpp:116); every kernel still exits normally, and the same
+    context solves the next, finite field correctly."""
+    n = 16
+    say("nan kappa")
+    k = wl.simp(wl.plain_density(n, 1))
+    bad = k.copy()
+    bad[3, 4, 5] = np.nan
+    hom = make(n)
+    hom.update(bad)
+    with pytest.raises(otm.OtmError) as e:
+        hom.heatMatrix()
+    assert e.value.code == _capi.OTM_E_NUMERIC
+    hom.reset_displacement()
+    hom.update(k)
+    C = hom.heatMatrix()
+    Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+    assert rel(C, Co) < 1e-5
+
+
+def test_iteration_cap_returns_noconv():
+    """max_cg_iter = 1: OTM_E_NOCONV from the single-cell and the batched
+    solve, with kappa^H and the statistics still published."""
+    n = 32
+    say("noconv")
+    k = wl.simp(wl.plain_density(n, 2))
+    hom = make(n)
+    hom.config.max_cg_iter = 1
+    hom.build(hom.config)
+    hom.update(k)
+    with pytest.raises(otm.OtmError) as e:
+        hom.heatMatrix()
+    assert e.value.code == _capi.OTM_E_NOCONV
+    its, rr = hom.solve_stats()
+    assert max(its) <= 2 and max(rr) > 1e-6
+    hom2 = make(n)
+    hom2.config.max_cg_iter = 1
+    hom2.build(hom2.config)
+    b = otm.Batch()
+    b.update([hom2], [k])
+    C, st = b.heat_matrix([hom2], status=True)
+    assert st == [_capi.OTM_E_NOCONV]
+    assert np.all(np.isfinite(C))
+    b.close()
+
+
+def test_exact_smoother_switch_matches_oracle(orc):
+    """exact_smoother = 1 runs the fallback every solve takes by itself when
+    the brick preconditioner loses positivity: exact multicolour
+    Gauss-Seidel on every level; kappa^H still matches the oracle."""
+    n = 32
+    say("exact smoother")
+    k = wl.simp(wl.plain_density(n, 4))
+    Co, _, _, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12)
+    hom = make(n, exact_smoother=1)
+    hom.update(k)
+    C = hom.heatMatrix()
+    its, rr = hom.solve_stats()
+    say("  its", its, "relres", rr)
+    assert rel(C, Co) < 1e-5 and max(rr) <= 1e-6
+    ref = make(n)
+    ref.update(k)
+    ref.heatMatrix()
+    # the exact and the brick schedules are different preconditioners
+    assert hom.solve_stats()[0] != ref.solve_stats()[0] or rel(hom.get_chi(), ref.get_chi()) > 0
+
+
+def test_server_unsupported_shape_keeps_budget():
+    """A request whose hierarchy the engine does not solve (90^3: the
+    coarsest level would have 45^3 nodes) is refused at submit with
+    OTM_E_UNSUPPORTED, and a request sized to the whole budget is still
+    admitted afterwards."""
+    say("server shape check")
+    from opentm_amd._capi import OtmError
+    sv = otm.Server(hbm_budget_gb=0.05, max_active=4)
+    with pytest.raises(OtmError) as e:
+        sv.submit(np.full((90, 90, 90), 0.5, np.float32))
+    assert e.value.code == _capi.OTM_E_UNSUPPORTED
+    k = wl.simp(wl.plain_density(64, 3))
+    rid = sv.submit(k, dC=-np.eye(3))
+    C, _, info = sv.wait(rid)
+    assert np.all(np.isfinite(C)) and max(info["relres"]) <= 1e-6
+    st = sv.stats()
+    assert st["resident_bytes"] <= st["budget_bytes"]
+    sv.close()
+
+
+def test_server_noconv_status_carried():
+    """The server reports a request that hit its iteration cap with
+    OTM_E_NOCONV (result published)."""
+    say("server noconv")
+    n = 32
+    sv = otm.Server(hbm_budget_gb=1.0, max_active=4)
+    cfg = otm.HomoConfig(n, (1.0, 1e-3))
+    cfg.max_cg_iter = 1
+    rid = sv.submit(wl.simp(wl.plain_density(n, 6)), cfg=cfg, dC=-np.eye(3))
+    with pytest.raises(otm.OtmError) as e:
+        sv.wait(rid)
+    assert e.value.code == _capi.OTM_E_NOCONV
+    sv.close()
+
+
+# ------------------------------------------------------------ MG_H entry points
+def test_solve_and_stationary_cycles(orc):
+    """otm_solve (MG_H::solveEquation) reaches a tolerance below the
+    context's; otm_mg_cycles (MG_H::v_cycle iterated as the reference
+    iterates it) reduces the residual every cycle and converges to the same
+    fluctuation fields."""
+    n = 32
+    say("solve / mg cycles")
+    k = wl.simp(wl.plain_density(n, 8))
+    hom = make(n, rtol=1e-6, fp64_refine=1)
+    hom.update(k)
+    rr = hom.solve(1e-9, warm=False)
+    assert max(rr) <= 1e-9
+    chi_pcg = hom.get_chi()
+    hom.reset_displacement()
+    first = max(hom.mg_cycles(1))
+    hom.mg_cycles(24)
+    r = max(hom.mg_cycles(1))  # measured at the start of the 26th cycle
+    say("  stationary relres", first, "->", r)
+    assert first == pytest.approx(1.0, rel=1e-6)  # zero start: the residual is the load
+    assert r < 1e-7
+    assert rel(hom.get_chi(), chi_pcg) < 1e-4
+
+
+# ------------------------------------------------------------ C++ drop-in
+def test_cxx_dropin_on_gpu(orc, tmp_path):
+    """The C++ classes with the reference's names, driven the way the
+    reference's callers drive them (device memory, TensorBuffer pitch),
+    against the oracle: Homogenization_H::update(rho, 128) / heatMatrix /
+    Sensitivity, MG_H::solveEquation, Grid_H::writeDensity and
+    MMAOptimizer::update with device gval."""
+    n = 32
+    say("c++ drop-in")
+    exe = os.path.join(ROOT, "tests", "cxx", "dropin_gpu")
+    assert os.path.exists(exe), "tests/cxx/dropin_gpu is built by __graft_entry__.build()"
+    k = wl.simp(wl.plain_density(n, 9), k1=1e-3)
+    otm.write_density(tmp_path / "kappa.bin", k)
+    rng = np.random.default_rng(9)
+    ne = n ** 3
+    x = rng.uniform(0.2, 0.6, ne).astype(np.float32)
+    dfdx = np.full(ne, 1.0 / ne, np.float32)
+    dgdx = (rng.standard_normal(ne) * 1e-3).astype(np.float32)
+    gval = np.array([0.05], np.float32)
+    np.concatenate([x, dfdx, dgdx, gval]).astype(np.float32).tofile(tmp_path / "mma_in.bin")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=300)
+    say("  ", r.stdout.strip(), r.stderr.strip()[-500:])
+    assert r.returncode == 0
+    C = np.loadtxt(tmp_path / "C.txt").reshape(3, 3)
+    Co, _, So, _ = orc.homogenize(k.astype(np.float64), rtol=1e-12, want_S=True)
+    assert rel(C, Co) < 1e-5
+    sens = np.fromfile(tmp_path / "sens.bin", np.float32).reshape(n * n, 128)
+    assert not sens[:, n:].any()  # the pitched padding is zero-filled (homogenization.cu:425)
+    assert rel(sens[:, :n].reshape(-1), orc.sensitivity(So, -np.eye(3))) < 1e-4
+    lines = open(tmp_path / "mg.txt").read().split("\n")
+    nl, rr = lines[0].split()
+    assert int(nl) == 4 and float(rr) <= 1e-9
+    assert [tuple(map(int, l.split())) for l in lines[1:5]] == [(32, 32, 32), (16, 16, 16), (8, 8, 8), (4, 4, 4)]
+    kout = otm.read_density(tmp_path / "kappa_out.bin")
+    assert np.all(np.abs(kout - k) <= 1.2e-7 * np.abs(k))  # one FP32 rounding of the stored floor
+    xg = np.fromfile(tmp_path / "mma_x.bin", np.float32)
+    mma = orc.MMA(1, ne, 1.0, 0.0, 1e7, 1.0, 1e-4, 1.0)
+    xo, _ = mma.update(0, x.astype(np.float64), dfdx.astype(np.float64), gval.astype(np.float64),
+                       dgdx.astype(np.float64)[None])
+    say("  mma max diff", float(np.abs(xg - xo).max()))
+    assert np.abs(xg - xo).max() < 2e-6

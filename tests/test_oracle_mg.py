@@ -1,0 +1,217 @@
+"""Pins the oracle's multigrid pieces (oracle/mg_impl.h) by independent
+formulations: Gauss-Seidel as a matrix splitting solved with a dense
+triangular solve, brick Gauss-Seidel as a block splitting, the stencil
+Galerkin product against the dense P^T K P, and the MG-preconditioned solver
+against the Jacobi-PCG oracle and closed forms.
+"""
+import numpy as np
+import pytest
+
+
+def dense(A, shape):
+    """Dense matrix of an ELL-27 periodic operator on a (nz, ny, nx) lattice."""
+    nz, ny, nx = shape
+    N = nx * ny * nz
+    D = np.zeros((N, N))
+    v = np.arange(N)
+    x, y, z = v % nx, v // nx % ny, v // (nx * ny)
+    for o in range(27):
+        dx, dy, dz = o % 3 - 1, o // 3 % 3 - 1, o // 9 - 1
+        j = (x + dx) % nx + nx * ((y + dy) % ny + ny * ((z + dz) % nz))
+        np.add.at(D, (v, j), A[o])
+    return D
+
+
+def prolongation(shape):
+    """Trilinear P (grid.cu:881-940) from the lattice of half the edges."""
+    nz, ny, nx = shape
+    cz, cy, cx = nz // 2, ny // 2, nx // 2
+    P = np.zeros((nx * ny * nz, cx * cy * cz))
+    w1 = {0: 1.0, 1: 0.5, -1: 0.5}
+    for z, y, x in np.ndindex(nz, ny, nx):
+        for Z, Y, X in np.ndindex(cz, cy, cx):
+            w = 1.0
+            for a, A, n in ((x, X, nx), (y, Y, ny), (z, Z, nz)):
+                d = (a - 2 * A) % n
+                d = d - n if d > n // 2 else d
+                w *= w1.get(d, 0.0)
+            if w:
+                P[x + nx * (y + ny * z), X + cx * (Y + cy * Z)] = w
+    return P
+
+
+def colours(shape):
+    """The reference's colour of each node: parity of the padded position
+    (x+1, y+1, z+1) (grid.cpp:520-545)."""
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return (((x + 1) & 1) + 2 * ((y + 1) & 1) + 4 * ((z + 1) & 1)).reshape(-1)
+
+
+def splitting_sweep(K, order_key, f, u):
+    """One Gauss-Seidel sweep in the node order given by order_key (nodes
+    with a smaller key first; equal keys must not couple): solve
+    (D + L) u' = f - U u with L the couplings to earlier nodes."""
+    N = K.shape[0]
+    earlier = order_key[None, :] < order_key[:, None]
+    later = order_key[None, :] > order_key[:, None]
+    same = (order_key[None, :] == order_key[:, None]) & ~np.eye(N, dtype=bool)
+    assert not np.any(K[same]), "nodes relaxed together must not couple"
+    L = np.where(earlier, K, 0.0)
+    U = np.where(later, K, 0.0)
+    return np.linalg.solve(np.diag(np.diag(K)) + L, f - U @ u)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 4, 8)])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_gs_is_the_colour_ordered_splitting(orc, shape, reverse):
+    rng = np.random.default_rng(3)
+    k = rng.uniform(1e-3, 1.0, shape)
+    A = orc.assemble(k + 1e-9)
+    K = dense(A, shape)
+    N = K.shape[0]
+    f = rng.standard_normal((2, N))
+    u = rng.standard_normal((2, N))
+    key = colours(shape)
+    if reverse:
+        key = 7 - key
+    out = orc.gs(shape, A, f, u, sweeps=2, reverse=reverse)
+    for r in range(2):
+        ref = splitting_sweep(K, key, f[r], splitting_sweep(K, key, f[r], u[r]))
+        np.testing.assert_allclose(out[r], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_gs_fixed_point_and_last_colour_residual(orc):
+    """The exact solution is a fixed point, and after a forward sweep the
+    nodes of the last colour (7) satisfy their equations exactly."""
+    shape = (8, 8, 8)
+    rng = np.random.default_rng(4)
+    A = orc.assemble(rng.uniform(1e-3, 1.0, shape) + 1e-9)
+    K = dense(A, shape)
+    f = rng.standard_normal(K.shape[0])
+    f -= f.mean()
+    xs = np.linalg.lstsq(K, f, rcond=None)[0]
+    np.testing.assert_allclose(orc.gs(shape, A, f[None], xs[None])[0], xs, atol=1e-10 * np.abs(xs).max())
+    u = orc.gs(shape, A, f[None], rng.standard_normal((1, K.shape[0])))[0]
+    r = f - K @ u
+    last = colours(shape) == 7
+    assert np.abs(r[last]).max() < 1e-12 * np.abs(f).max()
+    assert np.abs(r[~last]).max() > 1e-3 * np.abs(f).max()
+
+
+def brick_of(shape, brick):
+    nz, ny, nx = shape
+    bx, by, bz = brick
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    nbx, nby = -(-nx // bx), -(-ny // by)
+    return ((x // bx) + nbx * ((y // by) + nby * (z // bz))).reshape(-1)
+
+
+@pytest.mark.parametrize("shape,brick", [((8, 8, 8), (4, 4, 4)), ((12, 12, 12), (8, 8, 8)), ((8, 8, 16), (8, 4, 4)),
+                                         ((4, 4, 4), (8, 8, 8))])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_brick_gs_is_the_block_splitting(orc, shape, brick, reverse):
+    """Inside a brick the sweeps are exact in colour order; couplings that
+    leave the brick (including through the periodic seam of a brick that
+    spans a whole axis) use the entry values: u' = (D + L_B)^{-1}
+    (f - U_B u' - O_B u_in) iterated per sweep with O_B frozen."""
+    rng = np.random.default_rng(5)
+    A = orc.assemble(rng.uniform(1e-3, 1.0, shape) + 1e-9)
+    N = A.shape[1]
+    nz, ny, nx = shape
+    # couplings of the dense matrix split by whether they stay in the brick,
+    # with unwrapped coordinates: a periodic neighbour is "inside" only if
+    # the unwrapped position is inside the brick's own box
+    bx, by, bz = brick
+    v = np.arange(N)
+    x, y, z = v % nx, v // nx % ny, v // (nx * ny)
+    inside = np.zeros((N, N))
+    outside = np.zeros((N, N))
+    for o in range(27):
+        dx, dy, dz = o % 3 - 1, o // 3 % 3 - 1, o // 9 - 1
+        X, Y, Z = x + dx, y + dy, z + dz
+        same = (X // bx == x // bx) & (Y // by == y // by) & (Z // bz == z // bz) & \
+               (X >= 0) & (Y >= 0) & (Z >= 0) & (X < nx) & (Y < ny) & (Z < nz)
+        j = X % nx + nx * (Y % ny + ny * (Z % nz))
+        for i in range(N):
+            if o == 13:
+                continue
+            (inside if same[i] else outside)[i, j[i]] += A[o, i]
+    Dg = np.diag(A[13])
+    key = colours(shape)
+    if reverse:
+        key = 7 - key
+    Lb = np.where(key[None, :] < key[:, None], inside, 0.0)
+    Ub = np.where(key[None, :] > key[:, None], inside, 0.0)
+    f = rng.standard_normal(N)
+    u0 = rng.standard_normal(N)
+    out = orc.brick_gs(shape, A, f[None], u0[None], brick, sweeps=2, reverse=reverse)[0]
+    u = u0.copy()
+    for _ in range(2):
+        u = np.linalg.solve(Dg + Lb, f - Ub @ u - outside @ u0)
+    np.testing.assert_allclose(out, u, rtol=0, atol=1e-12 * np.abs(u).max())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (8, 4, 12)])
+def test_galerkin_equals_dense_triple_product(orc, shape):
+    rng = np.random.default_rng(6)
+    A = orc.assemble(rng.uniform(1e-3, 1.0, shape) + 1e-9)
+    Ac = orc.galerkin(shape, A)
+    P = prolongation(shape)
+    Kc = P.T @ dense(A, shape) @ P
+    cshape = tuple(s // 2 for s in shape)
+    np.testing.assert_allclose(dense(Ac, cshape), Kc, rtol=0, atol=1e-14 * np.abs(Kc).max())
+    np.testing.assert_allclose(P.sum(axis=1), 1.0)  # P 1 = 1
+
+
+@pytest.mark.parametrize("reso,seed", [((16, 16, 16), 0), ((24, 16, 8), 1)])
+def test_mg_homogenize_matches_jacobi_cg(orc, reso, seed):
+    nx, ny, nz = reso
+    k = np.random.default_rng(seed).uniform(1e-3, 1.0, (nz, ny, nx))
+    C1, chi1, S1, _ = orc.homogenize(k, rtol=1e-12, pin=False, want_chi=True, want_S=True)
+    C2, chi2, S2, its, rr = orc.mg_homogenize(k, rtol=1e-11, want_chi=True, want_S=True)
+    assert max(rr) <= 1e-11 and max(its) < 40
+    np.testing.assert_allclose(C2, C1, rtol=0, atol=1e-12 * np.abs(C1).max())
+    np.testing.assert_allclose(S2, S1, rtol=0, atol=1e-9 * np.abs(S1).max())
+    # the FP32 port reaches the plain path's 1e-6
+    C3, _, _, its3, rr3 = orc.mg_homogenize(k, rtol=1e-6, fp32=True)
+    assert max(rr3) <= 1e-6
+    assert np.abs(C3 - C1).max() <= 1e-6 * np.abs(C1).max()
+
+
+def test_mg_homogenize_laminate_closed_form(orc):
+    """x-laminate: harmonic mean across the layers (the reference's load
+    case 1 is along grid x), arithmetic mean along them."""
+    n = 32
+    vals = np.where(np.arange(n) % 8 < 3, 1.0, 0.01)
+    k = np.broadcast_to(vals[None, None, :], (n, n, n)).copy()
+    C, _, _, its, rr = orc.mg_homogenize(k, rtol=1e-12)
+    ve = vals + 1e-9
+    assert C[1, 1] == pytest.approx(1.0 / np.mean(1.0 / ve), rel=1e-10)
+    assert C[0, 0] == pytest.approx(np.mean(ve), rel=1e-10)
+    assert C[2, 2] == pytest.approx(np.mean(ve), rel=1e-10)
+
+
+def test_framework_restatement_hand_sequences():
+    """The oracle's ConvergeChecker / VolumeGovernor restatement against the
+    same hand-computed decisions the C++ drop-in is held to."""
+    from oracle.framework import ConvergeChecker, VolumeGovernor
+    c = ConvergeChecker(1e-3)
+    assert [c.is_converge(it, 2.0) for it in range(14)] == [False] * 12 + [True, True]
+    c = ConvergeChecker(1e-3)
+    assert [c.is_converge(it, 1.5 if it == 11 else 1.0) for it in range(16)] == [False] * 15 + [True]
+    g = VolumeGovernor()
+    assert g.volume_check(-0.5, 0.2, 0.9, 0) == 0
+    assert float(g.get_volume_bound()) == pytest.approx(0.2, abs=1e-6)
+    assert float(g.get_current_decrease()) == pytest.approx(0.64, abs=1e-6)
+    for it in range(1, 6):
+        g.volume_check(0.3, 0.2, 0.2, it)
+    assert float(g.get_volume_bound()) == pytest.approx(0.392, abs=1e-6)
+    g = VolumeGovernor()
+    bounds = []
+    for it in range(10):
+        g.volume_check(0.6 if it % 2 else 0.5, 0.1, 0.9, it)
+        bounds.append(float(g.get_volume_bound()))
+    assert bounds[:8] == [1.0] * 8 and bounds[8] == pytest.approx(0.1, abs=1e-6)
+    g = VolumeGovernor()
+    assert [g.volume_check(-1.0, 0.0, 0.5, it) for it in range(14)] == [0] * 13 + [1]
