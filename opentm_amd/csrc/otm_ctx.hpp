@@ -268,6 +268,8 @@ namespace otm {
 // heatMatrix with exact multicolour Gauss-Seidel on every level; returns
 // false when the solve ended above the relative residual target.
 bool ctx_solve_exact(otm_ctx* c, double C[9]);
+// Normalised radial filter taps (kind 0 Spline4, 1 Linear; radius r).
+Taps make_taps(int kind, double r, int periodic);
 // Level sizes (fine to coarse) of the hierarchy of a configuration; throws
 // OTM_E_UNSUPPORTED for shapes the engine does not solve.
 std::vector<Geom> plan_levels(const otm_config& cfg);

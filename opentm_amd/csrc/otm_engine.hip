@@ -78,10 +78,12 @@ GalerkinConsts make_galerkin_consts(double h) {
   return G;
 }
 
+}  // namespace
+
 // Radial taps (TensorExpression.h:1053-1093): Spline4 1-6q+8q^1.5-3q^2,
 // Linear 1-sqrt(q), q = |d|^2/r^2 < 1, window half-width int(r+0.5), each
 // weight divided by the window total.
-Taps make_taps(int kind, double r, int periodic) {
+Taps otm::make_taps(int kind, double r, int periodic) {
   if (!(r > 0.0)) fail(OTM_E_ARG, "filter radius must be positive");
   const int h = (int)(r + 0.5);
   if ((2 * h + 1) * (2 * h + 1) * (2 * h + 1) > kMaxTaps) fail(OTM_E_UNSUPPORTED, "filter radius too large (max 3.49)");
@@ -115,8 +117,6 @@ Taps make_taps(int kind, double r, int periodic) {
   }
   return T;
 }
-
-}  // namespace
 
 struct otm_oc {
   int ne = 0;
