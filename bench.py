@@ -1,28 +1,32 @@
 #!/usr/bin/env python3
 """Throughput of the solve -> sensitivity -> update loop on one MI355X.
 
-Workload (the plain optimisation loop): a 128^3 periodic cell, design
-density starting at 0.3 + 0.05 * seeded cos-mode noise, Spline4 r=1.5
-density filter, kappa = 1e-3 + (1 - 1e-3) rho^3, maximise trace(kappa^H)
-at volume 0.3 with OC (move 0.2, damping 0.5, bisection to 1e-6 on the
-volume, periodic linear r=2 sensitivity filter). One step = one OC
+Default workload (the largest configured cell, 256^3 = 16.8M elements): the
+precision form of the design loop. Density from the seeded cos-mode
+generator at volume fraction 0.25, Spline4 r=1.5 density filter,
+kappa = 1e-3 + (1 - 1e-3) rho^3; MMA (m = 3) maximising C00 subject to
+C11, C22 <= 0.05 and volume <= 0.25. Every step is one MMA design
 iteration = filter + SIMP + Galerkin hierarchy rebuild + three-load-case
-MG-PCG solve (FP32, FP64 reductions, relative residual 1e-6, warm start)
-+ kappa^H + sensitivity + adjoint filter + OC update. Every step is one
-homogenization + sensitivity solve, so value = solves/s.
+MG-PCG solve with FP64 iterative refinement to relative residual 1e-8
+(FP32 smoothing and transfers, FP16-stored coarse stencils with a per-level
+scale, FP64 dots, kappa^H and coarsest inverse) + kappa^H + three
+sensitivity contractions through the adjoint filter + the MMA update. One
+step is one homogenization + sensitivity solve, so value = solves/s. The
+same loop on the plain FP32 path (relative residual 1e-6) is timed beside
+it, and the 128^3 OC loop (the plain optimisation configuration) as extra
+keys.
 
 Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
 
---workload precision runs the precision form instead: a 256^3 cell
-(cos-mode density at volume 0.25), MMA maximising C00 with C11, C22 <= 0.05
-and volume <= 0.25, solved with FP64 iterative refinement to 1e-8 and FP16
-coarse stencils; the same loop is then timed on the plain FP32 path, and
---oracle adds the FP64 oracle's kappa^H and sensitivity errors on the
-first design (minutes of host time at 256^3).
+Other workloads: --workload oc (128^3 OC loop alone), batch (96 cells),
+serve (Poisson arrivals), copy (calibration of the byte counters).
+--oracle adds the FP64 oracle's kappa^H and sensitivity errors on the first
+256^3 design (MG-preconditioned CG on the host, minutes).
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -33,17 +37,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "homog. solves/s + p50/p99 latency at κᴴ rel err ≤1e-5; stencil HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fine brick smoother from rocprofv3 FETCH_SIZE /
-# WRITE_SIZE runs of this workload, scaled by a copy-kernel calibration
-# (profiles/traffic_fine_brick_128.md gives the commands and figures).
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_fine_brick_128.json")
+FINE_KERNEL = "k_fine_smooth_brick<32>"
 
 
 def measured_traffic(kernel, reso):
-    """HBM bytes per launch of `kernel` at `reso`, from the committed counter
-    summary, or None when no summary matches this kernel and size."""
+    """HBM bytes per launch of `kernel` at `reso` from the committed counter
+    summary profiles/traffic_fine_brick_<reso>.json (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy-kernel calibration; the matching .md
+    gives the commands), or None when there is none for this kernel/size."""
+    path = os.path.join(ROOT, "profiles", f"traffic_fine_brick_{reso}.json")
     try:
-        with open(TRAFFIC_PROFILE) as fh:
+        with open(path) as fh:
             t = json.load(fh)
     except (OSError, ValueError):
         return None
@@ -52,41 +56,165 @@ def measured_traffic(kernel, reso):
     return t.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(reso, seed, rtol):
-    """FP64 oracle (OpenMP) timed on the host cores on a bounded sample of
-    the same cell: operator assembly + one of the three load cases (cold
-    start) + element energies; the solve time is counted three times."""
+def host_description():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "omp_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline_mg(n_full, seed, rtol, mean):
+    """The FP64 MG-preconditioned CG of the oracle (OpenMP on the host cores;
+    same method: Galerkin V(2,2) multicolour Gauss-Seidel in the reference's
+    colour order) on a bounded sample: one full homogenization + sensitivity
+    of the same generator's design at half the edge (1/8 of the elements),
+    scaled by 8 to the configured cell (assembly, Galerkin setup, smoothing
+    and element energies all scale with the element count, and MG-PCG
+    iteration counts do not depend on the size: 7 at 64^3 and at 128^3)."""
     import oracle.oracle as orc
     from opentm_amd import workloads as wl
-    rho = wl.oc_start_density(reso, seed).astype(np.float64)
+    ns = n_full // 2
+    rho = wl.plain_density(ns, seed, mean=mean).astype(np.float64)
+    t0 = time.perf_counter()
     rf = orc.filter_fwd(rho, 0, 1.5, False)
     kappa = 1e-3 + (1 - 1e-3) * rf ** 3
-    ke = kappa + 1e-9
+    C, _, S, its, rr = orc.mg_homogenize(kappa, rtol=rtol, want_S=True)
+    for dC in (-np.diag([1.0, 0, 0]), np.diag([0, 1.0, 0]), np.diag([0, 0, 1.0])):
+        s = orc.sensitivity(S, dC).reshape(rf.shape)
+        orc.filter_adj(s * 3 * (1 - 1e-3) * rf ** 2, 0, 1.5, False)
+    dt = time.perf_counter() - t0
+    scale = (n_full / ns) ** 3
+    host = host_description()
+    cores = int(host["omp_threads"] or host["nproc"] or 1)
+    return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 MG-PCG (oracle, OpenMP) on the same generator's {ns}^3 design to rtol {rtol:g}: "
+                      f"filter + homogenization ({max(its)} PCG iterations) + element energies + 3 sensitivity "
+                      f"contractions in {dt:.1f} s, scaled x{scale:g} to {n_full}^3 (linear in the element count)",
+            "host": host}
+
+
+def timed_loop(step, steps, warmup, sync, label, on_timed_start=None):
+    """W untimed steps, then exactly K steps bracketed by synchronisations;
+    returns (total seconds, per-step seconds, per-step results)."""
+    for i in range(warmup):
+        step()
+        print(f"[bench] {label} warmup {i + 1}/{warmup}", file=sys.stderr, flush=True)
+    sync()
+    if on_timed_start:
+        on_timed_start()
+    lat, out = [], []
     t0 = time.perf_counter()
-    A = orc.assemble(ke)
-    b = orc.rhs(ke, 0)
-    t1 = time.perf_counter()
-    x, it, rr = orc.cg(ke.shape, A, b, rtol=rtol, pin=False)
-    t2 = time.perf_counter()
-    chi = np.stack([x, x, x])
-    orc.element_energy(ke.shape, chi)
-    t3 = time.perf_counter()
-    per_solve = (t1 - t0) + 3 * (t2 - t1) + (t3 - t2)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"{reso}^3 start design: FP64 assembly + one cold-start Jacobi-PCG load case to rtol "
-                      f"{rtol:g} ({it} iterations, {t2 - t1:.1f} s, counted x3) + element energies; "
-                      f"{t3 - t0:.1f} s measured"}
+    for i in range(steps):
+        ts = time.perf_counter()
+        out.append(step())
+        lat.append(time.perf_counter() - ts)
+        print(f"[bench] {label} step {i + 1}/{steps} {1e3 * lat[-1]:.2f} ms", file=sys.stderr, flush=True)
+    sync()
+    return time.perf_counter() - t0, lat, out
+
+
+def roofline_from(prof, reso):
+    """Fine brick smoother: algorithmic bytes per launch (28 B/node for the
+    zero-start pre-smoother, 40 B/node for the post-smoother, averaged over
+    the launches) over the mean launch time from HIP events on the engine's
+    stream."""
+    nl, ms, by = prof["fine_gs"]
+    if not nl or ms <= 0:
+        return None
+    avg_s = ms * 1e-3 / nl
+    achieved = (by / nl) / avg_s / 1e9
+    return {"bound": "hbm", "kernel": FINE_KERNEL, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": by / nl,
+            "avg_launch_us": round(avg_s * 1e6, 3), "launches": nl, "traffic": measured_traffic(FINE_KERNEL, reso)}
+
+
+def class_rows(prof):
+    rows = {}
+    for name, (nl, ms, by) in prof.items():
+        if nl:
+            rows[name] = {"launches": nl, "ms": round(ms, 3), "avg_us": round(1e3 * ms / nl, 3),
+                          "GB/s": round(by / (ms * 1e-3) / 1e9, 1) if ms > 0 and by > 0 else None}
+    return rows
+
+
+def precision_run(n, rho0, refine, steps, warmup, rtol, label, profile_classes=None):
+    """The precision-form MMA loop (or, refine=False, the same loop on the
+    plain FP32 path) on one cell."""
+    from opentm_amd import precision as pf
+    loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=refine)
+
+    def on_start():
+        loop.history.clear()
+        if profile_classes is not None:
+            loop.hom.profile_enable(True, profile_classes)
+
+    total, lat, hist = timed_loop(loop.step, steps, warmup, loop.hom.synchronize, label, on_start)
+    prof = loop.hom.profile_read() if profile_classes is not None else None
+    res = {"solves_per_s": steps / total, "ms_per_step": 1e3 * total / steps,
+           "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+           "pcg_iters_mean": float(np.mean([max(h["pcg_iters"]) for h in hist])),
+           "relres_max": float(max(max(h["relres"]) for h in hist)), "rtol": rtol,
+           "C_final": hist[-1]["C"].reshape(-1).tolist(), "g_final": hist[-1]["g"].tolist()}
+    loop.mma.close()
+    loop.hom.close()
+    for b in [loop.x, loop.df] + loop.dg:
+        b.free()
+    return res, prof
+
+
+def oc_loop(n, seed, steps, warmup, rtol, coarse_sweeps, profile_classes=None, verbose=False, min_coarse=None):
+    """The plain optimisation loop: OC maximising trace(kappa^H) at volume 0.3."""
+    import opentm_amd as otm
+    from opentm_amd import workloads as wl
+    cfg = otm.HomoConfig(n, (1.0, 1e-3))
+    cfg.femRelThres = rtol
+    cfg.coarse_sweeps = coarse_sweeps
+    if min_coarse is not None:
+        cfg.min_coarse = min_coarse
+    cfg.verbose = 1 if verbose else 0
+    hom = otm.Homogenization_H(cfg)
+    hom.set_design(wl.oc_start_density(n, seed))
+    oc = otm.OCOptimizer(n ** 3, 0.001, 0.2, 0.5, bisect_steps=60, bisect_tol=1e-6)
+    dC = -np.eye(3)
+    params = otm.DensityParams(0, 1.5, False)
+
+    def step():
+        C = hom.eval(params)
+        st = hom.solve_stats()
+        hom.backward(dC, want=False)
+        oc.step(hom, 2.0, 0.3)
+        return C, st
+
+    def on_start():
+        if profile_classes is not None:
+            hom.profile_enable(True, profile_classes)
+
+    total, lat, out = timed_loop(step, steps, warmup, hom.synchronize, f"oc{n}", on_start)
+    prof = hom.profile_read() if profile_classes is not None else None
+    res = {"solves_per_s": steps / total, "ms_per_step": 1e3 * total / steps,
+           "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+           "pcg_iters_mean": float(np.mean([max(st[0]) for _, st in out])),
+           "relres_max": float(max(max(st[1]) for _, st in out)),
+           "kappaH_final": [float(v) for v in out[-1][0].reshape(-1)]}
+    oc.close()
+    hom.close()
+    return res, prof
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30, help="timed design iterations")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--reso", type=int, default=128)
+    ap.add_argument("--reso", type=int, default=None, help="cell edge (default: 256 precision, 128 oc)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--rtol", type=float, default=1e-6)
+    ap.add_argument("--rtol", type=float, default=1e-6, help="relative residual of the plain FP32 path")
     ap.add_argument("--coarse-sweeps", type=int, default=1,
                     help="Gauss-Seidel sweeps per smoothing step on the coarse levels (fine level: 2)")
     ap.add_argument("--min-coarse", type=int, default=None,
@@ -95,7 +223,8 @@ def main():
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel class with HIP events (adds event overhead to the step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["oc", "precision", "batch", "serve", "copy"], default="oc",
+    ap.add_argument("--no-extras", action="store_true", help="precision workload: skip the plain-path and 128^3 runs")
+    ap.add_argument("--workload", choices=["precision", "oc", "batch", "serve", "copy"], default="precision",
                     help="copy: three 256 MiB device copies, the calibration run for rocprofv3 byte counters")
     ap.add_argument("--qps", default="100,200,300,400", help="serve workload: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=2000, help="serve workload: requests per arrival rate")
@@ -105,169 +234,115 @@ def main():
     ap.add_argument("--oracle", action="store_true", help="precision workload: FP64 oracle errors (slow)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path is built)"}))
         sys.exit(2)
     if args.workload == "precision":
         precision_main(args)
-        return
-    if args.workload == "batch":
+    elif args.workload == "oc":
+        oc_main(args)
+    elif args.workload == "batch":
         batch_main(args)
-        return
-    if args.workload == "serve":
+    elif args.workload == "serve":
         serve_main(args)
-        return
-    if args.workload == "copy":
+    else:
         import opentm_amd as otm
         from opentm_amd import _capi
         n = 64 * 1024 * 1024  # floats: 256 MiB per buffer
         a, b = otm.DeviceBuffer(n), otm.DeviceBuffer(n)
         for _ in range(3):
             _capi.check(_capi.load().otm_memcpy(b.ptr, a.ptr, 4 * n, 1, 1), "otm_memcpy")
+        a.free()
+        b.free()
         print(json.dumps({"workload": "copy", "bytes_read_per_copy": 4 * n, "bytes_written_per_copy": 4 * n,
                           "copies": 3}))
-        return
-
-    import opentm_amd as otm
-    from opentm_amd import workloads as wl
-
-    n = args.reso
-    cfg = otm.HomoConfig(n, (1.0, 1e-3))
-    cfg.femRelThres = args.rtol
-    cfg.coarse_sweeps = args.coarse_sweeps
-    if args.min_coarse is not None:
-        cfg.min_coarse = args.min_coarse
-    cfg.verbose = 1 if args.verbose else 0
-    hom = otm.Homogenization_H(cfg)
-    hom.set_design(wl.oc_start_density(n, args.seed))
-    oc = otm.OCOptimizer(n ** 3, 0.001, 0.2, 0.5, bisect_steps=60, bisect_tol=1e-6)
-    dC = -np.eye(3)
-    params = otm.DensityParams(0, 1.5, False)
-    its_log = []
-
-    def step():
-        C = hom.eval(params)
-        its_log.append(hom.solve_stats())
-        hom.backward(dC, want=False)
-        oc.step(hom, 2.0, 0.3)
-        return C
-
-    for i in range(args.warmup):
-        step()
-        print(f"[bench] warmup {i + 1}/{args.warmup}", file=sys.stderr, flush=True)
-    its_log.clear()
-    hom.synchronize()
-    hom.profile_enable(True, None if args.profile_all else ["fine_gs"])
-    lat = []
-    t0 = time.perf_counter()
-    C = None
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        C = step()
-        lat.append(time.perf_counter() - ts)
-        print(f"[bench] step {i + 1}/{args.steps} {1e3 * lat[-1]:.2f} ms its {its_log[-1][0]}",
-              file=sys.stderr, flush=True)
-    hom.synchronize()
-    total = time.perf_counter() - t0
-    prof = hom.profile_read()
-    hom.profile_enable(False)
-
-    gs = prof["fine_gs"]
-    rows = {}
-    for name, (nl, ms, by) in prof.items():
-        if nl:
-            rows[name] = {"launches": nl, "ms": round(ms, 3), "avg_us": round(1e3 * ms / nl, 3),
-                          "GB/s": round(by / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-    nl, ms, by = gs
-    avg_s = ms * 1e-3 / nl
-    achieved = (by / nl) / avg_s / 1e9
-    out = {
-        "metric": METRIC,
-        "value": args.steps / total,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": f"{n}^3 OC loop, max trace(kappaH) at volume 0.3 (plain optimisation loop)",
-                   "reso": n, "rhs_per_solve": 3, "rtol": args.rtol, "seed": args.seed,
-                   "mg": f"V({2},{2}) fine, {args.coarse_sweeps} sweep(s) coarse, 8-colour GS in LDS bricks",
-                   "parallelism": "single GPU"},
-        "p50_ms": 1e3 * float(np.percentile(lat, 50)),
-        "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-        "pcg_iters_mean": float(np.mean([max(s[0]) for s in its_log])),
-        "relres_max": float(max(max(s[1]) for s in its_log)),
-        "kappaH_final": [float(v) for v in C.reshape(-1)],
-        "roofline": {"bound": "hbm", "kernel": "k_fine_smooth_brick<32>", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "bytes_per_launch": by / nl, "avg_launch_us": round(avg_s * 1e6, 3),
-                     "traffic": measured_traffic("k_fine_smooth_brick<32>", n)},
-        "kernel_classes": rows,
-    }
-    if not args.no_cpu_baseline:
-        print("[bench] cpu baseline (FP64 oracle) ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(n, args.seed, args.rtol)
-    print(json.dumps(out))
 
 
 def precision_main(args):
-    from opentm_amd import precision as pf
     from opentm_amd import workloads as wl
-    n = args.reso if args.reso != 128 else 256
-    iters = args.steps if args.steps != 10 else 30
+    n = args.reso or 256
     rho0 = wl.plain_density(n, args.seed, mean=0.25)
-
-    def run(refine, fp16, rtol, label):
-        loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=fp16)
-        t0 = time.perf_counter()
-        for i in range(iters):
-            h = loop.step()
-            print(f"[bench] {label} itn {i} {1e3 * h['seconds']:.1f} ms C00 {h['C'][0, 0]:.6f} "
-                  f"g {np.round(h['g'], 4).tolist()} its {h['pcg_iters']} relres {max(h['relres']):.2e} "
-                  f"newton {h['newton']}", file=sys.stderr, flush=True)
-        total = time.perf_counter() - t0
-        secs = [h["seconds"] for h in loop.history]
-        timed = secs[args.warmup:] if len(secs) > args.warmup else secs
-        res = {"iters": iters, "total_s": total, "ms_per_iter": 1e3 * float(np.mean(timed)),
-               "p50_ms": 1e3 * float(np.percentile(timed, 50)), "p99_ms": 1e3 * float(np.percentile(timed, 99)),
-               "pcg_iters_mean": float(np.mean([max(h["pcg_iters"]) for h in loop.history])),
-               "relres_max": float(max(max(h["relres"]) for h in loop.history)),
-               "C_first": loop.history[0]["C"].reshape(-1).tolist(),
-               "C_final": loop.history[-1]["C"].reshape(-1).tolist(),
-               "g_final": loop.history[-1]["g"].tolist()}
-        return loop, res
-
-    loop_p, prec = run(True, True, 1e-8, "precision")
-    del loop_p
-    loop_f, plain = run(False, False, args.rtol, "plain")
-    del loop_f
-    out = {"metric": "MMA design iterations/s (one homogenization + 3 sensitivity contractions each)",
-           "value": 1e3 / prec["ms_per_iter"], "unit": "iterations/s", "n_gpus": 1, "steps": iters,
-           "warmup": args.warmup, "ms_per_step": prec["ms_per_iter"], "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp16 stencils+fp64 refinement",
-           "data": "synthetic",
-           "config": {"workload": f"{n}^3 precision form: MMA max C00 s.t. C11, C22 <= 0.05, volume <= 0.25",
-                      "reso": n, "seed": args.seed, "mma": "m=3, move 0.2, c=1e3"},
-           "precision": prec, "plain_fp32": plain,
-           "time_vs_plain": prec["ms_per_iter"] / plain["ms_per_iter"]}
+    classes = None if args.profile_all else ["fine_gs"]
+    prec, prof = precision_run(n, rho0, True, args.steps, args.warmup, 1e-8, f"precision{n}", classes)
+    out = {
+        "metric": METRIC, "value": prec["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": prec["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 smoother/transfers, fp16 coarse stencils, fp64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 precision form: MMA (m=3) max C00 s.t. C11, C22 <= 0.05, volume <= 0.25; "
+                               f"one step = one homogenization (3 load cases, FP64 refinement to 1e-8) + "
+                               f"3 sensitivities + MMA update",
+                   "reso": n, "elements": n ** 3, "seed": args.seed, "rhs_per_solve": 3,
+                   "mg": "V(2,2) fine, 1 sweep coarse, 8-colour GS in LDS bricks, Galerkin coarse levels",
+                   "parallelism": "single GPU"},
+        "p50_ms": prec["p50_ms"], "p99_ms": prec["p99_ms"], "pcg_iters_mean": prec["pcg_iters_mean"],
+        "relres_max": prec["relres_max"], "C_final": prec["C_final"], "g_final": prec["g_final"],
+        "roofline": roofline_from(prof, n), "kernel_classes": class_rows(prof),
+    }
+    if not args.no_extras:
+        plain, _ = precision_run(n, rho0, False, args.steps, args.warmup, args.rtol, f"plain{n}")
+        out["plain_fp32"] = {k: plain[k] for k in ("solves_per_s", "ms_per_step", "p50_ms", "p99_ms",
+                                                   "pcg_iters_mean", "relres_max", "rtol")}
+        out["time_vs_plain"] = prec["ms_per_step"] / plain["ms_per_step"]
+        oc, ocprof = oc_loop(128, args.seed, 20, 5, args.rtol, args.coarse_sweeps, ["fine_gs"])
+        oc["roofline"] = roofline_from(ocprof, 128)
+        out["config1_oc_loop_128"] = oc
     if args.oracle:
         import oracle.oracle as orc
-        print("[bench] FP64 oracle on the first design ...", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        rf = orc.filter_fwd(rho0.astype(np.float64), 0, 1.5, False)
-        Co, _, _, _ = orc.homogenize(1e-3 + (1 - 1e-3) * rf ** 3, rtol=1e-10)
-        dt = time.perf_counter() - t0
+        print("[bench] FP64 oracle (MG-PCG) on the first design ...", file=sys.stderr, flush=True)
+        out["oracle"] = oracle_errors(orc, n, rho0)
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (FP64 MG-PCG oracle, bounded sample) ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_mg(n, args.seed, 1e-8, 0.25)
+    print(json.dumps(out))
 
-        def relerr(c):
-            c = np.asarray(c).reshape(3, 3)
-            return float(np.abs(c - Co).max() / np.abs(Co).max())
-        out["oracle"] = {"rtol": 1e-10, "seconds": dt, "C": Co.reshape(-1).tolist(),
-                         "kappaH_rel_err_precision": relerr(prec["C_first"]),
-                         "kappaH_rel_err_plain": relerr(plain["C_first"])}
+
+def oracle_errors(orc, n, rho0):
+    """kappa^H and per-element sensitivity errors of the first precision-form
+    solve (and of the plain FP32 path) against the FP64 oracle on the same
+    design."""
+    import opentm_amd as otm
+    t0 = time.perf_counter()
+    rf = orc.filter_fwd(rho0.astype(np.float64), 0, 1.5, False)
+    kap = 1e-3 + (1 - 1e-3) * rf ** 3
+    Co, _, So, its, rr = orc.mg_homogenize(kap, rtol=1e-11, want_S=True)
+    dt = time.perf_counter() - t0
+    res = {"solver": "FP64 MG-PCG", "rtol": 1e-11, "relres": rr, "iterations": its, "seconds": dt,
+           "C": Co.reshape(-1).tolist()}
+    dCs = {"00": np.diag([1.0, 0, 0]), "11": np.diag([0, 1.0, 0]), "22": np.diag([0, 0, 1.0])}
+    for label, refine, rtol in (("precision", 1, 1e-8), ("plain", 0, 1e-6)):
+        cfg = otm.HomoConfig(n, (1.0, 1e-3))
+        cfg.femRelThres = rtol
+        cfg.fp64_refine = refine
+        cfg.fp16_stencils = refine
+        hom = otm.Homogenization_H(cfg)
+        hom.update(kap.astype(np.float32))
+        C = hom.heatMatrix()
+        e = {"kappaH_rel_err": float(np.abs(C - Co).max() / np.abs(Co).max())}
+        for name, dC in dCs.items():
+            s = hom.Sensitivity(dC).astype(np.float64)
+            so = orc.sensitivity(So, dC)
+            floor = 1e-3 * np.abs(so).max()
+            e[f"sens{name}_max_elementwise_rel_err"] = float((np.abs(s - so) / np.maximum(np.abs(so), floor)).max())
+            e[f"sens{name}_normwise_rel_err"] = float(np.abs(s - so).max() / np.abs(so).max())
+        e["relres"] = hom.solve_stats()[1]
+        res[label] = e
+        hom.close()
+    return res
+
+
+def oc_main(args):
+    n = args.reso or 128
+    classes = None if args.profile_all else ["fine_gs"]
+    res, prof = oc_loop(n, args.seed, args.steps, args.warmup, args.rtol, args.coarse_sweeps, classes,
+                        args.verbose, args.min_coarse)
+    out = {"metric": METRIC, "value": res["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"{n}^3 OC loop, max trace(kappaH) at volume 0.3 (plain optimisation loop)",
+                      "reso": n, "rhs_per_solve": 3, "rtol": args.rtol, "seed": args.seed,
+                      "parallelism": "single GPU"},
+           "roofline": roofline_from(prof, n), "kernel_classes": class_rows(prof)}
+    out.update({k: res[k] for k in ("p50_ms", "p99_ms", "pcg_iters_mean", "relres_max", "kappaH_final")})
     print(json.dumps(out))
 
 
@@ -354,6 +429,11 @@ def batch_main(args):
            "pcg_iters_mean": {"batched": float(np.mean(its_bat)), "looped": float(np.mean(its_loop))},
            "max_rel_diff_batched_vs_looped": float(np.max(np.abs(C_b - C_l)) / np.max(np.abs(C_l))),
            "batch_launches_total": launches, "batch_steps_total": steps}
+    batch.close()
+    for h in homs:
+        h.close()
+    for b in kbufs + sbufs:
+        b.free()
     print(json.dumps(out))
 
 
