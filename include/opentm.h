@@ -67,6 +67,8 @@ typedef struct otm_config {
   int exact_smoother;       /* 1: exact multicolour Gauss-Seidel on every level instead of the
                                LDS brick smoother (the fallback a solve takes by itself when
                                the brick preconditioner loses positivity) */
+  int graph_vcycle;         /* 1 (default): each V-cycle's fixed launch sequence is captured
+                               once as a HIP graph and replayed (0: launched one by one) */
 } otm_config;
 
 /* cfg::HomoConfig::init (cmdline.cpp:4-35) defaults where they apply;

@@ -122,6 +122,7 @@ class HomoConfig:
         self.fp64_refine = 0
         self.fp16_stencils = 0
         self.exact_smoother = 0
+        self.graph_vcycle = 1
         self.device = 0
         self.verbose = 0
 
@@ -140,6 +141,7 @@ class HomoConfig:
         c.fp64_refine = self.fp64_refine
         c.fp16_stencils = self.fp16_stencils
         c.exact_smoother = self.exact_smoother
+        c.graph_vcycle = self.graph_vcycle
         c.device = self.device
         c.verbose = self.verbose
         return c

@@ -48,6 +48,7 @@ class OtmConfig(ctypes.Structure):
         ("fp64_refine", _c.c_int),
         ("fp16_stencils", _c.c_int),
         ("exact_smoother", _c.c_int),
+        ("graph_vcycle", _c.c_int),
     ]
 
 

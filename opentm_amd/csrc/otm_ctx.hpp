@@ -174,6 +174,14 @@ struct otm_ctx {
   hipStream_t s_side = nullptr;
   hipEvent_t ev_main = nullptr, ev_side = nullptr;
   bool start_res_ready = false;
+  // V-cycles captured as HIP graphs, keyed by the launch-relevant state (the
+  // brick smoothers' buffer parities, the fused r.z mode, bricks on/off)
+  struct VGraph {
+    unsigned long long key;
+    hipGraphExec_t exec;
+  };
+  std::vector<VGraph> vgraphs;
+  bool graphs_failed = false;  // capture unavailable: run eagerly
   int iters[3] = {0, 0, 0};
   double relres[3] = {0, 0, 0};
   double C[9];

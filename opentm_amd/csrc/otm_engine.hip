@@ -163,6 +163,7 @@ static void ctx_free(otm_ctx* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  for (auto& v : c->vgraphs) (void)hipGraphExecDestroy(v.exec);
   if (c->s_side) (void)hipStreamSynchronize(c->s_side);
   if (c->ev_main) (void)hipEventDestroy(c->ev_main);
   if (c->ev_side) (void)hipEventDestroy(c->ev_side);
@@ -433,6 +434,90 @@ static void vcycle(otm_ctx* c, size_t l) {
   }
 }
 
+// ---- the V-cycle as a replayed HIP graph -------------------------------------
+// A V-cycle's launches are fixed by a few pieces of host state: which buffer
+// of each brick level holds the current correction (the brick smoothers
+// write a second buffer and the host swaps the pointers), whether the fine
+// post-smoother also forms the PCG's r.z (and for which finaliser), and
+// whether the bricks are on. Each state's launch sequence is captured once
+// into a graph and replayed; replaying applies the same pointer swaps on
+// the host as the eager V-cycle. Timing with events (otm_profile_enable)
+// runs the eager path.
+static unsigned long long vcycle_key(const otm_ctx* c) {
+  unsigned long long key = 0;
+  for (size_t l = 0; l < c->lv.size() && l < 48; l++) {
+    const Level& L = c->lv[l];
+    if (L.u2 && (uintptr_t)L.u > (uintptr_t)L.u2) key |= 1ull << l;
+  }
+  key |= (unsigned long long)(c->post_dot_mode + 1) << 56;  // -1, FIN_RZ_FIRST, FIN_RZ
+  key |= (unsigned long long)(c->bricks_on ? 1 : 0) << 62;
+  return key;
+}
+
+// The host-side effects of vcycle(c, 0) without its launches: the pointer
+// swaps of the brick levels it smooths and the fused r.z flag.
+static void vcycle_effects(otm_ctx* c) {
+  const size_t stop = std::min<size_t>((size_t)c->tail, c->lv.size() - 1);
+  c->post_dot_done = false;
+  for (size_t l = 0; l < stop; l++) {
+    Level& L = c->lv[l];
+    if (!L.brick) continue;
+    if (l == 0) {
+      c->post_dot_done = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g);
+      std::swap(c->z, c->z2);
+    }
+    std::swap(L.u, L.u2);
+  }
+}
+
+static void run_vcycle(otm_ctx* c) {
+  if (!c->cfg.graph_vcycle || c->graphs_failed || c->prof_on) {
+    vcycle(c, 0);
+    return;
+  }
+  const unsigned long long key = vcycle_key(c);
+  for (const auto& v : c->vgraphs)
+    if (v.key == key) {
+      hip_check(hipGraphLaunch(v.exec, c->s), "hipGraphLaunch");
+      vcycle_effects(c);
+      return;
+    }
+  // capture this state's launches; vcycle() applies the host effects itself
+  std::vector<std::pair<float*, float*>> saved;
+  for (auto& L : c->lv) saved.push_back({L.u, L.u2});
+  float *z = c->z, *z2 = c->z2;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed) == hipSuccess;
+  if (ok) {
+    try {
+      vcycle(c, 0);
+    } catch (...) {
+      ok = false;
+    }
+    const hipError_t e = hipStreamEndCapture(c->s, &graph);
+    ok = ok && e == hipSuccess && graph != nullptr;
+  }
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    // no graph: restore the host state and run eagerly from now on
+    (void)hipGetLastError();
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (size_t l = 0; l < c->lv.size(); l++) {
+      c->lv[l].u = saved[l].first;
+      c->lv[l].u2 = saved[l].second;
+    }
+    c->z = z;
+    c->z2 = z2;
+    c->graphs_failed = true;
+    vcycle(c, 0);
+    return;
+  }
+  c->vgraphs.push_back({key, exec});
+  hip_check(hipGraphLaunch(exec, c->s), "hipGraphLaunch");
+}
+
 // ---- PCG over the three load cases -------------------------------------------
 // Three preconditioned CG recurrences in lock step, one per load case, with
 // the V-cycle as preconditioner (the reference iterates the V-cycle as a
@@ -502,7 +587,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
     }
     c->post_dot_mode = FIN_RZ_FIRST;
     c->post_dot_done = false;
-    vcycle(c, 0);
+    run_vcycle(c);
     c->post_dot_mode = -1;
     {
       ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 24.0 * (double)n);
@@ -532,7 +617,7 @@ static bool pcg_iterate(otm_ctx* c, int* its, int* total, bool* fresh) {
       if (!c->pcg_host->active[0] && !c->pcg_host->active[1] && !c->pcg_host->active[2]) break;
       c->post_dot_mode = FIN_RZ;
       c->post_dot_done = false;
-      vcycle(c, 0);
+      run_vcycle(c);
       c->post_dot_mode = -1;
       {
         ProfScope ps(c, OTM_PROF_VECTOR, 24.0 * (double)n + 36.0 * (double)n);
@@ -680,6 +765,7 @@ OTM_API void otm_config_init(otm_config* cfg) {
   cfg->fp64_refine = 0;
   cfg->fp16_stencils = 0;
   cfg->exact_smoother = 0;
+  cfg->graph_vcycle = 1;
 }
 
 OTM_API const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -948,7 +1034,8 @@ OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where) {
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     const long long n = ctx->g0.nn;
     hip_check(hipMemcpyAsync(ctx->r, r, sizeof(float) * 3 * n, kind_of(OTM_DEVICE, where), ctx->s), "copy r");
-    vcycle(ctx, 0);
+    ctx->post_dot_mode = -1;
+    run_vcycle(ctx);
     hip_check(hipMemcpyAsync(z, ctx->z, sizeof(float) * 3 * n, kind_of(where, OTM_DEVICE), ctx->s), "copy z");
     ctx->sync();
   });
@@ -968,7 +1055,7 @@ static void stationary_cycle(otm_ctx* c, double relres[3]) {
   for (int k = 0; k < 3; k++) relres[k] = sqrt(c->pcg_host->rr[k] / c->pcg_host->bb[k]);
   launch_sub_mean3(c->s, n, c->r, c->pcg);
   c->post_dot_mode = -1;
-  vcycle(c, 0);
+  run_vcycle(c);
   if (c->x64)
     launch_refine_add(c->s, 3 * n, c->x64, c->z);
   else

@@ -225,3 +225,29 @@ def test_cxx_dropin_on_gpu(orc, tmp_path):
                        dgdx.astype(np.float64)[None])
     say("  mma max diff", float(np.abs(xg - xo).max()))
     assert np.abs(xg - xo).max() < 2e-6
+
+
+# ------------------------------------------------------------ graph replay
+@pytest.mark.parametrize("kw", [{}, {"fp64_refine": 1, "fp16_stencils": 1}, {"exact_smoother": 1}])
+def test_graph_vcycle_bitwise_equal_to_eager(kw):
+    """V-cycles replayed from captured HIP graphs give bitwise the results of
+    the eagerly launched V-cycles (kappa^H, iteration counts, fluctuation
+    fields), over warm-started re-solves that cycle through every buffer
+    parity."""
+    n = 64
+    say("graph vs eager", kw)
+    k = wl.simp(wl.plain_density(n, 12))
+    out = []
+    for graph in (1, 0):
+        hom = make(n, rtol=1e-8 if kw.get("fp64_refine") else 1e-6, graph_vcycle=graph, **kw)
+        res = []
+        for s in (1.0, 1.03, 0.97):
+            hom.update((k * s).astype(np.float32))
+            res.append((hom.heatMatrix(), hom.solve_stats()[0]))
+        out.append((res, hom.get_chi()))
+        hom.close()
+    (rg, chig), (re_, chie) = out
+    for (Cg, ig), (Ce, ie) in zip(rg, re_):
+        np.testing.assert_array_equal(Cg, Ce)
+        assert ig == ie
+    np.testing.assert_array_equal(chig, chie)
