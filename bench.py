@@ -123,7 +123,7 @@ def roofline_from(prof, reso):
     """Fine brick smoother: algorithmic bytes per launch (28 B/node for the
     zero-start pre-smoother, 40 B/node for the post-smoother, averaged over
     the launches) over the mean launch time from HIP events on the engine's
-    stream."""
+    stream (profiled_steps)."""
     nl, ms, by = prof["fine_gs"]
     if not nl or ms <= 0:
         return None
@@ -131,7 +131,8 @@ def roofline_from(prof, reso):
     achieved = (by / nl) / avg_s / 1e9
     return {"bound": "hbm", "kernel": FINE_KERNEL, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": by / nl,
-            "avg_launch_us": round(avg_s * 1e6, 3), "launches": nl, "traffic": measured_traffic(FINE_KERNEL, reso)}
+            "avg_launch_us": round(avg_s * 1e6, 3), "launches": nl, "timed_over": f"{PROFILE_STEPS} steps after the timed region",
+            "traffic": measured_traffic(FINE_KERNEL, reso)}
 
 
 def class_rows(prof):
@@ -143,19 +144,32 @@ def class_rows(prof):
     return rows
 
 
+PROFILE_STEPS = 3
+
+
+def profiled_steps(hom, step, classes, label):
+    """Kernel-class timing with HIP events on the engine's stream over a few
+    steps run right after the timed region (events are recorded around
+    eager launches; the timed region replays captured V-cycle graphs)."""
+    hom.synchronize()
+    hom.profile_enable(True, None if classes == "all" else classes)
+    for i in range(PROFILE_STEPS):
+        step()
+        print(f"[bench] {label} profiled step {i + 1}/{PROFILE_STEPS}", file=sys.stderr, flush=True)
+    prof = hom.profile_read()
+    hom.profile_enable(False)
+    return prof
+
+
 def precision_run(n, rho0, refine, steps, warmup, rtol, label, profile_classes=None):
     """The precision-form MMA loop (or, refine=False, the same loop on the
     plain FP32 path) on one cell."""
     from opentm_amd import precision as pf
     loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=refine)
 
-    def on_start():
-        loop.history.clear()
-        if profile_classes is not None:
-            loop.hom.profile_enable(True, profile_classes)
-
-    total, lat, hist = timed_loop(loop.step, steps, warmup, loop.hom.synchronize, label, on_start)
-    prof = loop.hom.profile_read() if profile_classes is not None else None
+    total, lat, hist = timed_loop(loop.step, steps, warmup, loop.hom.synchronize, label, loop.history.clear)
+    hist = list(hist)
+    prof = profiled_steps(loop.hom, loop.step, profile_classes, label) if profile_classes is not None else None
     res = {"solves_per_s": steps / total, "ms_per_step": 1e3 * total / steps,
            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
            "pcg_iters_mean": float(np.mean([max(h["pcg_iters"]) for h in hist])),
@@ -191,12 +205,8 @@ def oc_loop(n, seed, steps, warmup, rtol, coarse_sweeps, profile_classes=None, v
         oc.step(hom, 2.0, 0.3)
         return C, st
 
-    def on_start():
-        if profile_classes is not None:
-            hom.profile_enable(True, profile_classes)
-
-    total, lat, out = timed_loop(step, steps, warmup, hom.synchronize, f"oc{n}", on_start)
-    prof = hom.profile_read() if profile_classes is not None else None
+    total, lat, out = timed_loop(step, steps, warmup, hom.synchronize, f"oc{n}")
+    prof = profiled_steps(hom, step, profile_classes, f"oc{n}") if profile_classes is not None else None
     res = {"solves_per_s": steps / total, "ms_per_step": 1e3 * total / steps,
            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
            "pcg_iters_mean": float(np.mean([max(st[0]) for _, st in out])),
@@ -261,7 +271,7 @@ def precision_main(args):
     from opentm_amd import workloads as wl
     n = args.reso or 256
     rho0 = wl.plain_density(n, args.seed, mean=0.25)
-    classes = None if args.profile_all else ["fine_gs"]
+    classes = "all" if args.profile_all else ["fine_gs"]
     prec, prof = precision_run(n, rho0, True, args.steps, args.warmup, 1e-8, f"precision{n}", classes)
     out = {
         "metric": METRIC, "value": prec["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -332,7 +342,7 @@ def oracle_errors(orc, n, rho0):
 
 def oc_main(args):
     n = args.reso or 128
-    classes = None if args.profile_all else ["fine_gs"]
+    classes = "all" if args.profile_all else ["fine_gs"]
     res, prof = oc_loop(n, args.seed, args.steps, args.warmup, args.rtol, args.coarse_sweeps, classes,
                         args.verbose, args.min_coarse)
     out = {"metric": METRIC, "value": res["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,

@@ -251,13 +251,36 @@ OTM_API int otm_mma_set_move(otm_mma* mma, double move);
 OTM_API int otm_mma_update(otm_mma* mma, int itn, float* x, const float* dfdx, const double* gval,
                            const float* const* dgdx, int where, int* newton);
 
-/* Density initialisers of initDensity (Framework.cu:61-145), indexed like
+/* initDensity (Framework.cu:61-145) on the context's design, indexed like
  * cfg::InitWay: 0 random, 1 randcenter, 2 noise, 3 manual, 4 interp,
- * 5 rep_randcenter, 6 P, 7 G, 8 D, 9 IWP, 10 example. The TPMS fields P, G,
- * D, IWP and `example` follow the reference's closed forms; the random ones
- * are seeded here (the reference seeds cuRAND from the clock); manual/interp
- * read OpenVDB files and are not supported. The result is clamped to
- * [0.1, 1] as in Framework.cu:144 and becomes the context's design. */
+ * 5 rep_randcenter, 6 P, 7 G, 8 D, 9 IWP, 10 example.
+ *  - random / randcenter / rep_randcenter: randTri (Framework.cuh:238-310),
+ *    a random trigonometric series of 6P + 36P^2 terms (P = n_period; plus a
+ *    random rotation of the centred coordinates for randcenter) mapped
+ *    through sigmoid(., 15, t) with t bisected to the volume ratio;
+ *    coefficients uniform in [-1, 1] from `seed` (the reference draws them
+ *    from a clock-seeded cuRAND) or, for rep_randcenter, given in coeffs;
+ *  - noise (Framework.cu:75-88): uniform values from `seed` (a counter-based
+ *    generator), symmetrised, projected to {1e-9, 1}, scaled to the volume
+ *    ratio, clamped to [0.001, 1];
+ *  - P, G, D, IWP, example: the reference's closed forms;
+ *  - manual / interp read OpenVDB files: OTM_E_UNSUPPORTED.
+ * Then the field is symmetrised and clamped to [0.1, 1] (Framework.cu:
+ * 140-144) and becomes the context's design. coeffs_out (host, may be
+ * NULL) receives the randTri coefficients used (the reference writes them
+ * to a file that rep_randcenter reads back). */
+typedef struct otm_init_params {
+  int initway;          /* cfg::InitWay */
+  int seed;             /* random starts */
+  double volratio;      /* HomoConfig.volRatio */
+  int n_period;         /* HomoConfig.initperiod, 1..10 (default 10) */
+  int sym;              /* OTM_SYM_* (HomoConfig.sym) */
+  const float* coeffs;  /* randTri coefficients (host), NULL: drawn from seed */
+  int ncoeffs;
+} otm_init_params;
+OTM_API void otm_init_params_init(otm_init_params* p);
+OTM_API int otm_init_design(otm_ctx* ctx, const otm_init_params* p, float* coeffs_out);
+/* otm_init_design with n_period 10, no symmetry and seeded coefficients. */
 OTM_API int otm_init_density(otm_ctx* ctx, int initway, int seed, double volratio);
 
 /* cfg::Symmetry (cmdline.h:9), same order. */

@@ -343,6 +343,24 @@ class Homogenization_H:
         p, where, keep = _arg(rho, self.ne)
         _capi.check(_capi.load().otm_set_design(self._ctx, p, where), "otm_set_design")
 
+    def init_design(self, initway="IWP", seed=0, volratio=0.3, n_period=10, sym="NONE", coeffs=None):
+        """initDensity (Framework.cu:61-145) on the design; returns the randTri
+        coefficients used (random, randcenter, rep_randcenter) or None."""
+        lib = _capi.load()
+        p = _capi.OtmInitParams()
+        lib.otm_init_params_init(ctypes.byref(p))
+        iw = _INITWAYS.index(initway) if isinstance(initway, str) else int(initway)
+        p.initway, p.seed, p.volratio, p.n_period = iw, int(seed), float(volratio), int(n_period)
+        p.sym = _SYMMETRIES.index(sym) if isinstance(sym, str) else int(sym)
+        keep = None
+        if coeffs is not None:
+            keep = _f32(coeffs)
+            p.coeffs, p.ncoeffs = keep.ctypes.data, keep.size
+        nb = 6 * n_period + 36 * n_period ** 2 + (4 if iw in (1, 5) else 0)
+        out = np.empty(nb, np.float32)
+        _capi.check(lib.otm_init_design(self._ctx, ctypes.byref(p), _ptr(out)), "otm_init_design")
+        return out if iw in (0, 1, 5) else None
+
     def get_design(self, out=None):
         """Design density (host array, or written into a DeviceBuffer)."""
         if isinstance(out, DeviceBuffer):

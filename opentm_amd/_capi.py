@@ -52,6 +52,18 @@ class OtmConfig(ctypes.Structure):
     ]
 
 
+class OtmInitParams(ctypes.Structure):
+    _fields_ = [
+        ("initway", _c.c_int),
+        ("seed", _c.c_int),
+        ("volratio", _c.c_double),
+        ("n_period", _c.c_int),
+        ("sym", _c.c_int),
+        ("coeffs", _c.c_void_p),
+        ("ncoeffs", _c.c_int),
+    ]
+
+
 class OtmDensityParams(ctypes.Structure):
     _fields_ = [
         ("filter_kind", _c.c_int),
@@ -163,6 +175,8 @@ SIGNATURES = {
     "otm_mma_set_move": (_i, [_p, _d]),
     "otm_mma_update": (_i, [_p, _i, _p, _p, _dp, _c.POINTER(_p), _i, _ip]),
     "otm_init_density": (_i, [_p, _i, _i, _d]),
+    "otm_init_params_init": (None, [_c.POINTER(OtmInitParams)]),
+    "otm_init_design": (_i, [_p, _c.POINTER(OtmInitParams), _p]),
     "otm_symmetrize": (_i, [_c.POINTER(_i), _i, _p, _i]),
     "otm_write_density": (_i, [_c.c_char_p, _c.POINTER(_i), _p, _i]),
     "otm_read_density": (_i, [_c.c_char_p, _c.POINTER(_i), _p, _i]),

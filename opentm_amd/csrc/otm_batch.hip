@@ -355,14 +355,14 @@ struct OpSubMeanCast {
 struct OpKappaH {
   static constexpr int kThreads = kBlock;
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int nb) const {
-    fine_kappa_h_body(C.g[0], C.fc, C.kappa, C.x, cell_reduce(C, FIN_STORE), lb, nb);
+    fine_kappa_h_body(C.g[0], C.fc, C.kappa, ChiRef{C.x, C.xlo, C.x64}, cell_reduce(C, FIN_STORE), lb, nb);
   }
 };
 
 struct OpSensitivity {
   static constexpr int kThreads = kBlock;
   __device__ void operator()(const CellDev& C, const Ent&, int lb, int) const {
-    fine_sensitivity_body(C.g[0], C.fc, C.x, C.sw, C.sens, C.sens_pitch, (long long)lb * kBlock + threadIdx.x);
+    fine_sensitivity_body(C.g[0], C.fc, ChiRef{C.x, C.xlo, C.x64}, C.sw, C.sens, C.sens_pitch, (long long)lb * kBlock + threadIdx.x);
   }
 };
 

@@ -9,6 +9,7 @@
 
 #include "../../include/opentm.h"
 #include "otm_coarse_dev.hpp"
+#include "otm_fine_dev.hpp"
 #include "otm_kernels.hpp"
 
 namespace otm {
@@ -186,6 +187,9 @@ struct otm_ctx {
   double relres[3] = {0, 0, 0};
   double C[9];
 
+  // the fluctuation fields as the energy kernels read them: the FP64
+  // iterate when refining, else the FP32 pair
+  ChiRef chi_ref() const { return ChiRef{x, xlo, x64}; }
   Reduce reducer(int mode, double rtol2 = 0.0, double inv_nn = 0.0) const {
     Reduce R;
     R.partials = partials;

@@ -713,7 +713,7 @@ static void kappa_h(otm_ctx* c, double C[9]) {
   const unsigned nb = red_blocks_for(c->g0.nn);
   {
     ProfScope ps(c, OTM_PROF_ENERGY, 16.0 * (double)c->g0.nn);
-    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->x, c->reducer_host(FIN_STORE), nb);
+    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->chi_ref(), c->reducer_host(FIN_STORE), nb);
   }
   c->wait_host();
   const double* s = c->hm->out;
@@ -792,7 +792,10 @@ OTM_API int otm_create(const otm_config* cfg, otm_ctx** out) {
     c->device = cfg->device;
     try {
       hip_check(hipSetDevice(c->device), "hipSetDevice");
-      hip_check(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking), "hipStreamCreate");
+      // a blocking stream: work a caller queues on the default stream (a
+      // hipMemset of an output, say) is ordered with the context's work, as
+      // in the reference, which runs everything on the default stream
+      hip_check(hipStreamCreateWithFlags(&c->s, hipStreamDefault), "hipStreamCreate");
       build_hierarchy(c);
       if (cfg->exact_smoother) set_bricks(c, false);
       c->sync();
@@ -920,10 +923,10 @@ OTM_API int otm_sensitivity(otm_ctx* ctx, const double dC[9], float* sens, int p
       // the reference zero-fills the pitched destination first (homogenization.cu:427)
       if (pitch != ctx->g0.nx)
         hip_check(hipMemsetAsync(sens, 0, sizeof(float) * pitch * rows, ctx->s), "zero sens");
-      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->x, w, sens, pitch);
+      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), w, sens, pitch);
       ctx->sync();
     } else {
-      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->x, w, ctx->q, ctx->g0.nx);
+      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), w, ctx->q, ctx->g0.nx);
       std::vector<float> h((size_t)ctx->g0.nn);
       hip_check(hipMemcpyAsync(h.data(), ctx->q, sizeof(float) * ctx->g0.nn, hipMemcpyDeviceToHost, ctx->s),
                 "download sens");
@@ -944,11 +947,11 @@ OTM_API int otm_element_energy(otm_ctx* ctx, float* S6, int where) {
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     const long long n = ctx->g0.nn;
     if (where == OTM_DEVICE) {
-      launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->x, S6);
+      launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), S6);
       ctx->sync();
     } else {
       float* d = dalloc<float>(6 * n);
-      launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->x, d);
+      launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), d);
       hipError_t e = hipMemcpyAsync(S6, d, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, ctx->s);
       hipError_t e2 = hipStreamSynchronize(ctx->s);
       dfree(d);
@@ -1374,42 +1377,149 @@ OTM_API int otm_get_design(otm_ctx* ctx, float* rho, int where) {
   });
 }
 
-OTM_API int otm_init_density(otm_ctx* ctx, int initway, int seed, double volratio) {
+// symmetrizeField (Framework.cuh:221-235) of the context's design in place.
+static void symmetrize_rho(otm_ctx* c, int sym) {
+  if (sym == OTM_SYM_NONE) return;
+  const SymGroup G = make_sym_group(sym, c->g0);
+  launch_symmetrize(c->s, c->g0, G, c->rho, c->tmp);
+  hip_check(hipMemcpyAsync(c->rho, c->tmp, sizeof(float) * c->g0.nn, hipMemcpyDeviceToDevice, c->s), "copy");
+}
+
+static double mean_on_device(otm_ctx* c, const float* v) {
+  const long long n = c->g0.nn;
+  launch_sum2(c->s, n, v, v, c->reducer(FIN_STORE), red_blocks_for(n));
+  hip_check(hipMemcpyAsync(c->out_host, c->red_out, sizeof(double), hipMemcpyDeviceToHost, c->s), "read sum");
+  c->sync();
+  return c->out_host[0] / (double)n;
+}
+
+// randTri (Framework.cuh:238-310): coefficients uniform in [-1, 1] (the
+// reference draws them with cuRAND), the basis sum per node, symmetrisation,
+// then the level-set value t of sigmoid(., 15, t) bisected (at most 60
+// steps, relative volume error 1e-3) so that the mapped field has mean
+// volRatio, and the map itself (values in [0.001, min(1.5 volRatio, 1)]).
+static void rand_tri(otm_ctx* c, const otm_init_params& p, std::vector<float>& coeffs) {
+  const int P = p.n_period;
+  if (P < 1 || P > rand_tri_max_period()) fail(OTM_E_UNSUPPORTED, "randTri period must be 1..10");
+  const bool centered = p.initway == 1 || p.initway == 5;
+  const int nb = 6 * P + 36 * P * P + (centered ? 4 : 0);
+  if (p.coeffs) {
+    if (p.ncoeffs != nb) fail(OTM_E_ARG, "randTri needs " + std::to_string(nb) + " coefficients");
+    coeffs.assign(p.coeffs, p.coeffs + nb);
+  } else {
+    if (p.initway == 5) fail(OTM_E_ARG, "rep_randcenter needs the coefficients of an earlier randcenter start");
+    std::mt19937_64 gen((uint64_t)p.seed * 0x9E3779B97F4A7C15ull + 0x5851F42D4C957F2Dull);
+    std::uniform_real_distribution<float> ud(-1.f, 1.f);
+    coeffs.resize(nb);
+    for (auto& v : coeffs) v = ud(gen);
+  }
+  RandTriRot R;
+  memset(&R, 0, sizeof(R));
+  R.m[0] = R.m[4] = R.m[8] = 1.f;
+  if (centered) {
+    // glm::toMat3(normalize(quat(w, x, y, z))) from the last four coefficients
+    const int q0 = 6 * P + 36 * P * P;
+    double w = coeffs[q0], x = coeffs[q0 + 1], y = coeffs[q0 + 2], z = coeffs[q0 + 3];
+    const double nrm = sqrt(w * w + x * x + y * y + z * z);
+    if (!(nrm > 0)) fail(OTM_E_ARG, "randcenter rotation quaternion is zero");
+    w /= nrm; x /= nrm; y /= nrm; z /= nrm;
+    const double m[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                         2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                         2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)};
+    for (int i = 0; i < 9; i++) R.m[i] = (float)m[i];
+  }
+  const long long n = c->g0.nn;
+  float* dco = dalloc<float>(nb);
+  double* part = nullptr;
+  try {
+    hip_check(hipMemcpyAsync(dco, coeffs.data(), sizeof(float) * nb, hipMemcpyHostToDevice, c->s), "upload coeffs");
+    launch_rand_tri(c->s, c->g0, P, centered ? 1 : 0, R, dco, c->rho);
+    hip_check(hipGetLastError(), "randTri");
+    symmetrize_rho(c, p.sym);
+    const float V = (float)p.volratio;
+    const float hi = fminf(V * 1.5f, 1.f);
+    const unsigned nbk = level_sum_blocks(n);
+    part = dalloc<double>(nbk);
+    std::vector<double> hpart(nbk);
+    float tu = (float)P * (float)P * 37.f, tl = -tu, t = 0.5f * (tu + tl);
+    for (int it = 0; it < 60; it++) {
+      t = (tu + tl) / 2;
+      launch_level_sum(c->s, n, c->rho, t, hi, part);
+      hip_check(hipMemcpyAsync(hpart.data(), part, sizeof(double) * nbk, hipMemcpyDeviceToHost, c->s), "sums");
+      c->sync();
+      double sum = 0.0;
+      for (double v : hpart) sum += v;
+      const float sm = (float)(sum / (double)n);
+      const float rel = fabsf(sm - V) / V;
+      if (rel < 1e-3f) break;
+      if (sm > V) tl = t;
+      if (sm < V) tu = t;
+    }
+    launch_level_apply(c->s, n, c->rho, t, hi);
+    hip_check(hipGetLastError(), "randTri map");
+    c->sync();
+  } catch (...) {
+    dfree(dco);
+    dfree(part);
+    throw;
+  }
+  dfree(dco);
+  dfree(part);
+}
+
+OTM_API void otm_init_params_init(otm_init_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->initway = 9;      // IWP (cmdline.cpp:10)
+  p->volratio = 0.3;   // cmdline.cpp:11
+  p->n_period = 10;    // HomoConfig::initperiod (cmdline.cpp)
+  p->sym = OTM_SYM_NONE;
+}
+
+OTM_API int otm_init_design(otm_ctx* ctx, const otm_init_params* p, float* coeffs_out) {
   return guarded([&] {
-    if (!ctx) fail(OTM_E_ARG, "null context");
-    if (initway == 3 || initway == 4) fail(OTM_E_UNSUPPORTED, "OpenVDB initial densities are not supported");
-    if (initway < 0 || initway > 10) fail(OTM_E_ARG, "unknown initialisation");
+    if (!ctx || !p) fail(OTM_E_ARG, "null argument");
+    const int iw = p->initway;
+    if (iw == 3 || iw == 4) fail(OTM_E_UNSUPPORTED, "OpenVDB initial densities are not supported");
+    if (iw < 0 || iw > 10) fail(OTM_E_ARG, "unknown initialisation");
+    if (!(p->volratio > 0 && p->volratio <= 1)) fail(OTM_E_ARG, "volratio must be in (0, 1]");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     ensure_design_buffers(ctx);
-    InitModes M;
-    memset(&M, 0, sizeof(M));
-    M.mean = (float)volratio;
-    if (initway <= 5) {
-      // 8 seeded cosine modes, |k| <= 3, amplitude N(0,1)/8 scaled to 0.25
-      std::mt19937_64 gen((uint64_t)seed * 0x9E3779B97F4A7C15ull + 12345u);
-      std::normal_distribution<double> nd(0.0, 1.0);
-      std::uniform_real_distribution<double> ud(0.0, 6.283185307179586);
-      std::uniform_int_distribution<int> kd(-3, 3);
-      M.count = 8;
-      for (int m = 0; m < M.count; m++) {
-        int k0, k1, k2;
-        do {
-          k0 = kd(gen);
-          k1 = kd(gen);
-          k2 = kd(gen);
-        } while (k0 * k0 + k1 * k1 + k2 * k2 == 0 || k0 * k0 + k1 * k1 + k2 * k2 > 9);
-        M.k[m][0] = (float)k0;
-        M.k[m][1] = (float)k1;
-        M.k[m][2] = (float)k2;
-        M.amp[m] = (float)(0.25 * nd(gen) / 8.0);
-        M.phase[m] = (float)ud(gen);
-      }
+    const long long n = ctx->g0.nn;
+    if (iw == 0 || iw == 1 || iw == 5) {
+      std::vector<float> coeffs;
+      rand_tri(ctx, *p, coeffs);
+      if (coeffs_out) memcpy(coeffs_out, coeffs.data(), sizeof(float) * coeffs.size());
+    } else if (iw == 2) {
+      // noise (Framework.cu:75-88): uniform values, symmetrised, projected to
+      // {1e-9, 1} (proj(20, 0.5)), scaled to the volume ratio, clamped
+      launch_noise(ctx->s, n, (unsigned long long)(unsigned)p->seed, ctx->rho);
+      symmetrize_rho(ctx, p->sym);
+      launch_project_binary(ctx->s, n, ctx->rho);
+      const double m = mean_on_device(ctx, ctx->rho);
+      const float sc = (float)p->volratio / (float)m;
+      launch_scale_clamp(ctx->s, n, ctx->rho, sc, 0.001f, 1.f);
+    } else {
+      InitModes M;
+      memset(&M, 0, sizeof(M));
+      launch_init_density(ctx->s, ctx->g0, iw, M, ctx->rho);
     }
-    launch_init_density(ctx->s, ctx->g0, initway, M, ctx->rho);
     hip_check(hipGetLastError(), "init density");
+    // initDensity's last steps (Framework.cu:140-144): symmetrise, clamp to [0.1, 1]
+    symmetrize_rho(ctx, p->sym);
+    launch_scale_clamp(ctx->s, n, ctx->rho, 1.f, 0.1f, 1.f);
     ctx->sync();
     ctx->has_drho = false;
   });
+}
+
+OTM_API int otm_init_density(otm_ctx* ctx, int initway, int seed, double volratio) {
+  otm_init_params p;
+  otm_init_params_init(&p);
+  p.initway = initway;
+  p.seed = seed;
+  p.volratio = volratio;
+  return otm_init_design(ctx, &p, nullptr);
 }
 
 OTM_API int otm_design_stats(otm_ctx* ctx, double* mean_rho, double* mean_kappa) {
@@ -1466,7 +1576,7 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
     const long long n = ctx->g0.nn;
     float w[6];
     sens_weights(dC, w);
-    launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->x, w, ctx->sens, ctx->g0.nx);
+    launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), w, ctx->sens, ctx->g0.nx);
     launch_mul(ctx->s, n, ctx->sens, ctx->dkappa, ctx->tmp);
     if (ctx->taps_kind >= 0)
       launch_filter_adj(ctx->s, ctx->g0, ctx->taps, ctx->tmp, ctx->W, ctx->drho);
@@ -1504,7 +1614,7 @@ OTM_API int otm_oc_create(int ne, double min_rho, double step_limit, double damp
     o->damp = damp;
     try {
       hip_check(hipSetDevice(device), "hipSetDevice");
-      hip_check(hipStreamCreateWithFlags(&o->s, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipStreamCreateWithFlags(&o->s, hipStreamDefault), "hipStreamCreate");  // ordered with the default stream
       o->newsens = dalloc<float>(ne);
       o->a = dalloc<float>(ne);
       o->b = dalloc<float>(ne);
@@ -1690,7 +1800,7 @@ OTM_API int otm_mma_create(int m, int n, double a0, double a, double c, double d
     o->device = device;
     try {
       hip_check(hipSetDevice(device), "hipSetDevice");
-      hip_check(hipStreamCreateWithFlags(&o->s, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipStreamCreateWithFlags(&o->s, hipStreamDefault), "hipStreamCreate");  // ordered with the default stream
       auto vec = [&](long long cnt) {
         double* p = dalloc<double>(cnt);
         o->owned.push_back(p);

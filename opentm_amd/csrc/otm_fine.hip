@@ -56,18 +56,18 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual_case(Geom g, Fine
 }
 
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
-                                                         const float* __restrict__ chi, Reduce R) {
+                                                         ChiRef chi, Reduce R) {
   fine_kappa_h_body(g, fc, kappa, chi, R, xcd_block(), gridDim.x);
 }
 
-__global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, const float* __restrict__ chi,
+__global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, ChiRef chi,
                                                              float w0, float w1, float w2, float w3, float w4, float w5,
                                                              float* __restrict__ sens, int pitch) {
   const float w[6] = {w0, w1, w2, w3, w4, w5};
   fine_sensitivity_body(g, fc, chi, w, sens, pitch, gtid());
 }
 
-__global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, const float* __restrict__ chi,
+__global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, ChiRef chi,
                                                          float* __restrict__ S6) {
   fine_energy6_body(g, fc, chi, S6, gtid());
 }
@@ -164,19 +164,19 @@ void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& f
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 
-void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
+void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const ChiRef& chi,
                          const Reduce& R, unsigned nblocks) {
   hipLaunchKernelGGL(k_fine_kappa_h, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, chi, R);
   launch_reduce_final(s, R, 6, nblocks);
 }
 
-void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
+void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
                              float* sens, int pitch) {
   hipLaunchKernelGGL(k_fine_sensitivity, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, chi, w[0], w[1], w[2], w[3],
                      w[4], w[5], sens, pitch);
 }
 
-void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6) {
+void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, float* S6) {
   hipLaunchKernelGGL(k_fine_energy6, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, chi, S6);
 }
 

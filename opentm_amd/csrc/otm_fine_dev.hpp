@@ -386,44 +386,81 @@ __device__ inline void fine_init_residual_case_body(const Geom& g, const FineCon
 }
 
 // Element energies (homogenization.cu:13-153 and :255-412). For element e
-// with nodal fields d_k = disp[:,k] - chi_k(nodes): S_ij = d_i^T Ke d_j,
-// using (Ke d)_a = c (5 d_a + d_{a^1} + d_{a^2} + d_{a^4} - sum d).
-__device__ inline void element_S(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
-                                 int x, int y, int z, float S[6]) {
+// with nodal fields d_k = disp[:,k] - chi_k(nodes): S_ij = d_i^T Ke d_j.
+// Ke has zero row sums and couples a node only to its 3 face-diagonal and
+// its body-diagonal partners (-c each), so S_ij = c sum over those 16 node
+// pairs (a, b) of D_i(a,b) D_j(a,b) with D_k(a,b) = d_k,a - d_k,b. The
+// differences are formed from the fluctuation field first: disp differences
+// are 0 or +-h exactly, and chi differences of neighbouring nodes are exact
+// or nearly so, so S keeps the relative precision of its own value (the
+// direct form d^T (Ke d) cancels to it from O(1) products and loses a factor
+// of the cell edge: 1e-5 at 256^3 in FP32). The fluctuation field is read
+// as FP64 (iterative refinement), or as the FP32 pair hi + lo of the plain
+// solve, or as FP32 alone.
+struct ChiRef {
+  const float* hi;
+  const float* lo;    // may be null
+  const double* d64;  // when set, used instead of hi/lo
+};
+
+// the 16 coupled node pairs of an element (local index ax + 2 ay + 4 az)
+__device__ constexpr int kPairA[16] = {0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 4, 5, 0, 1, 2, 3};
+__device__ constexpr int kPairB[16] = {3, 5, 6, 2, 4, 7, 4, 7, 5, 6, 7, 6, 7, 6, 5, 4};
+
+__device__ inline void element_S(const Geom& g, const FineConsts& fc, const ChiRef& chi, int x, int y, int z,
+                                 float S[6]) {
   const int xp = x == g.nx - 1 ? 0 : x + 1;
   const int yp = y == g.ny - 1 ? 0 : y + 1;
   const int zp = z == g.nz - 1 ? 0 : z + 1;
   const long long r0 = rowb(g, y, z), r1 = rowb(g, yp, z), r2 = rowb(g, y, zp), r3 = rowb(g, yp, zp);
   const long long nid[8] = {r0 + x, r0 + xp, r1 + x, r1 + xp, r2 + x, r2 + xp, r3 + x, r3 + xp};
-  float d[3][8], kd[3][8];
+  if (chi.d64) {
+    double v[3][8];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float tot = 0.f;
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int m = 0; m < 8; m++) v[k][m] = chi.d64[k * g.nn + nid[m]];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int a = kPairA[q], b = kPairB[q];
+      double D[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) D[k] = (double)(fc.disp[a][k] - fc.disp[b][k]) - (v[k][a] - v[k][b]);
+      acc[0] += D[0] * D[0]; acc[1] += D[0] * D[1]; acc[2] += D[0] * D[2];
+      acc[3] += D[1] * D[1]; acc[4] += D[1] * D[2]; acc[5] += D[2] * D[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) S[c] = (float)((double)fc.c * acc[c]);
+    return;
+  }
+  float v[3][8], w[3][8];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
 #pragma unroll
     for (int m = 0; m < 8; m++) {
-      d[k][m] = fc.disp[m][k] - chi[k * g.nn + nid[m]];
-      tot += d[k][m];
+      v[k][m] = chi.hi[k * g.nn + nid[m]];
+      w[k][m] = chi.lo ? chi.lo[k * g.nn + nid[m]] : 0.f;
     }
+  float acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int a = 0; a < 8; a++)
-      kd[k][a] = fc.c * (5.f * d[k][a] + d[k][a ^ 1] + d[k][a ^ 2] + d[k][a ^ 4] - tot);
+  for (int q = 0; q < 16; q++) {
+    const int a = kPairA[q], b = kPairB[q];
+    float D[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      D[k] = (fc.disp[a][k] - fc.disp[b][k]) - ((v[k][a] - v[k][b]) + (w[k][a] - w[k][b]));
+    acc[0] += D[0] * D[0]; acc[1] += D[0] * D[1]; acc[2] += D[0] * D[2];
+    acc[3] += D[1] * D[1]; acc[4] += D[1] * D[2]; acc[5] += D[2] * D[2];
   }
-  int c = 0;
 #pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = i; j < 3; j++) {
-      float s = 0.f;
-#pragma unroll
-      for (int a = 0; a < 8; a++) s += d[i][a] * kd[j][a];
-      S[c++] = s;
-    }
+  for (int c = 0; c < 6; c++) S[c] = fc.c * acc[c];
 }
 
 // kappa^H_ij = sum_e (kappa_e) S_e(i,j) with FP64 accumulation; kappa
 // already holds the 1e-9 floor.
 __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
-                                         const float* __restrict__ chi, const Reduce& R, int lb, int nb) {
+                                         const ChiRef& chi, const Reduce& R, int lb, int nb) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const BlockRange br = block_range_v(g.nn, lb, nb);
   for (long long t = br.b + threadIdx.x; t < br.e; t += blockDim.x) {
@@ -440,8 +477,8 @@ __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, co
 
 // sens[x + pitch*(y + ny*z)] = sum_c w_c S_c (contracted sensitivity with
 // respect to kappa; the penalty exponent is 1, homogenization.cu:344-346).
-__device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
-                                             const float w[6], float* __restrict__ sens, int pitch, long long t) {
+__device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
+                                             float* __restrict__ sens, int pitch, long long t) {
   if (t >= g.nn) return;
   int x, y, z;
   split_node(g.nx, g.ny, t, x, y, z);
@@ -452,7 +489,7 @@ __device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc
 }
 
 // S6[c*nn + e]: the six element energies, for callers that keep them.
-__device__ inline void fine_energy6_body(const Geom& g, const FineConsts& fc, const float* __restrict__ chi,
+__device__ inline void fine_energy6_body(const Geom& g, const FineConsts& fc, const ChiRef& chi,
                                          float* __restrict__ S6, long long t) {
   if (t >= g.nn) return;
   int x, y, z;

@@ -139,8 +139,13 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
     if (rho0) {
       ck(otm_set_design(hom.c, rho0, OTM_HOST));
     } else {
-      ck(otm_init_density(hom.c, p->initway, p->seed, p->volratio));
-      ck(otm_symmetrize_design(hom.c, p->sym));  // initDensity (Framework.cu:140-141)
+      otm_init_params ip;  // initDensity (Framework.cu:61-145), symmetry included
+      otm_init_params_init(&ip);
+      ip.initway = p->initway;
+      ip.seed = p->seed;
+      ip.volratio = p->volratio;
+      ip.sym = p->sym;
+      ck(otm_init_design(hom.c, &ip, nullptr));
     }
     otm_density_params dp;
     otm_density_params_init(&dp);  // Spline4 r = 1.5, non-periodic (Framework.cu:163)

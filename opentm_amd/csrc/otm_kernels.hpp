@@ -17,11 +17,12 @@ void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& f
                                const float* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks);
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const double* x, float* r, const Reduce& R, unsigned nblocks);
-void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* chi,
+struct ChiRef;  // fluctuation fields as FP64, an FP32 pair or FP32 (otm_fine_dev.hpp)
+void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const ChiRef& chi,
                          const Reduce& R, unsigned nblocks);
-void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, const float w[6],
+void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
                              float* sens, int pitch);
-void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const float* chi, float* S6);
+void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, float* S6);
 // Fine brick smoother (BX x kFineBY x kFineBZ bricks, frozen halo, result in uout);
 // fine_brick_x returns BX (32 or 16) or 0 when the cell does not tile.
 // With dot (fine_brick_dot_ok cells), the blocks also store their partial
@@ -119,6 +120,20 @@ struct InitModes {
   float phase[16];
 };
 void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes& M, float* rho);
+// randTri (Framework.cuh:238-310): basis sum per node (period P <= 10),
+// level-set map sums and application, and the noise initialiser's pieces
+struct RandTriRot {
+  float m[9];  // row-major rotation of the centred coordinates
+};
+int rand_tri_max_period();
+void launch_rand_tri(hipStream_t s, const Geom& g, int P, int centered, const RandTriRot& R, const float* coeffs,
+                     float* out);
+unsigned level_sum_blocks(long long n);
+void launch_level_sum(hipStream_t s, long long n, const float* val, float t, float hi, double* part);
+void launch_level_apply(hipStream_t s, long long n, float* val, float t, float hi);
+void launch_noise(hipStream_t s, long long n, unsigned long long seed, float* out);
+void launch_project_binary(hipStream_t s, long long n, float* v);
+void launch_scale_clamp(hipStream_t s, long long n, float* v, float scale, float lo, float hi);
 void launch_oc_begin(hipStream_t s, long long n, const float* sens, unsigned int* bits, OcState* st);
 constexpr int kOcTree = 5;                   // bisection levels per pass over the design
 constexpr int kOcCand = (1 << kOcTree) - 1;  // midpoints evaluated per pass

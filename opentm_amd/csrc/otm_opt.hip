@@ -448,6 +448,161 @@ void launch_init_density(hipStream_t s, const Geom& g, int kind, const InitModes
   hipLaunchKernelGGL(k_init_density, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, kind, M, rho);
 }
 
+// ---- random trigonometric start (randTri, Framework.cuh:238-310) -------------
+// Node value s = sum_i c_i phi_i + sum_{j,k} c_{6P + 6P j + k} psi_j psi_k over
+// the basis of randTribase_sincos_kernel (Framework.cuh:106-214): with
+// p = node / (edge - 1) per axis (optionally centred and rotated),
+// C[a][f] = cos(2 pi (f+1) p_a), S[a][f] = sin(2 pi (f+1) p_a);
+// first order: c_i C[i%3][i/3] for i < 3P, c_{3P+i} S[i%3][i/3];
+// second order: psi_j = C[j/P][j%P] for j < 3P, S[j/P-3][j%P] otherwise.
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_rand_tri(Geom g, int centered, RandTriRot R,
+                                                     const float* __restrict__ coeffs, float* __restrict__ out) {
+  constexpr int NB1 = 6 * P, NB = 6 * P + 36 * P * P;
+  __shared__ float cs[NB];
+  for (int i = threadIdx.x; i < NB; i += blockDim.x) cs[i] = coeffs[i];
+  __syncthreads();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.nn) return;
+  int ix, iy, iz;
+  split_node(g.nx, g.ny, t, ix, iy, iz);
+  float p[3] = {(float)ix / ((float)g.nx - 1.f), (float)iy / ((float)g.ny - 1.f), (float)iz / ((float)g.nz - 1.f)};
+  if (centered) {
+    const float v0 = p[0] - 0.5f, v1 = p[1] - 0.5f, v2 = p[2] - 0.5f;
+    p[0] = R.m[0] * v0 + R.m[1] * v1 + R.m[2] * v2;
+    p[1] = R.m[3] * v0 + R.m[4] * v1 + R.m[5] * v2;
+    p[2] = R.m[6] * v0 + R.m[7] * v1 + R.m[8] * v2;
+  }
+  const float pi2 = 3.1415926f * 2.f;
+  float psi[NB1];  // psi[a*P + f] = C[a][f], psi[3P + a*P + f] = S[a][f]
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int f = 0; f < P; f++) {
+      psi[a * P + f] = cosf(pi2 * (float)(f + 1) * p[a]);
+      psi[3 * P + a * P + f] = sinf(pi2 * (float)(f + 1) * p[a]);
+    }
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int f = 0; f < P; f++) {
+      s += cs[3 * f + a] * psi[a * P + f];
+      s += cs[3 * P + 3 * f + a] * psi[3 * P + a * P + f];
+    }
+#pragma unroll
+  for (int j = 0; j < NB1; j++) {
+    const float* row = cs + NB1 + NB1 * j;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < NB1; k++) acc += row[k] * psi[k];
+    s += acc * psi[j];
+  }
+  out[t] = s;
+}
+
+__device__ inline float level_map(float val, float t, float hi) {
+  // sigmoid(val, 15, t) (lib.cuh:1381) scaled to [0.001, hi] (Framework.cuh:287-293)
+  const float v = 1.f / (1.f + expf(-15.f * (val - t)));
+  return v * (hi - 0.001f) + 0.001f;
+}
+
+__global__ void __launch_bounds__(kBlock) k_level_sum(long long n, const float* __restrict__ val, float t, float hi,
+                                                      double* __restrict__ part) {
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    acc += (double)level_map(val[i], t, hi);
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_level_apply(long long n, float* __restrict__ val, float t, float hi) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) val[i] = level_map(val[i], t, hi);
+}
+
+// Counter-based uniform in [0, 1): the top 24 bits of splitmix64 of
+// (seed << 32) + index (one output per node, reproducible on any device).
+__device__ inline float hash_uniform(unsigned long long seed, unsigned long long i) {
+  unsigned long long z = (seed << 32) + i + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (float)(z >> 40) * (1.f / 16777216.f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_noise(long long n, unsigned long long seed, float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = hash_uniform(seed, (unsigned long long)i);
+}
+
+// projectTensor_kernel (TensorExpression.cu:106-115) with beta 20, tau 0.5:
+// tanproj, then below 0.5 -> 1e-9, above -> 1.
+__global__ void __launch_bounds__(kBlock) k_project_binary(long long n, float* __restrict__ v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float r = tanproj(v[i], 20.f);
+  if (r < 0.5f) r = 1e-9f;
+  if (r > 0.5f) r = 1.f;
+  v[i] = r;
+}
+
+// rho = clamp(rho * s, lo, hi)
+__global__ void __launch_bounds__(kBlock) k_scale_clamp(long long n, float* __restrict__ v, float s, float lo,
+                                                        float hi) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float r = v[i] * s;
+  if (r < lo) r = lo;
+  if (r >= hi) r = hi;
+  v[i] = r;
+}
+
+int rand_tri_max_period() { return 10; }
+
+void launch_rand_tri(hipStream_t s, const Geom& g, int P, int centered, const RandTriRot& R, const float* coeffs,
+                     float* out) {
+  const dim3 grid(nblk(g.nn)), block(kBlock);
+  switch (P) {
+    case 1: hipLaunchKernelGGL(k_rand_tri<1>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 2: hipLaunchKernelGGL(k_rand_tri<2>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 3: hipLaunchKernelGGL(k_rand_tri<3>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 4: hipLaunchKernelGGL(k_rand_tri<4>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 5: hipLaunchKernelGGL(k_rand_tri<5>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 6: hipLaunchKernelGGL(k_rand_tri<6>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 7: hipLaunchKernelGGL(k_rand_tri<7>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 8: hipLaunchKernelGGL(k_rand_tri<8>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 9: hipLaunchKernelGGL(k_rand_tri<9>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    case 10: hipLaunchKernelGGL(k_rand_tri<10>, grid, block, 0, s, g, centered, R, coeffs, out); break;
+    default: throw std::invalid_argument("randTri period must be 1..10");
+  }
+}
+unsigned level_sum_blocks(long long n) {
+  const long long b = (n + kBlock - 1) / kBlock;
+  return (unsigned)(b > 512 ? 512 : (b < 1 ? 1 : b));
+}
+void launch_level_sum(hipStream_t s, long long n, const float* val, float t, float hi, double* part) {
+  hipLaunchKernelGGL(k_level_sum, dim3(level_sum_blocks(n)), dim3(kBlock), 0, s, n, val, t, hi, part);
+}
+void launch_level_apply(hipStream_t s, long long n, float* val, float t, float hi) {
+  hipLaunchKernelGGL(k_level_apply, dim3(nblk(n)), dim3(kBlock), 0, s, n, val, t, hi);
+}
+void launch_noise(hipStream_t s, long long n, unsigned long long seed, float* out) {
+  hipLaunchKernelGGL(k_noise, dim3(nblk(n)), dim3(kBlock), 0, s, n, seed, out);
+}
+void launch_project_binary(hipStream_t s, long long n, float* v) {
+  hipLaunchKernelGGL(k_project_binary, dim3(nblk(n)), dim3(kBlock), 0, s, n, v);
+}
+void launch_scale_clamp(hipStream_t s, long long n, float* v, float scale, float lo, float hi) {
+  hipLaunchKernelGGL(k_scale_clamp, dim3(nblk(n)), dim3(kBlock), 0, s, n, v, scale, lo, hi);
+}
+
 // Reach of the taps (largest offset along any axis) and the sum of all
 // weights in tap order, in FP32 as the kernel adds them.
 static void tap_stats(const Taps& T, int& h, float& wall) {

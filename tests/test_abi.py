@@ -106,6 +106,7 @@ def test_struct_layouts_match_header():
     import re
     hdr = open(os.path.join(ROOT, "include", "opentm.h")).read()
     for cname, py in [("otm_config", _capi.OtmConfig), ("otm_density_params", _capi.OtmDensityParams),
+                      ("otm_init_params", _capi.OtmInitParams),
                       ("otm_custom_params", _capi.OtmCustomParams), ("otm_custom_report", _capi.OtmCustomReport),
                       ("otm_request", _capi.OtmRequest), ("otm_result", _capi.OtmResult),
                       ("otm_server_counters", _capi.OtmServerCounters)]:

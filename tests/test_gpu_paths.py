@@ -251,3 +251,39 @@ def test_graph_vcycle_bitwise_equal_to_eager(kw):
         np.testing.assert_array_equal(Cg, Ce)
         assert ig == ie
     np.testing.assert_array_equal(chig, chie)
+
+
+# ------------------------------------------------------------ start designs
+@pytest.mark.parametrize("initway,sym", [("random", "NONE"), ("randcenter", "NONE"), ("random", "reflect3"),
+                                         ("noise", "NONE"), ("noise", "rotate2")])
+def test_random_start_designs_match_oracle(initway, sym):
+    """initDensity's random starts (randTri with its level-set bisection, and
+    noise) on the GPU against the numpy restatement with the same
+    coefficients / counter-based uniforms."""
+    from oracle import initializers as ini
+    n, P, vol = 24, 4, 0.3
+    say("start design", initway, sym)
+    hom = make(n)
+    coeffs = hom.init_design(initway, seed=5, volratio=vol, n_period=P, sym=sym)
+    rho = hom.get_design().reshape(n, n, n).astype(np.float64)
+    if initway == "noise":
+        ref = ini.noise((n, n, n), 5, vol, sym)
+        bad = np.abs(rho - ref) > 1e-6
+        say("  mismatched", int(bad.sum()))
+        assert bad.sum() <= 2  # a uniform within FP32 rounding of the projection's threshold
+    else:
+        ref, _ = ini.rand_tri((n, n, n), P, coeffs[:6 * P + 36 * P * P] if initway == "random" else coeffs, vol,
+                              centered=initway == "randcenter", sym=sym)
+        err = np.abs(rho - ref)
+        say("  max abs err", float(err.max()), "mean", float(err.mean()))
+        # the basis sum runs in FP32 over ~700 terms; the steep sigmoid (slope
+        # up to 15 / 4 per unit) turns its rounding into at most a few 1e-4
+        assert err.mean() < 1e-4 and err.max() < 2e-2
+    assert rho.min() >= 0.1 - 1e-7 and rho.max() <= 1.0
+    # rep_randcenter reproduces randcenter from its coefficients
+    if initway == "randcenter" and sym == "NONE":
+        hom2 = make(n)
+        hom2.init_design("rep_randcenter", volratio=vol, n_period=P, coeffs=coeffs)
+        np.testing.assert_array_equal(hom2.get_design(), hom.get_design())
+        hom2.close()
+    hom.close()
