@@ -148,11 +148,23 @@ def test_coarsest_solve_is_the_pseudo_inverse(reso):
     n = int(np.prod(shape))
     K = dense(hom.stencil(L).astype(np.float64), shape)
     K = 0.5 * (K + K.T)
+    # the FP32 stencil's rows sum to zero only to rounding (about 4e-8 of the
+    # largest entry): lstsq on that K sees no null space and its answer
+    # carries a large constant, so the reference operator is the exactly
+    # singular one: off-diagonals as stored, diagonal = -(their row sum)
+    rowsum = float(np.abs(K.sum(axis=1)).max() / np.abs(K).max())
+    say("  row sums", rowsum)
+    assert rowsum < 1e-6
+    K0 = K - np.diag(np.diag(K))
+    K0 -= np.diag(K0.sum(axis=1))
     f = rng.standard_normal((3, n))
     f -= f.mean(axis=1, keepdims=True)
     u = hom.coarsest_solve(f.astype(np.float32)).reshape(3, n).astype(np.float64)
     for c in range(3):
-        ref = np.linalg.lstsq(K, f[c], rcond=None)[0]
+        ref = np.linalg.lstsq(K0, f[c], rcond=None)[0]
+        ref -= ref.mean()
         err = rel(u[c] - u[c].mean(), ref)
-        say("  rhs", c, "n", n, "err", err)
+        res = rel(K0 @ u[c] - f[c], f[c])
+        say("  rhs", c, "n", n, "err", err, "residual", res)
         assert err < 1e-5
+        assert res < 1e-5
