@@ -1763,13 +1763,15 @@ OTM_API int otm_oc_step(otm_ctx* ctx, otm_oc* oc, double filter_radius, double v
 struct otm_mma {
   int m = 0, device = 0;
   long long n = 0;
-  double a0 = 1, a = 0, c = 1e3, d = 1, move = 1.0;
+  double a0 = 1, a = 0, c = 1e3, d = 1;
   hipStream_t s = nullptr;
   MmaArrays A;
   double* gval = nullptr;
   void* small = nullptr;
   void* small_host = nullptr;  // pinned
-  float* xstage = nullptr;     // FP32 staging for host callers
+  float* fx = nullptr;         // design, previous designs, gradients (FP32): (5 + m) n floats
+  float* fbounds = nullptr;    // per-element bounds (2n floats), once set
+  float* xout = nullptr;       // FP32 result staging for host callers
   std::vector<double*> owned;
 };
 
@@ -1779,7 +1781,9 @@ static void mma_free(otm_mma* o) {
   for (double* p : o->owned) dfree(p);
   dfree(o->gval);
   dfree(o->small);
-  dfree(o->xstage);
+  dfree(o->fx);
+  dfree(o->fbounds);
+  dfree(o->xout);
   if (o->small_host) (void)hipHostFree(o->small_host);
   if (o->s) (void)hipStreamDestroy(o->s);
   delete o;
@@ -1808,21 +1812,26 @@ OTM_API int otm_mma_create(int m, int n, double a0, double a, double c, double d
       };
       MmaArrays& A = o->A;
       A.n = n;
-      A.xvar = vec(n); A.xmin = vec(n); A.xmax = vec(n); A.xold1 = vec(n); A.xold2 = vec(n);
-      A.low = vec(n); A.upp = vec(n); A.alfa = vec(n); A.beta = vec(n); A.p0 = vec(n); A.q0 = vec(n);
-      A.P = vec((long long)m * n); A.Q = vec((long long)m * n);
-      A.df0dx = vec(n); A.dgdx = vec((long long)m * n);
-      A.xs = vec(n); A.xsi = vec(n); A.eta = vec(n); A.dx = vec(n);
+      const size_t nf = (size_t)n;
+      o->fx = dalloc<float>((size_t)(4 + m) * nf);
+      A.xvar = o->fx;
+      A.xold1 = o->fx + nf;
+      A.xold2 = o->fx + 2 * nf;
+      A.df0dx = o->fx + 3 * nf;
+      A.dgdx = o->fx + 4 * nf;
+      A.xmin = A.xmax = nullptr;  // uniform [0, 1] until set (mmaOptimizer.h:21-22)
+      A.xmin_u = 0.0;
+      A.xmax_u = 1.0;
+      A.move = 1.0;
+      A.low = vec(n); A.upp = vec(n);
+      A.xs = vec(n); A.xsi = vec(n); A.eta = vec(n);
       A.xs_t = vec(n); A.xsi_t = vec(n); A.eta_t = vec(n);
       A.partials = vec(1024 * 32);
       o->gval = dalloc<double>(kMmaMaxM);
       o->small = dalloc<char>(mma_small_bytes());
-      o->xstage = dalloc<float>(n);
+      o->xout = dalloc<float>(n);
       hip_check(hipHostMalloc(&o->small_host, mma_small_bytes(), hipHostMallocDefault), "hipHostMalloc");
-      launch_fill_d(o->s, A.xmin, n, 0.0);
-      launch_fill_d(o->s, A.xmax, n, 1.0);
-      launch_fill_d(o->s, A.xold1, n, 0.0);
-      launch_fill_d(o->s, A.xold2, n, 0.0);
+      launch_fill(o->s, A.xold1, 2 * n, 0.f);
       hip_check(hipStreamSynchronize(o->s), "sync");
     } catch (...) {
       mma_free(o);
@@ -1839,17 +1848,17 @@ OTM_API int otm_mma_destroy(otm_mma* mma) {
 OTM_API int otm_mma_set_bounds(otm_mma* o, double xmin, double xmax) {
   return guarded([&] {
     if (!o || !(xmax > xmin)) fail(OTM_E_ARG, "invalid bounds");
-    hip_check(hipSetDevice(o->device), "hipSetDevice");
-    launch_fill_d(o->s, o->A.xmin, o->n, xmin);
-    launch_fill_d(o->s, o->A.xmax, o->n, xmax);
-    hip_check(hipStreamSynchronize(o->s), "sync");
+    o->A.xmin = o->A.xmax = nullptr;
+    o->A.xmin_u = xmin;
+    o->A.xmax_u = xmax;
   });
 }
 
-static const float* mma_stage(otm_mma* o, const float* p, int where) {
-  if (where == OTM_DEVICE) return p;
-  hip_check(hipMemcpyAsync(o->xstage, p, sizeof(float) * o->n, hipMemcpyHostToDevice, o->s), "upload");
-  return o->xstage;
+// copies n floats from the caller's host or device array into dst (device)
+static void mma_copy_in(otm_mma* o, float* dst, const float* src, int where) {
+  hip_check(hipMemcpyAsync(dst, src, sizeof(float) * o->n,
+                           where == OTM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, o->s),
+            "MMA input copy");
 }
 
 OTM_API int otm_mma_set_bounds_array(otm_mma* o, const float* xmin, const float* xmax, int where) {
@@ -1857,16 +1866,19 @@ OTM_API int otm_mma_set_bounds_array(otm_mma* o, const float* xmin, const float*
     if (!o || !xmin || !xmax) fail(OTM_E_ARG, "null argument");
     check_where(where);
     hip_check(hipSetDevice(o->device), "hipSetDevice");
-    launch_f2d(o->s, o->n, mma_stage(o, xmin, where), o->A.xmin);
-    launch_f2d(o->s, o->n, mma_stage(o, xmax, where), o->A.xmax);
+    if (!o->fbounds) o->fbounds = dalloc<float>(2 * (size_t)o->n);
+    mma_copy_in(o, o->fbounds, xmin, where);
+    mma_copy_in(o, o->fbounds + o->n, xmax, where);
     hip_check(hipStreamSynchronize(o->s), "sync");
+    o->A.xmin = o->fbounds;
+    o->A.xmax = o->fbounds + o->n;
   });
 }
 
 OTM_API int otm_mma_set_move(otm_mma* o, double move) {
   return guarded([&] {
     if (!o || !(move > 0)) fail(OTM_E_ARG, "invalid move");
-    o->move = move;
+    o->A.move = move;
   });
 }
 
@@ -1875,23 +1887,22 @@ OTM_API int otm_mma_update(otm_mma* o, int itn, float* x, const float* dfdx, con
   return guarded([&] {
     if (!o || !x || !dfdx || !gval || !dgdx) fail(OTM_E_ARG, "null argument");
     check_where(where);
+    for (int i = 0; i < o->m; i++)
+      if (!dgdx[i]) fail(OTM_E_ARG, "null constraint gradient");
     hip_check(hipSetDevice(o->device), "hipSetDevice");
     const long long n = o->n;
-    // FP32 -> FP64 (mmaOptimizer.cu:63-68)
-    launch_f2d(o->s, n, mma_stage(o, x, where), o->A.xvar);
-    launch_f2d(o->s, n, mma_stage(o, dfdx, where), o->A.df0dx);
-    for (int i = 0; i < o->m; i++) {
-      if (!dgdx[i]) fail(OTM_E_ARG, "null constraint gradient");
-      launch_f2d(o->s, n, mma_stage(o, dgdx[i], where), o->A.dgdx + (long long)i * n);
-    }
+    // the FP32 design and gradients are kept as they are; every pass widens
+    // them to FP64 (exact), as the reference's type_cast does (mmaOptimizer.cu:63-68)
+    mma_copy_in(o, o->A.xvar, x, where);
+    mma_copy_in(o, (float*)o->A.df0dx, dfdx, where);
+    for (int i = 0; i < o->m; i++) mma_copy_in(o, (float*)o->A.dgdx + (long long)i * n, dgdx[i], where);
     hip_check(hipMemcpyAsync(o->gval, gval, sizeof(double) * o->m, hipMemcpyHostToDevice, o->s), "gval");
-    float* xout = where == OTM_DEVICE ? x : o->xstage;
-    const int rc = mma_update(o->s, o->m, o->A, o->small, o->small_host, o->gval, itn, o->move, o->a0, o->a, o->c,
-                              o->d, xout);
+    float* xout = where == OTM_DEVICE ? x : o->xout;
+    const int rc = mma_update(o->s, o->m, o->A, o->small, o->small_host, o->gval, itn, o->a0, o->a, o->c, o->d, xout);
     if (rc == -2) fail(OTM_E_NUMERIC, "MMA subsolver produced non-finite values");
     if (rc < 0) fail(OTM_E_HIP, "MMA update failed");
     if (where == OTM_HOST)
-      hip_check(hipMemcpyAsync(x, o->xstage, sizeof(float) * n, hipMemcpyDeviceToHost, o->s), "download x");
+      hip_check(hipMemcpyAsync(x, o->xout, sizeof(float) * n, hipMemcpyDeviceToHost, o->s), "download x");
     hip_check(hipStreamSynchronize(o->s), "sync");
     if (newton) *newton = rc;
   });

@@ -156,13 +156,17 @@ void launch_oc_apply(hipStream_t s, long long n, const float* sens, float* rho, 
 
 // ---- MMA (otm_mma.hip) ----
 constexpr int kMmaMaxM = 4;  // constraints supported by the device subsolver
+constexpr int kMmaMaxNS = 2 * kMmaMaxM + (kMmaMaxM * (kMmaMaxM + 1)) / 2;  // Newton-system sums
 struct MmaArrays {
   long long n;
-  double *xvar, *xmin, *xmax, *xold1, *xold2, *low, *upp;  // persistent state
-  double *alfa, *beta, *p0, *q0, *P, *Q;                   // subproblem
-  double *df0dx, *dgdx;                                    // inputs (dgdx: m x n)
-  double *xs, *xsi, *eta, *dx, *xs_t, *xsi_t, *eta_t;      // interior point
-  double* partials;                                        // [blocks * 32]
+  float *xvar, *xold1, *xold2;  // design and the two previous designs (FP32, as the caller holds them)
+  const float *xmin, *xmax;     // per-element bounds, or null: xmin_u / xmax_u for every element
+  double xmin_u, xmax_u, move;
+  double *low, *upp;              // asymptotes, carried between updates
+  const float *df0dx, *dgdx;      // gradients (dgdx: m x n)
+  double *xs, *xsi, *eta;         // interior point
+  double *xs_t, *xsi_t, *eta_t;   // its trial
+  double* partials;               // [blocks * 32]
 };
 inline unsigned red_blocks_for_mma(long long n) {
   long long b = (n + kBlock - 1) / kBlock;
@@ -174,7 +178,7 @@ size_t mma_small_bytes();
 // One MMA update on device arrays; returns Newton steps or a negative code
 // (-1 HIP error, -2 non-finite values, -3 unsupported m).
 int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
-               int itn, double move, double a0, double av, double cv, double dv, float* xout);
+               int itn, double a0, double av, double cv, double dv, float* xout);
 void launch_f2d(hipStream_t s, long long n, const float* in, double* out);
 void launch_fill_d(hipStream_t s, double* a, long long n, double v);
 

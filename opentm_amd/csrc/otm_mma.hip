@@ -2,14 +2,27 @@
 //
 // Restates mmasub_ker (mmaOpt.cu:834-1019) and its primal-dual interior
 // point subsolv_g (mmaOpt.cu:429-830). The reference runs every vector
-// expression as its own kernel and solves the (m+1) Newton system on the
-// host each step (solveLinearHost, mmaOpt.cpp:48-73). Here each Newton step
-// is three fused passes over the n design variables:
-//   newton: plambda/qlambda, delx, diagx, GG and the reductions that build
-//           the (m+1) system (gvec, GG.(delx/diagx), GG diag(1/diagx) GG^T);
-//   dir:    dx, dxsi, deta and the fraction-to-boundary bound;
-//   trial:  the trial point and its KKT residual norm/max;
-// each followed by a one-block finaliser that runs the scalar logic (the
+// expression as its own kernel, keeps the subproblem coefficients (alfa,
+// beta, p0, q0, P, Q) as FP64 arrays and solves the (m+1) Newton system on
+// the host each step (solveLinearHost, mmaOpt.cpp:48-73).
+//
+// Every pass over the n design variables is bound by HBM bandwidth, so its
+// time is its bytes per element. The coefficients are therefore not stored:
+// each pass recomputes them from the FP32 design x, the FP32 gradients (the
+// reference casts both to FP64, which is exact) and the FP64 asymptotes, for
+// about 60 FP64 operations per element instead of 8 (4 + 2m) bytes of
+// reads. A pass reads 4 + 16 + 4 (1 + m) + 24 bytes per element (x, low and
+// upp, gradients, the interior point; per-element bounds add 8). Each Newton
+// step is two passes:
+//   dir:   the Newton direction dx from the small system's dlam and the
+//          fraction-to-boundary bound (dx is not stored);
+//   point: the trial point (dx recomputed by the same function as in dir,
+//          so the same bits), its KKT residual norm/max and, speculatively,
+//          the sums that build the Newton system at that point (gvec,
+//          GG.(delx/diagx), GG diag(1/diagx) GG^T). When the line search
+//          accepts the point these are the next step's system; when it
+//          rejects it they are dropped.
+// Each pass is followed by a one-block finaliser for the scalar logic (the
 // (m+1) solve by Gaussian elimination with partial pivoting, step length,
 // line-search bookkeeping) on the device. m is limited to kMmaMaxM.
 #include <math.h>
@@ -29,6 +42,9 @@ struct MmaSmall {
   double yt[kMmaMaxM], lamt[kMmaMaxM], mut[kMmaMaxM], st[kMmaMaxM], zt, zett;
   double steg, stm_small;
   double residunorm, residumax, resinew, rmaxnew;
+  // Newton-system sums at the current point and at the trial point:
+  // gvec[m], GG.(delx/diagx)[m], upper triangle of GG diag(1/diagx) GG^T
+  double ns[kMmaMaxNS], ns_t[kMmaMaxNS];
 };
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -101,58 +117,140 @@ __device__ inline void final_sum_max(const double* partials, int nblocks, double
   }
 }
 
-// ---- mmasub: asymptotes, bounds, approximation coefficients ---------------
+// ---- per-element subproblem data (mmaOpt.cu:870-1000) ----------------------
+constexpr double kRaa0 = 1e-5, kAlbefa = 0.1, kAsyinit = 0.5, kAsyincr = 1.2, kAsydecr = 0.7;
+
 template <int M>
-__global__ void __launch_bounds__(kBlock) k_mma_prep(MmaArrays A, int itn, double move) {
-  constexpr double raa0 = 1e-5, albefa = 0.1, asyinit = 0.5, asyincr = 1.2, asydecr = 0.7;
+struct MmaElem {
+  double lo, up;      // asymptotes
+  double al, be;      // bounds of the subproblem variable
+  double p0, q0;      // objective approximation
+  double P[M], Q[M];  // constraint approximations
+};
+
+__device__ inline void mma_bounds(const MmaArrays& A, long long j, double& xmn, double& xmx) {
+  if (A.xmin) {
+    xmn = (double)A.xmin[j];
+    xmx = (double)A.xmax[j];
+  } else {
+    xmn = A.xmin_u;
+    xmx = A.xmax_u;
+  }
+}
+
+// alfa, beta, p0, q0, P, Q of element j at the design x, given e.lo/e.up.
+// Every pass calls this one function with contraction off, so every pass
+// works with the same coefficient bits.
+template <int M>
+__device__ inline void mma_coefficients(const MmaArrays& A, long long j, double x, double xmn, double xmx,
+                                        MmaElem<M>& e) {
+#pragma clang fp contract(off)
+  const double dxm = xmx - xmn;
+  e.al = fmax(fmax(e.lo + kAlbefa * (x - e.lo), x - A.move * dxm), xmn);
+  e.be = fmin(fmin(e.up - kAlbefa * (e.up - x), x + A.move * dxm), xmx);
+  const double xmamiinv = 1.0 / fmax(dxm, 1e-5);
+  const double ux1 = e.up - x, xl1 = x - e.lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
+  const double reg = kRaa0 * xmamiinv;
+  const double g0 = (double)A.df0dx[j];
+  const double dp = fmax(g0, 0.0), dq = fmax(-g0, 0.0);
+  const double pq0 = 1e-3 * (dp + dq) + reg;
+  e.p0 = (dp + pq0) * ux2;
+  e.q0 = (dq + pq0) * xl2;
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    const double g = (double)A.dgdx[i * A.n + j];
+    const double pp = fmax(g, 0.0), qq = fmax(-g, 0.0);
+    const double pq = 1e-3 * (pp + qq) + reg;
+    e.P[i] = (pp + pq) * ux2;
+    e.Q[i] = (qq + pq) * xl2;
+  }
+}
+
+template <int M>
+__device__ inline void mma_load(const MmaArrays& A, long long j, MmaElem<M>& e) {
+  double xmn, xmx;
+  mma_bounds(A, j, xmn, xmx);
+  e.lo = A.low[j];
+  e.up = A.upp[j];
+  mma_coefficients<M>(A, j, (double)A.xvar[j], xmn, xmx, e);
+}
+
+// Newton-step terms of one element at the point (x, xsi, eta) for the
+// multipliers lam (mmaOpt.cu:540-600, :680-700).
+template <int M>
+struct MmaNewtonElem {
+  double ux1, xl1, dpsidx, delx, diagx, GG[M];
+};
+
+template <int M>
+__device__ inline void mma_newton_elem(const MmaElem<M>& e, double x, double xsi, double eta, const double (&lam)[M],
+                                       double epsi, MmaNewtonElem<M>& q) {
+#pragma clang fp contract(off)
+  const double xa = x - e.al, bx = e.be - x;
+  const double ux1 = e.up - x, xl1 = x - e.lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
+  double plam = e.p0, qlam = e.q0;
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    plam += lam[i] * e.P[i];
+    qlam += lam[i] * e.Q[i];
+    q.GG[i] = e.P[i] / ux2 - e.Q[i] / xl2;
+  }
+  q.ux1 = ux1;
+  q.xl1 = xl1;
+  q.dpsidx = plam / ux2 - qlam / xl2;
+  q.delx = q.dpsidx - epsi / xa + epsi / bx;
+  q.diagx = 2.0 * (plam / (ux2 * ux1) + qlam / (xl2 * xl1)) + xsi / xa + eta / bx;
+}
+
+// dx = -(delx + GG^T dlam) / diagx (mmaOpt.cu:680-690)
+template <int M>
+__device__ inline double mma_dx(const MmaNewtonElem<M>& q, const double (&dlam)[M]) {
+#pragma clang fp contract(off)
+  double gd = 0.0;
+#pragma unroll
+  for (int i = 0; i < M; i++) gd += q.GG[i] * dlam[i];
+  return -q.delx / q.diagx - gd / q.diagx;
+}
+
+// ---- mmasub: asymptotes, right-hand sides, subsolver start ----------------------
+template <int M>
+__global__ void __launch_bounds__(kBlock) k_mma_prep(MmaArrays A, int itn) {
   double bs[M];
 #pragma unroll
   for (int i = 0; i < M; i++) bs[i] = 0.0;
   double dummy[1] = {-1e300};
   const BlockRange br = block_range(A.n);
   for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
-    const double x = A.xvar[j], dxm = A.xmax[j] - A.xmin[j];
-    double lo, up;
+    const double x = (double)A.xvar[j];
+    double xmn, xmx;
+    mma_bounds(A, j, xmn, xmx);
+    const double dxm = xmx - xmn;
+    MmaElem<M> e;
     if (itn <= 2) {
-      lo = x - asyinit * dxm;
-      up = x + asyinit * dxm;
+      e.lo = x - kAsyinit * dxm;
+      e.up = x + kAsyinit * dxm;
     } else {
-      const double x1 = A.xold1[j], x2 = A.xold2[j];
+      const double x1 = (double)A.xold1[j], x2 = (double)A.xold2[j];
       const double zzz = (x - x1) * (x1 - x2);
-      const double f = zzz > 0 ? asyincr : (zzz < 0 ? asydecr : 1.0);
-      lo = x - f * (x1 - A.low[j]);
-      up = x + f * (A.upp[j] - x1);
+      const double f = zzz > 0 ? kAsyincr : (zzz < 0 ? kAsydecr : 1.0);
+      double lo = x - f * (x1 - A.low[j]);
+      double up = x + f * (A.upp[j] - x1);
       lo = fmin(fmax(lo, x - 10.0 * dxm), x - 0.01 * dxm);
       up = fmax(fmin(up, x + 10.0 * dxm), x + 0.01 * dxm);
+      e.lo = lo;
+      e.up = up;
     }
-    A.low[j] = lo;
-    A.upp[j] = up;
-    const double al = fmax(fmax(lo + albefa * (x - lo), x - move * dxm), A.xmin[j]);
-    const double be = fmin(fmin(up - albefa * (up - x), x + move * dxm), A.xmax[j]);
-    A.alfa[j] = al;
-    A.beta[j] = be;
-    const double xmamiinv = 1.0 / fmax(dxm, 1e-5);
-    const double ux1 = up - x, xl1 = x - lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
-    const double g0 = A.df0dx[j];
-    const double dp = fmax(g0, 0.0), dq = fmax(-g0, 0.0);
-    const double pq0 = 1e-3 * (dp + dq) + raa0 * xmamiinv;
-    A.p0[j] = (dp + pq0) * ux2;
-    A.q0[j] = (dq + pq0) * xl2;
+    A.low[j] = e.lo;
+    A.upp[j] = e.up;
+    mma_coefficients<M>(A, j, x, xmn, xmx, e);
+    const double ux1 = e.up - x, xl1 = x - e.lo;
 #pragma unroll
-    for (int i = 0; i < M; i++) {
-      const double g = A.dgdx[i * A.n + j];
-      const double pp = fmax(g, 0.0), qq = fmax(-g, 0.0);
-      const double pq = 1e-3 * (pp + qq) + raa0 * xmamiinv;
-      const double Pi = (pp + pq) * ux2, Qi = (qq + pq) * xl2;
-      A.P[i * A.n + j] = Pi;
-      A.Q[i * A.n + j] = Qi;
-      bs[i] += Pi / ux1 + Qi / xl1;
-    }
+    for (int i = 0; i < M; i++) bs[i] += e.P[i] / ux1 + e.Q[i] / xl1;
     // subsolv start point (mmaOpt.cu:437-452)
-    const double xs = 0.5 * (al + be);
+    const double xs = 0.5 * (e.al + e.be);
     A.xs[j] = xs;
-    A.xsi[j] = fmax(1.0 / (xs - al), 1.0);
-    A.eta[j] = fmax(1.0 / (be - xs), 1.0);
+    A.xsi[j] = fmax(1.0 / (xs - e.al), 1.0);
+    A.eta[j] = fmax(1.0 / (e.be - xs), 1.0);
   }
   block_sum_max<M, 1>(bs, dummy, A.partials);
 }
@@ -203,25 +301,45 @@ __device__ inline void small_residual(const MmaSmall* S, const double* gvec, con
   }
 }
 
-// trial != 0: the point is old + steg * direction; it is written to the
-// trial buffers and the small-vector trial values to S->*t.
+// Sums of one point pass: [|residual|^2, gvec[M], GG.(delx/diagx)[M],
+// upper triangle of GG diag(1/diagx) GG^T]; gvec is shared by the residual
+// and the Newton system.
 template <int M>
-__global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmall* __restrict__ S, int trial) {
-  double lam[M];
+struct MmaPointNV {
+  static constexpr int NN = 2 * M + (M * (M + 1)) / 2;  // Newton-system sums
+  static constexpr int NS = 1 + NN;
+};
+static_assert(MmaPointNV<kMmaMaxM>::NN <= kMmaMaxNS, "Newton sums exceed their slots");
+static_assert(MmaPointNV<kMmaMaxM>::NS + 1 <= 32, "point pass partials exceed their slots");
+
+// TRIAL: the point is the current one + steg * (Newton direction); the
+// direction is recomputed as in k_mma_dir, the point written to the trial
+// buffers and the small-vector trial values are formed by the finaliser.
+template <int M, bool TRIAL>
+__global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmall* __restrict__ S) {
+  constexpr int NS = MmaPointNV<M>::NS;
+  double lam[M], lamc[M], dlam[M];
   const double epsi = S->epsi, steg = S->steg;
 #pragma unroll
-  for (int i = 0; i < M; i++) lam[i] = trial ? S->lam[i] + steg * S->dlam[i] : S->lam[i];
-  double sums[1 + M];
+  for (int i = 0; i < M; i++) {
+    lamc[i] = S->lam[i];
+    dlam[i] = S->dlam[i];
+    lam[i] = TRIAL ? lamc[i] + steg * dlam[i] : lamc[i];
+  }
+  double sums[NS];
 #pragma unroll
-  for (int i = 0; i < 1 + M; i++) sums[i] = 0.0;
+  for (int i = 0; i < NS; i++) sums[i] = 0.0;
   double mx[1] = {0.0};
   const BlockRange br = block_range(A.n);
   for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
-    const double al = A.alfa[j], be = A.beta[j];
+    MmaElem<M> e;
+    mma_load<M>(A, j, e);
     double x = A.xs[j], xsi = A.xsi[j], eta = A.eta[j];
-    if (trial) {
-      const double dx = A.dx[j];
-      const double xa = x - al, bx = be - x;
+    if (TRIAL) {
+      MmaNewtonElem<M> qc;
+      mma_newton_elem<M>(e, x, xsi, eta, lamc, epsi, qc);
+      const double dx = mma_dx<M>(qc, dlam);
+      const double xa = x - e.al, bx = e.be - x;
       const double dxsi = -xsi + epsi / xa - xsi * dx / xa;
       const double deta = -eta + epsi / bx + eta * dx / bx;
       x += steg * dx;
@@ -231,37 +349,44 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
       A.xsi_t[j] = xsi;
       A.eta_t[j] = eta;
     }
-    const double ux1 = A.upp[j] - x, xl1 = x - A.low[j];
-    double plam = A.p0[j], qlam = A.q0[j];
+    MmaNewtonElem<M> q;
+    mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
 #pragma unroll
-    for (int i = 0; i < M; i++) {
-      const double Pi = A.P[i * A.n + j], Qi = A.Q[i * A.n + j];
-      plam += lam[i] * Pi;
-      qlam += lam[i] * Qi;
-      sums[1 + i] += Pi / ux1 + Qi / xl1;
-    }
-    const double rex = plam / (ux1 * ux1) - qlam / (xl1 * xl1) - xsi + eta;
-    const double rexsi = xsi * (x - al) - epsi;
-    const double reeta = eta * (be - x) - epsi;
+    for (int i = 0; i < M; i++) sums[1 + i] += e.P[i] / q.ux1 + e.Q[i] / q.xl1;  // gvec
+    const double rex = q.dpsidx - xsi + eta;
+    const double rexsi = xsi * (x - e.al) - epsi;
+    const double reeta = eta * (e.be - x) - epsi;
     sums[0] += rex * rex + rexsi * rexsi + reeta * reeta;
     mx[0] = fmax(mx[0], fmax(fabs(rex), fmax(fabs(rexsi), fabs(reeta))));
+    // the Newton system at this point (mmaOpt.cu:560-600)
+    const double inv = 1.0 / q.diagx;
+    int t = 1 + 2 * M;
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      sums[1 + M + i] += q.GG[i] * q.delx * inv;
+#pragma unroll
+      for (int k = i; k < M; k++) sums[t++] += q.GG[i] * q.GG[k] * inv;
+    }
   }
-  block_sum_max<1 + M, 1>(sums, mx, A.partials);
+  block_sum_max<NS, 1>(sums, mx, A.partials);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmall* S, int nblocks, int trial) {
-  __shared__ double out[M + 2];
-  final_sum_max<1 + M, 1>(A.partials, nblocks, out);
+template <int M, bool TRIAL>
+__global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmall* S, int nblocks) {
+  constexpr int NS = MmaPointNV<M>::NS, NN = MmaPointNV<M>::NN;
+  __shared__ double out[NS + 1];
+  final_sum_max<NS, 1>(A.partials, nblocks, out);
   if (threadIdx.x != 0) return;
-  double ss = out[0], mx = out[1 + M];
-  const double steg = S->steg;
-  if (!trial) {
+  double ss = out[0], mx = out[NS];
+  double* ns = TRIAL ? S->ns_t : S->ns;
+  for (int i = 0; i < NN; i++) ns[i] = out[1 + i];
+  if (!TRIAL) {
     small_residual(S, out + 1, S->y, S->z, S->lam, S->mu, S->zet, S->s, ss, mx);
     S->residunorm = sqrt(ss);
     S->residumax = mx;
     return;
   }
+  const double steg = S->steg;
   for (int i = 0; i < M; i++) {
     S->yt[i] = S->y[i] + steg * S->dy[i];
     S->lamt[i] = S->lam[i] + steg * S->dlam[i];
@@ -281,7 +406,7 @@ __global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmal
   S->steg = steg / 2;
 }
 
-// After acceptance the trial becomes the current point.
+// After acceptance the trial becomes the current point, with its Newton sums.
 __global__ void k_mma_commit(MmaSmall* S) {
   for (int i = 0; i < S->m; i++) {
     S->y[i] = S->yt[i];
@@ -289,6 +414,7 @@ __global__ void k_mma_commit(MmaSmall* S) {
     S->mu[i] = S->mut[i];
     S->s[i] = S->st[i];
   }
+  for (int i = 0; i < kMmaMaxNS; i++) S->ns[i] = S->ns_t[i];
   S->z = S->zt;
   S->zet = S->zett;
   S->residunorm = S->resinew;
@@ -296,70 +422,19 @@ __global__ void k_mma_commit(MmaSmall* S) {
 }
 
 // ---- Newton system ---------------------------------------------------------------
+// Solves [Alam a; a^T -zet/z] [dlam; dz] = [blam; delz] from the point's
+// sums and the small directions (mmaOpt.cu:595-678). One thread.
 template <int M>
-struct NewtonNV {
-  static constexpr int NS = 2 * M + (M * (M + 1)) / 2;
-};
-
-template <int M>
-__global__ void __launch_bounds__(kBlock) k_mma_newton(MmaArrays A, const MmaSmall* __restrict__ S) {
-  constexpr int NS = NewtonNV<M>::NS;
-  double lam[M];
-#pragma unroll
-  for (int i = 0; i < M; i++) lam[i] = S->lam[i];
+__global__ void k_mma_newton_final(MmaSmall* S) {
   const double epsi = S->epsi;
-  double sums[NS];
-#pragma unroll
-  for (int i = 0; i < NS; i++) sums[i] = 0.0;
-  double dummy[1] = {-1e300};
-  const BlockRange br = block_range(A.n);
-  for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
-    const double x = A.xs[j], xsi = A.xsi[j], eta = A.eta[j];
-    const double xa = x - A.alfa[j], bx = A.beta[j] - x;
-    const double ux1 = A.upp[j] - x, xl1 = x - A.low[j];
-    const double ux2 = ux1 * ux1, xl2 = xl1 * xl1;
-    double plam = A.p0[j], qlam = A.q0[j];
-    double GG[M];
-#pragma unroll
-    for (int i = 0; i < M; i++) {
-      const double Pi = A.P[i * A.n + j], Qi = A.Q[i * A.n + j];
-      plam += lam[i] * Pi;
-      qlam += lam[i] * Qi;
-      sums[i] += Pi / ux1 + Qi / xl1;  // gvec
-      GG[i] = Pi / ux2 - Qi / xl2;
-    }
-    const double dpsidx = plam / ux2 - qlam / xl2;
-    const double delx = dpsidx - epsi / xa + epsi / bx;
-    const double diagx = 2.0 * (plam / (ux2 * ux1) + qlam / (xl2 * xl1)) + xsi / xa + eta / bx;
-    const double inv = 1.0 / diagx;
-    int t = 2 * M;
-#pragma unroll
-    for (int i = 0; i < M; i++) {
-      sums[M + i] += GG[i] * delx * inv;
-#pragma unroll
-      for (int k = i; k < M; k++) sums[t++] += GG[i] * GG[k] * inv;
-    }
-  }
-  block_sum_max<NS, 1>(sums, dummy, A.partials);
-}
-
-// Solves [Alam a; a^T -zet/z] [dlam; dz] = [blam; delz] and the small
-// directions (mmaOpt.cu:595-678).
-template <int M>
-__global__ void __launch_bounds__(kBlock) k_mma_newton_final(MmaArrays A, MmaSmall* S, int nblocks) {
-  constexpr int NS = NewtonNV<M>::NS;
-  __shared__ double out[NS + 1];
-  final_sum_max<NS, 1>(A.partials, nblocks, out);
-  if (threadIdx.x != 0) return;
-  const double epsi = S->epsi;
-  const double* gvec = out;
-  const double* GGdd = out + M;
+  const double* gvec = S->ns;
+  const double* GGdd = S->ns + M;
   double Alam[M][M];
   int t = 2 * M;
   for (int i = 0; i < M; i++)
     for (int k = i; k < M; k++) {
-      Alam[i][k] = out[t];
-      Alam[k][i] = out[t];
+      Alam[i][k] = S->ns[t];
+      Alam[k][i] = S->ns[t];
       t++;
     }
   constexpr int K = M + 1;
@@ -442,23 +517,13 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir(MmaArrays A, const MmaSmall*
   double mx[1] = {-1e300};
   const BlockRange br = block_range(A.n);
   for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
+    MmaElem<M> e;
+    mma_load<M>(A, j, e);
     const double x = A.xs[j], xsi = A.xsi[j], eta = A.eta[j];
-    const double xa = x - A.alfa[j], bx = A.beta[j] - x;
-    const double ux1 = A.upp[j] - x, xl1 = x - A.low[j];
-    const double ux2 = ux1 * ux1, xl2 = xl1 * xl1;
-    double plam = A.p0[j], qlam = A.q0[j], gd = 0.0;
-#pragma unroll
-    for (int i = 0; i < M; i++) {
-      const double Pi = A.P[i * A.n + j], Qi = A.Q[i * A.n + j];
-      plam += lam[i] * Pi;
-      qlam += lam[i] * Qi;
-      gd += (Pi / ux2 - Qi / xl2) * dlam[i];
-    }
-    const double dpsidx = plam / ux2 - qlam / xl2;
-    const double delx = dpsidx - epsi / xa + epsi / bx;
-    const double diagx = 2.0 * (plam / (ux2 * ux1) + qlam / (xl2 * xl1)) + xsi / xa + eta / bx;
-    const double dx = -delx / diagx - gd / diagx;
-    A.dx[j] = dx;
+    MmaNewtonElem<M> q;
+    mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
+    const double dx = mma_dx<M>(q, dlam);
+    const double xa = x - e.al, bx = e.be - x;
     const double dxsi = -xsi + epsi / xa - xsi * dx / xa;
     const double deta = -eta + epsi / bx + eta * dx / bx;
     mx[0] = fmax(mx[0], fmax(fmax(-1.01 * dxsi / xsi, -1.01 * deta / eta), fmax(-1.01 * dx / xa, 1.01 * dx / bx)));
@@ -478,48 +543,46 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir_final(MmaArrays A, MmaSmall*
 
 __global__ void k_mma_epsi(MmaSmall* S) { S->epsi *= 0.1; }
 
-// xold2 = xold1, xold1 = x, x = subsolver solution (mmaOpt.cu:1055-1057),
-// also the FP32 copy for the caller
+// xold2 = xold1, xold1 = x (mmaOpt.cu:1055-1057); the subsolver solution
+// goes to the caller's FP32 design
 __global__ void __launch_bounds__(kBlock) k_mma_finish(MmaArrays A, float* xout) {
   const long long j = gtid();
   if (j >= A.n) return;
-  const double xn = A.xs[j];
   A.xold2[j] = A.xold1[j];
   A.xold1[j] = A.xvar[j];
-  A.xvar[j] = xn;
-  if (xout) xout[j] = (float)xn;
+  xout[j] = (float)A.xs[j];
 }
 
 // ---- host driver ---------------------------------------------------------------
 template <int M>
 static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const double* gval_dev, int itn,
-                   double move, double a0, double av, double cv, double dv, float* xout, double epsimin) {
+                   double a0, double av, double cv, double dv, float* xout, double epsimin) {
   const unsigned nb = red_blocks_for_mma(A.n);
   auto read = [&]() -> int {
     if (hipMemcpyAsync(Sh, S, sizeof(MmaSmall), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
     return Sh->fault ? -2 : 0;
   };
-  hipLaunchKernelGGL(k_mma_prep<M>, dim3(nb), dim3(kBlock), 0, s, A, itn, move);
+  hipLaunchKernelGGL(k_mma_prep<M>, dim3(nb), dim3(kBlock), 0, s, A, itn);
   hipLaunchKernelGGL(k_mma_prep_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, gval_dev, (int)nb, a0, av, cv, dv);
   int newton = 0;
   double epsi = 1.0;
   while (epsi > epsimin) {
-    hipLaunchKernelGGL(k_mma_point<M>, dim3(nb), dim3(kBlock), 0, s, A, S, 0);
-    hipLaunchKernelGGL(k_mma_point_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb, 0);
+    // residual and Newton system at the start point of this epsi level
+    hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb), dim3(kBlock), 0, s, A, S);
+    hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
     int rc = read();
     if (rc) return rc;
     int ittt = 0;
     while (Sh->residumax > 0.9 * epsi && ittt < 200) {
       ittt++;
       newton++;
-      hipLaunchKernelGGL(k_mma_newton<M>, dim3(nb), dim3(kBlock), 0, s, A, S);
-      hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
+      hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(1), 0, s, S);
       hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb), dim3(kBlock), 0, s, A, S);
       hipLaunchKernelGGL(k_mma_dir_final, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
       do {
-        hipLaunchKernelGGL(k_mma_point<M>, dim3(nb), dim3(kBlock), 0, s, A, S, 1);
-        hipLaunchKernelGGL(k_mma_point_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb, 1);
+        hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb), dim3(kBlock), 0, s, A, S);
+        hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
         rc = read();
         if (rc) return rc;
       } while (!Sh->accept);
@@ -538,16 +601,16 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
 }
 
 int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
-               int itn, double move, double a0, double av, double cv, double dv, float* xout) {
+               int itn, double a0, double av, double cv, double dv, float* xout) {
   MmaSmall* S = (MmaSmall*)small_dev;
   MmaSmall* Sh = (MmaSmall*)small_host;
   MmaArrays B = A;  // the driver swaps the subsolver buffers locally
   int rc;
   switch (m) {
-    case 1: rc = mma_run<1>(s, B, S, Sh, gval_dev, itn, move, a0, av, cv, dv, xout, 1e-7); break;
-    case 2: rc = mma_run<2>(s, B, S, Sh, gval_dev, itn, move, a0, av, cv, dv, xout, 1e-7); break;
-    case 3: rc = mma_run<3>(s, B, S, Sh, gval_dev, itn, move, a0, av, cv, dv, xout, 1e-7); break;
-    case 4: rc = mma_run<4>(s, B, S, Sh, gval_dev, itn, move, a0, av, cv, dv, xout, 1e-7); break;
+    case 1: rc = mma_run<1>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
+    case 2: rc = mma_run<2>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
+    case 3: rc = mma_run<3>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
+    case 4: rc = mma_run<4>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
     default: rc = -3;
   }
   return rc;
