@@ -120,6 +120,19 @@ __device__ inline void final_sum_max(const double* partials, int nblocks, double
 // ---- per-element subproblem data (mmaOpt.cu:870-1000) ----------------------
 constexpr double kRaa0 = 1e-5, kAlbefa = 0.1, kAsyinit = 0.5, kAsyincr = 1.2, kAsydecr = 0.7;
 
+// 1/v for finite, normal v: the hardware estimate refined by two
+// Newton-Raphson steps, r += r (1 - v r) (error within an ulp). Every
+// divisor of the passes is such a value (distances to the asymptotes and
+// bounds, which the interior point keeps positive, and diagx > 0); an FP64
+// division costs about twice as many instructions and the passes carry some
+// ten of them per element.
+__device__ inline double rcp64(double v) {
+  double r = __builtin_amdgcn_rcp(v);
+  r = fma(fma(-v, r, 1.0), r, r);
+  r = fma(fma(-v, r, 1.0), r, r);
+  return r;
+}
+
 template <int M>
 struct MmaElem {
   double lo, up;      // asymptotes
@@ -148,7 +161,7 @@ __device__ inline void mma_coefficients(const MmaArrays& A, long long j, double 
   const double dxm = xmx - xmn;
   e.al = fmax(fmax(e.lo + kAlbefa * (x - e.lo), x - A.move * dxm), xmn);
   e.be = fmin(fmin(e.up - kAlbefa * (e.up - x), x + A.move * dxm), xmx);
-  const double xmamiinv = 1.0 / fmax(dxm, 1e-5);
+  const double xmamiinv = rcp64(fmax(dxm, 1e-5));
   const double ux1 = e.up - x, xl1 = x - e.lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
   const double reg = kRaa0 * xmamiinv;
   const double g0 = (double)A.df0dx[j];
@@ -176,30 +189,34 @@ __device__ inline void mma_load(const MmaArrays& A, long long j, MmaElem<M>& e) 
 }
 
 // Newton-step terms of one element at the point (x, xsi, eta) for the
-// multipliers lam (mmaOpt.cu:540-600, :680-700).
+// multipliers lam (mmaOpt.cu:540-600, :680-700), with the reciprocals of
+// the distances it divides by.
 template <int M>
 struct MmaNewtonElem {
-  double ux1, xl1, dpsidx, delx, diagx, GG[M];
+  double rux1, rxl1, rxa, rbx;  // 1/(upp - x), 1/(x - low), 1/(x - alfa), 1/(beta - x)
+  double dpsidx, delx, rdiagx, GG[M];
 };
 
 template <int M>
 __device__ inline void mma_newton_elem(const MmaElem<M>& e, double x, double xsi, double eta, const double (&lam)[M],
                                        double epsi, MmaNewtonElem<M>& q) {
 #pragma clang fp contract(off)
-  const double xa = x - e.al, bx = e.be - x;
-  const double ux1 = e.up - x, xl1 = x - e.lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
+  q.rxa = rcp64(x - e.al);
+  q.rbx = rcp64(e.be - x);
+  q.rux1 = rcp64(e.up - x);
+  q.rxl1 = rcp64(x - e.lo);
+  const double rux2 = q.rux1 * q.rux1, rxl2 = q.rxl1 * q.rxl1;
   double plam = e.p0, qlam = e.q0;
 #pragma unroll
   for (int i = 0; i < M; i++) {
     plam += lam[i] * e.P[i];
     qlam += lam[i] * e.Q[i];
-    q.GG[i] = e.P[i] / ux2 - e.Q[i] / xl2;
+    q.GG[i] = e.P[i] * rux2 - e.Q[i] * rxl2;
   }
-  q.ux1 = ux1;
-  q.xl1 = xl1;
-  q.dpsidx = plam / ux2 - qlam / xl2;
-  q.delx = q.dpsidx - epsi / xa + epsi / bx;
-  q.diagx = 2.0 * (plam / (ux2 * ux1) + qlam / (xl2 * xl1)) + xsi / xa + eta / bx;
+  q.dpsidx = plam * rux2 - qlam * rxl2;
+  q.delx = q.dpsidx - epsi * q.rxa + epsi * q.rbx;
+  const double diagx = 2.0 * (plam * (rux2 * q.rux1) + qlam * (rxl2 * q.rxl1)) + xsi * q.rxa + eta * q.rbx;
+  q.rdiagx = rcp64(diagx);
 }
 
 // dx = -(delx + GG^T dlam) / diagx (mmaOpt.cu:680-690)
@@ -209,7 +226,7 @@ __device__ inline double mma_dx(const MmaNewtonElem<M>& q, const double (&dlam)[
   double gd = 0.0;
 #pragma unroll
   for (int i = 0; i < M; i++) gd += q.GG[i] * dlam[i];
-  return -q.delx / q.diagx - gd / q.diagx;
+  return -(q.delx + gd) * q.rdiagx;
 }
 
 // ---- mmasub: asymptotes, right-hand sides, subsolver start ----------------------
@@ -339,9 +356,8 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
       MmaNewtonElem<M> qc;
       mma_newton_elem<M>(e, x, xsi, eta, lamc, epsi, qc);
       const double dx = mma_dx<M>(qc, dlam);
-      const double xa = x - e.al, bx = e.be - x;
-      const double dxsi = -xsi + epsi / xa - xsi * dx / xa;
-      const double deta = -eta + epsi / bx + eta * dx / bx;
+      const double dxsi = -xsi + epsi * qc.rxa - xsi * dx * qc.rxa;
+      const double deta = -eta + epsi * qc.rbx + eta * dx * qc.rbx;
       x += steg * dx;
       xsi += steg * dxsi;
       eta += steg * deta;
@@ -352,14 +368,14 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
     MmaNewtonElem<M> q;
     mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
 #pragma unroll
-    for (int i = 0; i < M; i++) sums[1 + i] += e.P[i] / q.ux1 + e.Q[i] / q.xl1;  // gvec
+    for (int i = 0; i < M; i++) sums[1 + i] += e.P[i] * q.rux1 + e.Q[i] * q.rxl1;  // gvec
     const double rex = q.dpsidx - xsi + eta;
     const double rexsi = xsi * (x - e.al) - epsi;
     const double reeta = eta * (e.be - x) - epsi;
     sums[0] += rex * rex + rexsi * rexsi + reeta * reeta;
     mx[0] = fmax(mx[0], fmax(fabs(rex), fmax(fabs(rexsi), fabs(reeta))));
     // the Newton system at this point (mmaOpt.cu:560-600)
-    const double inv = 1.0 / q.diagx;
+    const double inv = q.rdiagx;
     int t = 1 + 2 * M;
 #pragma unroll
     for (int i = 0; i < M; i++) {
@@ -523,10 +539,10 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir(MmaArrays A, const MmaSmall*
     MmaNewtonElem<M> q;
     mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
     const double dx = mma_dx<M>(q, dlam);
-    const double xa = x - e.al, bx = e.be - x;
-    const double dxsi = -xsi + epsi / xa - xsi * dx / xa;
-    const double deta = -eta + epsi / bx + eta * dx / bx;
-    mx[0] = fmax(mx[0], fmax(fmax(-1.01 * dxsi / xsi, -1.01 * deta / eta), fmax(-1.01 * dx / xa, 1.01 * dx / bx)));
+    const double dxsi = -xsi + epsi * q.rxa - xsi * dx * q.rxa;
+    const double deta = -eta + epsi * q.rbx + eta * dx * q.rbx;
+    // fraction to the boundary (mmaOpt.cu:700-716)
+    mx[0] = fmax(mx[0], fmax(fmax(-1.01 * dxsi * rcp64(xsi), -1.01 * deta * rcp64(eta)), fmax(-1.01 * dx * q.rxa, 1.01 * dx * q.rbx)));
   }
   block_sum_max<1, 1>(dummy, mx, A.partials);
 }
