@@ -38,13 +38,6 @@ __global__ void __launch_bounds__(kBlock) k_fine_apply_tile(Geom g, FineConsts f
   fine_apply_tile_body<BX, kBlock / BX>(g, fc, kappa, p, q, R, xcd_block(), gridDim.x);
 }
 
-template <typename XT>
-__global__ void __launch_bounds__(kBlock) k_fine_init_residual(Geom g, FineConsts fc, const float* __restrict__ kappa,
-                                                               const XT* __restrict__ x_, const float* __restrict__ xlo,
-                                                               float* __restrict__ rvec, Reduce R) {
-  fine_init_residual_body<XT>(g, fc, kappa, x_, xlo, rvec, R, xcd_block(), gridDim.x);
-}
-
 // grid (nblocks, 3): one load case per block row
 template <typename XT>
 __global__ void __launch_bounds__(kBlock) k_fine_init_residual_case(Geom g, FineConsts fc,
@@ -159,8 +152,8 @@ void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& f
 
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const double* x, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual<double>, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, x,
-                     (const float*)nullptr, r, R);
+  hipLaunchKernelGGL(k_fine_init_residual_case<double>, dim3(nblocks, OTM_NRHS), dim3(kBlock), 0, s, g, fc, kappa,
+                     x, (const float*)nullptr, r, R);
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 
