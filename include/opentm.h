@@ -182,6 +182,16 @@ OTM_API int otm_coarsest_solve(otm_ctx* ctx, const float* f, float* u, int where
  * transpose. Arrays are contiguous reso[0]*reso[1]*reso[2] floats. */
 OTM_API int otm_conv_filter(const int reso[3], int kind, double radius, int periodic, int adjoint,
                             const float* in, float* out, int where);
+/* otm_conv_filter for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops): no counterpart in the reference.
+ * in and out are device arrays of the current device that must not overlap;
+ * the launches go on `stream` (a hipStream_t; null = the default stream) and
+ * the call returns without waiting for them. The weight sums live in a
+ * static device array of 2^23 floats (32 MiB per device, kept until the
+ * process ends): larger cells get OTM_E_UNSUPPORTED. Every call on a device
+ * shares that array, so calls whose launches overlap on one GPU race. */
+OTM_API int otm_conv_filter_noalloc(const int reso[3], int kind, double radius, int periodic, int adjoint,
+                                    const float* in, float* out, void* stream);
 
 /* Design-density pipeline of runCustom (Framework.cu:163):
  * kappa = k1 + (k0-k1) * filter(rho)^3 with the Spline4 r=1.5 filter. */

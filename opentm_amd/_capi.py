@@ -157,6 +157,7 @@ SIGNATURES = {
     "otm_smooth_level": (_i, [_p, _i, _p, _p, _i, _i, _i]),
     "otm_coarsest_solve": (_i, [_p, _p, _p, _i]),
     "otm_conv_filter": (_i, [_ip, _i, _d, _i, _i, _p, _p, _i]),
+    "otm_conv_filter_noalloc": (_i, [_ip, _i, _d, _i, _i, _p, _p, _p]),
     "otm_density_params_init": (None, [_c.POINTER(OtmDensityParams)]),
     "otm_set_design": (_i, [_p, _p, _i]),
     "otm_get_design": (_i, [_p, _p, _i]),

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -1260,6 +1261,55 @@ OTM_API int otm_conv_filter(const int reso[3], int kind, double radius, int peri
     if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(float) * g.nn, kind_of(where, OTM_DEVICE));
     dfree(din); dfree(dout); dfree(dW); dfree(dtmp);
     hip_check(e, "conv filter");
+  });
+}
+
+// ---- the filter without allocation --------------------------------------------
+// Scratch of otm_conv_filter_noalloc: the per-node weight sums W of one
+// cell, at most kNoAllocNodes floats (32 MiB on every device the library
+// runs on, held until the process ends). In host code `scratch` names a
+// host-side stand-in: its device address comes from hipGetSymbolAddress,
+// once per device with that device current, and only that address is
+// handed to kernels.
+constexpr long long kNoAllocNodes = 1ll << 23;
+static __device__ __align__(256) float scratch[kNoAllocNodes];
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_dev[kNoAllocMaxDevices];
+
+static float* scratch_of_current_device() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kNoAllocMaxDevices) fail(OTM_E_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) fail(OTM_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_dev[dev] = (float*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
+OTM_API int otm_conv_filter_noalloc(const int reso[3], int kind, double radius, int periodic, int adjoint,
+                                    const float* in, float* out, void* stream) {
+  return guarded([&] {
+    if (!reso || !in || !out) fail(OTM_E_ARG, "null argument");
+    for (int a = 0; a < 3; a++)
+      if (reso[a] <= 0) fail(OTM_E_ARG, "reso must be positive");
+    const long long n = (long long)reso[0] * reso[1] * reso[2];
+    if (n > kNoAllocNodes)
+      fail(OTM_E_UNSUPPORTED, "cell exceeds the no-allocation filter's scratch of 8388608 nodes");
+    // the adjoint writes the forward output to out before it reads in again
+    if (in < out + n && out < in + n) fail(OTM_E_ARG, "in and out overlap");
+    float* W = scratch_of_current_device();
+    const Taps T = make_taps(kind, radius, periodic);
+    const Geom g{reso[0], reso[1], reso[2], n};
+    hipStream_t s = (hipStream_t)stream;
+    // the forward pass writes every node's weight sum before any read of W
+    launch_filter_fwd(s, g, T, in, out, W);
+    if (adjoint) launch_filter_adj(s, g, T, in, W, out);
+    hip_check(hipGetLastError(), "conv filter");
   });
 }
 
