@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "homog. solves/s + p50/p99 latency at κᴴ rel err ≤1e-5; stencil HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 FINE_KERNEL = "k_fine_smooth_brick<32>"
+GRAPH = True  # V-cycles replayed from captured HIP graphs (--no-graph: eager launches)
 
 
 def measured_traffic(kernel, reso):
@@ -165,7 +166,7 @@ def precision_run(n, rho0, refine, steps, warmup, rtol, label, profile_classes=N
     """The precision-form MMA loop (or, refine=False, the same loop on the
     plain FP32 path) on one cell."""
     from opentm_amd import precision as pf
-    loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=refine)
+    loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=refine, graph=GRAPH)
 
     total, lat, hist = timed_loop(loop.step, steps, warmup, loop.hom.synchronize, label, loop.history.clear)
     hist = list(hist)
@@ -189,6 +190,7 @@ def oc_loop(n, seed, steps, warmup, rtol, coarse_sweeps, profile_classes=None, v
     cfg = otm.HomoConfig(n, (1.0, 1e-3))
     cfg.femRelThres = rtol
     cfg.coarse_sweeps = coarse_sweeps
+    cfg.graph_vcycle = 1 if GRAPH else 0
     if min_coarse is not None:
         cfg.min_coarse = min_coarse
     cfg.verbose = 1 if verbose else 0
@@ -242,7 +244,10 @@ def main():
     ap.add_argument("--max-active", type=int, default=96, help="serve workload: requests on the GPU at once")
     ap.add_argument("--cells", type=int, default=96, help="batch workload: number of cells")
     ap.add_argument("--oracle", action="store_true", help="precision workload: FP64 oracle errors (slow)")
+    ap.add_argument("--no-graph", action="store_true", help="launch every V-cycle kernel eagerly (no HIP graph replay)")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = not args.no_graph
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path is built)"}))
         sys.exit(2)

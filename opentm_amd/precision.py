@@ -37,9 +37,11 @@ def objective(C, vol_mean, c_max=0.05, vol=0.25):
 class PrecisionLoop:
     """One cell and its MMA optimiser; `step()` runs one design iteration."""
 
-    def __init__(self, reso, rho0, k1=1e-3, rtol=1e-8, refine=True, fp16=True, c_max=0.05, vol=0.25, device=0):
+    def __init__(self, reso, rho0, k1=1e-3, rtol=1e-8, refine=True, fp16=True, c_max=0.05, vol=0.25, device=0,
+                 graph=True):
         cfg = HomoConfig(reso, (1.0, k1))
         cfg.femRelThres = rtol
+        cfg.graph_vcycle = 1 if graph else 0
         cfg.fp64_refine = 1 if refine else 0
         cfg.fp16_stencils = 1 if fp16 else 0
         cfg.device = device
