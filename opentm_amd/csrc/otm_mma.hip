@@ -141,38 +141,56 @@ struct MmaElem {
   double P[M], Q[M];  // constraint approximations
 };
 
-__device__ inline void mma_bounds(const MmaArrays& A, long long j, double& xmn, double& xmx) {
+// The values a pass reads for one element.
+template <int M>
+struct MmaRaw {
+  float x, g0, g[M];     // design and gradients (FP32 as the caller holds them)
+  double xmn, xmx;       // bounds
+  double lo, up;         // asymptotes
+  double xs, xsi, eta;   // interior point
+};
+
+template <int M>
+__device__ inline void mma_fetch(const MmaArrays& A, long long j, MmaRaw<M>& r) {
+  r.x = A.xvar[j];
+  r.lo = A.low[j];
+  r.up = A.upp[j];
+  r.g0 = A.df0dx[j];
+#pragma unroll
+  for (int i = 0; i < M; i++) r.g[i] = A.dgdx[i * A.n + j];
   if (A.xmin) {
-    xmn = (double)A.xmin[j];
-    xmx = (double)A.xmax[j];
+    r.xmn = (double)A.xmin[j];
+    r.xmx = (double)A.xmax[j];
   } else {
-    xmn = A.xmin_u;
-    xmx = A.xmax_u;
+    r.xmn = A.xmin_u;
+    r.xmx = A.xmax_u;
   }
+  r.xs = A.xs[j];
+  r.xsi = A.xsi[j];
+  r.eta = A.eta[j];
 }
 
-// alfa, beta, p0, q0, P, Q of element j at the design x, given e.lo/e.up.
-// Every pass calls this one function with contraction off, so every pass
-// works with the same coefficient bits.
+// alfa, beta, p0, q0, P, Q at the design x, given e.lo/e.up. Every pass
+// calls this one function with contraction off, so every pass works with
+// the same coefficient bits.
 template <int M>
-__device__ inline void mma_coefficients(const MmaArrays& A, long long j, double x, double xmn, double xmx,
+__device__ inline void mma_coefficients(double move, double x, double xmn, double xmx, double g0, const float (&g)[M],
                                         MmaElem<M>& e) {
 #pragma clang fp contract(off)
   const double dxm = xmx - xmn;
-  e.al = fmax(fmax(e.lo + kAlbefa * (x - e.lo), x - A.move * dxm), xmn);
-  e.be = fmin(fmin(e.up - kAlbefa * (e.up - x), x + A.move * dxm), xmx);
+  e.al = fmax(fmax(e.lo + kAlbefa * (x - e.lo), x - move * dxm), xmn);
+  e.be = fmin(fmin(e.up - kAlbefa * (e.up - x), x + move * dxm), xmx);
   const double xmamiinv = rcp64(fmax(dxm, 1e-5));
   const double ux1 = e.up - x, xl1 = x - e.lo, ux2 = ux1 * ux1, xl2 = xl1 * xl1;
   const double reg = kRaa0 * xmamiinv;
-  const double g0 = (double)A.df0dx[j];
   const double dp = fmax(g0, 0.0), dq = fmax(-g0, 0.0);
   const double pq0 = 1e-3 * (dp + dq) + reg;
   e.p0 = (dp + pq0) * ux2;
   e.q0 = (dq + pq0) * xl2;
 #pragma unroll
   for (int i = 0; i < M; i++) {
-    const double g = (double)A.dgdx[i * A.n + j];
-    const double pp = fmax(g, 0.0), qq = fmax(-g, 0.0);
+    const double gi = (double)g[i];
+    const double pp = fmax(gi, 0.0), qq = fmax(-gi, 0.0);
     const double pq = 1e-3 * (pp + qq) + reg;
     e.P[i] = (pp + pq) * ux2;
     e.Q[i] = (qq + pq) * xl2;
@@ -180,12 +198,29 @@ __device__ inline void mma_coefficients(const MmaArrays& A, long long j, double 
 }
 
 template <int M>
-__device__ inline void mma_load(const MmaArrays& A, long long j, MmaElem<M>& e) {
-  double xmn, xmx;
-  mma_bounds(A, j, xmn, xmx);
-  e.lo = A.low[j];
-  e.up = A.upp[j];
-  mma_coefficients<M>(A, j, (double)A.xvar[j], xmn, xmx, e);
+__device__ inline void mma_elem(const MmaArrays& A, const MmaRaw<M>& r, MmaElem<M>& e) {
+  e.lo = r.lo;
+  e.up = r.up;
+  mma_coefficients<M>(A.move, (double)r.x, r.xmn, r.xmx, (double)r.g0, r.g, e);
+}
+
+// f(j, raw) over this block's elements in the order of a plain stride loop,
+// with the loads of the next element issued before the arithmetic of the
+// current one: a pass is bound by the bytes in flight per CU, and a thread
+// then keeps two elements' loads outstanding.
+template <int M, typename F>
+__device__ inline void mma_for_each(const MmaArrays& A, F&& f) {
+  const BlockRange br = block_range(A.n);
+  const long long bs = blockDim.x;
+  for (long long j = br.b + threadIdx.x; j < br.e; j += 2 * bs) {
+    const long long j1 = j + bs;
+    const bool two = j1 < br.e;
+    MmaRaw<M> r0, r1;
+    mma_fetch<M>(A, j, r0);
+    if (two) mma_fetch<M>(A, j1, r1);
+    f(j, r0);
+    if (two) f(j1, r1);
+  }
 }
 
 // Newton-step terms of one element at the point (x, xsi, eta) for the
@@ -239,8 +274,8 @@ __global__ void __launch_bounds__(kBlock) k_mma_prep(MmaArrays A, int itn) {
   const BlockRange br = block_range(A.n);
   for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
     const double x = (double)A.xvar[j];
-    double xmn, xmx;
-    mma_bounds(A, j, xmn, xmx);
+    const double xmn = A.xmin ? (double)A.xmin[j] : A.xmin_u;
+    const double xmx = A.xmax ? (double)A.xmax[j] : A.xmax_u;
     const double dxm = xmx - xmn;
     MmaElem<M> e;
     if (itn <= 2) {
@@ -259,7 +294,10 @@ __global__ void __launch_bounds__(kBlock) k_mma_prep(MmaArrays A, int itn) {
     }
     A.low[j] = e.lo;
     A.upp[j] = e.up;
-    mma_coefficients<M>(A, j, x, xmn, xmx, e);
+    float g[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) g[i] = A.dgdx[i * A.n + j];
+    mma_coefficients<M>(A.move, x, xmn, xmx, (double)A.df0dx[j], g, e);
     const double ux1 = e.up - x, xl1 = x - e.lo;
 #pragma unroll
     for (int i = 0; i < M; i++) bs[i] += e.P[i] / ux1 + e.Q[i] / xl1;
@@ -347,11 +385,10 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
 #pragma unroll
   for (int i = 0; i < NS; i++) sums[i] = 0.0;
   double mx[1] = {0.0};
-  const BlockRange br = block_range(A.n);
-  for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
+  mma_for_each<M>(A, [&](long long j, const MmaRaw<M>& r) {
     MmaElem<M> e;
-    mma_load<M>(A, j, e);
-    double x = A.xs[j], xsi = A.xsi[j], eta = A.eta[j];
+    mma_elem<M>(A, r, e);
+    double x = r.xs, xsi = r.xsi, eta = r.eta;
     if (TRIAL) {
       MmaNewtonElem<M> qc;
       mma_newton_elem<M>(e, x, xsi, eta, lamc, epsi, qc);
@@ -383,7 +420,7 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
 #pragma unroll
       for (int k = i; k < M; k++) sums[t++] += q.GG[i] * q.GG[k] * inv;
     }
-  }
+  });
   block_sum_max<NS, 1>(sums, mx, A.partials);
 }
 
@@ -531,19 +568,19 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir(MmaArrays A, const MmaSmall*
   const double epsi = S->epsi;
   double dummy[1] = {0.0};
   double mx[1] = {-1e300};
-  const BlockRange br = block_range(A.n);
-  for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
+  mma_for_each<M>(A, [&](long long, const MmaRaw<M>& r) {
     MmaElem<M> e;
-    mma_load<M>(A, j, e);
-    const double x = A.xs[j], xsi = A.xsi[j], eta = A.eta[j];
+    mma_elem<M>(A, r, e);
+    const double x = r.xs, xsi = r.xsi, eta = r.eta;
     MmaNewtonElem<M> q;
     mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
     const double dx = mma_dx<M>(q, dlam);
     const double dxsi = -xsi + epsi * q.rxa - xsi * dx * q.rxa;
     const double deta = -eta + epsi * q.rbx + eta * dx * q.rbx;
     // fraction to the boundary (mmaOpt.cu:700-716)
-    mx[0] = fmax(mx[0], fmax(fmax(-1.01 * dxsi * rcp64(xsi), -1.01 * deta * rcp64(eta)), fmax(-1.01 * dx * q.rxa, 1.01 * dx * q.rbx)));
-  }
+    mx[0] = fmax(mx[0], fmax(fmax(-1.01 * dxsi * rcp64(xsi), -1.01 * deta * rcp64(eta)),
+                             fmax(-1.01 * dx * q.rxa, 1.01 * dx * q.rbx)));
+  });
   block_sum_max<1, 1>(dummy, mx, A.partials);
 }
 
