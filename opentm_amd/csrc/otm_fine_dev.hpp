@@ -856,6 +856,72 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
   __syncthreads();
   const float c = fc.c;
   constexpr int DY = HX, DZ = HX * HY;
+  constexpr int LO = DY + DZ;  // reads start at the node's (y-1, z-1) row
+  // The neighbours at x-1 and x+1 of a node are summed in the lanes that
+  // hold them (the same tile row, the next lower/higher lane), with the
+  // conductivities of the elements on their side, and moved over by DPP:
+  // with G_e(x') = u(x', y+sy, z) + u(x', y, z+sz) + u(x', y+sy, z+sz) for
+  // the element quadrant e = (ay, az) (sy = 2ay - 1, sz = 2az - 1),
+  //   sum_j w_j u_j = sum_e k(x-1,e) G_e(x-1) + k(x,e) G_e(x+1)
+  //                 + (k(x-1,e) + k(x,e)) u(x, y+sy, z+sz),
+  // so lane x' forms T1 = sum_e k(x',e) G_e(x') (needed by x'+1) and
+  // T0 = sum_e k(x'-1,e) G_e(x') (needed by x'-1): a node reads only the
+  // 8 rows of its own column from LDS, not 20 neighbours. The lanes at the
+  // tile's x edges would take T from the next tile row; their sends are
+  // zeroed and the frozen halo columns' terms are added to f once.
+  auto from_left = [](float v) {  // value of lane x - 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), BX == 16 ? 0x111 : 0x138, 0xF, 0xF, true));
+  };
+  auto from_right = [](float v) {  // value of lane x + 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), BX == 16 ? 0x101 : 0x130, 0xF, 0xF, true));
+  };
+  static_assert(BX == 16 || BX == 32, "tile rows are one DPP row (16) or half a wave (32)");
+  const bool send_left = lx != BX - 1;  // T1 goes to lane x + 1 in the same tile row
+  const bool send_right = lx != 0;      // T0 goes to lane x - 1
+  float fe[4][H][OTM_NRHS];
+#pragma unroll
+  for (int col = 0; col < 4; col++)
+#pragma unroll
+    for (int h = 0; h < H; h++)
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) fe[col][h][k] = fv[col][h][k];
+  if (!zero_in) {
+    // halo column x0 - 1 seen by the tile's first column: its T1; halo
+    // column x0 + BX seen by the last column: its T0 (frozen values)
+    __shared__ float hterm[2][BY * BZ][OTM_NRHS];
+    if (t < 2 * BY * BZ) {
+      const int side = t / (BY * BZ), r = t % (BY * BZ);
+      const int ly = r % BY, lz = r / BY;
+      const int hx = side ? HX - 1 : 0, kx = side ? BX : 0;
+      const int c0 = hx + DY * (ly + 1) + DZ * (lz + 1);
+#pragma unroll
+      for (int k = 0; k < OTM_NRHS; k++) {
+        const float* U = &su[k][0];
+        float T = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int ay = q & 1, az = q >> 1, sy = 2 * ay - 1, sz = 2 * az - 1;
+          const float G = U[c0 + sy * DY] + U[c0 + sz * DZ] + U[c0 + sy * DY + sz * DZ];
+          T += sk[kx + KX * ((ly + ay) + KY * (lz + az))] * G;
+        }
+        hterm[side][r][k] = T;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int col = 0; col < 4; col++)
+#pragma unroll
+      for (int h = 0; h < H; h++) {
+        const int rr = ty + TR * h;
+        const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
+        const int r = ly + BY * lz;
+#pragma unroll
+        for (int k = 0; k < OTM_NRHS; k++) {
+          if (lx == 0) fe[col][h][k] += c * hterm[0][r][k];
+          if (lx == BX - 1) fe[col][h][k] += c * hterm[1][r][k];
+        }
+      }
+  }
   for (int s = 0; s < sweeps; s++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -864,47 +930,40 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
       for (int h = 0; h < H; h++) {
         const int rr = ty + TR * h;
         const int ly = (col & 1) + 2 * (rr % (BY / 2)), lz = (col >> 1) + 2 * (rr / (BY / 2));
-        // kappa of the 8 elements around the node, e = ax + 2ay + 4az: element
-        // (x-1+ax, y-1+ay, z-1+az) at sk(lx+ax, ly+ay, lz+az)
+        // kappa of the elements (x-1+ax, y-1+ay, z-1+az): K0 for ax = 0, K1 for ax = 1
         const int kb = lx + KX * (ly + KY * lz);
-        float ke[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) ke[e] = sk[kb + (e & 1) + KX * (((e >> 1) & 1) + KY * (e >> 2))];
-        float wxy[4], wxz[4], wyz[4];
+        float K0[4], K1[4], W[4], K0s[4], K1s[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const int a0 = q & 1, a1 = q >> 1;
-          wxy[q] = ke[a0 + 2 * a1] + ke[a0 + 2 * a1 + 4];  // neighbour (+-1, +-1, 0)
-          wxz[q] = ke[a0 + 4 * a1] + ke[a0 + 2 + 4 * a1];  // neighbour (+-1, 0, +-1)
-          wyz[q] = ke[2 * a0 + 4 * a1] + ke[1 + 2 * a0 + 4 * a1];  // neighbour (0, +-1, +-1)
+          const int o = KX * ((q & 1) + KY * (q >> 1));
+          K0[q] = sk[kb + o];
+          K1[q] = sk[kb + 1 + o];
+          W[q] = K0[q] + K1[q];
+          K1s[q] = send_left ? K1[q] : 0.f;
+          K0s[q] = send_right ? K0[q] : 0.f;
         }
-        const float ks = (wxy[0] + wxy[1]) + (wxy[2] + wxy[3]);
+        const float ks = (W[0] + W[1]) + (W[2] + W[3]);
         const float inv = __builtin_amdgcn_rcpf(4.f * c * ks);
-        // The neighbours are addressed from the lowest one (cb) so that
-        // every offset, load cases included, is a non-negative constant
-        // that the LDS instructions take as an immediate; the empty asm
-        // keeps the compiler from folding cb back into c0 and negative
-        // offsets (each of which cost an address add per colour step).
-        constexpr int LO = 1 + DY + DZ;
+        // rows of the node's own column, addressed from the lowest one so
+        // that every offset (load cases included) is an LDS immediate
         int cb = (lx + 1) + DY * (ly + 1) + DZ * (lz + 1) - LO;
         asm volatile("" : "+v"(cb));
 #pragma unroll
         for (int k = 0; k < OTM_NRHS; k++) {
           const float* U = &su[k][0] + cb;
-          float acc = 0.f;
+          const float ym = U[LO - DY], yp = U[LO + DY], zm = U[LO - DZ], zp = U[LO + DZ];
+          const float D[4] = {U[0], U[2 * DY], U[2 * DZ], U[2 * DY + 2 * DZ]};  // (y-+1, z-+1)
+          const float G[4] = {ym + zm + D[0], yp + zm + D[1], ym + zp + D[2], yp + zp + D[3]};
+          float T1 = K1s[0] * G[0], T0 = K0s[0] * G[0], acc = W[0] * D[0];
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int s0 = 2 * (q & 1) - 1, s1 = 2 * (q >> 1) - 1;
-            acc += wxy[q] * U[LO + s0 + DY * s1];
-            acc += wxz[q] * U[LO + s0 + DZ * s1];
-            acc += wyz[q] * U[LO + DY * s0 + DZ * s1];
+          for (int q = 1; q < 4; q++) {
+            T1 += K1s[q] * G[q];
+            T0 += K0s[q] * G[q];
+            acc += W[q] * D[q];
           }
-#pragma unroll
-          for (int e = 0; e < 8; e++) {
-            const int sx = 2 * (e & 1) - 1, sy = 2 * ((e >> 1) & 1) - 1, sz = 2 * (e >> 2) - 1;
-            acc += ke[e] * U[LO + sx + DY * sy + DZ * sz];
-          }
-          (&su[k][0] + cb)[LO] = (fv[col][h][k] + c * acc) * inv;
+          acc += from_left(T1);
+          acc += from_right(T0);
+          (&su[k][0] + cb)[LO] = (fe[col][h][k] + c * acc) * inv;
         }
       }
       __syncthreads();
