@@ -39,6 +39,7 @@ METRIC = "homog. solves/s + p50/p99 latency at κᴴ rel err ≤1e-5; stencil HB
 HBM_PEAK_GBS = 8000.0
 FINE_KERNEL = "k_fine_smooth_brick<32>"
 GRAPH = True  # V-cycles replayed from captured HIP graphs (--no-graph: eager launches)
+AUTOTUNE = True  # the tuner picks the fine tile shape and coarse sweeps first (--no-autotune: defaults)
 
 
 def measured_traffic(kernel, reso):
@@ -167,6 +168,17 @@ def precision_run(n, rho0, refine, steps, warmup, rtol, label, profile_classes=N
     plain FP32 path) on one cell."""
     from opentm_amd import precision as pf
     loop = pf.PrecisionLoop(n, rho0, rtol=rtol, refine=refine, fp16=refine, graph=GRAPH)
+    tuned = None
+    if AUTOTUNE:
+        # after one design iteration the context holds the next design and
+        # the previous solution: the tuner times every (fine tile shape,
+        # coarse sweeps) candidate on that design from that warm start, as
+        # the loop's next solve would run
+        loop.step()
+        loop.hom.apply_design()
+        rows, chosen = loop.hom.autotune(warm=True)
+        tuned = {"chosen": rows[chosen], "candidates": rows}
+        print(f"[bench] {label} autotune chose {rows[chosen]}", file=sys.stderr, flush=True)
 
     total, lat, hist = timed_loop(loop.step, steps, warmup, loop.hom.synchronize, label, loop.history.clear)
     hist = list(hist)
@@ -176,6 +188,12 @@ def precision_run(n, rho0, refine, steps, warmup, rtol, label, profile_classes=N
            "pcg_iters_mean": float(np.mean([max(h["pcg_iters"]) for h in hist])),
            "relres_max": float(max(max(h["relres"]) for h in hist)), "rtol": rtol,
            "C_final": hist[-1]["C"].reshape(-1).tolist(), "g_final": hist[-1]["g"].tolist()}
+    if tuned:
+        res["autotune"] = tuned
+        res["coarse_sweeps"] = tuned["chosen"]["coarse_sweeps"]
+    else:
+        res["coarse_sweeps"] = 2  # the library default: as many as on the fine level
+    res["fine_tile"] = loop.hom.fine_tile()[1]
     loop.mma.close()
     loop.hom.close()
     for b in [loop.x, loop.df] + loop.dg:
@@ -245,9 +263,13 @@ def main():
     ap.add_argument("--cells", type=int, default=96, help="batch workload: number of cells")
     ap.add_argument("--oracle", action="store_true", help="precision workload: FP64 oracle errors (slow)")
     ap.add_argument("--no-graph", action="store_true", help="launch every V-cycle kernel eagerly (no HIP graph replay)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="precision workload: keep the default fine tile shape and coarse sweeps (no autotuner run "
+                         "on the first design before the timed loop)")
     args = ap.parse_args()
-    global GRAPH
+    global GRAPH, AUTOTUNE
     GRAPH = not args.no_graph
+    AUTOTUNE = not args.no_autotune
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path is built)"}))
         sys.exit(2)
@@ -287,12 +309,16 @@ def precision_main(args):
                                f"one step = one homogenization (3 load cases, FP64 refinement to 1e-8) + "
                                f"3 sensitivities + MMA update",
                    "reso": n, "elements": n ** 3, "seed": args.seed, "rhs_per_solve": 3,
-                   "mg": "V(2,2) fine, 1 sweep coarse, 8-colour GS in LDS bricks, Galerkin coarse levels",
+                   "mg": f"V(2,2) fine in {prec['fine_tile'][0]}x{prec['fine_tile'][1]}x{prec['fine_tile'][2]} LDS "
+                         f"tiles, {prec['coarse_sweeps']} sweep(s) on the coarse levels, 8-colour GS in LDS bricks, "
+                         f"Galerkin coarse levels",
                    "parallelism": "single GPU"},
         "p50_ms": prec["p50_ms"], "p99_ms": prec["p99_ms"], "pcg_iters_mean": prec["pcg_iters_mean"],
         "relres_max": prec["relres_max"], "C_final": prec["C_final"], "g_final": prec["g_final"],
         "roofline": roofline_from(prof, n), "kernel_classes": class_rows(prof),
     }
+    if "autotune" in prec:
+        out["autotune"] = prec["autotune"]
     if not args.no_extras:
         plain, _ = precision_run(n, rho0, False, args.steps, args.warmup, args.rtol, f"plain{n}")
         out["plain_fp32"] = {k: plain[k] for k in ("solves_per_s", "ms_per_step", "p50_ms", "p99_ms",

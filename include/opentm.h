@@ -69,6 +69,9 @@ typedef struct otm_config {
                                the brick preconditioner loses positivity) */
   int graph_vcycle;         /* 1 (default): each V-cycle's fixed launch sequence is captured
                                once as a HIP graph and replayed (0: launched one by one) */
+  int fine_tile;            /* fine brick smoother tile shape: -1 (default) 32x4x8, or 16x4x8
+                               where x is not a multiple of 32; 0..4 a variant of
+                               otm_fine_tile_shape (each is its own block Gauss-Seidel) */
 } otm_config;
 
 /* cfg::HomoConfig::init (cmdline.cpp:4-35) defaults where they apply;
@@ -105,6 +108,33 @@ OTM_API int otm_update(otm_ctx* ctx, const float* kappa, int pitch, int where);
  * otm_solve_stats are still written (the reference reports "Failed to
  * converge" and carries on, MG.cpp:101-118). OTM_E_NUMERIC: NaN/Inf. */
 OTM_API int otm_heat_matrix(otm_ctx* ctx, double C[9]);
+
+/* Fine smoother tile shapes: variant 0..4 -> shape = {bx, by, bz, threads};
+ * otm_fine_tile gives the context's variant (-1: exact colour sweeps). */
+OTM_API int otm_fine_tile_shape(int variant, int shape[4]);
+OTM_API int otm_fine_tile(const otm_ctx* ctx, int* variant);
+
+/* Autotuner: no counterpart in the reference, whose launch shapes are fixed
+ * (culib/lib.cuh:71-74). Times one full solve of the current design (after
+ * otm_update) for each fine tile shape the cell admits and coarse sweep
+ * counts 1 and 2, each from the same start (warm = 0: zero; warm = 1: the
+ * current iterate, which after otm_apply_design of a design loop's next
+ * design is the warm start its solve would take), keeps the fastest that
+ * reaches the tolerance without falling back to exact sweeps in the context
+ * (the iterate is restored, otm_heat_matrix must run again), and reports
+ * every candidate. The choice can be pinned with otm_config.fine_tile and
+ * coarse_sweeps. OTM_E_UNSUPPORTED when the fine level has no brick smoother. */
+#define OTM_TUNE_MAX 16
+typedef struct otm_tune_report {
+  int n;                          /* candidates timed */
+  int fine_tile[OTM_TUNE_MAX];    /* fine tile variant of candidate i */
+  int coarse_sweeps[OTM_TUNE_MAX];
+  double ms[OTM_TUNE_MAX];        /* solve time (the second of two runs) */
+  int iters[OTM_TUNE_MAX];        /* PCG iterations (max over the load cases) */
+  int ok[OTM_TUNE_MAX];           /* reached the tolerance with the brick preconditioner */
+  int chosen;                     /* index of the kept candidate */
+} otm_tune_report;
+OTM_API int otm_autotune(otm_ctx* ctx, int warm, otm_tune_report* report);
 
 /* PCG iterations and achieved relative residual of the last solve. */
 OTM_API int otm_solve_stats(const otm_ctx* ctx, int iters[3], double relres[3]);
@@ -209,6 +239,9 @@ OTM_API int otm_get_design(otm_ctx* ctx, float* rho, int where);
 
 /* objective.eval() path for the density expression and heat_tensor_t
  * (homoExpression.h:89-105): filter + SIMP + update + heatMatrix. */
+/* The first half of otm_eval: filter, SIMP map and operator rebuild for
+ * the current design, no solve (the iterate stays the warm start). */
+OTM_API int otm_apply_design(otm_ctx* ctx, const otm_density_params* p);
 OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]);
 
 /* objective.backward() (homoExpression.h:106-120): sensitivity with dC,

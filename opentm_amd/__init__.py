@@ -123,6 +123,7 @@ class HomoConfig:
         self.fp16_stencils = 0
         self.exact_smoother = 0
         self.graph_vcycle = 1
+        self.fine_tile = -1
         self.device = 0
         self.verbose = 0
 
@@ -142,6 +143,7 @@ class HomoConfig:
         c.fp16_stencils = self.fp16_stencils
         c.exact_smoother = self.exact_smoother
         c.graph_vcycle = self.graph_vcycle
+        c.fine_tile = self.fine_tile
         c.device = self.device
         c.verbose = self.verbose
         return c
@@ -230,6 +232,25 @@ class Homogenization_H:
         rr = (ctypes.c_double * 3)()
         _capi.check(_capi.load().otm_solve_stats(self._ctx, it, rr), "otm_solve_stats")
         return list(it), list(rr)
+
+    def fine_tile(self):
+        """The fine smoother's tile variant and shape (bx, by, bz, threads),
+        or (-1, None) when the fine level uses exact colour sweeps."""
+        v = ctypes.c_int()
+        _capi.check(_capi.load().otm_fine_tile(self._ctx, ctypes.byref(v)), "otm_fine_tile")
+        return v.value, (fine_tile_shape(v.value) if v.value >= 0 else None)
+
+    def autotune(self, warm=False):
+        """Times one solve of the current design per candidate (fine tile
+        shape x coarse sweeps 1, 2), from a zero start or (warm) from the
+        current iterate, and keeps the fastest; returns the candidates as
+        dicts and the index of the chosen one."""
+        rep = _capi.OtmTuneReport()
+        _capi.check(_capi.load().otm_autotune(self._ctx, 1 if warm else 0, ctypes.byref(rep)), "otm_autotune")
+        rows = [{"fine_tile": rep.fine_tile[i], "shape": fine_tile_shape(rep.fine_tile[i]),
+                 "coarse_sweeps": rep.coarse_sweeps[i], "ms": rep.ms[i], "iters": rep.iters[i], "ok": bool(rep.ok[i])}
+                for i in range(rep.n)]
+        return rows, rep.chosen
 
     def Sensitivity(self, dC, sens=None, pitchT=-1, lexiOrder=True):
         """Homogenization_H::Sensitivity with lexiOrder=true. Returns the
@@ -376,6 +397,13 @@ class Homogenization_H:
         C = (ctypes.c_double * 9)()
         _capi.check(_capi.load().otm_eval(self._ctx, ctypes.byref(p), C), "otm_eval")
         return np.array(C[:], dtype=np.float64).reshape(3, 3)
+
+    def apply_design(self, params=None):
+        """The density pipeline of eval (filter, SIMP, operator rebuild)
+        without the solve."""
+        params = params or DensityParams()
+        p = params._c()
+        _capi.check(_capi.load().otm_apply_design(self._ctx, ctypes.byref(p)), "otm_apply_design")
 
     def backward(self, dC, want=True, out=None):
         """d(sum dC_ij C_ij)/d rho through the SIMP map and the filter:
@@ -556,6 +584,13 @@ class MMAOptimizer:
                     "otm_mma_update")
         self.last_newton = newton.value
         return x
+
+
+def fine_tile_shape(variant):
+    """(bx, by, bz, threads) of a fine smoother tile variant."""
+    sh = (ctypes.c_int * 4)()
+    _capi.check(_capi.load().otm_fine_tile_shape(int(variant), sh), "otm_fine_tile_shape")
+    return tuple(sh)
 
 
 def conv_filter(x, kind=0, radius=1.5, periodic=False, adjoint=False):

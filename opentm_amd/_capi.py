@@ -49,6 +49,7 @@ class OtmConfig(ctypes.Structure):
         ("fp16_stencils", _c.c_int),
         ("exact_smoother", _c.c_int),
         ("graph_vcycle", _c.c_int),
+        ("fine_tile", _c.c_int),
     ]
 
 
@@ -120,6 +121,21 @@ class OtmServerCounters(ctypes.Structure):
     ]
 
 
+OTM_TUNE_MAX = 16
+
+
+class OtmTuneReport(ctypes.Structure):
+    _fields_ = [
+        ("n", _c.c_int),
+        ("fine_tile", _c.c_int * OTM_TUNE_MAX),
+        ("coarse_sweeps", _c.c_int * OTM_TUNE_MAX),
+        ("ms", _c.c_double * OTM_TUNE_MAX),
+        ("iters", _c.c_int * OTM_TUNE_MAX),
+        ("ok", _c.c_int * OTM_TUNE_MAX),
+        ("chosen", _c.c_int),
+    ]
+
+
 class OtmCustomReport(ctypes.Structure):
     _fields_ = [
         ("iterations", _c.c_int),
@@ -142,6 +158,9 @@ SIGNATURES = {
     "otm_update": (_i, [_p, _p, _i, _i]),
     "otm_heat_matrix": (_i, [_p, _dp]),
     "otm_solve_stats": (_i, [_p, _ip, _dp]),
+    "otm_fine_tile_shape": (_i, [_i, _ip]),
+    "otm_fine_tile": (_i, [_p, _ip]),
+    "otm_autotune": (_i, [_p, _i, _c.POINTER(OtmTuneReport)]),
     "otm_sensitivity": (_i, [_p, _dp, _p, _i, _i]),
     "otm_element_energy": (_i, [_p, _p, _i]),
     "otm_get_chi": (_i, [_p, _p, _i]),
@@ -162,6 +181,7 @@ SIGNATURES = {
     "otm_set_design": (_i, [_p, _p, _i]),
     "otm_get_design": (_i, [_p, _p, _i]),
     "otm_eval": (_i, [_p, _c.POINTER(OtmDensityParams), _dp]),
+    "otm_apply_design": (_i, [_p, _c.POINTER(OtmDensityParams)]),
     "otm_backward": (_i, [_p, _dp, _p, _i]),
     "otm_oc_create": (_i, [_i, _d, _d, _d, _i, _c.POINTER(_p)]),
     "otm_oc_destroy": (_i, [_p]),

@@ -535,6 +535,9 @@ CellDev describe(const Member& m) {
     d.ua[l] = L.u;
     d.ub[l] = L.u2 ? L.u2 : L.u;
   }
+  // the batched launches run the default fine tile shape
+  if (c->lv[0].brick && c->fine_tile != fine_tile_default(c->g0))
+    fail(OTM_E_UNSUPPORTED, "the batched form runs the default fine tile shape (otm_config.fine_tile = -1)");
   d.fine_bx = c->lv[0].brick ? fine_brick_x(c->g0) : 0;
   d.tile_bx = fine_brick_x(c->g0);
   d.sweeps_fine = c->cfg.pre_sweeps;

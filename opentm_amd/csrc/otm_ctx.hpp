@@ -162,6 +162,7 @@ struct otm_ctx {
   std::vector<Pending> pending;
   bool bricks_on = true;   // coarse levels use the LDS brick smoother
   int tail = -1;           // first level of the one-workgroup tail V-cycle
+  int fine_tile = -1;      // fine brick smoother tile variant (kFineTileShapes), -1: none
   int post_dot_mode = -1;  // >= 0: the next V-cycle's fine post-smoother also reduces r.z (that finaliser)
   bool post_dot_done = false;
   int fallbacks = 0;       // solves that fell back to exact Gauss-Seidel

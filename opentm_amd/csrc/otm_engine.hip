@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <mutex>
 #include <random>
 #include <stdexcept>
@@ -221,7 +222,10 @@ static void build_hierarchy(otm_ctx* c) {
       L.u = c->z;
       L.f = c->r;
       L.own_uf = false;
-      if (fine_brick_x(L.g) && cfg.pre_sweeps == cfg.post_sweeps) {
+      c->fine_tile = cfg.fine_tile < 0 ? fine_tile_default(L.g) : cfg.fine_tile;
+      if (c->fine_tile >= 0 && !fine_tile_ok(L.g, c->fine_tile))
+        fail(OTM_E_UNSUPPORTED, "the cell's edges are not multiples of the requested fine tile shape");
+      if (c->fine_tile >= 0 && cfg.pre_sweeps == cfg.post_sweeps) {
         L.brick = true;
         c->z2 = dalloc<float>(3 * n);
         L.u2 = c->z2;
@@ -383,7 +387,7 @@ static void vcycle(otm_ctx* c, size_t l) {
   if (L.brick && l == 0) {
     // compulsory bytes per node: kappa 4 + f 12 + u written 12 (zero start)
     ProfScope ps(c, OTM_PROF_FINE_GS, 28.0 * nn);
-    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, nullptr, L.u, nsw, true, true);
+    launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, nullptr, L.u, nsw, true, true, c->fine_tile);
   } else if (L.brick) {
     // pre-smoothing from a zero guess straight into u
     ProfScope ps(c, OTM_PROF_COARSE_GS, 132.0 * nn);  // stencil 108 + f 12 + u 12 bytes per node
@@ -414,14 +418,15 @@ static void vcycle(otm_ctx* c, size_t l) {
   }
   if (L.brick && l == 0) {
     // r.z of the PCG from the post-smoother's own values when asked for
-    const bool dot = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g);
+    const bool dot = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g, c->fine_tile);
     const Reduce RD = c->reducer(dot ? c->post_dot_mode : FIN_STORE);
     {
       // compulsory bytes per node: kappa 4 + u read 12 + f 12 + u written 12
       ProfScope ps(c, OTM_PROF_FINE_GS, 40.0 * nn);
-      launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, nsw, false, false, dot ? &RD : nullptr);
+      launch_fine_smooth_brick(c->s, L.g, c->fc, c->kappa, L.f, L.u, L.u2, nsw, false, false, c->fine_tile,
+                               dot ? &RD : nullptr);
     }
-    if (dot) launch_fine_brick_dot_final(c->s, L.g, RD);
+    if (dot) launch_fine_brick_dot_final(c->s, L.g, RD, c->fine_tile);
     c->post_dot_done = dot;
     std::swap(L.u, L.u2);
     std::swap(c->z, c->z2);
@@ -452,6 +457,8 @@ static unsigned long long vcycle_key(const otm_ctx* c) {
   }
   key |= (unsigned long long)(c->post_dot_mode + 1) << 56;  // -1, FIN_RZ_FIRST, FIN_RZ
   key |= (unsigned long long)(c->bricks_on ? 1 : 0) << 62;
+  key |= (unsigned long long)(c->fine_tile + 1) << 48;  // 0..5 in bits 48-50
+  key |= (unsigned long long)(c->cfg.coarse_sweeps & 7) << 51;
   return key;
 }
 
@@ -464,7 +471,7 @@ static void vcycle_effects(otm_ctx* c) {
     Level& L = c->lv[l];
     if (!L.brick) continue;
     if (l == 0) {
-      c->post_dot_done = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g);
+      c->post_dot_done = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g, c->fine_tile);
       std::swap(c->z, c->z2);
     }
     std::swap(L.u, L.u2);
@@ -767,6 +774,7 @@ OTM_API void otm_config_init(otm_config* cfg) {
   cfg->fp16_stencils = 0;
   cfg->exact_smoother = 0;
   cfg->graph_vcycle = 1;
+  cfg->fine_tile = -1;
 }
 
 OTM_API const char* otm_last_error(void) { return g_last_error.c_str(); }
@@ -787,6 +795,7 @@ OTM_API int otm_create(const otm_config* cfg, otm_ctx** out) {
     if (cfg->pre_sweeps != cfg->post_sweeps)
       fail(OTM_E_ARG, "pre and post sweeps must match (symmetric preconditioner for CG)");
     if (cfg->coarse_sweeps < 0) fail(OTM_E_ARG, "coarse_sweeps must be >= 0");
+    if (cfg->fine_tile < -1 || cfg->fine_tile >= kFineTileVariants) fail(OTM_E_ARG, "fine_tile must be -1..4");
     if (!(cfg->fem_rel_thres > 0)) fail(OTM_E_ARG, "fem_rel_thres must be positive");
     otm_ctx* c = new otm_ctx();
     c->cfg = *cfg;
@@ -897,6 +906,135 @@ OTM_API int otm_heat_matrix(otm_ctx* ctx, double C[9]) {
     if (!ctx || !C) fail(OTM_E_ARG, "null argument");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (!do_heat_matrix(ctx, C)) fail(OTM_E_NOCONV, noconv_message(ctx));
+  });
+}
+
+OTM_API int otm_fine_tile_shape(int variant, int shape[4]) {
+  return guarded([&] {
+    if (!shape || variant < 0 || variant >= kFineTileVariants) fail(OTM_E_ARG, "unknown fine tile variant");
+    const FineTileShape& t = kFineTileShapes[variant];
+    shape[0] = t.bx;
+    shape[1] = t.by;
+    shape[2] = t.bz;
+    shape[3] = t.bx * t.tr;
+  });
+}
+
+OTM_API int otm_fine_tile(const otm_ctx* ctx, int* variant) {
+  return guarded([&] {
+    if (!ctx || !variant) fail(OTM_E_ARG, "null argument");
+    *variant = ctx->lv[0].u2 ? ctx->fine_tile : -1;
+  });
+}
+
+// ---- autotuner ------------------------------------------------------------------
+// Times one full three-load-case solve of the current design for every
+// candidate (each fine tile shape the cell admits, times coarse sweep counts
+// 1 and 2), each from the same start (zero, or with warm the current
+// iterate, e.g. the previous design's solution after otm_apply_design of the
+// next one) and twice (the first run also
+// captures the candidate's V-cycle graphs; the second is timed), and keeps
+// the fastest in the context. A candidate whose solve misses the tolerance
+// or falls back to exact sweeps is not chosen. The shapes change the block
+// Gauss-Seidel and so the iteration counts: the tuner weighs both.
+OTM_API int otm_autotune(otm_ctx* c, int warm, otm_tune_report* rep) {
+  return guarded([&] {
+    if (!c || !rep) fail(OTM_E_ARG, "null argument");
+    if (!c->has_kappa) fail(OTM_E_STATE, "otm_autotune before otm_update");
+    if (!c->lv[0].u2 || c->fine_tile < 0) fail(OTM_E_UNSUPPORTED, "the fine level has no brick smoother to tune");
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    memset(rep, 0, sizeof(*rep));
+    rep->chosen = -1;
+    const size_t n3 = 3 * (size_t)c->g0.nn;
+    struct Saved {
+      float *x = nullptr, *xlo = nullptr;
+      double* x64 = nullptr;
+      ~Saved() {
+        dfree(x);
+        dfree(xlo);
+        dfree(x64);
+      }
+    } sv;
+    sv.x = dalloc<float>(n3);
+    hip_check(hipMemcpyAsync(sv.x, c->x, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->s), "save start");
+    if (c->xlo) {
+      sv.xlo = dalloc<float>(n3);
+      hip_check(hipMemcpyAsync(sv.xlo, c->xlo, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->s), "save start");
+    }
+    if (c->x64) {
+      sv.x64 = dalloc<double>(n3);
+      hip_check(hipMemcpyAsync(sv.x64, c->x64, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->s), "save start");
+    }
+    // candidates are timed from a zero start (a warm start from the current
+    // iterate may need no iteration at all); the iterate is restored after
+    auto cold = [&] {
+      launch_fill(c->s, c->x, (long long)n3, 0.f);
+      if (c->xlo) launch_fill(c->s, c->xlo, (long long)n3, 0.f);
+      if (c->x64) launch_fill_d(c->s, c->x64, (long long)n3, 0.0);
+      c->start_res_ready = false;
+      set_bricks(c, true);
+      c->sync();
+    };
+    auto restore = [&] {
+      hip_check(hipMemcpyAsync(c->x, sv.x, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->s), "restore start");
+      if (c->xlo)
+        hip_check(hipMemcpyAsync(c->xlo, sv.xlo, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->s), "restore start");
+      if (c->x64)
+        hip_check(hipMemcpyAsync(c->x64, sv.x64, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->s), "restore");
+      c->start_res_ready = false;
+      set_bricks(c, true);
+      c->sync();
+    };
+    const int keep_tile = c->fine_tile, keep_cs = c->cfg.coarse_sweeps;
+    double best = 1e300;
+    for (int v = 0; v < kFineTileVariants; v++) {
+      if (!fine_tile_ok(c->g0, v)) continue;
+      for (int cs = 1; cs <= 2 && rep->n < OTM_TUNE_MAX; cs++) {
+        const int i = rep->n++;
+        rep->fine_tile[i] = v;
+        rep->coarse_sweeps[i] = cs;
+        c->fine_tile = v;
+        c->cfg.coarse_sweeps = cs;
+        bool ok = true;
+        double ms = 0.0;
+        for (int run = 0; run < 2 && ok; run++) {
+          if (warm)
+            restore();
+          else
+            cold();
+          const int f0 = c->fallbacks;
+          const auto t0 = std::chrono::steady_clock::now();
+          double C[9];
+          try {
+            ok = do_heat_matrix(c, C);
+          } catch (const OtmError&) {
+            ok = false;
+          }
+          c->sync();
+          ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+          ok = ok && c->fallbacks == f0;
+        }
+        int its = 0;
+        for (int k = 0; k < 3; k++) its = std::max(its, c->iters[k]);
+        rep->ms[i] = ms;
+        rep->iters[i] = its;
+        rep->ok[i] = ok ? 1 : 0;
+        if (ok && ms < best) {
+          best = ms;
+          rep->chosen = i;
+        }
+      }
+    }
+    if (rep->chosen >= 0) {
+      c->fine_tile = rep->fine_tile[rep->chosen];
+      c->cfg.coarse_sweeps = rep->coarse_sweeps[rep->chosen];
+    } else {
+      c->fine_tile = keep_tile;
+      c->cfg.coarse_sweeps = keep_cs;
+    }
+    restore();
+    c->has_solution = false;
+    if (rep->chosen < 0) fail(OTM_E_NOCONV, "no tuning candidate reached the tolerance");
   });
 }
 
@@ -1167,7 +1305,8 @@ OTM_API int otm_smooth_level(otm_ctx* ctx, int level, const float* f, float* u, 
     if (brick) {
       c2.d = dalloc<float>(3 * n);
       if (level == 0)
-        launch_fine_smooth_brick(ctx->s, L.g, ctx->fc, ctx->kappa, df, du, c2.d, sweeps, forward, false);
+        launch_fine_smooth_brick(ctx->s, L.g, ctx->fc, ctx->kappa, df, du, c2.d, sweeps, forward, false,
+                                 ctx->fine_tile);
       else
         launch_st_smooth_brick(ctx->s, L.g, stref(L), df, du, c2.d, sweeps, forward, false);
       stage_out(ctx, u, c2.d, 3 * n, where);
@@ -1588,9 +1727,8 @@ OTM_API int otm_design_stats(otm_ctx* ctx, double* mean_rho, double* mean_kappa)
   });
 }
 
-OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
-  return guarded([&] {
-    if (!ctx || !p || !C) fail(OTM_E_ARG, "null argument");
+// The density pipeline of otm_eval: filter, SIMP map, operator rebuild.
+static void apply_design(otm_ctx* ctx, const otm_density_params* p) {
     if (!ctx->rho) fail(OTM_E_STATE, "otm_eval before otm_set_design");
     if (p->penal != 3.0) fail(OTM_E_UNSUPPORTED, "only the cubic SIMP penalty is supported");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -1614,6 +1752,21 @@ OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
     do_update(ctx, ctx->kraw, ctx->g0.nx, OTM_DEVICE, true);
     ctx->has_drho = false;
     ctx->taps_kind = p->filter_kind >= 0 ? ctx->taps_kind : -1;
+}
+
+OTM_API int otm_apply_design(otm_ctx* ctx, const otm_density_params* p) {
+  return guarded([&] {
+    if (!ctx || !p) fail(OTM_E_ARG, "null argument");
+    apply_design(ctx, p);
+    ctx->has_solution = false;
+    ctx->sync();
+  });
+}
+
+OTM_API int otm_eval(otm_ctx* ctx, const otm_density_params* p, double C[9]) {
+  return guarded([&] {
+    if (!ctx || !p || !C) fail(OTM_E_ARG, "null argument");
+    apply_design(ctx, p);
     if (!do_heat_matrix(ctx, C)) fail(OTM_E_NOCONV, noconv_message(ctx));
   });
 }

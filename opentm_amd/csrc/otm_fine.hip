@@ -68,39 +68,71 @@ __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, 
 // The occupancy hint keeps the fully unrolled colour loop from holding
 // every node's loads in flight at once (which costs occupancy); the 30 KB
 // LDS footprint of a 32x4x8 brick allows five workgroups per CU.
-template <int BX>
-__global__ void __launch_bounds__(BX * kFineTR) __attribute__((amdgpu_waves_per_eu(4, 8)))
+template <int BX, int BY, int BZ, int TR>
+__global__ void __launch_bounds__(BX * TR) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_fine_smooth_brick(Geom g, FineConsts fc, const float* __restrict__ kappa, const float* __restrict__ f,
                     const float* __restrict__ uin, float* __restrict__ uout, int sweeps, int forward, int zero_in,
                     int dot, Reduce RD) {
-  fine_brick_body<BX>(g, fc, kappa, f, uin, uout, sweeps, forward, zero_in, xcd_block(), dot, RD);
+  fine_brick_body<BX, BY, BZ, TR>(g, fc, kappa, f, uin, uout, sweeps, forward, zero_in, xcd_block(), dot, RD);
 }
 
-int fine_brick_x(const Geom& g) { return fine_tile_x(g); }
+bool fine_tile_ok(const Geom& g, int v) {
+  if (v < 0 || v >= kFineTileVariants) return false;
+  const FineTileShape& t = kFineTileShapes[v];
+  return g.nx % t.bx == 0 && g.ny % t.by == 0 && g.nz % t.bz == 0;
+}
+int fine_tile_default(const Geom& g) { return fine_tile_ok(g, 0) ? 0 : (fine_tile_ok(g, 1) ? 1 : -1); }
+int fine_brick_x(const Geom& g) {
+  const int v = fine_tile_default(g);
+  return v < 0 ? 0 : kFineTileShapes[v].bx;
+}
+static long long fine_variant_tiles(const Geom& g, int v) {
+  const FineTileShape& t = kFineTileShapes[v];
+  return (long long)(g.nx / t.bx) * (g.ny / t.by) * (g.nz / t.bz);
+}
 
 // r.z in the smoother: 256-thread tiles, one partial-sum slot per tile
 // (the reduction buffers hold kTileRedBlocks slots)
-bool fine_brick_dot_ok(const Geom& g) {
-  return fine_brick_x(g) == 32 && 32 * kFineTR == kBlock && fine_tiles(g, 32) <= kTileRedBlocks;
+bool fine_brick_dot_ok(const Geom& g, int v) {
+  if (!fine_tile_ok(g, v)) return false;
+  const FineTileShape& t = kFineTileShapes[v];
+  return t.bx == 32 && t.bx * t.tr == kBlock && fine_variant_tiles(g, v) <= kTileRedBlocks;
 }
 
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
-                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in,
+                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in, int v,
                               const Reduce* dot) {
-  const int bxw = fine_brick_x(g);
-  const unsigned nb = (unsigned)((g.nx / bxw) * (g.ny / kFineBY) * (g.nz / kFineBZ));
-  if (dot && !fine_brick_dot_ok(g)) throw std::invalid_argument("r.z in the smoother needs 32-wide tiles");
+  if (!fine_tile_ok(g, v)) throw std::invalid_argument("the cell does not tile with this fine tile shape");
+  if (dot && !fine_brick_dot_ok(g, v)) throw std::invalid_argument("r.z in the smoother needs 256-thread 32-wide tiles");
+  const unsigned nb = (unsigned)fine_variant_tiles(g, v);
   const Reduce RD = dot ? *dot : Reduce{};
-  if (bxw == 32)
-    hipLaunchKernelGGL(k_fine_smooth_brick<32>, dim3(nb), dim3(32 * kFineTR), 0, s, g, fc, kappa, f, uin, uout,
-                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0, dot ? 1 : 0, RD);
-  else
-    hipLaunchKernelGGL(k_fine_smooth_brick<16>, dim3(nb), dim3(16 * kFineTR), 0, s, g, fc, kappa, f, uin, uout,
-                       sweeps, forward ? 1 : 0, zero_in ? 1 : 0, 0, RD);
+  const int fw = forward ? 1 : 0, zi = zero_in ? 1 : 0, d = dot ? 1 : 0;
+  switch (v) {
+    case 0:
+      hipLaunchKernelGGL((k_fine_smooth_brick<32, 4, 8, 8>), dim3(nb), dim3(256), 0, s, g, fc, kappa, f, uin, uout,
+                         sweeps, fw, zi, d, RD);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_fine_smooth_brick<16, 4, 8, 8>), dim3(nb), dim3(128), 0, s, g, fc, kappa, f, uin, uout,
+                         sweeps, fw, zi, 0, RD);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_fine_smooth_brick<32, 8, 4, 8>), dim3(nb), dim3(256), 0, s, g, fc, kappa, f, uin, uout,
+                         sweeps, fw, zi, d, RD);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_fine_smooth_brick<32, 4, 4, 4>), dim3(nb), dim3(128), 0, s, g, fc, kappa, f, uin, uout,
+                         sweeps, fw, zi, 0, RD);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fine_smooth_brick<32, 8, 8, 8>), dim3(nb), dim3(256), 0, s, g, fc, kappa, f, uin, uout,
+                         sweeps, fw, zi, d, RD);
+      break;
+  }
 }
 
-void launch_fine_brick_dot_final(hipStream_t s, const Geom& g, const Reduce& R) {
-  launch_reduce_final(s, R, OTM_NRHS, (unsigned)((g.nx / 32) * (g.ny / kFineBY) * (g.nz / kFineBZ)));
+void launch_fine_brick_dot_final(hipStream_t s, const Geom& g, const Reduce& R, int v) {
+  launch_reduce_final(s, R, OTM_NRHS, (unsigned)fine_variant_tiles(g, v));
 }
 
 // ---- launchers ---------------------------------------------------------

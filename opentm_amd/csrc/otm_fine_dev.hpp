@@ -889,8 +889,8 @@ __device__ inline void fine_brick_body(const Geom& g, const FineConsts& fc, cons
     // halo column x0 - 1 seen by the tile's first column: its T1; halo
     // column x0 + BX seen by the last column: its T0 (frozen values)
     __shared__ float hterm[2][BY * BZ][OTM_NRHS];
-    if (t < 2 * BY * BZ) {
-      const int side = t / (BY * BZ), r = t % (BY * BZ);
+    for (int i = t; i < 2 * BY * BZ; i += T::NT) {
+      const int side = i / (BY * BZ), r = i % (BY * BZ);
       const int ly = r % BY, lz = r / BY;
       const int hx = side ? HX - 1 : 0, kx = side ? BX : 0;
       const int c0 = hx + DY * (ly + 1) + DZ * (lz + 1);

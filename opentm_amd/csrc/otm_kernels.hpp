@@ -23,17 +23,30 @@ void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, con
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
                              float* sens, int pitch);
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, float* S6);
-// Fine brick smoother (BX x kFineBY x kFineBZ bricks, frozen halo, result in uout);
-// fine_brick_x returns BX (32 or 16) or 0 when the cell does not tile.
-// With dot (fine_brick_dot_ok cells), the blocks also store their partial
-// sums of f.uout for that reducer (the PCG's r.z after a preconditioner
+// Fine brick smoother: tiles of bx x by x bz nodes, one node per thread and
+// colour step for tr thread rows of bx threads, halo frozen, result in
+// uout. Each tile shape is its own block Gauss-Seidel (the couplings that
+// leave a tile are frozen during a smoothing step), so the shape is a
+// per-context choice (otm_config.fine_tile, otm_autotune); variant 0 is the
+// default where the cell tiles with it, else variant 1.
+struct FineTileShape {
+  int bx, by, bz, tr;
+};
+constexpr int kFineTileVariants = 5;
+constexpr FineTileShape kFineTileShapes[kFineTileVariants] = {
+    {32, 4, 8, 8}, {16, 4, 8, 8}, {32, 8, 4, 8}, {32, 4, 4, 4}, {32, 8, 8, 8}};
+bool fine_tile_ok(const Geom& g, int variant);
+int fine_tile_default(const Geom& g);  // -1: the cell does not tile
+// fine_brick_x: bx of the default variant (0 when the cell does not tile).
+// With dot (fine_brick_dot_ok), the blocks also store their partial sums of
+// f.uout for that reducer (the PCG's r.z after a preconditioner
 // application); launch_fine_brick_dot_final then finishes the reduction.
 int fine_brick_x(const Geom& g);
-bool fine_brick_dot_ok(const Geom& g);
+bool fine_brick_dot_ok(const Geom& g, int variant);
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,
-                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in,
+                              const float* uin, float* uout, int sweeps, bool forward, bool zero_in, int variant,
                               const Reduce* dot = nullptr);
-void launch_fine_brick_dot_final(hipStream_t s, const Geom& g, const Reduce& R);
+void launch_fine_brick_dot_final(hipStream_t s, const Geom& g, const Reduce& R, int variant);
 
 // ---- coarse levels (stored 27-point stencils, otm_coarse.hip) ----
 // Stored stencil of a coarse level: FP32 (f16 == nullptr) or FP16 couplings

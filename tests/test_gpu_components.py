@@ -168,3 +168,57 @@ def test_coarsest_solve_is_the_pseudo_inverse(reso):
         say("  rhs", c, "n", n, "err", err, "residual", res)
         assert err < 1e-5
         assert res < 1e-5
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_fine_tile_variants_match_block_restatement(orc, variant):
+    """Every fine smoother tile shape the autotuner can pick is the oracle's
+    block Gauss-Seidel with that tile, both colour orders, two sweeps."""
+    shape3 = otm.fine_tile_shape(variant)[:3]
+    say("fine tile variant", variant, shape3)
+    rng = np.random.default_rng(25)
+    reso = (64, 16, 16)
+    k = rng.uniform(1e-3, 1.0, (16, 16, 64)).astype(np.float32)
+    hom = make(reso, fine_tile=variant)
+    assert hom.fine_tile()[0] == variant
+    hom.update(k)
+    shape = shape_of(hom.levels()[0])
+    n = int(np.prod(shape))
+    A = orc.assemble(k.astype(np.float64) + 1e-9)
+    f = rng.standard_normal((3, n)).astype(np.float32)
+    u = rng.standard_normal((3, n)).astype(np.float32)
+    for mode, reverse in ((3, False), (2, True)):
+        g = hom.smooth_level(0, f, u, sweeps=2, mode=mode).reshape(3, n)
+        o = orc.brick_gs(shape, A, f, u, shape3, sweeps=2, reverse=reverse)
+        errs = [rel(g[c], o[c]) for c in range(3)]
+        say("  mode", mode, "max rel err", max(errs))
+        assert max(errs) < 2e-6
+
+
+def test_autotune_picks_a_candidate_that_matches_the_oracle(orc):
+    """The autotuner times every admissible (tile shape, coarse sweeps)
+    candidate; the context keeps the fastest, its solve matches the FP64
+    oracle, and a context pinned to the same choice solves bitwise alike."""
+    n = 32
+    k = wl.simp(wl.plain_density(n, 5))
+    hom = make(n)
+    hom.update(k)
+    rows, chosen = hom.autotune()
+    say("autotune", [(r["fine_tile"], r["coarse_sweeps"], round(r["ms"], 3), r["iters"], r["ok"]) for r in rows],
+        "chosen", chosen)
+    assert len(rows) == 10 and 0 <= chosen < len(rows) and rows[chosen]["ok"]
+    assert rows[chosen]["ms"] == min(r["ms"] for r in rows if r["ok"])
+    assert hom.fine_tile()[0] == rows[chosen]["fine_tile"]
+    C = hom.heatMatrix()
+    Co = orc.homogenize(k.astype(np.float64), rtol=1e-12)[0]
+    assert rel(C, Co) < 1e-5
+    pinned = make(n, fine_tile=rows[chosen]["fine_tile"], coarse_sweeps=rows[chosen]["coarse_sweeps"])
+    pinned.update(k)
+    assert np.array_equal(pinned.heatMatrix(), C)
+    # warm: every candidate starts from the solution just computed and the
+    # iterate the context keeps afterwards is that solution
+    rows_w, chosen_w = hom.autotune(warm=True)
+    assert all(r["ok"] for r in rows_w) and 0 <= chosen_w < len(rows_w)
+    assert rel(hom.heatMatrix(), Co) < 1e-5
+    for h in (hom, pinned):
+        h.close()
