@@ -137,6 +137,40 @@ def roofline_from(prof, reso):
             "traffic": measured_traffic(FINE_KERNEL, reso)}
 
 
+MMA_KERNELS = {"trial": "k_mma_point<3, true>", "dir": "k_mma_dir<3>", "point": "k_mma_point<3, false>"}
+
+
+def mma_rooflines(mprof):
+    """The MMA subsolver passes: algorithmic bytes per launch (x 4, asymptotes
+    16, the FP32 gradients 4 (1 + m), the FP64 interior point 24 bytes per
+    element; the trial pass also writes the 24-byte trial point) over the
+    mean launch time from HIP events on the optimiser's stream."""
+    out = {}
+    for name, (nl, ms, by) in mprof.items():
+        if not nl or ms <= 0:
+            continue
+        avg_s = ms * 1e-3 / nl
+        achieved = (by / nl) / avg_s / 1e9
+        out[name] = {"bound": "hbm", "kernel": MMA_KERNELS[name], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "bytes_per_launch": by / nl, "avg_launch_us": round(avg_s * 1e6, 3), "launches": nl,
+                     "total_ms": round(ms, 3), "timed_over": f"{PROFILE_STEPS} steps after the timed region",
+                     "traffic": None}
+    return out
+
+
+def dominant_roofline(prof, mprof, reso):
+    """The roofline of the kernel with the most time among the live-timed
+    ones (the fine smoother and the MMA passes)."""
+    cands = []
+    st = roofline_from(prof, reso)
+    if st:
+        cands.append((st["avg_launch_us"] * st["launches"], st))
+    for r in mma_rooflines(mprof).values():
+        cands.append((r["avg_launch_us"] * r["launches"], r))
+    return max(cands, key=lambda c: c[0])[1] if cands else None
+
+
 def class_rows(prof):
     rows = {}
     for name, (nl, ms, by) in prof.items():
@@ -149,17 +183,24 @@ def class_rows(prof):
 PROFILE_STEPS = 3
 
 
-def profiled_steps(hom, step, classes, label):
+def profiled_steps(hom, step, classes, label, mma=None):
     """Kernel-class timing with HIP events on the engine's stream over a few
     steps run right after the timed region (events are recorded around
-    eager launches; the timed region replays captured V-cycle graphs)."""
+    eager launches; the timed region replays captured V-cycle graphs), and
+    of the MMA passes on the optimiser's stream when mma is given."""
     hom.synchronize()
     hom.profile_enable(True, None if classes == "all" else classes)
+    if mma is not None:
+        mma.profile_enable(True)
     for i in range(PROFILE_STEPS):
         step()
         print(f"[bench] {label} profiled step {i + 1}/{PROFILE_STEPS}", file=sys.stderr, flush=True)
     prof = hom.profile_read()
     hom.profile_enable(False)
+    if mma is not None:
+        prof = dict(prof)
+        prof["mma"] = mma.profile_read()
+        mma.profile_enable(False)
     return prof
 
 
@@ -182,7 +223,7 @@ def precision_run(n, rho0, refine, steps, warmup, rtol, label, profile_classes=N
 
     total, lat, hist = timed_loop(loop.step, steps, warmup, loop.hom.synchronize, label, loop.history.clear)
     hist = list(hist)
-    prof = profiled_steps(loop.hom, loop.step, profile_classes, label) if profile_classes is not None else None
+    prof = profiled_steps(loop.hom, loop.step, profile_classes, label, loop.mma) if profile_classes is not None else None
     res = {"solves_per_s": steps / total, "ms_per_step": 1e3 * total / steps,
            "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
            "pcg_iters_mean": float(np.mean([max(h["pcg_iters"]) for h in hist])),
@@ -258,6 +299,8 @@ def main():
                     help="copy: three 256 MiB device copies, the calibration run for rocprofv3 byte counters")
     ap.add_argument("--qps", default="100,200,300,400", help="serve workload: arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=2000, help="serve workload: requests per arrival rate")
+    ap.add_argument("--plain-requests", type=int, default=1000,
+                    help="serve workload: requests of the same mix run through the plain single-solve path")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve workload: HBM budget of the server")
     ap.add_argument("--max-active", type=int, default=96, help="serve workload: requests on the GPU at once")
     ap.add_argument("--cells", type=int, default=96, help="batch workload: number of cells")
@@ -315,7 +358,9 @@ def precision_main(args):
                    "parallelism": "single GPU"},
         "p50_ms": prec["p50_ms"], "p99_ms": prec["p99_ms"], "pcg_iters_mean": prec["pcg_iters_mean"],
         "relres_max": prec["relres_max"], "C_final": prec["C_final"], "g_final": prec["g_final"],
-        "roofline": roofline_from(prof, n), "kernel_classes": class_rows(prof),
+        "roofline": dominant_roofline(prof, prof["mma"], n), "stencil_roofline": roofline_from(prof, n),
+        "mma_passes": mma_rooflines(prof["mma"]),
+        "kernel_classes": class_rows({k: v for k, v in prof.items() if k != "mma"}),
     }
     if "autotune" in prec:
         out["autotune"] = prec["autotune"]
@@ -557,6 +602,34 @@ def serve_main(args):
               f"p99 {lv['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     st = sv.stats()
     sv.close()
+    # the same request mix through the plain single-solve FP32 path, one
+    # request after another (contexts cached per shape, as the server does)
+    plain_cfg = {}
+    for e in edges:
+        c = otm.HomoConfig(e, (1.0, 1e-3))
+        c.femRelThres = args.rtol
+        plain_cfg[e] = c
+    homs = {}
+    npl = min(args.requests, args.plain_requests)
+    picks = [(draw_edge(), int(rng.integers(0, pool_per_edge))) for _ in range(npl)]
+    for e in edges:  # hierarchies built before the clock starts, as for the server
+        homs[e] = otm.Homogenization_H(plain_cfg[e])
+    lat = []
+    t_start = time.perf_counter()
+    for e, j in picks:
+        ts = time.perf_counter()
+        h = homs[e]
+        h.reset_displacement()
+        h.update(pool[e][j])
+        h.heatMatrix()
+        h.Sensitivity(dC)
+        lat.append(1e3 * (time.perf_counter() - ts))
+    t_plain = time.perf_counter() - t_start
+    for h in homs.values():
+        h.close()
+    plain = {"requests": npl, "solves_per_s": npl / t_plain, "p50_ms": float(np.percentile(lat, 50)),
+             "p99_ms": float(np.percentile(lat, 99)), "path": "plain single-solve FP32 (rtol %g), looped" % args.rtol}
+    print(f"[bench] plain single-solve loop: {plain['solves_per_s']:.1f} solves/s", file=sys.stderr, flush=True)
     best = max(levels, key=lambda l: l["solves_per_s"])
     out = {"metric": "homog. solves/s + p50/p99 latency (arrival -> result), serving form", "value": best["solves_per_s"],
            "unit": "solves/s", "n_gpus": 1, "steps": sum(l["requests"] for l in levels), "warmup": len(edges),
@@ -566,7 +639,8 @@ def serve_main(args):
                                   "GRF densities (corr U[4,12], vol U[0.1,0.5], beta 8), kappaH + sensitivity",
                       "rtol": args.rtol, "seed": args.seed, "hbm_budget_gb": args.hbm_gb,
                       "max_active": args.max_active},
-           "levels": levels, "server": st}
+           "levels": levels, "server": st, "plain_single_solve": plain,
+           "served_over_plain": best["solves_per_s"] / plain["solves_per_s"]}
     print(json.dumps(out))
 
 

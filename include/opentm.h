@@ -294,6 +294,12 @@ OTM_API int otm_mma_set_move(otm_mma* mma, double move);
 OTM_API int otm_mma_update(otm_mma* mma, int itn, float* x, const float* dfdx, const double* gval,
                            const float* const* dgdx, int where, int* newton);
 
+/* HIP-event timing of the MMA passes (no counterpart in the reference):
+ * kclass 0 = trial-point pass, 1 = direction pass, 2 = start-of-level point
+ * pass; launches, milliseconds and algorithmic bytes since enabling. */
+OTM_API int otm_mma_profile_enable(otm_mma* mma, int enable);
+OTM_API int otm_mma_profile_read(otm_mma* mma, int kclass, long long* launches, double* ms, double* bytes);
+
 /* initDensity (Framework.cu:61-145) on the context's design, indexed like
  * cfg::InitWay: 0 random, 1 randcenter, 2 noise, 3 manual, 4 interp,
  * 5 rep_randcenter, 6 P, 7 G, 8 D, 9 IWP, 10 example.

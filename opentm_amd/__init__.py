@@ -561,6 +561,23 @@ class MMAOptimizer:
     def set_move(self, move):
         _capi.check(_capi.load().otm_mma_set_move(self._h, float(move)), "otm_mma_set_move")
 
+    MMA_PASSES = ("trial", "dir", "point")
+
+    def profile_enable(self, enable=True):
+        """HIP-event timing of the subsolver passes (trial point, direction,
+        start-of-level point)."""
+        _capi.check(_capi.load().otm_mma_profile_enable(self._h, 1 if enable else 0), "otm_mma_profile_enable")
+
+    def profile_read(self):
+        """{pass: (launches, ms, algorithmic bytes)} since profile_enable."""
+        out = {}
+        for k, name in enumerate(self.MMA_PASSES):
+            nl, ms, by = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_double()
+            _capi.check(_capi.load().otm_mma_profile_read(self._h, k, ctypes.byref(nl), ctypes.byref(ms),
+                                                          ctypes.byref(by)), "otm_mma_profile_read")
+            out[name] = (nl.value, ms.value, by.value)
+        return out
+
     def update(self, itn, x, dfdx, gval, dgdx):
         """In place on x (float32 array or DeviceBuffer); dgdx is a sequence
         of m arrays (all host or all device, like x)."""

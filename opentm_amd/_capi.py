@@ -195,6 +195,8 @@ SIGNATURES = {
     "otm_mma_set_bounds_array": (_i, [_p, _p, _p, _i]),
     "otm_mma_set_move": (_i, [_p, _d]),
     "otm_mma_update": (_i, [_p, _i, _p, _p, _dp, _c.POINTER(_p), _i, _ip]),
+    "otm_mma_profile_enable": (_i, [_p, _i]),
+    "otm_mma_profile_read": (_i, [_p, _i, _c.POINTER(_c.c_longlong), _dp, _dp]),
     "otm_init_density": (_i, [_p, _i, _i, _d]),
     "otm_init_params_init": (None, [_c.POINTER(OtmInitParams)]),
     "otm_init_design": (_i, [_p, _c.POINTER(OtmInitParams), _p]),

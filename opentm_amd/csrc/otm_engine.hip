@@ -1975,6 +1975,7 @@ struct otm_mma {
   float* fx = nullptr;         // design, previous designs, gradients (FP32): (5 + m) n floats
   float* fbounds = nullptr;    // per-element bounds (2n floats), once set
   float* xout = nullptr;       // FP32 result staging for host callers
+  MmaProf prof;
   std::vector<double*> owned;
 };
 
@@ -1987,6 +1988,7 @@ static void mma_free(otm_mma* o) {
   dfree(o->fx);
   dfree(o->fbounds);
   dfree(o->xout);
+  mma_prof_release(o->prof);
   if (o->small_host) (void)hipHostFree(o->small_host);
   if (o->s) (void)hipStreamDestroy(o->s);
   delete o;
@@ -2101,13 +2103,36 @@ OTM_API int otm_mma_update(otm_mma* o, int itn, float* x, const float* dfdx, con
     for (int i = 0; i < o->m; i++) mma_copy_in(o, (float*)o->A.dgdx + (long long)i * n, dgdx[i], where);
     hip_check(hipMemcpyAsync(o->gval, gval, sizeof(double) * o->m, hipMemcpyHostToDevice, o->s), "gval");
     float* xout = where == OTM_DEVICE ? x : o->xout;
-    const int rc = mma_update(o->s, o->m, o->A, o->small, o->small_host, o->gval, itn, o->a0, o->a, o->c, o->d, xout);
+    const int rc =
+        mma_update(o->s, o->m, o->A, o->small, o->small_host, o->gval, itn, o->a0, o->a, o->c, o->d, xout, &o->prof);
     if (rc == -2) fail(OTM_E_NUMERIC, "MMA subsolver produced non-finite values");
     if (rc < 0) fail(OTM_E_HIP, "MMA update failed");
     if (where == OTM_HOST)
       hip_check(hipMemcpyAsync(x, o->xout, sizeof(float) * n, hipMemcpyDeviceToHost, o->s), "download x");
     hip_check(hipStreamSynchronize(o->s), "sync");
     if (newton) *newton = rc;
+  });
+}
+
+OTM_API int otm_mma_profile_enable(otm_mma* o, int enable) {
+  return guarded([&] {
+    if (!o) fail(OTM_E_ARG, "null optimiser");
+    hip_check(hipStreamSynchronize(o->s), "sync");
+    for (int k = 0; k < 3; k++) {
+      o->prof.launches[k] = 0;
+      o->prof.ms[k] = 0.0;
+      o->prof.bytes[k] = 0.0;
+    }
+    o->prof.on = enable != 0;
+  });
+}
+
+OTM_API int otm_mma_profile_read(otm_mma* o, int kclass, long long* launches, double* ms, double* bytes) {
+  return guarded([&] {
+    if (!o || kclass < 0 || kclass > 2) fail(OTM_E_ARG, "invalid argument");
+    if (launches) *launches = o->prof.launches[kclass];
+    if (ms) *ms = o->prof.ms[kclass];
+    if (bytes) *bytes = o->prof.bytes[kclass];
   });
 }
 

@@ -1,6 +1,8 @@
 // otm_kernels.hpp -- host-side launchers of the gfx950 kernels.
 #pragma once
 
+#include <vector>
+
 #include "otm_common.hpp"
 
 namespace otm {
@@ -188,10 +190,25 @@ inline unsigned red_blocks_for_mma(long long n) {
   return (unsigned)b;
 }
 size_t mma_small_bytes();
+// HIP-event timing of the MMA passes (otm_mma_profile_*): class 0 the
+// trial-point pass, 1 the direction pass, 2 the point pass at the start of
+// each barrier level; bytes are the algorithmic bytes per launch.
+struct MmaProf {
+  bool on = false;
+  long long launches[3] = {0, 0, 0};
+  double ms[3] = {0, 0, 0}, bytes[3] = {0, 0, 0};
+  struct Pending {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+};
+void mma_prof_release(MmaProf& P);
 // One MMA update on device arrays; returns Newton steps or a negative code
 // (-1 HIP error, -2 non-finite values, -3 unsupported m).
 int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
-               int itn, double a0, double av, double cv, double dv, float* xout);
+               int itn, double a0, double av, double cv, double dv, float* xout, MmaProf* prof = nullptr);
 void launch_f2d(hipStream_t s, long long n, const float* in, double* out);
 void launch_fill_d(hipStream_t s, double* a, long long n, double v);
 

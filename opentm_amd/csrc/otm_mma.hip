@@ -607,14 +607,72 @@ __global__ void __launch_bounds__(kBlock) k_mma_finish(MmaArrays A, float* xout)
 }
 
 // ---- host driver ---------------------------------------------------------------
+static hipEvent_t prof_event(MmaProf& P) {
+  if (!P.pool.empty()) {
+    hipEvent_t e = P.pool.back();
+    P.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+// events of finished launches -> per-class time (after a stream sync)
+static void prof_collect(MmaProf& P) {
+  for (const auto& q : P.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) P.ms[q.cls] += ms;
+    P.pool.push_back(q.a);
+    P.pool.push_back(q.b);
+  }
+  P.pending.clear();
+}
+void mma_prof_release(MmaProf& P) {
+  for (const auto& q : P.pending) {
+    P.pool.push_back(q.a);
+    P.pool.push_back(q.b);
+  }
+  P.pending.clear();
+  for (hipEvent_t e : P.pool) (void)hipEventDestroy(e);
+  P.pool.clear();
+}
+
 template <int M>
 static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const double* gval_dev, int itn,
-                   double a0, double av, double cv, double dv, float* xout, double epsimin) {
+                   double a0, double av, double cv, double dv, float* xout, double epsimin, MmaProf* prof) {
   const unsigned nb = red_blocks_for_mma(A.n);
   auto read = [&]() -> int {
     if (hipMemcpyAsync(Sh, S, sizeof(MmaSmall), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (prof && prof->on) prof_collect(*prof);
     return Sh->fault ? -2 : 0;
+  };
+  // algorithmic bytes per element: x, low/upp, gradients, interior point
+  // (+ per-element bounds), and the trial point written
+  const double rd = 4.0 + 16.0 + 4.0 * (1 + M) + 24.0 + (A.xmin ? 8.0 : 0.0);
+  const double bytes_of[3] = {(rd + 24.0) * (double)A.n, rd * (double)A.n, rd * (double)A.n};
+  struct Timed {
+    MmaProf* P;
+    int cls;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    Timed(MmaProf* P_, int cls_, hipStream_t s_, const double* by) : P(P_), cls(cls_), s(s_) {
+      if (!P || !P->on) return;
+      a = prof_event(*P);
+      if (a && hipEventRecord(a, s) != hipSuccess) a = nullptr;
+      if (a) {
+        P->launches[cls]++;
+        P->bytes[cls] += by[cls];
+      }
+    }
+    ~Timed() {
+      if (!a) return;
+      hipEvent_t b = prof_event(*P);
+      if (b && hipEventRecord(b, s) == hipSuccess)
+        P->pending.push_back({cls, a, b});
+      else
+        P->pool.push_back(a);
+    }
   };
   hipLaunchKernelGGL(k_mma_prep<M>, dim3(nb), dim3(kBlock), 0, s, A, itn);
   hipLaunchKernelGGL(k_mma_prep_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, gval_dev, (int)nb, a0, av, cv, dv);
@@ -622,7 +680,10 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
   double epsi = 1.0;
   while (epsi > epsimin) {
     // residual and Newton system at the start point of this epsi level
-    hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb), dim3(kBlock), 0, s, A, S);
+    {
+      Timed t(prof, 2, s, bytes_of);
+      hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb), dim3(kBlock), 0, s, A, S);
+    }
     hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
     int rc = read();
     if (rc) return rc;
@@ -631,10 +692,16 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
       ittt++;
       newton++;
       hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(1), 0, s, S);
-      hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb), dim3(kBlock), 0, s, A, S);
+      {
+        Timed t(prof, 1, s, bytes_of);
+        hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb), dim3(kBlock), 0, s, A, S);
+      }
       hipLaunchKernelGGL(k_mma_dir_final, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
       do {
-        hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb), dim3(kBlock), 0, s, A, S);
+        {
+          Timed t(prof, 0, s, bytes_of);
+          hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb), dim3(kBlock), 0, s, A, S);
+        }
         hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
         rc = read();
         if (rc) return rc;
@@ -654,16 +721,16 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
 }
 
 int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
-               int itn, double a0, double av, double cv, double dv, float* xout) {
+               int itn, double a0, double av, double cv, double dv, float* xout, MmaProf* prof) {
   MmaSmall* S = (MmaSmall*)small_dev;
   MmaSmall* Sh = (MmaSmall*)small_host;
   MmaArrays B = A;  // the driver swaps the subsolver buffers locally
   int rc;
   switch (m) {
-    case 1: rc = mma_run<1>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
-    case 2: rc = mma_run<2>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
-    case 3: rc = mma_run<3>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
-    case 4: rc = mma_run<4>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7); break;
+    case 1: rc = mma_run<1>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 2: rc = mma_run<2>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 3: rc = mma_run<3>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 4: rc = mma_run<4>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
     default: rc = -3;
   }
   return rc;
