@@ -140,7 +140,20 @@ def roofline_from(prof, reso):
 MMA_KERNELS = {"trial": "k_mma_point<3, true>", "dir": "k_mma_dir<3>", "point": "k_mma_point<3, false>"}
 
 
-def mma_rooflines(mprof):
+def measured_mma_traffic(name, reso):
+    """HBM bytes per launch of an MMA pass at `reso` from the committed
+    counter summary profiles/traffic_mma_<reso>.json (FETCH_SIZE and
+    WRITE_SIZE runs scaled by the copy calibration; traffic_256.md gives the
+    commands), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", f"traffic_mma_{reso}.json")) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return t.get("passes", {}).get(name, {}).get("hbm_bytes_per_launch")
+
+
+def mma_rooflines(mprof, reso=None):
     """The MMA subsolver passes: algorithmic bytes per launch (x 4, asymptotes
     16, the FP32 gradients 4 (1 + m), the FP64 interior point 24 bytes per
     element; the trial pass also writes the 24-byte trial point) over the
@@ -155,7 +168,7 @@ def mma_rooflines(mprof):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "bytes_per_launch": by / nl, "avg_launch_us": round(avg_s * 1e6, 3), "launches": nl,
                      "total_ms": round(ms, 3), "timed_over": f"{PROFILE_STEPS} steps after the timed region",
-                     "traffic": None}
+                     "traffic": measured_mma_traffic(name, reso)}
     return out
 
 
@@ -166,7 +179,7 @@ def dominant_roofline(prof, mprof, reso):
     st = roofline_from(prof, reso)
     if st:
         cands.append((st["avg_launch_us"] * st["launches"], st))
-    for r in mma_rooflines(mprof).values():
+    for r in mma_rooflines(mprof, reso).values():
         cands.append((r["avg_launch_us"] * r["launches"], r))
     return max(cands, key=lambda c: c[0])[1] if cands else None
 
@@ -359,7 +372,7 @@ def precision_main(args):
         "p50_ms": prec["p50_ms"], "p99_ms": prec["p99_ms"], "pcg_iters_mean": prec["pcg_iters_mean"],
         "relres_max": prec["relres_max"], "C_final": prec["C_final"], "g_final": prec["g_final"],
         "roofline": dominant_roofline(prof, prof["mma"], n), "stencil_roofline": roofline_from(prof, n),
-        "mma_passes": mma_rooflines(prof["mma"]),
+        "mma_passes": mma_rooflines(prof["mma"], n),
         "kernel_classes": class_rows({k: v for k, v in prof.items() if k != "mma"}),
     }
     if "autotune" in prec:
