@@ -357,6 +357,35 @@ OTM_API int otm_design_stats(otm_ctx* ctx, double* mean_rho, double* mean_kappa)
  * (VolumeGovernor + ConvergeChecker) or MMA (minimise volume subject to the
  * tensor error). */
 enum { OTM_MODEL_MMA = 0, OTM_MODEL_OC = 1 };
+/* Iteration log of the design loop (logIter, Framework.cu:7-59): files are
+ * prefix + name (getPath). Every 5th iteration "rho" (the design) and "C"
+ * (the tensor); every logrho-th "rho_%04d"; every logc-th a line of "Clog";
+ * every logsens-th "sens_%04d" (the objective's design gradient); every
+ * logobj-th a line of "objlog". Intervals 0 switch a file off. Density and
+ * gradient files use the binary format of otm_write_density (the reference
+ * writes OpenVDB, which this build does not link). */
+typedef struct otm_log_params {
+  const char* prefix;  /* HomoConfig.outprefix; NULL: no log */
+  int logrho;          /* intervals of HomoConfig.logrho, logc, logsens, logobj */
+  int logc;
+  int logsens;
+  int logobj;
+} otm_log_params;
+OTM_API void otm_log_params_init(otm_log_params* p);
+/* One iteration's log: C (9 doubles), the objective, and the design
+ * gradient sens (reso^3 floats on `where`, NULL when logsens is off). */
+OTM_API int otm_log_iteration(otm_ctx* ctx, const otm_log_params* p, int iter, const double C[9], double obj,
+                              const float* sens, int where);
+
+/* Full reductions of a field with their gradient (reduce_tsexp_t and the
+ * associative ops of TensorExpression.h:1187-1251, 1342-1402): kind 0 sum,
+ * 1 p-norm (sum x^p)^(1/p), 2 log-sum-exp. The value is accumulated in FP64
+ * (deterministic two-stage sums); grad (optional, n floats) receives
+ * upstream * d value / d x_i. x and grad on `where`. */
+enum { OTM_REDUCE_SUM = 0, OTM_REDUCE_PNORM = 1, OTM_REDUCE_LOGSUMEXP = 2 };
+OTM_API int otm_field_reduce(const float* x, long long n, int kind, double p, int where, double* value, float* grad,
+                             double upstream);
+
 typedef struct otm_custom_params {
   otm_config base;     /* resolution, heat ratios, solver */
   int model;           /* OTM_MODEL_MMA or OTM_MODEL_OC (HomoConfig.model) */
@@ -367,6 +396,7 @@ typedef struct otm_custom_params {
   double volratio;     /* HomoConfig.volRatio (noise initialiser) */
   double target[6];    /* HomoConfig.target_tensor: xx yy zz xy yz xz */
   int sym;             /* HomoConfig.sym: OTM_SYM_* (default OTM_SYM_NONE) */
+  otm_log_params log;  /* iteration log (default: none) */
 } otm_custom_params;
 typedef struct otm_custom_report {
   int iterations;

@@ -663,9 +663,12 @@ def symmetrize(field, sym):
 
 def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
                target=(0.3, 0.2, 0.1, 0.0, 0.0, 0.0), heat_ratio=(1.0, 0.0), fem_rel_thres=1e-6,
-               finthres=5e-4, volratio=0.3, coarse_sweeps=0, sym="NONE"):
+               finthres=5e-4, volratio=0.3, coarse_sweeps=0, sym="NONE", log_prefix=None, logrho=0, logc=0,
+               logsens=0, logobj=0):
     """runCustom (Framework.cu:147-244) on the engine: returns the final
-    design density (x fastest) and a report dict."""
+    design density (x fastest) and a report dict. log_prefix (a path
+    prefix) turns on the iteration log of logIter (Framework.cu:7-59) with
+    the given intervals."""
     lib = _capi.load()
     p = _capi.OtmCustomParams()
     lib.otm_custom_params_init(ctypes.byref(p))
@@ -684,6 +687,11 @@ def run_custom(reso, model="oc", max_iter=500, rho0=None, initway="IWP", seed=0,
     p.sym = _SYMMETRIES.index(sym)
     for i in range(6):
         p.target[i] = float(target[i])
+    prefix = None
+    if log_prefix is not None:
+        prefix = str(log_prefix).encode()
+        p.log.prefix = prefix
+        p.log.logrho, p.log.logc, p.log.logsens, p.log.logobj = int(logrho), int(logc), int(logsens), int(logobj)
     n = r[0] * r[1] * r[2]
     out = np.empty(n, np.float32)
     rep = _capi.OtmCustomReport()

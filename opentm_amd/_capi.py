@@ -74,6 +74,16 @@ class OtmDensityParams(ctypes.Structure):
     ]
 
 
+class OtmLogParams(ctypes.Structure):
+    _fields_ = [
+        ("prefix", _c.c_char_p),
+        ("logrho", _c.c_int),
+        ("logc", _c.c_int),
+        ("logsens", _c.c_int),
+        ("logobj", _c.c_int),
+    ]
+
+
 class OtmCustomParams(ctypes.Structure):
     _fields_ = [
         ("base", OtmConfig),
@@ -85,6 +95,7 @@ class OtmCustomParams(ctypes.Structure):
         ("volratio", _c.c_double),
         ("target", _c.c_double * 6),
         ("sym", _c.c_int),
+        ("log", OtmLogParams),
     ]
 
 
@@ -206,6 +217,9 @@ SIGNATURES = {
     "otm_symmetrize_design": (_i, [_p, _i]),
     "otm_design_stats": (_i, [_p, _dp, _dp]),
     "otm_custom_params_init": (None, [_c.POINTER(OtmCustomParams)]),
+    "otm_log_params_init": (None, [_c.POINTER(OtmLogParams)]),
+    "otm_log_iteration": (_i, [_p, _c.POINTER(OtmLogParams), _i, _dp, _d, _p, _i]),
+    "otm_field_reduce": (_i, [_p, _c.c_longlong, _i, _d, _i, _dp, _p, _d]),
     "otm_run_custom": (_i, [_c.POINTER(OtmCustomParams), _p, _p, _c.POINTER(OtmCustomReport)]),
     "otm_profile_enable": (_i, [_p, _i]),
     "otm_profile_read": (_i, [_p, _i, _c.POINTER(_c.c_longlong), _dp, _dp]),

@@ -1452,6 +1452,62 @@ OTM_API int otm_conv_filter_noalloc(const int reso[3], int kind, double radius, 
   });
 }
 
+static_assert(kReduceSum == OTM_REDUCE_SUM && kReducePnorm == OTM_REDUCE_PNORM &&
+                  kReduceLogSumExp == OTM_REDUCE_LOGSUMEXP,
+              "reduction kinds");
+
+OTM_API int otm_field_reduce(const float* x, long long n, int kind, double p, int where, double* value, float* grad,
+                             double upstream) {
+  return guarded([&] {
+    if (!x || !value || n <= 0) fail(OTM_E_ARG, "invalid argument");
+    check_where(where);
+    if (kind < OTM_REDUCE_SUM || kind > OTM_REDUCE_LOGSUMEXP) fail(OTM_E_ARG, "unknown reduction");
+    if (kind == OTM_REDUCE_PNORM && !(p > 0)) fail(OTM_E_ARG, "p must be positive");
+    hipStream_t s = nullptr;
+    unsigned nb = (unsigned)std::min<long long>((n + kBlock - 1) / kBlock, 1024);
+    struct Bufs {
+      float *x = nullptr, *g = nullptr;
+      double* part = nullptr;
+      ~Bufs() {
+        dfree(x);
+        dfree(g);
+        dfree(part);
+      }
+    } b;
+    const float* dx = x;
+    if (where == OTM_HOST) {
+      b.x = dalloc<float>(n);
+      hip_check(hipMemcpy(b.x, x, sizeof(float) * n, hipMemcpyHostToDevice), "copy in");
+      dx = b.x;
+    }
+    b.part = dalloc<double>(nb);
+    std::vector<double> part(nb);
+    double shift = 0.0;
+    if (kind == OTM_REDUCE_LOGSUMEXP) {  // exp(x - max): the same value, no overflow
+      launch_field_max(s, n, dx, b.part, nb);
+      hip_check(hipMemcpy(part.data(), b.part, sizeof(double) * nb, hipMemcpyDeviceToHost), "max");
+      shift = -INFINITY;
+      for (double m : part) shift = std::max(shift, m);
+    }
+    launch_field_sum(s, kind, n, dx, p, shift, b.part, nb);
+    hip_check(hipMemcpy(part.data(), b.part, sizeof(double) * nb, hipMemcpyDeviceToHost), "sum");
+    double sum = 0.0;
+    for (double v : part) sum += v;
+    const double val = kind == OTM_REDUCE_SUM ? sum : (kind == OTM_REDUCE_PNORM ? pow(sum, 1.0 / p) : shift + log(sum));
+    *value = val;
+    if (grad) {
+      float* dg = grad;
+      if (where == OTM_HOST) dg = b.g = dalloc<float>(n);
+      launch_field_grad(s, kind, n, dx, p, val, upstream, dg);
+      hip_check(hipGetLastError(), "field gradient");
+      if (where == OTM_HOST)
+        hip_check(hipMemcpy(grad, dg, sizeof(float) * n, hipMemcpyDeviceToHost), "copy out");
+      else
+        hip_check(hipStreamSynchronize(s), "sync");
+    }
+  });
+}
+
 // Symmetry groups of symmetrizeField (Framework.cuh:221-235) on the cell's
 // centre: reflect3 = the 8 axis reflections, reflect6 = reflections x axis
 // permutations (48), rotate3 = the 24 rotations of the cube (permutation

@@ -78,6 +78,70 @@ OTM_API void otm_custom_params_init(otm_custom_params* p) {
   const double t[6] = {0.3, 0.2, 0.1, 0.0, 0.0, 0.0};  // cmdline.cpp:29-34
   for (int i = 0; i < 6; i++) p->target[i] = t[i];
   p->sym = OTM_SYM_NONE;  // cmdline.cpp:5
+  otm_log_params_init(&p->log);
+}
+
+OTM_API void otm_log_params_init(otm_log_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));  // no prefix, every interval off
+}
+
+// logIter (Framework.cu:7-59). Text lines are appended (the file is
+// started afresh at iteration 0); density and gradient files are written
+// whole in the binary density format.
+OTM_API int otm_log_iteration(otm_ctx* ctx, const otm_log_params* p, int iter, const double C[9], double obj,
+                              const float* sens, int where) {
+  try {
+    if (!ctx || !p || !C || iter < 0) return OTM_E_ARG;
+    if (!p->prefix) return OTM_OK;
+    int reso[3];
+    ck(otm_level_reso(ctx, 0, reso));
+    const size_t ne = (size_t)reso[0] * reso[1] * reso[2];
+    const std::string pre(p->prefix);
+    auto write_design = [&](const std::string& name) {
+      std::vector<float> rho(ne);
+      ck(otm_get_design(ctx, rho.data(), OTM_HOST));
+      ck(otm_write_density((pre + name).c_str(), reso, rho.data(), OTM_HOST));
+    };
+    auto append = [&](const std::string& name, const std::string& line) {
+      FILE* f = fopen((pre + name).c_str(), iter == 0 ? "w" : "a");
+      if (!f) throw Fail{OTM_E_ARG};
+      fputs(line.c_str(), f);
+      fclose(f);
+    };
+    char buf[64];
+    if (iter % 5 == 0) {
+      write_design("rho");
+      FILE* f = fopen((pre + "C").c_str(), "w");
+      if (!f) throw Fail{OTM_E_ARG};
+      for (int i = 0; i < 3; i++) fprintf(f, "%.9e %.9e %.9e\n", C[3 * i], C[3 * i + 1], C[3 * i + 2]);
+      fclose(f);
+    }
+    if (p->logrho > 0 && iter % p->logrho == 0) {
+      snprintf(buf, sizeof(buf), "rho_%04d", iter);
+      write_design(buf);
+    }
+    if (p->logc > 0 && iter % p->logc == 0) {
+      std::string line = "iter " + std::to_string(iter) + " ";
+      for (int q = 0; q < 9; q++) {
+        snprintf(buf, sizeof(buf), "%.9e ", C[q]);
+        line += buf;
+      }
+      append("Clog", line + "\n");
+    }
+    if (p->logsens > 0 && iter % p->logsens == 0) {
+      if (!sens) return OTM_E_ARG;
+      snprintf(buf, sizeof(buf), "sens_%04d", iter);
+      ck(otm_write_density((pre + buf).c_str(), reso, sens, where));
+    }
+    if (p->logobj > 0 && iter % p->logobj == 0) {
+      snprintf(buf, sizeof(buf), "obj = %.9e", obj);
+      append("objlog", "iter " + std::to_string(iter) + " " + buf + "\n");
+    }
+    return OTM_OK;
+  } catch (const Fail& f) {
+    return f.code;
+  }
 }
 
 // Binary density format of Grid_H::writeDensity (grid.cpp:330-345): three
@@ -172,6 +236,7 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
         obj = tensor_objective(C, p->target, dC);
         const float f0val = (float)obj;  // the reference's objective value is a float (Framework.cu:181)
         ck(otm_backward(hom.c, dC, dfdx.f(), OTM_DEVICE));
+        ck(otm_log_iteration(hom.c, &p->log, itn, C, obj, dfdx.f(), OTM_DEVICE));
         if (criteria.is_converge(itn, f0val)) {
           converged = true;
           break;
@@ -192,6 +257,7 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
         ~OcGuard() { otm_oc_destroy(o); }
       } guard{oc};
       homo::VolumeGovernor governor;
+      DevBuf sens_log(dev, p->log.prefix && p->log.logsens > 0 ? sizeof(float) * ne : sizeof(float));
       for (itn = 0; itn < p->max_iter; itn++) {
         eval_counted(hom.c, &dp, C, &unconverged);
         obj = tensor_objective(C, p->target, dC);
@@ -199,7 +265,13 @@ OTM_API int otm_run_custom(const otm_custom_params* p, const float* rho0, float*
         double vf = 0.0, lower = 0.0;
         ck(otm_design_stats(hom.c, &vf, &lower));
         governor.volume_check(val, (float)lower, (float)vf, itn);
-        ck(otm_backward(hom.c, dC, nullptr, OTM_DEVICE));
+        if (p->log.prefix && p->log.logsens > 0 && itn % p->log.logsens == 0) {
+          ck(otm_backward(hom.c, dC, sens_log.f(), OTM_DEVICE));
+          ck(otm_log_iteration(hom.c, &p->log, itn, C, obj, sens_log.f(), OTM_DEVICE));
+        } else {
+          ck(otm_backward(hom.c, dC, nullptr, OTM_DEVICE));
+          ck(otm_log_iteration(hom.c, &p->log, itn, C, obj, nullptr, OTM_DEVICE));
+        }
         if (criteria.is_converge(itn, val) && governor.get_current_decrease() < 1e-4) {
           converged = true;
           break;

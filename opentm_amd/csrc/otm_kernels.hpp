@@ -222,4 +222,13 @@ struct SymGroup {
 };
 void launch_symmetrize(hipStream_t s, const Geom& g, const SymGroup& G, const float* in, float* out);
 
+// Full reductions of a field (otm_field_reduce): block partials of the max
+// and of sum / sum x^p / sum exp(x - shift) in item order, and the gradient.
+constexpr int kReduceSum = 0, kReducePnorm = 1, kReduceLogSumExp = 2;
+void launch_field_max(hipStream_t s, long long n, const float* x, double* part, unsigned nb);
+void launch_field_sum(hipStream_t s, int kind, long long n, const float* x, double p, double shift, double* part,
+                      unsigned nb);
+void launch_field_grad(hipStream_t s, int kind, long long n, const float* x, double p, double value, double upstream,
+                       float* g);
+
 }  // namespace otm

@@ -719,3 +719,77 @@ void launch_symmetrize(hipStream_t s, const Geom& g, const SymGroup& G, const fl
 }
 
 }  // namespace otm
+
+namespace otm {
+
+// ---- full reductions of a field (reduce_tsexp_t, TensorExpression.h:1342-1402) ----
+// Block partials in the slot of the block's item range (deterministic
+// order), FP64 accumulation; the host sums the partials in order.
+__global__ void __launch_bounds__(kBlock) k_field_max(long long n, const float* __restrict__ x,
+                                                      double* __restrict__ part) {
+  __shared__ double sh[kWaves];
+  double m = -INFINITY;
+  const BlockRange br = block_range(n);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) m = fmax(m, (double)x[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; w++) m = fmax(m, sh[w]);
+    part[xcd_block()] = m;
+  }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_field_sum(long long n, const float* __restrict__ x, double p,
+                                                      double shift, double* __restrict__ part) {
+  __shared__ double sh[kWaves];
+  double v[1] = {0.0};
+  const BlockRange br = block_range(n);
+  for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
+    const double xi = (double)x[i];
+    v[0] += KIND == kReduceSum ? xi : (KIND == kReducePnorm ? pow(xi, p) : exp(xi - shift));
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[xcd_block()] = v[0];
+}
+
+// grad_i = upstream * d value / d x_i: 1, (x_i / value)^(p-1), exp(x_i - value)
+// (AssociativeOp::diff, TensorExpression.h:1220-1229)
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_field_grad(long long n, const float* __restrict__ x, double p,
+                                                       double value, double upstream, float* __restrict__ g) {
+  const long long i = gtid();
+  if (i >= n) return;
+  const double xi = (double)x[i];
+  const double d = KIND == kReduceSum ? 1.0
+                                          : (KIND == kReducePnorm ? pow(xi / value, p - 1.0) : exp(xi - value));
+  g[i] = (float)(upstream * d);
+}
+
+void launch_field_max(hipStream_t s, long long n, const float* x, double* part, unsigned nb) {
+  hipLaunchKernelGGL(k_field_max, dim3(nb), dim3(kBlock), 0, s, n, x, part);
+}
+void launch_field_sum(hipStream_t s, int kind, long long n, const float* x, double p, double shift, double* part,
+                      unsigned nb) {
+  if (kind == kReduceSum)
+    hipLaunchKernelGGL(k_field_sum<kReduceSum>, dim3(nb), dim3(kBlock), 0, s, n, x, p, shift, part);
+  else if (kind == kReducePnorm)
+    hipLaunchKernelGGL(k_field_sum<kReducePnorm>, dim3(nb), dim3(kBlock), 0, s, n, x, p, shift, part);
+  else
+    hipLaunchKernelGGL(k_field_sum<kReduceLogSumExp>, dim3(nb), dim3(kBlock), 0, s, n, x, p, shift, part);
+}
+void launch_field_grad(hipStream_t s, int kind, long long n, const float* x, double p, double value, double upstream,
+                       float* g) {
+  const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+  if (kind == kReduceSum)
+    hipLaunchKernelGGL(k_field_grad<kReduceSum>, dim3(nb), dim3(kBlock), 0, s, n, x, p, value, upstream, g);
+  else if (kind == kReducePnorm)
+    hipLaunchKernelGGL(k_field_grad<kReducePnorm>, dim3(nb), dim3(kBlock), 0, s, n, x, p, value, upstream, g);
+  else
+    hipLaunchKernelGGL(k_field_grad<kReduceLogSumExp>, dim3(nb), dim3(kBlock), 0, s, n, x, p, value, upstream,
+                       g);
+}
+
+}  // namespace otm

@@ -109,7 +109,8 @@ def test_struct_layouts_match_header():
                       ("otm_init_params", _capi.OtmInitParams),
                       ("otm_custom_params", _capi.OtmCustomParams), ("otm_custom_report", _capi.OtmCustomReport),
                       ("otm_request", _capi.OtmRequest), ("otm_result", _capi.OtmResult),
-                      ("otm_server_counters", _capi.OtmServerCounters), ("otm_tune_report", _capi.OtmTuneReport)]:
+                      ("otm_server_counters", _capi.OtmServerCounters), ("otm_tune_report", _capi.OtmTuneReport),
+                      ("otm_log_params", _capi.OtmLogParams)]:
         body = re.search(r"typedef struct\s*\w*\s*\{([^}]*)\}\s*" + cname + r"\s*;", hdr).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         names = [re.split(r"[\s\[;]", d.strip().split()[-1])[0] for d in body.split(";") if d.strip()]
