@@ -71,7 +71,7 @@ def host_description():
     return {"cpu_model": model, "nproc": os.cpu_count(), "omp_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
-def cpu_baseline_mg(n_full, seed, rtol, mean):
+def cpu_baseline_mg(n_full, seed, rtol, mean, fp32=False):
     """The FP64 MG-preconditioned CG of the oracle (OpenMP on the host cores;
     same method: Galerkin V(2,2) multicolour Gauss-Seidel in the reference's
     colour order) on a bounded sample: one full homogenization + sensitivity
@@ -86,7 +86,7 @@ def cpu_baseline_mg(n_full, seed, rtol, mean):
     t0 = time.perf_counter()
     rf = orc.filter_fwd(rho, 0, 1.5, False)
     kappa = 1e-3 + (1 - 1e-3) * rf ** 3
-    C, _, S, its, rr = orc.mg_homogenize(kappa, rtol=rtol, want_S=True)
+    C, _, S, its, rr = orc.mg_homogenize(kappa, rtol=rtol, want_S=True, fp32=fp32)
     for dC in (-np.diag([1.0, 0, 0]), np.diag([0, 1.0, 0]), np.diag([0, 0, 1.0])):
         s = orc.sensitivity(S, dC).reshape(rf.shape)
         orc.filter_adj(s * 3 * (1 - 1e-3) * rf ** 2, 0, 1.5, False)
@@ -95,7 +95,8 @@ def cpu_baseline_mg(n_full, seed, rtol, mean):
     host = host_description()
     cores = int(host["omp_threads"] or host["nproc"] or 1)
     return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 MG-PCG (oracle, OpenMP) on the same generator's {ns}^3 design to rtol {rtol:g}: "
+            "sample": f"{'FP32' if fp32 else 'FP64'} MG-PCG (oracle, OpenMP) on the same generator's {ns}^3 design "
+                      f"to rtol {rtol:g}: "
                       f"filter + homogenization ({max(its)} PCG iterations) + element energies + 3 sensitivity "
                       f"contractions in {dt:.1f} s, scaled x{scale:g} to {n_full}^3 (linear in the element count)",
             "host": host}
@@ -392,6 +393,8 @@ def precision_main(args):
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (FP64 MG-PCG oracle, bounded sample) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_mg(n, args.seed, 1e-8, 0.25)
+        # the plain path's method on the host: FP32 MG-PCG to the plain tolerance
+        out["cpu_baseline_fp32"] = cpu_baseline_mg(n, args.seed, args.rtol, 0.25, fp32=True)
     print(json.dumps(out))
 
 
