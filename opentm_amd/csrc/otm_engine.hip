@@ -2087,7 +2087,7 @@ OTM_API int otm_mma_create(int m, int n, double a0, double a, double c, double d
       A.low = vec(n); A.upp = vec(n);
       A.xs = vec(n); A.xsi = vec(n); A.eta = vec(n);
       A.xs_t = vec(n); A.xsi_t = vec(n); A.eta_t = vec(n);
-      A.partials = vec(1024 * 32);
+      A.partials = vec((long long)kMmaMaxBlocks * 32);
       o->gval = dalloc<double>(kMmaMaxM);
       o->small = dalloc<char>(mma_small_bytes());
       o->xout = dalloc<float>(n);

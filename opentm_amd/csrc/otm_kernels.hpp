@@ -183,9 +183,10 @@ struct MmaArrays {
   double *xs_t, *xsi_t, *eta_t;   // its trial
   double* partials;               // [blocks * 32]
 };
+constexpr int kMmaMaxBlocks = 1024;  // partial-sum slots of the MMA passes
 inline unsigned red_blocks_for_mma(long long n) {
   long long b = (n + kBlock - 1) / kBlock;
-  if (b > 1024) b = 1024;
+  if (b > kMmaMaxBlocks) b = kMmaMaxBlocks;
   if (b < 1) b = 1;
   return (unsigned)b;
 }

@@ -204,22 +204,14 @@ __device__ inline void mma_elem(const MmaArrays& A, const MmaRaw<M>& r, MmaElem<
   mma_coefficients<M>(A.move, (double)r.x, r.xmn, r.xmx, (double)r.g0, r.g, e);
 }
 
-// f(j, raw) over this block's elements in the order of a plain stride loop,
-// with the loads of the next element issued before the arithmetic of the
-// current one: a pass is bound by the bytes in flight per CU, and a thread
-// then keeps two elements' loads outstanding.
+// f(j, raw) over this block's elements in the order of a plain stride loop.
 template <int M, typename F>
 __device__ inline void mma_for_each(const MmaArrays& A, F&& f) {
   const BlockRange br = block_range(A.n);
-  const long long bs = blockDim.x;
-  for (long long j = br.b + threadIdx.x; j < br.e; j += 2 * bs) {
-    const long long j1 = j + bs;
-    const bool two = j1 < br.e;
-    MmaRaw<M> r0, r1;
-    mma_fetch<M>(A, j, r0);
-    if (two) mma_fetch<M>(A, j1, r1);
-    f(j, r0);
-    if (two) f(j1, r1);
+  for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
+    MmaRaw<M> r;
+    mma_fetch<M>(A, j, r);
+    f(j, r);
   }
 }
 
