@@ -255,7 +255,15 @@ struct OpInitResid {
     const XT* x;
     if constexpr (sizeof(XT) == 8) x = C.x64;
     else x = C.x;
-    fine_init_residual_body<XT>(C.g[0], C.fc, C.kappa, x, sizeof(XT) == 4 ? C.xlo : nullptr, C.r, R, lb, nb);
+    const float* xlo = sizeof(XT) == 4 ? C.xlo : nullptr;
+    // the single-cell launch's form for this row length, one load case after
+    // another: the same r and partial sums
+    if (!fine_rows_ok(C.g[0].nx)) {
+      fine_init_residual_body<XT>(C.g[0], C.fc, C.kappa, x, xlo, C.r, R, lb, nb);
+      return;
+    }
+    for (int k = 0; k < OTM_NRHS; k++)
+      fine_init_residual_rows_body<XT>(C.g[0], C.fc, C.kappa, x, xlo, C.r, R, lb, nb, k);
   }
 };
 

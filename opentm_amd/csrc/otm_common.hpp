@@ -183,6 +183,14 @@ __device__ inline BlockRange block_range_v(long long n, int lb, int nb) {
 }
 __device__ inline BlockRange block_range(long long n) { return block_range_v(n, xcd_block(), gridDim.x); }
 
+// Streaming kernels (every item read and written once, no data shared
+// between blocks) index in the hardware's dispatch order instead: the
+// blocks running at one time then cover one region of each array together.
+// XCD-contiguous ranges measured 13% lower HBM bandwidth on a read of ten
+// FP32/FP64 streams (4.8 against 5.5 TB/s, 1024-2048 blocks).
+__device__ inline long long stid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline BlockRange stream_range(long long n) { return block_range_v(n, blockIdx.x, gridDim.x); }
+
 // Sum over aligned groups of 8 lanes; every lane of a group receives the
 // same sum (IEEE addition commutes). Cross-lane moves through DPP: swap
 // pairs (quad_perm 1,0,3,2), swap pairs of pairs (quad_perm 2,3,0,1), then
@@ -259,10 +267,10 @@ __device__ inline void reduce_block_at(double (&v)[NV], const Reduce& R, int slo
   }
 }
 // The partial's slot is the block's range index, so the finaliser adds the
-// partials in item order whatever the launch shape.
+// partials in item order whatever the launch shape (stream_range order).
 template <int NV>
 __device__ inline void reduce_block(double (&v)[NV], const Reduce& R) {
-  reduce_block_at<NV>(v, R, xcd_block());
+  reduce_block_at<NV>(v, R, (int)blockIdx.x);
 }
 
 // Copies a finaliser's results to the host block and publishes the tag

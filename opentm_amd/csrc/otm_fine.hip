@@ -48,6 +48,16 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual_case(Geom g, Fine
   fine_init_residual_case_body<XT>(g, fc, kappa, x_, xlo, rvec, R, xcd_block(), gridDim.x, (int)blockIdx.y);
 }
 
+// the same by whole x-rows (fine_init_residual_rows_body), nx <= kBlock
+template <typename XT>
+__global__ void __launch_bounds__(kBlock) k_fine_init_residual_rows(Geom g, FineConsts fc,
+                                                                    const float* __restrict__ kappa,
+                                                                    const XT* __restrict__ x_,
+                                                                    const float* __restrict__ xlo,
+                                                                    float* __restrict__ rvec, Reduce R) {
+  fine_init_residual_rows_body<XT>(g, fc, kappa, x_, xlo, rvec, R, xcd_block(), gridDim.x, (int)blockIdx.y);
+}
+
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
                                                          ChiRef chi, Reduce R) {
   fine_kappa_h_body(g, fc, kappa, chi, R, xcd_block(), gridDim.x);
@@ -173,20 +183,29 @@ void launch_fine_apply_dot(hipStream_t s, const Geom& g, const FineConsts& fc, c
   launch_reduce_final(s, R, OTM_NRHS, nblocks);
 }
 
-// The FP64 residual stays pointwise: an LDS-tiled form (180 VGPRs, two
-// waves per SIMD) measured 25% slower than this one (three waves per SIMD).
-void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
-                               const float* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual_case<float>, dim3(nblocks, OTM_NRHS), dim3(kBlock), 0, s, g, fc, kappa, x,
-                     xlo, r, R);
+// The FP64 residual by whole x-rows where rows have at most kBlock nodes
+// (3 new iterate values per node and case); longer rows take the pointwise
+// form. An LDS-tiled form (180 VGPRs, two waves per SIMD) measured 25%
+// slower than the pointwise one.
+template <typename XT>
+static void launch_init_residual_xt(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
+                                    const XT* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks) {
+  const dim3 grid(nblocks, OTM_NRHS), blk(kBlock);
+  if (fine_rows_ok(g.nx))
+    hipLaunchKernelGGL(k_fine_init_residual_rows<XT>, grid, blk, 0, s, g, fc, kappa, x, xlo, r, R);
+  else
+    hipLaunchKernelGGL(k_fine_init_residual_case<XT>, grid, blk, 0, s, g, fc, kappa, x, xlo, r, R);
   launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
 }
 
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
+                               const float* x, const float* xlo, float* r, const Reduce& R, unsigned nblocks) {
+  launch_init_residual_xt<float>(s, g, fc, kappa, x, xlo, r, R, nblocks);
+}
+
+void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const double* x, float* r, const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_init_residual_case<double>, dim3(nblocks, OTM_NRHS), dim3(kBlock), 0, s, g, fc, kappa,
-                     x, (const float*)nullptr, r, R);
-  launch_reduce_final(s, R, 3 * OTM_NRHS, nblocks);
+  launch_init_residual_xt<double>(s, g, fc, kappa, x, nullptr, r, R, nblocks);
 }
 
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const ChiRef& chi,

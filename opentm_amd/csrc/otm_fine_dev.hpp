@@ -385,6 +385,190 @@ __device__ inline void fine_init_residual_case_body(const Geom& g, const FineCon
   }
 }
 
+// fine_init_residual_case_body by whole x-rows, for rows of at most kBlock
+// nodes: the neighbour sum of node (x, y, z) regrouped by element quadrant
+// q = (ay, az) (sides sy = 2ay - 1, sz = 2az - 1), with G_q(x') =
+// u(x', y+sy, z) + u(x', y, z+sz) + u(x', y+sy, z+sz) from column x' alone:
+//   far(x) = A(x-1) + B(x+1) + sum_q (k(x-1,q) + k(x,q)) u(x, y+sy, z+sz),
+//   A(x') = sum_q k(x',q) G_q(x'),  B(x') = sum_q k(x'-1,q) G_q(x'),
+// k(x', q) the element at (x', y-1+ay, z-1+az). A thread holds the 3 x 3
+// iterate values of its column around its row and the conductivities of
+// the two element rows y-1 and y, puts A and B in LDS and after a barrier
+// takes A(x-1) and B(x+1) of its row (periodic in x). Block lb of nb takes
+// a contiguous range of rows (y fastest) and splits it into one segment per
+// row group of nx threads; a group walks its segment in +y, so a step loads
+// only the values of the row entering the window (3 of the iterate at
+// y+2, 4 conductivities of element row y+1), issued before the barrier of
+// the step before. The window is a ring of three rows (the loop is unrolled
+// by three so no values move) and LDS is double-buffered: one barrier per
+// step. Writes values k, 3 + k and 6 + k of partial slot lb.
+__host__ __device__ inline bool fine_rows_ok(int nx) { return nx <= kBlock; }
+
+template <typename XT>
+struct RowsWindow {
+  double V[3][3];       // V[slot][bz]: iterate at row slot, plane z - 1 + bz
+  double KM[3][2], KP[3][2];  // element row slot: conductivity at x - 1 / x, plane z - 1 + az
+  double SQ[3][2], H[3];      // KM + KP, and its sum over az
+  double pu[3];         // the entering row's values
+  float pk[4];
+  unsigned y, z, pz[3];  // current row, node index of (0, 0, z - 1 + bz)
+};
+
+template <typename XT>
+__device__ inline double rows_val(const XT* __restrict__ xk, const float* __restrict__ xl, unsigned v) {
+  if constexpr (sizeof(XT) == 4) return xl ? (double)xk[v] + (double)xl[v] : (double)xk[v];
+  else return xk[v];
+}
+
+template <typename XT>
+__device__ inline void rows_elem_slot(RowsWindow<XT>& W, int e, float m0, float p0, float m1, float p1) {
+  W.KM[e][0] = (double)m0;
+  W.KP[e][0] = (double)p0;
+  W.KM[e][1] = (double)m1;
+  W.KP[e][1] = (double)p1;
+  W.SQ[e][0] = W.KM[e][0] + W.KP[e][0];
+  W.SQ[e][1] = W.KM[e][1] + W.KP[e][1];
+  W.H[e] = W.SQ[e][0] + W.SQ[e][1];
+}
+
+// One step of a row group at ring phase P: rows y-1, y, y+1 in slots P,
+// P+1, P+2 (mod 3); element rows y-1, y in slots P, P+1.
+template <typename XT, int P>
+__device__ inline void rows_step(RowsWindow<XT>& W, bool& have, unsigned row, bool act, unsigned ge, int x, int xm,
+                                 int j, int nx, unsigned ny, unsigned nz, double (*sA)[kBlock],
+                                 double (*sB)[kBlock], int buf, const float* __restrict__ kappa,
+                                 const XT* __restrict__ xk, const float* __restrict__ xl, float* __restrict__ rk,
+                                 const double fe[8], double c, double acc[3]) {
+  constexpr int s0 = P % 3, s1 = (P + 1) % 3, s2 = (P + 2) % 3;
+  const unsigned plane = (unsigned)nx * ny;
+  if (act && !have) {
+    W.y = row % ny;
+    W.z = row / ny;
+    const unsigned zm = W.z == 0 ? nz - 1 : W.z - 1, zp = W.z == nz - 1 ? 0 : W.z + 1;
+    W.pz[0] = zm * plane;
+    W.pz[1] = W.z * plane;
+    W.pz[2] = zp * plane;
+    const unsigned ym = W.y == 0 ? ny - 1 : W.y - 1, yp = W.y == ny - 1 ? 0 : W.y + 1;
+    const unsigned yo[3] = {ym * (unsigned)nx, W.y * (unsigned)nx, yp * (unsigned)nx};
+    constexpr int sl[3] = {s0, s1, s2};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int bz = 0; bz < 3; bz++) W.V[sl[a]][bz] = rows_val(xk, xl, W.pz[bz] + yo[a] + (unsigned)x);
+#pragma unroll
+    for (int ay = 0; ay < 2; ay++) {
+      const unsigned r0 = W.pz[0] + yo[ay], r1 = W.pz[1] + yo[ay];
+      rows_elem_slot(W, sl[ay], kappa[r0 + xm], kappa[r0 + x], kappa[r1 + xm], kappa[r1 + x]);
+    }
+  }
+  double Lv = 0.0, bl = 0.0;
+  if (act) {
+    double A = 0.0, B = 0.0, L = 0.0;
+#pragma unroll
+    for (int az = 0; az < 2; az++)
+#pragma unroll
+      for (int ay = 0; ay < 2; ay++) {
+        const int a = ay ? s2 : s0, e = ay ? s1 : s0, bz = 2 * az;
+        const double G = W.V[a][1] + W.V[s1][bz] + W.V[a][bz];
+        A += W.KP[e][az] * G;
+        B += W.KM[e][az] * G;
+        L += W.SQ[e][az] * W.V[a][bz];
+      }
+    // load vector: element (x-1+ax, y-1+ay, z-1+az) weighs fe[ax + 2 ay + 4 az]
+#pragma unroll
+    for (int az = 0; az < 2; az++)
+#pragma unroll
+      for (int ay = 0; ay < 2; ay++) {
+        const int e = ay ? s1 : s0;
+        bl += W.KM[e][az] * fe[2 * ay + 4 * az];
+        bl += W.KP[e][az] * fe[1 + 2 * ay + 4 * az];
+      }
+    const double ksum = W.H[s0] + W.H[s1];
+    sA[buf][j] = A;
+    sB[buf][j] = B;
+    Lv = 4.0 * ksum * W.V[s1][1] - L;
+  }
+  // the next row of the segment is (y + 1, z): fetch its entering values now
+  const bool slide = act && row + 1 < ge && W.y + 1 < ny;
+  if (slide) {
+    const unsigned y1 = (W.y + 1) * (unsigned)nx, y2 = W.y + 2 == ny ? 0u : y1 + (unsigned)nx;
+#pragma unroll
+    for (int bz = 0; bz < 3; bz++) W.pu[bz] = rows_val(xk, xl, W.pz[bz] + y2 + (unsigned)x);
+    const unsigned r0 = W.pz[0] + y1, r1 = W.pz[1] + y1;
+    W.pk[0] = kappa[r0 + xm];
+    W.pk[1] = kappa[r0 + x];
+    W.pk[2] = kappa[r1 + xm];
+    W.pk[3] = kappa[r1 + x];
+  }
+  __syncthreads();
+  if (act) {
+    const int jl = x == 0 ? j + nx - 1 : j - 1, jr = x == nx - 1 ? j - nx + 1 : j + 1;
+    const double kx = c * (Lv - (sA[buf][jl] + sB[buf][jr]));
+    const double rv = bl - kx;
+    rk[W.pz[1] + W.y * (unsigned)nx + (unsigned)x] = (float)rv;
+    acc[0] += bl * bl;
+    acc[1] += rv * rv;
+    acc[2] += rv;
+  }
+  if (slide) {
+    // row y-1 leaves slot s0 for row y+2; element row y+1 takes slot s2
+#pragma unroll
+    for (int bz = 0; bz < 3; bz++) W.V[s0][bz] = W.pu[bz];
+    rows_elem_slot(W, s2, W.pk[0], W.pk[1], W.pk[2], W.pk[3]);
+    W.y += 1;
+  }
+  have = slide;
+}
+
+template <typename XT>
+__device__ inline void fine_init_residual_rows_body(const Geom& g, const FineConsts& fc,
+                                                    const float* __restrict__ kappa, const XT* __restrict__ x_,
+                                                    const float* __restrict__ xlo, float* __restrict__ rvec,
+                                                    const Reduce& R, int lb, int nb, int k) {
+  __shared__ double sA[2][kBlock], sB[2][kBlock];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const double c = (double)fc.c;
+  double fe[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+    fe[e] = (double)(k == 0 ? fc.femu[7 - e][0] : (k == 1 ? fc.femu[7 - e][1] : fc.femu[7 - e][2]));
+  const XT* xk = x_ + (size_t)k * (size_t)g.nn;
+  const float* xl = xlo ? xlo + (size_t)k * (size_t)g.nn : nullptr;
+  float* rk = rvec + (size_t)k * (size_t)g.nn;
+  const int nx = g.nx;
+  const unsigned ny = (unsigned)g.ny, nz = (unsigned)g.nz, rows = ny * nz;
+  const unsigned per = (rows + (unsigned)nb - 1) / (unsigned)nb;
+  const unsigned rb = min((unsigned)lb * per, rows), re = min(rb + per, rows);
+  const int rpc = kBlock / nx;  // row groups
+  const int grp = (int)threadIdx.x / nx;
+  const int x = (int)threadIdx.x - grp * nx, xm = x == 0 ? nx - 1 : x - 1;
+  const unsigned seg = (re - rb + (unsigned)rpc - 1) / (unsigned)rpc;
+  const unsigned gs = rb + (unsigned)grp * seg, ge = min(gs + seg, re);
+  const bool grp_ok = grp < rpc;
+  RowsWindow<XT> W;
+  W.y = W.z = 0;
+  W.pz[0] = W.pz[1] = W.pz[2] = 0;
+  bool have = false;
+  for (unsigned t = 0; t < seg;) {
+    rows_step<XT, 0>(W, have, gs + t, grp_ok && gs + t < ge, ge, x, xm, (int)threadIdx.x, nx, ny, nz, sA, sB,
+                     (int)(t & 1), kappa, xk, xl, rk, fe, c, acc);
+    if (++t >= seg) break;
+    rows_step<XT, 1>(W, have, gs + t, grp_ok && gs + t < ge, ge, x, xm, (int)threadIdx.x, nx, ny, nz, sA, sB,
+                     (int)(t & 1), kappa, xk, xl, rk, fe, c, acc);
+    if (++t >= seg) break;
+    rows_step<XT, 2>(W, have, gs + t, grp_ok && gs + t < ge, ge, x, xm, (int)threadIdx.x, nx, ny, nz, sA, sB,
+                     (int)(t & 1), kappa, xk, xl, rk, fe, c, acc);
+    ++t;
+  }
+  __shared__ double sh[3 * kWaves];
+  block_sum<3>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) R.partials[(size_t)lb * (3 * OTM_NRHS) + k + OTM_NRHS * i] = acc[i];
+  }
+  __syncthreads();  // sh is free again for a following call in the same block
+}
+
 // Element energies (homogenization.cu:13-153 and :255-412). For element e
 // with nodal fields d_k = disp[:,k] - chi_k(nodes): S_ij = d_i^T Ke d_j.
 // Ke has zero row sums and couples a node only to its 3 face-diagonal and

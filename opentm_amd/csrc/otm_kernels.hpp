@@ -183,13 +183,7 @@ struct MmaArrays {
   double *xs_t, *xsi_t, *eta_t;   // its trial
   double* partials;               // [blocks * 32]
 };
-constexpr int kMmaMaxBlocks = 1024;  // partial-sum slots of the MMA passes
-inline unsigned red_blocks_for_mma(long long n) {
-  long long b = (n + kBlock - 1) / kBlock;
-  if (b > kMmaMaxBlocks) b = kMmaMaxBlocks;
-  if (b < 1) b = 1;
-  return (unsigned)b;
-}
+constexpr int kMmaMaxBlocks = 2048;  // partial-sum slots of the MMA passes
 size_t mma_small_bytes();
 // HIP-event timing of the MMA passes (otm_mma_profile_*): class 0 the
 // trial-point pass, 1 the direction pass, 2 the point pass at the start of

@@ -26,6 +26,8 @@
 // (m+1) solve by Gaussian elimination with partial pivoting, step length,
 // line-search bookkeeping) on the device. m is limited to kMmaMaxM.
 #include <math.h>
+
+#include <algorithm>
 #include <string.h>
 
 #include "otm_kernels.hpp"
@@ -207,7 +209,7 @@ __device__ inline void mma_elem(const MmaArrays& A, const MmaRaw<M>& r, MmaElem<
 // f(j, raw) over this block's elements in the order of a plain stride loop.
 template <int M, typename F>
 __device__ inline void mma_for_each(const MmaArrays& A, F&& f) {
-  const BlockRange br = block_range(A.n);
+  const BlockRange br = stream_range(A.n);
   for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
     MmaRaw<M> r;
     mma_fetch<M>(A, j, r);
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_mma_prep(MmaArrays A, int itn) {
 #pragma unroll
   for (int i = 0; i < M; i++) bs[i] = 0.0;
   double dummy[1] = {-1e300};
-  const BlockRange br = block_range(A.n);
+  const BlockRange br = stream_range(A.n);
   for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
     const double x = (double)A.xvar[j];
     const double xmn = A.xmin ? (double)A.xmin[j] : A.xmin_u;
@@ -591,7 +593,7 @@ __global__ void k_mma_epsi(MmaSmall* S) { S->epsi *= 0.1; }
 // xold2 = xold1, xold1 = x (mmaOpt.cu:1055-1057); the subsolver solution
 // goes to the caller's FP32 design
 __global__ void __launch_bounds__(kBlock) k_mma_finish(MmaArrays A, float* xout) {
-  const long long j = gtid();
+  const long long j = stid();
   if (j >= A.n) return;
   A.xold2[j] = A.xold1[j];
   A.xold1[j] = A.xvar[j];
@@ -629,10 +631,31 @@ void mma_prof_release(MmaProf& P) {
   P.pool.clear();
 }
 
+// Workgroups of a streaming pass: as many as the device holds at once at
+// the kernel's occupancy (its register count decides it), so that every CU
+// keeps that many waves' loads in flight through the whole pass; at most
+// one per kBlock elements and kMmaMaxBlocks partial-sum slots.
+template <typename K>
+static unsigned mma_grid(K kernel, long long n) {
+  int dev = 0, cus = 0, per_cu = 0;
+  long long b = 1024;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) == hipSuccess && per_cu > 0 &&
+      cus > 0)
+    b = (long long)per_cu * cus;
+  b = std::min<long long>(b, kMmaMaxBlocks);
+  b = std::min<long long>(b, (n + kBlock - 1) / kBlock);
+  return (unsigned)std::max<long long>(b, 1);
+}
+
 template <int M>
 static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const double* gval_dev, int itn,
                    double a0, double av, double cv, double dv, float* xout, double epsimin, MmaProf* prof) {
-  const unsigned nb = red_blocks_for_mma(A.n);
+  const unsigned nb_prep = mma_grid(k_mma_prep<M>, A.n);
+  const unsigned nb_point = mma_grid(k_mma_point<M, false>, A.n);
+  const unsigned nb_trial = mma_grid(k_mma_point<M, true>, A.n);
+  const unsigned nb_dir = mma_grid(k_mma_dir<M>, A.n);
   auto read = [&]() -> int {
     if (hipMemcpyAsync(Sh, S, sizeof(MmaSmall), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     if (hipStreamSynchronize(s) != hipSuccess) return -1;
@@ -666,17 +689,18 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
         P->pool.push_back(a);
     }
   };
-  hipLaunchKernelGGL(k_mma_prep<M>, dim3(nb), dim3(kBlock), 0, s, A, itn);
-  hipLaunchKernelGGL(k_mma_prep_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, gval_dev, (int)nb, a0, av, cv, dv);
+  hipLaunchKernelGGL(k_mma_prep<M>, dim3(nb_prep), dim3(kBlock), 0, s, A, itn);
+  hipLaunchKernelGGL(k_mma_prep_final<M>, dim3(1), dim3(kBlock), 0, s, A, S, gval_dev, (int)nb_prep, a0, av, cv,
+                     dv);
   int newton = 0;
   double epsi = 1.0;
   while (epsi > epsimin) {
     // residual and Newton system at the start point of this epsi level
     {
       Timed t(prof, 2, s, bytes_of);
-      hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb), dim3(kBlock), 0, s, A, S);
+      hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb_point), dim3(kBlock), 0, s, A, S);
     }
-    hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
+    hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point);
     int rc = read();
     if (rc) return rc;
     int ittt = 0;
@@ -686,15 +710,15 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
       hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(1), 0, s, S);
       {
         Timed t(prof, 1, s, bytes_of);
-        hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb), dim3(kBlock), 0, s, A, S);
+        hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb_dir), dim3(kBlock), 0, s, A, S);
       }
-      hipLaunchKernelGGL(k_mma_dir_final, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
+      hipLaunchKernelGGL(k_mma_dir_final, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_dir);
       do {
         {
           Timed t(prof, 0, s, bytes_of);
-          hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb), dim3(kBlock), 0, s, A, S);
+          hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb_trial), dim3(kBlock), 0, s, A, S);
         }
-        hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb);
+        hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_trial);
         rc = read();
         if (rc) return rc;
       } while (!Sh->accept);

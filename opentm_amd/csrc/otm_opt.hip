@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_tile(Geom g, Taps T, int h, f
 __global__ void __launch_bounds__(kBlock) k_simp_fwd(long long n, const float* __restrict__ rho, float k0, float k1,
                                                      float floor_, float* __restrict__ kappa,
                                                      float* __restrict__ dkappa) {
-  const long long t = gtid();
+  const long long t = stid();
   if (t >= n) return;
   const float p = rho[t];
   const float d = k0 - k1;
@@ -126,14 +126,14 @@ __global__ void __launch_bounds__(kBlock) k_simp_fwd(long long n, const float* _
 
 __global__ void __launch_bounds__(kBlock) k_mul(long long n, const float* __restrict__ a, const float* __restrict__ b,
                                                 float* __restrict__ out) {
-  const long long t = gtid();
+  const long long t = stid();
   if (t >= n) return;
   out[t] = a[t] * b[t];
 }
 
 __global__ void __launch_bounds__(kBlock) k_maxabs(long long n, const float* __restrict__ a, unsigned int* out_bits) {
   float m = 0.f;
-  for (long long i = block_range(n).b + threadIdx.x, i_end = block_range(n).e; i < i_end; i += blockDim.x)
+  for (long long i = stream_range(n).b + threadIdx.x, i_end = stream_range(n).e; i < i_end; i += blockDim.x)
     m = fmaxf(m, fabsf(a[i]));
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
@@ -258,7 +258,7 @@ __device__ inline void oc_block_partials(const float (&accf)[kOcCand], const Red
     double sum = 0.0;
 #pragma unroll
     for (int q = 0; q < kGroups; q++) sum += part[q][threadIdx.x];
-    R.partials[(size_t)xcd_block() * kOcCand + threadIdx.x] = sum;
+    R.partials[(size_t)blockIdx.x * kOcCand + threadIdx.x] = sum;
   }
 }
 
@@ -278,7 +278,7 @@ __device__ inline void oc_tree_sum_body(long long n, const float* __restrict__ s
     kc[c] = sk[c];
     accf[c] = 0.f;
   }
-  const BlockRange rg = block_range(n);
+  const BlockRange rg = stream_range(n);
   long long i = rg.b + threadIdx.x;
   for (; i + 3 * kBlock < rg.e; i += 4 * kBlock) {
     float sv[4], r[4];
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_oc_apply(long long n, const float* _
   }
   if (!apply) return;
   const float lg = __builtin_amdgcn_logf(st->g);
-  for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = stid(); i < n; i += (long long)gridDim.x * blockDim.x) {
     const float sv = sens[i];
     rho[i] = oc_rule_log(sv, __builtin_amdgcn_logf(-sv), rho[i], lg, c01, min_rho, step, damp);
   }
@@ -729,7 +729,7 @@ __global__ void __launch_bounds__(kBlock) k_field_max(long long n, const float* 
                                                       double* __restrict__ part) {
   __shared__ double sh[kWaves];
   double m = -INFINITY;
-  const BlockRange br = block_range(n);
+  const BlockRange br = stream_range(n);
   for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) m = fmax(m, (double)x[i]);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
@@ -737,7 +737,7 @@ __global__ void __launch_bounds__(kBlock) k_field_max(long long n, const float* 
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kWaves; w++) m = fmax(m, sh[w]);
-    part[xcd_block()] = m;
+    part[(int)blockIdx.x] = m;
   }
 }
 
@@ -746,13 +746,13 @@ __global__ void __launch_bounds__(kBlock) k_field_sum(long long n, const float* 
                                                       double shift, double* __restrict__ part) {
   __shared__ double sh[kWaves];
   double v[1] = {0.0};
-  const BlockRange br = block_range(n);
+  const BlockRange br = stream_range(n);
   for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
     const double xi = (double)x[i];
     v[0] += KIND == kReduceSum ? xi : (KIND == kReducePnorm ? pow(xi, p) : exp(xi - shift));
   }
   block_sum<1>(v, sh);
-  if (threadIdx.x == 0) part[xcd_block()] = v[0];
+  if (threadIdx.x == 0) part[(int)blockIdx.x] = v[0];
 }
 
 // grad_i = upstream * d value / d x_i: 1, (x_i / value)^(p-1), exp(x_i - value)
@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(kBlock) k_field_sum(long long n, const float* 
 template <int KIND>
 __global__ void __launch_bounds__(kBlock) k_field_grad(long long n, const float* __restrict__ x, double p,
                                                        double value, double upstream, float* __restrict__ g) {
-  const long long i = gtid();
+  const long long i = stid();
   if (i >= n) return;
   const double xi = (double)x[i];
   const double d = KIND == kReduceSum ? 1.0

@@ -7,32 +7,32 @@ namespace otm {
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 __global__ void __launch_bounds__(kBlock) k_fill(float* a, long long n, float v) {
-  fill_body(a, n, v, xcd_block(), gridDim.x);
+  fill_body(a, n, v, (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_dot3(long long nn, const float* __restrict__ a,
                                                  const float* __restrict__ b, Reduce R) {
-  dot3_body(nn, a, b, R, xcd_block(), gridDim.x);
+  dot3_body(nn, a, b, R, (int)blockIdx.x, gridDim.x);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sum3(long long nn, const T* __restrict__ a, Reduce R) {
-  sum3_body<T>(nn, a, R, xcd_block(), gridDim.x);
+  sum3_body<T>(nn, a, R, (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_refine_add(long long n, double* __restrict__ x64, float* __restrict__ d) {
-  refine_add_body(n, x64, d, xcd_block(), gridDim.x);
+  refine_add_body(n, x64, d, (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean_cast(long long nn, double* __restrict__ x64,
                                                           float* __restrict__ x32, const PcgScalars* __restrict__ P) {
-  sub_mean_cast_body(nn, x64, x32, P, xcd_block(), gridDim.x);
+  sub_mean_cast_body(nn, x64, x32, P, (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_sum2(long long nn, const float* __restrict__ a, const float* __restrict__ b,
                                                  Reduce R) {
   double acc[2] = {0, 0};
-  const BlockRange br = block_range(nn);
+  const BlockRange br = stream_range(nn);
   for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
     acc[0] += (double)a[i];
     acc[1] += (double)b[i];
@@ -41,25 +41,25 @@ __global__ void __launch_bounds__(kBlock) k_sum2(long long nn, const float* __re
 }
 
 __global__ void __launch_bounds__(kBlock) k_sub_mean3(long long nn, float* a, const PcgScalars* __restrict__ P) {
-  sub_mean3_body(nn, a, P, xcd_block(), gridDim.x);
+  sub_mean3_body(nn, a, P, (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_xr(long long nn, float* __restrict__ x, float* __restrict__ xlo,
                                                    float* __restrict__ r, const float* __restrict__ p,
                                                    const float* __restrict__ q, const PcgScalars* __restrict__ P,
                                                    Reduce R) {
-  pcg_xr_body(nn, x, xlo, r, p, q, P, R, xcd_block(), gridDim.x);
+  pcg_xr_body(nn, x, xlo, r, p, q, P, R, (int)blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_p(long long nn, float* __restrict__ p, const float* __restrict__ z,
                                                   const PcgScalars* __restrict__ P, int first) {
-  pcg_p_body(nn, p, z, P, xcd_block(), gridDim.x, first != 0);
+  pcg_p_body(nn, p, z, P, (int)blockIdx.x, gridDim.x, first != 0);
 }
 
 // dst[v] = src[x + pitch*(y + ny*z)] + add (pitched lexicographic input)
 __global__ void __launch_bounds__(kBlock) k_copy_pitched(Geom g, const float* __restrict__ src, int pitch,
                                                          float* __restrict__ dst, float add) {
-  const long long t = gtid();
+  const long long t = stid();
   if (t >= g.nn) return;
   const long long x = t % g.nx, r = t / g.nx;
   dst[t] = src[x + (long long)pitch * r] + add;
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_pitched(Geom g, const float* __
 
 __global__ void __launch_bounds__(kBlock) k_copy_to_pitched(Geom g, const float* __restrict__ src,
                                                             float* __restrict__ dst, int pitch) {
-  const long long t = gtid();
+  const long long t = stid();
   if (t >= g.nn) return;
   const long long x = t % g.nx, r = t / g.nx;
   dst[x + (long long)pitch * r] = src[t];
@@ -84,7 +84,7 @@ void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblock
 // x += d (x an FP32 pair when xlo is given), d cleared.
 __global__ void __launch_bounds__(kBlock) k_add_pair(long long n, float* __restrict__ x, float* __restrict__ xlo,
                                                      float* __restrict__ d) {
-  for (long long i = gtid(); i < n; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = stid(); i < n; i += (long long)gridDim.x * blockDim.x) {
     if (xlo) {
       float hi = x[i], lo = xlo[i];
       pair_add_prod(hi, lo, 1.f, d[i]);
@@ -103,11 +103,11 @@ void launch_add_pair(hipStream_t s, long long n, float* x, float* xlo, float* d)
 }
 
 __global__ void __launch_bounds__(kBlock) k_f2d(long long n, const float* __restrict__ in, double* __restrict__ out) {
-  const long long t = gtid();
+  const long long t = stid();
   if (t < n) out[t] = (double)in[t];
 }
 __global__ void __launch_bounds__(kBlock) k_fill_d(double* a, long long n, double v) {
-  const long long t = gtid();
+  const long long t = stid();
   if (t < n) a[t] = v;
 }
 void launch_f2d(hipStream_t s, long long n, const float* in, double* out) {

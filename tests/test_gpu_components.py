@@ -222,3 +222,40 @@ def test_autotune_picks_a_candidate_that_matches_the_oracle(orc):
     assert rel(hom.heatMatrix(), Co) < 1e-5
     for h in (hom, pinned):
         h.close()
+
+
+@pytest.mark.parametrize("refine", [0, 1])
+@pytest.mark.parametrize("reso", [(24, 8, 12), (40, 16, 8), (256, 8, 8), (384, 8, 8), (1000, 16, 16)])
+def test_start_residual_norms_match_oracle(orc, reso, refine):
+    """The FP64 start residual r = b - K x that every solve and restart
+    begins from (by whole x-rows for rows of up to 256 nodes: 24 to 256
+    here, one to ten row groups per workgroup, segments that cross z-planes;
+    pointwise for the 384- and 1000-node rows): the relative norms
+    ||r_k|| / ||b_k|| that otm_mg_cycles reports before its cycle equal the
+    oracle's assembly applied to the same iterate, from the FP32 iterate
+    (plain form) and from its FP64 copy (refinement form). The iterate is
+    scaled so that K x and b are of the same size and both terms count."""
+    say("start residual", reso, "refine", refine)
+    rng = np.random.default_rng(26)
+    nx, ny, nz = reso
+    shape = (nz, ny, nx)
+    n = nx * ny * nz
+    k = rng.uniform(1e-3, 1.0, shape).astype(np.float32)
+    hom = make(reso, fp64_refine=refine)
+    hom.update(k)
+    keff = (k + np.float32(1e-9)).astype(np.float64)  # the context's floor, added in FP32
+    A = orc.assemble(keff)
+    u = rng.standard_normal((3, n))
+    b = [orc.rhs(keff, c) for c in range(3)]
+    for c in range(3):
+        u[c] *= np.linalg.norm(b[c]) / np.linalg.norm(orc.matvec(shape, A, u[c]))
+    u = u.astype(np.float32)
+    hom.set_chi(u)
+    got = hom.mg_cycles(1)
+    for c in range(3):
+        r = b[c] - orc.matvec(shape, A, u[c].astype(np.float64))
+        want = np.linalg.norm(r) / np.linalg.norm(b[c])
+        say("  case", c, "relres", got[c], "oracle", want)
+        assert 0.5 < want < 3.0
+        assert abs(got[c] - want) < 1e-6 * want
+    hom.close()
