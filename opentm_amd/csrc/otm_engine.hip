@@ -2027,7 +2027,8 @@ struct otm_mma {
   MmaArrays A;
   double* gval = nullptr;
   void* small = nullptr;
-  void* small_host = nullptr;  // pinned
+  MmaHost* mh = nullptr;       // coherent pinned block the finalisers publish to
+  MmaHost* mh_dev = nullptr;   // its device address
   float* fx = nullptr;         // design, previous designs, gradients (FP32): (5 + m) n floats
   float* fbounds = nullptr;    // per-element bounds (2n floats), once set
   float* xout = nullptr;       // FP32 result staging for host callers
@@ -2045,7 +2046,7 @@ static void mma_free(otm_mma* o) {
   dfree(o->fbounds);
   dfree(o->xout);
   mma_prof_release(o->prof);
-  if (o->small_host) (void)hipHostFree(o->small_host);
+  if (o->mh) (void)hipHostFree(o->mh);
   if (o->s) (void)hipStreamDestroy(o->s);
   delete o;
 }
@@ -2091,7 +2092,9 @@ OTM_API int otm_mma_create(int m, int n, double a0, double a, double c, double d
       o->gval = dalloc<double>(kMmaMaxM);
       o->small = dalloc<char>(mma_small_bytes());
       o->xout = dalloc<float>(n);
-      hip_check(hipHostMalloc(&o->small_host, mma_small_bytes(), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&o->mh, sizeof(MmaHost), hipHostMallocCoherent), "hipHostMalloc");
+      memset(o->mh, 0, sizeof(MmaHost));
+      hip_check(hipHostGetDevicePointer((void**)&o->mh_dev, o->mh, 0), "hipHostGetDevicePointer");
       launch_fill(o->s, A.xold1, 2 * n, 0.f);
       hip_check(hipStreamSynchronize(o->s), "sync");
     } catch (...) {
@@ -2160,7 +2163,8 @@ OTM_API int otm_mma_update(otm_mma* o, int itn, float* x, const float* dfdx, con
     hip_check(hipMemcpyAsync(o->gval, gval, sizeof(double) * o->m, hipMemcpyHostToDevice, o->s), "gval");
     float* xout = where == OTM_DEVICE ? x : o->xout;
     const int rc =
-        mma_update(o->s, o->m, o->A, o->small, o->small_host, o->gval, itn, o->a0, o->a, o->c, o->d, xout, &o->prof);
+        mma_update(o->s, o->m, o->A, o->small, o->mh, o->mh_dev, o->gval, itn, o->a0, o->a, o->c, o->d, xout,
+                   &o->prof);
     if (rc == -2) fail(OTM_E_NUMERIC, "MMA subsolver produced non-finite values");
     if (rc < 0) fail(OTM_E_HIP, "MMA update failed");
     if (where == OTM_HOST)

@@ -200,10 +200,20 @@ struct MmaProf {
   std::vector<hipEvent_t> pool;
 };
 void mma_prof_release(MmaProf& P);
+// What the host loop of an MMA update needs after each point pass, in
+// coherent pinned host memory: the pass's finaliser writes it and then its
+// tag, and the host polls the tag (no copy, no stream synchronisation).
+struct MmaHost {
+  double residumax, rmaxnew;
+  int accept, fault;
+  unsigned seq;
+};
 // One MMA update on device arrays; returns Newton steps or a negative code
-// (-1 HIP error, -2 non-finite values, -3 unsupported m).
-int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
-               int itn, double a0, double av, double cv, double dv, float* xout, MmaProf* prof = nullptr);
+// (-1 HIP error, -2 non-finite values, -3 unsupported m). mh is the host
+// block (host address), mh_dev its device address.
+int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, MmaHost* mh, MmaHost* mh_dev,
+               const double* gval_dev, int itn, double a0, double av, double cv, double dv, float* xout,
+               MmaProf* prof = nullptr);
 void launch_f2d(hipStream_t s, long long n, const float* in, double* out);
 void launch_fill_d(hipStream_t s, double* a, long long n, double v);
 

@@ -418,8 +418,17 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
   block_sum_max<NS, 1>(sums, mx, A.partials);
 }
 
+// The host loop's inputs to the host block, then the tag (one thread).
+__device__ inline void mma_publish(const MmaSmall* S, MmaHost* mh, unsigned tag) {
+  mh->residumax = S->residumax;
+  mh->rmaxnew = S->rmaxnew;
+  mh->accept = S->accept;
+  mh->fault = S->fault;
+  __hip_atomic_store(&mh->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <int M, bool TRIAL>
-__global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmall* S, int nblocks) {
+__device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nblocks) {
   constexpr int NS = MmaPointNV<M>::NS, NN = MmaPointNV<M>::NN;
   __shared__ double out[NS + 1];
   final_sum_max<NS, 1>(A.partials, nblocks, out);
@@ -453,8 +462,17 @@ __global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmal
   S->steg = steg / 2;
 }
 
-// After acceptance the trial becomes the current point, with its Newton sums.
-__global__ void k_mma_commit(MmaSmall* S) {
+template <int M, bool TRIAL>
+__global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmall* S, int nblocks, MmaHost* mh,
+                                                            unsigned tag) {
+  mma_point_final_body<M, TRIAL>(A, S, nblocks);
+  if (threadIdx.x == 0) mma_publish(S, mh, tag);
+}
+
+// After acceptance the trial becomes the current point, with its Newton
+// sums; done by the kernel that follows the accepted trial (the next Newton
+// system or the next barrier level).
+__device__ inline void mma_commit(MmaSmall* S) {
   for (int i = 0; i < S->m; i++) {
     S->y[i] = S->yt[i];
     S->lam[i] = S->lamt[i];
@@ -472,7 +490,8 @@ __global__ void k_mma_commit(MmaSmall* S) {
 // Solves [Alam a; a^T -zet/z] [dlam; dz] = [blam; delz] from the point's
 // sums and the small directions (mmaOpt.cu:595-678). One thread.
 template <int M>
-__global__ void k_mma_newton_final(MmaSmall* S) {
+__global__ void k_mma_newton_final(MmaSmall* S, int commit) {
+  if (commit) mma_commit(S);
   const double epsi = S->epsi;
   const double* gvec = S->ns;
   const double* GGdd = S->ns + M;
@@ -588,7 +607,10 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir_final(MmaArrays A, MmaSmall*
   S->accept = 0;
 }
 
-__global__ void k_mma_epsi(MmaSmall* S) { S->epsi *= 0.1; }
+__global__ void k_mma_epsi(MmaSmall* S, int commit) {
+  if (commit) mma_commit(S);
+  S->epsi *= 0.1;
+}
 
 // xold2 = xold1, xold1 = x (mmaOpt.cu:1055-1057); the subsolver solution
 // goes to the caller's FP32 design
@@ -615,7 +637,7 @@ static hipEvent_t prof_event(MmaProf& P) {
 static void prof_collect(MmaProf& P) {
   for (const auto& q : P.pending) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) P.ms[q.cls] += ms;
+    if (hipEventSynchronize(q.b) == hipSuccess && hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) P.ms[q.cls] += ms;
     P.pool.push_back(q.a);
     P.pool.push_back(q.b);
   }
@@ -649,18 +671,34 @@ static unsigned mma_grid(K kernel, long long n) {
   return (unsigned)std::max<long long>(b, 1);
 }
 
+// Polls the host block for a tag; -1 when the stream finished without it
+// or failed.
+static int mma_wait(hipStream_t s, const MmaHost* mh, unsigned tag) {
+  unsigned spins = 0;
+  while (__atomic_load_n(&mh->seq, __ATOMIC_ACQUIRE) != tag) {
+    if ((++spins & 255u) == 0) {
+      const hipError_t e = hipStreamQuery(s);
+      if (e == hipSuccess) return __atomic_load_n(&mh->seq, __ATOMIC_ACQUIRE) == tag ? 0 : -1;
+      if (e != hipErrorNotReady) return -1;
+    }
+    __builtin_ia32_pause();
+  }
+  return 0;
+}
+
 template <int M>
-static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const double* gval_dev, int itn,
-                   double a0, double av, double cv, double dv, float* xout, double epsimin, MmaProf* prof) {
+static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost* mh_dev, const double* gval_dev,
+                   int itn, double a0, double av, double cv, double dv, float* xout, double epsimin, MmaProf* prof) {
   const unsigned nb_prep = mma_grid(k_mma_prep<M>, A.n);
   const unsigned nb_point = mma_grid(k_mma_point<M, false>, A.n);
   const unsigned nb_trial = mma_grid(k_mma_point<M, true>, A.n);
   const unsigned nb_dir = mma_grid(k_mma_dir<M>, A.n);
+  // no kernel of this optimiser is in flight: the block holds the last tag
+  unsigned tag = __atomic_load_n(&mh->seq, __ATOMIC_ACQUIRE);
   auto read = [&]() -> int {
-    if (hipMemcpyAsync(Sh, S, sizeof(MmaSmall), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    if (mma_wait(s, mh, tag) != 0) return -1;
     if (prof && prof->on) prof_collect(*prof);
-    return Sh->fault ? -2 : 0;
+    return mh->fault ? -2 : 0;
   };
   // algorithmic bytes per element: x, low/upp, gradients, interior point
   // (+ per-element bounds), and the trial point written
@@ -694,20 +732,24 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
                      dv);
   int newton = 0;
   double epsi = 1.0;
+  int commit = 0;  // an accepted trial waits to become the current point
   while (epsi > epsimin) {
     // residual and Newton system at the start point of this epsi level
     {
       Timed t(prof, 2, s, bytes_of);
       hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb_point), dim3(kBlock), 0, s, A, S);
     }
-    hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point);
+    hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point, mh_dev,
+                       ++tag);
     int rc = read();
     if (rc) return rc;
+    double residumax = mh->residumax;
     int ittt = 0;
-    while (Sh->residumax > 0.9 * epsi && ittt < 200) {
+    while (residumax > 0.9 * epsi && ittt < 200) {
       ittt++;
       newton++;
-      hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(1), 0, s, S);
+      hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(1), 0, s, S, commit);
+      commit = 0;
       {
         Timed t(prof, 1, s, bytes_of);
         hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb_dir), dim3(kBlock), 0, s, A, S);
@@ -718,17 +760,19 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
           Timed t(prof, 0, s, bytes_of);
           hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb_trial), dim3(kBlock), 0, s, A, S);
         }
-        hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_trial);
+        hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_trial, mh_dev,
+                           ++tag);
         rc = read();
         if (rc) return rc;
-      } while (!Sh->accept);
-      hipLaunchKernelGGL(k_mma_commit, dim3(1), dim3(1), 0, s, S);
+      } while (!mh->accept);
+      commit = 1;
       std::swap(A.xs, A.xs_t);
       std::swap(A.xsi, A.xsi_t);
       std::swap(A.eta, A.eta_t);
-      Sh->residumax = Sh->rmaxnew;
+      residumax = mh->rmaxnew;
     }
-    hipLaunchKernelGGL(k_mma_epsi, dim3(1), dim3(1), 0, s, S);
+    hipLaunchKernelGGL(k_mma_epsi, dim3(1), dim3(1), 0, s, S, commit);
+    commit = 0;
     epsi *= 0.1;
   }
   hipLaunchKernelGGL(k_mma_finish, dim3(nblk(A.n)), dim3(kBlock), 0, s, A, xout);
@@ -736,17 +780,17 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaSmall* Sh, const 
   return newton;
 }
 
-int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, void* small_host, const double* gval_dev,
-               int itn, double a0, double av, double cv, double dv, float* xout, MmaProf* prof) {
+int mma_update(hipStream_t s, int m, MmaArrays& A, void* small_dev, MmaHost* mh, MmaHost* mh_dev,
+               const double* gval_dev, int itn, double a0, double av, double cv, double dv, float* xout,
+               MmaProf* prof) {
   MmaSmall* S = (MmaSmall*)small_dev;
-  MmaSmall* Sh = (MmaSmall*)small_host;
   MmaArrays B = A;  // the driver swaps the subsolver buffers locally
   int rc;
   switch (m) {
-    case 1: rc = mma_run<1>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
-    case 2: rc = mma_run<2>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
-    case 3: rc = mma_run<3>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
-    case 4: rc = mma_run<4>(s, B, S, Sh, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 1: rc = mma_run<1>(s, B, S, mh, mh_dev, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 2: rc = mma_run<2>(s, B, S, mh, mh_dev, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 3: rc = mma_run<3>(s, B, S, mh, mh_dev, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
+    case 4: rc = mma_run<4>(s, B, S, mh, mh_dev, gval_dev, itn, a0, av, cv, dv, xout, 1e-7, prof); break;
     default: rc = -3;
   }
   return rc;
