@@ -276,12 +276,8 @@ struct OpFinal {  // one block per cell
     Reduce R = cell_reduce(C, mode);
     if (rtol2 >= 0.0) R.rtol2 = rtol2;
     const unsigned nb = (tiled && C.tile_bx) ? fine_apply_tile_blocks(C.g[0], C.tile_bx)
-                                              : red_blocks_for_dev(C.g[0].nn);
+                                              : red_blocks_for(C.g[0].nn);
     reduce_final_body(R, nv, (int)nb);
-  }
-  __device__ static unsigned red_blocks_for_dev(long long n) {
-    long long b = (n + kBlock - 1) / kBlock;
-    return (unsigned)(b > 1024 ? 1024 : (b < 1 ? 1 : b));
   }
 };
 

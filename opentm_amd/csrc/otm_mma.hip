@@ -43,6 +43,7 @@ struct MmaSmall {
   double dy[kMmaMaxM], dlam[kMmaMaxM], dmu[kMmaMaxM], ds[kMmaMaxM], dz, dzet;
   double yt[kMmaMaxM], lamt[kMmaMaxM], mut[kMmaMaxM], st[kMmaMaxM], zt, zett;
   double steg, stm_small;
+  double lamtr[kMmaMaxM];  // lam + steg dlam of the next trial (uniform for its pass)
   double residunorm, residumax, resinew, rmaxnew;
   // Newton-system sums at the current point and at the trial point:
   // gvec[m], GG.(delx/diagx)[m], upper triangle of GG diag(1/diagx) GG^T
@@ -153,13 +154,13 @@ struct MmaRaw {
 };
 
 template <int M>
-__device__ inline void mma_fetch(const MmaArrays& A, long long j, MmaRaw<M>& r) {
+__device__ inline void mma_fetch(const MmaArrays& A, unsigned j, MmaRaw<M>& r) {
   r.x = A.xvar[j];
   r.lo = A.low[j];
   r.up = A.upp[j];
   r.g0 = A.df0dx[j];
 #pragma unroll
-  for (int i = 0; i < M; i++) r.g[i] = A.dgdx[i * A.n + j];
+  for (int i = 0; i < M; i++) r.g[i] = A.dgdx[(size_t)i * (size_t)A.n + j];
   if (A.xmin) {
     r.xmn = (double)A.xmin[j];
     r.xmx = (double)A.xmax[j];
@@ -206,11 +207,14 @@ __device__ inline void mma_elem(const MmaArrays& A, const MmaRaw<M>& r, MmaElem<
   mma_coefficients<M>(A.move, (double)r.x, r.xmn, r.xmx, (double)r.g0, r.g, e);
 }
 
-// f(j, raw) over this block's elements in the order of a plain stride loop.
+// f(j, raw) over this block's elements in the order of a plain stride loop
+// (32-bit element index: a design has fewer than 2^31 elements, so one
+// register carries it through the loop).
 template <int M, typename F>
 __device__ inline void mma_for_each(const MmaArrays& A, F&& f) {
   const BlockRange br = stream_range(A.n);
-  for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
+  const unsigned e = (unsigned)br.e;
+  for (unsigned j = (unsigned)br.b + threadIdx.x; j < e; j += blockDim.x) {
     MmaRaw<M> r;
     mma_fetch<M>(A, j, r);
     f(j, r);
@@ -373,13 +377,13 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
   for (int i = 0; i < M; i++) {
     lamc[i] = S->lam[i];
     dlam[i] = S->dlam[i];
-    lam[i] = TRIAL ? lamc[i] + steg * dlam[i] : lamc[i];
+    lam[i] = TRIAL ? S->lamtr[i] : lamc[i];
   }
   double sums[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) sums[i] = 0.0;
   double mx[1] = {0.0};
-  mma_for_each<M>(A, [&](long long j, const MmaRaw<M>& r) {
+  mma_for_each<M>(A, [&](unsigned j, const MmaRaw<M>& r) {
     MmaElem<M> e;
     mma_elem<M>(A, r, e);
     double x = r.xs, xsi = r.xsi, eta = r.eta;
@@ -445,7 +449,7 @@ __device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nbloc
   const double steg = S->steg;
   for (int i = 0; i < M; i++) {
     S->yt[i] = S->y[i] + steg * S->dy[i];
-    S->lamt[i] = S->lam[i] + steg * S->dlam[i];
+    S->lamt[i] = S->lamtr[i];  // the multipliers the pass used
     S->mut[i] = S->mu[i] + steg * S->dmu[i];
     S->st[i] = S->s[i] + steg * S->ds[i];
   }
@@ -460,6 +464,7 @@ __device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nbloc
   S->accept = (!(S->resinew > S->residunorm) || S->itto >= 50) ? 1 : 0;
   if (!(S->resinew - S->resinew == 0.0)) S->fault = 1;
   S->steg = steg / 2;
+  for (int i = 0; i < M; i++) S->lamtr[i] = S->lam[i] + S->steg * S->dlam[i];
 }
 
 template <int M, bool TRIAL>
@@ -581,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir(MmaArrays A, const MmaSmall*
   const double epsi = S->epsi;
   double dummy[1] = {0.0};
   double mx[1] = {-1e300};
-  mma_for_each<M>(A, [&](long long, const MmaRaw<M>& r) {
+  mma_for_each<M>(A, [&](unsigned, const MmaRaw<M>& r) {
     MmaElem<M> e;
     mma_elem<M>(A, r, e);
     const double x = r.xs, xsi = r.xsi, eta = r.eta;
@@ -603,6 +608,7 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir_final(MmaArrays A, MmaSmall*
   if (threadIdx.x != 0) return;
   const double stm = fmax(S->stm_small, out[1]);
   S->steg = 1.0 / stm;
+  for (int i = 0; i < S->m; i++) S->lamtr[i] = S->lam[i] + S->steg * S->dlam[i];
   S->itto = 0;
   S->accept = 0;
 }
