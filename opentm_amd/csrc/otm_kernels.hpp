@@ -204,7 +204,7 @@ void mma_prof_release(MmaProf& P);
 // coherent pinned host memory: the pass's finaliser writes it and then its
 // tag, and the host polls the tag (no copy, no stream synchronisation).
 struct MmaHost {
-  double residumax, rmaxnew;
+  double residumax, rmaxnew, rmaxnext;
   int accept, fault;
   unsigned seq;
 };

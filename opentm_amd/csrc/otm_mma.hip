@@ -48,6 +48,9 @@ struct MmaSmall {
   // Newton-system sums at the current point and at the trial point:
   // gvec[m], GG.(delx/diagx)[m], upper triangle of GG diag(1/diagx) GG^T
   double ns[kMmaMaxNS], ns_t[kMmaMaxNS];
+  // the trial point's residual and Newton sums at the next barrier level
+  // (epsi / 10): the start of that level when the trial ends this one
+  double ns_n[kMmaMaxNS], resn, rmaxn;
 };
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -230,6 +233,13 @@ struct MmaNewtonElem {
   double dpsidx, delx, rdiagx, GG[M];
 };
 
+// The one term of the Newton system that depends on the barrier epsi.
+template <int M>
+__device__ inline double mma_delx(const MmaNewtonElem<M>& q, double epsi) {
+#pragma clang fp contract(off)
+  return q.dpsidx - epsi * q.rxa + epsi * q.rbx;
+}
+
 template <int M>
 __device__ inline void mma_newton_elem(const MmaElem<M>& e, double x, double xsi, double eta, const double (&lam)[M],
                                        double epsi, MmaNewtonElem<M>& q) {
@@ -247,7 +257,7 @@ __device__ inline void mma_newton_elem(const MmaElem<M>& e, double x, double xsi
     q.GG[i] = e.P[i] * rux2 - e.Q[i] * rxl2;
   }
   q.dpsidx = plam * rux2 - qlam * rxl2;
-  q.delx = q.dpsidx - epsi * q.rxa + epsi * q.rbx;
+  q.delx = mma_delx<M>(q, epsi);
   const double diagx = 2.0 * (plam * (rux2 * q.rux1) + qlam * (rxl2 * q.rxl1)) + xsi * q.rxa + eta * q.rbx;
   q.rdiagx = rcp64(diagx);
 }
@@ -336,9 +346,8 @@ __global__ void __launch_bounds__(kBlock) k_mma_prep_final(MmaArrays A, MmaSmall
 // Small-vector part of the residual (mmaOpt.cu:514-522); adds to ss / mx.
 __device__ inline void small_residual(const MmaSmall* S, const double* gvec, const double* y, double z,
                                       const double* lam, const double* mu, double zet, const double* s, double& ss,
-                                      double& mx) {
+                                      double& mx, double epsi) {
   const int m = S->m;
-  const double epsi = S->epsi;
   double alam = 0.0;
   for (int i = 0; i < m; i++) alam += S->a[i] * lam[i];
   const double rez = S->a0 - zet - alam, rezet = zet * z - epsi;
@@ -357,32 +366,40 @@ __device__ inline void small_residual(const MmaSmall* S, const double* gvec, con
 // Sums of one point pass: [|residual|^2, gvec[M], GG.(delx/diagx)[M],
 // upper triangle of GG diag(1/diagx) GG^T]; gvec is shared by the residual
 // and the Newton system.
+// A trial pass also sums, at the next barrier level epsi / 10, the residual
+// and GG.(delx/diagx) (NX more sums, one more maximum): when the trial
+// ends its level, these with the epsi-free sums are the next level's start
+// and its point pass is not needed.
 template <int M>
 struct MmaPointNV {
   static constexpr int NN = 2 * M + (M * (M + 1)) / 2;  // Newton-system sums
   static constexpr int NS = 1 + NN;
+  static constexpr int NX = 1 + M;
 };
 static_assert(MmaPointNV<kMmaMaxM>::NN <= kMmaMaxNS, "Newton sums exceed their slots");
-static_assert(MmaPointNV<kMmaMaxM>::NS + 1 <= 32, "point pass partials exceed their slots");
+static_assert(MmaPointNV<kMmaMaxM>::NS + MmaPointNV<kMmaMaxM>::NX + 2 <= 32, "point pass partials exceed their slots");
 
 // TRIAL: the point is the current one + steg * (Newton direction); the
 // direction is recomputed as in k_mma_dir, the point written to the trial
 // buffers and the small-vector trial values are formed by the finaliser.
 template <int M, bool TRIAL>
-__global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmall* __restrict__ S) {
-  constexpr int NS = MmaPointNV<M>::NS;
+__device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__ S) {
+  constexpr int NS = MmaPointNV<M>::NS, NX = TRIAL ? MmaPointNV<M>::NX : 0, NM = TRIAL ? 2 : 1;
   double lam[M], lamc[M], dlam[M];
   const double epsi = S->epsi, steg = S->steg;
+  const double epsi2 = epsi * 0.1;  // the next level's, as k_mma_epsi forms it
 #pragma unroll
   for (int i = 0; i < M; i++) {
     lamc[i] = S->lam[i];
     dlam[i] = S->dlam[i];
     lam[i] = TRIAL ? S->lamtr[i] : lamc[i];
   }
-  double sums[NS];
+  double sums[NS + NX];
 #pragma unroll
-  for (int i = 0; i < NS; i++) sums[i] = 0.0;
-  double mx[1] = {0.0};
+  for (int i = 0; i < NS + NX; i++) sums[i] = 0.0;
+  double mx[NM];
+#pragma unroll
+  for (int i = 0; i < NM; i++) mx[i] = 0.0;
   mma_for_each<M>(A, [&](unsigned j, const MmaRaw<M>& r) {
     MmaElem<M> e;
     mma_elem<M>(A, r, e);
@@ -418,14 +435,42 @@ __global__ void __launch_bounds__(kBlock) k_mma_point(MmaArrays A, const MmaSmal
 #pragma unroll
       for (int k = i; k < M; k++) sums[t++] += q.GG[i] * q.GG[k] * inv;
     }
+    if (TRIAL) {
+      const double rexsi2 = xsi * (x - e.al) - epsi2;
+      const double reeta2 = eta * (e.be - x) - epsi2;
+      sums[NS] += rex * rex + rexsi2 * rexsi2 + reeta2 * reeta2;
+      mx[NM - 1] = fmax(mx[NM - 1], fmax(fabs(rex), fmax(fabs(rexsi2), fabs(reeta2))));
+      const double delx2 = mma_delx<M>(q, epsi2);
+#pragma unroll
+      for (int i = 0; i < M; i++) sums[NS + 1 + i] += q.GG[i] * delx2 * inv;
+    }
   });
-  block_sum_max<NS, 1>(sums, mx, A.partials);
+  block_sum_max<NS + NX, NM>(sums, mx, A.partials);
+}
+
+// Up to m = 3 the compiler is asked for five waves per SIMD (the trial pass
+// fits 96 VGPRs without spilling; left alone it takes 100 and four waves);
+// at m = 4 that would spill, so the kernel has no such request there.
+template <int M, bool TRIAL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_mma_point(MmaArrays A, const MmaSmall* __restrict__ S) {
+  mma_point_body<M, TRIAL>(A, S);
+}
+template <int M, bool TRIAL>
+__global__ void __launch_bounds__(kBlock) k_mma_point_wide(MmaArrays A, const MmaSmall* __restrict__ S) {
+  mma_point_body<M, TRIAL>(A, S);
+}
+template <int M, bool TRIAL>
+static constexpr auto mma_point_kernel() {
+  if constexpr (M <= 3) return k_mma_point<M, TRIAL>;
+  else return k_mma_point_wide<M, TRIAL>;
 }
 
 // The host loop's inputs to the host block, then the tag (one thread).
 __device__ inline void mma_publish(const MmaSmall* S, MmaHost* mh, unsigned tag) {
   mh->residumax = S->residumax;
   mh->rmaxnew = S->rmaxnew;
+  mh->rmaxnext = S->rmaxn;
   mh->accept = S->accept;
   mh->fault = S->fault;
   __hip_atomic_store(&mh->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -434,14 +479,15 @@ __device__ inline void mma_publish(const MmaSmall* S, MmaHost* mh, unsigned tag)
 template <int M, bool TRIAL>
 __device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nblocks) {
   constexpr int NS = MmaPointNV<M>::NS, NN = MmaPointNV<M>::NN;
-  __shared__ double out[NS + 1];
-  final_sum_max<NS, 1>(A.partials, nblocks, out);
+  constexpr int NX = TRIAL ? MmaPointNV<M>::NX : 0, NM = TRIAL ? 2 : 1;
+  __shared__ double out[NS + NX + NM];
+  final_sum_max<NS + NX, NM>(A.partials, nblocks, out);
   if (threadIdx.x != 0) return;
-  double ss = out[0], mx = out[NS];
+  double ss = out[0], mx = out[NS + NX];
   double* ns = TRIAL ? S->ns_t : S->ns;
   for (int i = 0; i < NN; i++) ns[i] = out[1 + i];
   if (!TRIAL) {
-    small_residual(S, out + 1, S->y, S->z, S->lam, S->mu, S->zet, S->s, ss, mx);
+    small_residual(S, out + 1, S->y, S->z, S->lam, S->mu, S->zet, S->s, ss, mx, S->epsi);
     S->residunorm = sqrt(ss);
     S->residumax = mx;
     return;
@@ -455,9 +501,17 @@ __device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nbloc
   }
   S->zt = S->z + steg * S->dz;
   S->zett = S->zet + steg * S->dzet;
-  small_residual(S, out + 1, S->yt, S->zt, S->lamt, S->mut, S->zett, S->st, ss, mx);
+  small_residual(S, out + 1, S->yt, S->zt, S->lamt, S->mut, S->zett, S->st, ss, mx, S->epsi);
   S->resinew = sqrt(ss);
   S->rmaxnew = mx;
+  // the same point at the next barrier level (used if this trial ends the level)
+  const double epsi2 = S->epsi * 0.1;
+  double ss2 = out[NS], mx2 = out[NS + NX + 1];
+  for (int i = 0; i < NN; i++) S->ns_n[i] = out[1 + i];
+  for (int i = 0; i < M; i++) S->ns_n[M + i] = out[NS + 1 + i];
+  small_residual(S, out + 1, S->yt, S->zt, S->lamt, S->mut, S->zett, S->st, ss2, mx2, epsi2);
+  S->resn = sqrt(ss2);
+  S->rmaxn = mx2;
   S->itto += 1;
   // line search rule (mmaOpt.cu:718-805): accept when the norm does not
   // grow, or after 50 halvings
@@ -613,9 +667,17 @@ __global__ void __launch_bounds__(kBlock) k_mma_dir_final(MmaArrays A, MmaSmall*
   S->accept = 0;
 }
 
-__global__ void k_mma_epsi(MmaSmall* S, int commit) {
+// Next barrier level: epsi / 10; with spec, the accepted trial's sums at
+// that level become the current residual and Newton system (the level's
+// point pass is skipped).
+__global__ void k_mma_epsi(MmaSmall* S, int commit, int spec) {
   if (commit) mma_commit(S);
   S->epsi *= 0.1;
+  if (spec) {
+    for (int i = 0; i < kMmaMaxNS; i++) S->ns[i] = S->ns_n[i];
+    S->residunorm = S->resn;
+    S->residumax = S->rmaxn;
+  }
 }
 
 // xold2 = xold1, xold1 = x (mmaOpt.cu:1055-1057); the subsolver solution
@@ -696,8 +758,8 @@ template <int M>
 static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost* mh_dev, const double* gval_dev,
                    int itn, double a0, double av, double cv, double dv, float* xout, double epsimin, MmaProf* prof) {
   const unsigned nb_prep = mma_grid(k_mma_prep<M>, A.n);
-  const unsigned nb_point = mma_grid(k_mma_point<M, false>, A.n);
-  const unsigned nb_trial = mma_grid(k_mma_point<M, true>, A.n);
+  const unsigned nb_point = mma_grid(mma_point_kernel<M, false>(), A.n);
+  const unsigned nb_trial = mma_grid(mma_point_kernel<M, true>(), A.n);
   const unsigned nb_dir = mma_grid(k_mma_dir<M>, A.n);
   // no kernel of this optimiser is in flight: the block holds the last tag
   unsigned tag = __atomic_load_n(&mh->seq, __ATOMIC_ACQUIRE);
@@ -738,18 +800,27 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
                      dv);
   int newton = 0;
   double epsi = 1.0;
-  int commit = 0;  // an accepted trial waits to become the current point
+  int commit = 0;   // an accepted trial waits to become the current point
+  bool spec = false;  // that trial's sums at this level are known (its pass formed them)
+  double residumax_next = 0.0;
   while (epsi > epsimin) {
-    // residual and Newton system at the start point of this epsi level
-    {
-      Timed t(prof, 2, s, bytes_of);
-      hipLaunchKernelGGL((k_mma_point<M, false>), dim3(nb_point), dim3(kBlock), 0, s, A, S);
+    double residumax;
+    if (spec) {
+      residumax = residumax_next;
+    } else {
+      // residual and Newton system at the start point of this epsi level
+      {
+        Timed t(prof, 2, s, bytes_of);
+        hipLaunchKernelGGL((mma_point_kernel<M, false>()), dim3(nb_point), dim3(kBlock), 0, s, A, S);
+      }
+      hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point, mh_dev,
+                         ++tag);
+      const int rc = read();
+      if (rc) return rc;
+      residumax = mh->residumax;
     }
-    hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point, mh_dev,
-                       ++tag);
-    int rc = read();
-    if (rc) return rc;
-    double residumax = mh->residumax;
+    spec = false;
+    int rc = 0;
     int ittt = 0;
     while (residumax > 0.9 * epsi && ittt < 200) {
       ittt++;
@@ -764,7 +835,7 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
       do {
         {
           Timed t(prof, 0, s, bytes_of);
-          hipLaunchKernelGGL((k_mma_point<M, true>), dim3(nb_trial), dim3(kBlock), 0, s, A, S);
+          hipLaunchKernelGGL((mma_point_kernel<M, true>()), dim3(nb_trial), dim3(kBlock), 0, s, A, S);
         }
         hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_trial, mh_dev,
                            ++tag);
@@ -776,8 +847,10 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
       std::swap(A.xsi, A.xsi_t);
       std::swap(A.eta, A.eta_t);
       residumax = mh->rmaxnew;
+      residumax_next = mh->rmaxnext;
+      spec = true;
     }
-    hipLaunchKernelGGL(k_mma_epsi, dim3(1), dim3(1), 0, s, S, commit);
+    hipLaunchKernelGGL(k_mma_epsi, dim3(1), dim3(1), 0, s, S, commit, spec ? 1 : 0);
     commit = 0;
     epsi *= 0.1;
   }
