@@ -967,6 +967,7 @@ void plan_setup(otm_batch* b, std::vector<Member>& tmp, const std::vector<const 
 Member& add_member(otm_batch* b, otm_ctx* c, int slot, int out) {
   Member m;
   m.c = c;
+  c->energy_ok = false;  // the batched solve changes its iterate
   m.slot = slot;
   m.out = out;
   m.nlev = (int)c->lv.size();

@@ -177,6 +177,11 @@ struct otm_ctx {
   double prof_bytes[OTM_PROF_NCLASSES] = {};
   // status
   bool has_kappa = false, has_solution = false, has_drho = false;
+  // Element energies of the current fluctuation fields, written by the
+  // kappa^H kernel; every sensitivity of that solve is then a contraction
+  // of them. energy_ok is cleared wherever the iterate changes.
+  float* energy = nullptr;  // 6 x nn
+  bool energy_ok = false;
   // Start residual of the next solve launched on s_side while the coarse
   // operators are rebuilt on s (otm_eval); s waits for it through ev_side.
   hipStream_t s_side = nullptr;

@@ -152,6 +152,7 @@ static void ctx_free(otm_ctx* c) {
     dfree(L.scale);
     if (L.own_uf) dfree(L.u2);
   }
+  dfree(c->energy);
   dfree(c->kappa); dfree(c->x); dfree(c->x64); dfree(c->xlo); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
   dfree(c->cB); dfree(c->cBinv);
   dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
@@ -666,6 +667,7 @@ static void remove_constant(otm_ctx* c) {
 // (iteration cap, or the true residual stopped improving); the iterate and
 // its statistics are kept either way.
 static bool pcg_solve(otm_ctx* c) {
+  c->energy_ok = false;
   const long long n = c->g0.nn;
   const unsigned nb = red_blocks_for(n);
   int its[3] = {0, 0, 0};
@@ -719,10 +721,15 @@ static std::string noconv_message(const otm_ctx* c) {
 
 static void kappa_h(otm_ctx* c, double C[9]) {
   const unsigned nb = red_blocks_for(c->g0.nn);
+  if (!c->energy && hipMalloc((void**)&c->energy, sizeof(float) * 6 * (size_t)c->g0.nn) != hipSuccess) {
+    (void)hipGetLastError();  // no room: sensitivities recompute the energies
+    c->energy = nullptr;
+  }
   {
     ProfScope ps(c, OTM_PROF_ENERGY, 16.0 * (double)c->g0.nn);
-    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->chi_ref(), c->reducer_host(FIN_STORE), nb);
+    launch_fine_kappa_h(c->s, c->g0, c->fc, c->kappa, c->chi_ref(), c->reducer_host(FIN_STORE), nb, c->energy);
   }
+  c->energy_ok = c->energy != nullptr;
   c->wait_host();
   const double* s = c->hm->out;
   // order (00,01,02,11,12,22); |Y| = 1 (homogenization.cu:207)
@@ -1034,6 +1041,7 @@ OTM_API int otm_autotune(otm_ctx* c, int warm, otm_tune_report* rep) {
     }
     restore();
     c->has_solution = false;
+    c->energy_ok = false;
     if (rep->chosen < 0) fail(OTM_E_NOCONV, "no tuning candidate reached the tolerance");
   });
 }
@@ -1046,6 +1054,16 @@ OTM_API int otm_solve_stats(const otm_ctx* ctx, int iters[3], double relres[3]) 
       if (relres) relres[k] = ctx->relres[k];
     }
   });
+}
+
+// Contracted sensitivity of the current solve: from the element energies
+// the kappa^H kernel stored when they are still those of this iterate, else
+// recomputed from the fluctuation fields (the same bits either way).
+static void launch_sensitivity(otm_ctx* c, const float w[6], float* sens, int pitch) {
+  if (c->energy_ok)
+    launch_sens_from_energy(c->s, c->g0, c->energy, w, sens, pitch);
+  else
+    launch_fine_sensitivity(c->s, c->g0, c->fc, c->chi_ref(), w, sens, pitch);
 }
 
 OTM_API int otm_sensitivity(otm_ctx* ctx, const double dC[9], float* sens, int pitch, int where) {
@@ -1062,10 +1080,10 @@ OTM_API int otm_sensitivity(otm_ctx* ctx, const double dC[9], float* sens, int p
       // the reference zero-fills the pitched destination first (homogenization.cu:427)
       if (pitch != ctx->g0.nx)
         hip_check(hipMemsetAsync(sens, 0, sizeof(float) * pitch * rows, ctx->s), "zero sens");
-      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), w, sens, pitch);
+      launch_sensitivity(ctx, w, sens, pitch);
       ctx->sync();
     } else {
-      launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), w, ctx->q, ctx->g0.nx);
+      launch_sensitivity(ctx, w, ctx->q, ctx->g0.nx);
       std::vector<float> h((size_t)ctx->g0.nn);
       hip_check(hipMemcpyAsync(h.data(), ctx->q, sizeof(float) * ctx->g0.nn, hipMemcpyDeviceToHost, ctx->s),
                 "download sens");
@@ -1085,7 +1103,10 @@ OTM_API int otm_element_energy(otm_ctx* ctx, float* S6, int where) {
     if (!ctx->has_solution) fail(OTM_E_STATE, "otm_element_energy before otm_heat_matrix");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     const long long n = ctx->g0.nn;
-    if (where == OTM_DEVICE) {
+    if (ctx->energy_ok) {
+      hip_check(hipMemcpyAsync(S6, ctx->energy, sizeof(float) * 6 * n, kind_of(where, OTM_DEVICE), ctx->s), "copy S6");
+      ctx->sync();
+    } else if (where == OTM_DEVICE) {
       launch_fine_energy6(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), S6);
       ctx->sync();
     } else {
@@ -1116,6 +1137,7 @@ OTM_API int otm_set_chi(otm_ctx* ctx, const float* chi, int where) {
     if (!ctx || !chi) fail(OTM_E_ARG, "null argument");
     check_where(where);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->energy_ok = false;
     hip_check(hipMemcpyAsync(ctx->x, chi, sizeof(float) * 3 * ctx->g0.nn, kind_of(OTM_DEVICE, where), ctx->s),
               "copy chi");
     if (ctx->x64) launch_f2d(ctx->s, 3 * ctx->g0.nn, ctx->x, ctx->x64);
@@ -1128,6 +1150,7 @@ OTM_API int otm_reset_displacement(otm_ctx* ctx) {
   return guarded([&] {
     if (!ctx) fail(OTM_E_ARG, "null context");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->energy_ok = false;
     launch_fill(ctx->s, ctx->x, 3 * ctx->g0.nn, 0.f);
     if (ctx->x64) launch_fill_d(ctx->s, ctx->x64, 3 * ctx->g0.nn, 0.0);
     if (ctx->xlo) launch_fill(ctx->s, ctx->xlo, 3 * ctx->g0.nn, 0.f);
@@ -1185,6 +1208,7 @@ OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where) {
 
 // ---- MG_H entry points and level operators ---------------------------------------
 static void stationary_cycle(otm_ctx* c, double relres[3]) {
+  c->energy_ok = false;
   const long long n = c->g0.nn;
   const unsigned nb = red_blocks_for(n);
   const Reduce R = c->reducer_host(FIN_BB_RR, 0.0, 1.0 / (double)n);
@@ -1835,7 +1859,7 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
     const long long n = ctx->g0.nn;
     float w[6];
     sens_weights(dC, w);
-    launch_fine_sensitivity(ctx->s, ctx->g0, ctx->fc, ctx->chi_ref(), w, ctx->sens, ctx->g0.nx);
+    launch_sensitivity(ctx, w, ctx->sens, ctx->g0.nx);
     launch_mul(ctx->s, n, ctx->sens, ctx->dkappa, ctx->tmp);
     if (ctx->taps_kind >= 0)
       launch_filter_adj(ctx->s, ctx->g0, ctx->taps, ctx->tmp, ctx->W, ctx->drho);

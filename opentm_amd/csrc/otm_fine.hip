@@ -59,8 +59,15 @@ __global__ void __launch_bounds__(kBlock) k_fine_init_residual_rows(Geom g, Fine
 }
 
 __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, const float* __restrict__ kappa,
-                                                         ChiRef chi, Reduce R) {
-  fine_kappa_h_body(g, fc, kappa, chi, R, xcd_block(), gridDim.x);
+                                                         ChiRef chi, Reduce R, float* __restrict__ S6) {
+  fine_kappa_h_body(g, fc, kappa, chi, R, xcd_block(), gridDim.x, S6);
+}
+
+__global__ void __launch_bounds__(kBlock) k_sens_from_energy(Geom g, const float* __restrict__ S6, float w0, float w1,
+                                                             float w2, float w3, float w4, float w5,
+                                                             float* __restrict__ sens, int pitch) {
+  const float w[6] = {w0, w1, w2, w3, w4, w5};
+  sens_from_energy_body(g, S6, w, sens, pitch, stid());
 }
 
 __global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, ChiRef chi,
@@ -209,9 +216,14 @@ void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& f
 }
 
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const ChiRef& chi,
-                         const Reduce& R, unsigned nblocks) {
-  hipLaunchKernelGGL(k_fine_kappa_h, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, chi, R);
+                         const Reduce& R, unsigned nblocks, float* S6) {
+  hipLaunchKernelGGL(k_fine_kappa_h, dim3(nblocks), dim3(kBlock), 0, s, g, fc, kappa, chi, R, S6);
   launch_reduce_final(s, R, 6, nblocks);
+}
+
+void launch_sens_from_energy(hipStream_t s, const Geom& g, const float* S6, const float w[6], float* sens, int pitch) {
+  hipLaunchKernelGGL(k_sens_from_energy, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, S6, w[0], w[1], w[2], w[3], w[4],
+                     w[5], sens, pitch);
 }
 
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],

@@ -643,8 +643,11 @@ __device__ inline void element_S(const Geom& g, const FineConsts& fc, const ChiR
 
 // kappa^H_ij = sum_e (kappa_e) S_e(i,j) with FP64 accumulation; kappa
 // already holds the 1e-9 floor.
+// S6 (may be null) receives the element energies as fine_energy6_body
+// writes them, for the sensitivities that follow the same solve.
 __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, const float* __restrict__ kappa,
-                                         const ChiRef& chi, const Reduce& R, int lb, int nb) {
+                                         const ChiRef& chi, const Reduce& R, int lb, int nb,
+                                         float* __restrict__ S6 = nullptr) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const BlockRange br = block_range_v(g.nn, lb, nb);
   for (long long t = br.b + threadIdx.x; t < br.e; t += blockDim.x) {
@@ -652,6 +655,10 @@ __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, co
     split_node(g.nx, g.ny, t, x, y, z);
     float S[6];
     element_S(g, fc, chi, x, y, z, S);
+    if (S6) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) S6[c * g.nn + t] = S[c];
+    }
     const double ke = (double)kappa[t];
 #pragma unroll
     for (int c = 0; c < 6; c++) acc[c] += ke * (double)S[c];
@@ -670,6 +677,22 @@ __device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc
   element_S(g, fc, chi, x, y, z, S);
   const float s = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3] + S[4] * w[4] + S[5] * w[5];
   sens[(long long)x + (long long)pitch * ((long long)y + (long long)g.ny * z)] = s;
+}
+
+// The contraction of fine_sensitivity_body from stored element energies
+// (the same float expression, so the same bits).
+__device__ inline void sens_from_energy_body(const Geom& g, const float* __restrict__ S6, const float w[6],
+                                             float* __restrict__ sens, int pitch, long long t) {
+  if (t >= g.nn) return;
+  const long long nn = g.nn;
+  const float s = S6[t] * w[0] + S6[nn + t] * w[1] + S6[2 * nn + t] * w[2] + S6[3 * nn + t] * w[3] +
+                  S6[4 * nn + t] * w[4] + S6[5 * nn + t] * w[5];
+  if (pitch == g.nx) {
+    sens[t] = s;
+  } else {
+    const long long x = t % g.nx, r = t / g.nx;
+    sens[x + (long long)pitch * r] = s;
+  }
 }
 
 // S6[c*nn + e]: the six element energies, for callers that keep them.

@@ -20,8 +20,11 @@ void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& f
 void launch_fine_init_residual(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa,
                                const double* x, float* r, const Reduce& R, unsigned nblocks);
 struct ChiRef;  // fluctuation fields as FP64, an FP32 pair or FP32 (otm_fine_dev.hpp)
+// S6 (may be null): the element energies are also stored there (6 x nn floats)
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const ChiRef& chi,
-                         const Reduce& R, unsigned nblocks);
+                         const Reduce& R, unsigned nblocks, float* S6 = nullptr);
+// sens (pitched) = sum_c w_c S6[c] from stored element energies
+void launch_sens_from_energy(hipStream_t s, const Geom& g, const float* S6, const float w[6], float* sens, int pitch);
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
                              float* sens, int pitch);
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, float* S6);

@@ -56,6 +56,37 @@ def test_sensitivity_elementwise(orc, dC):
     assert err.max() <= 1e-4
 
 
+@pytest.mark.parametrize("refine", [0, 1])
+def test_sensitivity_follows_the_current_iterate(refine):
+    """The kappa^H kernel stores the element energies and the sensitivities
+    of that solve contract them; once the iterate changes (here a new one
+    set by otm_set_chi) the energies and the sensitivity are those of the
+    new iterate, never the stored ones. Each sensitivity is checked against
+    numpy's contraction of the element energies reported with it."""
+    say("sensitivity after an iterate change, refine", refine)
+    n = 16
+    rng = np.random.default_rng(31)
+    hom = make(n, fp64_refine=refine)
+    hom.update(wl.simp(wl.plain_density(n, 5)))
+    hom.heatMatrix()
+    D = np.array([[1.0, 0.3, -0.2], [0.1, -0.5, 0.4], [0.25, 0.0, 2.0]])
+    w = np.array([D[0, 0], D[0, 1] + D[1, 0], D[0, 2] + D[2, 0], D[1, 1], D[1, 2] + D[2, 1], D[2, 2]])
+
+    def check():
+        s = hom.Sensitivity(D).astype(np.float64)
+        S6 = hom.element_energy().astype(np.float64)
+        ref = (w[:, None] * S6).sum(axis=0)
+        say("  rel err against the energies", rel(s, ref))
+        assert rel(s, ref) < 1e-5
+        return s
+
+    s1 = check()
+    hom.set_chi(rng.standard_normal((3, n ** 3)).astype(np.float32) * 1e-2)
+    s2 = check()
+    assert rel(s2, s1) > 1e-2
+    hom.close()
+
+
 # ------------------------------------------------------------ error paths
 def test_nan_kappa_is_numeric_and_context_recovers(orc):
     """NaN in the conductivity: OTM_E_NUMERIC (the reference's "MG numerical
