@@ -55,7 +55,9 @@ struct MmaSmall {
 
 static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-// sums and maxima of one block into partials[block * (NS + NM) + ...]
+// Sums and maxima of one block into the partials, value-major:
+// partials[v * gridDim.x + block] for the NS sums then the NM maxima, so a
+// finaliser reads each value's partials contiguously.
 template <int NS, int NM>
 __device__ inline void block_sum_max(double (&s)[NS], double (&mx)[NM], double* partials) {
   __shared__ double sh[(NS + NM) * kWaves];
@@ -73,54 +75,57 @@ __device__ inline void block_sum_max(double (&s)[NS], double (&mx)[NM], double* 
     if (lane == 0) sh[(NS + i) * kWaves + wid] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < NS; i++) {
-      double v = 0.0;
-      for (int w = 0; w < kWaves; w++) v += sh[i * kWaves + w];
-      partials[(size_t)blockIdx.x * (NS + NM) + i] = v;
-    }
-    for (int i = 0; i < NM; i++) {
-      double v = -1e300;
-      for (int w = 0; w < kWaves; w++) v = fmax(v, sh[(NS + i) * kWaves + w]);
-      partials[(size_t)blockIdx.x * (NS + NM) + NS + i] = v;
-    }
+  const size_t nb = gridDim.x;
+  if (threadIdx.x < NS) {
+    double v = 0.0;
+    for (int w = 0; w < kWaves; w++) v += sh[threadIdx.x * kWaves + w];
+    partials[threadIdx.x * nb + blockIdx.x] = v;
+  } else if (threadIdx.x < NS + NM) {
+    double v = -1e300;
+    for (int w = 0; w < kWaves; w++) v = fmax(v, sh[threadIdx.x * kWaves + w]);
+    partials[threadIdx.x * nb + blockIdx.x] = v;
   }
 }
 
-// one-block reduction of the partials: out[0..NS) sums, out[NS..) maxima
+// One-block reduction of the value-major partials of nblocks blocks:
+// out[0..NS) sums (in block order per thread, then a fixed tree), out[NS..)
+// maxima.
 template <int NS, int NM>
 __device__ inline void final_sum_max(const double* partials, int nblocks, double* out) {
-  __shared__ double sh[(NS + NM) * kWaves];
-  double s[NS + NM];
+  constexpr int NV = NS + NM;
+  __shared__ double sh[NV * kWaves];
+  double s[NV];
 #pragma unroll
   for (int i = 0; i < NS; i++) s[i] = 0.0;
 #pragma unroll
   for (int i = 0; i < NM; i++) s[NS + i] = -1e300;
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
-    for (int i = 0; i < NS; i++) s[i] += partials[(size_t)b * (NS + NM) + i];
+    for (int i = 0; i < NS; i++) s[i] += partials[(size_t)i * nblocks + b];
 #pragma unroll
-    for (int i = 0; i < NM; i++) s[NS + i] = fmax(s[NS + i], partials[(size_t)b * (NS + NM) + NS + i]);
+    for (int i = 0; i < NM; i++) s[NS + i] = fmax(s[NS + i], partials[(size_t)(NS + i) * nblocks + b]);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < NS + NM; i++) {
+  for (int i = 0; i < NS; i++) {
+    const double v = wave_sum(s[i]);
+    if (lane == 0) sh[i * kWaves + wid] = v;
+  }
+#pragma unroll
+  for (int i = NS; i < NV; i++) {
     double v = s[i];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double w = __shfl_down(v, o, 64);
-      v = i < NS ? v + w : fmax(v, w);
-    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
     if (lane == 0) sh[i * kWaves + wid] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < NS + NM; i++) {
-      double v = i < NS ? 0.0 : -1e300;
-      for (int w = 0; w < kWaves; w++) v = i < NS ? v + sh[i * kWaves + w] : fmax(v, sh[i * kWaves + w]);
-      out[i] = v;
-    }
+  if (threadIdx.x < NV) {
+    const int i = threadIdx.x;
+    double v = i < NS ? 0.0 : -1e300;
+    for (int w = 0; w < kWaves; w++) v = i < NS ? v + sh[i * kWaves + w] : fmax(v, sh[i * kWaves + w]);
+    out[i] = v;
   }
+  __syncthreads();
 }
 
 // ---- per-element subproblem data (mmaOpt.cu:870-1000) ----------------------
