@@ -140,6 +140,7 @@ struct otm_ctx {
   double *cB = nullptr, *cBinv = nullptr;  // Gauss-Jordan work matrix, explicit inverse
   // reductions
   double* partials = nullptr;
+  double* tile_partials = nullptr;  // the fine post-smoother's per-tile r.z
   double* red_out = nullptr;
   PcgScalars* pcg = nullptr;
   PcgScalars* pcg_host = nullptr;  // pinned

@@ -155,7 +155,7 @@ static void ctx_free(otm_ctx* c) {
   dfree(c->energy);
   dfree(c->kappa); dfree(c->x); dfree(c->x64); dfree(c->xlo); dfree(c->r); dfree(c->z); dfree(c->p); dfree(c->q); dfree(c->z2);
   dfree(c->cB); dfree(c->cBinv);
-  dfree(c->partials); dfree(c->red_out); dfree(c->pcg);
+  dfree(c->partials); dfree(c->tile_partials); dfree(c->red_out); dfree(c->pcg);
   if (c->pcg_host) (void)hipHostFree(c->pcg_host);
   if (c->out_host) (void)hipHostFree(c->out_host);
   if (c->hm) (void)hipHostFree(c->hm);
@@ -270,6 +270,8 @@ static void build_hierarchy(otm_ctx* c) {
   }
   c->cBinv = dalloc<double>(ncoarse * ncoarse);
   c->partials = dalloc<double>((size_t)kMaxReduceBlocks * kMaxReduce);
+  // one row per fine tile of the smallest shape (32 x 4 x 4 = 512 nodes)
+  c->tile_partials = dalloc<double>((size_t)std::min<long long>(kTileDotSlots, n0 / 512 + 1) * OTM_NRHS);
   c->red_out = dalloc<double>(16);
   c->pcg = dalloc<PcgScalars>(1);
   hip_check(hipMemsetAsync(c->pcg, 0, sizeof(PcgScalars), c->s), "hipMemset pcg");
@@ -420,7 +422,8 @@ static void vcycle(otm_ctx* c, size_t l) {
   if (L.brick && l == 0) {
     // r.z of the PCG from the post-smoother's own values when asked for
     const bool dot = c->post_dot_mode >= 0 && fine_brick_dot_ok(L.g, c->fine_tile);
-    const Reduce RD = c->reducer(dot ? c->post_dot_mode : FIN_STORE);
+    Reduce RD = c->reducer(dot ? c->post_dot_mode : FIN_STORE);
+    RD.partials = c->tile_partials;
     {
       // compulsory bytes per node: kappa 4 + u read 12 + f 12 + u written 12
       ProfScope ps(c, OTM_PROF_FINE_GS, 40.0 * nn);

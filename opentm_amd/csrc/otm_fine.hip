@@ -108,12 +108,12 @@ static long long fine_variant_tiles(const Geom& g, int v) {
   return (long long)(g.nx / t.bx) * (g.ny / t.by) * (g.nz / t.bz);
 }
 
-// r.z in the smoother: 256-thread tiles, one partial-sum slot per tile
-// (the reduction buffers hold kTileRedBlocks slots)
+// r.z in the smoother: 256-thread tiles, one partial-sum row per tile in
+// the context's tile partials (one row per 512 nodes, at most kTileDotSlots)
 bool fine_brick_dot_ok(const Geom& g, int v) {
   if (!fine_tile_ok(g, v)) return false;
   const FineTileShape& t = kFineTileShapes[v];
-  return t.bx == 32 && t.bx * t.tr == kBlock && fine_variant_tiles(g, v) <= kTileRedBlocks;
+  return t.bx == 32 && t.bx * t.tr == kBlock && fine_variant_tiles(g, v) <= kTileDotSlots;
 }
 
 void launch_fine_smooth_brick(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const float* f,

@@ -741,6 +741,10 @@ __host__ __device__ inline int fine_tiles(const Geom& g, int bx) {
 // Workgroups of the tiled operator apply: one tile each up to the partial-sum
 // capacity of a reduction, several tiles each beyond.
 constexpr int kTileRedBlocks = kMaxReduceBlocks;
+// Rows of the fine smoother's per-tile r.z partials (a context's tile
+// partials hold kTileDotSlots x 3 doubles): 65536 tiles of 512 nodes and
+// more cover a 1024^3 cell.
+constexpr int kTileDotSlots = 65536;
 __host__ __device__ inline unsigned fine_apply_tile_blocks(const Geom& g, int bx) {
   const int t = fine_tiles(g, bx);
   return (unsigned)(t > kTileRedBlocks ? kTileRedBlocks : t);

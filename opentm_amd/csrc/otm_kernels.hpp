@@ -82,6 +82,9 @@ void launch_st_smooth_brick(hipStream_t s, const Geom& g, const StRef& S, const 
                             float* uout, int sweeps, bool forward, bool zero_in);
 
 // ---- vectors and reductions (otm_vec.hip) ----
+// One-block second stage of a reduction over nblocks partial rows; above
+// kMaxReduceBlocks rows (only for OTM_NRHS values: the smoother's per-tile
+// r.z) the rows are summed in a loop.
 void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblocks);
 void launch_fill(hipStream_t s, float* a, long long n, float v);
 void launch_dot3(hipStream_t s, long long nn, const float* a, const float* b, const Reduce& R, unsigned nblocks);

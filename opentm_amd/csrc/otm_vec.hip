@@ -1,5 +1,7 @@
 // otm_vec.hip -- single-cell launches of the PCG vector updates and FP64
 // reductions (device bodies in otm_vec_dev.hpp), plus copies and fills.
+#include <stdexcept>
+
 #include "otm_vec_dev.hpp"
 
 namespace otm {
@@ -77,7 +79,16 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(Reduce R, int nv, int n
   reduce_final_body(R, nv, nblocks);
 }
 
+__global__ void __launch_bounds__(kBlock) k_reduce_final_many3(Reduce R, int nblocks) {
+  reduce_final_many_nv<OTM_NRHS>(R, nblocks);
+}
+
 void launch_reduce_final(hipStream_t s, const Reduce& R, int nv, unsigned nblocks) {
+  if (nblocks > (unsigned)kMaxReduceBlocks) {
+    if (nv != OTM_NRHS) throw std::invalid_argument("more partial rows than slots for this reduction");
+    hipLaunchKernelGGL(k_reduce_final_many3, dim3(1), dim3(kBlock), 0, s, R, (int)nblocks);
+    return;
+  }
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s, R, nv, (int)nblocks);
 }
 

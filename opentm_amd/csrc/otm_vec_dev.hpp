@@ -162,6 +162,26 @@ __device__ inline void reduce_final_nv(const Reduce& R, int nblocks) {
   }
 }
 
+// The same for more partial rows than kMaxReduceBlocks (the fine smoother's
+// per-tile r.z): each thread sums its rows in order, four loads in flight.
+template <int NV>
+__device__ inline void reduce_final_many_nv(const Reduce& R, int nblocks) {
+  __shared__ double sh[NV * kWaves];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) acc[i] = 0.0;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) acc[i] += R.partials[(size_t)b * NV + i];
+  }
+  block_sum<NV>(acc, sh);
+  if (threadIdx.x == 0) {
+    finalize(acc, NV, R);
+    publish_host(acc, NV, R);
+  }
+}
+
 __device__ inline void reduce_final_body(const Reduce& R, int nv, int nblocks) {
   switch (nv) {
     case 2: reduce_final_nv<2>(R, nblocks); return;
