@@ -238,8 +238,55 @@ __device__ inline void st_residual_body(const Geom& g, const StRef& S, const flo
   st_resid_node<HALF>(g, S, f, u, res, t);
 }
 
+// restrict_node when coarse rows are whole waves (cg.nx a multiple of 64,
+// t = this wave's first coarse node + lane): a lane loads the fine pair
+// (2X, 2X+1) of each of its nine rows as one 8-byte load (a wave reads 512
+// contiguous bytes) and takes 2X-1 from the lane before it by DPP; lane 0
+// loads it. The same terms in the same order: the same bits.
+__device__ inline void restrict_node_pairs(const Geom& fg, const Geom& cg, const float* res, float* fc, long long t) {
+  int X, Y, Z;
+  split_node(cg.nx, cg.ny, t, X, Y, Z);
+  const int x = 2 * X, y = 2 * Y, z = 2 * Z;
+  const int xm = x == 0 ? fg.nx - 1 : x - 1;
+  const int ys[3] = {y == 0 ? fg.ny - 1 : y - 1, y, y + 1};
+  const int zs[3] = {z == 0 ? fg.nz - 1 : z - 1, z, z + 1};
+  const float w1[3] = {0.5f, 1.f, 0.5f};
+  const bool lane0 = (threadIdx.x & 63) == 0;
+  unsigned rb[9];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) rb[b + 3 * c] = (unsigned)fg.nx * ((unsigned)ys[b] + (unsigned)fg.ny * (unsigned)zs[c]);
+#pragma unroll
+  for (int k = 0; k < OTM_NRHS; k++) {
+    const float* rk = res + (size_t)k * (size_t)fg.nn;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        const unsigned r = rb[b + 3 * c];
+        const float2 v = *(const float2*)((const char*)rk + 4u * (r + (unsigned)x));
+        float left = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), 0x138, 0xF, 0xF, false));
+        if (lane0) left = ld_off(rk, 4u * (r + (unsigned)xm));
+        float row = 0.f;
+        row += w1[0] * left;
+        row += w1[1] * v.x;
+        row += w1[2] * v.y;
+        s += w1[b] * w1[c] * row;
+      }
+    fc[k * cg.nn + t] = s;
+  }
+}
+
+// One thread per coarse node (t); the grid covers whole waves, so the
+// paired form may be taken whenever coarse rows are whole waves.
 __device__ inline void restrict_body(const Geom& fg, const Geom& cg, const float* __restrict__ res,
                                      float* __restrict__ fc, long long t) {
+  if (cg.nx % 64 == 0 && blockDim.x % 64 == 0) {
+    if (t < cg.nn) restrict_node_pairs(fg, cg, res, fc, t);
+    return;
+  }
   if (t >= cg.nn) return;
   restrict_node(fg, cg, res, fc, t);
 }

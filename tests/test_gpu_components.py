@@ -41,8 +41,12 @@ def shape_of(level_reso):
     return (nz, ny, nx)
 
 
-@pytest.mark.parametrize("reso", [(16, 16, 16), (16, 8, 12)])
+@pytest.mark.parametrize("reso", [(16, 16, 16), (16, 8, 12), (128, 8, 8)])
 def test_transfers_against_dense_P(reso):
+    """Restriction and prolongation against the dense trilinear P; (128, 8, 8)
+    has 64-node coarse rows, where the restriction reads fine pairs and takes
+    the left neighbour from the lane before (its lane 0 wraps to x = nx-1
+    for the first wave of a row)."""
     say("transfers", reso)
     rng = np.random.default_rng(21)
     nx, ny, nz = reso
