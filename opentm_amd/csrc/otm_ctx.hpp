@@ -151,7 +151,7 @@ struct otm_ctx {
   bool hm_pending = false;         // that finaliser's results not yet taken
   // design pipeline
   float *rho = nullptr, *rho_f = nullptr, *W = nullptr, *kraw = nullptr, *dkappa = nullptr;
-  float *sens = nullptr, *tmp = nullptr, *drho = nullptr;
+  float *tmp = nullptr, *drho = nullptr;
   Taps taps;
   int taps_kind = -2;
   double taps_r = 0;

@@ -160,7 +160,7 @@ static void ctx_free(otm_ctx* c) {
   if (c->out_host) (void)hipHostFree(c->out_host);
   if (c->hm) (void)hipHostFree(c->hm);
   dfree(c->rho); dfree(c->rho_f); dfree(c->W); dfree(c->kraw); dfree(c->dkappa);
-  dfree(c->sens); dfree(c->tmp); dfree(c->drho);
+  dfree(c->tmp); dfree(c->drho);
   for (auto& p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -750,7 +750,6 @@ static void ensure_design_buffers(otm_ctx* c) {
   c->kraw = dalloc<float>(n);
   launch_fill(c->s, c->kraw, n, 0.f);
   c->dkappa = dalloc<float>(n);
-  c->sens = dalloc<float>(n);
   c->tmp = dalloc<float>(n);
   c->drho = dalloc<float>(n);
 }
@@ -1062,11 +1061,11 @@ OTM_API int otm_solve_stats(const otm_ctx* ctx, int iters[3], double relres[3]) 
 // Contracted sensitivity of the current solve: from the element energies
 // the kappa^H kernel stored when they are still those of this iterate, else
 // recomputed from the fluctuation fields (the same bits either way).
-static void launch_sensitivity(otm_ctx* c, const float w[6], float* sens, int pitch) {
+static void launch_sensitivity(otm_ctx* c, const float w[6], float* sens, int pitch, const float* scale = nullptr) {
   if (c->energy_ok)
-    launch_sens_from_energy(c->s, c->g0, c->energy, w, sens, pitch);
+    launch_sens_from_energy(c->s, c->g0, c->energy, w, sens, pitch, scale);
   else
-    launch_fine_sensitivity(c->s, c->g0, c->fc, c->chi_ref(), w, sens, pitch);
+    launch_fine_sensitivity(c->s, c->g0, c->fc, c->chi_ref(), w, sens, pitch, scale);
 }
 
 OTM_API int otm_sensitivity(otm_ctx* ctx, const double dC[9], float* sens, int pitch, int where) {
@@ -1862,8 +1861,7 @@ OTM_API int otm_backward(otm_ctx* ctx, const double dC[9], float* drho, int wher
     const long long n = ctx->g0.nn;
     float w[6];
     sens_weights(dC, w);
-    launch_sensitivity(ctx, w, ctx->sens, ctx->g0.nx);
-    launch_mul(ctx->s, n, ctx->sens, ctx->dkappa, ctx->tmp);
+    launch_sensitivity(ctx, w, ctx->tmp, ctx->g0.nx, ctx->dkappa);  // times d kappa / d rho
     if (ctx->taps_kind >= 0)
       launch_filter_adj(ctx->s, ctx->g0, ctx->taps, ctx->tmp, ctx->W, ctx->drho);
     else

@@ -65,16 +65,18 @@ __global__ void __launch_bounds__(kBlock) k_fine_kappa_h(Geom g, FineConsts fc, 
 
 __global__ void __launch_bounds__(kBlock) k_sens_from_energy(Geom g, const float* __restrict__ S6, float w0, float w1,
                                                              float w2, float w3, float w4, float w5,
-                                                             float* __restrict__ sens, int pitch) {
+                                                             float* __restrict__ sens, int pitch,
+                                                             const float* __restrict__ scale) {
   const float w[6] = {w0, w1, w2, w3, w4, w5};
-  sens_from_energy_body(g, S6, w, sens, pitch, stid());
+  sens_from_energy_body(g, S6, w, sens, pitch, stid(), scale);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fine_sensitivity(Geom g, FineConsts fc, ChiRef chi,
                                                              float w0, float w1, float w2, float w3, float w4, float w5,
-                                                             float* __restrict__ sens, int pitch) {
+                                                             float* __restrict__ sens, int pitch,
+                                                             const float* __restrict__ scale) {
   const float w[6] = {w0, w1, w2, w3, w4, w5};
-  fine_sensitivity_body(g, fc, chi, w, sens, pitch, gtid());
+  fine_sensitivity_body(g, fc, chi, w, sens, pitch, gtid(), scale);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fine_energy6(Geom g, FineConsts fc, ChiRef chi,
@@ -221,15 +223,16 @@ void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, con
   launch_reduce_final(s, R, 6, nblocks);
 }
 
-void launch_sens_from_energy(hipStream_t s, const Geom& g, const float* S6, const float w[6], float* sens, int pitch) {
+void launch_sens_from_energy(hipStream_t s, const Geom& g, const float* S6, const float w[6], float* sens, int pitch,
+                             const float* scale) {
   hipLaunchKernelGGL(k_sens_from_energy, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, S6, w[0], w[1], w[2], w[3], w[4],
-                     w[5], sens, pitch);
+                     w[5], sens, pitch, scale);
 }
 
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
-                             float* sens, int pitch) {
+                             float* sens, int pitch, const float* scale) {
   hipLaunchKernelGGL(k_fine_sensitivity, dim3(nblk(g.nn)), dim3(kBlock), 0, s, g, fc, chi, w[0], w[1], w[2], w[3],
-                     w[4], w[5], sens, pitch);
+                     w[4], w[5], sens, pitch, scale);
 }
 
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, float* S6) {

@@ -668,25 +668,31 @@ __device__ inline void fine_kappa_h_body(const Geom& g, const FineConsts& fc, co
 
 // sens[x + pitch*(y + ny*z)] = sum_c w_c S_c (contracted sensitivity with
 // respect to kappa; the penalty exponent is 1, homogenization.cu:344-346).
+// scale (may be null): the sensitivity times scale[e] (d kappa / d rho of
+// the element, for the design gradient).
 __device__ inline void fine_sensitivity_body(const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
-                                             float* __restrict__ sens, int pitch, long long t) {
+                                             float* __restrict__ sens, int pitch, long long t,
+                                             const float* __restrict__ scale = nullptr) {
   if (t >= g.nn) return;
   int x, y, z;
   split_node(g.nx, g.ny, t, x, y, z);
   float S[6];
   element_S(g, fc, chi, x, y, z, S);
-  const float s = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3] + S[4] * w[4] + S[5] * w[5];
+  float s = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3] + S[4] * w[4] + S[5] * w[5];
+  if (scale) s = s * scale[t];
   sens[(long long)x + (long long)pitch * ((long long)y + (long long)g.ny * z)] = s;
 }
 
 // The contraction of fine_sensitivity_body from stored element energies
 // (the same float expression, so the same bits).
 __device__ inline void sens_from_energy_body(const Geom& g, const float* __restrict__ S6, const float w[6],
-                                             float* __restrict__ sens, int pitch, long long t) {
+                                             float* __restrict__ sens, int pitch, long long t,
+                                             const float* __restrict__ scale = nullptr) {
   if (t >= g.nn) return;
   const long long nn = g.nn;
-  const float s = S6[t] * w[0] + S6[nn + t] * w[1] + S6[2 * nn + t] * w[2] + S6[3 * nn + t] * w[3] +
-                  S6[4 * nn + t] * w[4] + S6[5 * nn + t] * w[5];
+  float s = S6[t] * w[0] + S6[nn + t] * w[1] + S6[2 * nn + t] * w[2] + S6[3 * nn + t] * w[3] +
+            S6[4 * nn + t] * w[4] + S6[5 * nn + t] * w[5];
+  if (scale) s = s * scale[t];
   if (pitch == g.nx) {
     sens[t] = s;
   } else {

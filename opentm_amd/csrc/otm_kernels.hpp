@@ -24,9 +24,11 @@ struct ChiRef;  // fluctuation fields as FP64, an FP32 pair or FP32 (otm_fine_de
 void launch_fine_kappa_h(hipStream_t s, const Geom& g, const FineConsts& fc, const float* kappa, const ChiRef& chi,
                          const Reduce& R, unsigned nblocks, float* S6 = nullptr);
 // sens (pitched) = sum_c w_c S6[c] from stored element energies
-void launch_sens_from_energy(hipStream_t s, const Geom& g, const float* S6, const float w[6], float* sens, int pitch);
+void launch_sens_from_energy(hipStream_t s, const Geom& g, const float* S6, const float w[6], float* sens, int pitch,
+                             const float* scale = nullptr);
+// scale (may be null): each element's sensitivity times scale[e]
 void launch_fine_sensitivity(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, const float w[6],
-                             float* sens, int pitch);
+                             float* sens, int pitch, const float* scale = nullptr);
 void launch_fine_energy6(hipStream_t s, const Geom& g, const FineConsts& fc, const ChiRef& chi, float* S6);
 // Fine brick smoother: tiles of bx x by x bz nodes, one node per thread and
 // colour step for tr thread rows of bx threads, halo frozen, result in
