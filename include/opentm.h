@@ -162,6 +162,10 @@ OTM_API int otm_reset_displacement(otm_ctx* ctx);
  * of level l, 3*nodes_l floats) and z = M^{-1} r (one V-cycle of the
  * preconditioner on the fine level; MG_H::v_cycle, MG.cpp:46-71). */
 OTM_API int otm_apply_level(otm_ctx* ctx, int level, const float* u, float* q, int where);
+/* The fine operator as the PCG applies it (MG_H::solveEquation's K p,
+ * MG.cpp:80-142): q = K u for three vectors of the fine level in the
+ * difference form, and the three dot products u.q (pq, may be NULL). */
+OTM_API int otm_apply_fine_pcg(otm_ctx* ctx, const float* u, float* q, double pq[3], int where);
 OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where);
 /* Grid_H::getDensity (grid.cpp:330-345 writes it): the element conductivity
  * last given to otm_update (to one FP32 rounding), lexicographic, x fastest. */

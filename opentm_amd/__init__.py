@@ -292,6 +292,14 @@ class Homogenization_H:
         _capi.check(_capi.load().otm_apply_level(self._ctx, level, _ptr(a), _ptr(q), OTM_HOST), "otm_apply_level")
         return q
 
+    def apply_fine_pcg(self, u):
+        """The PCG's fine operator apply: (q = K u, [u.q per load case])."""
+        a = _f32(u, 3 * self.ne)
+        q = np.empty(3 * self.ne, np.float32)
+        pq = (ctypes.c_double * 3)()
+        _capi.check(_capi.load().otm_apply_fine_pcg(self._ctx, _ptr(a), _ptr(q), pq, OTM_HOST), "otm_apply_fine_pcg")
+        return q, list(pq)
+
     def vcycle(self, r):
         a = _f32(r, 3 * self.ne)
         z = np.empty(3 * self.ne, np.float32)

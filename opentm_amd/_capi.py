@@ -178,6 +178,7 @@ SIGNATURES = {
     "otm_set_chi": (_i, [_p, _p, _i]),
     "otm_reset_displacement": (_i, [_p]),
     "otm_apply_level": (_i, [_p, _i, _p, _p, _i]),
+    "otm_apply_fine_pcg": (_i, [_p, _p, _p, _p, _i]),
     "otm_vcycle": (_i, [_p, _p, _p, _i]),
     "otm_get_stencil": (_i, [_p, _i, _p, _i]),
     "otm_get_kappa": (_i, [_p, _p, _i]),

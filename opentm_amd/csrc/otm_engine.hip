@@ -1193,6 +1193,32 @@ OTM_API int otm_apply_level(otm_ctx* ctx, int level, const float* u, float* q, i
   });
 }
 
+OTM_API int otm_apply_fine_pcg(otm_ctx* ctx, const float* u, float* q, double pq[3], int where) {
+  return guarded([&] {
+    if (!ctx || !u || !q) fail(OTM_E_ARG, "null argument");
+    check_where(where);
+    if (!ctx->has_kappa) fail(OTM_E_STATE, "otm_apply_fine_pcg before otm_update");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long long n = ctx->g0.nn;
+    float* du = dalloc<float>(3 * n);
+    float* dq = dalloc<float>(3 * n);
+    try {
+      hip_check(hipMemcpyAsync(du, u, sizeof(float) * 3 * n, kind_of(OTM_DEVICE, where), ctx->s), "copy u");
+      launch_fine_apply_dot(ctx->s, ctx->g0, ctx->fc, ctx->kappa, du, dq, ctx->reducer(FIN_STORE), red_blocks_for(n));
+      hip_check(hipMemcpyAsync(q, dq, sizeof(float) * 3 * n, kind_of(where, OTM_DEVICE), ctx->s), "copy q");
+      double out[OTM_NRHS];
+      hip_check(hipMemcpyAsync(out, ctx->red_out, sizeof(out), hipMemcpyDeviceToHost, ctx->s), "copy u.q");
+      ctx->sync();
+      if (pq)
+        for (int k = 0; k < OTM_NRHS; k++) pq[k] = out[k];
+    } catch (...) {
+      dfree(du); dfree(dq);
+      throw;
+    }
+    dfree(du); dfree(dq);
+  });
+}
+
 OTM_API int otm_vcycle(otm_ctx* ctx, const float* r, float* z, int where) {
   return guarded([&] {
     if (!ctx || !r || !z) fail(OTM_E_ARG, "null argument");
