@@ -213,7 +213,7 @@ void mma_prof_release(MmaProf& P);
 // tag, and the host polls the tag (no copy, no stream synchronisation).
 struct MmaHost {
   double residumax, rmaxnew, rmaxnext;
-  int accept, fault;
+  int accept, respec, fault;
   unsigned seq;
 };
 // One MMA update on device arrays; returns Newton steps or a negative code

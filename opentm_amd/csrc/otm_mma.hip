@@ -44,6 +44,7 @@ struct MmaSmall {
   double yt[kMmaMaxM], lamt[kMmaMaxM], mut[kMmaMaxM], st[kMmaMaxM], zt, zett;
   double steg, stm_small;
   double lamtr[kMmaMaxM];  // lam + steg dlam of the next trial (uniform for its pass)
+  int respec;              // the last trial's step bound was wrong: run it again at steg
   double residunorm, residumax, resinew, rmaxnew;
   // Newton-system sums at the current point and at the trial point:
   // gvec[m], GG.(delx/diagx)[m], upper triangle of GG diag(1/diagx) GG^T
@@ -385,11 +386,16 @@ static_assert(MmaPointNV<kMmaMaxM>::NN <= kMmaMaxNS, "Newton sums exceed their s
 static_assert(MmaPointNV<kMmaMaxM>::NS + MmaPointNV<kMmaMaxM>::NX + 2 <= 32, "point pass partials exceed their slots");
 
 // TRIAL: the point is the current one + steg * (Newton direction); the
-// direction is recomputed as in k_mma_dir, the point written to the trial
+// direction is recomputed from the current point, the point written to the trial
 // buffers and the small-vector trial values are formed by the finaliser.
-template <int M, bool TRIAL>
+// BOUND (the first trial of a Newton step): the step was taken as
+// 1 / stm_small, the bound of the small variables alone, and the pass also
+// forms the elements' fraction-to-boundary bound (mmaOpt.cu:700-716) as
+// its third maximum; the finaliser redoes the trial if that bound is the
+// larger one.
+template <int M, bool TRIAL, bool BOUND = false>
 __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__ S) {
-  constexpr int NS = MmaPointNV<M>::NS, NX = TRIAL ? MmaPointNV<M>::NX : 0, NM = TRIAL ? 2 : 1;
+  constexpr int NS = MmaPointNV<M>::NS, NX = TRIAL ? MmaPointNV<M>::NX : 0, NM = TRIAL ? (BOUND ? 3 : 2) : 1;
   double lam[M], lamc[M], dlam[M];
   const double epsi = S->epsi, steg = S->steg;
   const double epsi2 = epsi * 0.1;  // the next level's, as k_mma_epsi forms it
@@ -405,6 +411,7 @@ __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__
   double mx[NM];
 #pragma unroll
   for (int i = 0; i < NM; i++) mx[i] = 0.0;
+  if (BOUND) mx[NM - 1] = -1e300;
   mma_for_each<M>(A, [&](unsigned j, const MmaRaw<M>& r) {
     MmaElem<M> e;
     mma_elem<M>(A, r, e);
@@ -415,6 +422,9 @@ __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__
       const double dx = mma_dx<M>(qc, dlam);
       const double dxsi = -xsi + epsi * qc.rxa - xsi * dx * qc.rxa;
       const double deta = -eta + epsi * qc.rbx + eta * dx * qc.rbx;
+      if (BOUND)
+        mx[NM - 1] = fmax(mx[NM - 1], fmax(fmax(-1.01 * dxsi * rcp64(xsi), -1.01 * deta * rcp64(eta)),
+                                           fmax(-1.01 * dx * qc.rxa, 1.01 * dx * qc.rbx)));
       x += steg * dx;
       xsi += steg * dxsi;
       eta += steg * deta;
@@ -444,7 +454,7 @@ __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__
       const double rexsi2 = xsi * (x - e.al) - epsi2;
       const double reeta2 = eta * (e.be - x) - epsi2;
       sums[NS] += rex * rex + rexsi2 * rexsi2 + reeta2 * reeta2;
-      mx[NM - 1] = fmax(mx[NM - 1], fmax(fabs(rex), fmax(fabs(rexsi2), fabs(reeta2))));
+      mx[1] = fmax(mx[1], fmax(fabs(rex), fmax(fabs(rexsi2), fabs(reeta2))));
       const double delx2 = mma_delx<M>(q, epsi2);
 #pragma unroll
       for (int i = 0; i < M; i++) sums[NS + 1 + i] += q.GG[i] * delx2 * inv;
@@ -456,19 +466,19 @@ __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__
 // Up to m = 3 the compiler is asked for five waves per SIMD (the trial pass
 // fits 96 VGPRs without spilling; left alone it takes 100 and four waves);
 // at m = 4 that would spill, so the kernel has no such request there.
-template <int M, bool TRIAL>
+template <int M, bool TRIAL, bool BOUND>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_mma_point(MmaArrays A, const MmaSmall* __restrict__ S) {
-  mma_point_body<M, TRIAL>(A, S);
+  mma_point_body<M, TRIAL, BOUND>(A, S);
 }
-template <int M, bool TRIAL>
+template <int M, bool TRIAL, bool BOUND>
 __global__ void __launch_bounds__(kBlock) k_mma_point_wide(MmaArrays A, const MmaSmall* __restrict__ S) {
-  mma_point_body<M, TRIAL>(A, S);
+  mma_point_body<M, TRIAL, BOUND>(A, S);
 }
-template <int M, bool TRIAL>
+template <int M, bool TRIAL, bool BOUND = false>
 static constexpr auto mma_point_kernel() {
-  if constexpr (M <= 3) return k_mma_point<M, TRIAL>;
-  else return k_mma_point_wide<M, TRIAL>;
+  if constexpr (M <= 3 && !BOUND) return k_mma_point<M, TRIAL, BOUND>;
+  else return k_mma_point_wide<M, TRIAL, BOUND>;
 }
 
 // The host loop's inputs to the host block, then the tag (one thread).
@@ -477,17 +487,32 @@ __device__ inline void mma_publish(const MmaSmall* S, MmaHost* mh, unsigned tag)
   mh->rmaxnew = S->rmaxnew;
   mh->rmaxnext = S->rmaxn;
   mh->accept = S->accept;
+  mh->respec = S->respec;
   mh->fault = S->fault;
   __hip_atomic_store(&mh->seq, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-template <int M, bool TRIAL>
+template <int M, bool TRIAL, bool BOUND = false>
 __device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nblocks) {
   constexpr int NS = MmaPointNV<M>::NS, NN = MmaPointNV<M>::NN;
-  constexpr int NX = TRIAL ? MmaPointNV<M>::NX : 0, NM = TRIAL ? 2 : 1;
+  constexpr int NX = TRIAL ? MmaPointNV<M>::NX : 0, NM = TRIAL ? (BOUND ? 3 : 2) : 1;
   __shared__ double out[NS + NX + NM];
   final_sum_max<NS + NX, NM>(A.partials, nblocks, out);
   if (threadIdx.x != 0) return;
+  S->respec = 0;
+  if (BOUND) {
+    // step = 1 / max(stm_small, element bound) (mmaOpt.cu:700-716): when the
+    // elements bound it, the trial taken at 1 / stm_small is discarded and
+    // the line search starts at the right step
+    const double stm_elem = out[NS + NX + NM - 1];
+    if (stm_elem > S->stm_small) {
+      S->steg = 1.0 / fmax(S->stm_small, stm_elem);
+      for (int i = 0; i < M; i++) S->lamtr[i] = S->lam[i] + S->steg * S->dlam[i];
+      S->accept = 0;
+      S->respec = 1;
+      return;
+    }
+  }
   double ss = out[0], mx = out[NS + NX];
   double* ns = TRIAL ? S->ns_t : S->ns;
   for (int i = 0; i < NN; i++) ns[i] = out[1 + i];
@@ -526,10 +551,10 @@ __device__ inline void mma_point_final_body(MmaArrays& A, MmaSmall* S, int nbloc
   for (int i = 0; i < M; i++) S->lamtr[i] = S->lam[i] + S->steg * S->dlam[i];
 }
 
-template <int M, bool TRIAL>
+template <int M, bool TRIAL, bool BOUND>
 __global__ void __launch_bounds__(kBlock) k_mma_point_final(MmaArrays A, MmaSmall* S, int nblocks, MmaHost* mh,
                                                             unsigned tag) {
-  mma_point_final_body<M, TRIAL>(A, S, nblocks);
+  mma_point_final_body<M, TRIAL, BOUND>(A, S, nblocks);
   if (threadIdx.x == 0) mma_publish(S, mh, tag);
 }
 
@@ -631,50 +656,14 @@ __global__ void k_mma_newton_final(MmaSmall* S, int commit) {
   stm = fmax(stm, fmax(-1.01 * dz / S->z, -1.01 * dzet / S->zet));
   S->stm_small = stm;
   if (!(stm - stm == 0.0)) S->fault = 1;
-}
-
-// ---- direction and fraction-to-boundary bound ------------------------------------
-template <int M>
-__global__ void __launch_bounds__(kBlock) k_mma_dir(MmaArrays A, const MmaSmall* __restrict__ S) {
-  double lam[M], dlam[M];
-#pragma unroll
-  for (int i = 0; i < M; i++) {
-    lam[i] = S->lam[i];
-    dlam[i] = S->dlam[i];
-  }
-  const double epsi = S->epsi;
-  double dummy[1] = {0.0};
-  double mx[1] = {-1e300};
-  mma_for_each<M>(A, [&](unsigned, const MmaRaw<M>& r) {
-    MmaElem<M> e;
-    mma_elem<M>(A, r, e);
-    const double x = r.xs, xsi = r.xsi, eta = r.eta;
-    MmaNewtonElem<M> q;
-    mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
-    const double dx = mma_dx<M>(q, dlam);
-    const double dxsi = -xsi + epsi * q.rxa - xsi * dx * q.rxa;
-    const double deta = -eta + epsi * q.rbx + eta * dx * q.rbx;
-    // fraction to the boundary (mmaOpt.cu:700-716)
-    mx[0] = fmax(mx[0], fmax(fmax(-1.01 * dxsi * rcp64(xsi), -1.01 * deta * rcp64(eta)),
-                             fmax(-1.01 * dx * q.rxa, 1.01 * dx * q.rbx)));
-  });
-  block_sum_max<1, 1>(dummy, mx, A.partials);
-}
-
-__global__ void __launch_bounds__(kBlock) k_mma_dir_final(MmaArrays A, MmaSmall* S, int nblocks) {
-  __shared__ double out[2];
-  final_sum_max<1, 1>(A.partials, nblocks, out);
-  if (threadIdx.x != 0) return;
-  const double stm = fmax(S->stm_small, out[1]);
+  // the first trial takes the small variables' step; its pass forms the
+  // elements' bound and the finaliser redoes it if that one is larger
   S->steg = 1.0 / stm;
-  for (int i = 0; i < S->m; i++) S->lamtr[i] = S->lam[i] + S->steg * S->dlam[i];
+  for (int i = 0; i < M; i++) S->lamtr[i] = S->lam[i] + S->steg * S->dlam[i];
   S->itto = 0;
   S->accept = 0;
 }
 
-// Next barrier level: epsi / 10; with spec, the accepted trial's sums at
-// that level become the current residual and Newton system (the level's
-// point pass is skipped).
 __global__ void k_mma_epsi(MmaSmall* S, int commit, int spec) {
   if (commit) mma_commit(S);
   S->epsi *= 0.1;
@@ -765,7 +754,7 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
   const unsigned nb_prep = mma_grid(k_mma_prep<M>, A.n);
   const unsigned nb_point = mma_grid(mma_point_kernel<M, false>(), A.n);
   const unsigned nb_trial = mma_grid(mma_point_kernel<M, true>(), A.n);
-  const unsigned nb_dir = mma_grid(k_mma_dir<M>, A.n);
+  const unsigned nb_bound = mma_grid(mma_point_kernel<M, true, true>(), A.n);
   // no kernel of this optimiser is in flight: the block holds the last tag
   unsigned tag = __atomic_load_n(&mh->seq, __ATOMIC_ACQUIRE);
   auto read = [&]() -> int {
@@ -818,8 +807,8 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
         Timed t(prof, 2, s, bytes_of);
         hipLaunchKernelGGL((mma_point_kernel<M, false>()), dim3(nb_point), dim3(kBlock), 0, s, A, S);
       }
-      hipLaunchKernelGGL((k_mma_point_final<M, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point, mh_dev,
-                         ++tag);
+      hipLaunchKernelGGL((k_mma_point_final<M, false, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_point,
+                         mh_dev, ++tag);
       const int rc = read();
       if (rc) return rc;
       residumax = mh->residumax;
@@ -832,18 +821,23 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
       newton++;
       hipLaunchKernelGGL(k_mma_newton_final<M>, dim3(1), dim3(1), 0, s, S, commit);
       commit = 0;
-      {
-        Timed t(prof, 1, s, bytes_of);
-        hipLaunchKernelGGL(k_mma_dir<M>, dim3(nb_dir), dim3(kBlock), 0, s, A, S);
-      }
-      hipLaunchKernelGGL(k_mma_dir_final, dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_dir);
+      // line search; the first trial also checks the step it was given
+      bool first = true;
       do {
         {
           Timed t(prof, 0, s, bytes_of);
-          hipLaunchKernelGGL((mma_point_kernel<M, true>()), dim3(nb_trial), dim3(kBlock), 0, s, A, S);
+          if (first)
+            hipLaunchKernelGGL((mma_point_kernel<M, true, true>()), dim3(nb_bound), dim3(kBlock), 0, s, A, S);
+          else
+            hipLaunchKernelGGL((mma_point_kernel<M, true>()), dim3(nb_trial), dim3(kBlock), 0, s, A, S);
         }
-        hipLaunchKernelGGL((k_mma_point_final<M, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_trial, mh_dev,
-                           ++tag);
+        if (first)
+          hipLaunchKernelGGL((k_mma_point_final<M, true, true>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_bound,
+                             mh_dev, ++tag);
+        else
+          hipLaunchKernelGGL((k_mma_point_final<M, true, false>), dim3(1), dim3(kBlock), 0, s, A, S, (int)nb_trial,
+                             mh_dev, ++tag);
+        first = false;
         rc = read();
         if (rc) return rc;
       } while (!mh->accept);
