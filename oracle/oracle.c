@@ -558,6 +558,9 @@ static double mma_residual(const mma_sub* S, double epsi, const double* x, const
   return sqrt(ss);
 }
 
+/* Newton steps of the subsolver and those whose step length the design
+ * variables' bounds set (not the small variables'): a test statistic. */
+static long orc_mma_steps = 0, orc_mma_elem_bound_steps = 0;
 static int mma_subsolv(const mma_sub* S, double epsimin, double* xout) {
   int m = S->m, n = S->n;
   double* x = (double*)malloc(sizeof(double) * n);
@@ -649,6 +652,7 @@ static int mma_subsolv(const mma_sub* S, double epsimin, double* xout) {
       double dzet = -zet + epsi / z - zet * dz / z;
       /* fraction-to-boundary step */
       double stm = 1.0;
+      double stm_small;
       for (int i = 0; i < m; i++) {
         double v;
         v = -1.01 * dy[i] / y[i]; if (v > stm) stm = v;
@@ -657,6 +661,7 @@ static int mma_subsolv(const mma_sub* S, double epsimin, double* xout) {
         v = -1.01 * ds[i] / s[i]; if (v > stm) stm = v;
       }
       { double v = -1.01 * dz / z; if (v > stm) stm = v; v = -1.01 * dzet / zet; if (v > stm) stm = v; }
+      stm_small = stm;
       for (int j = 0; j < n; j++) {
         double v;
         v = -1.01 * dxsi[j] / xsi[j]; if (v > stm) stm = v;
@@ -665,6 +670,8 @@ static int mma_subsolv(const mma_sub* S, double epsimin, double* xout) {
         v = 1.01 * dx[j] / (S->beta[j] - x[j]); if (v > stm) stm = v;
       }
       double steg = 1.0 / stm;
+      orc_mma_steps++;
+      if (stm > stm_small) orc_mma_elem_bound_steps++;
       memcpy(xo, x, sizeof(double) * n);
       memcpy(xsio, xsi, sizeof(double) * n);
       memcpy(etao, eta, sizeof(double) * n);
@@ -701,6 +708,12 @@ static int mma_subsolv(const mma_sub* S, double epsimin, double* xout) {
   free(dxsi); free(deta); free(delx); free(diagx); free(GG);
   return total;
 }
+
+void orc_mma_stats(long* steps, long* elem_bound_steps) {
+  *steps = orc_mma_steps;
+  *elem_bound_steps = orc_mma_elem_bound_steps;
+}
+void orc_mma_stats_reset(void) { orc_mma_steps = orc_mma_elem_bound_steps = 0; }
 
 int orc_mmasub(int m, int n, int itn, double* x, const double* xmin, const double* xmax,
                double* xold1, double* xold2, const double* df0dx, const double* gval,

@@ -39,6 +39,10 @@ def lib():
         L.orc_oc_update.restype = i
         L.orc_mmasub.argtypes = [i, i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, d, _dp, _dp, _dp, d, d]
         L.orc_mmasub.restype = i
+        L.orc_mma_stats.argtypes = [ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+        L.orc_mma_stats.restype = None
+        L.orc_mma_stats_reset.argtypes = []
+        L.orc_mma_stats_reset.restype = None
         ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         L.orc_gs_ell.argtypes = [i, i, i, _dp, _dp, _dp, i, i, i]
         L.orc_brick_gs_ell.argtypes = [i, i, i, _dp, _dp, _dp, _dp, i, i, i, i, i, i]
@@ -234,6 +238,16 @@ class MMA:
         self.upp = np.zeros(n)
         self.move = 1.0
         self.epsimin = 1e-7  # the reference's last barrier level (mmaOpt.cu:846)
+
+    @staticmethod
+    def stats(reset=False):
+        """(Newton steps, steps whose length the design variables' bounds set)
+        over the subsolver runs since the last reset."""
+        a, b = ctypes.c_long(), ctypes.c_long()
+        lib().orc_mma_stats(ctypes.byref(a), ctypes.byref(b))
+        if reset:
+            lib().orc_mma_stats_reset()
+        return a.value, b.value
 
     def update(self, itn, x, df0dx, gval, dgdx):
         """Returns the new x (float64) and the Newton step count."""

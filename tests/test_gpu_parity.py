@@ -428,6 +428,39 @@ def test_mma_matches_oracle(orc, m):
         assert np.abs(xg - xo).max() < 2e-6
 
 
+@pytest.mark.parametrize("m", [1, 3])
+def test_mma_element_bound_steps_match_oracle(orc, m):
+    """MMA updates whose Newton steps are sometimes limited by the design
+    variables' fraction-to-boundary bound rather than the small variables'
+    (objective gradients spread over a factor 40): the GPU's first trial of
+    such a step is taken too long and redone at the elements' bound. Six
+    updates against the oracle, which counts those steps (at least one)."""
+    say("mma element-bound steps", m)
+    rng = np.random.default_rng(7)
+    n = 4096
+    t = rng.uniform(0, 1, n)
+    W = rng.uniform(0.5, 1.5, (m, n))
+    scale = rng.uniform(0.5, 20.0, n)
+    bounds = np.linspace(0.2, 0.35, m)
+    gpu = otm.MMAOptimizer(m, n, 1.0, 0.0, 1e3, 1.0)
+    gpu.setBound(1e-3, 1.0)
+    ref = orc.MMA(m, n, 1.0, 0.0, 1e3, 1.0, 1e-3, 1.0)
+    xg = rng.uniform(1e-3, 1.0, n).astype(np.float32)
+    orc.MMA.stats(reset=True)
+    for it in range(6):
+        x32 = xg.astype(np.float64)
+        df = (2 * (x32 - t) * scale).astype(np.float32)
+        g = (W * x32).mean(axis=1) - bounds
+        dg = (W / n).astype(np.float32)
+        gpu.update(it, xg, df, g, [dg[i] for i in range(m)])
+        xo, its = ref.update(it, x32, df.astype(np.float64), g, dg.astype(np.float64))
+        say("  it", it, "newton gpu", gpu.last_newton, "oracle", its, "max diff", float(np.abs(xg - xo).max()))
+        assert np.abs(xg - xo).max() < 2e-6
+    steps, bound_steps = orc.MMA.stats()
+    say("  oracle Newton steps", steps, "bounded by the elements", bound_steps)
+    assert bound_steps >= 1
+
+
 def test_run_custom_mma_matches_oracle_loop(orc):
     """runCustom's MMA branch (Framework.cu:171-201): minimise the volume
     subject to the tensor error, three updates, against the FP64 oracle loop
