@@ -138,7 +138,8 @@ def roofline_from(prof, reso):
             "traffic": measured_traffic(FINE_KERNEL, reso)}
 
 
-MMA_KERNELS = {"trial": "k_mma_point<3, true>", "dir": "k_mma_dir<3>", "point": "k_mma_point<3, false>"}
+MMA_KERNELS = {"trial": "k_mma_point<3, true, false>", "bound": "k_mma_point_wide<3, true, true>",
+               "point": "k_mma_point<3, false, false>"}
 
 
 def measured_mma_traffic(name, reso):
@@ -157,7 +158,7 @@ def measured_mma_traffic(name, reso):
 def mma_rooflines(mprof, reso=None):
     """The MMA subsolver passes: algorithmic bytes per launch (x 4, asymptotes
     16, the FP32 gradients 4 (1 + m), the FP64 interior point 24 bytes per
-    element; the trial pass also writes the 24-byte trial point) over the
+    element; the trial passes also write the 24-byte trial point) over the
     mean launch time from HIP events on the optimiser's stream."""
     out = {}
     for name, (nl, ms, by) in mprof.items():

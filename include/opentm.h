@@ -299,8 +299,10 @@ OTM_API int otm_mma_update(otm_mma* mma, int itn, float* x, const float* dfdx, c
                            const float* const* dgdx, int where, int* newton);
 
 /* HIP-event timing of the MMA passes (no counterpart in the reference):
- * kclass 0 = trial-point pass, 1 = direction pass, 2 = start-of-level point
- * pass; launches, milliseconds and algorithmic bytes since enabling. */
+ * kclass 0 = later trial-point passes of a line search, 1 = the first trial
+ * of each Newton step (it also forms the elements' step bound), 2 =
+ * start-of-level point pass; launches, milliseconds and algorithmic bytes
+ * since enabling. */
 OTM_API int otm_mma_profile_enable(otm_mma* mma, int enable);
 OTM_API int otm_mma_profile_read(otm_mma* mma, int kclass, long long* launches, double* ms, double* bytes);
 

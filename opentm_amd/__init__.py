@@ -569,10 +569,11 @@ class MMAOptimizer:
     def set_move(self, move):
         _capi.check(_capi.load().otm_mma_set_move(self._h, float(move)), "otm_mma_set_move")
 
-    MMA_PASSES = ("trial", "dir", "point")
+    MMA_PASSES = ("trial", "bound", "point")
 
     def profile_enable(self, enable=True):
-        """HIP-event timing of the subsolver passes (trial point, direction,
+        """HIP-event timing of the subsolver passes (later trial points of a
+        line search, each Newton step's first trial with the step bound,
         start-of-level point)."""
         _capi.check(_capi.load().otm_mma_profile_enable(self._h, 1 if enable else 0), "otm_mma_profile_enable")
 

@@ -191,6 +191,13 @@ __device__ inline BlockRange block_range(long long n) { return block_range_v(n, 
 __device__ inline long long stid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline BlockRange stream_range(long long n) { return block_range_v(n, blockIdx.x, gridDim.x); }
 
+// A load of a value this pass reads once, marked non-temporal (the caches
+// do not keep it for reuse).
+template <typename T>
+__device__ inline T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 // Sum over aligned groups of 8 lanes; every lane of a group receives the
 // same sum (IEEE addition commutes). Cross-lane moves through DPP: swap
 // pairs (quad_perm 1,0,3,2), swap pairs of pairs (quad_perm 2,3,0,1), then

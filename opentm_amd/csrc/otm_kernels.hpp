@@ -194,8 +194,9 @@ struct MmaArrays {
 constexpr int kMmaMaxBlocks = 2048;  // partial-sum slots of the MMA passes
 size_t mma_small_bytes();
 // HIP-event timing of the MMA passes (otm_mma_profile_*): class 0 the
-// trial-point pass, 1 the direction pass, 2 the point pass at the start of
-// each barrier level; bytes are the algorithmic bytes per launch.
+// line search's later trial-point passes, 1 each Newton step's first trial
+// (which also forms the elements' step bound), 2 the point pass at the
+// start of a barrier level; bytes are the algorithmic bytes per launch.
 struct MmaProf {
   bool on = false;
   long long launches[3] = {0, 0, 0};

@@ -164,22 +164,22 @@ struct MmaRaw {
 
 template <int M>
 __device__ inline void mma_fetch(const MmaArrays& A, unsigned j, MmaRaw<M>& r) {
-  r.x = A.xvar[j];
-  r.lo = A.low[j];
-  r.up = A.upp[j];
-  r.g0 = A.df0dx[j];
+  r.x = ld_stream(&A.xvar[j]);
+  r.lo = ld_stream(&A.low[j]);
+  r.up = ld_stream(&A.upp[j]);
+  r.g0 = ld_stream(&A.df0dx[j]);
 #pragma unroll
-  for (int i = 0; i < M; i++) r.g[i] = A.dgdx[(size_t)i * (size_t)A.n + j];
+  for (int i = 0; i < M; i++) r.g[i] = ld_stream(&A.dgdx[(size_t)i * (size_t)A.n + j]);
   if (A.xmin) {
-    r.xmn = (double)A.xmin[j];
-    r.xmx = (double)A.xmax[j];
+    r.xmn = (double)ld_stream(&A.xmin[j]);
+    r.xmx = (double)ld_stream(&A.xmax[j]);
   } else {
     r.xmn = A.xmin_u;
     r.xmx = A.xmax_u;
   }
-  r.xs = A.xs[j];
-  r.xsi = A.xsi[j];
-  r.eta = A.eta[j];
+  r.xs = ld_stream(&A.xs[j]);
+  r.xsi = ld_stream(&A.xsi[j]);
+  r.eta = ld_stream(&A.eta[j]);
 }
 
 // alfa, beta, p0, q0, P, Q at the design x, given e.lo/e.up. Every pass
@@ -428,9 +428,12 @@ __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__
       x += steg * dx;
       xsi += steg * dxsi;
       eta += steg * deta;
-      A.xs_t[j] = x;
-      A.xsi_t[j] = xsi;
-      A.eta_t[j] = eta;
+      // non-temporal, like the loads: a pass streams more than the caches
+      // hold, and with both marked the pass ran 2-3% faster (loads alone
+      // 1%, stores alone not at all)
+      __builtin_nontemporal_store(x, &A.xs_t[j]);
+      __builtin_nontemporal_store(xsi, &A.xsi_t[j]);
+      __builtin_nontemporal_store(eta, &A.eta_t[j]);
     }
     MmaNewtonElem<M> q;
     mma_newton_elem<M>(e, x, xsi, eta, lam, epsi, q);
@@ -765,7 +768,7 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
   // algorithmic bytes per element: x, low/upp, gradients, interior point
   // (+ per-element bounds), and the trial point written
   const double rd = 4.0 + 16.0 + 4.0 * (1 + M) + 24.0 + (A.xmin ? 8.0 : 0.0);
-  const double bytes_of[3] = {(rd + 24.0) * (double)A.n, rd * (double)A.n, rd * (double)A.n};
+  const double bytes_of[3] = {(rd + 24.0) * (double)A.n, (rd + 24.0) * (double)A.n, rd * (double)A.n};
   struct Timed {
     MmaProf* P;
     int cls;
@@ -825,7 +828,7 @@ static int mma_run(hipStream_t s, MmaArrays A, MmaSmall* S, MmaHost* mh, MmaHost
       bool first = true;
       do {
         {
-          Timed t(prof, 0, s, bytes_of);
+          Timed t(prof, first ? 1 : 0, s, bytes_of);
           if (first)
             hipLaunchKernelGGL((mma_point_kernel<M, true, true>()), dim3(nb_bound), dim3(kBlock), 0, s, A, S);
           else

@@ -41,7 +41,7 @@ __device__ inline void sum3_body(long long nn, const T* __restrict__ a, const Re
 __device__ inline void refine_add_body(long long n, double* __restrict__ x64, float* __restrict__ d, int lb, int nb) {
   const BlockRange br = block_range_v(n, lb, nb);
   for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
-    x64[i] += (double)d[i];
+    x64[i] = ld_stream(x64 + i) + (double)ld_stream(d + i);
     d[i] = 0.f;
   }
 }
@@ -54,7 +54,7 @@ __device__ inline void sub_mean_cast_body(long long nn, double* __restrict__ x64
   for (long long i = br.b + threadIdx.x; i < br.e; i += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
-      const double v = x64[k * nn + i] - m[k];
+      const double v = ld_stream(x64 + k * nn + i) - m[k];
       x64[k * nn + i] = v;
       x32[k * nn + i] = (float)v;
     }
@@ -103,14 +103,14 @@ __device__ inline void pcg_xr_body(long long nn, float* __restrict__ x, float* _
     for (int k = 0; k < OTM_NRHS; k++) {
       const long long j = k * nn + i;
       if (xlo) {
-        float hi = x[j], lo = xlo[j];
-        pair_add_prod(hi, lo, al[k], p[j]);
+        float hi = ld_stream(x + j), lo = ld_stream(xlo + j);
+        pair_add_prod(hi, lo, al[k], ld_stream(p + j));
         x[j] = hi;
         xlo[j] = lo;
       } else {
-        x[j] += al[k] * p[j];
+        x[j] = ld_stream(x + j) + al[k] * ld_stream(p + j);
       }
-      const float rv = r[j] - al[k] * q[j];
+      const float rv = ld_stream(r + j) - al[k] * ld_stream(q + j);
       r[j] = rv;
       acc[k] += (double)rv * (double)rv;
     }
@@ -130,7 +130,7 @@ __device__ inline void pcg_p_body(long long nn, float* __restrict__ p, const flo
 #pragma unroll
     for (int k = 0; k < OTM_NRHS; k++) {
       const long long j = k * nn + i;
-      p[j] = first ? z[j] : z[j] + be[k] * p[j];
+      p[j] = first ? ld_stream(z + j) : ld_stream(z + j) + be[k] * ld_stream(p + j);
     }
   }
 }
