@@ -216,14 +216,17 @@ __device__ inline void mma_elem(const MmaArrays& A, const MmaRaw<M>& r, MmaElem<
   mma_coefficients<M>(A.move, (double)r.x, r.xmn, r.xmx, (double)r.g0, r.g, e);
 }
 
-// f(j, raw) over this block's elements in the order of a plain stride loop
-// (32-bit element index: a design has fewer than 2^31 elements, so one
-// register carries it through the loop).
+// f(j, raw) over this block's elements: the whole grid sweeps the arrays
+// front to back (block b takes elements b*256 .. b*256+255 of every
+// grid-wide stride), so the HBM sees the pass's 13 streams advancing
+// together rather than one stream per block and array. Against a
+// contiguous range per block the first trial pass went from 289 to 265 us
+// at 2^24 elements, with 1024, 1280 or 2048 blocks alike. (32-bit element
+// index: a design has fewer than 2^31 elements.)
 template <int M, typename F>
 __device__ inline void mma_for_each(const MmaArrays& A, F&& f) {
-  const BlockRange br = stream_range(A.n);
-  const unsigned e = (unsigned)br.e;
-  for (unsigned j = (unsigned)br.b + threadIdx.x; j < e; j += blockDim.x) {
+  const unsigned e = (unsigned)A.n, stride = gridDim.x * blockDim.x;
+  for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < e; j += stride) {
     MmaRaw<M> r;
     mma_fetch<M>(A, j, r);
     f(j, r);
@@ -285,8 +288,8 @@ __global__ void __launch_bounds__(kBlock) k_mma_prep(MmaArrays A, int itn) {
 #pragma unroll
   for (int i = 0; i < M; i++) bs[i] = 0.0;
   double dummy[1] = {-1e300};
-  const BlockRange br = stream_range(A.n);
-  for (long long j = br.b + threadIdx.x; j < br.e; j += blockDim.x) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < A.n; j += stride) {
     const double x = (double)A.xvar[j];
     const double xmn = A.xmin ? (double)A.xmin[j] : A.xmin_u;
     const double xmx = A.xmax ? (double)A.xmax[j] : A.xmax_u;
