@@ -138,7 +138,7 @@ def roofline_from(prof, reso):
             "traffic": measured_traffic(FINE_KERNEL, reso)}
 
 
-MMA_KERNELS = {"trial": "k_mma_point<3, true, false>", "bound": "k_mma_point_wide<3, true, true>",
+MMA_KERNELS = {"trial": "k_mma_point_wide<3, true, false>", "bound": "k_mma_point_wide<3, true, true>",
                "point": "k_mma_point<3, false, false>"}
 
 

@@ -469,9 +469,11 @@ __device__ inline void mma_point_body(MmaArrays& A, const MmaSmall* __restrict__
   block_sum_max<NS + NX, NM>(sums, mx, A.partials);
 }
 
-// Up to m = 3 the compiler is asked for five waves per SIMD (the trial pass
-// fits 96 VGPRs without spilling; left alone it takes 100 and four waves);
-// at m = 4 that would spill, so the kernel has no such request there.
+// The start-of-level point pass (up to m = 3) asks the compiler for five
+// waves per SIMD. The trial passes do not: in grid-stride order a trial
+// pass with that request (96 VGPRs, 1280 workgroups) took 282 us at 2^24
+// elements, without it (four waves, 1024 workgroups) 262 us; at m = 4 the
+// request would spill.
 template <int M, bool TRIAL, bool BOUND>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_mma_point(MmaArrays A, const MmaSmall* __restrict__ S) {
@@ -483,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_mma_point_wide(MmaArrays A, const Mm
 }
 template <int M, bool TRIAL, bool BOUND = false>
 static constexpr auto mma_point_kernel() {
-  if constexpr (M <= 3 && !BOUND) return k_mma_point<M, TRIAL, BOUND>;
+  if constexpr (M <= 3 && !BOUND && !TRIAL) return k_mma_point<M, TRIAL, BOUND>;
   else return k_mma_point_wide<M, TRIAL, BOUND>;
 }
 
