@@ -402,13 +402,14 @@ def test_oc_loop_parity(orc):
         assert drho < 1e-4
 
 
-@pytest.mark.parametrize("m", [1, 3])
-def test_mma_matches_oracle(orc, m):
+@pytest.mark.parametrize("m,n", [(1, 4096), (3, 4096), (3, 4099), (2, 300001)])
+def test_mma_matches_oracle(orc, m, n):
     """Six consecutive MMA updates (asymptote initialisation and adaptation)
-    on the GPU and in the oracle from the same FP32 inputs."""
-    say("mma", m)
+    on the GPU and in the oracle from the same FP32 inputs; ragged sizes (not
+    a multiple of a wave or a block, and more elements than one sweep of the
+    grid covers) exercise the passes' grid-stride loops."""
+    say("mma", m, n)
     rng = np.random.default_rng(30 + m)
-    n = 4096
     t = rng.uniform(0, 1, n)
     W = rng.uniform(0.5, 1.5, (m, n))
     bounds = np.linspace(0.2, 0.35, m)
